@@ -1,0 +1,158 @@
+"""ctypes mirror of include/dustk.h and include/dustk_setup.h, plus the host-side pieces of the
+reference driver that the C-ABI takes as plain numbers: the uniform MeshBlock table
+(src/mesh/meshblock.cpp:60-120, src/coordinates/cell_locations.hpp:24-27) and the imex2+ / rk2
+stage coefficients (src/driver/driver.cpp:106-118,195-203,278-300)."""
+
+import ctypes as C
+
+DK_NRDATA = 17
+DK_NIDATA = 5
+DK_NGAS = 4
+(IPX, IPVX, IPY, IPVY, IPZ, IPVZ, IPX1, IPVX1, IPY1, IPVY1, IPZ1, IPVZ1,
+ IPRX, IPRY, IPRZ, IPTS, IPM) = range(17)
+PGID, PTAG, PSP, PLEV, PTAGH = range(5)
+DK_NGP, DK_CIC, DK_TSC = 0, 1, 2
+DK_IMEX, DK_PC2 = 0, 1
+DK_LOCAL, DK_APPLYA, DK_DC1, DK_DC2 = 0, 1, 2, 3
+DK_X1_PERIODIC, DK_X1_SHEAR = 0, 1
+DK_X3_PERIODIC, DK_X3_REFLECT, DK_X3_OUTFLOW = 0, 1, 2
+DK_FORM_FP64, DK_FORM_FP32 = 0, 1
+DK_MAXSPEC = 8
+
+ERRORS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory",
+          4: "particle outside deposit halo", 5: "particle routing", 6: "stopping time <= 0",
+          7: "unsupported option", 8: "capacity", 9: "call order", 10: "non-finite value"}
+
+
+class dk_block(C.Structure):
+    _fields_ = [("x1min", C.c_double), ("x1max", C.c_double),
+                ("x2min", C.c_double), ("x2max", C.c_double),
+                ("x3min", C.c_double), ("x3max", C.c_double),
+                ("lx1", C.c_int32), ("lx2", C.c_int32), ("lx3", C.c_int32),
+                ("level", C.c_int32), ("nghbr", C.c_int32 * 27)]
+
+
+class dk_mesh(C.Structure):
+    _fields_ = [("nmb", C.c_int32), ("nx1", C.c_int32), ("nx2", C.c_int32),
+                ("nx3", C.c_int32), ("ng", C.c_int32),
+                ("x1min", C.c_double), ("x1max", C.c_double),
+                ("x2min", C.c_double), ("x2max", C.c_double),
+                ("x3min", C.c_double), ("x3max", C.c_double),
+                ("nmbx1", C.c_int32), ("nmbx2", C.c_int32), ("nmbx3", C.c_int32),
+                ("bc_x1", C.c_int32), ("bc_x3", C.c_int32), ("x3_dep_drop", C.c_int32)]
+
+
+class dk_physics(C.Structure):
+    _fields_ = [("deposit", C.c_int32), ("coupling", C.c_int32), ("solver", C.c_int32),
+                ("back_reaction", C.c_int32), ("consistent_gas", C.c_int32),
+                ("shearing_box", C.c_int32), ("stratified", C.c_int32),
+                ("omega0", C.c_double), ("qshear", C.c_double),
+                ("pc2_damping", C.c_int32), ("stiffness", C.c_int32)]
+
+
+class dk_stage(C.Structure):
+    _fields_ = [("stage", C.c_int32), ("nexp_stages", C.c_int32),
+                ("time", C.c_double), ("dt", C.c_double),
+                ("gam0", C.c_double * 3), ("gam1", C.c_double * 3),
+                ("beta", C.c_double * 3), ("a_twid22", C.c_double),
+                ("a_impl", C.c_double)]
+
+
+class dk_gas_src(C.Structure):
+    _fields_ = [("shearing_box", C.c_int32), ("stratified", C.c_int32),
+                ("omega0", C.c_double), ("qshear", C.c_double),
+                ("const_accel", C.c_double), ("const_accel_dir", C.c_int32)]
+
+
+class dk_setup(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("ppc", C.c_int32), ("nspecies", C.c_int32),
+                ("taus", C.c_double * DK_MAXSPEC), ("eps", C.c_double * DK_MAXSPEC),
+                ("continuous", C.c_int32),
+                ("tau_lo", C.c_double * DK_MAXSPEC), ("tau_hi", C.c_double * DK_MAXSPEC),
+                ("mass_slope", C.c_double), ("rho0", C.c_double),
+                ("gas_vx", C.c_double), ("gas_vy", C.c_double), ("gas_vz", C.c_double),
+                ("gas_noise", C.c_double), ("dust_noise", C.c_double),
+                ("nsh", C.c_int32), ("omega0", C.c_double), ("qshear", C.c_double),
+                ("gauss_z", C.c_int32), ("hp", C.c_double), ("gas_strat", C.c_int32),
+                ("nclump", C.c_int32), ("clump_frac", C.c_double),
+                ("clump_sigma", C.c_double), ("zmetal", C.c_double)]
+
+
+def _left_edge(ith, n, xmin, xmax):
+    """LeftEdgeX of src/coordinates/cell_locations.hpp:24-27 (same operation order)."""
+    x = float(ith) / float(n)
+    return (x * xmax - x * xmin) - (0.5 * xmax - 0.5 * xmin) + (0.5 * xmin + 0.5 * xmax)
+
+
+def uniform_mesh(nx, nb, box, ng=2, bc_x1=DK_X1_PERIODIC, bc_x3=DK_X3_PERIODIC,
+                 x3_dep_drop=0):
+    """A uniform mesh of nb[0]*nb[1]*nb[2] blocks of nx[0]*nx[1]*nx[2] cells over
+    box = ((x1min, x1max), (x2min, x2max), (x3min, x3max)).  Block gid = bx + nbx*(by + nby*bz);
+    block edges as MeshBlock::SetPositions (src/mesh/meshblock.cpp:40-120).  Neighbours are
+    periodic in x1 and x2 (the shear-periodic x1 face is also listed with its plain periodic
+    neighbour; the library applies the shear), periodic in x3 unless bc_x3 is a physical face
+    (then -1).  Returns (dk_mesh, array of dk_block)."""
+    (x1a, x1b), (x2a, x2b), (x3a, x3b) = box
+    nbx, nby, nbz = nb
+    mesh = dk_mesh(nmb=nbx * nby * nbz, nx1=nx[0], nx2=nx[1], nx3=nx[2], ng=ng,
+                   x1min=x1a, x1max=x1b, x2min=x2a, x2max=x2b, x3min=x3a, x3max=x3b,
+                   nmbx1=nbx, nmbx2=nby, nmbx3=nbz, bc_x1=bc_x1, bc_x3=bc_x3,
+                   x3_dep_drop=x3_dep_drop)
+    blocks = (dk_block * mesh.nmb)()
+
+    def edges(l, n, a, b):
+        lo = a if l == 0 else _left_edge(l, n, a, b)
+        hi = b if l == n - 1 else _left_edge(l + 1, n, a, b)
+        return lo, hi
+
+    for bz in range(nbz):
+        for by in range(nby):
+            for bx in range(nbx):
+                g = bx + nbx * (by + nby * bz)
+                b = blocks[g]
+                b.x1min, b.x1max = edges(bx, nbx, x1a, x1b)
+                b.x2min, b.x2max = edges(by, nby, x2a, x2b)
+                b.x3min, b.x3max = edges(bz, nbz, x3a, x3b)
+                b.lx1, b.lx2, b.lx3, b.level = bx, by, bz, 0
+                for o3 in (-1, 0, 1):
+                    for o2 in (-1, 0, 1):
+                        for o1 in (-1, 0, 1):
+                            d = (o1 + 1) + 3 * (o2 + 1) + 9 * (o3 + 1)
+                            z = bz + o3
+                            if bc_x3 != DK_X3_PERIODIC and (z < 0 or z >= nbz):
+                                b.nghbr[d] = -1
+                                continue
+                            gx, gy, gz = (bx + o1) % nbx, (by + o2) % nby, z % nbz
+                            b.nghbr[d] = gx + nbx * (gy + nby * gz)
+    return mesh, blocks
+
+
+IMEX2P_GAMMA = 1.707106781186547   # the literal of src/driver/driver.cpp:203
+
+
+def imex2p_stages(time, dt, gamma=IMEX2P_GAMMA):
+    """The imex2+ tableau (Driver::SetImEx2PlusCoefficients, src/driver/driver.cpp:278-300)
+    as three dk_stage records (stage 3 is the identity assembly stage)."""
+    out = []
+    for s in range(3):
+        st = dk_stage(stage=s + 1, nexp_stages=3, time=time, dt=dt)
+        st.gam0[0], st.gam1[0], st.beta[0] = 1.0, 0.0, gamma
+        st.gam0[1] = (2.0 * gamma - 1.0) / (2.0 * gamma * gamma)
+        st.gam1[1] = (1.0 - (2.0 * gamma - 1.0) / (2.0 * gamma * gamma))
+        st.beta[1] = 1.0 / (2.0 * gamma)
+        st.gam0[2], st.gam1[2], st.beta[2] = 1.0, 0.0, 0.0
+        st.a_twid22 = (1.0 - 2.0 * gamma * gamma) / 2.0 / gamma
+        st.a_impl = gamma
+        out.append(st)
+    return out
+
+
+def rk2_stages(time, dt):
+    """Heun / SSPRK(2,2) (src/driver/driver.cpp:106-118) as two dk_stage records."""
+    out = []
+    for s in range(2):
+        st = dk_stage(stage=s + 1, nexp_stages=2, time=time, dt=dt)
+        st.gam0[0], st.gam1[0], st.beta[0] = 0.0, 1.0, 1.0
+        st.gam0[1], st.gam1[1], st.beta[1] = 0.5, 0.5, 0.5
+        out.append(st)
+    return out

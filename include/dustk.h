@@ -1,0 +1,231 @@
+/*
+ * dustk.h -- C-ABI of the MI355X (gfx950) particle-gas drag coupling.
+ *
+ * Every entry point replaces one task method of the reference's DustGasDrag task list
+ * (src/dust/dust_tasks.cpp:38-120).  A task there has the form
+ *     TaskStatus DustGasDrag::X(Driver *pdrive, int stage)
+ * (src/tasklist/task_list.hpp:180-199); here it is
+ *     int dk_x(dk_ctx *ctx, const dk_stage *st)
+ * where dk_stage carries the numbers the task read from Driver and Mesh (stage, dt, time,
+ * the RK/IMEX tableau of src/driver/driver.cpp:106-118,278-300).  Every function returns 0 on
+ * success or a positive DK_E* code; none aborts.  Calls are asynchronous and ordered on the
+ * context's HIP stream; only dk_diag_sums, dk_error and dk_new_time_step wait for the stream.
+ *
+ * Layouts are the reference's:
+ *   particles  rdata[c*capacity + p], c < DK_NRDATA (double), idata[c*capacity + p],
+ *              c < DK_NIDATA (int32): Particles::prtcl_rdata/idata, LayoutRight
+ *              (src/particles/particles.cpp:190-205, src/athena.hpp:67-80)
+ *   gas        u[(((m*DK_NGAS + v)*n3 + k)*n2 + j)*n1 + i], n_d = nx_d + 2 ng, ghosts
+ *              included: Hydro::u0/u1 (src/hydro/hydro.cpp:64)
+ * All pointers passed to the library are device pointers unless stated otherwise.  The
+ * library owns its work fields (qdep, ustar, dmom, solver scratch, sort keys, FP32 shadows).
+ */
+#ifndef DUSTK_H_
+#define DUSTK_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* particle columns: src/athena.hpp:77-80 */
+enum {
+  DK_IPX = 0, DK_IPVX = 1, DK_IPY = 2, DK_IPVY = 3, DK_IPZ = 4, DK_IPVZ = 5,
+  DK_IPX1 = 6, DK_IPVX1 = 7, DK_IPY1 = 8, DK_IPVY1 = 9, DK_IPZ1 = 10, DK_IPVZ1 = 11,
+  DK_IPRX = 12, DK_IPRY = 13, DK_IPRZ = 14, DK_IPTS = 15, DK_IPM = 16
+};
+enum { DK_PGID = 0, DK_PTAG = 1, DK_PSP = 2, DK_PLEV = 3, DK_PTAGH = 4 };
+#define DK_NRDATA 17
+#define DK_NIDATA 5
+
+/* isothermal gas conserved variables: src/athena.hpp:59 */
+enum { DK_IDN = 0, DK_IM1 = 1, DK_IM2 = 2, DK_IM3 = 3 };
+#define DK_NGAS 4
+
+/* src/dust/dust.hpp:59 */
+enum { DK_NGP = 0, DK_CIC = 1, DK_TSC = 2 };
+/* src/dust/dust.hpp:74 */
+enum { DK_IMEX = 0, DK_PC2 = 1 };
+/* src/dust/dust.hpp:70 (pcg and adaptive are not provided: DK_EUNSUPPORTED) */
+enum { DK_LOCAL = 0, DK_APPLYA = 1, DK_DC1 = 2, DK_DC2 = 3 };
+/* mesh boundaries handled by the drag path */
+enum { DK_X1_PERIODIC = 0, DK_X1_SHEAR = 1 };
+enum { DK_X3_PERIODIC = 0, DK_X3_REFLECT = 1, DK_X3_OUTFLOW = 2 };
+
+/* execution forms of a context */
+enum {
+  DK_FORM_FP64 = 0,   /* plain FP64: one kernel per reference kernel, atomic deposits,
+                         operating directly on the caller's arrays */
+  DK_FORM_FP32 = 1    /* FP32 bulk, FP64 positions and sums: cell-sorted particles,
+                         atomic-free deposits, fused stage passes (dk_stage_run only) */
+};
+
+/* error codes */
+enum {
+  DK_OK = 0,
+  DK_EINVAL = 1,        /* bad argument */
+  DK_EHIP = 2,          /* a HIP runtime call failed */
+  DK_ENOMEM = 3,        /* device allocation failed */
+  DK_EHALO = 4,         /* a particle lies outside the deposit halo of its block
+                           (Kokkos::abort of src/dust/dust_pm.cpp:241-251) */
+  DK_EROUTE = 5,        /* a particle moved more than one block
+                           (Kokkos::abort of src/bvals/bvals_part.cpp:245-252) */
+  DK_ESTOP = 6,         /* a non-positive stopping time (src/dust/dust_newdt.cpp:73-79) */
+  DK_EUNSUPPORTED = 7,  /* option not provided by this build */
+  DK_ECAPACITY = 8,     /* a size beyond a fixed maximum */
+  DK_ESTATE = 9,        /* call order violated (e.g. particles not bound) */
+  DK_ENOTFINITE = 10    /* a non-finite value met by a guarded kernel */
+};
+
+/* A MeshBlock: its box (RegionSize, src/mesh/mesh.hpp:25-29), logical location and level,
+ * and the gid of the same-level neighbour in each direction
+ * d = (ox1+1) + 3*(ox2+1) + 9*(ox3+1) (d = 13 is the block itself; -1 = none, a physical
+ * face).  Replaces MeshBlock::mb_size/nghbr (src/mesh/meshblock.hpp). */
+typedef struct dk_block {
+  double x1min, x1max, x2min, x2max, x3min, x3max;
+  int32_t lx1, lx2, lx3, level;
+  int32_t nghbr[27];
+} dk_block;
+
+/* The mesh: block size, ghost depth, root box and the physical/shear boundaries
+ * (Mesh::mb_indcs, mesh_size, mesh_bcs). */
+typedef struct dk_mesh {
+  int32_t nmb, nx1, nx2, nx3, ng;
+  double x1min, x1max, x2min, x2max, x3min, x3max;
+  int32_t nmbx1, nmbx2, nmbx3;   /* blocks per direction (uniform mesh) */
+  int32_t bc_x1;                 /* DK_X1_PERIODIC or DK_X1_SHEAR */
+  int32_t bc_x3;                 /* DK_X3_PERIODIC, DK_X3_REFLECT or DK_X3_OUTFLOW: the
+                                    particle x3 boundary (bvals_part.cpp:159-197) and the u*
+                                    ghost values (reflect: mirror, else zero gradient) */
+  int32_t x3_dep_drop;           /* physical x3 faces: 0 fold the ghost deposits back (default
+                                    <dust>/x3_ghost_deposit = fold), 1 drop them
+                                    (src/dust/dust.cpp:1323-1333) */
+} dk_mesh;
+
+/* The drag options of the DustGasDrag constructor (src/dust/dust.cpp:60-450). */
+typedef struct dk_physics {
+  int32_t deposit;          /* DK_TSC (default), DK_CIC, DK_NGP */
+  int32_t coupling;         /* DK_IMEX or DK_PC2 */
+  int32_t solver;           /* DK_DC1 default for imex point particles, DK_LOCAL for pc2 */
+  int32_t back_reaction;    /* 1: the gas feels the dust */
+  int32_t consistent_gas;   /* imex point consistent-gas split (dust.cpp:359, default 1) */
+  int32_t shearing_box;     /* 3-D shearing box forces and frames */
+  int32_t stratified;       /* vertical gravity -Omega^2 z */
+  double omega0, qshear;
+  int32_t pc2_damping;      /* pc2 feedback damping of stiff cells (dust_pm.cpp:389-398) */
+  int32_t stiffness;        /* deposit the pc2 rate field qdep[4] (StiffnessActive) */
+} dk_physics;
+
+/* What a task read from Driver and Mesh for one stage. */
+typedef struct dk_stage {
+  int32_t stage;            /* 1-based */
+  int32_t nexp_stages;      /* 3 for imex2+, 2 for rk2 */
+  double time, dt;          /* t^n and the step */
+  double gam0[3], gam1[3], beta[3];
+  double a_twid22;          /* imex2+ a_twid[2][2] */
+  double a_impl;            /* imex2+ gamma */
+} dk_stage;
+
+/* Gas source terms of the harness's explicit gas update (a stand-in for Hydro::RKUpdate with
+ * zero fluxes plus Hydro::HydroSrcTerms, src/hydro/hydro_update.cpp:23-83,
+ * src/hydro/hydro_tasks.cpp:268-297). */
+typedef struct dk_gas_src {
+  int32_t shearing_box;     /* ShearingBoxCC::SourceTermsCC (3-D) */
+  int32_t stratified;
+  double omega0, qshear;
+  double const_accel;       /* SourceTerms::ConstantAccel value (0 = off) */
+  int32_t const_accel_dir;  /* 1..3 */
+} dk_gas_src;
+
+typedef struct dk_ctx dk_ctx;
+
+/* ---- context ------------------------------------------------------------------------- */
+
+/* Creates a context on the current HIP device.  blocks: host array of mesh->nmb entries.
+ * stream: a hipStream_t (NULL = the null stream).  Replaces the allocations of the
+ * DustGasDrag constructor (src/dust/dust.cpp:1160-1300). */
+int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phys,
+              int32_t form, void *stream, dk_ctx **out);
+int dk_destroy(dk_ctx *ctx);
+
+/* Binds the caller's particle SoA (Particles::prtcl_rdata/idata) and gas registers
+ * (Hydro::u0/u1).  In DK_FORM_FP32 the particles are copied into the library's
+ * mixed-precision cell-sorted store; dk_export_particles writes them back. */
+int dk_bind_particles(dk_ctx *ctx, double *rdata, int32_t *idata, int64_t capacity,
+                      int64_t npart);
+int dk_bind_gas(dk_ctx *ctx, double *u0, double *u1);
+int dk_export_particles(dk_ctx *ctx);
+int64_t dk_npart(const dk_ctx *ctx);
+
+/* ---- the task entry points (DK_FORM_FP64) ------------------------------------------ */
+
+/* DustGasDrag::FirstTwoImpRK, src/dust/dust_push.cpp:35-80 */
+int dk_first_two_imp_rk(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::ExplicitPush, src/dust/dust_push.cpp:91-300 */
+int dk_explicit_push(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::UpdateParticleGIDs -> ShiftShearCrossings + SetNewPrtclGID,
+ * src/dust/dust_tasks.cpp:149-162, src/dust/dust.cpp:1668-1702,
+ * src/bvals/bvals_part.cpp:115-471 (one rank: no send list) */
+int dk_update_particle_gids(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::DepositDrag, src/dust/dust_pm.cpp:104-277 */
+int dk_deposit_drag(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::SendDepQP + RecvDepQP (additive ghost fold of qdep),
+ * src/dust/dust_tasks.cpp:249-263, src/bvals/bvals_dep.cpp:320-562 */
+int dk_fold_deposit_qp(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::GasImplicitSolve (local / dc1 / dc2 / applya, pc2 predictor apply),
+ * src/dust/dust_pm.cpp:372-442, src/dust/dust_solver.cpp:126-263,630-724 */
+int dk_gas_implicit_solve(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::SendUstar/RecvUstar/SendUstarShr/RecvUstarShr (u* ghost fill),
+ * src/dust/dust_tasks.cpp:265-312,429-469 */
+int dk_fill_ustar(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::GatherKickPMBR, src/dust/dust_pm.cpp:450-568 */
+int dk_gather_kick_pmbr(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::SendPMBR + RecvPMBR (additive ghost fold of dmom),
+ * src/dust/dust_tasks.cpp:314-328 */
+int dk_fold_pmbr(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::ApplyPMBR, src/dust/dust_pm.cpp:576-711 */
+int dk_apply_pmbr(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::AddDragHistoryGas, src/dust/dust_pm.cpp:721-742 */
+int dk_add_drag_history_gas(dk_ctx *ctx, const dk_stage *st);
+/* DustGasDrag::SortParticles (cell sort of point particles),
+ * src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell), all columns */
+int dk_sort_particles(dk_ctx *ctx);
+/* DustGasDrag::NewTimeStep, src/dust/dust_newdt.cpp:108-149: dt_out = dt_cfl min dx/|v| */
+int dk_new_time_step(dk_ctx *ctx, double dt_cfl, double *dt_out);
+
+/* ---- whole stage ------------------------------------------------------------------- */
+
+/* All dust tasks of one stage in task-graph order (dust_tasks.cpp:58-109), gas hook
+ * included when src != NULL.  The only stage entry of DK_FORM_FP32, where it runs the
+ * fused cell-sorted passes; with graphs enabled (dk_set_graphs) a stage is one
+ * hipGraphLaunch. */
+int dk_stage_run(dk_ctx *ctx, const dk_stage *st, const dk_gas_src *src);
+int dk_set_graphs(dk_ctx *ctx, int32_t enable);
+
+/* ---- gas stand-ins of the harness (not part of the reference's dust path) --------------- */
+
+/* Hydro::RKUpdate with zero fluxes + Hydro::HydroSrcTerms (see dk_gas_src). */
+int dk_gas_hook(dk_ctx *ctx, const dk_stage *st, const dk_gas_src *src);
+/* Hydro::SendU/RecvU: periodic (and shear-periodic) copy of the u0 ghost cells. */
+int dk_fill_gas_ghosts(dk_ctx *ctx, const dk_stage *st);
+
+/* ---- diagnostics ------------------------------------------------------------------- */
+
+/* FP64 sums over particles and active cells (waits for the stream):
+ * out[0..2] dust momentum sum m v, out[3..5] gas momentum sum V M, out[6] dust mass,
+ * out[7] gas mass sum V rho, out[8] sum m |v| + sum V |M| (the conservation scale).
+ * Cf. DustGasDrag::ParticleHistory, src/dust/dust_science.cpp:391-484. */
+int dk_diag_sums(dk_ctx *ctx, double out[9]);
+/* The first device-side error (code, particle index); clears it. */
+int dk_error(dk_ctx *ctx, int32_t *code, int64_t *particle);
+/* Device pointer to a work field: 0 qdep(5), 1 ustar(3), 2 dmom(3), 3 solver ap(3),
+ * 4 imex residual(4); layout as the gas (nvar components).  NULL if absent. */
+void *dk_field(dk_ctx *ctx, int32_t which, int32_t *nvar);
+const char *dk_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* DUSTK_H_ */

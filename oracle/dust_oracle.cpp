@@ -1,0 +1,1575 @@
+// dust_oracle.cpp -- CPU FP64 oracle of the dust drag coupling.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h): a restatement, in the reference's operation order,
+// of the AthenaK dust drag path for point particles on a uniform mesh of MeshBlocks.  Each
+// function cites the reference file:line it follows.  With one thread every particle loop
+// runs in index order and deposits are plain adds (Kokkos::Serial, src/dust/dust.hpp:986-998);
+// with more threads deposits use omp atomics (Kokkos::OpenMP).
+//
+// Pinning of each function (tests/ hold the checks):
+//   PMWeights                 partition of unity, closed-form TSC/CIC values, ip ranges
+//   deposit / gather          brute-force dense G and G^T, lattice uniform deposit,
+//                             sum Q V = sum m c
+//   local u*                  single particle closed form; BS10 deceleration
+//   imex2+ stage sequence     damping ODE vs RK4 (dust_damping.cpp:404-447), momentum
+//                             conservation to round-off, NSH fixed point
+//   dc1 sweep                 brute-force dense solve of A on a 4^3 mesh (error decreases)
+//   consistent-gas split      momentum conservation; parity unpinned beyond that (no closed
+//                             form; restated from src/dust/dust_pm.cpp:624-698)
+//   pc2 predictor/advance     damping ODE (second order), momentum conservation
+//   ghost copy / fold         brute force against an unpadded periodic mesh
+//   shear wrap                position invariants (x in box, y offset = shear offset)
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle.h"
+
+namespace {
+
+constexpr int NG_VARS = DK_NGAS;
+
+inline double SQR(double x) { return x*x; }
+
+// src/dust/dust.hpp:1088-1111
+inline void PMWeights(const double x, const double xmin, const double xmax, const int nx,
+                      const int is, const int scheme, int &ip, double w[3]) {
+  double cell = (x - xmin)*(static_cast<double>(nx)/(xmax - xmin));
+  int ig = static_cast<int>(cell + 1.0) - 1;
+  double del = cell - (static_cast<double>(ig) + 0.5);
+  ip = ig + is;
+  if (scheme == 0) {
+    w[0] = 0.0; w[1] = 1.0; w[2] = 0.0;
+  } else if (scheme == 1) {
+    w[0] = std::fmax(0.0, -del);
+    w[1] = 1.0 - std::fabs(del);
+    w[2] = std::fmax(0.0, del);
+  } else {
+    w[0] = 0.5*SQR(0.5 - del);
+    w[1] = 0.75 - SQR(del);
+    w[2] = 0.5*SQR(0.5 + del);
+  }
+}
+
+// src/pgen/dust_amr/dust_setup.hpp:35-50
+inline double HashUniform01(std::uint64_t key) {
+  key += UINT64_C(0x9e3779b97f4a7c15);
+  key = (key ^ (key >> 30)) * UINT64_C(0xbf58476d1ce4e5b9);
+  key = (key ^ (key >> 27)) * UINT64_C(0x94d049bb133111eb);
+  key ^= key >> 31;
+  return static_cast<double>(key >> 11) * (1.0/9007199254740992.0);
+}
+
+inline double Gauss(std::uint64_t k) {
+  const double u1 = std::fmax(HashUniform01(k), 1.0e-300);
+  const double u2 = HashUniform01(k + UINT64_C(0x1d8e4e27c47d124f));
+  return std::sqrt(-2.0*std::log(u1))*std::cos(2.0*M_PI*u2);
+}
+
+// src/pgen/dust_amr/dust_setup.hpp:58-64
+inline void NSHDustDrift(double omega0, double qshear, double ts, double ux, double uphi,
+                         double &vx, double &vphi) {
+  const double t = omega0*ts, b = 2.0 - qshear;
+  vx = (ux + 2.0*t*uphi)/(1.0 + 2.0*b*t*t);
+  vphi = uphi - b*t*vx;
+}
+
+// src/mesh/nghbr_index.hpp:29-54: the slot of a same-level neighbour (n1 = n2 = 0)
+inline int NeighborIndex(int ix, int iy, int iz) {
+  if (iz == 0) {
+    if (ix*iy == 0) return std::abs(ix)*2*(ix + 1) + std::abs(iy)*2*(iy + 5);
+    return 16 + (ix + 1) + 2*(iy + 1);
+  }
+  if (ix*iy == 0) return 24 + std::abs(ix)*(ix + 9) + std::abs(iy)*(iy + 17) + 2*(iz + 1);
+  return 48 + (ix + 1)/2 + (iy + 1) + 2*(iz + 1);
+}
+
+inline int Dir(int ox1, int ox2, int ox3) { return (ox1 + 1) + 3*(ox2 + 1) + 9*(ox3 + 1); }
+
+}  // namespace
+
+struct or_sim {
+  dk_mesh M;
+  dk_physics P;
+  std::vector<dk_block> B;
+  int ng, nx1, nx2, nx3, n1, n2, n3;
+  int is, ie, js, je, ks, ke;
+  int64_t ncb;   // cells per block and variable, ghosts included
+  std::vector<double> pr;
+  std::vector<int32_t> pi;
+  int64_t cap = 0, np = 0;
+  std::vector<double> u0, u1, qdep, ustar, dmom, ap, resid;
+  std::vector<double> dx1, dx2, dx3;   // per block (meshblock.cpp:113-118)
+  int nthreads = 1;
+  int err = 0;
+  int64_t err_p = -1;
+  double x3_removed_mass = 0.0, x3_removed_count = 0.0;
+  // fold order: the same-level neighbour slots in NeighborIndex order
+  std::vector<int> slot_dir;   // dir index of each used slot, ordered
+
+  int64_t Idx(int m, int nvar, int v, int k, int j, int i) const {
+    return ((((int64_t)m*nvar + v)*n3 + k)*n2 + j)*(int64_t)n1 + i;
+  }
+  double &R(int c, int64_t p) { return pr[(int64_t)c*cap + p]; }
+  int32_t &I(int c, int64_t p) { return pi[(int64_t)c*cap + p]; }
+  void Fail(int code, int64_t p) {
+#pragma omp critical(or_err)
+    {
+      if (err == 0) {err = code; err_p = p;}
+    }
+  }
+};
+
+namespace {
+
+// deposit add: plain on one thread, atomic otherwise (src/dust/dust.hpp:986-998)
+inline void DepositAdd(or_sim *s, double *dst, double v) {
+  if (s->nthreads <= 1) {
+    *dst += v;
+  } else {
+#pragma omp atomic
+    *dst += v;
+  }
+}
+
+bool ActiveStage(const or_sim *s, const dk_stage *st) {   // src/dust/dust.cpp:1646-1649
+  if (s->P.coupling == DK_PC2) return true;
+  return !(st->nexp_stages == 3 && st->stage == st->nexp_stages);
+}
+
+bool MigrationActive(const or_sim *s, const dk_stage *st) {   // src/dust/dust.cpp:1651-1654
+  if (s->P.coupling == DK_IMEX) return ActiveStage(s, st);
+  return st->stage == 2;
+}
+
+double DragStep(const or_sim *s, const dk_stage *st) {   // src/dust/dust.cpp:1704-1707
+  if (s->P.coupling == DK_PC2) return st->dt;
+  return st->a_impl*st->dt;
+}
+
+// src/dust/dust_imex3d.cpp:46-50 and src/dust/dust.hpp:1699-1702
+double ImplicitShearShift(const or_sim *s, const dk_stage *st) {
+  if (!s->P.shearing_box || s->P.coupling != DK_IMEX) return 0.0;
+  double c = 0.0;
+  for (int k=0; k<st->stage; ++k) c = st->gam0[k]*c + st->beta[k];
+  const double timp = st->time + c*st->dt;
+  return s->P.qshear*s->P.omega0*(timp - st->time);
+}
+
+// src/dust/dust.cpp:1668-1676
+double ShearRouteTime(const or_sim *s, const dk_stage *st) {
+  const double t = st->time, dt = st->dt;
+  if (s->P.coupling != DK_IMEX) return t + dt;
+  double w = 0.0;
+  for (int k = st->stage + 1; k <= st->nexp_stages; ++k) w = st->gam0[k-1]*w + st->beta[k-1];
+  return t + (1.0 - w)*dt;
+}
+
+void Zero(std::vector<double> &a) { std::fill(a.begin(), a.end(), 0.0); }
+
+// The three per-axis weight sets of particle p at position (x, y, z) in block m.
+struct Stencil {
+  int ip, jp, kp;
+  double wx[3], wy[3], wz[3];
+};
+
+inline void MakeStencil(or_sim *s, int m, double x, double y, double z, Stencil &w) {
+  const dk_block &b = s->B[m];
+  PMWeights(x, b.x1min, b.x1max, s->nx1, s->is, s->P.deposit, w.ip, w.wx);
+  PMWeights(y, b.x2min, b.x2max, s->nx2, s->js, s->P.deposit, w.jp, w.wy);
+  PMWeights(z, b.x3min, b.x3max, s->nx3, s->ks, s->P.deposit, w.kp, w.wz);
+}
+
+inline bool HaloOK(const or_sim *s, const Stencil &w) {   // src/dust/dust_pm.cpp:241-242
+  return !(w.ip < (s->is-1) || w.ip > (s->is+s->nx1) || w.jp < (s->js-1) ||
+           w.jp > (s->js+s->nx2) || w.kp < (s->ks-1) || w.kp > (s->ks+s->nx3));
+}
+
+//----------------------------------------------------------------------------------------
+// ghost copy exchange of a cell field (periodic same-level), src/bvals/bvals_cc.cpp:53-296;
+// with shear-periodic x1 the x1 ghosts of the blocks on the mesh x1 faces are then replaced
+// by the shear remap, and at physical x3 faces by UstarX3BCs (src/dust/dust_tasks.cpp:445-469)
+void CopyGhosts(or_sim *s, std::vector<double> &a, int nvar) {
+  const int nmb = s->M.nmb;
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m) {
+    for (int k=0; k<s->n3; ++k) {
+      const int ox3 = (k < s->ks) ? -1 : ((k > s->ke) ? 1 : 0);
+      for (int j=0; j<s->n2; ++j) {
+        const int ox2 = (j < s->js) ? -1 : ((j > s->je) ? 1 : 0);
+        for (int i=0; i<s->n1; ++i) {
+          const int ox1 = (i < s->is) ? -1 : ((i > s->ie) ? 1 : 0);
+          if (ox1 == 0 && ox2 == 0 && ox3 == 0) continue;
+          const int nb = s->B[m].nghbr[Dir(ox1, ox2, ox3)];
+          if (nb < 0) continue;   // physical face: set by the boundary conditions
+          for (int v=0; v<nvar; ++v) {
+            a[s->Idx(m,nvar,v,k,j,i)] =
+                a[s->Idx(nb,nvar,v,k-ox3*s->nx3,j-ox2*s->nx2,i-ox1*s->nx1)];
+          }
+        }
+      }
+    }
+  }
+}
+
+// src/dust/dust_tasks.cpp:445-469 (reflect: mirror with v_z negated; else zero gradient)
+void X3BCs(or_sim *s, std::vector<double> &a, int nvar, bool velocity) {
+  if (s->M.bc_x3 == DK_X3_PERIODIC) return;
+  const bool refl = (s->M.bc_x3 == DK_X3_REFLECT);
+  const int ng = s->ng;
+  for (int m=0; m<s->M.nmb; ++m) {
+    const bool lo = (s->B[m].nghbr[Dir(0,0,-1)] < 0);
+    const bool hi = (s->B[m].nghbr[Dir(0,0,1)] < 0);
+    for (int g=0; g<ng; ++g) {
+      for (int j=0; j<s->n2; ++j) {
+        for (int i=0; i<s->n1; ++i) {
+          if (lo) {
+            const int kg = s->ks - 1 - g, kc = refl ? (s->ks + g) : s->ks;
+            for (int v=0; v<nvar; ++v) {
+              const double val = a[s->Idx(m,nvar,v,kc,j,i)];
+              a[s->Idx(m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+            }
+          }
+          if (hi) {
+            const int kg = s->ke + 1 + g, kc = refl ? (s->ke - g) : s->ke;
+            for (int v=0; v<nvar; ++v) {
+              const double val = a[s->Idx(m,nvar,v,kc,j,i)];
+              a[s->Idx(m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// The shear offset of the x1-face crossing at time t folded into [0, Ly)
+// (src/bvals/bvals_part.cpp:138-142)
+double ShearOffset(const or_sim *s, double t) {
+  const double lx = s->M.x1max - s->M.x1min, ly = s->M.x2max - s->M.x2min;
+  return std::fmod(s->P.qshear*s->P.omega0*lx*t, ly);
+}
+
+// Shear-periodic x1 ghost fill of a 3-component velocity field (the u* exchange of the 3-D
+// shearing box, src/shearing_box/shearing_box_cc.cpp:68-260 with PLM remap fluxes,
+// src/shearing_box/remap_fluxes.hpp).  The ghost cells of the blocks on the mesh x1 faces
+// take the values of the opposite face shifted in x2 by the offset at time t, conservatively
+// remapped with piecewise-linear (van Leer limited) reconstruction along x2.
+void ShearGhostsX1(or_sim *s, std::vector<double> &a, int nvar, double t);
+
+// Additive fold of the ghost deposits (src/bvals/bvals_dep.cpp:320-562): every block sums the
+// ghost strips of its same-level neighbours into its active edge strips, buffer by buffer in
+// NeighborIndex order.  With shear-periodic x1 or physical x3 faces the general remap
+// (src/bvals/bvals_dep_general.cpp) replaces the parts that cross those faces.
+void FoldShear(or_sim *s, std::vector<double> &a, int nvar, double t);
+
+void Fold(or_sim *s, std::vector<double> &a, int nvar, double t) {
+  const int ng1 = s->ng - 1;
+  const bool shear = (s->M.bc_x1 == DK_X1_SHEAR);
+  // physical x3 faces first: the ghost deposits beyond the face are mirrored back into the
+  // first ng layers (reflect) or dropped (outflow), over the whole (j, i) plane, so that the
+  // x1/x2 ghost columns then carry them on to the neighbours
+  // (src/bvals/bvals_dep_general.cpp:131-140)
+  if (s->M.bc_x3 != DK_X3_PERIODIC && !s->M.x3_dep_drop) {
+    for (int m=0; m<s->M.nmb; ++m) {
+      const bool lo = (s->B[m].nghbr[Dir(0,0,-1)] < 0);
+      const bool hi = (s->B[m].nghbr[Dir(0,0,1)] < 0);
+      for (int v=0; v<nvar; ++v)
+        for (int g=0; g<s->ng; ++g)
+          for (int j=0; j<s->n2; ++j)
+            for (int i=0; i<s->n1; ++i) {
+              if (lo) a[s->Idx(m,nvar,v,s->ks+g,j,i)] += a[s->Idx(m,nvar,v,s->ks-1-g,j,i)];
+              if (hi) a[s->Idx(m,nvar,v,s->ke-g,j,i)] += a[s->Idx(m,nvar,v,s->ke+1+g,j,i)];
+            }
+    }
+  }
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int mv=0; mv<s->M.nmb*nvar; ++mv) {
+    const int m = mv/nvar, v = mv - m*nvar;
+    for (int d : s->slot_dir) {
+      const int ox1 = d % 3 - 1, ox2 = (d/3) % 3 - 1, ox3 = d/9 - 1;
+      const int nb = s->B[m].nghbr[d];
+      if (nb < 0) continue;
+      // the sender's strip crosses the shear-periodic x1 face: handled by the remap
+      if (shear && ox1 != 0) {
+        const bool crosses = (ox1 < 0) ? (s->B[m].lx1 == 0) : (s->B[m].lx1 == s->M.nmbx1 - 1);
+        if (crosses) continue;
+      }
+      const int il = (ox1 > 0) ? (s->ie - ng1) : s->is, iu = (ox1 < 0) ? (s->is + ng1) : s->ie;
+      const int jl = (ox2 > 0) ? (s->je - ng1) : s->js, ju = (ox2 < 0) ? (s->js + ng1) : s->je;
+      const int kl = (ox3 > 0) ? (s->ke - ng1) : s->ks, ku = (ox3 < 0) ? (s->ks + ng1) : s->ke;
+      for (int k=kl; k<=ku; ++k)
+        for (int j=jl; j<=ju; ++j)
+          for (int i=il; i<=iu; ++i)
+            a[s->Idx(m,nvar,v,k,j,i)] +=
+                a[s->Idx(nb,nvar,v,k-ox3*s->nx3,j-ox2*s->nx2,i-ox1*s->nx1)];
+    }
+  }
+  if (shear) FoldShear(s, a, nvar, t);
+}
+
+}  // namespace
+
+//========================================================================================
+// shear-periodic x1 pieces
+
+namespace {
+
+// van Leer limited slope (the PLM of src/shearing_box/remap_fluxes.hpp)
+inline double VLSlope(double ql, double qc, double qr) {
+  const double dl = qc - ql, dr = qr - qc;
+  const double d2 = dl*dr;
+  return (d2 > 0.0) ? 2.0*d2/(dl + dr) : 0.0;
+}
+
+}  // namespace
+
+namespace {
+
+// The value of a global x2 column (all blocks of one x1 face row, active cells only) of
+// component v at global x2 index gj (periodic), for given block-x3 row and local k, i.
+struct ShearColumn {
+  or_sim *s;
+  const std::vector<double> *a;
+  int nvar, v, bx3, k, i, face;   // face: 0 inner x1 row of blocks, 1 outer
+  int ny;                         // global active cells along x2
+  double operator()(int gj) const {
+    gj = ((gj % ny) + ny) % ny;
+    const int by = gj / s->nx2, jl = gj - by*s->nx2;
+    const int bx = (face == 0) ? 0 : s->M.nmbx1 - 1;
+    const int m = bx + s->M.nmbx1*(by + s->M.nmbx2*bx3);
+    return (*a)[s->Idx(m, nvar, v, k, s->js + jl, i)];
+  }
+};
+
+}  // namespace
+
+namespace {
+
+void ShearGhostsX1(or_sim *s, std::vector<double> &a, int nvar, double t) {
+  // The x1 ghosts of a block on the inner mesh face hold the cells of the outer face shifted
+  // by +ysh in x2 (they are the cells a particle reaches when it crosses inward, whose y
+  // is moved by -ysh: bvals_part.cpp:274-283), and vice versa.
+  const int ny = s->nx2*s->M.nmbx2;
+  const double dy = (s->M.x2max - s->M.x2min)/ny;
+  const double ysh = ShearOffset(s, t);
+  const double shc = ysh/dy;                 // offset in cells
+  const int ish = static_cast<int>(std::floor(shc));
+  const double f = shc - ish;                // fractional part in [0,1)
+  for (int m=0; m<s->M.nmb; ++m) {
+    const dk_block &b = s->B[m];
+    for (int side=0; side<2; ++side) {
+      const bool on_face = (side == 0) ? (b.lx1 == 0) : (b.lx1 == s->M.nmbx1 - 1);
+      if (!on_face) continue;
+      // inner face: the source is the outer face, shifted by +ysh; outer: inner face, -ysh
+      const double sgn = (side == 0) ? 1.0 : -1.0;
+      for (int v=0; v<nvar; ++v) {
+        for (int k=0; k<s->n3; ++k) {
+          const int kk = std::min(std::max(k, s->ks), s->ke);   // x3 ghosts: from the copy
+          for (int g=0; g<s->ng; ++g) {
+            const int ig = (side == 0) ? (s->is - 1 - g) : (s->ie + 1 + g);
+            const int isrc = (side == 0) ? (s->ie - g) : (s->is + g);
+            ShearColumn col{s, &a, nvar, v, b.lx3, kk, isrc, 1 - side, ny};
+            for (int j=0; j<s->n2; ++j) {
+              const int gj = b.lx2*s->nx2 + (j - s->js);
+              // the ghost cell [gj, gj+1) maps to [gj + sgn*shc, gj + 1 + sgn*shc) of the
+              // source column: the average of the PLM reconstruction over that interval
+              const double lo = gj + sgn*shc;
+              const int c0 = static_cast<int>(std::floor(lo));
+              const double fr = lo - c0;           // part of cell c0 not covered
+              // cell c0 covers [c0, c0+1): overlap (1 - fr); cell c0+1: overlap fr
+              const double q0l = col(c0-1), q0 = col(c0), q0r = col(c0+1);
+              const double q1 = q0r, q1r = col(c0+2);
+              const double s0 = VLSlope(q0l, q0, q0r), s1 = VLSlope(q0, q1, q1r);
+              // mean of q0 + s0 (x - 1/2) over x in [fr, 1): q0 + s0*(fr/2)
+              // mean of q1 + s1 (x - 1/2) over x in [0, fr): q1 + s1*(fr - 1)/2
+              const double val = (1.0 - fr)*(q0 + 0.5*s0*fr) + fr*(q1 + 0.5*s1*(fr - 1.0));
+              (void)ish; (void)f;
+              a[s->Idx(m, nvar, v, k, j, ig)] = val;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Ghost deposits beyond a shear-periodic x1 face: a ghost cell of a block on the outer face
+// lies (after x -= Lx, y += ysh: bvals_part.cpp:274-283) on the inner face, shifted by +ysh in
+// x2; it is given to the two active cells it overlaps by the overlap fractions (piecewise-
+// constant remap, conservative: src/bvals/bvals_dep_general.cpp:1-30,100-140).
+void FoldShear(or_sim *s, std::vector<double> &a, int nvar, double t) {
+  const int ny = s->nx2*s->M.nmbx2;
+  const double dy = (s->M.x2max - s->M.x2min)/ny;
+  const double ysh = ShearOffset(s, t);
+  const double shc = ysh/dy;
+  const int ng = s->ng;
+  // Snapshot of every source ghost cell on the two mesh x1 faces (all j, k of the ghost
+  // slabs, including x2/x3 ghost corners), so that the sums read unmodified values.
+  struct Src {int m, k, j, i; double val[8];};
+  std::vector<Src> src;
+  for (int m=0; m<s->M.nmb; ++m) {
+    const dk_block &b = s->B[m];
+    for (int side=0; side<2; ++side) {
+      const bool on_face = (side == 0) ? (b.lx1 == 0) : (b.lx1 == s->M.nmbx1 - 1);
+      if (!on_face) continue;
+      for (int k=0; k<s->n3; ++k)
+        for (int j=0; j<s->n2; ++j)
+          for (int g=0; g<ng; ++g) {
+            const int i = (side == 0) ? (s->is - 1 - g) : (s->ie + 1 + g);
+            Src e{m, k, j, i, {0}};
+            for (int v=0; v<nvar; ++v) e.val[v] = a[s->Idx(m,nvar,v,k,j,i)];
+            src.push_back(e);
+          }
+    }
+  }
+  const int nx3t = s->nx3*s->M.nmbx3;
+  for (const Src &e : src) {
+    const dk_block &b = s->B[e.m];
+    const int side = (e.i < s->is) ? 0 : 1;
+    // global indices of the ghost cell
+    int gk = b.lx3*s->nx3 + (e.k - s->ks);
+    const int gj = b.lx2*s->nx2 + (e.j - s->js);
+    // x1: the opposite face; inner ghost (x < x1min) -> outer active, shifted by -ysh;
+    // outer ghost -> inner active, shifted by +ysh
+    const int bx = (side == 0) ? s->M.nmbx1 - 1 : 0;
+    const int it = (side == 0) ? (s->ie - (s->is - 1 - e.i)) : (s->is + (e.i - s->ie - 1));
+    if (gk < 0 || gk >= nx3t) {
+      if (s->M.bc_x3 == DK_X3_PERIODIC) {
+        gk = (gk + nx3t) % nx3t;
+      } else if (!s->M.x3_dep_drop) {
+        gk = (gk < 0) ? (-1 - gk) : (2*nx3t - 1 - gk);
+      } else {
+        continue;   // dropped
+      }
+    }
+    const int by3 = gk / s->nx3, kt = s->ks + (gk - by3*s->nx3);
+    const double lo = gj + ((side == 0) ? -shc : shc);
+    const int c0 = static_cast<int>(std::floor(lo));
+    const double fr = lo - c0;
+    const int cells[2] = {c0, c0 + 1};
+    const double wts[2] = {1.0 - fr, fr};
+    for (int q=0; q<2; ++q) {
+      if (wts[q] <= 0.0) continue;
+      const int gjt = ((cells[q] % ny) + ny) % ny;
+      const int by = gjt / s->nx2, jt = s->js + (gjt - by*s->nx2);
+      const int mt = bx + s->M.nmbx1*(by + s->M.nmbx2*by3);
+      for (int v=0; v<nvar; ++v) a[s->Idx(mt,nvar,v,kt,jt,it)] += wts[q]*e.val[v];
+    }
+  }
+}
+
+}  // namespace
+
+//========================================================================================
+// construction and storage
+
+extern "C" {
+
+or_sim *or_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phys,
+                  int64_t capacity) {
+  if (mesh == nullptr || blocks == nullptr || phys == nullptr) return nullptr;
+  if (mesh->nx3 < 2 || mesh->ng < 1) return nullptr;   // 3-D meshes only
+  or_sim *s = new or_sim();
+  s->M = *mesh;
+  s->P = *phys;
+  s->B.assign(blocks, blocks + mesh->nmb);
+  s->ng = mesh->ng; s->nx1 = mesh->nx1; s->nx2 = mesh->nx2; s->nx3 = mesh->nx3;
+  s->n1 = s->nx1 + 2*s->ng; s->n2 = s->nx2 + 2*s->ng; s->n3 = s->nx3 + 2*s->ng;
+  s->is = s->ng; s->ie = s->ng + s->nx1 - 1;
+  s->js = s->ng; s->je = s->ng + s->nx2 - 1;
+  s->ks = s->ng; s->ke = s->ng + s->nx3 - 1;
+  s->ncb = (int64_t)s->n1*s->n2*s->n3;
+  const int64_t nc = s->ncb*mesh->nmb;
+  s->cap = std::max<int64_t>(capacity, 1);
+  s->pr.assign(DK_NRDATA*s->cap, 0.0);
+  s->pi.assign(DK_NIDATA*s->cap, 0);
+  s->u0.assign(NG_VARS*nc, 0.0);
+  s->u1.assign(NG_VARS*nc, 0.0);
+  s->qdep.assign(5*nc, 0.0);
+  s->ustar.assign(3*nc, 0.0);
+  s->dmom.assign(3*nc, 0.0);
+  s->ap.assign(3*nc, 0.0);
+  s->resid.assign(4*nc, 0.0);
+  s->dx1.resize(mesh->nmb); s->dx2.resize(mesh->nmb); s->dx3.resize(mesh->nmb);
+  for (int m=0; m<mesh->nmb; ++m) {
+    s->dx1[m] = (s->B[m].x1max - s->B[m].x1min)/static_cast<double>(s->nx1);
+    s->dx2[m] = (s->B[m].x2max - s->B[m].x2min)/static_cast<double>(s->nx2);
+    s->dx3[m] = (s->B[m].x3max - s->B[m].x3min)/static_cast<double>(s->nx3);
+  }
+  // the 26 same-level slots in NeighborIndex order
+  std::vector<std::pair<int,int>> order;
+  for (int ox3=-1; ox3<=1; ++ox3)
+    for (int ox2=-1; ox2<=1; ++ox2)
+      for (int ox1=-1; ox1<=1; ++ox1) {
+        if (ox1 == 0 && ox2 == 0 && ox3 == 0) continue;
+        order.push_back({NeighborIndex(ox1, ox2, ox3), Dir(ox1, ox2, ox3)});
+      }
+  std::sort(order.begin(), order.end());
+  for (auto &o : order) s->slot_dir.push_back(o.second);
+  return s;
+}
+
+void or_destroy(or_sim *s) { delete s; }
+void or_set_threads(or_sim *s, int n) { s->nthreads = std::max(1, n); }
+double *or_rdata(or_sim *s) { return s->pr.data(); }
+int32_t *or_idata(or_sim *s) { return s->pi.data(); }
+int64_t or_capacity(or_sim *s) { return s->cap; }
+int64_t or_npart(or_sim *s) { return s->np; }
+void or_set_npart(or_sim *s, int64_t n) { s->np = std::min(n, s->cap); }
+double *or_u0(or_sim *s) { return s->u0.data(); }
+double *or_u1(or_sim *s) { return s->u1.data(); }
+int64_t or_cells_per_block(or_sim *s) { return s->ncb; }
+
+double *or_field(or_sim *s, int which, int *nvar) {
+  switch (which) {
+    case 0: if (nvar) *nvar = 5; return s->qdep.data();
+    case 1: if (nvar) *nvar = 3; return s->ustar.data();
+    case 2: if (nvar) *nvar = 3; return s->dmom.data();
+    case 3: if (nvar) *nvar = 3; return s->ap.data();
+    case 4: if (nvar) *nvar = 4; return s->resid.data();
+    default: if (nvar) *nvar = 0; return nullptr;
+  }
+}
+
+int or_last_error(or_sim *s, int64_t *particle) {
+  const int e = s->err;
+  if (particle) *particle = s->err_p;
+  s->err = 0; s->err_p = -1;
+  return e;
+}
+
+//========================================================================================
+// problem set-up (dustk_setup.h)
+
+int or_setup(or_sim *s, const dk_setup *su) {
+  const int nmb = s->M.nmb;
+  const int64_t ncell_b = (int64_t)s->nx1*s->nx2*s->nx3;
+  const int64_t npb = (int64_t)su->ppc*ncell_b;
+  const int64_t ntot = npb*nmb;
+  if (ntot > s->cap || su->nspecies < 1 || su->nspecies > DK_MAXSPEC) return DK_ECAPACITY;
+  s->np = ntot;
+  const uint64_t base = su->seed*UINT64_C(0x9e3779b97f4a7c15);
+  const double lxm = s->M.x1max - s->M.x1min, lym = s->M.x2max - s->M.x2min;
+  const double lzm = s->M.x3max - s->M.x3min;
+  const uint64_t saltv[3] = {DK_SALT_V0, DK_SALT_V1, DK_SALT_V2};
+  const uint64_t saltg[3] = {DK_SALT_G0, DK_SALT_G1, DK_SALT_G2};
+  // total gas mass (zmetal normalisation)
+  double gas_mass = 0.0;
+  for (int m=0; m<nmb; ++m) {
+    const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+    for (int k=s->ks; k<=s->ke; ++k) {
+      const double z = s->B[m].x3min + (k - s->ks + 0.5)*s->dx3[m];
+      const double rho = su->gas_strat ? su->rho0*std::exp(-0.5*z*z) : su->rho0;
+      gas_mass += rho*vol*s->nx1*s->nx2;
+    }
+  }
+  // mean mass weight of each continuous bin: <tau^slope> over log-uniform tau (closed form)
+  double wmean[DK_MAXSPEC];
+  for (int sp=0; sp<su->nspecies; ++sp) {
+    wmean[sp] = 1.0;
+    if (su->continuous && su->mass_slope != 0.0) {
+      const double a = su->mass_slope, l = std::log(su->tau_hi[sp]/su->tau_lo[sp]);
+      wmean[sp] = (std::pow(su->tau_hi[sp], a) - std::pow(su->tau_lo[sp], a))/(a*l);
+    }
+  }
+  const double ppc_s = static_cast<double>(su->ppc)/su->nspecies;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t q=0; q<ntot; ++q) {
+    const uint64_t key = static_cast<uint64_t>(q) + base;
+    int m = static_cast<int>(q/npb);
+    const int sp = static_cast<int>(q % su->nspecies);
+    double x, y, z;
+    if (su->nclump > 0) {
+      if (HashUniform01(key + DK_SALT_CLUMP) < su->clump_frac) {
+        const int ci = std::min(su->nclump - 1, static_cast<int>(
+            HashUniform01(key + DK_SALT_CLUMP + 1)*su->nclump));
+        const uint64_t ck = base + static_cast<uint64_t>(ci)*UINT64_C(0x2545f4914f6cdd1d);
+        const double cx = s->M.x1min + HashUniform01(ck + UINT64_C(0x632be59bd9b4e019))*lxm;
+        const double cy = s->M.x2min + HashUniform01(ck + UINT64_C(0x8cb92baa3f3d8dd7))*lym;
+        const double cz = s->M.x3min + HashUniform01(ck + UINT64_C(0x58f38ded6f7c55b5))*lzm;
+        x = cx + su->clump_sigma*Gauss(key + UINT64_C(0x632be59bd9b4e019));
+        y = cy + su->clump_sigma*Gauss(key + UINT64_C(0x8cb92baa3f3d8dd7));
+        z = cz + su->clump_sigma*Gauss(key + UINT64_C(0x58f38ded6f7c55b5));
+        x -= lxm*std::floor((x - s->M.x1min)/lxm);
+        y -= lym*std::floor((y - s->M.x2min)/lym);
+        z -= lzm*std::floor((z - s->M.x3min)/lzm);
+        if (x >= s->M.x1max) x = s->M.x1min;
+        if (y >= s->M.x2max) y = s->M.x2min;
+        if (z >= s->M.x3max) z = s->M.x3min;
+      } else {
+        x = s->M.x1min + HashUniform01(key + UINT64_C(0x632be59bd9b4e019))*lxm;
+        y = s->M.x2min + HashUniform01(key + UINT64_C(0x8cb92baa3f3d8dd7))*lym;
+        z = s->M.x3min + HashUniform01(key + UINT64_C(0x58f38ded6f7c55b5))*lzm;
+      }
+      // the block holding the position (uniform mesh, gid = bx + nx*(by + ny*bz))
+      const int bx = std::min(s->M.nmbx1 - 1, static_cast<int>((x - s->M.x1min)/lxm*s->M.nmbx1));
+      const int by = std::min(s->M.nmbx2 - 1, static_cast<int>((y - s->M.x2min)/lym*s->M.nmbx2));
+      const int bz = std::min(s->M.nmbx3 - 1, static_cast<int>((z - s->M.x3min)/lzm*s->M.nmbx3));
+      m = bx + s->M.nmbx1*(by + s->M.nmbx2*bz);
+      // a position on a block face by round-off: keep it inside [xmin, xmax) of the block
+      const dk_block &b = s->B[m];
+      if (x < b.x1min) x = b.x1min;
+      if (x >= b.x1max) x = std::nextafter(b.x1max, b.x1min);
+      if (y < b.x2min) y = b.x2min;
+      if (y >= b.x2max) y = std::nextafter(b.x2max, b.x2min);
+      if (z < b.x3min) z = b.x3min;
+      if (z >= b.x3max) z = std::nextafter(b.x3max, b.x3min);
+    } else {
+      const dk_block &b = s->B[m];
+      const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
+      x = b.x1min + HashUniform01(key + UINT64_C(0x632be59bd9b4e019))*lx;
+      y = b.x2min + HashUniform01(key + UINT64_C(0x8cb92baa3f3d8dd7))*ly;
+      if (su->gauss_z) {
+        // Gaussian heights over the whole mesh (dust_setup.hpp:157-171), then the block
+        z = 0.0;
+        for (int t=0; t<64; ++t) {
+          const uint64_t kz = key + UINT64_C(0x58f38ded6f7c55b5) +
+                              static_cast<uint64_t>(t)*UINT64_C(0x2545f4914f6cdd1d);
+          const double u1 = std::fmax(HashUniform01(kz), 1.0e-300);
+          const double u2 = HashUniform01(kz + UINT64_C(0x1d8e4e27c47d124f));
+          const double zz = su->hp*std::sqrt(-2.0*std::log(u1))*std::cos(2.0*M_PI*u2);
+          if (zz >= s->M.x3min && zz < s->M.x3max) {z = zz; break;}
+        }
+        const int bz = std::min(s->M.nmbx3 - 1,
+                                static_cast<int>((z - s->M.x3min)/lzm*s->M.nmbx3));
+        m = b.lx1 + s->M.nmbx1*(b.lx2 + s->M.nmbx2*bz);
+        const dk_block &bb = s->B[m];
+        if (z < bb.x3min) z = bb.x3min;
+        if (z >= bb.x3max) z = std::nextafter(bb.x3max, bb.x3min);
+      } else {
+        z = b.x3min + HashUniform01(key + UINT64_C(0x58f38ded6f7c55b5))*lz;
+      }
+    }
+    // stopping time and mass
+    double ts = su->taus[sp], wm = 1.0;
+    if (su->continuous) {
+      const double u = HashUniform01(key + DK_SALT_TAU);
+      ts = std::exp(std::log(su->tau_lo[sp]) + u*std::log(su->tau_hi[sp]/su->tau_lo[sp]));
+      if (su->mass_slope != 0.0) wm = std::pow(ts, su->mass_slope)/wmean[sp];
+    }
+    const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+    double mass;
+    if (su->zmetal > 0.0) {
+      mass = wm*su->zmetal*gas_mass/(static_cast<double>(ntot));
+    } else {
+      mass = wm*su->eps[sp]*su->rho0*vol/ppc_s;
+    }
+    double vm[3] = {su->gas_vx, su->gas_vy, su->gas_vz};
+    if (su->nsh) {
+      NSHDustDrift(su->omega0, su->qshear, ts, su->gas_vx, su->gas_vy, vm[0], vm[1]);
+      vm[2] = 0.0;
+    }
+    double v[3];
+    for (int c=0; c<3; ++c) v[c] = vm[c] + su->dust_noise*Gauss(key + saltv[c]);
+    s->R(DK_IPX,q) = x; s->R(DK_IPY,q) = y; s->R(DK_IPZ,q) = z;
+    s->R(DK_IPVX,q) = v[0]; s->R(DK_IPVY,q) = v[1]; s->R(DK_IPVZ,q) = v[2];
+    s->R(DK_IPX1,q) = x; s->R(DK_IPY1,q) = y; s->R(DK_IPZ1,q) = z;
+    s->R(DK_IPVX1,q) = v[0]; s->R(DK_IPVY1,q) = v[1]; s->R(DK_IPVZ1,q) = v[2];
+    s->R(DK_IPRX,q) = 0.0; s->R(DK_IPRY,q) = 0.0; s->R(DK_IPRZ,q) = 0.0;
+    s->R(DK_IPTS,q) = ts; s->R(DK_IPM,q) = mass;
+    s->I(DK_PGID,q) = m; s->I(DK_PTAG,q) = static_cast<int32_t>(q);
+    s->I(DK_PSP,q) = sp; s->I(DK_PLEV,q) = 0; s->I(DK_PTAGH,q) = 0;
+  }
+  // gas
+  const int ny = s->nx2*s->M.nmbx2, nxg = s->nx1*s->M.nmbx1;
+  Zero(s->u0);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m) {
+    const dk_block &b = s->B[m];
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i) {
+          const int64_t gi = b.lx1*s->nx1 + (i - s->is), gj = b.lx2*s->nx2 + (j - s->js);
+          const int64_t gk = b.lx3*s->nx3 + (k - s->ks);
+          const uint64_t ck = static_cast<uint64_t>((gk*ny + gj)*nxg + gi) + base;
+          const double z = b.x3min + (k - s->ks + 0.5)*s->dx3[m];
+          const double rho = su->gas_strat ? su->rho0*std::exp(-0.5*z*z) : su->rho0;
+          const double um[3] = {su->gas_vx, su->gas_vy, su->gas_vz};
+          s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)] = rho;
+          for (int c=0; c<3; ++c) {
+            const double u = um[c] + su->gas_noise*Gauss(ck + saltg[c]);
+            s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)] = rho*u;
+          }
+        }
+  }
+  CopyGhosts(s, s->u0, NG_VARS);
+  if (s->M.bc_x1 == DK_X1_SHEAR) ShearGhostsX1(s, s->u0, NG_VARS, 0.0);
+  X3BCs(s, s->u0, NG_VARS, false);
+  s->u1 = s->u0;
+  return DK_OK;
+}
+
+//========================================================================================
+// tasks
+
+// src/dust/dust_push.cpp:35-80
+int or_first_two_imp_rk(or_sim *s, const dk_stage *st) {
+  if (s->P.coupling == DK_PC2 && st->stage == 2) {
+    const int64_t nc = s->ncb*s->M.nmb;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int64_t c=0; c<nc; ++c) {
+      const int64_t m = c/s->ncb, r = c - m*s->ncb;
+      auto U = [&](std::vector<double> &u, int v) -> double& {
+        return u[(m*NG_VARS + v)*s->ncb + r];
+      };
+      double inv_rho_sum = 1.0/(U(s->u1,DK_IDN) + U(s->u0,DK_IDN));
+      for (int v=0; v<3; ++v) {
+        s->ustar[(m*3 + v)*s->ncb + r] = (U(s->u1,DK_IM1+v) + U(s->u0,DK_IM1+v))*inv_rho_sum;
+      }
+    }
+    return DK_OK;
+  }
+  if (st->stage != 1) return DK_OK;
+  s->u1 = s->u0;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t p=0; p<s->np; ++p) {
+    s->R(DK_IPX1,p) = s->R(DK_IPX,p);
+    s->R(DK_IPVX1,p) = s->R(DK_IPVX,p);
+    s->R(DK_IPY1,p) = s->R(DK_IPY,p);
+    s->R(DK_IPVY1,p) = s->R(DK_IPVY,p);
+    s->R(DK_IPZ1,p) = s->R(DK_IPZ,p);
+    s->R(DK_IPVZ1,p) = s->R(DK_IPVZ,p);
+  }
+  return DK_OK;
+}
+
+// src/dust/dust_push.cpp:91-300
+int or_explicit_push(or_sim *s, const dk_stage *st) {
+  if (!ActiveStage(s, st)) return DK_OK;
+  const bool sbox = s->P.shearing_box, strat = s->P.stratified;
+  const double qs = s->P.qshear, om = s->P.omega0;
+  if (s->P.coupling == DK_PC2) {
+    if (st->stage != 2) return DK_OK;
+    const double dt = st->dt;
+    const bool br = s->P.back_reaction;
+    if (br) Zero(s->dmom);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int64_t p=0; p<s->np; ++p) {
+      const int m = s->I(DK_PGID,p);
+      const double x0 = s->R(DK_IPX1,p), y0 = s->R(DK_IPY1,p), z0 = s->R(DK_IPZ1,p);
+      const double vx0 = s->R(DK_IPVX1,p), vy0 = s->R(DK_IPVY1,p), vz0 = s->R(DK_IPVZ1,p);
+      const double xh = x0 + 0.5*dt*vx0;
+      const double yh = y0 + 0.5*dt*(vy0 - (sbox ? qs*om*x0 : 0.0));
+      const double zh = z0 + 0.5*dt*vz0;
+      const double yg = yh + (sbox ? qs*om*xh*0.5*dt : 0.0);
+      Stencil w;
+      MakeStencil(s, m, xh, yg, zh, w);
+      double ugx = 0.0, ugy = 0.0, ugz = 0.0;
+      for (int c=0; c<=2; ++c)
+        for (int b=0; b<3; ++b) {
+          const double wcb = w.wz[c]*w.wy[b];
+          if (wcb == 0.0) continue;
+          for (int a=0; a<3; ++a) {
+            const double ww = wcb*w.wx[a];
+            const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+            ugx += ww*s->ustar[s->Idx(m,3,0,kk,jj,ii)];
+            ugy += ww*s->ustar[s->Idx(m,3,1,kk,jj,ii)];
+            ugz += ww*s->ustar[s->Idx(m,3,2,kk,jj,ii)];
+          }
+        }
+      const double h = 0.5*dt;
+      const double lambda = h/s->R(DK_IPTS,p);
+      const double aa = 1.0 + lambda;
+      const double rx = vx0 + lambda*ugx;
+      const double ry = vy0 + lambda*ugy;
+      double rz = vz0 + lambda*ugz;
+      double vxh, vyh, vzh;
+      if (sbox) {
+        const double cxy = 2.0*h*om;
+        const double cyx = h*(2.0-qs)*om;
+        const double det = aa*aa + cxy*cyx;
+        vxh = (aa*rx + cxy*ry)/det;
+        vyh = (aa*ry - cyx*rx)/det;
+        if (strat) rz -= h*SQR(om)*zh;
+        vzh = rz/aa;
+      } else {
+        vxh = rx/aa; vyh = ry/aa; vzh = rz/aa;
+      }
+      s->R(DK_IPVX,p) = 2.0*vxh - vx0;
+      s->R(DK_IPVY,p) = 2.0*vyh - vy0;
+      s->R(DK_IPVZ,p) = 2.0*vzh - vz0;
+      s->R(DK_IPX,p) = x0 + dt*vxh;
+      s->R(DK_IPY,p) = y0 + dt*(vyh - (sbox ? qs*om*xh : 0.0));
+      s->R(DK_IPZ,p) = z0 + dt*vzh;
+      s->R(DK_IPX1,p) = xh; s->R(DK_IPY1,p) = yh; s->R(DK_IPZ1,p) = zh;
+      const double ts = s->R(DK_IPTS,p);
+      const double dvx = dt*(ugx-vxh)/ts, dvy = dt*(ugy-vyh)/ts, dvz = dt*(ugz-vzh)/ts;
+      s->R(DK_IPRX,p) = dvx; s->R(DK_IPRY,p) = dvy; s->R(DK_IPRZ,p) = dvz;
+      if (br) {
+        const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+        const double fac = -s->R(DK_IPM,p)/vol;
+        for (int c=0; c<=2; ++c)
+          for (int b=0; b<3; ++b) {
+            const double wcb = w.wz[c]*w.wy[b]*fac;
+            if (wcb == 0.0) continue;
+            for (int a=0; a<3; ++a) {
+              const double ww = wcb*w.wx[a];
+              const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+              DepositAdd(s, &s->dmom[s->Idx(m,3,0,kk,jj,ii)], ww*dvx);
+              DepositAdd(s, &s->dmom[s->Idx(m,3,1,kk,jj,ii)], ww*dvy);
+              DepositAdd(s, &s->dmom[s->Idx(m,3,2,kk,jj,ii)], ww*dvz);
+            }
+          }
+      }
+    }
+    return DK_OK;
+  }
+  const double dt = st->dt;
+  const double gam0 = st->gam0[st->stage-1], gam1 = st->gam1[st->stage-1];
+  const double beta_dt = st->beta[st->stage-1]*dt;
+  const double atw_dt = (st->stage == 2) ? st->a_twid22*dt : 0.0;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t p=0; p<s->np; ++p) {
+    const double x_old = s->R(DK_IPX,p), y_old = s->R(DK_IPY,p), z_old = s->R(DK_IPZ,p);
+    const double vx_old = s->R(DK_IPVX,p), vy_old = s->R(DK_IPVY,p);
+    const double vz_old = s->R(DK_IPVZ,p);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (sbox) {
+      fx = 2.0*om*vy_old;
+      fy = -(2.0-qs)*om*vx_old;
+      if (strat) fz = -SQR(om)*z_old;
+    }
+    s->R(DK_IPVX,p) = gam0*vx_old + gam1*s->R(DK_IPVX1,p) + beta_dt*fx + atw_dt*s->R(DK_IPRX,p);
+    s->R(DK_IPVY,p) = gam0*vy_old + gam1*s->R(DK_IPVY1,p) + beta_dt*fy + atw_dt*s->R(DK_IPRY,p);
+    s->R(DK_IPVZ,p) = gam0*vz_old + gam1*s->R(DK_IPVZ1,p) + beta_dt*fz + atw_dt*s->R(DK_IPRZ,p);
+    double ydot = vy_old;
+    if (sbox) ydot -= qs*om*x_old;
+    s->R(DK_IPX,p) = gam0*x_old + gam1*s->R(DK_IPX1,p) + beta_dt*vx_old;
+    s->R(DK_IPY,p) = gam0*y_old + gam1*s->R(DK_IPY1,p) + beta_dt*ydot;
+    s->R(DK_IPZ,p) = gam0*z_old + gam1*s->R(DK_IPZ1,p) + beta_dt*vz_old;
+  }
+  return DK_OK;
+}
+
+// src/dust/dust_tasks.cpp:149-162 -> src/dust/dust.cpp:1678-1702 and
+// src/bvals/bvals_part.cpp:115-471 (uniform mesh, one rank)
+int or_update_particle_gids(or_sim *s, const dk_stage *st) {
+  if (!MigrationActive(s, st)) return DK_OK;
+  const dk_mesh &ms = s->M;
+  const bool shear = (ms.bc_x1 == DK_X1_SHEAR);
+  if (shear) {   // ShiftShearCrossings(ShearRouteTime)
+    const double tau = ShearRouteTime(s, st);
+    const double dsh = s->P.qshear*s->P.omega0*(ms.x1max - ms.x1min)*(tau - st->time);
+    if (dsh != 0.0) {
+      const double xa = ms.x1min, xb = ms.x1max;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+      for (int64_t p=0; p<s->np; ++p) {
+        const int m = s->I(DK_PGID,p);
+        const double b0 = s->B[m].x1min, b1 = s->B[m].x1max, lx = b1 - b0;
+        const int ix = static_cast<int>((s->R(DK_IPX,p) - b0 + lx)/lx) - 1;
+        const bool lo1 = (ix < 0) && (b0 < xa + 0.5*lx);
+        const bool hi1 = (ix > 0) && (b1 > xb - 0.5*lx);
+        if (lo1 || hi1) {
+          const double d = hi1 ? dsh : -dsh;
+          s->R(DK_IPY,p) += d;
+          s->R(DK_IPY1,p) += d;
+        }
+      }
+    }
+  }
+  double ysh = 0.0;
+  if (shear) ysh = ShearOffset(s, st->time);
+  const int x3bc = ms.bc_x3;
+  const double z3a = ms.x3min, z3b = ms.x3max;
+  double rm_mass = 0.0, rm_count = 0.0;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(+:rm_mass,rm_count)
+  for (int64_t p=0; p<s->np; ++p) {
+    const int m = s->I(DK_PGID,p);
+    const dk_block &b = s->B[m];
+    if (x3bc != 0 && (s->R(DK_IPZ,p) < z3a || s->R(DK_IPZ,p) >= z3b)) {
+      const bool lo = (s->R(DK_IPZ,p) < z3a);
+      if (x3bc == DK_X3_REFLECT) {
+        const double zf = lo ? z3a : z3b;
+        double z = 2.0*zf - s->R(DK_IPZ,p);
+        if (z >= z3b) z = z3b - 1.0e-12*(z3b - z3a);
+        if (z < z3a) z = z3a;
+        s->R(DK_IPZ,p) = z;
+        s->R(DK_IPVZ,p) = -s->R(DK_IPVZ,p);
+        s->R(DK_IPZ1,p) = 2.0*zf - s->R(DK_IPZ1,p);
+        s->R(DK_IPVZ1,p) = -s->R(DK_IPVZ1,p);
+      } else {
+        rm_mass += s->R(DK_IPM,p);
+        rm_count += 1.0;
+        s->R(DK_IPM,p) = 0.0;
+        const double dz = s->dx3[m];
+        const double z = lo ? (b.x3min + 0.5*dz) : (b.x3max - 0.5*dz);
+        s->R(DK_IPZ,p) = z;
+        s->R(DK_IPVX,p) = 0.0; s->R(DK_IPVY,p) = 0.0; s->R(DK_IPVZ,p) = 0.0;
+        s->R(DK_IPX1,p) = s->R(DK_IPX,p); s->R(DK_IPY1,p) = s->R(DK_IPY,p);
+        s->R(DK_IPZ1,p) = z;
+        s->R(DK_IPVX1,p) = 0.0; s->R(DK_IPVY1,p) = 0.0; s->R(DK_IPVZ1,p) = 0.0;
+      }
+    }
+    double x1 = s->R(DK_IPX,p), x2 = s->R(DK_IPY,p), x3 = s->R(DK_IPZ,p);
+    const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
+    const int ix = static_cast<int>((x1 - b.x1min + lx)/lx) - 1;
+    const int iy = static_cast<int>((x2 - b.x2min + ly)/ly) - 1;
+    const int iz = static_cast<int>((x3 - b.x3min + lz)/lz) - 1;
+    auto snap = [&](const int off, double &x, const double xa, const double xb, const int ip,
+                    const int ir) {
+      if (off != 0) return;
+      double xs = x;
+      if (x < xa) {
+        xs = xa;
+      } else if (x >= xb) {
+        xs = std::nextafter(xb, xa);
+      }
+      if (xs != x) {
+        s->R(ir,p) += xs - x;
+        s->R(ip,p) = xs;
+        x = xs;
+      }
+    };
+    snap(ix, x1, b.x1min, b.x1max, DK_IPX, DK_IPX1);
+    snap(iy, x2, b.x2min, b.x2max, DK_IPY, DK_IPY1);
+    snap(iz, x3, b.x3min, b.x3max, DK_IPZ, DK_IPZ1);
+    if (std::abs(ix) > 1 || std::abs(iy) > 1 || std::abs(iz) > 1) {
+      s->Fail(DK_EROUTE, p);
+      continue;
+    }
+    if ((std::abs(ix) + std::abs(iy) + std::abs(iz)) != 0) {
+      const bool lo1 = (ix < 0) && (b.x1min < ms.x1min + 0.5*lx);
+      const bool hi1 = (ix > 0) && (b.x1max > ms.x1max - 0.5*lx);
+      const bool lo2 = (iy < 0) && (b.x2min < ms.x2min + 0.5*ly);
+      const bool hi2 = (iy > 0) && (b.x2max > ms.x2max - 0.5*ly);
+      const bool lo3 = (iz < 0) && (b.x3min < ms.x3min + 0.5*lz);
+      const bool hi3 = (iz > 0) && (b.x3max > ms.x3max - 0.5*lz);
+      const bool cross_in = shear && lo1, cross_out = shear && hi1;
+      if (cross_in || cross_out) {
+        const double lxm = ms.x1max - ms.x1min;
+        s->R(DK_IPX,p) += cross_in ? lxm : -lxm;
+        s->R(DK_IPX1,p) += cross_in ? lxm : -lxm;
+        const double lym = ms.x2max - ms.x2min;
+        double ynew = s->R(DK_IPY,p) + (cross_in ? -ysh : ysh);
+        if (ynew < ms.x2min) ynew += lym;
+        if (ynew >= ms.x2max) ynew -= lym;
+        const double dy = ynew - s->R(DK_IPY,p);
+        s->R(DK_IPY,p) = ynew;
+        s->R(DK_IPY1,p) += dy;
+        const double lzm = ms.x3max - ms.x3min;
+        if (lo3) {
+          s->R(DK_IPZ,p) += lzm; s->R(DK_IPZ1,p) += lzm;
+        } else if (hi3) {
+          s->R(DK_IPZ,p) -= lzm; s->R(DK_IPZ1,p) -= lzm;
+        }
+        int b2 = static_cast<int>(((s->R(DK_IPY,p) - ms.x2min)/lym)*static_cast<double>(ms.nmbx2));
+        b2 = (b2 < 0) ? 0 : ((b2 > ms.nmbx2-1) ? (ms.nmbx2-1) : b2);
+        int b3 = static_cast<int>(((s->R(DK_IPZ,p) - ms.x3min)/lzm)*static_cast<double>(ms.nmbx3));
+        b3 = (b3 < 0) ? 0 : ((b3 > ms.nmbx3-1) ? (ms.nmbx3-1) : b3);
+        const int b1 = cross_in ? (ms.nmbx1 - 1) : 0;   // the boundary block on the other side
+        s->I(DK_PGID,p) = b1 + ms.nmbx1*(b2 + ms.nmbx2*b3);
+      } else {
+        const int nb = b.nghbr[Dir(ix, iy, iz)];
+        if (nb < 0) {
+          s->Fail(DK_EROUTE, p);
+          continue;
+        }
+        s->I(DK_PGID,p) = nb;
+        if (lo1) {
+          s->R(DK_IPX,p) += (ms.x1max - ms.x1min); s->R(DK_IPX1,p) += (ms.x1max - ms.x1min);
+        } else if (hi1) {
+          s->R(DK_IPX,p) -= (ms.x1max - ms.x1min); s->R(DK_IPX1,p) -= (ms.x1max - ms.x1min);
+        }
+        if (lo2) {
+          s->R(DK_IPY,p) += (ms.x2max - ms.x2min); s->R(DK_IPY1,p) += (ms.x2max - ms.x2min);
+        } else if (hi2) {
+          s->R(DK_IPY,p) -= (ms.x2max - ms.x2min); s->R(DK_IPY1,p) -= (ms.x2max - ms.x2min);
+        }
+        if (lo3) {
+          s->R(DK_IPZ,p) += (ms.x3max - ms.x3min); s->R(DK_IPZ1,p) += (ms.x3max - ms.x3min);
+        } else if (hi3) {
+          s->R(DK_IPZ,p) -= (ms.x3max - ms.x3min); s->R(DK_IPZ1,p) -= (ms.x3max - ms.x3min);
+        }
+      }
+    }
+  }
+  s->x3_removed_mass += rm_mass;
+  s->x3_removed_count += rm_count;
+  return s->err;
+}
+
+// src/dust/dust_pm.cpp:104-277
+int or_deposit_drag(or_sim *s, const dk_stage *st) {
+  if (!ActiveStage(s, st) || !s->P.back_reaction) return DK_OK;
+  const bool rate_dep = s->P.stiffness;
+  if (s->P.coupling == DK_PC2) {
+    if (st->stage != 1) return DK_OK;
+    Zero(s->qdep);
+    const double dt = st->dt;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int64_t p=0; p<s->np; ++p) {
+      const int m = s->I(DK_PGID,p);
+      Stencil w;
+      MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p), w);
+      double ugx = 0.0, ugy = 0.0, ugz = 0.0;
+      for (int c=0; c<=2; ++c)
+        for (int b=0; b<3; ++b) {
+          const double wcb = w.wz[c]*w.wy[b];
+          if (wcb == 0.0) continue;
+          for (int a=0; a<3; ++a) {
+            const double ww = wcb*w.wx[a];
+            const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+            const double invrho = 1.0/s->u1[s->Idx(m,NG_VARS,DK_IDN,kk,jj,ii)];
+            ugx += ww*s->u1[s->Idx(m,NG_VARS,DK_IM1,kk,jj,ii)]*invrho;
+            ugy += ww*s->u1[s->Idx(m,NG_VARS,DK_IM2,kk,jj,ii)]*invrho;
+            ugz += ww*s->u1[s->Idx(m,NG_VARS,DK_IM3,kk,jj,ii)]*invrho;
+          }
+        }
+      const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+      const double rate = s->R(DK_IPM,p)/(s->R(DK_IPTS,p)*vol);
+      const double gx = -dt*rate*(ugx-s->R(DK_IPVX,p));
+      const double gy = -dt*rate*(ugy-s->R(DK_IPVY,p));
+      const double gz = -dt*rate*(ugz-s->R(DK_IPVZ,p));
+      for (int c=0; c<=2; ++c)
+        for (int b=0; b<3; ++b) {
+          const double wcb = w.wz[c]*w.wy[b];
+          if (wcb == 0.0) continue;
+          for (int a=0; a<3; ++a) {
+            const double ww = wcb*w.wx[a];
+            const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+            DepositAdd(s, &s->qdep[s->Idx(m,5,1,kk,jj,ii)], ww*gx);
+            DepositAdd(s, &s->qdep[s->Idx(m,5,2,kk,jj,ii)], ww*gy);
+            DepositAdd(s, &s->qdep[s->Idx(m,5,3,kk,jj,ii)], ww*gz);
+            if (rate_dep) DepositAdd(s, &s->qdep[s->Idx(m,5,4,kk,jj,ii)], ww*rate);
+          }
+        }
+    }
+    return DK_OK;
+  }
+  Zero(s->qdep);
+  const double a_dt = DragStep(s, st);
+  const double sh = ImplicitShearShift(s, st);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t p=0; p<s->np; ++p) {
+    const int m = s->I(DK_PGID,p);
+    Stencil w;
+    MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p) + sh*s->R(DK_IPX,p), s->R(DK_IPZ,p), w);
+    if (!HaloOK(s, w)) {s->Fail(DK_EHALO, p); continue;}
+    const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+    const double cj = a_dt/(s->R(DK_IPTS,p) + a_dt);
+    const double muc = s->R(DK_IPM,p)*cj/vol;
+    const double vx = s->R(DK_IPVX,p), vy = s->R(DK_IPVY,p), vz = s->R(DK_IPVZ,p);
+    for (int c=0; c<=2; ++c)
+      for (int b=0; b<3; ++b) {
+        const double wcb = w.wz[c]*w.wy[b]*muc;
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+          DepositAdd(s, &s->qdep[s->Idx(m,5,0,kk,jj,ii)], ww);
+          DepositAdd(s, &s->qdep[s->Idx(m,5,1,kk,jj,ii)], ww*vx);
+          DepositAdd(s, &s->qdep[s->Idx(m,5,2,kk,jj,ii)], ww*vy);
+          DepositAdd(s, &s->qdep[s->Idx(m,5,3,kk,jj,ii)], ww*vz);
+        }
+      }
+  }
+  return s->err;
+}
+
+// src/dust/dust_tasks.cpp:249-263 (the shear part of the remap at t^n: InitRecvDep,
+// dust_tasks.cpp:223-225)
+int or_fold_deposit_qp(or_sim *s, const dk_stage *st) {
+  const bool use = (s->P.coupling == DK_IMEX) || st->stage == 1;
+  if (!ActiveStage(s, st) || !s->P.back_reaction || !use) return DK_OK;
+  Fold(s, s->qdep, 5, st->time);
+  return DK_OK;
+}
+
+namespace {
+
+// src/dust/dust_solver.cpp:126-203 (PointOperatorParticles): result += G^T (m c/V) G field
+void PointOperatorParticles(or_sim *s, const std::vector<double> &field,
+                            std::vector<double> &result, double a_dt, double sh) {
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t p=0; p<s->np; ++p) {
+    const int m = s->I(DK_PGID,p);
+    Stencil w;
+    MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p) + sh*s->R(DK_IPX,p), s->R(DK_IPZ,p), w);
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int c=0; c<=2; ++c)
+      for (int b=0; b<3; ++b) {
+        const double wcb = w.wz[c]*w.wy[b];
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+          g[0] += ww*field[s->Idx(m,3,0,kk,jj,ii)];
+          g[1] += ww*field[s->Idx(m,3,1,kk,jj,ii)];
+          g[2] += ww*field[s->Idx(m,3,2,kk,jj,ii)];
+        }
+      }
+    const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+    const double cj = a_dt/(s->R(DK_IPTS,p) + a_dt);
+    const double fac = s->R(DK_IPM,p)*cj/vol;
+    for (int c=0; c<=2; ++c)
+      for (int b=0; b<3; ++b) {
+        const double wcb = w.wz[c]*w.wy[b]*fac;
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+          DepositAdd(s, &result[s->Idx(m,3,0,kk,jj,ii)], ww*g[0]);
+          DepositAdd(s, &result[s->Idx(m,3,1,kk,jj,ii)], ww*g[1]);
+          DepositAdd(s, &result[s->Idx(m,3,2,kk,jj,ii)], ww*g[2]);
+        }
+      }
+  }
+}
+
+// the u* ghost fill of the stage (periodic copy, shear remap at t^n, x3 values)
+void FillVelocityGhosts(or_sim *s, std::vector<double> &f, double t) {
+  CopyGhosts(s, f, 3);
+  if (s->M.bc_x1 == DK_X1_SHEAR) ShearGhostsX1(s, f, 3, t);
+  X3BCs(s, f, 3, true);
+}
+
+// src/dust/dust_solver.cpp:226-263 (ApplyCoupledOperatorImpl)
+void ApplyCoupledOperatorImpl(or_sim *s, std::vector<double> &field, std::vector<double> &result,
+                              double a_dt, double sh, bool correct, double t) {
+  FillVelocityGhosts(s, field, t);
+  Zero(result);
+  PointOperatorParticles(s, field, result, a_dt, sh);
+  Fold(s, result, 3, t);
+  const int nmb = s->M.nmb;
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i) {
+          const double rho = s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+          if (correct) {
+            const double pinv = 1.0/(rho + s->qdep[s->Idx(m,5,0,k,j,i)]);
+            for (int d=0; d<3; ++d) {
+              const double av = result[s->Idx(m,3,d,k,j,i)] + rho*field[s->Idx(m,3,d,k,j,i)];
+              const double rv = s->u0[s->Idx(m,NG_VARS,DK_IM1+d,k,j,i)] +
+                                s->qdep[s->Idx(m,5,1+d,k,j,i)] - av;
+              field[s->Idx(m,3,d,k,j,i)] += pinv*rv;
+            }
+          } else {
+            for (int d=0; d<3; ++d)
+              result[s->Idx(m,3,d,k,j,i)] += rho*field[s->Idx(m,3,d,k,j,i)];
+          }
+        }
+}
+
+}  // namespace
+
+// src/dust/dust_pm.cpp:372-442 and src/dust/dust_solver.cpp:630-724
+int or_gas_implicit_solve(or_sim *s, const dk_stage *st) {
+  if (!ActiveStage(s, st)) return DK_OK;
+  const int nmb = s->M.nmb;
+  if (s->P.coupling == DK_PC2) {
+    if (st->stage == 1 && s->P.back_reaction) {
+      const bool damp = s->P.pc2_damping;
+      const double dt = st->dt;
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+      for (int m=0; m<nmb; ++m)
+        for (int k=s->ks; k<=s->ke; ++k)
+          for (int j=s->js; j<=s->je; ++j)
+            for (int i=s->is; i<=s->ie; ++i) {
+              if (damp) {
+                const double chi = dt*s->qdep[s->Idx(m,5,4,k,j,i)]/
+                                   s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+                if (chi > 1.0) {
+                  const double sc = 1.0/chi;
+                  for (int c=1; c<=3; ++c) s->qdep[s->Idx(m,5,c,k,j,i)] *= sc;
+                }
+              }
+              for (int c=0; c<3; ++c)
+                s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)] += s->qdep[s->Idx(m,5,1+c,k,j,i)];
+            }
+    }
+    return DK_OK;
+  }
+  const bool br = s->P.back_reaction;
+  const double a_dt = DragStep(s, st);
+  const bool coupled = br && s->P.solver != DK_LOCAL;
+  // the local guess (dust_pm.cpp:426-436; dust_solver.cpp:643-649: the same arithmetic)
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i) {
+          const double qd0 = br ? s->qdep[s->Idx(m,5,0,k,j,i)] : 0.0;
+          const double denom = 1.0/(s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)] + qd0);
+          for (int c=0; c<3; ++c) {
+            const double qd = br ? s->qdep[s->Idx(m,5,1+c,k,j,i)] : 0.0;
+            s->ustar[s->Idx(m,3,c,k,j,i)] = (s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)] + qd)*denom;
+          }
+        }
+  if (!coupled) return DK_OK;
+  const double sh = ImplicitShearShift(s, st);
+  if (s->P.solver == DK_APPLYA) {
+    ApplyCoupledOperatorImpl(s, s->ustar, s->ap, a_dt, sh, false, st->time);
+  } else {
+    const int nsweeps = (s->P.solver == DK_DC2) ? 2 : 1;
+    for (int sw=0; sw<nsweeps; ++sw)
+      ApplyCoupledOperatorImpl(s, s->ustar, s->ap, a_dt, sh, true, st->time);
+  }
+  return DK_OK;
+}
+
+// src/dust/dust_tasks.cpp:265-312
+int or_fill_ustar(or_sim *s, const dk_stage *st) {
+  if (!ActiveStage(s, st) || s->P.coupling != DK_IMEX) return DK_OK;
+  FillVelocityGhosts(s, s->ustar, st->time);
+  return DK_OK;
+}
+
+// src/dust/dust_pm.cpp:450-568
+int or_gather_kick_pmbr(or_sim *s, const dk_stage *st) {
+  if (!ActiveStage(s, st) || s->P.coupling == DK_PC2) return DK_OK;
+  const bool br = s->P.back_reaction;
+  if (br) Zero(s->dmom);
+  const double a_dt = DragStep(s, st);
+  const double sh = ImplicitShearShift(s, st);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t p=0; p<s->np; ++p) {
+    const int m = s->I(DK_PGID,p);
+    Stencil w;
+    MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p) + sh*s->R(DK_IPX,p), s->R(DK_IPZ,p), w);
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    for (int c=0; c<=2; ++c)
+      for (int b=0; b<3; ++b) {
+        const double wcb = w.wz[c]*w.wy[b];
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+          ux += ww*s->ustar[s->Idx(m,3,0,kk,jj,ii)];
+          uy += ww*s->ustar[s->Idx(m,3,1,kk,jj,ii)];
+          uz += ww*s->ustar[s->Idx(m,3,2,kk,jj,ii)];
+        }
+      }
+    const double cj = a_dt/(s->R(DK_IPTS,p) + a_dt);
+    const double dvx = cj*(ux - s->R(DK_IPVX,p));
+    const double dvy = cj*(uy - s->R(DK_IPVY,p));
+    const double dvz = cj*(uz - s->R(DK_IPVZ,p));
+    s->R(DK_IPVX,p) += dvx;
+    s->R(DK_IPVY,p) += dvy;
+    s->R(DK_IPVZ,p) += dvz;
+    s->R(DK_IPRX,p) = dvx/a_dt;
+    s->R(DK_IPRY,p) = dvy/a_dt;
+    s->R(DK_IPRZ,p) = dvz/a_dt;
+    if (br) {
+      const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+      const double fac = -s->R(DK_IPM,p)/vol;
+      for (int c=0; c<=2; ++c)
+        for (int b=0; b<3; ++b) {
+          const double wcb = w.wz[c]*w.wy[b]*fac;
+          if (wcb == 0.0) continue;
+          for (int a=0; a<3; ++a) {
+            const double ww = wcb*w.wx[a];
+            const int kk = w.kp+c-1, jj = w.jp+b-1, ii = w.ip+a-1;
+            DepositAdd(s, &s->dmom[s->Idx(m,3,0,kk,jj,ii)], ww*dvx);
+            DepositAdd(s, &s->dmom[s->Idx(m,3,1,kk,jj,ii)], ww*dvy);
+            DepositAdd(s, &s->dmom[s->Idx(m,3,2,kk,jj,ii)], ww*dvz);
+          }
+        }
+    }
+  }
+  return DK_OK;
+}
+
+// src/dust/dust_tasks.cpp:314-328
+int or_fold_pmbr(or_sim *s, const dk_stage *st) {
+  const bool use = (s->P.coupling == DK_IMEX) || st->stage == 2;
+  if (!ActiveStage(s, st) || !s->P.back_reaction || !use) return DK_OK;
+  Fold(s, s->dmom, 3, st->time);
+  return DK_OK;
+}
+
+// src/dust/dust_pm.cpp:576-711
+int or_apply_pmbr(or_sim *s, const dk_stage *st) {
+  if (!ActiveStage(s, st) || !s->P.back_reaction) return DK_OK;
+  const int nmb = s->M.nmb;
+  if (s->P.coupling == DK_PC2) {
+    if (st->stage != 2) return DK_OK;
+    const double wpred = st->gam1[st->nexp_stages-1];
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+    for (int m=0; m<nmb; ++m)
+      for (int k=s->ks; k<=s->ke; ++k)
+        for (int j=s->js; j<=s->je; ++j)
+          for (int i=s->is; i<=s->ie; ++i)
+            for (int c=0; c<3; ++c)
+              s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)] +=
+                  s->dmom[s->Idx(m,3,c,k,j,i)] - wpred*s->qdep[s->Idx(m,5,1+c,k,j,i)];
+    return DK_OK;
+  }
+  const double inv_adt = 1.0/(st->a_impl*st->dt);
+  if (s->P.consistent_gas) {
+    const double a_dt = DragStep(s, st);
+    Zero(s->resid);
+    // dust_imex_point_center_mass (dust_pm.cpp:660-667); the centre uses the unshifted y
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int64_t p=0; p<s->np; ++p) {
+      if (s->R(DK_IPM,p) <= 0.0) continue;
+      const int m = s->I(DK_PGID,p);
+      Stencil w;
+      MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p), w);
+      const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+      DepositAdd(s, &s->resid[s->Idx(m,4,3,w.kp,w.jp,w.ip)],
+                 s->R(DK_IPM,p)*(a_dt/(s->R(DK_IPTS,p) + a_dt))/vol);
+    }
+    // dust_imex_point_consistent_gas (dust_pm.cpp:670-685)
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+    for (int m=0; m<nmb; ++m)
+      for (int k=s->ks; k<=s->ke; ++k)
+        for (int j=s->js; j<=s->je; ++j)
+          for (int i=s->is; i<=s->ie; ++i) {
+            const double rho = s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+            const double qt = s->resid[s->Idx(m,4,3,k,j,i)];
+            const double q = s->qdep[s->Idx(m,5,0,k,j,i)];
+            for (int c=0; c<3; ++c) {
+              double dgas = s->dmom[s->Idx(m,3,c,k,j,i)];
+              if (qt > 0.0) {
+                const double g = rho*s->ustar[s->Idx(m,3,c,k,j,i)] -
+                                 s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)];
+                const double r = (s->dmom[s->Idx(m,3,c,k,j,i)] - g)/(rho + q);
+                dgas = g + rho*r;
+                s->resid[s->Idx(m,4,c,k,j,i)] = r*q/qt;
+              }
+              s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)] += dgas;
+              s->dmom[s->Idx(m,3,c,k,j,i)] = dgas*inv_adt;
+            }
+          }
+    // dust_imex_point_consistent_dust (dust_pm.cpp:686-696)
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int64_t p=0; p<s->np; ++p) {
+      if (s->R(DK_IPM,p) <= 0.0) continue;
+      const int m = s->I(DK_PGID,p);
+      Stencil w;
+      MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p), w);
+      const double cj = a_dt/(s->R(DK_IPTS,p) + a_dt);
+      for (int c=0; c<3; ++c) {
+        const double dv = cj*s->resid[s->Idx(m,4,c,w.kp,w.jp,w.ip)];
+        s->R(DK_IPVX+2*c,p) += dv;
+        s->R(DK_IPRX+c,p) += dv/a_dt;
+      }
+    }
+    return DK_OK;
+  }
+  // dust_pmbr (dust_pm.cpp:700-708)
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i)
+          for (int c=0; c<3; ++c) {
+            s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)] += s->dmom[s->Idx(m,3,c,k,j,i)];
+            s->dmom[s->Idx(m,3,c,k,j,i)] *= inv_adt;
+          }
+  return DK_OK;
+}
+
+// src/dust/dust_pm.cpp:721-742
+int or_add_drag_history_gas(or_sim *s, const dk_stage *st) {
+  if (s->P.coupling != DK_IMEX) return DK_OK;
+  if (st->stage != 2 || !s->P.back_reaction) return DK_OK;
+  const double atw_dt = st->a_twid22*st->dt;
+  const int nmb = s->M.nmb;
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i)
+          for (int c=0; c<3; ++c)
+            s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)] += atw_dt*s->dmom[s->Idx(m,3,c,k,j,i)];
+  return DK_OK;
+}
+
+// Hydro::RKUpdate with zero fluxes (src/hydro/hydro_update.cpp:23-83) and HydroSrcTerms
+// (src/hydro/hydro_tasks.cpp:268-297: ConstantAccel srcterms.cpp:128-145, ShearingBoxCC::
+// SourceTermsCC shearing_box_srcterms.cpp:26-60), primitives from u0 at the stage start
+int or_gas_hook(or_sim *s, const dk_stage *st, const dk_gas_src *src) {
+  const int sidx = st->stage - 1;
+  const double gam0 = st->gam0[sidx], gam1 = st->gam1[sidx];
+  const double bdt = st->beta[sidx]*st->dt;
+  if (bdt == 0.0 && gam0 == 1.0 && gam1 == 0.0) return DK_OK;   // identity stage
+  const int nmb = s->M.nmb;
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i) {
+          double w[4];
+          const double den = s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+          w[0] = den;
+          for (int c=1; c<4; ++c) w[c] = s->u0[s->Idx(m,NG_VARS,c,k,j,i)]/den;
+          for (int v=0; v<NG_VARS; ++v) {
+            double &u = s->u0[s->Idx(m,NG_VARS,v,k,j,i)];
+            u = gam0*u + gam1*s->u1[s->Idx(m,NG_VARS,v,k,j,i)];
+          }
+          if (src == nullptr) continue;
+          if (src->const_accel != 0.0) {
+            s->u0[s->Idx(m,NG_VARS,src->const_accel_dir,k,j,i)] += bdt*src->const_accel*w[0];
+          }
+          if (src->shearing_box) {
+            const double coef1 = 2.0*bdt*src->omega0;
+            const double coef2 = (2.0-src->qshear)*bdt*src->omega0;
+            const double coef3 = bdt*SQR(src->omega0);
+            const double mom1 = w[0]*w[1], mom2 = w[0]*w[2];
+            s->u0[s->Idx(m,NG_VARS,DK_IM1,k,j,i)] += coef1*mom2;
+            s->u0[s->Idx(m,NG_VARS,DK_IM2,k,j,i)] -= coef2*mom1;
+            if (src->stratified) {
+              const dk_block &b = s->B[m];
+              const double xx = (static_cast<double>(k - s->ks) + 0.5)/s->nx3;
+              const double x3v = (xx*b.x3max - xx*b.x3min) - (0.5*b.x3max - 0.5*b.x3min) +
+                                 (0.5*b.x3min + 0.5*b.x3max);
+              s->u0[s->Idx(m,NG_VARS,DK_IM3,k,j,i)] -= coef3*w[0]*x3v;
+            }
+          }
+        }
+  return DK_OK;
+}
+
+// Hydro::SendU/RecvU/SendU_Shr/RecvU_Shr + ApplyPhysicalBCs stand-in on u0
+int or_fill_gas_ghosts(or_sim *s, const dk_stage *st) {
+  CopyGhosts(s, s->u0, NG_VARS);
+  if (s->M.bc_x1 == DK_X1_SHEAR) ShearGhostsX1(s, s->u0, NG_VARS, st->time + st->dt);
+  X3BCs(s, s->u0, NG_VARS, false);
+  return DK_OK;
+}
+
+// The stage in the order of the combined task graph (src/dust/dust_tasks.cpp:58-109)
+int or_stage_run(or_sim *s, const dk_stage *st, const dk_gas_src *src) {
+  int e;
+  if ((e = or_first_two_imp_rk(s, st))) return e;
+  if ((e = or_gas_hook(s, st, src))) return e;
+  if ((e = or_add_drag_history_gas(s, st))) return e;
+  if ((e = or_explicit_push(s, st))) return e;
+  if ((e = or_update_particle_gids(s, st))) return e;
+  if ((e = or_deposit_drag(s, st))) return e;
+  if ((e = or_fold_deposit_qp(s, st))) return e;
+  if ((e = or_gas_implicit_solve(s, st))) return e;
+  if ((e = or_fill_ustar(s, st))) return e;
+  if ((e = or_gather_kick_pmbr(s, st))) return e;
+  if ((e = or_fold_pmbr(s, st))) return e;
+  if ((e = or_apply_pmbr(s, st))) return e;
+  // the hydro tail: u0 ghosts for the next stage (pc2 gathers read them)
+  if (ActiveStage(s, st)) {
+    if ((e = or_fill_gas_ghosts(s, st))) return e;
+  }
+  return DK_OK;
+}
+
+// src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell of the position)
+int or_sort_particles(or_sim *s) {
+  const int64_t n = s->np;
+  std::vector<int64_t> key(n), ord(n);
+  for (int64_t p=0; p<n; ++p) {
+    const int m = s->I(DK_PGID,p);
+    Stencil w;
+    MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p), w);
+    const int64_t cell = ((int64_t)(w.kp - s->ks)*s->nx2 + (w.jp - s->js))*s->nx1 + (w.ip - s->is);
+    key[p] = (int64_t)m*s->nx1*s->nx2*s->nx3 + cell;
+    ord[p] = p;
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {return key[a] < key[b];});
+  std::vector<double> nr(s->pr.size());
+  std::vector<int32_t> ni(s->pi.size());
+  for (int c=0; c<DK_NRDATA; ++c)
+    for (int64_t q=0; q<n; ++q) nr[(int64_t)c*s->cap + q] = s->pr[(int64_t)c*s->cap + ord[q]];
+  for (int c=0; c<DK_NIDATA; ++c)
+    for (int64_t q=0; q<n; ++q) ni[(int64_t)c*s->cap + q] = s->pi[(int64_t)c*s->cap + ord[q]];
+  s->pr.swap(nr);
+  s->pi.swap(ni);
+  return DK_OK;
+}
+
+// src/dust/dust_newdt.cpp:108-149 (no shearing-box correction of |v_y|)
+int or_new_time_step(or_sim *s, double dt_cfl, double *dt_out) {
+  double dtmin = std::numeric_limits<double>::max();
+  for (int64_t p=0; p<s->np; ++p) {
+    const int m = s->I(DK_PGID,p);
+    if (s->R(DK_IPTS,p) <= 0.0) return DK_ESTOP;
+    const double vx = std::fabs(s->R(DK_IPVX,p)), vy = std::fabs(s->R(DK_IPVY,p));
+    const double vz = std::fabs(s->R(DK_IPVZ,p));
+    if (vx > 0.0) dtmin = std::fmin(dtmin, s->dx1[m]/vx);
+    if (vy > 0.0) dtmin = std::fmin(dtmin, s->dx2[m]/vy);
+    if (vz > 0.0) dtmin = std::fmin(dtmin, s->dx3[m]/vz);
+  }
+  if (dt_out) *dt_out = dt_cfl*dtmin;
+  return DK_OK;
+}
+
+int or_apply_operator(or_sim *s, double a_dt, double sh, const double *x, double *y) {
+  const int64_t nc = 3*s->ncb*s->M.nmb;
+  std::vector<double> f(x, x + nc), r(nc, 0.0);
+  CopyGhosts(s, f, 3);
+  PointOperatorParticles(s, f, r, a_dt, sh);
+  Fold(s, r, 3, 0.0);
+  for (int m=0; m<s->M.nmb; ++m)
+    for (int d=0; d<3; ++d)
+      for (int k=s->ks; k<=s->ke; ++k)
+        for (int j=s->js; j<=s->je; ++j)
+          for (int i=s->is; i<=s->ie; ++i) {
+            const int64_t id = s->Idx(m,3,d,k,j,i);
+            y[id] = r[id] + s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)]*f[id];
+          }
+  return DK_OK;
+}
+
+int or_diag_sums(or_sim *s, double out[9]) {
+  for (int c=0; c<9; ++c) out[c] = 0.0;
+  for (int64_t p=0; p<s->np; ++p) {
+    const double m = s->R(DK_IPM,p);
+    out[0] += m*s->R(DK_IPVX,p);
+    out[1] += m*s->R(DK_IPVY,p);
+    out[2] += m*s->R(DK_IPVZ,p);
+    out[6] += m;
+    out[8] += m*(std::fabs(s->R(DK_IPVX,p)) + std::fabs(s->R(DK_IPVY,p)) +
+                 std::fabs(s->R(DK_IPVZ,p)));
+  }
+  for (int m=0; m<s->M.nmb; ++m) {
+    const double vol = s->dx1[m]*s->dx2[m]*s->dx3[m];
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i) {
+          out[7] += vol*s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+          for (int c=0; c<3; ++c) {
+            const double mom = s->u0[s->Idx(m,NG_VARS,DK_IM1+c,k,j,i)];
+            out[3+c] += vol*mom;
+            out[8] += vol*std::fabs(mom);
+          }
+        }
+  }
+  return DK_OK;
+}
+
+// src/driver/driver.cpp:195-203,278-300 (gamma passed as the literal 1.707106781186547)
+void or_imex2p_coef(double time, double dt, dk_stage *st) {
+  const double gamma = 1.707106781186547;
+  dk_stage c;
+  std::memset(&c, 0, sizeof(c));
+  c.nexp_stages = 3; c.time = time; c.dt = dt;
+  c.gam0[0] = 1.0; c.gam1[0] = 0.0; c.beta[0] = gamma;
+  c.gam0[1] = (2.0*gamma-1.0)/(2.0*gamma*gamma);
+  c.gam1[1] = (1.0-(2.0*gamma-1.0)/(2.0*gamma*gamma));
+  c.beta[1] = 1.0/(2.0*gamma);
+  c.gam0[2] = 1.0; c.gam1[2] = 0.0; c.beta[2] = 0.0;
+  c.a_twid22 = (1.0-2.0*gamma*gamma)/2.0/gamma;
+  c.a_impl = gamma;
+  for (int k=0; k<3; ++k) {st[k] = c; st[k].stage = k + 1;}
+}
+
+// src/driver/driver.cpp:106-118
+void or_rk2_coef(double time, double dt, dk_stage *st) {
+  dk_stage c;
+  std::memset(&c, 0, sizeof(c));
+  c.nexp_stages = 2; c.time = time; c.dt = dt;
+  c.gam0[0] = 0.0; c.gam1[0] = 1.0; c.beta[0] = 1.0;
+  c.gam0[1] = 0.5; c.gam1[1] = 0.5; c.beta[1] = 0.5;
+  c.a_impl = 0.0;
+  for (int k=0; k<2; ++k) {st[k] = c; st[k].stage = k + 1;}
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+"""Python handle on the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the package smoke check and by the CPU
+baseline of bench.py, always as the checker, never as the measured or shipped path."""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from athenak_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.or_create.restype = P
+        L.or_create.argtypes = [C.POINTER(abi.dk_mesh), C.POINTER(abi.dk_block),
+                                C.POINTER(abi.dk_physics), C.c_int64]
+        L.or_destroy.argtypes = [P]
+        L.or_set_threads.argtypes = [P, C.c_int]
+        for n in ("or_rdata", "or_u0", "or_u1"):
+            getattr(L, n).restype = C.POINTER(C.c_double)
+            getattr(L, n).argtypes = [P]
+        L.or_idata.restype = C.POINTER(C.c_int32)
+        L.or_idata.argtypes = [P]
+        for n in ("or_capacity", "or_npart", "or_cells_per_block"):
+            getattr(L, n).restype = C.c_int64
+            getattr(L, n).argtypes = [P]
+        L.or_set_npart.argtypes = [P, C.c_int64]
+        L.or_field.restype = C.POINTER(C.c_double)
+        L.or_field.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+        L.or_setup.argtypes = [P, C.POINTER(abi.dk_setup)]
+        st = C.POINTER(abi.dk_stage)
+        for n in ("or_first_two_imp_rk", "or_explicit_push", "or_update_particle_gids",
+                  "or_deposit_drag", "or_fold_deposit_qp", "or_gas_implicit_solve",
+                  "or_fill_ustar", "or_gather_kick_pmbr", "or_fold_pmbr", "or_apply_pmbr",
+                  "or_add_drag_history_gas", "or_fill_gas_ghosts"):
+            getattr(L, n).argtypes = [P, st]
+        L.or_gas_hook.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
+        L.or_stage_run.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
+        L.or_sort_particles.argtypes = [P]
+        L.or_new_time_step.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
+        L.or_apply_operator.argtypes = [P, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
+        L.or_diag_sums.argtypes = [P, C.POINTER(C.c_double)]
+        L.or_last_error.argtypes = [P, C.POINTER(C.c_int64)]
+        _LIB = L
+    return _LIB
+
+
+class Oracle:
+    """One oracle simulation: mesh + particles + gas, stepped with the reference's stage
+    sequence."""
+
+    def __init__(self, mesh, blocks, phys, capacity, threads=1):
+        L = lib()
+        self.mesh, self.blocks, self.phys = mesh, blocks, phys
+        self.h = L.or_create(C.byref(mesh), blocks, C.byref(phys), int(capacity))
+        if not self.h:
+            raise ValueError("or_create rejected the mesh")
+        L.or_set_threads(self.h, int(threads))
+        self.cap = L.or_capacity(self.h)
+        self.ncb = L.or_cells_per_block(self.h)
+        self.time = 0.0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def set_threads(self, n):
+        lib().or_set_threads(self.h, int(n))
+
+    # ---- views (numpy arrays aliasing oracle storage) ----
+    @property
+    def npart(self):
+        return lib().or_npart(self.h)
+
+    def rdata(self):
+        p = lib().or_rdata(self.h)
+        return np.ctypeslib.as_array(p, shape=(abi.DK_NRDATA, self.cap))
+
+    def idata(self):
+        p = lib().or_idata(self.h)
+        return np.ctypeslib.as_array(p, shape=(abi.DK_NIDATA, self.cap))
+
+    def _field(self, ptr, nvar):
+        m = self.mesh
+        shape = (m.nmb, nvar, m.nx3 + 2 * m.ng, m.nx2 + 2 * m.ng, m.nx1 + 2 * m.ng)
+        return np.ctypeslib.as_array(ptr, shape=shape)
+
+    def u0(self):
+        return self._field(lib().or_u0(self.h), abi.DK_NGAS)
+
+    def u1(self):
+        return self._field(lib().or_u1(self.h), abi.DK_NGAS)
+
+    def field(self, which):
+        nv = C.c_int(0)
+        p = lib().or_field(self.h, int(which), C.byref(nv))
+        return self._field(p, nv.value)
+
+    # ---- set-up and stepping ----
+    def setup(self, su):
+        e = lib().or_setup(self.h, C.byref(su))
+        if e:
+            raise RuntimeError("or_setup: %s" % abi.ERRORS.get(e, e))
+
+    def check(self, e, what):
+        if e:
+            pp = C.c_int64(-1)
+            code = lib().or_last_error(self.h, C.byref(pp))
+            raise RuntimeError("%s: %s (particle %d)" % (what, abi.ERRORS.get(e, e), pp.value))
+
+    def task(self, name, st, *extra):
+        e = getattr(lib(), "or_" + name)(self.h, C.byref(st), *extra)
+        self.check(e, name)
+
+    def stage(self, st, src=None):
+        e = lib().or_stage_run(self.h, C.byref(st), C.byref(src) if src is not None else None)
+        self.check(e, "stage %d" % st.stage)
+
+    def step(self, dt, src=None):
+        stages = (abi.imex2p_stages(self.time, dt) if self.phys.coupling == abi.DK_IMEX
+                  else abi.rk2_stages(self.time, dt))
+        for st in stages:
+            self.stage(st, src)
+        self.time += dt
+
+    def sort(self):
+        lib().or_sort_particles(self.h)
+
+    def diag(self):
+        out = (C.c_double * 9)()
+        lib().or_diag_sums(self.h, out)
+        return np.array(out[:])
+
+    def apply_operator(self, a_dt, x, sh=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        lib().or_apply_operator(self.h, a_dt, sh, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                y.ctypes.data_as(C.POINTER(C.c_double)))
+        return y
