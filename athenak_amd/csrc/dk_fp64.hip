@@ -1,0 +1,1392 @@
+// dk_fp64.hip -- context management and the plain FP64 form of the drag coupling.
+//
+// One kernel per reference kernel, thread per particle or per cell, deposits with hardware
+// FP64 atomics, operating in place on the caller's particle SoA and gas arrays.  This is the
+// baseline form the FP32 cell-sorted form is measured against.  Each kernel cites the
+// reference lambda it computes; the arithmetic is written in the reference's operation order.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+#include "dk_internal.hpp"
+
+using dk::Geo;
+using dk::DevBlock;
+using dk::Idx;
+using dk::Dir;
+using dk::kBlock;
+
+namespace dk {
+
+int Check(hipError_t e) { return (e == hipSuccess) ? DK_OK : DK_EHIP; }
+
+long long NCells(const dk_ctx *c) { return c->g.ncb*c->g.nmb; }
+
+namespace {
+
+inline int Grid(long long n, int cap = 65536) {
+  long long b = (n + kBlock - 1)/kBlock;
+  if (b < 1) b = 1;
+  return static_cast<int>(std::min<long long>(b, cap));
+}
+
+__global__ void k_fill(double *a, long long n, double v) {
+  for (long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x*blockDim.x)
+    a[t] = v;
+}
+
+}  // namespace
+
+int Zero(dk_ctx *c, double *a, long long n) {
+  // a fill kernel rather than hipMemsetAsync, so that the sequence can be graph-captured
+  const int t = ProfBegin(c, "fill");
+  k_fill<<<Grid(n), kBlock, 0, c->stream>>>(a, n, 0.0);
+  ProfEnd(c, t);
+  return Check(hipGetLastError());
+}
+
+}  // namespace dk
+
+namespace {
+
+#define GRID_STRIDE(t, n) \
+  for (long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x; t < (n); \
+       t += (long long)gridDim.x*blockDim.x)
+
+inline int Grid(long long n) {
+  long long b = (n + kBlock - 1)/kBlock;
+  if (b < 1) b = 1;
+  return static_cast<int>(std::min<long long>(b, 65536));
+}
+
+struct Stencil {
+  int ip, jp, kp;
+  double wx[3], wy[3], wz[3];
+};
+
+__device__ inline void MakeStencil(const Geo &g, const DevBlock &b, double x, double y,
+                                   double z, Stencil &w) {
+  dk::PMWeights(x, b.x1min, b.x1max, g.nx1, g.is, g.deposit, w.ip, w.wx);
+  dk::PMWeights(y, b.x2min, b.x2max, g.nx2, g.js, g.deposit, w.jp, w.wy);
+  dk::PMWeights(z, b.x3min, b.x3max, g.nx3, g.ks, g.deposit, w.kp, w.wz);
+}
+
+__device__ inline bool HaloOK(const Geo &g, const Stencil &w) {   // dust_pm.cpp:241-242
+  return !(w.ip < (g.is-1) || w.ip > (g.is+g.nx1) || w.jp < (g.js-1) || w.jp > (g.js+g.nx2) ||
+           w.kp < (g.ks-1) || w.kp > (g.ks+g.nx3));
+}
+
+// a stencil footprint must lie inside the padded block (guards every gather/deposit)
+__device__ inline bool InPad(const Geo &g, const Stencil &w) {
+  return w.ip >= 1 && w.ip <= g.n1 - 2 && w.jp >= 1 && w.jp <= g.n2 - 2 && w.kp >= 1 &&
+         w.kp <= g.n3 - 2;
+}
+
+__device__ inline void AtomicAdd(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+#define PR(c, p) r[(long long)(c)*cap + (p)]
+#define PI(c, p) iv[(long long)(c)*cap + (p)]
+
+//----------------------------------------------------------------------------------------
+// dust_push.cpp:69-77 (dust_reg_copy)
+__global__ void k_reg_copy(double *r, long long cap, long long np) {
+  GRID_STRIDE(p, np) {
+    PR(DK_IPX1,p) = PR(DK_IPX,p);
+    PR(DK_IPVX1,p) = PR(DK_IPVX,p);
+    PR(DK_IPY1,p) = PR(DK_IPY,p);
+    PR(DK_IPVY1,p) = PR(DK_IPVY,p);
+    PR(DK_IPZ1,p) = PR(DK_IPZ,p);
+    PR(DK_IPVZ1,p) = PR(DK_IPVZ,p);
+  }
+}
+
+// dust_push.cpp:53-59 (dust_gas_midpoint), ghosts included
+__global__ void k_gas_midpoint(Geo g, const double *u0, const double *u1, double *us) {
+  const long long n = g.ncb*g.nmb;
+  GRID_STRIDE(c, n) {
+    const long long m = c/g.ncb, rr = c - m*g.ncb;
+    const double inv_rho_sum = 1.0/(u1[(m*DK_NGAS + DK_IDN)*g.ncb + rr] +
+                                    u0[(m*DK_NGAS + DK_IDN)*g.ncb + rr]);
+    for (int v=0; v<3; ++v)
+      us[(m*3 + v)*g.ncb + rr] = (u1[(m*DK_NGAS + DK_IM1 + v)*g.ncb + rr] +
+                                  u0[(m*DK_NGAS + DK_IM1 + v)*g.ncb + rr])*inv_rho_sum;
+  }
+}
+
+// dust_push.cpp:262-297 (dust_push)
+__global__ void k_push_imex(double *r, long long cap, long long np, double gam0, double gam1,
+                            double beta_dt, double atw_dt, int sbox, int strat, double qs,
+                            double om) {
+  GRID_STRIDE(p, np) {
+    const double x_old = PR(DK_IPX,p), y_old = PR(DK_IPY,p), z_old = PR(DK_IPZ,p);
+    const double vx_old = PR(DK_IPVX,p), vy_old = PR(DK_IPVY,p), vz_old = PR(DK_IPVZ,p);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (sbox) {
+      fx = 2.0*om*vy_old;
+      fy = -(2.0-qs)*om*vx_old;
+      if (strat) fz = -(om*om)*z_old;
+    }
+    PR(DK_IPVX,p) = gam0*vx_old + gam1*PR(DK_IPVX1,p) + beta_dt*fx + atw_dt*PR(DK_IPRX,p);
+    PR(DK_IPVY,p) = gam0*vy_old + gam1*PR(DK_IPVY1,p) + beta_dt*fy + atw_dt*PR(DK_IPRY,p);
+    PR(DK_IPVZ,p) = gam0*vz_old + gam1*PR(DK_IPVZ1,p) + beta_dt*fz + atw_dt*PR(DK_IPRZ,p);
+    double ydot = vy_old;
+    if (sbox) ydot -= qs*om*x_old;
+    PR(DK_IPX,p) = gam0*x_old + gam1*PR(DK_IPX1,p) + beta_dt*vx_old;
+    PR(DK_IPY,p) = gam0*y_old + gam1*PR(DK_IPY1,p) + beta_dt*ydot;
+    PR(DK_IPZ,p) = gam0*z_old + gam1*PR(DK_IPZ1,p) + beta_dt*vz_old;
+  }
+}
+
+// dust_push.cpp:132-245 (dust_pc2_advance)
+__global__ void k_pc2_advance(Geo g, const DevBlock *B, double *r, const int32_t *iv,
+                              long long cap, long long np, const double *us, double *dmom,
+                              double dt, int br, int sbox, int strat, double qs, double om,
+                              dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    const double x0 = PR(DK_IPX1,p), y0 = PR(DK_IPY1,p), z0 = PR(DK_IPZ1,p);
+    const double vx0 = PR(DK_IPVX1,p), vy0 = PR(DK_IPVY1,p), vz0 = PR(DK_IPVZ1,p);
+    const double xh = x0 + 0.5*dt*vx0;
+    const double yh = y0 + 0.5*dt*(vy0 - (sbox ? qs*om*x0 : 0.0));
+    const double zh = z0 + 0.5*dt*vz0;
+    const double yg = yh + (sbox ? qs*om*xh*0.5*dt : 0.0);
+    Stencil w;
+    MakeStencil(g, b, xh, yg, zh, w);
+    if (!InPad(g, w)) {dk::RecordError(err, DK_EHALO, p); continue;}
+    double ugx = 0.0, ugy = 0.0, ugz = 0.0;
+    for (int c=0; c<=2; ++c)
+      for (int bb=0; bb<3; ++bb) {
+        const double wcb = w.wz[c]*w.wy[bb];
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+          ugx += ww*us[Idx(g,m,3,0,kk,jj,ii)];
+          ugy += ww*us[Idx(g,m,3,1,kk,jj,ii)];
+          ugz += ww*us[Idx(g,m,3,2,kk,jj,ii)];
+        }
+      }
+    const double h = 0.5*dt;
+    const double lambda = h/PR(DK_IPTS,p);
+    const double aa = 1.0 + lambda;
+    const double rx = vx0 + lambda*ugx;
+    const double ry = vy0 + lambda*ugy;
+    double rz = vz0 + lambda*ugz;
+    double vxh, vyh, vzh;
+    if (sbox) {
+      const double cxy = 2.0*h*om;
+      const double cyx = h*(2.0-qs)*om;
+      const double det = aa*aa + cxy*cyx;
+      vxh = (aa*rx + cxy*ry)/det;
+      vyh = (aa*ry - cyx*rx)/det;
+      if (strat) rz -= h*(om*om)*zh;
+      vzh = rz/aa;
+    } else {
+      vxh = rx/aa; vyh = ry/aa; vzh = rz/aa;
+    }
+    PR(DK_IPVX,p) = 2.0*vxh - vx0;
+    PR(DK_IPVY,p) = 2.0*vyh - vy0;
+    PR(DK_IPVZ,p) = 2.0*vzh - vz0;
+    PR(DK_IPX,p) = x0 + dt*vxh;
+    PR(DK_IPY,p) = y0 + dt*(vyh - (sbox ? qs*om*xh : 0.0));
+    PR(DK_IPZ,p) = z0 + dt*vzh;
+    PR(DK_IPX1,p) = xh; PR(DK_IPY1,p) = yh; PR(DK_IPZ1,p) = zh;
+    const double ts = PR(DK_IPTS,p);
+    const double dvx = dt*(ugx-vxh)/ts, dvy = dt*(ugy-vyh)/ts, dvz = dt*(ugz-vzh)/ts;
+    PR(DK_IPRX,p) = dvx; PR(DK_IPRY,p) = dvy; PR(DK_IPRZ,p) = dvz;
+    if (br) {
+      const double vol = b.dx1*b.dx2*b.dx3;
+      const double fac = -PR(DK_IPM,p)/vol;
+      for (int c=0; c<=2; ++c)
+        for (int bb=0; bb<3; ++bb) {
+          const double wcb = w.wz[c]*w.wy[bb]*fac;
+          if (wcb == 0.0) continue;
+          for (int a=0; a<3; ++a) {
+            const double ww = wcb*w.wx[a];
+            const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+            AtomicAdd(&dmom[Idx(g,m,3,0,kk,jj,ii)], ww*dvx);
+            AtomicAdd(&dmom[Idx(g,m,3,1,kk,jj,ii)], ww*dvy);
+            AtomicAdd(&dmom[Idx(g,m,3,2,kk,jj,ii)], ww*dvz);
+          }
+        }
+    }
+  }
+}
+
+// dust.cpp:1678-1702 (dust_shear_route_time)
+__global__ void k_shift_shear_crossings(const DevBlock *B, double *r, const int32_t *iv,
+                                        long long cap, long long np, double dsh, double xa,
+                                        double xb) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    const double b0 = B[m].x1min, b1 = B[m].x1max, lx = b1 - b0;
+    const int ix = static_cast<int>((PR(DK_IPX,p) - b0 + lx)/lx) - 1;
+    const bool lo1 = (ix < 0) && (b0 < xa + 0.5*lx);
+    const bool hi1 = (ix > 0) && (b1 > xb - 0.5*lx);
+    if (lo1 || hi1) {
+      const double d = hi1 ? dsh : -dsh;
+      PR(DK_IPY,p) += d;
+      PR(DK_IPY1,p) += d;
+    }
+  }
+}
+
+// bvals_part.cpp:164-429 (part_update) on a uniform mesh of one rank
+__global__ void k_part_update(Geo g, const DevBlock *B, double *r, int32_t *iv, long long cap,
+                              long long np, int shear, double ysh, double *x3rem,
+                              dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    const double z3a = g.x3min, z3b = g.x3max;
+    if (g.bc_x3 != 0 && (PR(DK_IPZ,p) < z3a || PR(DK_IPZ,p) >= z3b)) {
+      const bool lo = (PR(DK_IPZ,p) < z3a);
+      if (g.bc_x3 == DK_X3_REFLECT) {
+        const double zf = lo ? z3a : z3b;
+        double z = 2.0*zf - PR(DK_IPZ,p);
+        if (z >= z3b) z = z3b - 1.0e-12*(z3b - z3a);
+        if (z < z3a) z = z3a;
+        PR(DK_IPZ,p) = z;
+        PR(DK_IPVZ,p) = -PR(DK_IPVZ,p);
+        PR(DK_IPZ1,p) = 2.0*zf - PR(DK_IPZ1,p);
+        PR(DK_IPVZ1,p) = -PR(DK_IPVZ1,p);
+      } else {
+        AtomicAdd(&x3rem[0], PR(DK_IPM,p));
+        AtomicAdd(&x3rem[1], 1.0);
+        PR(DK_IPM,p) = 0.0;
+        const double dz = b.dx3;
+        const double z = lo ? (b.x3min + 0.5*dz) : (b.x3max - 0.5*dz);
+        PR(DK_IPZ,p) = z;
+        PR(DK_IPVX,p) = 0.0; PR(DK_IPVY,p) = 0.0; PR(DK_IPVZ,p) = 0.0;
+        PR(DK_IPX1,p) = PR(DK_IPX,p); PR(DK_IPY1,p) = PR(DK_IPY,p); PR(DK_IPZ1,p) = z;
+        PR(DK_IPVX1,p) = 0.0; PR(DK_IPVY1,p) = 0.0; PR(DK_IPVZ1,p) = 0.0;
+      }
+    }
+    double x1 = PR(DK_IPX,p), x2 = PR(DK_IPY,p), x3 = PR(DK_IPZ,p);
+    const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
+    const int ix = static_cast<int>((x1 - b.x1min + lx)/lx) - 1;
+    const int iy = static_cast<int>((x2 - b.x2min + ly)/ly) - 1;
+    const int iz = static_cast<int>((x3 - b.x3min + lz)/lz) - 1;
+    // snap onto [xmin, xmax) of the block when the offset keeps the particle there
+    auto snap = [&](const int off, double &x, const double xa, const double xb, const int ip,
+                    const int ir) {
+      if (off != 0) return;
+      double xs = x;
+      if (x < xa) {
+        xs = xa;
+      } else if (x >= xb) {
+        xs = nextafter(xb, xa);
+      }
+      if (xs != x) {
+        PR(ir,p) += xs - x;
+        PR(ip,p) = xs;
+        x = xs;
+      }
+    };
+    snap(ix, x1, b.x1min, b.x1max, DK_IPX, DK_IPX1);
+    snap(iy, x2, b.x2min, b.x2max, DK_IPY, DK_IPY1);
+    snap(iz, x3, b.x3min, b.x3max, DK_IPZ, DK_IPZ1);
+    if (abs(ix) > 1 || abs(iy) > 1 || abs(iz) > 1) {
+      dk::RecordError(err, DK_EROUTE, p);
+      continue;
+    }
+    if ((abs(ix) + abs(iy) + abs(iz)) == 0) continue;
+    const bool lo1 = (ix < 0) && (b.x1min < g.x1min + 0.5*lx);
+    const bool hi1 = (ix > 0) && (b.x1max > g.x1max - 0.5*lx);
+    const bool lo2 = (iy < 0) && (b.x2min < g.x2min + 0.5*ly);
+    const bool hi2 = (iy > 0) && (b.x2max > g.x2max - 0.5*ly);
+    const bool lo3 = (iz < 0) && (b.x3min < g.x3min + 0.5*lz);
+    const bool hi3 = (iz > 0) && (b.x3max > g.x3max - 0.5*lz);
+    const bool cross_in = shear && lo1, cross_out = shear && hi1;
+    if (cross_in || cross_out) {
+      const double lxm = g.x1max - g.x1min;
+      PR(DK_IPX,p) += cross_in ? lxm : -lxm;
+      PR(DK_IPX1,p) += cross_in ? lxm : -lxm;
+      const double lym = g.x2max - g.x2min;
+      double ynew = PR(DK_IPY,p) + (cross_in ? -ysh : ysh);
+      if (ynew < g.x2min) ynew += lym;
+      if (ynew >= g.x2max) ynew -= lym;
+      const double dy = ynew - PR(DK_IPY,p);
+      PR(DK_IPY,p) = ynew;
+      PR(DK_IPY1,p) += dy;
+      const double lzm = g.x3max - g.x3min;
+      if (lo3) {
+        PR(DK_IPZ,p) += lzm; PR(DK_IPZ1,p) += lzm;
+      } else if (hi3) {
+        PR(DK_IPZ,p) -= lzm; PR(DK_IPZ1,p) -= lzm;
+      }
+      int b2 = static_cast<int>(((PR(DK_IPY,p) - g.x2min)/lym)*static_cast<double>(g.nmbx2));
+      b2 = (b2 < 0) ? 0 : ((b2 > g.nmbx2-1) ? (g.nmbx2-1) : b2);
+      int b3 = static_cast<int>(((PR(DK_IPZ,p) - g.x3min)/lzm)*static_cast<double>(g.nmbx3));
+      b3 = (b3 < 0) ? 0 : ((b3 > g.nmbx3-1) ? (g.nmbx3-1) : b3);
+      const int b1 = cross_in ? (g.nmbx1 - 1) : 0;
+      PI(DK_PGID,p) = b1 + g.nmbx1*(b2 + g.nmbx2*b3);
+      continue;
+    }
+    const int nb = b.nghbr[Dir(ix, iy, iz)];
+    if (nb < 0) {dk::RecordError(err, DK_EROUTE, p); continue;}
+    PI(DK_PGID,p) = nb;
+    const double lxm = g.x1max - g.x1min, lym = g.x2max - g.x2min, lzm = g.x3max - g.x3min;
+    if (lo1) {PR(DK_IPX,p) += lxm; PR(DK_IPX1,p) += lxm;}
+    else if (hi1) {PR(DK_IPX,p) -= lxm; PR(DK_IPX1,p) -= lxm;}
+    if (lo2) {PR(DK_IPY,p) += lym; PR(DK_IPY1,p) += lym;}
+    else if (hi2) {PR(DK_IPY,p) -= lym; PR(DK_IPY1,p) -= lym;}
+    if (lo3) {PR(DK_IPZ,p) += lzm; PR(DK_IPZ1,p) += lzm;}
+    else if (hi3) {PR(DK_IPZ,p) -= lzm; PR(DK_IPZ1,p) -= lzm;}
+  }
+}
+
+// dust_pm.cpp:225-274 (dust_scatter)
+__global__ void k_scatter_imex(Geo g, const DevBlock *B, const double *r, const int32_t *iv,
+                               long long cap, long long np, double *qdep, double a_dt,
+                               double sh, dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    Stencil w;
+    MakeStencil(g, b, PR(DK_IPX,p), PR(DK_IPY,p) + sh*PR(DK_IPX,p), PR(DK_IPZ,p), w);
+    if (!HaloOK(g, w)) {dk::RecordError(err, DK_EHALO, p); continue;}
+    const double vol = b.dx1*b.dx2*b.dx3;
+    const double cj = a_dt/(PR(DK_IPTS,p) + a_dt);
+    const double muc = PR(DK_IPM,p)*cj/vol;
+    const double vx = PR(DK_IPVX,p), vy = PR(DK_IPVY,p), vz = PR(DK_IPVZ,p);
+    for (int c=0; c<=2; ++c)
+      for (int bb=0; bb<3; ++bb) {
+        const double wcb = w.wz[c]*w.wy[bb]*muc;
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+          AtomicAdd(&qdep[Idx(g,m,5,0,kk,jj,ii)], ww);
+          AtomicAdd(&qdep[Idx(g,m,5,1,kk,jj,ii)], ww*vx);
+          AtomicAdd(&qdep[Idx(g,m,5,2,kk,jj,ii)], ww*vy);
+          AtomicAdd(&qdep[Idx(g,m,5,3,kk,jj,ii)], ww*vz);
+        }
+      }
+  }
+}
+
+// dust_pm.cpp:141-193 (dust_pc2_predictor)
+__global__ void k_pc2_predictor(Geo g, const DevBlock *B, const double *r, const int32_t *iv,
+                                long long cap, long long np, const double *u1, double *qdep,
+                                double dt, int rate_dep, dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    Stencil w;
+    MakeStencil(g, b, PR(DK_IPX,p), PR(DK_IPY,p), PR(DK_IPZ,p), w);
+    if (!InPad(g, w)) {dk::RecordError(err, DK_EHALO, p); continue;}
+    double ugx = 0.0, ugy = 0.0, ugz = 0.0;
+    for (int c=0; c<=2; ++c)
+      for (int bb=0; bb<3; ++bb) {
+        const double wcb = w.wz[c]*w.wy[bb];
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+          const double invrho = 1.0/u1[Idx(g,m,DK_NGAS,DK_IDN,kk,jj,ii)];
+          ugx += ww*u1[Idx(g,m,DK_NGAS,DK_IM1,kk,jj,ii)]*invrho;
+          ugy += ww*u1[Idx(g,m,DK_NGAS,DK_IM2,kk,jj,ii)]*invrho;
+          ugz += ww*u1[Idx(g,m,DK_NGAS,DK_IM3,kk,jj,ii)]*invrho;
+        }
+      }
+    const double vol = b.dx1*b.dx2*b.dx3;
+    const double rate = PR(DK_IPM,p)/(PR(DK_IPTS,p)*vol);
+    const double gx = -dt*rate*(ugx-PR(DK_IPVX,p));
+    const double gy = -dt*rate*(ugy-PR(DK_IPVY,p));
+    const double gz = -dt*rate*(ugz-PR(DK_IPVZ,p));
+    for (int c=0; c<=2; ++c)
+      for (int bb=0; bb<3; ++bb) {
+        const double wcb = w.wz[c]*w.wy[bb];
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+          AtomicAdd(&qdep[Idx(g,m,5,1,kk,jj,ii)], ww*gx);
+          AtomicAdd(&qdep[Idx(g,m,5,2,kk,jj,ii)], ww*gy);
+          AtomicAdd(&qdep[Idx(g,m,5,3,kk,jj,ii)], ww*gz);
+          if (rate_dep) AtomicAdd(&qdep[Idx(g,m,5,4,kk,jj,ii)], ww*rate);
+        }
+      }
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// dust_pm.cpp:426-436 / dust_solver.cpp:643-649 (dust_ustar, dust_solver_local_guess)
+__global__ void k_local_guess(Geo g, const double *u0, const double *qdep, double *us, int br) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double qd0 = br ? qdep[Idx(g,m,5,0,k,j,i)] : 0.0;
+    const double denom = 1.0/(u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)] + qd0);
+    for (int c=0; c<3; ++c) {
+      const double qd = br ? qdep[Idx(g,m,5,1+c,k,j,i)] : 0.0;
+      us[Idx(g,m,3,c,k,j,i)] = (u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)] + qd)*denom;
+    }
+  }
+}
+
+// dust_solver.cpp:151-201 (dust_applya_gtsg)
+__global__ void k_applya_gtsg(Geo g, const DevBlock *B, const double *r, const int32_t *iv,
+                              long long cap, long long np, const double *field, double *res,
+                              double a_dt, double sh, dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    Stencil w;
+    MakeStencil(g, b, PR(DK_IPX,p), PR(DK_IPY,p) + sh*PR(DK_IPX,p), PR(DK_IPZ,p), w);
+    if (!InPad(g, w)) {dk::RecordError(err, DK_EHALO, p); continue;}
+    double gth[3] = {0.0, 0.0, 0.0};
+    for (int c=0; c<=2; ++c)
+      for (int bb=0; bb<3; ++bb) {
+        const double wcb = w.wz[c]*w.wy[bb];
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+          gth[0] += ww*field[Idx(g,m,3,0,kk,jj,ii)];
+          gth[1] += ww*field[Idx(g,m,3,1,kk,jj,ii)];
+          gth[2] += ww*field[Idx(g,m,3,2,kk,jj,ii)];
+        }
+      }
+    const double vol = b.dx1*b.dx2*b.dx3;
+    const double cj = a_dt/(PR(DK_IPTS,p) + a_dt);
+    const double fac = PR(DK_IPM,p)*cj/vol;
+    for (int c=0; c<=2; ++c)
+      for (int bb=0; bb<3; ++bb) {
+        const double wcb = w.wz[c]*w.wy[bb]*fac;
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+          AtomicAdd(&res[Idx(g,m,3,0,kk,jj,ii)], ww*gth[0]);
+          AtomicAdd(&res[Idx(g,m,3,1,kk,jj,ii)], ww*gth[1]);
+          AtomicAdd(&res[Idx(g,m,3,2,kk,jj,ii)], ww*gth[2]);
+        }
+      }
+  }
+}
+
+// dust_solver.cpp:243-260 (dust_applya_dc / dust_applya_rho)
+__global__ void k_applya_cell(Geo g, const double *u0, const double *qdep, double *field,
+                              double *res, int correct) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double rho = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+    if (correct) {
+      const double pinv = 1.0/(rho + qdep[Idx(g,m,5,0,k,j,i)]);
+      for (int d=0; d<3; ++d) {
+        const double av = res[Idx(g,m,3,d,k,j,i)] + rho*field[Idx(g,m,3,d,k,j,i)];
+        const double rv = u0[Idx(g,m,DK_NGAS,DK_IM1+d,k,j,i)] + qdep[Idx(g,m,5,1+d,k,j,i)] - av;
+        field[Idx(g,m,3,d,k,j,i)] += pinv*rv;
+      }
+    } else {
+      for (int d=0; d<3; ++d) res[Idx(g,m,3,d,k,j,i)] += rho*field[Idx(g,m,3,d,k,j,i)];
+    }
+  }
+}
+
+// dust_pm.cpp:387-402 (dust_pc2_apply_predictor)
+__global__ void k_pc2_apply_predictor(Geo g, double *u0, double *qdep, int damp, double dt) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    if (damp) {
+      const double chi = dt*qdep[Idx(g,m,5,4,k,j,i)]/u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+      if (chi > 1.0) {
+        const double s = 1.0/chi;
+        for (int c=1; c<=3; ++c) qdep[Idx(g,m,5,c,k,j,i)] *= s;
+      }
+    }
+    for (int c=0; c<3; ++c) u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)] += qdep[Idx(g,m,5,1+c,k,j,i)];
+  }
+}
+
+// dust_pm.cpp:500-564 (dust_gather)
+__global__ void k_gather_kick(Geo g, const DevBlock *B, double *r, const int32_t *iv,
+                              long long cap, long long np, const double *us, double *dmom,
+                              double a_dt, double sh, int br, dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    Stencil w;
+    MakeStencil(g, b, PR(DK_IPX,p), PR(DK_IPY,p) + sh*PR(DK_IPX,p), PR(DK_IPZ,p), w);
+    if (!InPad(g, w)) {dk::RecordError(err, DK_EHALO, p); continue;}
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    for (int c=0; c<=2; ++c)
+      for (int bb=0; bb<3; ++bb) {
+        const double wcb = w.wz[c]*w.wy[bb];
+        if (wcb == 0.0) continue;
+        for (int a=0; a<3; ++a) {
+          const double ww = wcb*w.wx[a];
+          const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+          ux += ww*us[Idx(g,m,3,0,kk,jj,ii)];
+          uy += ww*us[Idx(g,m,3,1,kk,jj,ii)];
+          uz += ww*us[Idx(g,m,3,2,kk,jj,ii)];
+        }
+      }
+    const double cj = a_dt/(PR(DK_IPTS,p) + a_dt);
+    const double dvx = cj*(ux - PR(DK_IPVX,p));
+    const double dvy = cj*(uy - PR(DK_IPVY,p));
+    const double dvz = cj*(uz - PR(DK_IPVZ,p));
+    PR(DK_IPVX,p) += dvx;
+    PR(DK_IPVY,p) += dvy;
+    PR(DK_IPVZ,p) += dvz;
+    PR(DK_IPRX,p) = dvx/a_dt;
+    PR(DK_IPRY,p) = dvy/a_dt;
+    PR(DK_IPRZ,p) = dvz/a_dt;
+    if (br) {
+      const double vol = b.dx1*b.dx2*b.dx3;
+      const double fac = -PR(DK_IPM,p)/vol;
+      for (int c=0; c<=2; ++c)
+        for (int bb=0; bb<3; ++bb) {
+          const double wcb = w.wz[c]*w.wy[bb]*fac;
+          if (wcb == 0.0) continue;
+          for (int a=0; a<3; ++a) {
+            const double ww = wcb*w.wx[a];
+            const int kk = w.kp+c-1, jj = w.jp+bb-1, ii = w.ip+a-1;
+            AtomicAdd(&dmom[Idx(g,m,3,0,kk,jj,ii)], ww*dvx);
+            AtomicAdd(&dmom[Idx(g,m,3,1,kk,jj,ii)], ww*dvy);
+            AtomicAdd(&dmom[Idx(g,m,3,2,kk,jj,ii)], ww*dvz);
+          }
+        }
+    }
+  }
+}
+
+// dust_pm.cpp:660-667 (dust_imex_point_center_mass)
+__global__ void k_center_mass(Geo g, const DevBlock *B, const double *r, const int32_t *iv,
+                              long long cap, long long np, double *res, double a_dt) {
+  GRID_STRIDE(p, np) {
+    if (PR(DK_IPM,p) <= 0.0) continue;
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    Stencil w;
+    MakeStencil(g, b, PR(DK_IPX,p), PR(DK_IPY,p), PR(DK_IPZ,p), w);
+    if (!InPad(g, w)) continue;
+    const double vol = b.dx1*b.dx2*b.dx3;
+    AtomicAdd(&res[Idx(g,m,4,3,w.kp,w.jp,w.ip)], PR(DK_IPM,p)*(a_dt/(PR(DK_IPTS,p) + a_dt))/vol);
+  }
+}
+
+// dust_pm.cpp:670-685 (dust_imex_point_consistent_gas)
+__global__ void k_consistent_gas(Geo g, double *u0, const double *us, const double *qdep,
+                                 double *dmom, double *res, double inv_adt) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double rho = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)], qt = res[Idx(g,m,4,3,k,j,i)];
+    const double q = qdep[Idx(g,m,5,0,k,j,i)];
+    for (int c=0; c<3; ++c) {
+      double dgas = dmom[Idx(g,m,3,c,k,j,i)];
+      if (qt > 0.0) {
+        const double gg = rho*us[Idx(g,m,3,c,k,j,i)] - u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)];
+        const double rr2 = (dmom[Idx(g,m,3,c,k,j,i)] - gg)/(rho + q);
+        dgas = gg + rho*rr2;
+        res[Idx(g,m,4,c,k,j,i)] = rr2*q/qt;
+      }
+      u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)] += dgas;
+      dmom[Idx(g,m,3,c,k,j,i)] = dgas*inv_adt;
+    }
+  }
+}
+
+// dust_pm.cpp:686-696 (dust_imex_point_consistent_dust)
+__global__ void k_consistent_dust(Geo g, const DevBlock *B, double *r, const int32_t *iv,
+                                  long long cap, long long np, const double *res, double a_dt) {
+  GRID_STRIDE(p, np) {
+    if (PR(DK_IPM,p) <= 0.0) continue;
+    const int m = PI(DK_PGID,p);
+    const DevBlock &b = B[m];
+    Stencil w;
+    MakeStencil(g, b, PR(DK_IPX,p), PR(DK_IPY,p), PR(DK_IPZ,p), w);
+    if (!InPad(g, w)) continue;
+    const double cj = a_dt/(PR(DK_IPTS,p) + a_dt);
+    for (int c=0; c<3; ++c) {
+      const double dv = cj*res[Idx(g,m,4,c,w.kp,w.jp,w.ip)];
+      PR(DK_IPVX+2*c,p) += dv;
+      PR(DK_IPRX+c,p) += dv/a_dt;
+    }
+  }
+}
+
+// dust_pm.cpp:700-708 (dust_pmbr), 596-601 (dust_pc2_commit), 734-739 (dust_gatwid)
+__global__ void k_cell_update(Geo g, double *u0, double *dmom, const double *qdep, int mode,
+                              double c0) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    for (int c=0; c<3; ++c) {
+      double &u = u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)];
+      double &d = dmom[Idx(g,m,3,c,k,j,i)];
+      if (mode == 0) {         // pmbr: u0 += dmom, dmom *= 1/(a dt)
+        u += d;
+        d *= c0;
+      } else if (mode == 1) {  // pc2 commit
+        u += d - c0*qdep[Idx(g,m,5,1+c,k,j,i)];
+      } else {                 // drag history
+        u += c0*d;
+      }
+    }
+  }
+}
+
+// Hydro::RKUpdate with zero fluxes + HydroSrcTerms (see dustk.h dk_gas_src)
+__global__ void k_gas_hook(Geo g, const DevBlock *B, double *u0, const double *u1, double gam0,
+                           double gam1, double bdt, int use_src, dk_gas_src src) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    double w[4];
+    const double den = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+    w[0] = den;
+    for (int c=1; c<4; ++c) w[c] = u0[Idx(g,m,DK_NGAS,c,k,j,i)]/den;
+    for (int v=0; v<DK_NGAS; ++v) {
+      double &u = u0[Idx(g,m,DK_NGAS,v,k,j,i)];
+      u = gam0*u + gam1*u1[Idx(g,m,DK_NGAS,v,k,j,i)];
+    }
+    if (!use_src) continue;
+    if (src.const_accel != 0.0)
+      u0[Idx(g,m,DK_NGAS,src.const_accel_dir,k,j,i)] += bdt*src.const_accel*w[0];
+    if (src.shearing_box) {
+      const double coef1 = 2.0*bdt*src.omega0;
+      const double coef2 = (2.0-src.qshear)*bdt*src.omega0;
+      const double coef3 = bdt*(src.omega0*src.omega0);
+      const double mom1 = w[0]*w[1], mom2 = w[0]*w[2];
+      u0[Idx(g,m,DK_NGAS,DK_IM1,k,j,i)] += coef1*mom2;
+      u0[Idx(g,m,DK_NGAS,DK_IM2,k,j,i)] -= coef2*mom1;
+      if (src.stratified) {
+        const DevBlock &b = B[m];
+        const double xx = (static_cast<double>(k - g.ks) + 0.5)/g.nx3;
+        const double x3v = (xx*b.x3max - xx*b.x3min) - (0.5*b.x3max - 0.5*b.x3min) +
+                           (0.5*b.x3min + 0.5*b.x3max);
+        u0[Idx(g,m,DK_NGAS,DK_IM3,k,j,i)] -= coef3*w[0]*x3v;
+      }
+    }
+  }
+}
+
+// FP64 sums of the particles and active cells; fixed grid, partials reduced in order
+constexpr int kDiagGrid = 512;
+__global__ void k_diag_part(Geo g, const DevBlock *B, const double *r, long long cap,
+                            long long np, const double *u0, double *partial) {
+  __shared__ double sh[kBlock][9];
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  GRID_STRIDE(p, np) {
+    const double m = PR(DK_IPM,p);
+    acc[0] += m*PR(DK_IPVX,p);
+    acc[1] += m*PR(DK_IPVY,p);
+    acc[2] += m*PR(DK_IPVZ,p);
+    acc[6] += m;
+    acc[8] += m*(fabs(PR(DK_IPVX,p)) + fabs(PR(DK_IPVY,p)) + fabs(PR(DK_IPVZ,p)));
+  }
+  const long long nc = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, nc) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double vol = B[m].dx1*B[m].dx2*B[m].dx3;
+    acc[7] += vol*u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+    for (int c=0; c<3; ++c) {
+      const double mom = u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)];
+      acc[3+c] += vol*mom;
+      acc[8] += vol*fabs(mom);
+    }
+  }
+  for (int c=0; c<9; ++c) sh[threadIdx.x][c] = acc[c];
+  __syncthreads();
+  for (int s=kBlock/2; s>0; s>>=1) {
+    if (threadIdx.x < s)
+      for (int c=0; c<9; ++c) sh[threadIdx.x][c] += sh[threadIdx.x + s][c];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) partial[blockIdx.x*9 + threadIdx.x] = sh[0][threadIdx.x];
+}
+
+__global__ void k_reduce_partials(const double *partial, int nblk, int ncomp, double *out) {
+  const int c = threadIdx.x;
+  if (c >= ncomp) return;
+  double s = 0.0;
+  for (int b=0; b<nblk; ++b) s += partial[b*ncomp + c];
+  out[c] = s;
+}
+
+// dust_newdt.cpp:135-145: min over particles of dx/|v| (and the stopping-time check)
+__global__ void k_newdt(const DevBlock *B, const double *r, const int32_t *iv, long long cap,
+                        long long np, double *partial, dk::ErrWord *err) {
+  __shared__ double sh[kBlock];
+  double dtmin = 1.0e300;
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    if (PR(DK_IPTS,p) <= 0.0) dk::RecordError(err, DK_ESTOP, p);
+    const double vx = fabs(PR(DK_IPVX,p)), vy = fabs(PR(DK_IPVY,p)), vz = fabs(PR(DK_IPVZ,p));
+    if (vx > 0.0) dtmin = fmin(dtmin, B[m].dx1/vx);
+    if (vy > 0.0) dtmin = fmin(dtmin, B[m].dx2/vy);
+    if (vz > 0.0) dtmin = fmin(dtmin, B[m].dx3/vz);
+  }
+  sh[threadIdx.x] = dtmin;
+  __syncthreads();
+  for (int s=kBlock/2; s>0; s>>=1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// cell sort (dust_cloud_device.cpp:1903-1950): key = block*ncell + cell of the position
+__global__ void k_sort_keys(Geo g, const DevBlock *B, const double *r, const int32_t *iv,
+                            long long cap, long long np, unsigned long long *keys, int *idx) {
+  GRID_STRIDE(p, np) {
+    const int m = PI(DK_PGID,p);
+    Stencil w;
+    MakeStencil(g, B[m], PR(DK_IPX,p), PR(DK_IPY,p), PR(DK_IPZ,p), w);
+    const long long cell = ((long long)(w.kp - g.ks)*g.nx2 + (w.jp - g.js))*g.nx1 + (w.ip - g.is);
+    keys[p] = (unsigned long long)m*g.nx1*g.nx2*g.nx3 + (unsigned long long)cell;
+    idx[p] = static_cast<int>(p);
+  }
+}
+
+__global__ void k_permute(const double *r, const int32_t *iv, long long cap, long long np,
+                          const int *order, double *rbuf, int32_t *ibuf) {
+  GRID_STRIDE(q, np) {
+    const long long p = order[q];
+    for (int c=0; c<DK_NRDATA; ++c) rbuf[(long long)c*np + q] = r[(long long)c*cap + p];
+    for (int c=0; c<DK_NIDATA; ++c) ibuf[(long long)c*np + q] = iv[(long long)c*cap + p];
+  }
+}
+
+__global__ void k_unpermute_copy(double *r, int32_t *iv, long long cap, long long np,
+                                 const double *rbuf, const int32_t *ibuf) {
+  GRID_STRIDE(q, np) {
+    for (int c=0; c<DK_NRDATA; ++c) r[(long long)c*cap + q] = rbuf[(long long)c*np + q];
+    for (int c=0; c<DK_NIDATA; ++c) iv[(long long)c*cap + q] = ibuf[(long long)c*np + q];
+  }
+}
+
+#undef PR
+#undef PI
+
+//----------------------------------------------------------------------------------------
+// host side
+
+bool ActiveStage(const dk_ctx *c, const dk_stage *st) {   // dust.cpp:1646-1649
+  if (c->P.coupling == DK_PC2) return true;
+  return !(st->nexp_stages == 3 && st->stage == st->nexp_stages);
+}
+
+bool MigrationActive(const dk_ctx *c, const dk_stage *st) {   // dust.cpp:1651-1654
+  if (c->P.coupling == DK_IMEX) return ActiveStage(c, st);
+  return st->stage == 2;
+}
+
+double DragStep(const dk_ctx *c, const dk_stage *st) {   // dust.cpp:1704-1707
+  if (c->P.coupling == DK_PC2) return st->dt;
+  return st->a_impl*st->dt;
+}
+
+double ImplicitShearShift(const dk_ctx *c, const dk_stage *st) {   // dust.hpp:1699-1702
+  if (!c->P.shearing_box || c->P.coupling != DK_IMEX) return 0.0;
+  double cc = 0.0;
+  for (int k=0; k<st->stage; ++k) cc = st->gam0[k]*cc + st->beta[k];
+  const double timp = st->time + cc*st->dt;
+  return c->P.qshear*c->P.omega0*(timp - st->time);
+}
+
+double ShearRouteTime(const dk_ctx *c, const dk_stage *st) {   // dust.cpp:1668-1676
+  const double t = st->time, dt = st->dt;
+  if (c->P.coupling != DK_IMEX) return t + dt;
+  double w = 0.0;
+  for (int k = st->stage + 1; k <= st->nexp_stages; ++k) w = st->gam0[k-1]*w + st->beta[k-1];
+  return t + (1.0 - w)*dt;
+}
+
+int Launched(dk_ctx *c) { return dk::Check(hipGetLastError()); }
+
+int Ready(const dk_ctx *c, bool need_particles = true) {
+  if (c == nullptr) return DK_EINVAL;
+  if (c->form != DK_FORM_FP64) return DK_EUNSUPPORTED;
+  if (need_particles && c->rdata == nullptr) return DK_ESTATE;
+  if (c->u0 == nullptr) return DK_ESTATE;
+  return DK_OK;
+}
+
+#define PARGS c->rdata, c->idata, (long long)c->cap, (long long)c->np
+#define PGRID Grid(c->np), kBlock, 0, c->stream
+#define CGRID(n) Grid(n), kBlock, 0, c->stream
+
+long long ActiveCells(const dk_ctx *c) {
+  return (long long)c->g.nmb*c->g.nx3*c->g.nx2*c->g.nx1;
+}
+
+int FillGhosts(dk_ctx *c, double *a, int nvar, bool velocity, double t) {
+  return dk::FillGhostsT<double>(c, a, nvar, velocity, t);
+}
+
+int Fold(dk_ctx *c, double *a, int nvar, double t) { return dk::FoldT<double>(c, a, nvar, t); }
+
+int ApplyCoupledOperator(dk_ctx *c, double a_dt, double sh, bool correct, double t) {
+  int e;
+  if ((e = FillGhosts(c, c->ustar, 3, true, t))) return e;
+  if ((e = dk::Zero(c, c->ap, 3*dk::NCells(c)))) return e;
+  {const int _pt = dk::ProfBegin(c, "applya_gtsg");
+  k_applya_gtsg<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->ustar, c->ap, a_dt, sh, c->d_err);
+   dk::ProfEnd(c, _pt);}
+  if ((e = Fold(c, c->ap, 3, t))) return e;
+  {const int _pt = dk::ProfBegin(c, "applya_cell");
+  k_applya_cell<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, c->ustar, c->ap,
+                                           correct ? 1 : 0);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+}  // namespace
+
+namespace dk {
+
+int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
+  const int s = st->stage - 1;
+  const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
+  if (bdt == 0.0 && gam0 == 1.0 && gam1 == 0.0) return DK_OK;   // identity stage
+  dk_gas_src z;
+  std::memset(&z, 0, sizeof(z));
+  {const int _pt = dk::ProfBegin(c, "gas_hook");
+  k_gas_hook<<<CGRID(ActiveCells(c))>>>(c->g, c->d_blocks, c->u0, c->u1, gam0, gam1, bdt,
+                                        src ? 1 : 0, src ? *src : z);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+}  // namespace dk
+
+//========================================================================================
+// C-ABI
+
+extern "C" {
+
+const char *dk_strerror(int code) {
+  switch (code) {
+    case DK_OK: return "ok";
+    case DK_EINVAL: return "invalid argument";
+    case DK_EHIP: return "HIP runtime error";
+    case DK_ENOMEM: return "device allocation failed";
+    case DK_EHALO: return "particle outside the deposit halo of its block";
+    case DK_EROUTE: return "particle moved more than one block";
+    case DK_ESTOP: return "non-positive stopping time";
+    case DK_EUNSUPPORTED: return "option not provided";
+    case DK_ECAPACITY: return "size beyond a fixed maximum";
+    case DK_ESTATE: return "call order violated";
+    case DK_ENOTFINITE: return "non-finite value";
+    default: return "unknown error";
+  }
+}
+
+int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phys,
+              int32_t form, void *stream, dk_ctx **out) {
+  if (mesh == nullptr || blocks == nullptr || phys == nullptr || out == nullptr)
+    return DK_EINVAL;
+  if (mesh->nmb < 1 || mesh->nx1 < 2 || mesh->nx2 < 2 || mesh->nx3 < 2 || mesh->ng < 1 ||
+      mesh->nmb != mesh->nmbx1*mesh->nmbx2*mesh->nmbx3)
+    return DK_EINVAL;
+  if (form != DK_FORM_FP64 && form != DK_FORM_FP32) return DK_EINVAL;
+  if (phys->solver < DK_LOCAL || phys->solver > DK_DC2) return DK_EUNSUPPORTED;
+  dk_ctx *c = new dk_ctx();
+  c->M = *mesh;
+  c->P = *phys;
+  c->B.assign(blocks, blocks + mesh->nmb);
+  c->form = form;
+  c->stream = static_cast<hipStream_t>(stream);
+  (void)hipGetDevice(&c->device);
+  Geo &g = c->g;
+  g.nmb = mesh->nmb; g.nx1 = mesh->nx1; g.nx2 = mesh->nx2; g.nx3 = mesh->nx3; g.ng = mesh->ng;
+  g.n1 = g.nx1 + 2*g.ng; g.n2 = g.nx2 + 2*g.ng; g.n3 = g.nx3 + 2*g.ng;
+  g.is = g.ng; g.ie = g.ng + g.nx1 - 1;
+  g.js = g.ng; g.je = g.ng + g.nx2 - 1;
+  g.ks = g.ng; g.ke = g.ng + g.nx3 - 1;
+  g.ncb = (long long)g.n1*g.n2*g.n3;
+  g.nmbx1 = mesh->nmbx1; g.nmbx2 = mesh->nmbx2; g.nmbx3 = mesh->nmbx3;
+  g.bc_x1 = mesh->bc_x1; g.bc_x3 = mesh->bc_x3; g.x3_dep_drop = mesh->x3_dep_drop;
+  g.x1min = mesh->x1min; g.x1max = mesh->x1max;
+  g.x2min = mesh->x2min; g.x2max = mesh->x2max;
+  g.x3min = mesh->x3min; g.x3max = mesh->x3max;
+  g.deposit = phys->deposit;
+  std::vector<DevBlock> hb(mesh->nmb);
+  for (int m=0; m<mesh->nmb; ++m) {
+    const dk_block &b = blocks[m];
+    DevBlock &d = hb[m];
+    d.x1min = b.x1min; d.x1max = b.x1max; d.x2min = b.x2min; d.x2max = b.x2max;
+    d.x3min = b.x3min; d.x3max = b.x3max;
+    d.dx1 = (b.x1max - b.x1min)/static_cast<double>(g.nx1);   // meshblock.cpp:113-118
+    d.dx2 = (b.x2max - b.x2min)/static_cast<double>(g.nx2);
+    d.dx3 = (b.x3max - b.x3min)/static_cast<double>(g.nx3);
+    d.lx1 = b.lx1; d.lx2 = b.lx2; d.lx3 = b.lx3; d.pad = 0;
+    for (int q=0; q<27; ++q) d.nghbr[q] = b.nghbr[q];
+  }
+  // the same-level fold slots in NeighborIndex order (src/mesh/nghbr_index.hpp:19-54)
+  std::vector<std::pair<int,int>> order;
+  for (int o3=-1; o3<=1; ++o3)
+    for (int o2=-1; o2<=1; ++o2)
+      for (int o1=-1; o1<=1; ++o1) {
+        if (o1 == 0 && o2 == 0 && o3 == 0) continue;
+        int slot;
+        if (o3 == 0) slot = (o1*o2 == 0) ? (std::abs(o1)*2*(o1 + 1) + std::abs(o2)*2*(o2 + 5))
+                                         : (16 + (o1 + 1) + 2*(o2 + 1));
+        else slot = (o1*o2 == 0) ? (24 + std::abs(o1)*(o1 + 9) + std::abs(o2)*(o2 + 17) +
+                                    2*(o3 + 1))
+                                 : (48 + (o1 + 1)/2 + (o2 + 1) + 2*(o3 + 1));
+        order.push_back({slot, Dir(o1, o2, o3)});
+      }
+  std::sort(order.begin(), order.end());
+  int sd[26];
+  for (int q=0; q<26; ++q) sd[q] = order[q].second;
+  const long long nc = dk::NCells(c);
+  int e = DK_OK;
+  auto alloc = [&](void **p, size_t bytes) {
+    if (e) return;
+    if (hipMalloc(p, bytes) != hipSuccess) {*p = nullptr; e = DK_ENOMEM;}
+  };
+  alloc((void**)&c->d_blocks, sizeof(DevBlock)*mesh->nmb);
+  alloc((void**)&c->d_slot_dir, sizeof(int)*26);
+  alloc((void**)&c->d_err, sizeof(dk::ErrWord));
+  alloc((void**)&c->d_partial, sizeof(double)*9*4096);
+  alloc((void**)&c->d_scalar, sizeof(double)*16);
+  if (form == DK_FORM_FP64) {
+    alloc((void**)&c->qdep, sizeof(double)*5*nc);
+    alloc((void**)&c->ustar, sizeof(double)*3*nc);
+    alloc((void**)&c->dmom, sizeof(double)*3*nc);
+    alloc((void**)&c->ap, sizeof(double)*3*nc);
+    alloc((void**)&c->resid, sizeof(double)*4*nc);
+  }
+  if (!e) e = dk::Check(hipMemcpy(c->d_blocks, hb.data(), sizeof(DevBlock)*mesh->nmb,
+                                  hipMemcpyHostToDevice));
+  if (!e) e = dk::Check(hipMemcpy(c->d_slot_dir, sd, sizeof(sd), hipMemcpyHostToDevice));
+  if (!e) e = dk::Check(hipMemset(c->d_err, 0, sizeof(dk::ErrWord)));
+  if (!e) e = dk::Check(hipMemset(c->d_scalar, 0, sizeof(double)*16));
+  if (!e && form == DK_FORM_FP64) {
+    for (double *p : {c->qdep, c->ustar, c->dmom, c->ap, c->resid}) {
+      const long long nv = (p == c->qdep) ? 5 : ((p == c->resid) ? 4 : 3);
+      if (!e) e = dk::Check(hipMemset(p, 0, sizeof(double)*nv*nc));
+    }
+  }
+  if (!e && form == DK_FORM_FP32) e = dk::FastCreate(c);
+  if (e) {dk_destroy(c); return e;}
+  *out = c;
+  return DK_OK;
+}
+
+int dk_destroy(dk_ctx *c) {
+  if (c == nullptr) return DK_OK;
+  (void)hipStreamSynchronize(c->stream);
+  if (c->fast) dk::FastDestroy(c);
+  dk::ProfDestroy(c);
+  for (void *p : {(void*)c->d_blocks, (void*)c->d_slot_dir, (void*)c->d_err,
+                  (void*)c->d_partial, (void*)c->d_scalar, (void*)c->qdep, (void*)c->ustar,
+                  (void*)c->dmom, (void*)c->ap, (void*)c->resid, c->sort_tmp,
+                  (void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
+                  (void*)c->sort_ibuf})
+    if (p) (void)hipFree(p);
+  delete c;
+  return DK_OK;
+}
+
+int dk_bind_particles(dk_ctx *c, double *rdata, int32_t *idata, int64_t capacity,
+                      int64_t npart) {
+  if (c == nullptr || rdata == nullptr || idata == nullptr || capacity < npart || npart < 0 ||
+      npart > INT32_MAX)
+    return DK_EINVAL;
+  c->rdata = rdata; c->idata = idata; c->cap = capacity; c->np = npart;
+  if (c->form == DK_FORM_FP32) return dk::FastBind(c);
+  return DK_OK;
+}
+
+int dk_bind_gas(dk_ctx *c, double *u0, double *u1) {
+  if (c == nullptr || u0 == nullptr || u1 == nullptr) return DK_EINVAL;
+  c->u0 = u0; c->u1 = u1;
+  return DK_OK;
+}
+
+int dk_export_particles(dk_ctx *c) {
+  if (c == nullptr) return DK_EINVAL;
+  if (c->form == DK_FORM_FP32) return dk::FastExport(c);
+  return DK_OK;
+}
+
+int64_t dk_npart(const dk_ctx *c) { return c ? c->np : 0; }
+
+// src/dust/dust_push.cpp:35-80
+int dk_first_two_imp_rk(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c);
+  if (e) return e;
+  if (c->P.coupling == DK_PC2 && st->stage == 2) {
+    {const int _pt = dk::ProfBegin(c, "gas_midpoint");
+    k_gas_midpoint<<<CGRID(dk::NCells(c))>>>(c->g, c->u0, c->u1, c->ustar);
+     dk::ProfEnd(c, _pt);}
+    return Launched(c);
+  }
+  if (st->stage != 1) return DK_OK;
+  e = dk::Check(hipMemcpyAsync(c->u1, c->u0, sizeof(double)*DK_NGAS*dk::NCells(c),
+                               hipMemcpyDeviceToDevice, c->stream));
+  if (e) return e;
+  {const int _pt = dk::ProfBegin(c, "reg_copy");
+  k_reg_copy<<<PGRID>>>(c->rdata, (long long)c->cap, (long long)c->np);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+// src/dust/dust_push.cpp:91-300
+int dk_explicit_push(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c);
+  if (e || !ActiveStage(c, st)) return e;
+  const int sbox = c->P.shearing_box, strat = c->P.stratified;
+  const double qs = c->P.qshear, om = c->P.omega0;
+  if (c->P.coupling == DK_PC2) {
+    if (st->stage != 2) return DK_OK;
+    if (c->P.back_reaction && (e = dk::Zero(c, c->dmom, 3*dk::NCells(c)))) return e;
+    {const int _pt = dk::ProfBegin(c, "pc2_advance");
+    k_pc2_advance<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->ustar, c->dmom, st->dt,
+                             c->P.back_reaction, sbox, strat, qs, om, c->d_err);
+     dk::ProfEnd(c, _pt);}
+    return Launched(c);
+  }
+  const int s = st->stage - 1;
+  const double atw_dt = (st->stage == 2) ? st->a_twid22*st->dt : 0.0;
+  {const int _pt = dk::ProfBegin(c, "push_imex");
+  k_push_imex<<<PGRID>>>(c->rdata, (long long)c->cap, (long long)c->np, st->gam0[s],
+                         st->gam1[s], st->beta[s]*st->dt, atw_dt, sbox, strat, qs, om);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+// src/dust/dust_tasks.cpp:149-162, dust.cpp:1678-1702, bvals_part.cpp:115-471
+int dk_update_particle_gids(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c);
+  if (e || !MigrationActive(c, st)) return e;
+  const bool shear = (c->M.bc_x1 == DK_X1_SHEAR);
+  double ysh = 0.0;
+  if (shear) {
+    const double tau = ShearRouteTime(c, st);
+    const double dsh = c->P.qshear*c->P.omega0*(c->M.x1max - c->M.x1min)*(tau - st->time);
+    if (dsh != 0.0)
+      {const int _pt = dk::ProfBegin(c, "shift_shear_crossings");
+      k_shift_shear_crossings<<<PGRID>>>(c->d_blocks, PARGS, dsh, c->M.x1min, c->M.x1max);
+       dk::ProfEnd(c, _pt);}
+    const double lx = c->M.x1max - c->M.x1min, ly = c->M.x2max - c->M.x2min;
+    ysh = std::fmod(c->P.qshear*c->P.omega0*lx*st->time, ly);
+  }
+  {const int _pt = dk::ProfBegin(c, "part_update");
+  k_part_update<<<PGRID>>>(c->g, c->d_blocks, PARGS, shear ? 1 : 0, ysh, c->d_scalar + 12,
+                           c->d_err);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+// src/dust/dust_pm.cpp:104-277
+int dk_deposit_drag(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c);
+  if (e || !ActiveStage(c, st) || !c->P.back_reaction) return e;
+  if (c->P.coupling == DK_PC2) {
+    if (st->stage != 1) return DK_OK;
+    if ((e = dk::Zero(c, c->qdep, 5*dk::NCells(c)))) return e;
+    {const int _pt = dk::ProfBegin(c, "pc2_predictor");
+    k_pc2_predictor<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->u1, c->qdep, st->dt,
+                               c->P.stiffness, c->d_err);
+     dk::ProfEnd(c, _pt);}
+    return Launched(c);
+  }
+  if ((e = dk::Zero(c, c->qdep, 5*dk::NCells(c)))) return e;
+  {const int _pt = dk::ProfBegin(c, "scatter_imex");
+  k_scatter_imex<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->qdep, DragStep(c, st),
+                            ImplicitShearShift(c, st), c->d_err);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+// src/dust/dust_tasks.cpp:249-263
+int dk_fold_deposit_qp(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c, false);
+  const bool use = (c && c->P.coupling == DK_IMEX) || st->stage == 1;
+  if (e || !ActiveStage(c, st) || !c->P.back_reaction || !use) return e;
+  return Fold(c, c->qdep, 5, st->time);
+}
+
+// src/dust/dust_pm.cpp:372-442, src/dust/dust_solver.cpp:630-724
+int dk_gas_implicit_solve(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c);
+  if (e || !ActiveStage(c, st)) return e;
+  if (c->P.coupling == DK_PC2) {
+    if (st->stage == 1 && c->P.back_reaction) {
+      {const int _pt = dk::ProfBegin(c, "pc2_apply_predictor");
+      k_pc2_apply_predictor<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, c->P.pc2_damping,
+                                                       st->dt);
+       dk::ProfEnd(c, _pt);}
+      return Launched(c);
+    }
+    return DK_OK;
+  }
+  const int br = c->P.back_reaction;
+  {const int _pt = dk::ProfBegin(c, "local_guess");
+  k_local_guess<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, c->ustar, br);
+   dk::ProfEnd(c, _pt);}
+  if ((e = Launched(c))) return e;
+  if (!br || c->P.solver == DK_LOCAL) return DK_OK;
+  const double a_dt = DragStep(c, st), sh = ImplicitShearShift(c, st);
+  if (c->P.solver == DK_APPLYA) return ApplyCoupledOperator(c, a_dt, sh, false, st->time);
+  const int nsweeps = (c->P.solver == DK_DC2) ? 2 : 1;
+  for (int s=0; s<nsweeps; ++s)
+    if ((e = ApplyCoupledOperator(c, a_dt, sh, true, st->time))) return e;
+  return DK_OK;
+}
+
+// src/dust/dust_tasks.cpp:265-312
+int dk_fill_ustar(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c, false);
+  if (e || !ActiveStage(c, st) || c->P.coupling != DK_IMEX) return e;
+  return FillGhosts(c, c->ustar, 3, true, st->time);
+}
+
+// src/dust/dust_pm.cpp:450-568
+int dk_gather_kick_pmbr(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c);
+  if (e || !ActiveStage(c, st) || c->P.coupling == DK_PC2) return e;
+  if (c->P.back_reaction && (e = dk::Zero(c, c->dmom, 3*dk::NCells(c)))) return e;
+  {const int _pt = dk::ProfBegin(c, "gather_kick");
+  k_gather_kick<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->ustar, c->dmom, DragStep(c, st),
+                           ImplicitShearShift(c, st), c->P.back_reaction, c->d_err);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+// src/dust/dust_tasks.cpp:314-328
+int dk_fold_pmbr(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c, false);
+  const bool use = (c && c->P.coupling == DK_IMEX) || st->stage == 2;
+  if (e || !ActiveStage(c, st) || !c->P.back_reaction || !use) return e;
+  return Fold(c, c->dmom, 3, st->time);
+}
+
+// src/dust/dust_pm.cpp:576-711
+int dk_apply_pmbr(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c);
+  if (e || !ActiveStage(c, st) || !c->P.back_reaction) return e;
+  if (c->P.coupling == DK_PC2) {
+    if (st->stage != 2) return DK_OK;
+    {const int _pt = dk::ProfBegin(c, "cell_update");
+    k_cell_update<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->dmom, c->qdep, 1,
+                                             st->gam1[st->nexp_stages-1]);
+     dk::ProfEnd(c, _pt);}
+    return Launched(c);
+  }
+  const double inv_adt = 1.0/(st->a_impl*st->dt);
+  if (c->P.consistent_gas) {
+    const double a_dt = DragStep(c, st);
+    if ((e = dk::Zero(c, c->resid, 4*dk::NCells(c)))) return e;
+    {const int _pt = dk::ProfBegin(c, "center_mass");
+    k_center_mass<<<PGRID>>>(c->g, c->d_blocks, c->rdata, c->idata, (long long)c->cap,
+                             (long long)c->np, c->resid, a_dt);
+     dk::ProfEnd(c, _pt);}
+    {const int _pt = dk::ProfBegin(c, "consistent_gas");
+    k_consistent_gas<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->ustar, c->qdep, c->dmom,
+                                                 c->resid, inv_adt);
+     dk::ProfEnd(c, _pt);}
+    {const int _pt = dk::ProfBegin(c, "consistent_dust");
+    k_consistent_dust<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->resid, a_dt);
+     dk::ProfEnd(c, _pt);}
+    return Launched(c);
+  }
+  {const int _pt = dk::ProfBegin(c, "cell_update");
+  k_cell_update<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->dmom, c->qdep, 0, inv_adt);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+// src/dust/dust_pm.cpp:721-742
+int dk_add_drag_history_gas(dk_ctx *c, const dk_stage *st) {
+  int e = Ready(c, false);
+  if (e || c->P.coupling != DK_IMEX || st->stage != 2 || !c->P.back_reaction) return e;
+  {const int _pt = dk::ProfBegin(c, "cell_update");
+  k_cell_update<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->dmom, c->qdep, 2,
+                                           st->a_twid22*st->dt);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+int dk_gas_hook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
+  if (c == nullptr || c->u0 == nullptr) return DK_ESTATE;
+  return dk::GasHook(c, st, src);
+}
+
+int dk_fill_gas_ghosts(dk_ctx *c, const dk_stage *st) {
+  if (c == nullptr || c->u0 == nullptr) return DK_ESTATE;
+  return FillGhosts(c, c->u0, DK_NGAS, false, st->time + st->dt);
+}
+
+int dk_stage_run(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
+  if (c == nullptr || st == nullptr) return DK_EINVAL;
+  if (c->form == DK_FORM_FP32) return dk::FastStage(c, st, src);
+  int e;
+  if ((e = dk_first_two_imp_rk(c, st))) return e;
+  if ((e = dk_gas_hook(c, st, src))) return e;
+  if ((e = dk_add_drag_history_gas(c, st))) return e;
+  if ((e = dk_explicit_push(c, st))) return e;
+  if ((e = dk_update_particle_gids(c, st))) return e;
+  if ((e = dk_deposit_drag(c, st))) return e;
+  if ((e = dk_fold_deposit_qp(c, st))) return e;
+  if ((e = dk_gas_implicit_solve(c, st))) return e;
+  if ((e = dk_fill_ustar(c, st))) return e;
+  if ((e = dk_gather_kick_pmbr(c, st))) return e;
+  if ((e = dk_fold_pmbr(c, st))) return e;
+  if ((e = dk_apply_pmbr(c, st))) return e;
+  if (ActiveStage(c, st) && (e = dk_fill_gas_ghosts(c, st))) return e;
+  return DK_OK;
+}
+
+int dk_set_graphs(dk_ctx *c, int32_t enable) {
+  if (c == nullptr) return DK_EINVAL;
+  c->graphs = enable ? 1 : 0;
+  return DK_OK;
+}
+
+// src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell)
+int dk_sort_particles(dk_ctx *c) {
+  int e = Ready(c);
+  if (e) return e;
+  const long long n = c->np;
+  if (n == 0) return DK_OK;
+  if (c->sort_cap < n) {
+    for (void *p : {(void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
+                    (void*)c->sort_ibuf})
+      if (p) (void)hipFree(p);
+    c->sort_keys = nullptr; c->sort_idx = nullptr; c->sort_rbuf = nullptr;
+    c->sort_ibuf = nullptr;
+    if (hipMalloc((void**)&c->sort_keys, sizeof(unsigned long long)*2*n) != hipSuccess ||
+        hipMalloc((void**)&c->sort_idx, sizeof(int)*2*n) != hipSuccess ||
+        hipMalloc((void**)&c->sort_rbuf, sizeof(double)*DK_NRDATA*n) != hipSuccess ||
+        hipMalloc((void**)&c->sort_ibuf, sizeof(int32_t)*DK_NIDATA*n) != hipSuccess)
+      return DK_ENOMEM;
+    c->sort_cap = n;
+  }
+  {const int _pt = dk::ProfBegin(c, "sort_keys");
+  k_sort_keys<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->sort_keys, c->sort_idx);
+   dk::ProfEnd(c, _pt);}
+  size_t bytes = 0;
+  const int bits = 64 - __builtin_clzll((unsigned long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3);
+  if ((e = dk::Check(hipcub::DeviceRadixSort::SortPairs(
+           nullptr, bytes, c->sort_keys, c->sort_keys + n, c->sort_idx, c->sort_idx + n, (int)n,
+           0, bits, c->stream))))
+    return e;
+  if (bytes > c->sort_tmp_bytes) {
+    if (c->sort_tmp) (void)hipFree(c->sort_tmp);
+    if (hipMalloc(&c->sort_tmp, bytes) != hipSuccess) {c->sort_tmp = nullptr; return DK_ENOMEM;}
+    c->sort_tmp_bytes = bytes;
+  }
+  if ((e = dk::Check(hipcub::DeviceRadixSort::SortPairs(
+           c->sort_tmp, bytes, c->sort_keys, c->sort_keys + n, c->sort_idx, c->sort_idx + n,
+           (int)n, 0, bits, c->stream))))
+    return e;
+  {const int _pt = dk::ProfBegin(c, "permute");
+  k_permute<<<PGRID>>>(c->rdata, c->idata, (long long)c->cap, n, c->sort_idx + n, c->sort_rbuf,
+                       c->sort_ibuf);
+   dk::ProfEnd(c, _pt);}
+  {const int _pt = dk::ProfBegin(c, "unpermute_copy");
+  k_unpermute_copy<<<PGRID>>>(c->rdata, c->idata, (long long)c->cap, n, c->sort_rbuf,
+                              c->sort_ibuf);
+   dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+int dk_new_time_step(dk_ctx *c, double dt_cfl, double *dt_out) {
+  int e = Ready(c);
+  if (e) return e;
+  const int nb = 1024;
+  k_newdt<<<nb, kBlock, 0, c->stream>>>(c->d_blocks, c->rdata, c->idata, (long long)c->cap,
+                                        (long long)c->np, c->d_partial, c->d_err);
+  std::vector<double> h(nb);
+  if ((e = dk::Check(hipMemcpyAsync(h.data(), c->d_partial, sizeof(double)*nb,
+                                    hipMemcpyDeviceToHost, c->stream))))
+    return e;
+  if ((e = dk::Check(hipStreamSynchronize(c->stream)))) return e;
+  double m = 1.0e300;
+  for (double x : h) m = std::fmin(m, x);
+  if (dt_out) *dt_out = dt_cfl*m;
+  return DK_OK;
+}
+
+int dk_diag_sums(dk_ctx *c, double out[9]) {
+  if (c == nullptr || out == nullptr) return DK_EINVAL;
+  if (c->form == DK_FORM_FP32) return dk::FastDiag(c, out);
+  if (c->u0 == nullptr || c->rdata == nullptr) return DK_ESTATE;
+  k_diag_part<<<kDiagGrid, kBlock, 0, c->stream>>>(c->g, c->d_blocks, c->rdata,
+                                                   (long long)c->cap, (long long)c->np, c->u0,
+                                                   c->d_partial);
+  k_reduce_partials<<<1, 64, 0, c->stream>>>(c->d_partial, kDiagGrid, 9, c->d_scalar);
+  int e = dk::Check(hipMemcpyAsync(out, c->d_scalar, sizeof(double)*9, hipMemcpyDeviceToHost,
+                                   c->stream));
+  if (!e) e = dk::Check(hipStreamSynchronize(c->stream));
+  return e;
+}
+
+int dk_error(dk_ctx *c, int32_t *code, int64_t *particle) {
+  if (c == nullptr) return DK_EINVAL;
+  dk::ErrWord h;
+  int e = dk::Check(hipMemcpyAsync(&h, c->d_err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  if (!e) e = dk::Check(hipStreamSynchronize(c->stream));
+  if (e) return e;
+  if (code) *code = h.code;
+  if (particle) *particle = h.particle;
+  if (h.code != 0) {
+    e = dk::Check(hipMemsetAsync(c->d_err, 0, sizeof(dk::ErrWord), c->stream));
+    if (!e) e = dk::Check(hipStreamSynchronize(c->stream));
+  }
+  return e;
+}
+
+void *dk_field(dk_ctx *c, int32_t which, int32_t *nvar) {
+  if (c == nullptr) return nullptr;
+  int nv = 0;
+  void *p = nullptr;
+  switch (which) {
+    case 0: nv = 5; p = c->qdep; break;
+    case 1: nv = 3; p = c->ustar; break;
+    case 2: nv = 3; p = c->dmom; break;
+    case 3: nv = 3; p = c->ap; break;
+    case 4: nv = 4; p = c->resid; break;
+    default: break;
+  }
+  if (nvar) *nvar = nv;
+  return p;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// dk_internal.hpp -- shared definitions of the gfx950 drag-coupling library.
+#ifndef ATHENAK_AMD_CSRC_DK_INTERNAL_HPP_
+#define ATHENAK_AMD_CSRC_DK_INTERNAL_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/dustk.h"
+#include "../../include/dustk_setup.h"
+
+namespace dk {
+
+constexpr int kBlock = 256;   // threads per workgroup of the streaming kernels
+
+// Mesh geometry handed to kernels by value.
+struct Geo {
+  int nmb, nx1, nx2, nx3, ng, n1, n2, n3;
+  int is, ie, js, je, ks, ke;
+  long long ncb;           // cells per block and variable, ghosts included
+  int nmbx1, nmbx2, nmbx3;
+  int bc_x1, bc_x3, x3_dep_drop;
+  double x1min, x1max, x2min, x2max, x3min, x3max;
+  int deposit;
+};
+
+// Block geometry on the device: the box and cell sizes (MeshBlock::mb_size) and neighbours.
+struct DevBlock {
+  double x1min, x1max, x2min, x2max, x3min, x3max;
+  double dx1, dx2, dx3;
+  int lx1, lx2, lx3, pad;
+  int nghbr[27];
+};
+
+__host__ __device__ inline long long Idx(const Geo &g, int m, int nvar, int v, int k, int j,
+                                         int i) {
+  return ((((long long)m*nvar + v)*g.n3 + k)*g.n2 + j)*(long long)g.n1 + i;
+}
+
+__host__ __device__ inline int Dir(int ox1, int ox2, int ox3) {
+  return (ox1 + 1) + 3*(ox2 + 1) + 9*(ox3 + 1);
+}
+
+// src/dust/dust.hpp:1088-1111 (same operations in the same order)
+template <typename T>
+__device__ inline void PMWeights(const double x, const double xmin, const double xmax,
+                                 const int nx, const int is, const int scheme, int &ip, T w[3]) {
+  double cell = (x - xmin)*(static_cast<double>(nx)/(xmax - xmin));
+  int ig = static_cast<int>(cell + 1.0) - 1;
+  double del = cell - (static_cast<double>(ig) + 0.5);
+  ip = ig + is;
+  if (scheme == 0) {
+    w[0] = 0.0; w[1] = 1.0; w[2] = 0.0;
+  } else if (scheme == 1) {
+    w[0] = fmax(0.0, -del);
+    w[1] = 1.0 - fabs(del);
+    w[2] = fmax(0.0, del);
+  } else {
+    w[0] = 0.5*(0.5 - del)*(0.5 - del);
+    w[1] = 0.75 - del*del;
+    w[2] = 0.5*(0.5 + del)*(0.5 + del);
+  }
+}
+
+// First error wins: code and particle index.
+struct ErrWord {
+  int code;
+  int pad;
+  long long particle;
+};
+
+__device__ inline void RecordError(ErrWord *e, int code, long long p) {
+  if (atomicCAS(&e->code, 0, code) == 0) e->particle = p;
+}
+
+}  // namespace dk
+
+struct dk_ctx {
+  dk_mesh M;
+  dk_physics P;
+  std::vector<dk_block> B;
+  std::vector<double> dx1, dx2, dx3;
+  int form = DK_FORM_FP64;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  dk::Geo g;
+  dk::DevBlock *d_blocks = nullptr;
+  int *d_slot_dir = nullptr;       // the 26 same-level fold slots in NeighborIndex order
+  // caller-owned
+  double *rdata = nullptr;
+  int32_t *idata = nullptr;
+  int64_t cap = 0, np = 0;
+  double *u0 = nullptr, *u1 = nullptr;
+  // library-owned work fields (layout of the gas, nvar components)
+  double *qdep = nullptr, *ustar = nullptr, *dmom = nullptr, *ap = nullptr, *resid = nullptr;
+  dk::ErrWord *d_err = nullptr;
+  double *d_partial = nullptr;     // reduction partials
+  double *d_scalar = nullptr;
+  // sort scratch (grown on demand)
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  unsigned long long *sort_keys = nullptr;
+  int *sort_idx = nullptr;
+  double *sort_rbuf = nullptr;
+  int32_t *sort_ibuf = nullptr;
+  int64_t sort_cap = 0;
+  // the FP32 form (dk_fp32.hip)
+  struct Fast *fast = nullptr;
+  int graphs = 0;
+  // dk_mesh.hip: ghost fill (copy, shear remap, x3 values) and additive ghost fold
+template <typename T> int FillGhostsT(dk_ctx *c, T *a, int nvar, bool velocity, double t);
+template <typename T> int FoldT(dk_ctx *c, T *a, int nvar, double t);
+
+// per-kernel event timing
+  struct Prof *prof = nullptr;
+};
+
+namespace dk {
+
+int Check(hipError_t e);
+long long NCells(const dk_ctx *c);   // nmb * ncb
+int Zero(dk_ctx *c, double *a, long long n);
+
+// dk_mesh.hip: ghost fill (copy, shear remap, x3 values) and additive ghost fold
+template <typename T> int FillGhostsT(dk_ctx *c, T *a, int nvar, bool velocity, double t);
+template <typename T> int FoldT(dk_ctx *c, T *a, int nvar, double t);
+
+// per-kernel event timing (no-ops unless enabled): Begin returns a token for End
+int ProfBegin(dk_ctx *c, const char *label);
+void ProfEnd(dk_ctx *c, int token);
+void ProfDestroy(dk_ctx *c);
+
+// dk_fp32.hip
+int FastCreate(dk_ctx *c);
+void FastDestroy(dk_ctx *c);
+int FastBind(dk_ctx *c);
+int FastExport(dk_ctx *c);
+int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);
+int FastDiag(dk_ctx *c, double out[9]);
+
+// dk_fp64.hip pieces reused by the FP32 form
+int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);
+
+}  // namespace dk
+
+#endif  // ATHENAK_AMD_CSRC_DK_INTERNAL_HPP_

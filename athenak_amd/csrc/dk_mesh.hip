@@ -1,0 +1,335 @@
+// dk_mesh.hip -- mesh-side exchanges of the drag path on one device, for FP64 and FP32 fields.
+//
+// Ghost copy of a cell field (src/bvals/bvals_cc.cpp:53-296), the physical x3 values of u*
+// (src/dust/dust_tasks.cpp:445-469), the shear-periodic x1 ghost remap
+// (src/shearing_box/shearing_box_cc.cpp:68-260, remap_fluxes.hpp, shearing_box.cpp:260-330) and
+// the additive ghost-deposit fold (src/bvals/bvals_dep.cpp:320-562 and the shear / x3 parts of
+// src/bvals/bvals_dep_general.cpp).  Every kernel is a gather: each cell it writes is written by
+// one thread, summing its inputs in a fixed order, so the results do not depend on scheduling.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "dk_internal.hpp"
+
+using dk::DevBlock;
+using dk::Dir;
+using dk::Geo;
+using dk::Idx;
+using dk::kBlock;
+
+namespace {
+
+#define GRID_STRIDE(t, n) \
+  for (long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x; t < (n); \
+       t += (long long)gridDim.x*blockDim.x)
+
+inline int Grid(long long n) {
+  long long b = (n + kBlock - 1)/kBlock;
+  if (b < 1) b = 1;
+  return static_cast<int>(std::min<long long>(b, 65536));
+}
+
+template <typename T>
+__global__ void k_copy_ghosts(Geo g, const DevBlock *B, T *a, int nvar) {
+  const long long n = g.ncb*g.nmb;
+  GRID_STRIDE(c, n) {
+    const int m = static_cast<int>(c/g.ncb);
+    long long rr = c - (long long)m*g.ncb;
+    const int i = static_cast<int>(rr % g.n1); rr /= g.n1;
+    const int j = static_cast<int>(rr % g.n2);
+    const int k = static_cast<int>(rr/g.n2);
+    const int ox1 = (i < g.is) ? -1 : ((i > g.ie) ? 1 : 0);
+    const int ox2 = (j < g.js) ? -1 : ((j > g.je) ? 1 : 0);
+    const int ox3 = (k < g.ks) ? -1 : ((k > g.ke) ? 1 : 0);
+    if (ox1 == 0 && ox2 == 0 && ox3 == 0) continue;
+    const int nb = B[m].nghbr[Dir(ox1, ox2, ox3)];
+    if (nb < 0) continue;
+    for (int v=0; v<nvar; ++v)
+      a[Idx(g,m,nvar,v,k,j,i)] = a[Idx(g,nb,nvar,v,k-ox3*g.nx3,j-ox2*g.nx2,i-ox1*g.nx1)];
+  }
+}
+
+template <typename T>
+__global__ void k_x3bcs(Geo g, const DevBlock *B, T *a, int nvar, int velocity) {
+  const long long n = (long long)g.nmb*g.ng*g.n2*g.n1;
+  const bool refl = (g.bc_x3 == DK_X3_REFLECT);
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = static_cast<int>(rr % g.n1); rr /= g.n1;
+    const int j = static_cast<int>(rr % g.n2); rr /= g.n2;
+    const int gg = static_cast<int>(rr % g.ng);
+    const int m = static_cast<int>(rr/g.ng);
+    if (B[m].nghbr[Dir(0,0,-1)] < 0) {
+      const int kg = g.ks - 1 - gg, kc = refl ? (g.ks + gg) : g.ks;
+      for (int v=0; v<nvar; ++v) {
+        const T val = a[Idx(g,m,nvar,v,kc,j,i)];
+        a[Idx(g,m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+      }
+    }
+    if (B[m].nghbr[Dir(0,0,1)] < 0) {
+      const int kg = g.ke + 1 + gg, kc = refl ? (g.ke - gg) : g.ke;
+      for (int v=0; v<nvar; ++v) {
+        const T val = a[Idx(g,m,nvar,v,kc,j,i)];
+        a[Idx(g,m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+      }
+    }
+  }
+}
+
+// PLM_RemapFlx of src/shearing_box/remap_fluxes.hpp on a periodic column u(G)
+template <class Col>
+__device__ inline double PLMRemapFlx(const Col &u, int j, double eps) {
+  if (eps > 0.0) {
+    const double dql = u(j-1) - u(j-2);
+    const double dqr = u(j  ) - u(j-1);
+    const double dq2 = dql*dqr;
+    double dqm = 2.0*dq2/(dql + dqr);
+    if (dq2 <= 0.0) dqm = 0.0;
+    return eps*(u(j-1) + 0.5*(1.0 - eps)*dqm);
+  }
+  const double dql = u(j  ) - u(j-1);
+  const double dqr = u(j+1) - u(j  );
+  const double dq2 = dql*dqr;
+  double dqm = 2.0*dq2/(dql + dqr);
+  if (dq2 <= 0.0) dqm = 0.0;
+  return eps*(u(j) - 0.5*(1.0 + eps)*dqm);
+}
+
+// the x1 ghosts of the blocks on the mesh x1 faces: the opposite face remapped by the fractional
+// offset eps = fmod(yshear, dy)/dy (negated on the outer face) and moved by joffset cells
+template <typename T>
+__global__ void k_shear_ghosts(Geo g, const DevBlock *B, T *a, int nvar, double yshear) {
+  const int nface = g.nmbx2*g.nmbx3;
+  const long long n = 2LL*nface*nvar*g.n3*g.ng*g.n2;
+  const int ny = g.nx2*g.nmbx2, nzt = g.nx3*g.nmbx3;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int j = static_cast<int>(rr % g.n2); rr /= g.n2;
+    const int gg = static_cast<int>(rr % g.ng); rr /= g.ng;
+    const int k = static_cast<int>(rr % g.n3); rr /= g.n3;
+    const int v = static_cast<int>(rr % nvar); rr /= nvar;
+    const int f = static_cast<int>(rr % nface);
+    const int side = static_cast<int>(rr/nface);
+    const int by0 = f % g.nmbx2, bz0 = f/g.nmbx2;
+    const int bx = (side == 0) ? 0 : g.nmbx1 - 1;
+    const int m = bx + g.nmbx1*(by0 + g.nmbx2*bz0);
+    const int bxs = (side == 0) ? g.nmbx1 - 1 : 0;
+    const double dy = B[m].dx2;
+    const int joffset = static_cast<int>(yshear/dy);
+    double eps = fmod(yshear, dy)/dy;
+    if (side == 1) eps *= -1.0;
+    int gk = B[m].lx3*g.nx3 + (k - g.ks);
+    if (g.bc_x3 == DK_X3_PERIODIC) gk = ((gk % nzt) + nzt) % nzt;
+    else gk = min(max(gk, 0), nzt - 1);
+    const int bz = gk/g.nx3, kl = g.ks + (gk - bz*g.nx3);
+    const int ig = (side == 0) ? (g.is - g.ng + gg) : (g.ie + 1 + gg);
+    const int isrc = (side == 0) ? (ig + g.nx1) : (ig - g.nx1);
+    auto col = [&](int G) -> double {
+      G = ((G % ny) + ny) % ny;
+      const int by = G/g.nx2;
+      const int ms = bxs + g.nmbx1*(by + g.nmbx2*bz);
+      return static_cast<double>(a[Idx(g,ms,nvar,v,kl,g.js + (G - by*g.nx2),isrc)]);
+    };
+    const int Gt = B[m].lx2*g.nx2 + (j - g.js);
+    const int G = (side == 0) ? (Gt - joffset) : (Gt + joffset);
+    a[Idx(g,m,nvar,v,k,j,ig)] =
+        static_cast<T>(col(G) - (PLMRemapFlx(col, G+1, eps) - PLMRemapFlx(col, G, eps)));
+  }
+}
+
+// physical x3 faces: the ghost layers are mirrored into the first ng active layers over the
+// whole padded plane (each thread owns one (m, v, g, j, i) pair of cells)
+template <typename T>
+__global__ void k_fold_x3(Geo g, const DevBlock *B, T *a, int nvar) {
+  const long long n = (long long)g.nmb*nvar*g.ng*g.n2*g.n1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = static_cast<int>(rr % g.n1); rr /= g.n1;
+    const int j = static_cast<int>(rr % g.n2); rr /= g.n2;
+    const int gg = static_cast<int>(rr % g.ng); rr /= g.ng;
+    const int v = static_cast<int>(rr % nvar);
+    const int m = static_cast<int>(rr/nvar);
+    if (B[m].nghbr[Dir(0,0,-1)] < 0)
+      a[Idx(g,m,nvar,v,g.ks+gg,j,i)] += a[Idx(g,m,nvar,v,g.ks-1-gg,j,i)];
+    if (B[m].nghbr[Dir(0,0,1)] < 0)
+      a[Idx(g,m,nvar,v,g.ke-gg,j,i)] += a[Idx(g,m,nvar,v,g.ke+1+gg,j,i)];
+  }
+}
+
+// same-level neighbours: every active edge cell sums the neighbours' ghost strips in
+// NeighborIndex order (src/mesh/nghbr_index.hpp:19-54); the strips that cross a shear-periodic
+// x1 face are left to k_fold_shear
+template <typename T>
+__global__ void k_fold(Geo g, const DevBlock *B, const int *slot_dir, T *a, int nvar) {
+  const long long nrow = (long long)g.nmb*g.nx3*g.nx2;
+  const int ng1 = g.ng - 1;
+  const bool shear = (g.bc_x1 == DK_X1_SHEAR);
+  GRID_STRIDE(t, nrow) {
+    long long rr = t;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const bool edge = (k < g.ks + g.ng) || (k > g.ke - g.ng) || (j < g.js + g.ng) ||
+                      (j > g.je - g.ng);
+    for (int i=g.is; i<=g.ie; ++i) {
+      if (!edge && i >= g.is + g.ng && i <= g.ie - g.ng) continue;
+      for (int v=0; v<nvar; ++v) {
+        T sum = a[Idx(g,m,nvar,v,k,j,i)];
+        for (int s=0; s<26; ++s) {
+          const int d = slot_dir[s];
+          const int ox1 = d % 3 - 1, ox2 = (d/3) % 3 - 1, ox3 = d/9 - 1;
+          const int nb = B[m].nghbr[d];
+          if (nb < 0) continue;
+          if (shear && ox1 != 0) {
+            const bool crosses = (ox1 < 0) ? (B[m].lx1 == 0) : (B[m].lx1 == g.nmbx1 - 1);
+            if (crosses) continue;
+          }
+          const int il = (ox1 > 0) ? (g.ie - ng1) : g.is, iu = (ox1 < 0) ? (g.is + ng1) : g.ie;
+          const int jl = (ox2 > 0) ? (g.je - ng1) : g.js, ju = (ox2 < 0) ? (g.js + ng1) : g.je;
+          const int kl = (ox3 > 0) ? (g.ke - ng1) : g.ks, ku = (ox3 < 0) ? (g.ks + ng1) : g.ke;
+          if (i < il || i > iu || j < jl || j > ju || k < kl || k > ku) continue;
+          sum += a[Idx(g,nb,nvar,v,k-ox3*g.nx3,j-ox2*g.nx2,i-ox1*g.nx1)];
+        }
+        a[Idx(g,m,nvar,v,k,j,i)] = sum;
+      }
+    }
+  }
+}
+
+// The cells of a padded block field that hold global row r (0 <= r < nx*nb) along one axis:
+// the active row of block r/nx and the ghost copies in the adjacent blocks (periodic wrap, or
+// none beyond a physical face).  Returns the count; blk/loc receive block row and padded index.
+__device__ inline int RowCopies(int r, int nx, int nb, int ng, int start, bool periodic,
+                                int blk[3], int loc[3]) {
+  int n = 0;
+  const int b = r/nx, l = r - b*nx;
+  blk[n] = b; loc[n] = start + l; ++n;
+  if (l < ng) {                 // the upper ghost rows of block b - 1
+    int b2 = b - 1;
+    if (b2 < 0) b2 = periodic ? b2 + nb : -1;
+    if (b2 >= 0) {blk[n] = b2; loc[n] = start + nx + l; ++n;}
+  }
+  if (l >= nx - ng) {           // the lower ghost rows of block b + 1
+    int b2 = b + 1;
+    if (b2 >= nb) b2 = periodic ? b2 - nb : -1;
+    if (b2 >= 0) {blk[n] = b2; loc[n] = start - (nx - l); ++n;}
+  }
+  return n;
+}
+
+// The shear part as a gather: a ghost cell beyond the outer x1 face deposits into the inner
+// face at x2 shifted by +ysh, beyond the inner face into the outer face at -ysh, split between
+// the two cells its shifted interval overlaps (the piecewise-constant remap of
+// src/bvals/bvals_dep_general.cpp:100-160).  Every target cell (thread) enumerates the source
+// ghost cells of the opposite face (x2 and x3 ghost copies included) whose shifted interval can
+// touch it and adds their shares in a fixed order; each share is computed from the source's
+// own row index exactly as a scatter would.
+template <typename T>
+__global__ void k_fold_shear(Geo g, const DevBlock *B, T *a, int nvar, double shc) {
+  const int nface = g.nmbx2*g.nmbx3;
+  const int ny = g.nx2*g.nmbx2;
+  const long long n = 2LL*nface*g.nx3*g.nx2*g.ng;
+  const bool perz = (g.bc_x3 == DK_X3_PERIODIC);
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int gg = static_cast<int>(rr % g.ng); rr /= g.ng;
+    const int jt = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int kt = g.ks + static_cast<int>(rr % g.nx3); rr /= g.nx3;
+    const int f = static_cast<int>(rr % nface);
+    const int tside = static_cast<int>(rr/nface);   // 0: target on the inner face
+    const int byt = f % g.nmbx2, bzt = f/g.nmbx2;
+    const int mt = ((tside == 0) ? 0 : g.nmbx1 - 1) + g.nmbx1*(byt + g.nmbx2*bzt);
+    const int it = (tside == 0) ? (g.is + gg) : (g.ie - gg);
+    // the sources sit in the ghost column beyond the opposite face at the same depth gg
+    const int sside = 1 - tside;
+    const int bxs = (sside == 0) ? 0 : g.nmbx1 - 1;
+    const int isrc = (sside == 0) ? (g.is - 1 - gg) : (g.ie + 1 + gg);
+    const double sh = (sside == 0) ? -shc : shc;
+    const int GJt = byt*g.nx2 + (jt - g.js);
+    const int GKt = bzt*g.nx3 + (kt - g.ks);
+    int kb[3], kl[3];
+    const int nk = RowCopies(GKt, g.nx3, g.nmbx3, g.ng, g.ks, perz, kb, kl);
+    const int nd = (ny < 5) ? ny : 5;
+    const int base = GJt - static_cast<int>(floor(sh)) - 2;
+    for (int v=0; v<nvar; ++v) {
+      T sum = a[Idx(g,mt,nvar,v,kt,jt,it)];
+      for (int q=0; q<nk; ++q) {
+        for (int d=0; d<nd; ++d) {
+          const int r = (((base + d) % ny) + ny) % ny;
+          int jb[3], jl[3];
+          const int nj = RowCopies(r, g.nx2, g.nmbx2, g.ng, g.js, true, jb, jl);
+          for (int p=0; p<nj; ++p) {
+            const int ms = bxs + g.nmbx1*(jb[p] + g.nmbx2*kb[q]);
+            const int gj = jb[p]*g.nx2 + (jl[p] - g.js);    // the source's unwrapped row
+            const double lo = gj + sh;
+            const int c0 = static_cast<int>(floor(lo));
+            const double fr = lo - c0;
+            const T val = a[Idx(g,ms,nvar,v,kl[q],jl[p],isrc)];
+            if ((((c0 % ny) + ny) % ny) == GJt && (1.0 - fr) > 0.0)
+              sum += static_cast<T>(1.0 - fr)*val;
+            if (((((c0 + 1) % ny) + ny) % ny) == GJt && fr > 0.0)
+              sum += static_cast<T>(fr)*val;
+          }
+        }
+      }
+      a[Idx(g,mt,nvar,v,kt,jt,it)] = sum;
+    }
+  }
+}
+
+}  // namespace
+
+namespace dk {
+
+template <typename T>
+int FillGhostsT(dk_ctx *c, T *a, int nvar, bool velocity, double t) {
+  const Geo &g = c->g;
+  {const int tk = ProfBegin(c, "copy_ghosts");
+   k_copy_ghosts<T><<<Grid(NCells(c)), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar);
+   ProfEnd(c, tk);}
+  if (c->M.bc_x1 == DK_X1_SHEAR) {
+    const double yshear = (c->P.qshear*c->P.omega0)*(c->M.x1max - c->M.x1min)*t;
+    const long long n = 2LL*g.nmbx2*g.nmbx3*nvar*g.n3*g.ng*g.n2;
+    const int tk = ProfBegin(c, "shear_ghosts");
+    k_shear_ghosts<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar, yshear);
+    ProfEnd(c, tk);
+  }
+  if (c->M.bc_x3 != DK_X3_PERIODIC) {
+    const long long n = (long long)g.nmb*g.ng*g.n2*g.n1;
+    k_x3bcs<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar, velocity ? 1 : 0);
+  }
+  return Check(hipGetLastError());
+}
+
+template <typename T>
+int FoldT(dk_ctx *c, T *a, int nvar, double t) {
+  const Geo &g = c->g;
+  if (c->M.bc_x3 != DK_X3_PERIODIC && !c->M.x3_dep_drop) {
+    const long long n = (long long)g.nmb*nvar*g.ng*g.n2*g.n1;
+    k_fold_x3<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar);
+  }
+  {const int tk = ProfBegin(c, "fold");
+   k_fold<T><<<Grid((long long)g.nmb*g.nx3*g.nx2), kBlock, 0, c->stream>>>(
+       g, c->d_blocks, c->d_slot_dir, a, nvar);
+   ProfEnd(c, tk);}
+  if (c->M.bc_x1 == DK_X1_SHEAR) {
+    const double lx = c->M.x1max - c->M.x1min, ly = c->M.x2max - c->M.x2min;
+    const double ysh = std::fmod(c->P.qshear*c->P.omega0*lx*t, ly);
+    const double dy = ly/(g.nx2*g.nmbx2);
+    const long long n = 2LL*g.nmbx2*g.nmbx3*g.nx3*g.nx2*g.ng;
+    const int tk = ProfBegin(c, "fold_shear");
+    k_fold_shear<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar, ysh/dy);
+    ProfEnd(c, tk);
+  }
+  return Check(hipGetLastError());
+}
+
+template int FillGhostsT<double>(dk_ctx *, double *, int, bool, double);
+template int FillGhostsT<float>(dk_ctx *, float *, int, bool, double);
+template int FoldT<double>(dk_ctx *, double *, int, double);
+template int FoldT<float>(dk_ctx *, float *, int, double);
+
+}  // namespace dk

@@ -1,0 +1,255 @@
+"""Python handle on libdustk.so (the gfx950 drag-coupling library) through its C-ABI.
+
+torch supplies device memory and the stream; every computation runs in the library's HIP
+kernels.  There is no fallback: without the built library or a HIP device this module raises.
+"""
+
+import ctypes as C
+import os
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdustk.so")
+_LIB = None
+
+TASKS = ("first_two_imp_rk", "explicit_push", "update_particle_gids", "deposit_drag",
+         "fold_deposit_qp", "gas_implicit_solve", "fill_ustar", "gather_kick_pmbr",
+         "fold_pmbr", "apply_pmbr", "add_drag_history_gas", "fill_gas_ghosts")
+
+EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
+            "dk_export_particles", "dk_npart", "dk_stage_run", "dk_set_graphs",
+            "dk_gas_hook", "dk_sort_particles", "dk_new_time_step", "dk_diag_sums",
+            "dk_error", "dk_field", "dk_strerror", "dk_setup_problem", "dk_prof_enable",
+            "dk_prof_reset", "dk_prof_read"] +
+           ["dk_" + t for t in TASKS])
+
+
+class DustError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise DustError("libdustk.so is not built (python -c 'import __graft_entry__ as g; "
+                            "g.build()')")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.dk_create.argtypes = [C.POINTER(abi.dk_mesh), C.POINTER(abi.dk_block),
+                                C.POINTER(abi.dk_physics), C.c_int32, P, C.POINTER(P)]
+        L.dk_destroy.argtypes = [P]
+        L.dk_bind_particles.argtypes = [P, P, P, C.c_int64, C.c_int64]
+        L.dk_bind_gas.argtypes = [P, P, P]
+        L.dk_export_particles.argtypes = [P]
+        L.dk_npart.restype = C.c_int64
+        L.dk_npart.argtypes = [P]
+        st = C.POINTER(abi.dk_stage)
+        for t in TASKS:
+            getattr(L, "dk_" + t).argtypes = [P, st]
+        L.dk_stage_run.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
+        L.dk_gas_hook.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
+        L.dk_set_graphs.argtypes = [P, C.c_int32]
+        L.dk_sort_particles.argtypes = [P]
+        L.dk_new_time_step.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
+        L.dk_diag_sums.argtypes = [P, C.POINTER(C.c_double)]
+        L.dk_error.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        L.dk_field.restype = P
+        L.dk_field.argtypes = [P, C.c_int32, C.POINTER(C.c_int32)]
+        L.dk_strerror.restype = C.c_char_p
+        L.dk_strerror.argtypes = [C.c_int]
+        L.dk_setup_problem.argtypes = [P, C.POINTER(abi.dk_setup)]
+        L.dk_prof_enable.argtypes = [P, C.c_int32]
+        L.dk_prof_reset.argtypes = [P]
+        L.dk_prof_read.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        _LIB = L
+    return _LIB
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise DustError("no HIP device: the drag library runs on the GPU only")
+    return torch
+
+
+class DustContext:
+    """A library context with its particle SoA and gas registers in device memory (torch
+    tensors owned here, as the reference's Particles and Hydro own theirs)."""
+
+    def __init__(self, mesh, blocks, phys, capacity, form=abi.DK_FORM_FP64, stream=None):
+        torch = _torch()
+        L = lib()
+        self.torch = torch
+        self.mesh, self.blocks, self.phys, self.form = mesh, blocks, phys, form
+        self.cap = int(capacity)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        self.rdata = torch.zeros((abi.DK_NRDATA, self.cap), dtype=torch.float64, device=dev)
+        self.idata = torch.zeros((abi.DK_NIDATA, self.cap), dtype=torch.int32, device=dev)
+        shape = (mesh.nmb, abi.DK_NGAS, mesh.nx3 + 2 * mesh.ng, mesh.nx2 + 2 * mesh.ng,
+                 mesh.nx1 + 2 * mesh.ng)
+        self.u0 = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.u1 = torch.zeros(shape, dtype=torch.float64, device=dev)
+        h = C.c_void_p()
+        e = L.dk_create(C.byref(mesh), blocks, C.byref(phys), int(form),
+                        C.c_void_p(self.stream.cuda_stream), C.byref(h))
+        self._check(e, "dk_create")
+        self.h = h
+        self._check(L.dk_bind_gas(h, C.c_void_p(self.u0.data_ptr()),
+                                  C.c_void_p(self.u1.data_ptr())), "dk_bind_gas")
+        self.time = 0.0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _check(e, what):
+        if e:
+            raise DustError("%s: %s" % (what, lib().dk_strerror(e).decode()))
+
+    def check_device_error(self):
+        code, part = C.c_int32(0), C.c_int64(-1)
+        self._check(lib().dk_error(self.h, C.byref(code), C.byref(part)), "dk_error")
+        if code.value:
+            raise DustError("device error: %s (particle %d)"
+                            % (lib().dk_strerror(code.value).decode(), part.value))
+
+    # ---- particles ----
+    def bind_particles(self, npart):
+        self.npart = int(npart)
+        self._check(lib().dk_bind_particles(self.h, C.c_void_p(self.rdata.data_ptr()),
+                                            C.c_void_p(self.idata.data_ptr()), self.cap,
+                                            self.npart), "dk_bind_particles")
+
+    def load(self, rdata, idata, u0, u1=None):
+        """Copies host arrays (numpy, the oracle's layout) in and binds them."""
+        torch = self.torch
+        n = rdata.shape[1]
+        self.rdata.zero_()
+        self.idata.zero_()
+        self.rdata[:, :n] = torch.from_numpy(rdata).to(self.rdata.device)
+        self.idata[:, :n] = torch.from_numpy(idata).to(self.idata.device)
+        self.u0.copy_(torch.from_numpy(u0))
+        self.u1.copy_(torch.from_numpy(u1 if u1 is not None else u0))
+        self.bind_particles(n)
+
+    def setup(self, su):
+        self._check(lib().dk_bind_particles(self.h, C.c_void_p(self.rdata.data_ptr()),
+                                            C.c_void_p(self.idata.data_ptr()), self.cap, 0),
+                    "dk_bind_particles")
+        self._check(lib().dk_setup_problem(self.h, C.byref(su)), "dk_setup_problem")
+        self.npart = lib().dk_npart(self.h)
+
+    def export(self):
+        self._check(lib().dk_export_particles(self.h), "dk_export_particles")
+
+    # ---- stepping ----
+    def task(self, name, st):
+        self._check(getattr(lib(), "dk_" + name)(self.h, C.byref(st)), name)
+
+    def stage(self, st, src=None):
+        self._check(lib().dk_stage_run(self.h, C.byref(st),
+                                       C.byref(src) if src is not None else None),
+                    "stage %d" % st.stage)
+
+    def stages(self, dt):
+        return (abi.imex2p_stages(self.time, dt) if self.phys.coupling == abi.DK_IMEX
+                else abi.rk2_stages(self.time, dt))
+
+    def step(self, dt, src=None):
+        for st in self.stages(dt):
+            self.stage(st, src)
+        self.time += dt
+
+    def set_graphs(self, on):
+        self._check(lib().dk_set_graphs(self.h, 1 if on else 0), "dk_set_graphs")
+
+    def sort(self):
+        self._check(lib().dk_sort_particles(self.h), "dk_sort_particles")
+
+    def new_time_step(self, dt_cfl):
+        out = C.c_double(0.0)
+        self._check(lib().dk_new_time_step(self.h, dt_cfl, C.byref(out)), "dk_new_time_step")
+        return out.value
+
+    def diag(self):
+        import numpy as np
+        out = (C.c_double * 9)()
+        self._check(lib().dk_diag_sums(self.h, out), "dk_diag_sums")
+        return np.array(out[:])
+
+    def prof(self, on=True):
+        self._check(lib().dk_prof_enable(self.h, 1 if on else 0), "dk_prof_enable")
+
+    def prof_reset(self):
+        self._check(lib().dk_prof_reset(self.h), "dk_prof_reset")
+
+    def prof_read(self):
+        """{label: (total ms, launches)} of the kernels timed since the last reset."""
+        out = {}
+        k = 0
+        buf = C.create_string_buffer(64)
+        while True:
+            ms, cnt = C.c_double(0), C.c_int64(0)
+            if lib().dk_prof_read(self.h, k, buf, 64, C.byref(ms), C.byref(cnt)):
+                break
+            out[buf.value.decode()] = (ms.value, cnt.value)
+            k += 1
+        return out
+
+    def field(self, which):
+        """A host copy (numpy) of work field `which` (0 qdep, 1 ustar, 2 dmom, 3 ap,
+        4 resid) in the gas layout."""
+        import numpy as np
+        nv = C.c_int32(0)
+        p = lib().dk_field(self.h, int(which), C.byref(nv))
+        if not p:
+            return None
+        m = self.mesh
+        shape = (m.nmb, nv.value, m.nx3 + 2 * m.ng, m.nx2 + 2 * m.ng, m.nx1 + 2 * m.ng)
+        host = np.empty(shape, dtype=np.float64)
+        self.torch.cuda.synchronize()
+        _copy_raw(host.ctypes.data, p, host.nbytes, kind=2)
+        return host
+
+
+def _copy_raw(dst, src, nbytes, kind=3):
+    """hipMemcpy of raw pointers (kind 2: device to host, 3: device to device) through the
+    HIP runtime that torch loads."""
+    import ctypes
+    hip = _hip()
+    r = hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(nbytes),
+                      kind)
+    if r != 0:
+        raise DustError("hipMemcpy failed (%d)" % r)
+
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _HIP = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _HIP is None:
+            raise DustError("libamdhip64.so not found")
+        _HIP.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                   ctypes.c_int]
+    return _HIP

@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Throughput of the dust drag coupling on one MI355X.
+
+A step is one imex2+ cycle (two working stages of push, deposit, dc1 solve, gather-kick and
+back-reaction apply, plus the harness's gas source-term update) over every particle of the
+workload, with all inputs resident in HBM.  The default workload is the unstratified shearing
+box of 256^3 cells in 64 blocks of 64^3 with 4 particles per cell (67,108,864 particles), St 0.1,
+eps 3, dc1.
+
+Prints one JSON line: value = Mparticle-steps/s of the measured form, the roofline of the
+dominant kernel (algorithmic bytes per launch / measured average launch time, HIP events on the
+library's stream), the whole-step roofline (algorithmic bytes per particle-step), and the CPU
+baseline (the oracle's OpenMP restatement of the reference kernels on the host cores, on a
+bounded sample of the same workload).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of the minimal-pass schedule per particle-step (ppc = particles per cell):
+# imex2+/dc1 point TSC, FP64: 648 + 1176/ppc; FP32 form (positions and u0 in FP64): 456 + 732/ppc
+def balg_step(form, ppc):
+    if form == "fp64":
+        return 648.0 + 1176.0 / ppc
+    return 456.0 + 732.0 / ppc
+
+
+# Algorithmic bytes of one launch of a kernel: (bytes per particle, bytes per active cell).
+# Each particle column or mesh value the kernel must read or write counts once.
+KERNEL_BYTES = {
+    # FP32 form (positions 8 B, other particle values 4 B; averaged over the two stages):
+    # P1 stage 1 reads x, v, ts, m and writes x, v, x1, v1; stage 2 reads x, v, x1, v1, R, ts, m
+    # and writes x, v; Q, P written per cell.  P2 reads x, ts, m; u* read, result written.
+    # P3 reads x, v, ts, m and writes v (and R at stage 1); u* read, D (FP64) written.
+    "p1_push_deposit": ((116 + 128) / 2.0, 4 * 4),
+    "p2_operator": (32, 6 * 4),
+    "p3_gather_kick": ((68 + 56) / 2.0, 3 * 4 + 3 * 8),
+    # FP64 form
+    "push_imex": (15 * 8 + 6 * 8, 0),          # x, v, x1, v1, R read; x, v written
+    "scatter_imex": (8 * 8 + 4, 4 * 8),        # x, v, ts, m, gid read; Q, P written
+    "applya_gtsg": (5 * 8 + 4, 6 * 8),         # x, ts, m, gid; field read, result written
+    "gather_kick": (8 * 8 + 4 + 6 * 8, 6 * 8),  # x, v, ts, m, gid; v, R written; u*, dmom
+    "consistent_dust": (5 * 8 + 4 + 12 * 8, 4 * 8),
+    "center_mass": (5 * 8 + 4, 8),
+    "part_update": (3 * 8 + 4, 0),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--form", default="both", choices=["fp64", "fp32", "both"])
+    ap.add_argument("--n", type=int, default=None, help="override the mesh size (tests)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def make_workload(name, n=None):
+    from athenak_amd import workloads
+    f = workloads.WORKLOADS[name]
+    return f() if n is None else f(n=n)
+
+
+def describe(w):
+    m = w.mesh
+    return {"workload": "%s: %dx%dx%d cells in %d blocks of %dx%dx%d, ppc %d, %d particles, "
+                        "imex2+/%s, dt %g" % (
+                            w.name, m.nx1 * m.nmbx1, m.nx2 * m.nmbx2, m.nx3 * m.nmbx3, m.nmb,
+                            m.nx1, m.nx2, m.nx3, w.ppc, w.npart,
+                            {0: "local", 1: "applya", 2: "dc1", 3: "dc2"}[w.phys.solver], w.dt),
+            "particles": w.npart, "cells": w.ncells, "ppc": w.ppc, "blocks": m.nmb,
+            "shear_periodic_x1": bool(m.bc_x1), "solver": "dc1"}
+
+
+def gpu_run(w, form, steps, warmup):
+    import torch
+    from athenak_amd import abi
+    from athenak_amd.lib import DustContext
+    fid = abi.DK_FORM_FP64 if form == "fp64" else abi.DK_FORM_FP32
+    ctx = DustContext(w.mesh, w.blocks, w.phys, w.npart, form=fid)
+    ctx.setup(w.setup)
+    torch.cuda.synchronize()
+    d0 = ctx.diag()
+    for _ in range(warmup):
+        ctx.step(w.dt, w.src)
+    ctx.check_device_error()
+    torch.cuda.synchronize()
+    ctx.prof(True)
+    ctx.prof_reset()
+    stream = ctx.stream
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        ctx.step(w.dt, w.src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    prof = ctx.prof_read()
+    ctx.prof(False)
+    ctx.check_device_error()
+    d1 = ctx.diag()
+    drift = float(abs((d1[0:3] + d1[3:6]) - (d0[0:3] + d0[3:6])).max() / d0[8])
+    ctx.export()
+    n = ctx.npart
+    order = torch.argsort(ctx.idata[abi.PTAG, :n])
+    vel = ctx.rdata[[abi.IPVX, abi.IPVY, abi.IPVZ], :n][:, order].clone()
+    ctx.close()
+    del ctx
+    torch.cuda.empty_cache()
+    return {"ms": ms, "wall_s": wall, "prof": prof, "drift": drift, "vel": vel}
+
+
+def cpu_baseline(name, budget_s):
+    """The oracle (OpenMP, host cores) on the same physics and block size on a smaller mesh:
+    one block of the workload's block size (the whole workload for c1)."""
+    from athenak_amd import workloads
+    from oracle.pyoracle import Oracle
+    full = make_workload(name)
+    nx = full.mesh.nx1
+    w = workloads.WORKLOADS[name](n=nx, nbx=1) if name != "c1" else full
+    cores = min(16, os.cpu_count() or 1)
+    o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16, threads=cores)
+    o.setup(w.setup)
+    o.step(w.dt, w.src)          # warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(w.dt, w.src)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 20:
+            break
+    rate = w.npart * steps / el / 1e6
+    return {"value": round(rate, 4), "unit": "Mparticle-steps/s", "cores": cores, "kind": "port",
+            "sample": "CPU FP64 oracle (reference kernels restated, OpenMP atomics) on %s physics "
+                      "with %d blocks of %d^3 (%d particles), %d steps after 1 warm-up, %.1f s"
+                      % (name, w.mesh.nmb, nx, w.npart, steps, el)}
+
+
+def kernel_roofline(r, w):
+    label, (tot, cnt) = max(r["prof"].items(), key=lambda kv: kv[1][0])
+    per_p, per_c = KERNEL_BYTES.get(label, (0, 0))
+    alg = per_p * w.npart + per_c * w.ncells
+    avg_s = tot / cnt * 1e-3
+    achieved = alg / avg_s / 1e9 if alg else None
+    return {"bound": "hbm", "kernel": label,
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+            "avg_launch_ms": round(tot / cnt, 4), "alg_bytes_per_launch": alg,
+            "share_of_step": round(tot / r["ms"], 3)}
+
+
+def rate(r, w, steps):
+    ms_step = r["ms"] / steps
+    return ms_step, w.npart / (ms_step * 1e-3) / 1e6
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has a single MI355X, so --gpus %d is "
+              "not run (the path would shard by MeshBlock; see DESIGN.md)" % args.gpus)
+        sys.exit(2)
+    w = make_workload(args.workload, args.n)
+    forms = ["fp64", "fp32"] if args.form == "both" else [args.form]
+    res = {}
+    for f in forms:
+        print("[bench] %s form: %s, %d warm-up + %d timed steps" % (f, w.name, args.warmup,
+                                                                     args.steps),
+              file=sys.stderr, flush=True)
+        res[f] = gpu_run(w, f, args.steps, args.warmup)
+    main_form = forms[-1]
+    r = res[main_form]
+    ms_step, mps = rate(r, w, args.steps)
+    step_alg = balg_step(main_form, w.ppc)
+    out = {
+        "metric": "Mparticle-steps/s & %HBM peak, FP32+batched vs FP64 GPU at 1e-5 rel err",
+        "value": round(mps, 3), "unit": "Mparticle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if main_form == "fp64" else "f32 (f64 positions, gas momentum, sums)",
+        "data": "synthetic (seeded hash draws, generated on the device)",
+        "config": dict(describe(w), form=main_form),
+        "roofline": kernel_roofline(r, w),
+        "step_roofline": {"alg_bytes_per_particle_step": step_alg,
+                          "achieved": round(mps * 1e6 * step_alg / 1e9, 1), "unit": "GB/s",
+                          "frac": round(mps * 1e6 * step_alg / 1e9 / HBM_PEAK_GBS, 4)},
+        "momentum_drift": r["drift"],
+        "kernels_ms": {k: round(v[0], 3) for k, v in sorted(r["prof"].items(),
+                                                             key=lambda kv: -kv[1][0])},
+    }
+    if "fp64" in res and "fp32" in res:
+        ms64, mps64 = rate(res["fp64"], w, args.steps)
+        v64, v32 = res["fp64"]["vel"], res["fp32"]["vel"]
+        verr = float((v32 - v64).abs().max() / v64.abs().max())
+        out["fp64_form"] = {"value": round(mps64, 3), "ms_per_step": round(ms64, 3),
+                            "step_roofline_frac": round(
+                                mps64 * 1e6 * balg_step("fp64", w.ppc) / 1e9 / HBM_PEAK_GBS, 4),
+                            "roofline": kernel_roofline(res["fp64"], w),
+                            "momentum_drift": res["fp64"]["drift"]}
+        out["speedup_vs_fp64"] = round(mps / mps64, 3)
+        out["max_rel_velocity_err_vs_fp64"] = verr
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

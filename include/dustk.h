@@ -217,6 +217,14 @@ int dk_fill_gas_ghosts(dk_ctx *ctx, const dk_stage *st);
  * out[7] gas mass sum V rho, out[8] sum m |v| + sum V |M| (the conservation scale).
  * Cf. DustGasDrag::ParticleHistory, src/dust/dust_science.cpp:391-484. */
 int dk_diag_sums(dk_ctx *ctx, double out[9]);
+/* Per-kernel timing with HIP events recorded around the library's launches on the context's
+ * stream (the counterpart of the reference's fenced dust ProfTimer, src/dust/dust.hpp:902-954,
+ * without fences).  dk_prof_read returns entry idx (label, total milliseconds, launches) and
+ * DK_EINVAL past the last entry; it waits for the stream. */
+int dk_prof_enable(dk_ctx *ctx, int32_t on);
+int dk_prof_reset(dk_ctx *ctx);
+int dk_prof_read(dk_ctx *ctx, int32_t idx, char *label, int32_t label_len, double *ms,
+                 int64_t *count);
 /* The first device-side error (code, particle index); clears it. */
 int dk_error(dk_ctx *ctx, int32_t *code, int64_t *particle);
 /* Device pointer to a work field: 0 qdep(5), 1 ustar(3), 2 dmom(3), 3 solver ap(3),

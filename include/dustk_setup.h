@@ -68,6 +68,13 @@ typedef struct dk_setup {
 #define DK_SALT_TAU 0xcbbb9d5dc1059ed8ULL
 #define DK_SALT_CLUMP 0x629a292a367cd507ULL
 
+struct dk_ctx;
+
+/* Fills the particle and gas arrays bound to ctx (dk_bind_particles, dk_bind_gas) with the
+ * problem of su on the device and sets the particle count (capacity must hold
+ * ppc * nmb * nx1*nx2*nx3); u0 ghosts are filled and u1 = u0. */
+int dk_setup_problem(struct dk_ctx *ctx, const dk_setup *su);
+
 #ifdef __cplusplus
 }
 #endif

@@ -257,11 +257,6 @@ double ShearOffset(const or_sim *s, double t) {
   return std::fmod(s->P.qshear*s->P.omega0*lx*t, ly);
 }
 
-// Shear-periodic x1 ghost fill of a 3-component velocity field (the u* exchange of the 3-D
-// shearing box, src/shearing_box/shearing_box_cc.cpp:68-260 with PLM remap fluxes,
-// src/shearing_box/remap_fluxes.hpp).  The ghost cells of the blocks on the mesh x1 faces
-// take the values of the opposite face shifted in x2 by the offset at time t, conservatively
-// remapped with piecewise-linear (van Leer limited) reconstruction along x2.
 void ShearGhostsX1(or_sim *s, std::vector<double> &a, int nvar, double t);
 
 // Additive fold of the ghost deposits (src/bvals/bvals_dep.cpp:320-562): every block sums the
@@ -320,79 +315,73 @@ void Fold(or_sim *s, std::vector<double> &a, int nvar, double t) {
 //========================================================================================
 // shear-periodic x1 pieces
 
+
 namespace {
 
-// van Leer limited slope (the PLM of src/shearing_box/remap_fluxes.hpp)
-inline double VLSlope(double ql, double qc, double qr) {
-  const double dl = qc - ql, dr = qr - qc;
-  const double d2 = dl*dr;
-  return (d2 > 0.0) ? 2.0*d2/(dl + dr) : 0.0;
+// PLM remap "flux" through face j of a column u shifted by eps cells
+// (src/shearing_box/remap_fluxes.hpp, PLM_RemapFlx)
+template <class U>
+inline double PLMRemapFlx(const U &u, int j, double eps) {
+  if (eps > 0.0) {
+    const double dql = u(j-1) - u(j-2);
+    const double dqr = u(j  ) - u(j-1);
+    const double dq2 = dql*dqr;
+    double dqm = 2.0*dq2/(dql + dqr);
+    if (dq2 <= 0.0) dqm = 0.0;
+    return eps*(u(j-1) + 0.5*(1.0 - eps)*dqm);
+  }
+  const double dql = u(j  ) - u(j-1);
+  const double dqr = u(j+1) - u(j  );
+  const double dq2 = dql*dqr;
+  double dqm = 2.0*dq2/(dql + dqr);
+  if (dq2 <= 0.0) dqm = 0.0;
+  return eps*(u(j) - 0.5*(1.0 + eps)*dqm);
 }
 
-}  // namespace
-
-namespace {
-
-// The value of a global x2 column (all blocks of one x1 face row, active cells only) of
-// component v at global x2 index gj (periodic), for given block-x3 row and local k, i.
-struct ShearColumn {
-  or_sim *s;
-  const std::vector<double> *a;
-  int nvar, v, bx3, k, i, face;   // face: 0 inner x1 row of blocks, 1 outer
-  int ny;                         // global active cells along x2
-  double operator()(int gj) const {
-    gj = ((gj % ny) + ny) % ny;
-    const int by = gj / s->nx2, jl = gj - by*s->nx2;
-    const int bx = (face == 0) ? 0 : s->M.nmbx1 - 1;
-    const int m = bx + s->M.nmbx1*(by + s->M.nmbx2*bx3);
-    return (*a)[s->Idx(m, nvar, v, k, s->js + jl, i)];
-  }
-};
-
-}  // namespace
-
-namespace {
-
+// Shear-periodic x1 ghost fill of a cell field after the periodic copy
+// (ShearingBoxCC::PackAndSend/RecvAndUnpack, src/shearing_box/shearing_box_cc.cpp:68-260, and
+// ShearingBox::Pieces, src/shearing_box/shearing_box.cpp:260-330): the x1 ghost column of a
+// block on a mesh x1 face, which the copy filled with the opposite face at the same x2, is
+// remapped by the fractional offset eps = fmod(yshear, dy)/dy (negated on the outer face) with
+// the PLM fluxes and moved by joffset = int(yshear/dy) cells: inner ghost at global x2 cell G
+// takes the remapped value of cell G - joffset, outer ghost the one of G + joffset;
+// yshear = q Omega Lx t (src/shearing_box/shearing_box_tasks.cpp:143-147).  All k rows,
+// x3 ghosts included (ShearingBox::Set, shearing_box.cpp:236-250).
 void ShearGhostsX1(or_sim *s, std::vector<double> &a, int nvar, double t) {
-  // The x1 ghosts of a block on the inner mesh face hold the cells of the outer face shifted
-  // by +ysh in x2 (they are the cells a particle reaches when it crosses inward, whose y
-  // is moved by -ysh: bvals_part.cpp:274-283), and vice versa.
-  const int ny = s->nx2*s->M.nmbx2;
-  const double dy = (s->M.x2max - s->M.x2min)/ny;
-  const double ysh = ShearOffset(s, t);
-  const double shc = ysh/dy;                 // offset in cells
-  const int ish = static_cast<int>(std::floor(shc));
-  const double f = shc - ish;                // fractional part in [0,1)
+  const int ny = s->nx2*s->M.nmbx2, nzt = s->nx3*s->M.nmbx3;
+  const double lx = s->M.x1max - s->M.x1min;
+  const double yshear = (s->P.qshear*s->P.omega0)*lx*t;
+  std::vector<double> col(ny);
   for (int m=0; m<s->M.nmb; ++m) {
     const dk_block &b = s->B[m];
     for (int side=0; side<2; ++side) {
       const bool on_face = (side == 0) ? (b.lx1 == 0) : (b.lx1 == s->M.nmbx1 - 1);
       if (!on_face) continue;
-      // inner face: the source is the outer face, shifted by +ysh; outer: inner face, -ysh
-      const double sgn = (side == 0) ? 1.0 : -1.0;
+      const int bxs = (side == 0) ? s->M.nmbx1 - 1 : 0;   // the opposite face
+      const double dy = s->dx2[m];
+      const int joffset = static_cast<int>(yshear/dy);
+      double eps = std::fmod(yshear, dy)/dy;
+      if (side == 1) eps *= -1.0;
       for (int v=0; v<nvar; ++v) {
         for (int k=0; k<s->n3; ++k) {
-          const int kk = std::min(std::max(k, s->ks), s->ke);   // x3 ghosts: from the copy
+          int gk = b.lx3*s->nx3 + (k - s->ks);
+          if (s->M.bc_x3 == DK_X3_PERIODIC) gk = ((gk % nzt) + nzt) % nzt;
+          else gk = std::min(std::max(gk, 0), nzt - 1);   // replaced by the x3 values later
+          const int bz = gk/s->nx3, kl = s->ks + (gk - bz*s->nx3);
           for (int g=0; g<s->ng; ++g) {
-            const int ig = (side == 0) ? (s->is - 1 - g) : (s->ie + 1 + g);
-            const int isrc = (side == 0) ? (s->ie - g) : (s->is + g);
-            ShearColumn col{s, &a, nvar, v, b.lx3, kk, isrc, 1 - side, ny};
+            const int ig = (side == 0) ? (s->is - s->ng + g) : (s->ie + 1 + g);
+            const int isrc = (side == 0) ? (ig + s->nx1) : (ig - s->nx1);
+            for (int G=0; G<ny; ++G) {
+              const int by = G/s->nx2;
+              const int ms = bxs + s->M.nmbx1*(by + s->M.nmbx2*bz);
+              col[G] = a[s->Idx(ms,nvar,v,kl,s->js + (G - by*s->nx2),isrc)];
+            }
+            auto u = [&](int G) {return col[((G % ny) + ny) % ny];};
             for (int j=0; j<s->n2; ++j) {
-              const int gj = b.lx2*s->nx2 + (j - s->js);
-              // the ghost cell [gj, gj+1) maps to [gj + sgn*shc, gj + 1 + sgn*shc) of the
-              // source column: the average of the PLM reconstruction over that interval
-              const double lo = gj + sgn*shc;
-              const int c0 = static_cast<int>(std::floor(lo));
-              const double fr = lo - c0;           // part of cell c0 not covered
-              // cell c0 covers [c0, c0+1): overlap (1 - fr); cell c0+1: overlap fr
-              const double q0l = col(c0-1), q0 = col(c0), q0r = col(c0+1);
-              const double q1 = q0r, q1r = col(c0+2);
-              const double s0 = VLSlope(q0l, q0, q0r), s1 = VLSlope(q0, q1, q1r);
-              // mean of q0 + s0 (x - 1/2) over x in [fr, 1): q0 + s0*(fr/2)
-              // mean of q1 + s1 (x - 1/2) over x in [0, fr): q1 + s1*(fr - 1)/2
-              const double val = (1.0 - fr)*(q0 + 0.5*s0*fr) + fr*(q1 + 0.5*s1*(fr - 1.0));
-              (void)ish; (void)f;
-              a[s->Idx(m, nvar, v, k, j, ig)] = val;
+              const int Gt = b.lx2*s->nx2 + (j - s->js);
+              const int G = (side == 0) ? (Gt - joffset) : (Gt + joffset);
+              const double r = u(G) - (PLMRemapFlx(u, G+1, eps) - PLMRemapFlx(u, G, eps));
+              a[s->Idx(m,nvar,v,k,j,ig)] = r;
             }
           }
         }
@@ -442,13 +431,10 @@ void FoldShear(or_sim *s, std::vector<double> &a, int nvar, double t) {
     const int bx = (side == 0) ? s->M.nmbx1 - 1 : 0;
     const int it = (side == 0) ? (s->ie - (s->is - 1 - e.i)) : (s->is + (e.i - s->ie - 1));
     if (gk < 0 || gk >= nx3t) {
-      if (s->M.bc_x3 == DK_X3_PERIODIC) {
-        gk = (gk + nx3t) % nx3t;
-      } else if (!s->M.x3_dep_drop) {
-        gk = (gk < 0) ? (-1 - gk) : (2*nx3t - 1 - gk);
-      } else {
-        continue;   // dropped
-      }
+      // beyond a physical x3 face the first pass of the fold has already mirrored (or
+      // dropped) this layer into the active layers of the same ghost column
+      if (s->M.bc_x3 != DK_X3_PERIODIC) continue;
+      gk = (gk + nx3t) % nx3t;
     }
     const int by3 = gk / s->nx3, kt = s->ks + (gk - by3*s->nx3);
     const double lo = gj + ((side == 0) ? -shc : shc);

@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every entry point the headers declare (no GPU call)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from athenak_amd import abi, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in ("dustk.h", "dustk_setup.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(dk_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_the_task_entry_points():
+    names = declared_symbols()
+    for t in lib.TASKS:
+        assert "dk_" + t in names
+    assert {"dk_create", "dk_destroy", "dk_stage_run", "dk_diag_sums", "dk_setup_problem"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libdustk.so not built")
+    L = ctypes.CDLL(lib.LIB_PATH)
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_the_header_layout(tmp_path):
+    """The ctypes mirror has the sizes the C compiler gives the header's structs."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include "dustk.h"\n#include "dustk_setup.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(dk_block),'
+                   ' sizeof(dk_stage), sizeof(dk_mesh), sizeof(dk_physics), sizeof(dk_gas_src),'
+                   ' sizeof(dk_setup)); return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
+    want = [ctypes.sizeof(t) for t in (abi.dk_block, abi.dk_stage, abi.dk_mesh, abi.dk_physics,
+                                       abi.dk_gas_src, abi.dk_setup)]
+    assert got == want
+
+
+def test_strerror_without_device():
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libdustk.so not built")
+    L = lib.lib()
+    assert L.dk_strerror(4).decode().startswith("particle outside")
