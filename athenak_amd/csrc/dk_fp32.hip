@@ -46,8 +46,8 @@ constexpr int kBigMax = 16384;      // longest list of one cell in the FP32 form
 constexpr int kTileThreads = 256;
 
 struct Store {
-  double *x[3], *x1[3];
-  float *v[3], *v1[3], *r[3];
+  double *x[3], *x1[3], *r[3];   // positions, RK registers, drag rates (FP64: see k_p3)
+  float *v[3], *v1[3];
   float *ts, *m;
   int *gid, *orig;
 };
@@ -78,6 +78,7 @@ struct Fast {
   float *ustar = nullptr, *ap = nullptr, *res = nullptr;
   double *dmom = nullptr;           // D of the kick, then R_g
   double *dc[3] = {nullptr, nullptr, nullptr};  // actual dust correction: own, below, above
+  double *pres = nullptr;           // push rounding residue per cell (to the gas)
   float *scr_f = nullptr;
   double *scr_d = nullptr;
   int pending = 0;
@@ -152,8 +153,11 @@ __device__ inline int CellOf(double x, double xmin, double xmax, int nx) {
 // block.  Returns false on a routing error.
 struct PState {
   double x[3], x1[3];
-  float v[3], v1[3], r[3];
+  double r[3];
+  float v[3], v1[3];
+  double vres[3];   // exact FP64 push result minus its FP32 rounding
   int gid;
+  int parked;       // left through an outflow x3 face: zero mass from now on
 };
 
 __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
@@ -170,11 +174,20 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
   if (write_regs) {   // stage 1: (x1, v1) <- (x, v) (dust_push.cpp:69-77)
     for (int d=0; d<3; ++d) {s.x1[d] = s.x[d]; s.v1[d] = s.v[d];}
   }
-  const float g0 = static_cast<float>(A.gam0), g1 = static_cast<float>(A.gam1);
-  const float bdt = static_cast<float>(A.beta_dt), adt = static_cast<float>(A.atw_dt);
-  s.v[0] = g0*vxo + g1*s.v1[0] + bdt*fx + adt*s.r[0];
-  s.v[1] = g0*vyo + g1*s.v1[1] + bdt*fy + adt*s.r[1];
-  s.v[2] = g0*vzo + g1*s.v1[2] + bdt*fz + adt*s.r[2];
+  // the new velocity in FP64 from the stored FP32 values, rounded once to FP32; the rounding
+  // residue goes to the gas (momentum-conserving rounding)
+  const double fxd = A.sbox ? 2.0*A.om*static_cast<double>(vyo) : 0.0;
+  const double fyd = A.sbox ? -(2.0-A.qs)*A.om*static_cast<double>(vxo) : 0.0;
+  const double fzd = (A.sbox && A.strat) ? -(A.om*A.om)*zo : 0.0;
+  (void)fx; (void)fy; (void)fz;
+  const double vn[3] = {
+      A.gam0*vxo + A.gam1*s.v1[0] + A.beta_dt*fxd + A.atw_dt*s.r[0],
+      A.gam0*vyo + A.gam1*s.v1[1] + A.beta_dt*fyd + A.atw_dt*s.r[1],
+      A.gam0*vzo + A.gam1*s.v1[2] + A.beta_dt*fzd + A.atw_dt*s.r[2]};
+  for (int d=0; d<3; ++d) {
+    s.v[d] = static_cast<float>(vn[d]);
+    s.vres[d] = vn[d] - static_cast<double>(s.v[d]);
+  }
   double ydot = vyo;
   if (A.sbox) ydot -= A.qs*A.om*xo;
   s.x[0] = A.gam0*xo + A.gam1*s.x1[0] + A.beta_dt*static_cast<double>(vxo);
@@ -200,10 +213,17 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
       double z = 2.0*zf - s.x[2];
       if (z >= g.x3max) z = g.x3max - 1.0e-12*(g.x3max - g.x3min);
       if (z < g.x3min) z = g.x3min;
-      s.x[2] = z; s.v[2] = -s.v[2];
+      s.x[2] = z; s.v[2] = -s.v[2]; s.vres[2] = -s.vres[2];
       s.x1[2] = 2.0*zf - s.x1[2]; s.v1[2] = -s.v1[2];
     } else {
-      return false;   // outflow is not provided by this form (checked at creation)
+      // outflow: park the particle at rest in the boundary cell of its block with zero mass
+      // (bvals_part.cpp:184-196); its momentum leaves with it
+      const DevBlock &bo = A.B[s.gid];
+      const double z = lo ? (bo.x3min + 0.5*bo.dx3) : (bo.x3max - 0.5*bo.dx3);
+      s.x[2] = z;
+      for (int d=0; d<3; ++d) {s.v[d] = 0.0f; s.vres[d] = 0.0; s.v1[d] = 0.0f;}
+      s.x1[0] = s.x[0]; s.x1[1] = s.x[1]; s.x1[2] = z;
+      s.parked = 1;
     }
   }
   const DevBlock &b = B[s.gid];
@@ -303,9 +323,10 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
     s.v[d] = S.v[d][p];
     s.x1[d] = regs ? S.x1[d][p] : 0.0;
     s.v1[d] = regs ? S.v1[d][p] : 0.0f;
-    s.r[d] = rates ? S.r[d][p] : 0.0f;
+    s.r[d] = rates ? S.r[d][p] : 0.0;
   }
   s.gid = S.gid[p];
+  s.parked = 0;
 }
 
 //----------------------------------------------------------------------------------------
@@ -339,15 +360,17 @@ k_key(KArgs A, int do_key) {
         const float cjf = static_cast<float>(A.a_prev/(static_cast<double>(A.src.ts[q]) +
                                                        A.a_prev));
         const long long ic = Idx(g, m, 3, 0, g.ks + ck, g.js + jc, g.is + ci);
-        float rr[3];
+        double rr[3];
         for (int d=0; d<3; ++d) {
           const float res = A.res[ic + d*g.ncb];
           const float dv = cjf*res;
           const float vo = s.v[d];
           const float vn = vo + dv;
           s.v[d] = vn;
-          rr[d] = A.src.r[d][q] + dv/static_cast<float>(A.a_prev);
+          // the recorded rate takes the actual change, so that the history terms of dust and
+          // gas stay opposite (dust_pm.cpp:694)
           const double delta = static_cast<double>(vn) - static_cast<double>(vo);
+          rr[d] = A.src.r[d][q] + delta/A.a_prev;
           dc[dj + 1][d] += static_cast<double>(mass)*delta/vol;
         }
         for (int d=0; d<3; ++d) {A.src.v[d][q] = s.v[d]; A.src.r[d][q] = rr[d];}
@@ -461,7 +484,7 @@ __device__ inline void FlushLDS(const T *lds, T *scr, long long tile, int n) {
 // P1: push + migration into the new order, and the deposit of Q = sum m c W/V and
 // P = sum m c W v/V (dust_pm.cpp:225-274) plus the centre mass m c/V (dust_pm.cpp:660-667)
 __global__ void __launch_bounds__(kTileThreads)
-k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
+k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
   extern __shared__ float ldsf[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -472,6 +495,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   float acc[27][4];
   for (int o=0; o<27; ++o) for (int n=0; n<4; ++n) acc[o][n] = 0.0f;
   float q3[3] = {0.0f, 0.0f, 0.0f};
+  double pr3[3] = {0.0, 0.0, 0.0};   // push rounding residue of this cell's particles
   int nshift = 0;   // particles whose deposit centre is one cell beyond the block in x2
   int s0 = 0, s1 = 0;
   const DevBlock &b = A.B[m];
@@ -492,7 +516,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
       PState s;
       LoadP(A.src, p, s, true, !A.stage1);
       if (!PushMigrate(A, s, A.stage1 != 0)) {dk::RecordError(A.err, DK_EROUTE, p); continue;}
-      const float ts = A.src.ts[p], mass = A.src.m[p];
+      const float ts = A.src.ts[p], mass = s.parked ? 0.0f : A.src.m[p];
       if (pass == 0) {
         for (int d=0; d<3; ++d) {
           A.dst.x[d][q] = s.x[d]; A.dst.x1[d][q] = s.x1[d];
@@ -506,6 +530,20 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
       if (s.gid != m || ni != ci || nj != cj || nk != ck) {
         dk::RecordError(A.err, DK_EHALO, p); continue;
       }
+      const double cjd = A.a_dt/(static_cast<double>(ts) + A.a_dt);
+      const float muc = static_cast<float>(static_cast<double>(mass)*cjd/vol);
+      if (pass == 0) {
+        for (int d=0; d<3; ++d) pr3[d] += static_cast<double>(mass)*s.vres[d]/vol;
+        int ic_, jc, kc_;
+        if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
+          // the centre of the unshifted position: this cell or one beside it in x2
+          if (ic_ != ci || kc_ != ck || jc - cj < -1 || jc - cj > 1) {
+            dk::RecordError(A.err, DK_EHALO, p);
+          } else {
+            q3[jc - cj + 1] += muc;
+          }
+        }
+      }
       // 0, or +-1: the deposit centre beyond the block's x2 face (frame shift)
       const int sj = Tl.SY ? (nju - nj) : 0;
       if (pass == 0 && sj != 0) {++nshift; continue;}
@@ -515,8 +553,6 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
       AxisW(s.x[0], b.x1min, b.x1max, g.nx1, g.deposit, ci, wx);
       AxisW(s.x[1] + A.sh*s.x[0], b.x2min, b.x2max, g.nx2, g.deposit, cj + sj, wy);
       AxisW(s.x[2], b.x3min, b.x3max, g.nx3, g.deposit, ck, wz);
-      const double cjd = A.a_dt/(static_cast<double>(ts) + A.a_dt);
-      const float muc = static_cast<float>(static_cast<double>(mass)*cjd/vol);
       const float vx = s.v[0], vy = s.v[1], vz = s.v[2];
       for (int c=0; c<3; ++c)
         for (int bb=0; bb<3; ++bb) {
@@ -530,15 +566,6 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
             acc[o][3] += w*vz;
           }
         }
-      int ic_, jc, kc_;
-      if (pass == 0 && mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
-        // the centre of the unshifted position: this cell or one beside it in x2
-        if (ic_ != ci || kc_ != ck || jc - cj < -1 || jc - cj > 1) {
-          dk::RecordError(A.err, DK_EHALO, p);
-        } else {
-          q3[jc - cj + 1] += muc;
-        }
-      }
     }
     Phases<float, 4>(acc, ldsf, Tl, lx + 1, ly + 1 + Tl.SY + dshift, lz + 1, act);
   }
@@ -547,6 +574,8 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
     qt0[i0] = q3[1];
     qt1[i0 + g.n1] = q3[2];
     qt2[i0 - g.n1] = q3[0];
+    const long long i3 = Idx(g, m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+    for (int d=0; d<3; ++d) pres[i3 + d*g.ncb] = pr3[d];
   }
   FlushLDS(ldsf, scr, blockIdx.x, 4*Tl.RN);
 }
@@ -631,7 +660,6 @@ k_p3(KArgs A, const float *us, double *scr) {
     const long long kc = CellKey(g, m, ci, cj, ck);
     s0 = A.start[kc]; s1 = A.start[kc + 1];
   }
-  const float inv_a = static_cast<float>(1.0/A.a_dt);
   for (int pass=0; pass<2; ++pass) {
     if (pass == 1 && !__syncthreads_or(nshift > 0)) break;
     for (int o=0; o<27; ++o) for (int n=0; n<3; ++n) acc[o][n] = 0.0;
@@ -668,8 +696,8 @@ k_p3(KArgs A, const float *us, double *scr) {
         const float dv = cj32*(u[d] - vo);
         const float vn = vo + dv;
         A.src.v[d][q] = vn;
-        A.src.r[d][q] = dv*inv_a;
         delta[d] = static_cast<double>(vn) - static_cast<double>(vo);
+        A.src.r[d][q] = delta[d]/A.a_dt;
       }
       const double fac = -static_cast<double>(A.src.m[q])/vol;
       for (int c=0; c<3; ++c)
@@ -744,6 +772,7 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, int nvar, T *out) {
 // (stage 2, dust_pm.cpp:734-739)
 __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, double *dmom,
                              const double *dc0, const double *dc1, const double *dc2,
+                             const double *pres,
                              int pending, double inv_aprev, int stage1, int hook, double gam0,
                              double gam1, double bdt, int use_src, dk_gas_src src,
                              double atw_dt) {
@@ -755,7 +784,7 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
       for (int c=0; c<3; ++c) {
         const long long id = Idx(g,m,3,c,k,j,i);
         const double dgas = dmom[id] - ((dc0[id] + dc1[id]) + dc2[id]);
-        u[1+c] += dgas;
+        u[1+c] += dgas + pres[id];   // pres: what the FP32 rounding of the push withheld
         rg[c] = dgas*inv_aprev;
         dmom[id] = rg[c];
       }
@@ -853,7 +882,7 @@ __global__ void k_bind(const double *r, const int32_t *iv, long long cap, long l
       S.v[d][p] = static_cast<float>(r[(long long)(DK_IPVX + 2*d)*cap + p]);
       S.x1[d][p] = r[(long long)(DK_IPX1 + 2*d)*cap + p];
       S.v1[d][p] = static_cast<float>(r[(long long)(DK_IPVX1 + 2*d)*cap + p]);
-      S.r[d][p] = static_cast<float>(r[(long long)(DK_IPRX + d)*cap + p]);
+      S.r[d][p] = r[(long long)(DK_IPRX + d)*cap + p];
     }
     S.ts[p] = static_cast<float>(r[(long long)DK_IPTS*cap + p]);
     S.m[p] = static_cast<float>(r[(long long)DK_IPM*cap + p]);
@@ -896,7 +925,7 @@ __global__ void k_export(Store S, long long np, const int32_t *itag, long long n
       r[(long long)(DK_IPVX + 2*d)*cap + q] = static_cast<double>(S.v[d][q]);
       r[(long long)(DK_IPX1 + 2*d)*cap + q] = S.x1[d][q];
       r[(long long)(DK_IPVX1 + 2*d)*cap + q] = static_cast<double>(S.v1[d][q]);
-      r[(long long)(DK_IPRX + d)*cap + q] = static_cast<double>(S.r[d][q]);
+      r[(long long)(DK_IPRX + d)*cap + q] = S.r[d][q];
     }
     r[(long long)DK_IPTS*cap + q] = static_cast<double>(S.ts[q]);
     r[(long long)DK_IPM*cap + q] = static_cast<double>(S.m[q]);
@@ -1029,7 +1058,7 @@ int Flush(dk_ctx *c) {
   std::memset(&z, 0, sizeof(z));
   const long long nact = (long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3;
   TIMED("cell_start", (k_cell_start<<<Grid(nact), kBlock, 0, c->stream>>>(
-      c->g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], 1,
+      c->g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->pres, 1,
       1.0/f->a_prev, 0, 0, 1.0, 0.0, 0.0, 0, z, 0.0)));
   f->pending = 0;
   return dk::Check(hipGetLastError());
@@ -1041,7 +1070,6 @@ namespace dk {
 
 int FastCreate(dk_ctx *c) {
   if (c->P.coupling != DK_IMEX) return DK_EUNSUPPORTED;          // pc2: the FP64 form
-  if (c->M.bc_x3 == DK_X3_OUTFLOW) return DK_EUNSUPPORTED;       // parked particles: FP64 form
   if (c->P.solver == DK_APPLYA) return DK_EUNSUPPORTED;
   if (c->g.ng < 2 || !c->P.back_reaction) return DK_EUNSUPPORTED;
   Fast *f = new Fast();
@@ -1069,6 +1097,7 @@ int FastCreate(dk_ctx *c) {
   e = e ? e : Alloc((void**)&f->ap, sizeof(float)*3*nc);
   e = e ? e : Alloc((void**)&f->res, sizeof(float)*3*nc);
   e = e ? e : Alloc((void**)&f->dmom, sizeof(double)*3*nc);
+  e = e ? e : Alloc((void**)&f->pres, sizeof(double)*3*nc);
   for (int q=0; q<3; ++q) e = e ? e : Alloc((void**)&f->dc[q], sizeof(double)*3*nc);
   e = e ? e : Alloc((void**)&f->scr_f, sizeof(float)*4*(size_t)T.RN*T.nt);
   e = e ? e : Alloc((void**)&f->scr_d, sizeof(double)*3*(size_t)T.RN*T.nt);
@@ -1078,6 +1107,7 @@ int FastCreate(dk_ctx *c) {
   (void)hipMemset(f->qt[2], 0, sizeof(float)*nc);
   for (int q=0; q<3; ++q) (void)hipMemset(f->dc[q], 0, sizeof(double)*3*nc);
   (void)hipMemset(f->dmom, 0, sizeof(double)*3*nc);
+  (void)hipMemset(f->pres, 0, sizeof(double)*3*nc);
   (void)hipMemset(f->ustar, 0, sizeof(float)*3*nc);
   (void)hipMemset(f->res, 0, sizeof(float)*3*nc);
   if (hipcub::DeviceScan::ExclusiveSum(nullptr, f->scan_bytes, f->count, f->start,
@@ -1100,7 +1130,7 @@ void FastDestroy(dk_ctx *c) {
   }
   for (void *p : {(void*)f->key, (void*)f->rank, (void*)f->count, (void*)f->start,
                   (void*)f->perm, (void*)f->big, f->scan_tmp, (void*)f->itag, (void*)f->qdep,
-                  (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom,
+                  (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom, (void*)f->pres,
                   (void*)f->scr_f, (void*)f->scr_d})
     if (p) (void)hipFree(p);
   for (int q=0; q<3; ++q) {
@@ -1121,11 +1151,11 @@ int FastBind(dk_ctx *c) {
     for (int b=0; b<2; ++b) {
       Store &S = f->s[b];
       for (int d=0; d<3; ++d) {
-        for (void **p : {(void**)&S.x[d], (void**)&S.x1[d]}) {
+        for (void **p : {(void**)&S.x[d], (void**)&S.x1[d], (void**)&S.r[d]}) {
           if (*p) (void)hipFree(*p);
           if (Alloc(p, sizeof(double)*n)) return DK_ENOMEM;
         }
-        for (void **p : {(void**)&S.v[d], (void**)&S.v1[d], (void**)&S.r[d]}) {
+        for (void **p : {(void**)&S.v[d], (void**)&S.v1[d]}) {
           if (*p) (void)hipFree(*p);
           if (Alloc(p, sizeof(float)*n)) return DK_ENOMEM;
         }
@@ -1220,7 +1250,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     std::memset(&z, 0, sizeof(z));
     const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
     TIMED("cell_start", (k_cell_start<<<Grid(nact), kBlock, 0, c->stream>>>(
-        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->pending,
+        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->pres, f->pending,
         f->pending ? 1.0/f->a_prev : 0.0, A.stage1, hook, gam0, gam1, bdt, src ? 1 : 0,
         src ? *src : z, (c->P.back_reaction && st->stage == 2) ? st->a_twid22*st->dt : 0.0)));
   }
@@ -1232,7 +1262,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   const size_t lds4 = sizeof(float)*4*T.RN, lds3 = sizeof(float)*3*T.RN;
   const size_t ldsd = sizeof(double)*3*T.RN;
   TIMED("p1_push_deposit", (k_p1<<<T.nt, kTileThreads, lds4, c->stream>>>(
-      A, f->qt[0], f->qt[1], f->qt[2], f->scr_f)));
+      A, f->qt[0], f->qt[1], f->qt[2], f->pres, f->scr_f)));
   f->cur = 1 - f->cur;
   A.src = f->s[f->cur];
   A.dst = f->s[1 - f->cur];

@@ -745,18 +745,21 @@ __global__ void k_reduce_partials(const double *partial, int nblk, int ncomp, do
   out[c] = s;
 }
 
-// dust_newdt.cpp:135-145: min over particles of dx/|v| (and the stopping-time check)
+// dust_newdt.cpp:135-145: min over particles of dx/max(|v|, 1e-30), the azimuthal transport
+// velocity with the background shear in a 3-D shearing box; stopping times validated
+// (dust_newdt.cpp:34-99)
 __global__ void k_newdt(const DevBlock *B, const double *r, const int32_t *iv, long long cap,
-                        long long np, double *partial, dk::ErrWord *err) {
+                        long long np, double qo, double *partial, dk::ErrWord *err) {
   __shared__ double sh[kBlock];
-  double dtmin = 1.0e300;
+  double dtmin = 3.4028234663852886e38;   // the largest float, the reference's start value
   GRID_STRIDE(p, np) {
     const int m = PI(DK_PGID,p);
-    if (PR(DK_IPTS,p) <= 0.0) dk::RecordError(err, DK_ESTOP, p);
-    const double vx = fabs(PR(DK_IPVX,p)), vy = fabs(PR(DK_IPVY,p)), vz = fabs(PR(DK_IPVZ,p));
-    if (vx > 0.0) dtmin = fmin(dtmin, B[m].dx1/vx);
-    if (vy > 0.0) dtmin = fmin(dtmin, B[m].dx2/vy);
-    if (vz > 0.0) dtmin = fmin(dtmin, B[m].dx3/vz);
+    const double ts = PR(DK_IPTS,p);
+    if (!isfinite(ts) || !(ts > 0.0)) dk::RecordError(err, DK_ESTOP, p);
+    const double vy = PR(DK_IPVY,p) - qo*PR(DK_IPX,p);
+    dtmin = fmin(B[m].dx1/fmax(fabs(PR(DK_IPVX,p)), 1.0e-30), dtmin);
+    dtmin = fmin(B[m].dx2/fmax(fabs(vy), 1.0e-30), dtmin);
+    dtmin = fmin(B[m].dx3/fmax(fabs(PR(DK_IPVZ,p)), 1.0e-30), dtmin);
   }
   sh[threadIdx.x] = dtmin;
   __syncthreads();
@@ -1331,15 +1334,20 @@ int dk_new_time_step(dk_ctx *c, double dt_cfl, double *dt_out) {
   int e = Ready(c);
   if (e) return e;
   const int nb = 1024;
+  const double qo = c->P.shearing_box ? c->P.qshear*c->P.omega0 : 0.0;
   k_newdt<<<nb, kBlock, 0, c->stream>>>(c->d_blocks, c->rdata, c->idata, (long long)c->cap,
-                                        (long long)c->np, c->d_partial, c->d_err);
+                                        (long long)c->np, qo, c->d_partial, c->d_err);
   std::vector<double> h(nb);
   if ((e = dk::Check(hipMemcpyAsync(h.data(), c->d_partial, sizeof(double)*nb,
                                     hipMemcpyDeviceToHost, c->stream))))
     return e;
   if ((e = dk::Check(hipStreamSynchronize(c->stream)))) return e;
-  double m = 1.0e300;
+  double m = 3.4028234663852886e38;
   for (double x : h) m = std::fmin(m, x);
+  int32_t code = 0;
+  int64_t part = -1;
+  if ((e = dk_error(c, &code, &part))) return e;
+  if (code == DK_ESTOP) return DK_ESTOP;
   if (dt_out) *dt_out = dt_cfl*m;
   return DK_OK;
 }

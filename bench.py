@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--n", type=int, default=None, help="override the mesh size (tests)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run plain device copies of this many GiB (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
@@ -91,6 +94,10 @@ def gpu_run(w, form, steps, warmup):
     fid = abi.DK_FORM_FP64 if form == "fp64" else abi.DK_FORM_FP32
     ctx = DustContext(w.mesh, w.blocks, w.phys, w.npart, form=fid)
     ctx.setup(w.setup)
+    if form == "fp64":
+        # the reference sorts point particles by cell at cycle 0 and every 50 cycles
+        # (SortParticles, src/dust/dust_cloud.cpp:615-628)
+        ctx.sort()
     torch.cuda.synchronize()
     d0 = ctx.diag()
     for _ in range(warmup):
@@ -172,8 +179,26 @@ def rate(r, w, steps):
     return ms_step, w.npart / (ms_step * 1e-3) / 1e6
 
 
+def copy_calibration(gib):
+    """Three launches of a plain elementwise copy of `gib` GiB (fp32) on the device: a kernel
+    whose HBM traffic is known (gib GiB read + gib GiB written)."""
+    import torch
+    n = int(gib * 2**30 / 4)
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": 3}), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, so --gpus %d is "
               "not run (the path would shard by MeshBlock; see DESIGN.md)" % args.gpus)

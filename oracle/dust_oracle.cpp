@@ -1471,19 +1471,23 @@ int or_sort_particles(or_sim *s) {
   return DK_OK;
 }
 
-// src/dust/dust_newdt.cpp:108-149 (no shearing-box correction of |v_y|)
+// src/dust/dust_newdt.cpp:108-149: dt_cfl min dx/max(|v|, 1e-30), with the background shear in
+// the azimuthal transport velocity of the 3-D shearing box; stopping times validated
+// (dust_newdt.cpp:34-99)
 int or_new_time_step(or_sim *s, double dt_cfl, double *dt_out) {
-  double dtmin = std::numeric_limits<double>::max();
+  double dtp = static_cast<double>(std::numeric_limits<float>::max());
+  const double qo = s->P.qshear*s->P.omega0;
   for (int64_t p=0; p<s->np; ++p) {
     const int m = s->I(DK_PGID,p);
-    if (s->R(DK_IPTS,p) <= 0.0) return DK_ESTOP;
-    const double vx = std::fabs(s->R(DK_IPVX,p)), vy = std::fabs(s->R(DK_IPVY,p));
-    const double vz = std::fabs(s->R(DK_IPVZ,p));
-    if (vx > 0.0) dtmin = std::fmin(dtmin, s->dx1[m]/vx);
-    if (vy > 0.0) dtmin = std::fmin(dtmin, s->dx2[m]/vy);
-    if (vz > 0.0) dtmin = std::fmin(dtmin, s->dx3[m]/vz);
+    const double ts = s->R(DK_IPTS,p);
+    if (!std::isfinite(ts) || !(ts > 0.0)) return DK_ESTOP;
+    double vy = s->R(DK_IPVY,p);
+    if (s->P.shearing_box) vy -= qo*s->R(DK_IPX,p);
+    dtp = std::fmin(s->dx1[m]/std::fmax(std::fabs(s->R(DK_IPVX,p)), 1.0e-30), dtp);
+    dtp = std::fmin(s->dx2[m]/std::fmax(std::fabs(vy), 1.0e-30), dtp);
+    dtp = std::fmin(s->dx3[m]/std::fmax(std::fabs(s->R(DK_IPVZ,p)), 1.0e-30), dtp);
   }
-  if (dt_out) *dt_out = dt_cfl*dtmin;
+  if (dt_out) *dt_out = dt_cfl*dtp;
   return DK_OK;
 }
 

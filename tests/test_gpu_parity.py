@@ -157,8 +157,11 @@ def test_sort_by_cell_is_the_oracles_stable_order():
     o.sort()
     g.sort()
     n = o.npart
+    # the permutation is exact; the values carry the set-up's last-bit differences of the
+    # velocity draws (log/cos of the host and device libraries)
     np.testing.assert_array_equal(o.idata()[:, :n], g.idata[:, :n].cpu().numpy())
-    np.testing.assert_array_equal(o.rdata()[:, :n], g.rdata[:, :n].cpu().numpy())
+    np.testing.assert_allclose(o.rdata()[:, :n], g.rdata[:, :n].cpu().numpy(), rtol=1e-15,
+                               atol=1e-18)
 
 
 def test_new_time_step_matches_oracle():
