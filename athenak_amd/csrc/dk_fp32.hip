@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "dk_internal.hpp"
@@ -43,20 +45,40 @@ namespace {
 
 constexpr int kSegSmall = 32;       // lists longer than this are ordered by k_bigsort
 constexpr int kBigMax = 16384;      // longest list of one cell in the FP32 form
-constexpr int kTileThreads = 256;
+constexpr int kKeyThreads = 256;    // the key pass: one particle per thread per chunk
+constexpr int kCellThreads = 384;   // P1-P3: one thread per (cell, stencil slab), <= 128 cells
+constexpr int kMaxTileCells = 128;
 
+// The particle store: three buffers in groups of 64 particles (one wavefront), each group
+// holding its columns one after another (double: x[3], x1[3], r[3]; float: v[3], v1[3], t_s, m;
+// int: block, original index).  A wavefront reading one column of 64 consecutive particles
+// reads one contiguous 256 B or 512 B piece, and every column of a particle is addressed from
+// one base with a constant offset, so that a kernel holds few addresses in registers.
+constexpr int kGrp = 64;
 struct Store {
-  double *x[3], *x1[3], *r[3];   // positions, RK registers, drag rates (FP64: see k_p3)
-  float *v[3], *v1[3];
-  float *ts, *m;
-  int *gid, *orig;
+  double *d;
+  float *f;
+  int *i;
+  long long cap;    // a multiple of kGrp
+  __device__ static long long At(int ncol, int c, long long p) {
+    return ((p >> 6)*ncol + c)*kGrp + (p & (kGrp - 1));
+  }
+  __device__ double &X(int c, long long p) const { return d[At(9, c, p)]; }
+  __device__ double &X1(int c, long long p) const { return d[At(9, 3 + c, p)]; }
+  __device__ double &R(int c, long long p) const { return d[At(9, 6 + c, p)]; }
+  __device__ float &V(int c, long long p) const { return f[At(8, c, p)]; }
+  __device__ float &V1(int c, long long p) const { return f[At(8, 3 + c, p)]; }
+  __device__ float &TS(long long p) const { return f[At(8, 6, p)]; }
+  __device__ float &M(long long p) const { return f[At(8, 7, p)]; }
+  __device__ int &GID(long long p) const { return i[At(2, 0, p)]; }
+  __device__ int &ORIG(long long p) const { return i[At(2, 1, p)]; }
 };
 
 // Tile geometry: a tile is TX x TY x TZ active cells of one block, one thread per cell; its LDS
 // region adds a halo of 1 cell (plus SY more in x2 when deposit centres may sit one cell beyond
 // a block's x2 face: the orbital-advection frame shift of the 3-D shearing box).
 struct Tiles {
-  int TX, TY, TZ, SY, RX, RY, RZ, RN;
+  int TX, TY, TZ, TC, SY, RX, RY, RZ, RN;
   int ntx, nty, ntz, ntb, nt;
 };
 
@@ -68,6 +90,7 @@ struct Fast {
   int cur = 0;
   long long np = 0, cap = 0, nkeys = 0;
   int *key = nullptr, *rank = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr;
+  int *stay = nullptr;              // per cell: the particles that stayed in its tile
   int *big = nullptr;               // [0] = count of long lists, then their cells
   void *scan_tmp = nullptr;
   size_t scan_bytes = 0;
@@ -110,7 +133,7 @@ struct KArgs {
   const DevBlock *B;
   Store src, dst;
   const int *perm, *start;
-  int *key, *rank, *count;
+  int *key, *rank, *count, *stay;
   dk::ErrWord *err;
   // stage numbers
   double gam0, gam1, beta_dt, atw_dt, a_dt, sh, dt;
@@ -124,27 +147,10 @@ struct KArgs {
   double *dc0, *dc1, *dc2;
 };
 
-__device__ inline void TileCell(const Tiles &T, const Geo &g, int tile, int t, int &m, int &ci,
-                                int &cj, int &ck, int &lx, int &ly, int &lz) {
-  m = tile/T.ntb;
-  int rr = tile - m*T.ntb;
-  const int tx = rr % T.ntx; rr /= T.ntx;
-  const int ty = rr % T.nty;
-  const int tz = rr/T.nty;
-  lx = t % T.TX;
-  ly = (t/T.TX) % T.TY;
-  lz = t/(T.TX*T.TY);
-  ci = tx*T.TX + lx; cj = ty*T.TY + ly; ck = tz*T.TZ + lz;
-}
-
-__device__ inline long long CellKey(const Geo &g, int m, int ci, int cj, int ck) {
-  return (long long)m*g.nx1*g.nx2*g.nx3 + ((long long)ck*g.nx2 + cj)*g.nx1 + ci;
-}
-
 // the cell (active index, unclamped) of position x along one axis, as PMWeights
-__device__ inline int CellOf(double x, double xmin, double xmax, int nx) {
+__device__ inline int CellOf(double x, double xmin, double scale) {
 #pragma clang fp contract(off)
-  const double cell = (x - xmin)*(static_cast<double>(nx)/(xmax - xmin));
+  const double cell = (x - xmin)*scale;
   return static_cast<int>(cell + 1.0) - 1;
 }
 
@@ -159,6 +165,13 @@ struct PState {
   int gid;
   int parked;       // left through an outflow x3 face: zero mass from now on
 };
+
+// the block offset of x along one axis (bvals_part.cpp:164-429 form int((x - lo + l)/l) - 1),
+// with the division skipped for a position inside [lo, hi)
+__device__ inline int BlockOff(double x, double lo, double hi, double l) {
+  if (x >= lo && x < hi) return 0;
+  return static_cast<int>((x - lo + l)/l) - 1;
+}
 
 __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
 #pragma clang fp contract(off)
@@ -197,7 +210,7 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
   const DevBlock *B = A.B;
   if (A.shear && A.dsh != 0.0) {
     const double b0 = B[s.gid].x1min, b1 = B[s.gid].x1max, lx = b1 - b0;
-    const int ix = static_cast<int>((s.x[0] - b0 + lx)/lx) - 1;
+    const int ix = BlockOff(s.x[0], b0, b1, lx);
     const bool lo1 = (ix < 0) && (b0 < g.x1min + 0.5*lx);
     const bool hi1 = (ix > 0) && (b1 > g.x1max - 0.5*lx);
     if (lo1 || hi1) {
@@ -228,9 +241,9 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
   }
   const DevBlock &b = B[s.gid];
   const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
-  const int ix = static_cast<int>((s.x[0] - b.x1min + lx)/lx) - 1;
-  const int iy = static_cast<int>((s.x[1] - b.x2min + ly)/ly) - 1;
-  const int iz = static_cast<int>((s.x[2] - b.x3min + lz)/lz) - 1;
+  const int ix = BlockOff(s.x[0], b.x1min, b.x1max, lx);
+  const int iy = BlockOff(s.x[1], b.x2min, b.x2max, ly);
+  const int iz = BlockOff(s.x[2], b.x3min, b.x3max, lz);
   const double bmin[3] = {b.x1min, b.x2min, b.x3min}, bmax[3] = {b.x1max, b.x2max, b.x3max};
   const int off[3] = {ix, iy, iz};
   for (int d=0; d<3; ++d) {
@@ -279,125 +292,390 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
 __device__ inline void DepositCell(const Geo &g, const DevBlock &b, const double x[3],
                                    double sh, int &ci, int &cj, int &ck, int &cju) {
 #pragma clang fp contract(off)
-  ci = CellOf(x[0], b.x1min, b.x1max, g.nx1);
-  cju = CellOf(x[1] + sh*x[0], b.x2min, b.x2max, g.nx2);
-  ck = CellOf(x[2], b.x3min, b.x3max, g.nx3);
+  ci = CellOf(x[0], b.x1min, b.sx1);
+  cju = CellOf(x[1] + sh*x[0], b.x2min, b.sx2);
+  ck = CellOf(x[2], b.x3min, b.sx3);
   ci = min(max(ci, 0), g.nx1 - 1);
   ck = min(max(ck, 0), g.nx3 - 1);
   cj = min(max(cju, 0), g.nx2 - 1);
-}
-
-// The three weights of one axis (PMWeights, src/dust/dust.hpp:1088-1111) about a given
-// centre cell c: del = cell - (c + 1/2).  With c = PMWeights' own cell this is PMWeights; with
-// c one below it (a position that rounds onto the upper face, cell = nx, del = -1/2) it gives
-// the same two non-zero weights one slot lower (del = +1/2), inside the block.
-__device__ inline void AxisW(double x, double xmin, double xmax, int nx, int scheme, int c,
-                             double w[3]) {
-#pragma clang fp contract(off)
-  const double cell = (x - xmin)*(static_cast<double>(nx)/(xmax - xmin));
-  const double del = cell - (static_cast<double>(c) + 0.5);
-  if (scheme == 0) {
-    w[0] = 0.0; w[1] = 1.0; w[2] = 0.0;
-  } else if (scheme == 1) {
-    w[0] = fmax(0.0, -del); w[1] = 1.0 - fabs(del); w[2] = fmax(0.0, del);
-  } else {
-    w[0] = 0.5*(0.5 - del)*(0.5 - del);
-    w[1] = 0.75 - del*del;
-    w[2] = 0.5*(0.5 + del)*(0.5 + del);
-  }
 }
 
 // the centre cell of the unshifted position (dust_pm.cpp:649-659); false when it rounds onto
 // a ghost cell, where the reference's centre-mass and correction touch no active cell
 __device__ inline bool CenterCell(const Geo &g, const DevBlock &b, const double x[3], int &ci,
                                   int &cj, int &ck) {
-  ci = CellOf(x[0], b.x1min, b.x1max, g.nx1);
-  cj = CellOf(x[1], b.x2min, b.x2max, g.nx2);
-  ck = CellOf(x[2], b.x3min, b.x3max, g.nx3);
+  ci = CellOf(x[0], b.x1min, b.sx1);
+  cj = CellOf(x[1], b.x2min, b.sx2);
+  ck = CellOf(x[2], b.x3min, b.sx3);
   return ci >= 0 && ci < g.nx1 && cj >= 0 && cj < g.nx2 && ck >= 0 && ck < g.nx3;
 }
 
 __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, bool rates) {
   for (int d=0; d<3; ++d) {
-    s.x[d] = S.x[d][p];
-    s.v[d] = S.v[d][p];
-    s.x1[d] = regs ? S.x1[d][p] : 0.0;
-    s.v1[d] = regs ? S.v1[d][p] : 0.0f;
-    s.r[d] = rates ? S.r[d][p] : 0.0;
+    s.x[d] = S.X(d, p);
+    s.v[d] = S.V(d, p);
+    s.x1[d] = regs ? S.X1(d, p) : 0.0;
+    s.v1[d] = regs ? S.V1(d, p) : 0.0f;
+    s.r[d] = rates ? S.R(d, p) : 0.0;
   }
-  s.gid = S.gid[p];
+  s.gid = S.GID(p);
   s.parked = 0;
 }
 
 //----------------------------------------------------------------------------------------
-// key pass: the pending consistent-dust correction of the previous stage (dust_pm.cpp:686-696;
-// the actual change of every particle is recorded for the gas), then the new cell of every
-// particle after this stage's push and migration, and its slot in the atomic histogram
-__global__ void __launch_bounds__(kTileThreads)
-k_key(KArgs A, int do_key) {
-  const Geo &g = A.g;
-  int m, ci, cj, ck, lx, ly, lz;
-  TileCell(A.T, g, blockIdx.x, threadIdx.x, m, ci, cj, ck, lx, ly, lz);
-  if (threadIdx.x >= A.T.TX*A.T.TY*A.T.TZ) return;
-  const long long kc = CellKey(g, m, ci, cj, ck);
-  const int s0 = A.start[kc], s1 = A.start[kc + 1];
-  const DevBlock &b = A.B[m];
-  const double vol = b.dx1*b.dx2*b.dx3;
-  double dc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // [own, from below, from above][comp]
-  for (int q=s0; q<s1; ++q) {
-    PState s;
-    LoadP(A.src, q, s, !A.stage1, false);
-    if (A.pending) {
-      const float mass = A.src.m[q];
-      int ic_, jc, kc_;
-      if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
-        // the centre: the own cell or one cell beside it in x2
-        const int dj = jc - cj;   // -1, 0 or +1
-        if (ic_ != ci || kc_ != ck || dj < -1 || dj > 1) {
-          dk::RecordError(A.err, DK_EHALO, q);
-          continue;
-        }
-        const float cjf = static_cast<float>(A.a_prev/(static_cast<double>(A.src.ts[q]) +
-                                                       A.a_prev));
-        const long long ic = Idx(g, m, 3, 0, g.ks + ck, g.js + jc, g.is + ci);
-        double rr[3];
-        for (int d=0; d<3; ++d) {
-          const float res = A.res[ic + d*g.ncb];
-          const float dv = cjf*res;
-          const float vo = s.v[d];
-          const float vn = vo + dv;
-          s.v[d] = vn;
-          // the recorded rate takes the actual change, so that the history terms of dust and
-          // gas stay opposite (dust_pm.cpp:694)
-          const double delta = static_cast<double>(vn) - static_cast<double>(vo);
-          rr[d] = A.src.r[d][q] + delta/A.a_prev;
-          dc[dj + 1][d] += static_cast<double>(mass)*delta/vol;
-        }
-        for (int d=0; d<3; ++d) {A.src.v[d][q] = s.v[d]; A.src.r[d][q] = rr[d];}
-      }
-    }
-    if (!do_key) continue;
-    if (!PushMigrate(A, s, A.stage1 != 0)) {dk::RecordError(A.err, DK_EROUTE, q); continue;}
-    int ni, nj, nk, nju;
-    DepositCell(g, A.B[s.gid], s.x, A.sh, ni, nj, nk, nju);
-    const long long key = CellKey(g, s.gid, ni, nj, nk);
-    A.key[q] = static_cast<int>(key);
-    A.rank[q] = atomicAdd(&A.count[key], 1);
+// Tile kernels.  A workgroup owns one tile; the tile's particles are one contiguous range of
+// the store (the keys are tile-major).  A kernel walks the range in chunks of CH particles: a
+// particle phase (each thread takes kPPT particles, coalesced loads and stores) computes each
+// particle's work and leaves one record per particle in LDS; a cell phase (one thread per
+// (cell, stencil slab), the slab being the x3 offset) sums the records of its cell, in store
+// order, into its 9 stencil offsets; the sums are then added into the LDS copy of the tile
+// region in 27 phases, one offset per phase, so that no two threads of a phase touch the same
+// word.  The cells are dealt to the threads in order of decreasing particle count (so that the
+// lanes of a wave loop a similar number of times); which thread forms a cell's sums does not
+// change them.  At the end the region is written to the tile's scratch.
+
+constexpr int kPPT = 2;             // particles per thread in the particle phase
+
+struct TileInfo {
+  int m, ci0, cj0, ck0;
+  long long key0;
+};
+
+// workgroups are dealt to the 8 XCDs in turn: consecutive tiles go to one XCD so that
+// neighbouring tiles (whose regions overlap) share its L2
+__device__ inline int TileIndex(const Tiles &T) {
+  const int b = blockIdx.x;
+  if (T.nt % 8 != 0) return b;
+  return (b % 8)*(T.nt/8) + b/8;
+}
+
+__device__ inline TileInfo TileOf(const Tiles &T, int tile) {
+  TileInfo t;
+  t.m = tile/T.ntb;
+  int r = tile - t.m*T.ntb;
+  const int tx = r % T.ntx; r /= T.ntx;
+  const int ty = r % T.nty;
+  const int tz = r/T.nty;
+  t.ci0 = tx*T.TX; t.cj0 = ty*T.TY; t.ck0 = tz*T.TZ;
+  t.key0 = (long long)tile*T.TC;
+  return t;
+}
+
+__host__ __device__ inline long long TileKey(const Tiles &T, int m, int ci, int cj, int ck) {
+  const int tx = ci/T.TX, ty = cj/T.TY, tz = ck/T.TZ;
+  const int lx = ci - tx*T.TX, ly = cj - ty*T.TY, lz = ck - tz*T.TZ;
+  return ((long long)m*T.ntb + (tz*T.nty + ty)*T.ntx + tx)*T.TC + (lz*T.TY + ly)*T.TX + lx;
+}
+
+// the tile-local cell of store slot q: the last c with st[c] <= q (st[0] <= q < st[TC])
+__device__ inline int SlotCell(const int *st, int TC, int q) {
+  int lo = 0, hi = TC;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (st[mid] <= q) lo = mid; else hi = mid;
   }
-  if (A.pending) {
-    // own cell, and the cells beside it in x2 (every target written by exactly one thread)
-    const long long i0 = Idx(g, m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
-    for (int d=0; d<3; ++d) {
-      A.dc0[i0 + d*g.ncb] = dc[1][d];
-      A.dc1[i0 + g.n1 + d*g.ncb] = dc[2][d];   // to the cell above: "from below" there
-      A.dc2[i0 - g.n1 + d*g.ncb] = dc[0][d];   // to the cell below: "from above" there
+  return lo;
+}
+
+// del of one axis about centre cell c (PMWeights, src/dust/dust.hpp:1088-1111, with the
+// block's nx/(xmax - xmin) formed once)
+__device__ inline double DelOf(double x, double xmin, double scale, int c) {
+#pragma clang fp contract(off)
+  return (x - xmin)*scale - (static_cast<double>(c) + 0.5);
+}
+
+// the three 1-D weights of PMWeights for a given del (S: 0 NGP, 1 CIC, 2 TSC)
+template <int S, typename T>
+__device__ inline void W3(T d, T w[3]) {
+  if constexpr (S == 2) {
+    const T h0 = T(0.5) - d, h2 = T(0.5) + d;
+    w[0] = T(0.5)*h0*h0;
+    w[1] = T(0.75) - d*d;
+    w[2] = T(0.5)*h2*h2;
+  } else if constexpr (S == 1) {
+    w[0] = fmax(T(0), -d); w[1] = T(1) - fabs(d); w[2] = fmax(T(0), d);
+  } else {
+    w[0] = T(0); w[1] = T(1); w[2] = T(0);
+  }
+}
+
+template <int S, typename T>
+__device__ inline T W1(int a, T d) {
+  T w[3];
+  W3<S, T>(d, w);
+  return (a == 0) ? w[0] : ((a == 1) ? w[1] : w[2]);
+}
+
+// One record's 9 terms of one slab, w_z w_y(b) w_x(a) val[n], into acc[b][a][n].  With SY (the
+// shearing box) a record centred one cell off in x2 (sj = -1 or +1: the deposit centre beyond
+// the block's x2 face in the shifted frame) moves its x2 weights by sj, and the term that leaves
+// the thread's three rows goes to spl[a][n], deposited at x2 offset 2 sj.  Branch-free: the
+// lanes of a wave may hold records of both kinds.
+template <typename T, int NC, bool SY>
+__device__ inline void AddSlab(T (&acc)[3][3][NC], T (&spl)[3][NC], int &spdir, int sj, T wz,
+                               const T wy[3], const T wx[3], const T val[NC]) {
+  T e0 = wy[0], e1 = wy[1], e2 = wy[2];
+  if constexpr (SY) {
+    const T z = T(0);
+    const T sp = (sj < 0) ? wy[0] : ((sj > 0) ? wy[2] : z);
+    e0 = (sj == 0) ? wy[0] : ((sj < 0) ? wy[1] : z);
+    e1 = (sj == 0) ? wy[1] : ((sj < 0) ? wy[2] : wy[0]);
+    e2 = (sj == 0) ? wy[2] : ((sj < 0) ? z : wy[1]);
+    if (sj != 0) spdir = sj;
+    const T wsp = wz*sp;
+#pragma unroll
+    for (int a=0; a<3; ++a) {
+      const T w = wsp*wx[a];
+#pragma unroll
+      for (int n=0; n<NC; ++n) spl[a][n] += w*val[n];
+    }
+  }
+  const T wzy[3] = {wz*e0, wz*e1, wz*e2};
+#pragma unroll
+  for (int b=0; b<3; ++b)
+#pragma unroll
+    for (int a=0; a<3; ++a) {
+      const T w = wzy[b]*wx[a];
+#pragma unroll
+      for (int n=0; n<NC; ++n) acc[b][a][n] += w*val[n];
+    }
+}
+
+// one phase's add into the LDS region (one contributor per word and phase, so the order of
+// adds to a word is the phase order): FP64 through the LDS adder (no read-back), FP32 as a read
+// and a write, which measured faster than the FP32 LDS adder on gfx950
+template <typename T>
+__device__ inline void AddLDS(T *p, T v) {
+  if constexpr (sizeof(T) == 8) atomicAdd(p, v);
+  else *p += v;
+}
+
+// the 27 phases (and 9 more for the spill terms when some cell of the tile has them): each
+// thread adds its sums at its cell + offset of the region; region cell (rx, ry, rz) is cell
+// (ci0 - 1 + rx, cj0 - 1 - SY + ry, ck0 - 1 + rz).  Ends with a barrier.
+template <typename T, int NC, bool SY>
+__device__ inline void DepositPhases(const Tiles &Tl, T *reg, int l, int slab,
+                                     const T (&acc)[3][3][NC], const T (&spl)[3][NC],
+                                     int spdir) {
+  const int lx = l % Tl.TX, ly = (l/Tl.TX) % Tl.TY, lz = l/(Tl.TX*Tl.TY);
+  const int rbase = ((lz + slab)*Tl.RY + (ly + Tl.SY))*Tl.RX + lx;
+#pragma unroll
+  for (int pc=0; pc<3; ++pc)
+#pragma unroll
+    for (int pb=0; pb<3; ++pb)
+#pragma unroll
+      for (int pa=0; pa<3; ++pa) {
+        if (slab == pc) {
+          const int rc = rbase + pb*Tl.RX + pa;
+#pragma unroll
+          for (int n=0; n<NC; ++n) AddLDS<T>(&reg[n*Tl.RN + rc], acc[pb][pa][n]);
+        }
+        __syncthreads();
+      }
+  if constexpr (SY) {
+    if (__syncthreads_or(spdir != 0)) {
+#pragma unroll
+      for (int pc=0; pc<3; ++pc)
+#pragma unroll
+        for (int pa=0; pa<3; ++pa) {
+          if (slab == pc && spdir != 0) {
+            const int rc = rbase + (1 + 2*spdir)*Tl.RX + pa;
+#pragma unroll
+            for (int n=0; n<NC; ++n) AddLDS<T>(&reg[n*Tl.RN + rc], spl[pa][n]);
+          }
+          __syncthreads();
+        }
     }
   }
 }
 
-__global__ void k_place(const int *key, const int *rank, const int *start, long long np,
-                        int *perm) {
-  GRID_STRIDE(p, np) perm[start[key[p]] + rank[p]] = static_cast<int>(p);
+// the cells of the tile in order of decreasing particle count (64 buckets; the order inside a
+// bucket is arbitrary).  hb: 64 ints of LDS.  Ends with a barrier.
+__device__ inline void OrderCells(const int *st, int TC, int *lord, int *hb) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < 64; i += nt) hb[i] = 0;
+  __syncthreads();
+  for (int l = tid; l < TC; l += nt) atomicAdd(&hb[63 - min(st[l + 1] - st[l], 63)], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int s = 0;
+    for (int i=0; i<64; ++i) {const int c = hb[i]; hb[i] = s; s += c;}
+  }
+  __syncthreads();
+  for (int l = tid; l < TC; l += nt) lord[atomicAdd(&hb[63 - min(st[l + 1] - st[l], 63)], 1)] = l;
+  __syncthreads();
+}
+
+// the tile region of a padded cell field into LDS
+template <typename T>
+__device__ inline void LoadRegion(const Geo &g, const Tiles &Tl, const TileInfo &t,
+                                  const T *f, int nvar, T *reg) {
+  const int n = nvar*Tl.RN;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int v = i/Tl.RN;
+    int rc = i - v*Tl.RN;
+    const int rx = rc % Tl.RX; rc /= Tl.RX;
+    const int ry = rc % Tl.RY;
+    const int rz = rc/Tl.RY;
+    reg[i] = f[Idx(g, t.m, nvar, v, g.ks + t.ck0 - 1 + rz, g.js + t.cj0 - 1 - Tl.SY + ry,
+                   g.is + t.ci0 - 1 + rx)];
+  }
+}
+
+template <typename T>
+__device__ inline void FlushRegion(const T *reg, T *scr, int tile, int n) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) scr[(long long)tile*n + i] = reg[i];
+}
+
+// active cell indices of tile-local cell l
+__device__ inline void CellIdx(const Tiles &Tl, const TileInfo &t, int l, int &ci, int &cj,
+                               int &ck) {
+  ci = t.ci0 + l % Tl.TX;
+  cj = t.cj0 + (l/Tl.TX) % Tl.TY;
+  ck = t.ck0 + l/(Tl.TX*Tl.TY);
+}
+
+// the tile-local cell (clamped into the tile) and the x2 shift of a particle's deposit centre,
+// from its position (the cell it is sorted into)
+template <bool SY>
+__device__ inline void PartCell(const Geo &g, const Tiles &Tl, const TileInfo &t,
+                                const DevBlock &b, const double x[3], double sh, int &ci,
+                                int &cj, int &ck, int &sj) {
+  int nju;
+  DepositCell(g, b, x, sh, ci, cj, ck, nju);
+  ci = min(max(ci, t.ci0), t.ci0 + Tl.TX - 1);
+  cj = min(max(cj, t.cj0), t.cj0 + Tl.TY - 1);
+  ck = min(max(ck, t.ck0), t.ck0 + Tl.TZ - 1);
+  sj = 0;
+  if constexpr (SY) {
+    const int nj = min(max(nju, 0), g.nx2 - 1);
+    sj = min(max(nju - nj, -1), 1);
+  }
+}
+
+//----------------------------------------------------------------------------------------
+// key pass: the pending consistent-dust correction of the previous stage (dust_pm.cpp:686-696;
+// the actual change of every particle, summed per centre cell, is recorded for the gas), then
+// every particle's cell after this stage's push and migration and its rank: a particle that
+// stays in the tile takes its rank from a count in LDS (the tile owns those cells: stay[]), one
+// that leaves it from a global count of arrivals (count[])
+__global__ void __launch_bounds__(kKeyThreads)
+k_key(KArgs A, int do_key) {
+  extern __shared__ double lds_d[];
+  const Geo &g = A.g;
+  const Tiles &Tl = A.T;
+  const int tile = TileIndex(Tl);
+  const TileInfo t = TileOf(Tl, tile);
+  const int tid = threadIdx.x, CH = blockDim.x;
+  double *rdc = lds_d;                                   // [3][CH] m delta/V
+  int *rcode = reinterpret_cast<int*>(rdc + 3*CH);       // [CH] centre row + 2 (0: none)
+  int *st = rcode + CH;                                  // [TC + 1]
+  int *lcnt = st + Tl.TC + 1;                            // [TC]
+  for (int i = tid; i <= Tl.TC; i += CH) st[i] = A.start[t.key0 + i];
+  for (int i = tid; i < Tl.TC; i += CH) lcnt[i] = 0;
+  __syncthreads();
+  const int P0 = st[0], P1 = st[Tl.TC];
+  const DevBlock &b = A.B[t.m];
+  const double ivol = b.ivol;
+  double dcs[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int c0 = P0; c0 < P1; c0 += CH) {
+    const int c1 = min(c0 + CH, P1);
+    const int q = c0 + tid;
+    if (q < c1) {
+      PState s;
+      LoadP(A.src, q, s, !A.stage1, false);
+      int code = 0;
+      double dv[3] = {0.0, 0.0, 0.0};
+      if (A.pending) {
+        const float mass = A.src.M(q);
+        int ic_, jc, kc_;
+        if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
+          int ci, cj, ck;
+          CellIdx(Tl, t, SlotCell(st, Tl.TC, q), ci, cj, ck);
+          const int dj = jc - cj;   // the centre: this cell or one beside it in x2
+          if (ic_ != ci || kc_ != ck || dj < -1 || dj > 1) {
+            dk::RecordError(A.err, DK_EHALO, q);
+          } else {
+            const float apf = static_cast<float>(A.a_prev);
+            const float cjf = apf/(A.src.TS(q) + apf);
+            const long long ic = Idx(g, t.m, 3, 0, g.ks + ck, g.js + jc, g.is + ci);
+            for (int d=0; d<3; ++d) {
+              const float vo = s.v[d];
+              const float vn = vo + cjf*A.res[ic + d*g.ncb];
+              s.v[d] = vn;
+              // the recorded rate takes the actual change, so that the history terms of dust
+              // and gas stay opposite (dust_pm.cpp:694)
+              const double delta = static_cast<double>(vn) - static_cast<double>(vo);
+              A.src.R(d, q) = A.src.R(d, q) + delta/A.a_prev;
+              A.src.V(d, q) = vn;
+              dv[d] = static_cast<double>(mass)*delta*ivol;
+            }
+            code = dj + 2;
+          }
+        }
+      }
+      if (do_key) {
+        if (!PushMigrate(A, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+        // a particle that failed to route keeps its block: its key stays valid
+        int ni, nj, nk, nju;
+        DepositCell(g, A.B[s.gid], s.x, A.sh, ni, nj, nk, nju);
+        const long long key = TileKey(Tl, s.gid, ni, nj, nk);
+        A.key[q] = static_cast<int>(key);
+        const long long lk = key - t.key0;
+        A.rank[q] = (lk >= 0 && lk < Tl.TC) ? atomicAdd(&lcnt[lk], 1)
+                                            : -1 - atomicAdd(&A.count[key], 1);
+      }
+      if (A.pending) {
+        for (int d=0; d<3; ++d) rdc[d*CH + tid] = dv[d];
+        rcode[tid] = code;
+      }
+    }
+    if (A.pending) {
+      __syncthreads();
+      if (tid < Tl.TC) {
+        const int s0 = max(st[tid], c0), s1 = min(st[tid + 1], c1);
+        for (int qq = s0; qq < s1; ++qq) {
+          const int r = qq - c0;
+          const int code = rcode[r];
+          const double d0 = rdc[r], d1 = rdc[CH + r], d2 = rdc[2*CH + r];
+          if (code == 1) {dcs[0][0] += d0; dcs[0][1] += d1; dcs[0][2] += d2;}
+          else if (code == 2) {dcs[1][0] += d0; dcs[1][1] += d1; dcs[1][2] += d2;}
+          else if (code == 3) {dcs[2][0] += d0; dcs[2][1] += d1; dcs[2][2] += d2;}
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (do_key) {
+    __syncthreads();
+    for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
+  }
+  if (A.pending && tid < Tl.TC) {
+    // own cell, and the cells beside it in x2 (every target written by exactly one thread)
+    int ci, cj, ck;
+    CellIdx(Tl, t, tid, ci, cj, ck);
+    const long long i0 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+    for (int d=0; d<3; ++d) {
+      A.dc0[i0 + d*g.ncb] = dcs[1][d];
+      A.dc1[i0 + g.n1 + d*g.ncb] = dcs[2][d];   // to the cell above: "from below" there
+      A.dc2[i0 - g.n1 + d*g.ncb] = dcs[0][d];   // to the cell below: "from above" there
+    }
+  }
+}
+
+// the count of every cell: the particles that stay in its tile plus the arrivals
+__global__ void k_total(int *count, const int *stay, long long nkeys) {
+  GRID_STRIDE(c, nkeys) count[c] += stay[c];
+}
+
+// placement: stayers first (their LDS rank), then arrivals (rank -1 - r)
+__global__ void k_place(const int *key, const int *rank, const int *start, const int *stay,
+                        long long np, int *perm) {
+  GRID_STRIDE(p, np) {
+    const int k = key[p], r = rank[p];
+    perm[start[k] + ((r >= 0) ? r : (stay[k] - 1 - r))] = static_cast<int>(p);
+  }
 }
 
 // every cell's list in ascending source slot (the stable order); long lists are queued
@@ -454,267 +732,330 @@ __global__ void __launch_bounds__(1024) k_bigsort(const int *start, int *perm, c
 }
 
 //----------------------------------------------------------------------------------------
-// add per-offset sums into the tile's LDS region, one stencil offset per phase
-template <typename T, int NC>
-__device__ inline void Phases(T (&acc)[27][NC], T *lds, const Tiles &Tl, int rx, int ry, int rz,
-                              bool active) {
-  for (int o=0; o<27; ++o) {
-    const int a = o % 3, bb = (o/3) % 3, cz = o/9;
-    if (active) {
-      const int rc = ((rz + cz - 1)*Tl.RY + (ry + bb - 1))*Tl.RX + (rx + a - 1);
-      for (int n=0; n<NC; ++n) lds[n*Tl.RN + rc] += acc[o][n];
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T>
-__device__ inline void ZeroLDS(T *lds, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = T(0);
-  __syncthreads();
-}
-
-template <typename T>
-__device__ inline void FlushLDS(const T *lds, T *scr, long long tile, int n) {
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) scr[tile*n + i] = lds[i];
-}
-
-//----------------------------------------------------------------------------------------
 // P1: push + migration into the new order, and the deposit of Q = sum m c W/V and
-// P = sum m c W v/V (dust_pm.cpp:225-274) plus the centre mass m c/V (dust_pm.cpp:660-667)
-__global__ void __launch_bounds__(kTileThreads)
+// P = sum m c W v/V (dust_pm.cpp:225-274), the centre mass m c/V (dust_pm.cpp:660-667) and
+// the push rounding residue of each cell
+template <int S, bool SY>
+__global__ void __launch_bounds__(kCellThreads)
 k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
-  extern __shared__ float ldsf[];
+  extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  int m, ci, cj, ck, lx, ly, lz;
-  TileCell(Tl, g, blockIdx.x, threadIdx.x, m, ci, cj, ck, lx, ly, lz);
-  const bool act = threadIdx.x < Tl.TX*Tl.TY*Tl.TZ;
-  ZeroLDS(ldsf, 4*Tl.RN);
-  float acc[27][4];
-  for (int o=0; o<27; ++o) for (int n=0; n<4; ++n) acc[o][n] = 0.0f;
+  const int tile = TileIndex(Tl);
+  const TileInfo t = TileOf(Tl, tile);
+  const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+  float *rw = reinterpret_cast<float*>(lds_d);           // [9][CH] weights x0-2, y0-2, z0-2
+  float *rv = rw + 9*CH;                                 // [4][CH] mu, mu v
+  float *rp = rv + 4*CH;                                 // [3][CH] m vres/V
+  int *rcode = reinterpret_cast<int*>(rp + 3*CH);        // [CH] centre | (sj + 1) << 2
+  float *reg = reinterpret_cast<float*>(rcode + CH);     // [4][RN]
+  int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
+  int *lord = st + Tl.TC + 1;                            // [TC]
+  int *hb = lord + Tl.TC;                                // [64]
+  for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+  for (int i = tid; i < 4*Tl.RN; i += NT) reg[i] = 0.0f;
+  __syncthreads();
+  OrderCells(st, Tl.TC, lord, hb);
+  const int P0 = st[0], P1 = st[Tl.TC];
+  const DevBlock &b = A.B[t.m];
+  const double ivol = b.ivol;
+  const int slab = tid/Tl.TC;
+  const int l = lord[tid - slab*Tl.TC];
+  float prs[3] = {0.0f, 0.0f, 0.0f};
   float q3[3] = {0.0f, 0.0f, 0.0f};
-  double pr3[3] = {0.0, 0.0, 0.0};   // push rounding residue of this cell's particles
-  int nshift = 0;   // particles whose deposit centre is one cell beyond the block in x2
-  int s0 = 0, s1 = 0;
-  const DevBlock &b = A.B[m];
-  const double vol = b.dx1*b.dx2*b.dx3;
-  if (act) {
-    const long long kc = CellKey(g, m, ci, cj, ck);
-    s0 = A.start[kc]; s1 = A.start[kc + 1];
-  }
-  for (int pass=0; pass<2; ++pass) {
-    if (pass == 1) {
-      // second pass only where some thread of the tile has shifted centres
-      if (!__syncthreads_or(nshift > 0)) break;
-      for (int o=0; o<27; ++o) for (int n=0; n<4; ++n) acc[o][n] = 0.0f;
-    }
-    int dshift = 0;
-    for (int q=s0; q<s1; ++q) {
+  for (int c0 = P0; c0 < P1; c0 += CH) {
+    const int c1 = min(c0 + CH, P1);
+#pragma unroll 1
+    for (int k=0; k<kPPT; ++k) {
+      const int r = tid + k*NT;
+      const int q = c0 + r;
+      if (q >= c1) break;
       const int p = A.perm[q];
       PState s;
-      LoadP(A.src, p, s, true, !A.stage1);
-      if (!PushMigrate(A, s, A.stage1 != 0)) {dk::RecordError(A.err, DK_EROUTE, p); continue;}
-      const float ts = A.src.ts[p], mass = s.parked ? 0.0f : A.src.m[p];
-      if (pass == 0) {
-        for (int d=0; d<3; ++d) {
-          A.dst.x[d][q] = s.x[d]; A.dst.x1[d][q] = s.x1[d];
-          A.dst.v[d][q] = s.v[d]; A.dst.v1[d][q] = s.v1[d];
-        }
-        A.dst.ts[q] = ts; A.dst.m[q] = mass; A.dst.gid[q] = s.gid; A.dst.orig[q] = A.src.orig[p];
+      LoadP(A.src, p, s, !A.stage1, A.atw_dt != 0.0);
+      const bool ok = PushMigrate(A, s, A.stage1 != 0);
+      const float ts = A.src.TS(p);
+      const float mass = s.parked ? 0.0f : A.src.M(p);
+      for (int d=0; d<3; ++d) {
+        A.dst.X(d, q) = s.x[d]; A.dst.X1(d, q) = s.x1[d];
+        A.dst.V(d, q) = s.v[d]; A.dst.V1(d, q) = s.v1[d];
       }
-      int ni, nj, nk, nju;
-      const DevBlock &bn = A.B[s.gid];
-      DepositCell(g, bn, s.x, A.sh, ni, nj, nk, nju);
-      if (s.gid != m || ni != ci || nj != cj || nk != ck) {
-        dk::RecordError(A.err, DK_EHALO, p); continue;
-      }
-      const double cjd = A.a_dt/(static_cast<double>(ts) + A.a_dt);
-      const float muc = static_cast<float>(static_cast<double>(mass)*cjd/vol);
-      if (pass == 0) {
-        for (int d=0; d<3; ++d) pr3[d] += static_cast<double>(mass)*s.vres[d]/vol;
-        int ic_, jc, kc_;
-        if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
-          // the centre of the unshifted position: this cell or one beside it in x2
-          if (ic_ != ci || kc_ != ck || jc - cj < -1 || jc - cj > 1) {
-            dk::RecordError(A.err, DK_EHALO, p);
-          } else {
-            q3[jc - cj + 1] += muc;
+      A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = s.gid; A.dst.ORIG(q) = A.src.ORIG(p);
+      float w9[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      float mu = 0.0f, pr[3] = {0.0f, 0.0f, 0.0f};
+      int code = 1 << 2;   // no centre, sj = 0
+      if (!ok) {
+        dk::RecordError(A.err, DK_EROUTE, p);
+      } else {
+        int ni, nj, nk, nju;
+        DepositCell(g, A.B[s.gid], s.x, A.sh, ni, nj, nk, nju);
+        // the key pass placed the particle by the same push: the cell must be its slot's
+        if (TileKey(Tl, s.gid, ni, nj, nk) != A.key[p]) {
+          dk::RecordError(A.err, DK_EHALO, p);
+        } else {
+          const float af = static_cast<float>(A.a_dt);
+          mu = mass*(af/(ts + af))*static_cast<float>(ivol);
+          for (int d=0; d<3; ++d)
+            pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
+          int ic_, jc, kc_;
+          int cen = 0;
+          if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
+            // the centre of the unshifted position: this cell or one beside it in x2
+            if (ic_ != ni || kc_ != nk || jc - nj < -1 || jc - nj > 1)
+              dk::RecordError(A.err, DK_EHALO, p);
+            else
+              cen = jc - nj + 2;
           }
+          const int sj = SY ? min(max(nju - nj, -1), 1) : 0;
+          code = cen | ((sj + 1) << 2);
+          W3<S, float>(static_cast<float>(DelOf(s.x[0], b.x1min, b.sx1, ni)), w9);
+          W3<S, float>(static_cast<float>(DelOf(s.x[1] + A.sh*s.x[0], b.x2min, b.sx2, nj + sj)),
+                       w9 + 3);
+          W3<S, float>(static_cast<float>(DelOf(s.x[2], b.x3min, b.sx3, nk)), w9 + 6);
         }
       }
-      // 0, or +-1: the deposit centre beyond the block's x2 face (frame shift)
-      const int sj = Tl.SY ? (nju - nj) : 0;
-      if (pass == 0 && sj != 0) {++nshift; continue;}
-      if (pass == 1 && sj == 0) continue;
-      if (pass == 1) dshift = sj;
-      double wx[3], wy[3], wz[3];
-      AxisW(s.x[0], b.x1min, b.x1max, g.nx1, g.deposit, ci, wx);
-      AxisW(s.x[1] + A.sh*s.x[0], b.x2min, b.x2max, g.nx2, g.deposit, cj + sj, wy);
-      AxisW(s.x[2], b.x3min, b.x3max, g.nx3, g.deposit, ck, wz);
-      const float vx = s.v[0], vy = s.v[1], vz = s.v[2];
-      for (int c=0; c<3; ++c)
-        for (int bb=0; bb<3; ++bb) {
-          const float wcb = static_cast<float>(wz[c]*wy[bb])*muc;
-          for (int a=0; a<3; ++a) {
-            const float w = wcb*static_cast<float>(wx[a]);
-            const int o = (c*3 + bb)*3 + a;
-            acc[o][0] += w;
-            acc[o][1] += w*vx;
-            acc[o][2] += w*vy;
-            acc[o][3] += w*vz;
-          }
-        }
+#pragma unroll
+      for (int i=0; i<9; ++i) rw[i*CH + r] = w9[i];
+      rv[r] = mu;
+      rv[CH + r] = mu*s.v[0]; rv[2*CH + r] = mu*s.v[1]; rv[3*CH + r] = mu*s.v[2];
+      rp[r] = pr[0]; rp[CH + r] = pr[1]; rp[2*CH + r] = pr[2];
+      rcode[r] = code;
     }
-    Phases<float, 4>(acc, ldsf, Tl, lx + 1, ly + 1 + Tl.SY + dshift, lz + 1, act);
+    __syncthreads();
+    float acc[3][3][4], spl[3][4];
+#pragma unroll
+    for (int a=0; a<3; ++a)
+#pragma unroll
+      for (int n=0; n<4; ++n) {
+        spl[a][n] = 0.0f;
+#pragma unroll
+        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+      }
+    int spdir = 0;
+    const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+    for (int qq = s0; qq < s1; ++qq) {
+      const int r = qq - c0;
+      const float wz = rw[(6 + slab)*CH + r];
+      const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
+      const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+      const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
+      const int code = rcode[r];
+      AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
+      if (slab == 0) {
+        prs[0] += rp[r]; prs[1] += rp[CH + r]; prs[2] += rp[2*CH + r];
+      } else if (slab == 1) {
+        const int cen = code & 3;
+        q3[0] += (cen == 1) ? val[0] : 0.0f;
+        q3[1] += (cen == 2) ? val[0] : 0.0f;
+        q3[2] += (cen == 3) ? val[0] : 0.0f;
+      }
+    }
+    DepositPhases<float, 4, SY>(Tl, reg, l, slab, acc, spl, spdir);
   }
-  if (act) {
-    const long long i0 = Idx(g, m, 1, 0, g.ks + ck, g.js + cj, g.is + ci);
+  int ci, cj, ck;
+  CellIdx(Tl, t, l, ci, cj, ck);
+  if (slab == 0) {
+    const long long i3 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+    for (int d=0; d<3; ++d) pres[i3 + d*g.ncb] = static_cast<double>(prs[d]);
+  } else if (slab == 1) {
+    const long long i0 = Idx(g, t.m, 1, 0, g.ks + ck, g.js + cj, g.is + ci);
     qt0[i0] = q3[1];
     qt1[i0 + g.n1] = q3[2];
     qt2[i0 - g.n1] = q3[0];
-    const long long i3 = Idx(g, m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
-    for (int d=0; d<3; ++d) pres[i3 + d*g.ncb] = pr3[d];
   }
-  FlushLDS(ldsf, scr, blockIdx.x, 4*Tl.RN);
+  FlushRegion<float>(reg, scr, tile, 4*Tl.RN);
 }
 
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
-__global__ void __launch_bounds__(kTileThreads)
+template <int S, bool SY>
+__global__ void __launch_bounds__(kCellThreads)
 k_p2(KArgs A, const float *us, float *scr) {
-  extern __shared__ float ldsf[];
+  extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  int m, ci, cj, ck, lx, ly, lz;
-  TileCell(Tl, g, blockIdx.x, threadIdx.x, m, ci, cj, ck, lx, ly, lz);
-  const bool act = threadIdx.x < Tl.TX*Tl.TY*Tl.TZ;
-  ZeroLDS(ldsf, 3*Tl.RN);
-  float acc[27][3];
-  int nshift = 0, s0 = 0, s1 = 0;
-  const DevBlock &b = A.B[m];
-  const double vol = b.dx1*b.dx2*b.dx3;
-  if (act) {
-    const long long kc = CellKey(g, m, ci, cj, ck);
-    s0 = A.start[kc]; s1 = A.start[kc + 1];
-  }
-  for (int pass=0; pass<2; ++pass) {
-    if (pass == 1 && !__syncthreads_or(nshift > 0)) break;
-    for (int o=0; o<27; ++o) for (int n=0; n<3; ++n) acc[o][n] = 0.0f;
-    int dshift = 0;
-    for (int q=s0; q<s1; ++q) {
-      const double x[3] = {A.src.x[0][q], A.src.x[1][q], A.src.x[2][q]};
-      const int jdu = CellOf(x[1] + A.sh*x[0], b.x2min, b.x2max, g.nx2);
-      const int sj = Tl.SY ? (jdu - min(max(jdu, 0), g.nx2 - 1)) : 0;
-      const int jd = cj + sj;
-      if (pass == 0 && sj != 0) {++nshift; continue;}
-      if (pass == 1 && sj == 0) continue;
-      if (pass == 1) dshift = sj;
-      double wx[3], wy[3], wz[3];
-      AxisW(x[0], b.x1min, b.x1max, g.nx1, g.deposit, ci, wx);
-      AxisW(x[1] + A.sh*x[0], b.x2min, b.x2max, g.nx2, g.deposit, jd, wy);
-      AxisW(x[2], b.x3min, b.x3max, g.nx3, g.deposit, ck, wz);
-      float ww[27];
-      for (int c=0; c<3; ++c)
-        for (int bb=0; bb<3; ++bb)
-          for (int a=0; a<3; ++a)
-            ww[(c*3 + bb)*3 + a] = static_cast<float>(wz[c]*wy[bb])*static_cast<float>(wx[a]);
+  const int tile = TileIndex(Tl);
+  const TileInfo t = TileOf(Tl, tile);
+  const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+  float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
+  float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
+  float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
+  float *rg = rw + 9*CH;                                 // [3][CH] (m c/V) G u*
+  int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1
+  int *st = rcode + CH;                                  // [TC + 1]
+  int *lord = st + Tl.TC + 1;                            // [TC]
+  int *hb = lord + Tl.TC;                                // [64]
+  for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+  for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
+  LoadRegion<float>(g, Tl, t, us, 3, us_);
+  __syncthreads();
+  OrderCells(st, Tl.TC, lord, hb);
+  const int P0 = st[0], P1 = st[Tl.TC];
+  const DevBlock &b = A.B[t.m];
+  const double ivol = b.ivol;
+  const int slab = tid/Tl.TC;
+  const int l = lord[tid - slab*Tl.TC];
+  for (int c0 = P0; c0 < P1; c0 += CH) {
+    const int c1 = min(c0 + CH, P1);
+#pragma unroll 1
+    for (int k=0; k<kPPT; ++k) {
+      const int r = tid + k*NT;
+      const int q = c0 + r;
+      if (q >= c1) break;
+      const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
+      int ci, cj, ck, sj;
+      PartCell<SY>(g, Tl, t, b, x, A.sh, ci, cj, ck, sj);
+      float wx[3], wy[3], wz[3];
+      W3<S, float>(static_cast<float>(DelOf(x[0], b.x1min, b.sx1, ci)), wx);
+      W3<S, float>(static_cast<float>(DelOf(x[1] + A.sh*x[0], b.x2min, b.sx2, cj + sj)), wy);
+      W3<S, float>(static_cast<float>(DelOf(x[2], b.x3min, b.sx3, ck)), wz);
+      const int rb = ((ck - t.ck0)*Tl.RY + (cj - t.cj0 + Tl.SY + sj))*Tl.RX + (ci - t.ci0);
       float gth[3] = {0.0f, 0.0f, 0.0f};
-      const long long ic = Idx(g, m, 3, 0, g.ks + ck, g.js + jd, g.is + ci);
+#pragma unroll
       for (int c=0; c<3; ++c)
-        for (int bb=0; bb<3; ++bb)
+#pragma unroll
+        for (int bb=0; bb<3; ++bb) {
+          const float wcb = wz[c]*wy[bb];
+          const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+#pragma unroll
           for (int a=0; a<3; ++a) {
-            const long long id = ic + ((long long)(c - 1)*g.n2 + (bb - 1))*g.n1 + (a - 1);
-            const float w = ww[(c*3 + bb)*3 + a];
-            gth[0] += w*us[id];
-            gth[1] += w*us[id + g.ncb];
-            gth[2] += w*us[id + 2*g.ncb];
+            const float w = wcb*wx[a];
+            gth[0] += w*us_[rr + a];
+            gth[1] += w*us_[Tl.RN + rr + a];
+            gth[2] += w*us_[2*Tl.RN + rr + a];
           }
-      const double cjd = A.a_dt/(static_cast<double>(A.src.ts[q]) + A.a_dt);
-      const float fac = static_cast<float>(static_cast<double>(A.src.m[q])*cjd/vol);
-      for (int d=0; d<3; ++d) gth[d] *= fac;
-      for (int o=0; o<27; ++o)
-        for (int d=0; d<3; ++d) acc[o][d] += ww[o]*gth[d];
+        }
+      const float af = static_cast<float>(A.a_dt);
+      const float fac = A.src.M(q)*(af/(A.src.TS(q) + af))*static_cast<float>(ivol);
+      rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
+      rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
+      rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
+      rg[r] = fac*gth[0]; rg[CH + r] = fac*gth[1]; rg[2*CH + r] = fac*gth[2];
+      rcode[r] = sj + 1;
     }
-    Phases<float, 3>(acc, ldsf, Tl, lx + 1, ly + 1 + Tl.SY + dshift, lz + 1, act);
+    __syncthreads();
+    float acc[3][3][3], spl[3][3];
+#pragma unroll
+    for (int a=0; a<3; ++a)
+#pragma unroll
+      for (int n=0; n<3; ++n) {
+        spl[a][n] = 0.0f;
+#pragma unroll
+        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+      }
+    int spdir = 0;
+    const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+    for (int qq = s0; qq < s1; ++qq) {
+      const int r = qq - c0;
+      const float wz = rw[(6 + slab)*CH + r];
+      const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
+      const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+      const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
+      AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+    }
+    DepositPhases<float, 3, SY>(Tl, dep, l, slab, acc, spl, spdir);
   }
-  FlushLDS(ldsf, scr, blockIdx.x, 3*Tl.RN);
+  FlushRegion<float>(dep, scr, tile, 3*Tl.RN);
 }
 
 // P3: gather u*, kick, drag rate, and the FP64 deposit of the actual impulse
 // (dust_pm.cpp:500-564)
-__global__ void __launch_bounds__(kTileThreads)
+template <int S, bool SY>
+__global__ void __launch_bounds__(kCellThreads)
 k_p3(KArgs A, const float *us, double *scr) {
-  extern __shared__ double ldsd[];
+  extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  int m, ci, cj, ck, lx, ly, lz;
-  TileCell(Tl, g, blockIdx.x, threadIdx.x, m, ci, cj, ck, lx, ly, lz);
-  const bool act = threadIdx.x < Tl.TX*Tl.TY*Tl.TZ;
-  ZeroLDS(ldsd, 3*Tl.RN);
-  double acc[27][3];
-  int nshift = 0, s0 = 0, s1 = 0;
-  const DevBlock &b = A.B[m];
-  const double vol = b.dx1*b.dx2*b.dx3;
-  if (act) {
-    const long long kc = CellKey(g, m, ci, cj, ck);
-    s0 = A.start[kc]; s1 = A.start[kc + 1];
-  }
-  for (int pass=0; pass<2; ++pass) {
-    if (pass == 1 && !__syncthreads_or(nshift > 0)) break;
-    for (int o=0; o<27; ++o) for (int n=0; n<3; ++n) acc[o][n] = 0.0;
-    int dshift = 0;
-    for (int q=s0; q<s1; ++q) {
-      const double x[3] = {A.src.x[0][q], A.src.x[1][q], A.src.x[2][q]};
-      const int jdu = CellOf(x[1] + A.sh*x[0], b.x2min, b.x2max, g.nx2);
-      const int sj = Tl.SY ? (jdu - min(max(jdu, 0), g.nx2 - 1)) : 0;
-      const int jd = cj + sj;
-      if (pass == 0 && sj != 0) {++nshift; continue;}
-      if (pass == 1 && sj == 0) continue;
-      if (pass == 1) dshift = sj;
-      double wx[3], wy[3], wz[3];
-      AxisW(x[0], b.x1min, b.x1max, g.nx1, g.deposit, ci, wx);
-      AxisW(x[1] + A.sh*x[0], b.x2min, b.x2max, g.nx2, g.deposit, jd, wy);
-      AxisW(x[2], b.x3min, b.x3max, g.nx3, g.deposit, ck, wz);
+  const int tile = TileIndex(Tl);
+  const TileInfo t = TileOf(Tl, tile);
+  const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+  double *dep = lds_d;                                   // [3][RN] the deposit (FP64)
+  double *rd = dep + 3*Tl.RN;                            // [6][CH] del x,y,z; -m delta/V
+  float *us_ = reinterpret_cast<float*>(rd + 6*CH);      // [3][RN] u*
+  int *rcode = reinterpret_cast<int*>(us_ + 3*Tl.RN);    // [CH] sj + 1
+  int *st = rcode + CH;                                  // [TC + 1]
+  int *lord = st + Tl.TC + 1;                            // [TC]
+  int *hb = lord + Tl.TC;                                // [64]
+  for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+  for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0;
+  LoadRegion<float>(g, Tl, t, us, 3, us_);
+  __syncthreads();
+  OrderCells(st, Tl.TC, lord, hb);
+  const int P0 = st[0], P1 = st[Tl.TC];
+  const DevBlock &b = A.B[t.m];
+  const double ivol = b.ivol;
+  const int slab = tid/Tl.TC;
+  const int l = lord[tid - slab*Tl.TC];
+  for (int c0 = P0; c0 < P1; c0 += CH) {
+    const int c1 = min(c0 + CH, P1);
+#pragma unroll 1
+    for (int k=0; k<kPPT; ++k) {
+      const int r = tid + k*NT;
+      const int q = c0 + r;
+      if (q >= c1) break;
+      const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
+      int ci, cj, ck, sj;
+      PartCell<SY>(g, Tl, t, b, x, A.sh, ci, cj, ck, sj);
+      const double dx = DelOf(x[0], b.x1min, b.sx1, ci);
+      const double dy = DelOf(x[1] + A.sh*x[0], b.x2min, b.sx2, cj + sj);
+      const double dz = DelOf(x[2], b.x3min, b.sx3, ck);
+      float wx[3], wy[3], wz[3];
+      W3<S, float>(static_cast<float>(dx), wx);
+      W3<S, float>(static_cast<float>(dy), wy);
+      W3<S, float>(static_cast<float>(dz), wz);
+      const int rb = ((ck - t.ck0)*Tl.RY + (cj - t.cj0 + Tl.SY + sj))*Tl.RX + (ci - t.ci0);
       float u[3] = {0.0f, 0.0f, 0.0f};
-      const long long ic = Idx(g, m, 3, 0, g.ks + ck, g.js + jd, g.is + ci);
+#pragma unroll
       for (int c=0; c<3; ++c)
+#pragma unroll
         for (int bb=0; bb<3; ++bb) {
-          const float wcb = static_cast<float>(wz[c]*wy[bb]);
+          const float wcb = wz[c]*wy[bb];
+          const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+#pragma unroll
           for (int a=0; a<3; ++a) {
-            const long long id = ic + ((long long)(c - 1)*g.n2 + (bb - 1))*g.n1 + (a - 1);
-            const float w = wcb*static_cast<float>(wx[a]);
-            u[0] += w*us[id];
-            u[1] += w*us[id + g.ncb];
-            u[2] += w*us[id + 2*g.ncb];
+            const float w = wcb*wx[a];
+            u[0] += w*us_[rr + a];
+            u[1] += w*us_[Tl.RN + rr + a];
+            u[2] += w*us_[2*Tl.RN + rr + a];
           }
         }
-      const float cj32 = static_cast<float>(A.a_dt/(static_cast<double>(A.src.ts[q]) + A.a_dt));
-      double delta[3];
+      const float af = static_cast<float>(A.a_dt);
+      const float cj32 = af/(A.src.TS(q) + af);
+      const double fac = -static_cast<double>(A.src.M(q))*ivol;
       for (int d=0; d<3; ++d) {
-        const float vo = A.src.v[d][q];
-        const float dv = cj32*(u[d] - vo);
-        const float vn = vo + dv;
-        A.src.v[d][q] = vn;
-        delta[d] = static_cast<double>(vn) - static_cast<double>(vo);
-        A.src.r[d][q] = delta[d]/A.a_dt;
+        const float vo = A.src.V(d, q);
+        const float vn = vo + cj32*(u[d] - vo);
+        A.src.V(d, q) = vn;
+        const double delta = static_cast<double>(vn) - static_cast<double>(vo);
+        A.src.R(d, q) = delta/A.a_dt;
+        rd[(3 + d)*CH + r] = fac*delta;
       }
-      const double fac = -static_cast<double>(A.src.m[q])/vol;
-      for (int c=0; c<3; ++c)
-        for (int bb=0; bb<3; ++bb) {
-          const double wcb = wz[c]*wy[bb]*fac;
-          for (int a=0; a<3; ++a) {
-            const double w = wcb*wx[a];
-            const int o = (c*3 + bb)*3 + a;
-            acc[o][0] += w*delta[0];
-            acc[o][1] += w*delta[1];
-            acc[o][2] += w*delta[2];
-          }
-        }
+      rd[r] = dx; rd[CH + r] = dy; rd[2*CH + r] = dz;
+      rcode[r] = sj + 1;
     }
-    Phases<double, 3>(acc, ldsd, Tl, lx + 1, ly + 1 + Tl.SY + dshift, lz + 1, act);
+    __syncthreads();
+    double acc[3][3][3], spl[3][3];
+#pragma unroll
+    for (int a=0; a<3; ++a)
+#pragma unroll
+      for (int n=0; n<3; ++n) {
+        spl[a][n] = 0.0;
+#pragma unroll
+        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0;
+      }
+    int spdir = 0;
+    const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+    for (int qq = s0; qq < s1; ++qq) {
+      const int r = qq - c0;
+      double wy[3], wx[3];
+      W3<S, double>(rd[CH + r], wy);
+      W3<S, double>(rd[r], wx);
+      const double wz = W1<S, double>(slab, rd[2*CH + r]);
+      const double val[3] = {rd[3*CH + r], rd[4*CH + r], rd[5*CH + r]};
+      AddSlab<double, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+    }
+    DepositPhases<double, 3, SY>(Tl, dep, l, slab, acc, spl, spdir);
   }
-  FlushLDS(ldsd, scr, blockIdx.x, 3*Tl.RN);
+  FlushRegion<double>(dep, scr, tile, 3*Tl.RN);
 }
 
 // every padded cell sums the tile regions covering it, in ascending tile order
@@ -878,30 +1219,30 @@ __global__ void k_bind(const double *r, const int32_t *iv, long long cap, long l
                        int32_t *itag) {
   GRID_STRIDE(p, np) {
     for (int d=0; d<3; ++d) {
-      S.x[d][p] = r[(long long)(DK_IPX + 2*d)*cap + p];
-      S.v[d][p] = static_cast<float>(r[(long long)(DK_IPVX + 2*d)*cap + p]);
-      S.x1[d][p] = r[(long long)(DK_IPX1 + 2*d)*cap + p];
-      S.v1[d][p] = static_cast<float>(r[(long long)(DK_IPVX1 + 2*d)*cap + p]);
-      S.r[d][p] = r[(long long)(DK_IPRX + d)*cap + p];
+      S.X(d, p) = r[(long long)(DK_IPX + 2*d)*cap + p];
+      S.V(d, p) = static_cast<float>(r[(long long)(DK_IPVX + 2*d)*cap + p]);
+      S.X1(d, p) = r[(long long)(DK_IPX1 + 2*d)*cap + p];
+      S.V1(d, p) = static_cast<float>(r[(long long)(DK_IPVX1 + 2*d)*cap + p]);
+      S.R(d, p) = r[(long long)(DK_IPRX + d)*cap + p];
     }
-    S.ts[p] = static_cast<float>(r[(long long)DK_IPTS*cap + p]);
-    S.m[p] = static_cast<float>(r[(long long)DK_IPM*cap + p]);
-    S.gid[p] = iv[(long long)DK_PGID*cap + p];
-    S.orig[p] = static_cast<int>(p);
+    S.TS(p) = static_cast<float>(r[(long long)DK_IPTS*cap + p]);
+    S.M(p) = static_cast<float>(r[(long long)DK_IPM*cap + p]);
+    S.GID(p) = iv[(long long)DK_PGID*cap + p];
+    S.ORIG(p) = static_cast<int>(p);
     for (int c=1; c<DK_NIDATA; ++c) itag[(long long)(c - 1)*np + p] = iv[(long long)c*cap + p];
   }
 }
 
 // the initial order: every particle's deposit cell at its present position
-__global__ void k_key_static(Geo g, const DevBlock *B, Store S, long long np, double sh,
-                             int *key, int *rank, int *count) {
+__global__ void k_key_static(Geo g, Tiles Tl, const DevBlock *B, Store S, long long np,
+                             double sh, int *key, int *rank, int *count) {
   GRID_STRIDE(p, np) {
-    const double x[3] = {S.x[0][p], S.x[1][p], S.x[2][p]};
+    const double x[3] = {S.X(0, p), S.X(1, p), S.X(2, p)};
     int ni, nj, nk, nju;
-    DepositCell(g, B[S.gid[p]], x, sh, ni, nj, nk, nju);
-    const long long k = CellKey(g, S.gid[p], ni, nj, nk);
+    DepositCell(g, B[S.GID(p)], x, sh, ni, nj, nk, nju);
+    const long long k = TileKey(Tl, S.GID(p), ni, nj, nk);
     key[p] = static_cast<int>(k);
-    rank[p] = atomicAdd(&count[k], 1);
+    rank[p] = -1 - atomicAdd(&count[k], 1);
   }
 }
 
@@ -909,11 +1250,11 @@ __global__ void k_permute_all(Store src, Store dst, const int *perm, long long n
   GRID_STRIDE(q, np) {
     const int p = perm[q];
     for (int d=0; d<3; ++d) {
-      dst.x[d][q] = src.x[d][p]; dst.x1[d][q] = src.x1[d][p];
-      dst.v[d][q] = src.v[d][p]; dst.v1[d][q] = src.v1[d][p]; dst.r[d][q] = src.r[d][p];
+      dst.X(d, q) = src.X(d, p); dst.X1(d, q) = src.X1(d, p);
+      dst.V(d, q) = src.V(d, p); dst.V1(d, q) = src.V1(d, p); dst.R(d, q) = src.R(d, p);
     }
-    dst.ts[q] = src.ts[p]; dst.m[q] = src.m[p]; dst.gid[q] = src.gid[p];
-    dst.orig[q] = src.orig[p];
+    dst.TS(q) = src.TS(p); dst.M(q) = src.M(p); dst.GID(q) = src.GID(p);
+    dst.ORIG(q) = src.ORIG(p);
   }
 }
 
@@ -921,16 +1262,16 @@ __global__ void k_export(Store S, long long np, const int32_t *itag, long long n
                          int32_t *iv, long long cap) {
   GRID_STRIDE(q, np) {
     for (int d=0; d<3; ++d) {
-      r[(long long)(DK_IPX + 2*d)*cap + q] = S.x[d][q];
-      r[(long long)(DK_IPVX + 2*d)*cap + q] = static_cast<double>(S.v[d][q]);
-      r[(long long)(DK_IPX1 + 2*d)*cap + q] = S.x1[d][q];
-      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = static_cast<double>(S.v1[d][q]);
-      r[(long long)(DK_IPRX + d)*cap + q] = S.r[d][q];
+      r[(long long)(DK_IPX + 2*d)*cap + q] = S.X(d, q);
+      r[(long long)(DK_IPVX + 2*d)*cap + q] = static_cast<double>(S.V(d, q));
+      r[(long long)(DK_IPX1 + 2*d)*cap + q] = S.X1(d, q);
+      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = static_cast<double>(S.V1(d, q));
+      r[(long long)(DK_IPRX + d)*cap + q] = S.R(d, q);
     }
-    r[(long long)DK_IPTS*cap + q] = static_cast<double>(S.ts[q]);
-    r[(long long)DK_IPM*cap + q] = static_cast<double>(S.m[q]);
-    iv[(long long)DK_PGID*cap + q] = S.gid[q];
-    const int o = S.orig[q];
+    r[(long long)DK_IPTS*cap + q] = static_cast<double>(S.TS(q));
+    r[(long long)DK_IPM*cap + q] = static_cast<double>(S.M(q));
+    iv[(long long)DK_PGID*cap + q] = S.GID(q);
+    const int o = S.ORIG(q);
     for (int c=1; c<DK_NIDATA; ++c) iv[(long long)c*cap + q] = itag[(long long)(c - 1)*np0 + o];
   }
 }
@@ -942,8 +1283,8 @@ __global__ void k_fdiag(Geo g, const DevBlock *B, Store S, long long np, const d
   __shared__ double sh[kBlock][9];
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   GRID_STRIDE(p, np) {
-    const double m = S.m[p];
-    const double vx = S.v[0][p], vy = S.v[1][p], vz = S.v[2][p];
+    const double m = S.M(p);
+    const double vx = S.V(0, p), vy = S.V(1, p), vz = S.V(2, p);
     acc[0] += m*vx; acc[1] += m*vy; acc[2] += m*vz;
     acc[6] += m;
     acc[8] += m*(fabs(vx) + fabs(vy) + fabs(vz));
@@ -989,6 +1330,59 @@ int FillV(dk_ctx *c, T *a, long long n, T v) {
   return dk::Check(hipGetLastError());
 }
 
+// dynamic LDS of the tile kernels (NT = 3 TC threads, CH = kPPT NT records)
+size_t LdsKey(const Tiles &T) {
+  return (size_t)kKeyThreads*(3*sizeof(double) + sizeof(int)) + sizeof(int)*(2*T.TC + 1);
+}
+size_t LdsTail(const Tiles &T) { return sizeof(int)*(2*T.TC + 1 + 64); }
+size_t LdsP1(const Tiles &T) {
+  const size_t ch = (size_t)kPPT*3*T.TC;
+  return ch*(16*sizeof(float) + sizeof(int)) + sizeof(float)*4*T.RN + LdsTail(T);
+}
+size_t LdsP2(const Tiles &T) {
+  const size_t ch = (size_t)kPPT*3*T.TC;
+  return sizeof(float)*6*T.RN + ch*(12*sizeof(float) + sizeof(int)) + LdsTail(T);
+}
+size_t LdsP3(const Tiles &T) {
+  const size_t ch = (size_t)kPPT*3*T.TC;
+  return sizeof(double)*3*T.RN + ch*(6*sizeof(double) + sizeof(int)) +
+         sizeof(float)*3*T.RN + LdsTail(T);
+}
+
+// a tile kernel instantiated for the deposit scheme and the frame-shift flag of the context,
+// launched with grid, blk, lds in scope
+#define SWITCH_SCHEME(KERNEL, ARGS) \
+  switch (c->g.deposit*2 + c->fast->T.SY) { \
+    case 5: KERNEL<2, true><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 4: KERNEL<2, false><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 3: KERNEL<1, true><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 2: KERNEL<1, false><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 1: KERNEL<0, true><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    default: KERNEL<0, false><<<grid, blk, lds, c->stream>>> ARGS; break; \
+  }
+
+void LaunchP1(dk_ctx *c, const KArgs &A, float *qt0, float *qt1, float *qt2, double *pres,
+              float *scr) {
+  const Tiles &T = c->fast->T;
+  const dim3 grid(T.nt), blk(3*T.TC);
+  const size_t lds = LdsP1(T);
+  SWITCH_SCHEME(k_p1, (A, qt0, qt1, qt2, pres, scr));
+}
+
+void LaunchP2(dk_ctx *c, const KArgs &A, const float *us, float *scr) {
+  const Tiles &T = c->fast->T;
+  const dim3 grid(T.nt), blk(3*T.TC);
+  const size_t lds = LdsP2(T);
+  SWITCH_SCHEME(k_p2, (A, us, scr));
+}
+
+void LaunchP3(dk_ctx *c, const KArgs &A, const float *us, double *scr) {
+  const Tiles &T = c->fast->T;
+  const dim3 grid(T.nt), blk(3*T.TC);
+  const size_t lds = LdsP3(T);
+  SWITCH_SCHEME(k_p3, (A, us, scr));
+}
+
 #define TIMED(label, launch) \
   do { const int tk_ = dk::ProfBegin(c, label); launch; dk::ProfEnd(c, tk_); } while (0)
 
@@ -1022,6 +1416,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.g = c->g; A.T = f->T; A.B = c->d_blocks;
   A.src = f->s[f->cur]; A.dst = f->s[1 - f->cur];
   A.perm = f->perm; A.start = f->start; A.key = f->key; A.rank = f->rank; A.count = f->count;
+  A.stay = f->stay;
   A.err = c->d_err;
   A.sbox = c->P.shearing_box; A.strat = c->P.stratified;
   A.shear = (c->M.bc_x1 == DK_X1_SHEAR);
@@ -1035,11 +1430,13 @@ KArgs MakeArgs(dk_ctx *c) {
 int SortKeys(dk_ctx *c) {
   Fast *f = c->fast;
   int e;
+  TIMED("total", (k_total<<<Grid(f->nkeys), kBlock, 0, c->stream>>>(f->count, f->stay,
+                                                                     f->nkeys)));
   TIMED("scan", e = dk::Check(hipcub::DeviceScan::ExclusiveSum(
       f->scan_tmp, f->scan_bytes, f->count, f->start, (int)(f->nkeys + 1), c->stream)));
   if (e) return e;
   TIMED("place", (k_place<<<Grid(f->np), kBlock, 0, c->stream>>>(f->key, f->rank, f->start,
-                                                                   f->np, f->perm)));
+                                                                   f->stay, f->np, f->perm)));
   if ((e = FillV<int>(c, f->big, 1, 0))) return e;
   TIMED("segsort", (k_segsort<<<Grid(f->nkeys), kBlock, 0, c->stream>>>(f->start, f->nkeys,
                                                                          f->perm, f->big)));
@@ -1053,7 +1450,7 @@ int Flush(dk_ctx *c) {
   Fast *f = c->fast;
   if (!f->pending) return DK_OK;
   KArgs A = MakeArgs(c);
-  TIMED("key", (k_key<<<f->T.nt, kTileThreads, 0, c->stream>>>(A, 0)));
+  TIMED("key", (k_key<<<f->T.nt, kKeyThreads, LdsKey(f->T), c->stream>>>(A, 0)));
   dk_gas_src z;
   std::memset(&z, 0, sizeof(z));
   const long long nact = (long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3;
@@ -1075,12 +1472,28 @@ int FastCreate(dk_ctx *c) {
   Fast *f = new Fast();
   c->fast = f;
   Tiles &T = f->T;
-  T.TX = Largest(c->g.nx1, 16);
+  // tile shape: 8 x 4 x 4 cells by default (or the largest divisors of the block size below
+  // those); DK_TILE=AxBxC selects another shape whose sides divide the block and whose cell
+  // count is at most kMaxTileCells
+  T.TX = Largest(c->g.nx1, 8);
   T.TY = Largest(c->g.nx2, 4);
   T.TZ = Largest(c->g.nx3, 4);
+  if (const char *e = std::getenv("DK_TILE")) {
+    int a = 0, b = 0, d = 0;
+    if (std::sscanf(e, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 &&
+        c->g.nx1 % a == 0 && c->g.nx2 % b == 0 && c->g.nx3 % d == 0 &&
+        a*b*d <= kMaxTileCells) {
+      T.TX = a; T.TY = b; T.TZ = d;
+    }
+  }
+  T.TC = T.TX*T.TY*T.TZ;
   T.SY = (c->P.shearing_box) ? 1 : 0;
   T.RX = T.TX + 2; T.RY = T.TY + 2 + 2*T.SY; T.RZ = T.TZ + 2;
   T.RN = T.RX*T.RY*T.RZ;
+  // every tile kernel's LDS within 64 KiB
+  if (std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)) > 65536 - 1024) return DK_EUNSUPPORTED;
+  // a block's first and last x2 rows must differ (cells centred one row off in x2)
+  if (T.SY && c->g.nx2 < 2) return DK_EUNSUPPORTED;
   T.ntx = c->g.nx1/T.TX; T.nty = c->g.nx2/T.TY; T.ntz = c->g.nx3/T.TZ;
   T.ntb = T.ntx*T.nty*T.ntz;
   T.nt = T.ntb*c->g.nmb;
@@ -1091,6 +1504,7 @@ int FastCreate(dk_ctx *c) {
   e = e ? e : Alloc((void**)&f->count, sizeof(int)*(f->nkeys + 1));
   e = e ? e : Alloc((void**)&f->start, sizeof(int)*(f->nkeys + 1));
   e = e ? e : Alloc((void**)&f->big, sizeof(int)*(f->nkeys + 1));
+  e = e ? e : Alloc((void**)&f->stay, sizeof(int)*(f->nkeys + 1));
   e = e ? e : Alloc((void**)&f->qdep, sizeof(float)*4*nc);
   for (int q=0; q<3; ++q) e = e ? e : Alloc((void**)&f->qt[q], sizeof(float)*nc);
   e = e ? e : Alloc((void**)&f->ustar, sizeof(float)*3*nc);
@@ -1121,15 +1535,11 @@ void FastDestroy(dk_ctx *c) {
   if (f == nullptr) return;
   for (int b=0; b<2; ++b) {
     Store &S = f->s[b];
-    for (int d=0; d<3; ++d)
-      for (void *p : {(void*)S.x[d], (void*)S.x1[d], (void*)S.v[d], (void*)S.v1[d],
-                      (void*)S.r[d]})
-        if (p) (void)hipFree(p);
-    for (void *p : {(void*)S.ts, (void*)S.m, (void*)S.gid, (void*)S.orig})
+    for (void *p : {(void*)S.d, (void*)S.f, (void*)S.i})
       if (p) (void)hipFree(p);
   }
   for (void *p : {(void*)f->key, (void*)f->rank, (void*)f->count, (void*)f->start,
-                  (void*)f->perm, (void*)f->big, f->scan_tmp, (void*)f->itag, (void*)f->qdep,
+                  (void*)f->perm, (void*)f->big, (void*)f->stay, f->scan_tmp, (void*)f->itag, (void*)f->qdep,
                   (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom, (void*)f->pres,
                   (void*)f->scr_f, (void*)f->scr_d})
     if (p) (void)hipFree(p);
@@ -1148,22 +1558,17 @@ int FastBind(dk_ctx *c) {
   if (c->np > f->cap) {
     // (re)allocate the store for the new capacity
     const long long n = std::max<long long>(c->np, 1);
+    const long long ng = (n + kGrp - 1)/kGrp*kGrp;
     for (int b=0; b<2; ++b) {
       Store &S = f->s[b];
-      for (int d=0; d<3; ++d) {
-        for (void **p : {(void**)&S.x[d], (void**)&S.x1[d], (void**)&S.r[d]}) {
-          if (*p) (void)hipFree(*p);
-          if (Alloc(p, sizeof(double)*n)) return DK_ENOMEM;
-        }
-        for (void **p : {(void**)&S.v[d], (void**)&S.v1[d]}) {
-          if (*p) (void)hipFree(*p);
-          if (Alloc(p, sizeof(float)*n)) return DK_ENOMEM;
-        }
-      }
-      for (void **p : {(void**)&S.ts, (void**)&S.m, (void**)&S.gid, (void**)&S.orig}) {
+      for (void **p : {(void**)&S.d, (void**)&S.f, (void**)&S.i}) {
         if (*p) (void)hipFree(*p);
-        if (Alloc(p, 4*n)) return DK_ENOMEM;
+        *p = nullptr;
       }
+      if (Alloc((void**)&S.d, sizeof(double)*9*ng)) return DK_ENOMEM;
+      if (Alloc((void**)&S.f, sizeof(float)*8*ng)) return DK_ENOMEM;
+      if (Alloc((void**)&S.i, sizeof(int)*2*ng)) return DK_ENOMEM;
+      S.cap = ng;
     }
     for (void **p : {(void**)&f->key, (void**)&f->rank, (void**)&f->perm}) {
       if (*p) (void)hipFree(*p);
@@ -1181,8 +1586,9 @@ int FastBind(dk_ctx *c) {
                                                  f->s[0], f->itag);
   int e;
   if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
-  k_key_static<<<Grid(f->np), kBlock, 0, c->stream>>>(c->g, c->d_blocks, f->s[0], f->np, 0.0,
-                                                      f->key, f->rank, f->count);
+  if ((e = FillV<int>(c, f->stay, f->nkeys + 1, 0))) return e;
+  k_key_static<<<Grid(f->np), kBlock, 0, c->stream>>>(c->g, f->T, c->d_blocks, f->s[0], f->np,
+                                                      0.0, f->key, f->rank, f->count);
   if ((e = SortKeys(c))) return e;
   k_permute_all<<<Grid(f->np), kBlock, 0, c->stream>>>(f->s[0], f->s[1], f->perm, f->np);
   f->cur = 1;
@@ -1242,7 +1648,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   }
   // 1. key pass (with the pending correction), histogram
   if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
-  TIMED("key", (k_key<<<T.nt, kTileThreads, 0, c->stream>>>(A, 1)));
+  TIMED("key", (k_key<<<T.nt, kKeyThreads, LdsKey(T), c->stream>>>(A, 1)));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history
   {
     const int hook = 1;
@@ -1259,10 +1665,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   // 3. counting sort
   if ((e = SortKeys(c))) return e;
   // 4. push into the new order + deposit Q, P, qt
-  const size_t lds4 = sizeof(float)*4*T.RN, lds3 = sizeof(float)*3*T.RN;
-  const size_t ldsd = sizeof(double)*3*T.RN;
-  TIMED("p1_push_deposit", (k_p1<<<T.nt, kTileThreads, lds4, c->stream>>>(
-      A, f->qt[0], f->qt[1], f->qt[2], f->pres, f->scr_f)));
+  TIMED("p1_push_deposit", LaunchP1(c, A, f->qt[0], f->qt[1], f->qt[2], f->pres, f->scr_f));
   f->cur = 1 - f->cur;
   A.src = f->s[f->cur];
   A.dst = f->s[1 - f->cur];
@@ -1274,7 +1677,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   const int nsweeps = (c->P.solver == DK_DC2) ? 2 : ((c->P.solver == DK_DC1) ? 1 : 0);
   for (int sw=0; sw<nsweeps; ++sw) {
     if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
-    TIMED("p2_operator", (k_p2<<<T.nt, kTileThreads, lds3, c->stream>>>(A, f->ustar, f->scr_f)));
+    TIMED("p2_operator", LaunchP2(c, A, f->ustar, f->scr_f));
     TIMED("combine", (k_combine<float><<<Grid(NCells(c)), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, 3, f->ap)));
     if ((e = FoldT<float>(c, f->ap, 3, st->time))) return e;
@@ -1282,8 +1685,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   }
   if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
   // 6. gather-kick + impulse deposit
-  TIMED("p3_gather_kick", (k_p3<<<T.nt, kTileThreads, ldsd, c->stream>>>(A, f->ustar,
-                                                                         f->scr_d)));
+  TIMED("p3_gather_kick", LaunchP3(c, A, f->ustar, f->scr_d));
   TIMED("combine", (k_combine<double><<<Grid(NCells(c)), kBlock, 0, c->stream>>>(
       g, T, f->scr_d, 3, f->dmom)));
   if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;

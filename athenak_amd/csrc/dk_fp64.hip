@@ -955,6 +955,10 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     d.dx1 = (b.x1max - b.x1min)/static_cast<double>(g.nx1);   // meshblock.cpp:113-118
     d.dx2 = (b.x2max - b.x2min)/static_cast<double>(g.nx2);
     d.dx3 = (b.x3max - b.x3min)/static_cast<double>(g.nx3);
+    d.sx1 = static_cast<double>(g.nx1)/(b.x1max - b.x1min);
+    d.sx2 = static_cast<double>(g.nx2)/(b.x2max - b.x2min);
+    d.sx3 = static_cast<double>(g.nx3)/(b.x3max - b.x3min);
+    d.ivol = 1.0/(d.dx1*d.dx2*d.dx3);
     d.lx1 = b.lx1; d.lx2 = b.lx2; d.lx3 = b.lx3; d.pad = 0;
     for (int q=0; q<27; ++q) d.nghbr[q] = b.nghbr[q];
   }

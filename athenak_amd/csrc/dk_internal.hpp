@@ -29,6 +29,8 @@ struct Geo {
 struct DevBlock {
   double x1min, x1max, x2min, x2max, x3min, x3max;
   double dx1, dx2, dx3;
+  double sx1, sx2, sx3;    // nx/(xmax - xmin) of each axis, as PMWeights forms it
+  double ivol;             // 1/(dx1 dx2 dx3)
   int lx1, lx2, lx3, pad;
   int nghbr[27];
 };
@@ -108,11 +110,7 @@ struct dk_ctx {
   // the FP32 form (dk_fp32.hip)
   struct Fast *fast = nullptr;
   int graphs = 0;
-  // dk_mesh.hip: ghost fill (copy, shear remap, x3 values) and additive ghost fold
-template <typename T> int FillGhostsT(dk_ctx *c, T *a, int nvar, bool velocity, double t);
-template <typename T> int FoldT(dk_ctx *c, T *a, int nvar, double t);
-
-// per-kernel event timing
+  // per-kernel event timing
   struct Prof *prof = nullptr;
 };
 

@@ -159,42 +159,86 @@ __global__ void k_fold_x3(Geo g, const DevBlock *B, T *a, int nvar) {
   }
 }
 
-// same-level neighbours: every active edge cell sums the neighbours' ghost strips in
-// NeighborIndex order (src/mesh/nghbr_index.hpp:19-54); the strips that cross a shear-periodic
-// x1 face are left to k_fold_shear
+// one of e = min(2 ng, n) edge layers of an axis of n cells: the first ng, then the last ones
+__device__ inline int EdgeLayer(int t, int n, int ng, int e) {
+  return (t < ng) ? t : (n - e + t);
+}
+
+// the s-th cell (0 <= s < ShellCount) of the active cells within ng of a block face: the x3
+// edge layers, then the x2 edge rows of the other layers, then the x1 edge cells of the rest
+__host__ __device__ inline long long ShellCount(const Geo &g) {
+  const int e1 = min(2*g.ng, g.nx1), e2 = min(2*g.ng, g.nx2), e3 = min(2*g.ng, g.nx3);
+  return (long long)e3*g.nx2*g.nx1 + (long long)(g.nx3 - e3)*e2*g.nx1 +
+         (long long)(g.nx3 - e3)*(g.nx2 - e2)*e1;
+}
+
+__device__ inline void ShellCell(const Geo &g, long long s, int &i, int &j, int &k) {
+  const int e1 = min(2*g.ng, g.nx1), e2 = min(2*g.ng, g.nx2), e3 = min(2*g.ng, g.nx3);
+  const long long a = (long long)e3*g.nx2*g.nx1;
+  const long long b = (long long)(g.nx3 - e3)*e2*g.nx1;
+  if (s < a) {
+    const int t = static_cast<int>(s/((long long)g.nx2*g.nx1));
+    const long long r = s - (long long)t*g.nx2*g.nx1;
+    k = EdgeLayer(t, g.nx3, g.ng, e3);
+    j = static_cast<int>(r/g.nx1);
+    i = static_cast<int>(r - (long long)j*g.nx1);
+  } else if (s < a + b) {
+    const long long r0 = s - a;
+    const int t = static_cast<int>(r0/((long long)e2*g.nx1));
+    const long long r = r0 - (long long)t*e2*g.nx1;
+    k = g.ng + t;
+    j = EdgeLayer(static_cast<int>(r/g.nx1), g.nx2, g.ng, e2);
+    i = static_cast<int>(r % g.nx1);
+  } else {
+    const long long r0 = s - a - b;
+    const int t = static_cast<int>(r0/((long long)(g.nx2 - e2)*e1));
+    const long long r = r0 - (long long)t*(g.nx2 - e2)*e1;
+    k = g.ng + t;
+    j = g.ng + static_cast<int>(r/e1);
+    i = EdgeLayer(static_cast<int>(r % e1), g.nx1, g.ng, e1);
+  }
+  i += g.is; j += g.js; k += g.ks;
+}
+
+// same-level neighbours: every active cell within ng of a block face sums the neighbours'
+// ghost strips in NeighborIndex order (src/mesh/nghbr_index.hpp:19-54), one thread per such
+// cell.  A strip of direction (ox1, ox2, ox3) covers the cell when each non-zero ox puts the cell
+// in the first (ox < 0) or last (ox > 0) ng layers of that axis.  The strips that cross a
+// shear-periodic x1 face are left to k_fold_shear.
 template <typename T>
 __global__ void k_fold(Geo g, const DevBlock *B, const int *slot_dir, T *a, int nvar) {
-  const long long nrow = (long long)g.nmb*g.nx3*g.nx2;
-  const int ng1 = g.ng - 1;
+  __shared__ int sdir[26];
+  if (threadIdx.x < 26) sdir[threadIdx.x] = slot_dir[threadIdx.x];
+  __syncthreads();
+  const long long nsh = ShellCount(g);
+  const long long n = nsh*g.nmb;
   const bool shear = (g.bc_x1 == DK_X1_SHEAR);
-  GRID_STRIDE(t, nrow) {
-    long long rr = t;
-    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
-    const int k = g.ks + static_cast<int>(rr % g.nx3);
-    const int m = static_cast<int>(rr/g.nx3);
-    const bool edge = (k < g.ks + g.ng) || (k > g.ke - g.ng) || (j < g.js + g.ng) ||
-                      (j > g.je - g.ng);
-    for (int i=g.is; i<=g.ie; ++i) {
-      if (!edge && i >= g.is + g.ng && i <= g.ie - g.ng) continue;
-      for (int v=0; v<nvar; ++v) {
-        T sum = a[Idx(g,m,nvar,v,k,j,i)];
-        for (int s=0; s<26; ++s) {
-          const int d = slot_dir[s];
-          const int ox1 = d % 3 - 1, ox2 = (d/3) % 3 - 1, ox3 = d/9 - 1;
-          const int nb = B[m].nghbr[d];
-          if (nb < 0) continue;
-          if (shear && ox1 != 0) {
-            const bool crosses = (ox1 < 0) ? (B[m].lx1 == 0) : (B[m].lx1 == g.nmbx1 - 1);
-            if (crosses) continue;
-          }
-          const int il = (ox1 > 0) ? (g.ie - ng1) : g.is, iu = (ox1 < 0) ? (g.is + ng1) : g.ie;
-          const int jl = (ox2 > 0) ? (g.je - ng1) : g.js, ju = (ox2 < 0) ? (g.js + ng1) : g.je;
-          const int kl = (ox3 > 0) ? (g.ke - ng1) : g.ks, ku = (ox3 < 0) ? (g.ks + ng1) : g.ke;
-          if (i < il || i > iu || j < jl || j > ju || k < kl || k > ku) continue;
-          sum += a[Idx(g,nb,nvar,v,k-ox3*g.nx3,j-ox2*g.nx2,i-ox1*g.nx1)];
+  GRID_STRIDE(t, n) {
+    const int m = static_cast<int>(t/nsh);
+    int i, j, k;
+    ShellCell(g, t - (long long)m*nsh, i, j, k);
+    const DevBlock &bm = B[m];
+    // the allowed offsets per axis: bit (o + 1) set when offset o covers this cell
+    const int a1 = 2 | ((i < g.is + g.ng) ? 1 : 0) | ((i > g.ie - g.ng) ? 4 : 0);
+    const int a2 = 2 | ((j < g.js + g.ng) ? 1 : 0) | ((j > g.je - g.ng) ? 4 : 0);
+    const int a3 = 2 | ((k < g.ks + g.ng) ? 1 : 0) | ((k > g.ke - g.ng) ? 4 : 0);
+    for (int v=0; v<nvar; ++v) {
+      const long long it = Idx(g, m, nvar, v, k, j, i);
+      T sum = a[it];
+      for (int s=0; s<26; ++s) {
+        const int d = sdir[s];
+        const int o1 = d % 3, o2 = (d/3) % 3, o3 = d/9;
+        if (!((a1 >> o1) & 1) || !((a2 >> o2) & 1) || !((a3 >> o3) & 1)) continue;
+        const int nb = bm.nghbr[d];
+        if (nb < 0) continue;
+        if (shear && o1 != 1) {
+          const bool crosses = (o1 == 0) ? (bm.lx1 == 0) : (bm.lx1 == g.nmbx1 - 1);
+          if (crosses) continue;
         }
-        a[Idx(g,m,nvar,v,k,j,i)] = sum;
+        sum += a[Idx(g, nb, nvar, v, k - (o3 - 1)*g.nx3, j - (o2 - 1)*g.nx2,
+                     i - (o1 - 1)*g.nx1)];
       }
+      a[it] = sum;
     }
   }
 }
@@ -312,7 +356,7 @@ int FoldT(dk_ctx *c, T *a, int nvar, double t) {
     k_fold_x3<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar);
   }
   {const int tk = ProfBegin(c, "fold");
-   k_fold<T><<<Grid((long long)g.nmb*g.nx3*g.nx2), kBlock, 0, c->stream>>>(
+   k_fold<T><<<Grid(ShellCount(g)*g.nmb), kBlock, 0, c->stream>>>(
        g, c->d_blocks, c->d_slot_dir, a, nvar);
    ProfEnd(c, tk);}
   if (c->M.bc_x1 == DK_X1_SHEAR) {
