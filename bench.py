@@ -160,7 +160,33 @@ def cpu_baseline(name, budget_s):
                       % (name, w.mesh.nmb, nx, w.npart, steps, el)}
 
 
-def kernel_roofline(r, w):
+# measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of
+# known size) of the C2 FP32 run, committed under profiles/; timer label -> kernel name prefix
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "hbm_traffic_c2_fp32.md")
+TRAFFIC_KERNEL = {"p1_push_deposit": "k_p1", "p2_operator": "k_p2", "p3_gather_kick": "k_p3",
+                  "key": "k_key"}
+
+
+def measured_traffic(label, w, form):
+    if w.name != "c2" or form != "fp32" or label not in TRAFFIC_KERNEL:
+        return None
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    want = TRAFFIC_KERNEL[label]
+    for ln in lines:
+        parts = ln.split()
+        if len(parts) < 3 or parts[0] != "traffic":
+            continue
+        name = " ".join(parts[1:-1])
+        if name == want or name.startswith(want + "<"):
+            return float(parts[-1])
+    return None
+
+
+def kernel_roofline(r, w, form="fp32"):
     label, (tot, cnt) = max(r["prof"].items(), key=lambda kv: kv[1][0])
     per_p, per_c = KERNEL_BYTES.get(label, (0, 0))
     alg = per_p * w.npart + per_c * w.ncells
@@ -169,7 +195,8 @@ def kernel_roofline(r, w):
     return {"bound": "hbm", "kernel": label,
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": measured_traffic(label, w, form),
             "avg_launch_ms": round(tot / cnt, 4), "alg_bytes_per_launch": alg,
             "share_of_step": round(tot / r["ms"], 3)}
 
@@ -223,7 +250,7 @@ def main():
         "dtype": "f64" if main_form == "fp64" else "f32 (f64 positions, gas momentum, sums)",
         "data": "synthetic (seeded hash draws, generated on the device)",
         "config": dict(describe(w), form=main_form),
-        "roofline": kernel_roofline(r, w),
+        "roofline": kernel_roofline(r, w, main_form),
         "step_roofline": {"alg_bytes_per_particle_step": step_alg,
                           "achieved": round(mps * 1e6 * step_alg / 1e9, 1), "unit": "GB/s",
                           "frac": round(mps * 1e6 * step_alg / 1e9 / HBM_PEAK_GBS, 4)},
@@ -238,7 +265,7 @@ def main():
         out["fp64_form"] = {"value": round(mps64, 3), "ms_per_step": round(ms64, 3),
                             "step_roofline_frac": round(
                                 mps64 * 1e6 * balg_step("fp64", w.ppc) / 1e9 / HBM_PEAK_GBS, 4),
-                            "roofline": kernel_roofline(res["fp64"], w),
+                            "roofline": kernel_roofline(res["fp64"], w, "fp64"),
                             "momentum_drift": res["fp64"]["drift"]}
         out["speedup_vs_fp64"] = round(mps / mps64, 3)
         out["max_rel_velocity_err_vs_fp64"] = verr
