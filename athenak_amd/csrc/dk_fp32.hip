@@ -79,7 +79,9 @@ struct Store {
 // a block's x2 face: the orbital-advection frame shift of the 3-D shearing box).
 struct Tiles {
   int TX, TY, TZ, TC, SY, RX, RY, RZ, RN;
+  int LX, LY;                       // log2 TX, log2 TY (the tile sides are powers of 2)
   int ntx, nty, ntz, ntb, nt;
+  float iRX, iRY;                   // 1/RX, 1/RY for FDiv
 };
 
 }  // namespace
@@ -120,10 +122,11 @@ inline int Grid(long long n) {
   return static_cast<int>(std::min<long long>(b, 65536));
 }
 
+// the largest power of two <= cap that divides n
 int Largest(int n, int cap) {
-  for (int t = std::min(n, cap); t >= 1; --t)
-    if (n % t == 0) return t;
-  return 1;
+  int t = 1;
+  while (2*t <= cap && n % (2*t) == 0) t *= 2;
+  return t;
 }
 
 // all numbers a tile kernel needs, by value
@@ -140,6 +143,8 @@ struct KArgs {
   double ysh, dsh;                  // shear crossing offsets (bvals_part.cpp, dust.cpp)
   int stage1, sbox, strat, shear;
   double qs, om;
+  double c_fx, c_fy, c_fz, c_sh;    // 2 om, -(2 - q) om, -om^2 (stratified), q om; 0 without
+  double lxm, lym, lzm;             // mesh lengths
   // pending correction
   int pending;
   double a_prev;
@@ -173,46 +178,43 @@ __device__ inline int BlockOff(double x, double lo, double hi, double l) {
   return static_cast<int>((x - lo + l)/l) - 1;
 }
 
+// VEL: also the new velocity (the key pass needs the position only)
+template <bool VEL = true>
 __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
 #pragma clang fp contract(off)
   const Geo &g = A.g;
   const double xo = s.x[0], yo = s.x[1], zo = s.x[2];
   const float vxo = s.v[0], vyo = s.v[1], vzo = s.v[2];
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  if (A.sbox) {
-    fx = static_cast<float>(2.0*A.om)*vyo;
-    fy = static_cast<float>(-(2.0-A.qs)*A.om)*vxo;
-    if (A.strat) fz = static_cast<float>(-(A.om*A.om)*zo);
-  }
   if (write_regs) {   // stage 1: (x1, v1) <- (x, v) (dust_push.cpp:69-77)
     for (int d=0; d<3; ++d) {s.x1[d] = s.x[d]; s.v1[d] = s.v[d];}
   }
   // the new velocity in FP64 from the stored FP32 values, rounded once to FP32; the rounding
-  // residue goes to the gas (momentum-conserving rounding)
-  const double fxd = A.sbox ? 2.0*A.om*static_cast<double>(vyo) : 0.0;
-  const double fyd = A.sbox ? -(2.0-A.qs)*A.om*static_cast<double>(vxo) : 0.0;
-  const double fzd = (A.sbox && A.strat) ? -(A.om*A.om)*zo : 0.0;
-  (void)fx; (void)fy; (void)fz;
-  const double vn[3] = {
-      A.gam0*vxo + A.gam1*s.v1[0] + A.beta_dt*fxd + A.atw_dt*s.r[0],
-      A.gam0*vyo + A.gam1*s.v1[1] + A.beta_dt*fyd + A.atw_dt*s.r[1],
-      A.gam0*vzo + A.gam1*s.v1[2] + A.beta_dt*fzd + A.atw_dt*s.r[2]};
-  for (int d=0; d<3; ++d) {
-    s.v[d] = static_cast<float>(vn[d]);
-    s.vres[d] = vn[d] - static_cast<double>(s.v[d]);
+  // residue goes to the gas (momentum-conserving rounding).  Forces: Coriolis and tidal terms
+  // of the shearing box (zero coefficients without it)
+  if constexpr (VEL) {
+    const double fxd = A.c_fx*static_cast<double>(vyo);
+    const double fyd = A.c_fy*static_cast<double>(vxo);
+    const double fzd = A.c_fz*zo;
+    const double vn[3] = {
+        A.gam0*vxo + A.gam1*s.v1[0] + A.beta_dt*fxd + A.atw_dt*s.r[0],
+        A.gam0*vyo + A.gam1*s.v1[1] + A.beta_dt*fyd + A.atw_dt*s.r[1],
+        A.gam0*vzo + A.gam1*s.v1[2] + A.beta_dt*fzd + A.atw_dt*s.r[2]};
+    for (int d=0; d<3; ++d) {
+      s.v[d] = static_cast<float>(vn[d]);
+      s.vres[d] = vn[d] - static_cast<double>(s.v[d]);
+    }
   }
-  double ydot = vyo;
-  if (A.sbox) ydot -= A.qs*A.om*xo;
+  const double ydot = static_cast<double>(vyo) - A.c_sh*xo;
   s.x[0] = A.gam0*xo + A.gam1*s.x1[0] + A.beta_dt*static_cast<double>(vxo);
   s.x[1] = A.gam0*yo + A.gam1*s.x1[1] + A.beta_dt*ydot;
   s.x[2] = A.gam0*zo + A.gam1*s.x1[2] + A.beta_dt*static_cast<double>(vzo);
   // ShiftShearCrossings
   const DevBlock *B = A.B;
   if (A.shear && A.dsh != 0.0) {
-    const double b0 = B[s.gid].x1min, b1 = B[s.gid].x1max, lx = b1 - b0;
-    const int ix = BlockOff(s.x[0], b0, b1, lx);
-    const bool lo1 = (ix < 0) && (b0 < g.x1min + 0.5*lx);
-    const bool hi1 = (ix > 0) && (b1 > g.x1max - 0.5*lx);
+    const DevBlock &bo = B[s.gid];
+    const int ix = BlockOff(s.x[0], bo.x1min, bo.x1max, bo.x1max - bo.x1min);
+    const bool lo1 = (ix < 0) && (bo.face & 1);
+    const bool hi1 = (ix > 0) && (bo.face & 2);
     if (lo1 || hi1) {
       const double d = hi1 ? A.dsh : -A.dsh;
       s.x[1] += d; s.x1[1] += d;
@@ -224,7 +226,7 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
     if (g.bc_x3 == DK_X3_REFLECT) {
       const double zf = lo ? g.x3min : g.x3max;
       double z = 2.0*zf - s.x[2];
-      if (z >= g.x3max) z = g.x3max - 1.0e-12*(g.x3max - g.x3min);
+      if (z >= g.x3max) z = g.x3max - 1.0e-12*A.lzm;
       if (z < g.x3min) z = g.x3min;
       s.x[2] = z; s.v[2] = -s.v[2]; s.vres[2] = -s.vres[2];
       s.x1[2] = 2.0*zf - s.x1[2]; s.v1[2] = -s.v1[2];
@@ -240,10 +242,9 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
     }
   }
   const DevBlock &b = B[s.gid];
-  const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
-  const int ix = BlockOff(s.x[0], b.x1min, b.x1max, lx);
-  const int iy = BlockOff(s.x[1], b.x2min, b.x2max, ly);
-  const int iz = BlockOff(s.x[2], b.x3min, b.x3max, lz);
+  const int ix = BlockOff(s.x[0], b.x1min, b.x1max, b.x1max - b.x1min);
+  const int iy = BlockOff(s.x[1], b.x2min, b.x2max, b.x2max - b.x2min);
+  const int iz = BlockOff(s.x[2], b.x3min, b.x3max, b.x3max - b.x3min);
   const double bmin[3] = {b.x1min, b.x2min, b.x3min}, bmax[3] = {b.x1max, b.x2max, b.x3max};
   const int off[3] = {ix, iy, iz};
   for (int d=0; d<3; ++d) {
@@ -255,13 +256,11 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
   }
   if (abs(ix) > 1 || abs(iy) > 1 || abs(iz) > 1) return false;
   if ((abs(ix) + abs(iy) + abs(iz)) == 0) return true;
-  const bool lo1 = (ix < 0) && (b.x1min < g.x1min + 0.5*lx);
-  const bool hi1 = (ix > 0) && (b.x1max > g.x1max - 0.5*lx);
-  const bool lo2 = (iy < 0) && (b.x2min < g.x2min + 0.5*ly);
-  const bool hi2 = (iy > 0) && (b.x2max > g.x2max - 0.5*ly);
-  const bool lo3 = (iz < 0) && (b.x3min < g.x3min + 0.5*lz);
-  const bool hi3 = (iz > 0) && (b.x3max > g.x3max - 0.5*lz);
-  const double lxm = g.x1max - g.x1min, lym = g.x2max - g.x2min, lzm = g.x3max - g.x3min;
+  // the block's faces within half a block of a mesh face (the reference's periodic test)
+  const bool lo1 = (ix < 0) && (b.face & 1), hi1 = (ix > 0) && (b.face & 2);
+  const bool lo2 = (iy < 0) && (b.face & 4), hi2 = (iy > 0) && (b.face & 8);
+  const bool lo3 = (iz < 0) && (b.face & 16), hi3 = (iz > 0) && (b.face & 32);
+  const double lxm = A.lxm, lym = A.lym, lzm = A.lzm;
   if (A.shear && (lo1 || hi1)) {
     const bool cin = lo1;
     s.x[0] += cin ? lxm : -lxm; s.x1[0] += cin ? lxm : -lxm;
@@ -310,13 +309,15 @@ __device__ inline bool CenterCell(const Geo &g, const DevBlock &b, const double 
   return ci >= 0 && ci < g.nx1 && cj >= 0 && cj < g.nx2 && ck >= 0 && ck < g.nx3;
 }
 
-__device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, bool rates) {
+__device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, bool rates,
+                             bool vregs = true) {
   for (int d=0; d<3; ++d) {
     s.x[d] = S.X(d, p);
     s.v[d] = S.V(d, p);
     s.x1[d] = regs ? S.X1(d, p) : 0.0;
-    s.v1[d] = regs ? S.V1(d, p) : 0.0f;
+    s.v1[d] = (regs && vregs) ? S.V1(d, p) : 0.0f;
     s.r[d] = rates ? S.R(d, p) : 0.0;
+    s.vres[d] = 0.0;
   }
   s.gid = S.GID(p);
   s.parked = 0;
@@ -337,7 +338,7 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
 constexpr int kPPT = 2;             // particles per thread in the particle phase
 
 struct TileInfo {
-  int m, ci0, cj0, ck0;
+  int m, tx, ty, tz, ci0, cj0, ck0;
   long long key0;
 };
 
@@ -353,18 +354,26 @@ __device__ inline TileInfo TileOf(const Tiles &T, int tile) {
   TileInfo t;
   t.m = tile/T.ntb;
   int r = tile - t.m*T.ntb;
-  const int tx = r % T.ntx; r /= T.ntx;
-  const int ty = r % T.nty;
-  const int tz = r/T.nty;
-  t.ci0 = tx*T.TX; t.cj0 = ty*T.TY; t.ck0 = tz*T.TZ;
+  t.tx = r % T.ntx; r /= T.ntx;
+  t.ty = r % T.nty;
+  t.tz = r/T.nty;
+  t.ci0 = t.tx*T.TX; t.cj0 = t.ty*T.TY; t.ck0 = t.tz*T.TZ;
   t.key0 = (long long)tile*T.TC;
   return t;
 }
 
 __host__ __device__ inline long long TileKey(const Tiles &T, int m, int ci, int cj, int ck) {
-  const int tx = ci/T.TX, ty = cj/T.TY, tz = ck/T.TZ;
-  const int lx = ci - tx*T.TX, ly = cj - ty*T.TY, lz = ck - tz*T.TZ;
-  return ((long long)m*T.ntb + (tz*T.nty + ty)*T.ntx + tx)*T.TC + (lz*T.TY + ly)*T.TX + lx;
+  const int tx = ci >> T.LX, ty = cj >> T.LY, tz = ck/T.TZ;
+  const int lx = ci & (T.TX - 1), ly = cj & (T.TY - 1), lz = ck - tz*T.TZ;
+  return ((long long)m*T.ntb + (tz*T.nty + ty)*T.ntx + tx)*T.TC +
+         (((lz << T.LY) + ly) << T.LX) + lx;
+}
+
+// i/d for 0 <= i < 4096 and a divisor d < 64, from the float reciprocal inv = 1/d: the error of
+// (i + 1/2) inv (below 2^-22 i) is far below the distance 1/(2 d) of (i + 1/2)/d from an
+// integer
+__device__ inline int FDiv(int i, float inv) {
+  return static_cast<int>((static_cast<float>(i) + 0.5f)*inv);
 }
 
 // the tile-local cell of store slot q: the last c with st[c] <= q (st[0] <= q < st[TC])
@@ -375,13 +384,6 @@ __device__ inline int SlotCell(const int *st, int TC, int q) {
     if (st[mid] <= q) lo = mid; else hi = mid;
   }
   return lo;
-}
-
-// del of one axis about centre cell c (PMWeights, src/dust/dust.hpp:1088-1111, with the
-// block's nx/(xmax - xmin) formed once)
-__device__ inline double DelOf(double x, double xmin, double scale, int c) {
-#pragma clang fp contract(off)
-  return (x - xmin)*scale - (static_cast<double>(c) + 0.5);
 }
 
 // the three 1-D weights of PMWeights for a given del (S: 0 NGP, 1 CIC, 2 TSC)
@@ -441,52 +443,53 @@ __device__ inline void AddSlab(T (&acc)[3][3][NC], T (&spl)[3][NC], int &spdir, 
     }
 }
 
-// one phase's add into the LDS region (one contributor per word and phase, so the order of
-// adds to a word is the phase order): FP64 through the LDS adder (no read-back), FP32 as a read
-// and a write, which measured faster than the FP32 LDS adder on gfx950
-template <typename T>
-__device__ inline void AddLDS(T *p, T v) {
-  if constexpr (sizeof(T) == 8) atomicAdd(p, v);
-  else *p += v;
-}
-
-// the 27 phases (and 9 more for the spill terms when some cell of the tile has them): each
-// thread adds its sums at its cell + offset of the region; region cell (rx, ry, rz) is cell
-// (ci0 - 1 + rx, cj0 - 1 - SY + ry, ck0 - 1 + rz).  Ends with a barrier.
+// The deposit of one chunk's sums into the LDS tile region.  Each slab first adds into its own
+// copy of the region (cp: 3 copies, placed in the record area, which the chunk no longer needs):
+// 9 phases, one (x1, x2) offset per phase, all slabs at once, no two threads of a slab touching
+// one word in a phase; the spill terms of cells centred one row off in x2 take 3 more phases.
+// The copies are then added into the region in slab order.  Every word's sum has a fixed order.
+// Starts and ends with a barrier.
 template <typename T, int NC, bool SY>
-__device__ inline void DepositPhases(const Tiles &Tl, T *reg, int l, int slab,
-                                     const T (&acc)[3][3][NC], const T (&spl)[3][NC],
-                                     int spdir) {
-  const int lx = l % Tl.TX, ly = (l/Tl.TX) % Tl.TY, lz = l/(Tl.TX*Tl.TY);
+__device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int slab,
+                                    const T (&acc)[3][3][NC], const T (&spl)[3][NC], int spdir) {
+  const int n3 = 3*NC*Tl.RN;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n3; i += blockDim.x) cp[i] = T(0);
+  __syncthreads();
+  const int lx = l & (Tl.TX - 1), ly = (l >> Tl.LX) & (Tl.TY - 1), lz = l >> (Tl.LX + Tl.LY);
   const int rbase = ((lz + slab)*Tl.RY + (ly + Tl.SY))*Tl.RX + lx;
+  T *mine = cp + slab*NC*Tl.RN;
 #pragma unroll
-  for (int pc=0; pc<3; ++pc)
+  for (int pb=0; pb<3; ++pb)
 #pragma unroll
-    for (int pb=0; pb<3; ++pb)
+    for (int pa=0; pa<3; ++pa) {
+      const int rc = rbase + pb*Tl.RX + pa;
 #pragma unroll
-      for (int pa=0; pa<3; ++pa) {
-        if (slab == pc) {
-          const int rc = rbase + pb*Tl.RX + pa;
-#pragma unroll
-          for (int n=0; n<NC; ++n) AddLDS<T>(&reg[n*Tl.RN + rc], acc[pb][pa][n]);
-        }
-        __syncthreads();
-      }
+      for (int n=0; n<NC; ++n) mine[n*Tl.RN + rc] += acc[pb][pa][n];
+      __syncthreads();
+    }
   if constexpr (SY) {
     if (__syncthreads_or(spdir != 0)) {
 #pragma unroll
-      for (int pc=0; pc<3; ++pc)
+      for (int pa=0; pa<3; ++pa) {
+        if (spdir != 0) {
+          const int rc = rbase + (1 + 2*spdir)*Tl.RX + pa;
 #pragma unroll
-        for (int pa=0; pa<3; ++pa) {
-          if (slab == pc && spdir != 0) {
-            const int rc = rbase + (1 + 2*spdir)*Tl.RX + pa;
-#pragma unroll
-            for (int n=0; n<NC; ++n) AddLDS<T>(&reg[n*Tl.RN + rc], spl[pa][n]);
-          }
-          __syncthreads();
+          for (int n=0; n<NC; ++n) mine[n*Tl.RN + rc] += spl[pa][n];
         }
+        __syncthreads();
+      }
     }
   }
+  const int nr = NC*Tl.RN;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+    T v = reg[i];
+    v += cp[i];
+    v += cp[nr + i];
+    v += cp[2*nr + i];
+    reg[i] = v;
+  }
+  __syncthreads();
 }
 
 // the cells of the tile in order of decreasing particle count (64 buckets; the order inside a
@@ -510,48 +513,93 @@ __device__ inline void OrderCells(const int *st, int TC, int *lord, int *hb) {
 template <typename T>
 __device__ inline void LoadRegion(const Geo &g, const Tiles &Tl, const TileInfo &t,
                                   const T *f, int nvar, T *reg) {
-  const int n = nvar*Tl.RN;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int v = i/Tl.RN;
-    int rc = i - v*Tl.RN;
-    const int rx = rc % Tl.RX; rc /= Tl.RX;
-    const int ry = rc % Tl.RY;
-    const int rz = rc/Tl.RY;
-    reg[i] = f[Idx(g, t.m, nvar, v, g.ks + t.ck0 - 1 + rz, g.js + t.cj0 - 1 - Tl.SY + ry,
-                   g.is + t.ci0 - 1 + rx)];
-  }
+  for (int v=0; v<nvar; ++v)
+    for (int rc = threadIdx.x; rc < Tl.RN; rc += blockDim.x) {
+      const int rr = FDiv(rc, Tl.iRX);
+      const int rx = rc - rr*Tl.RX;
+      const int rz = FDiv(rr, Tl.iRY);
+      const int ry = rr - rz*Tl.RY;
+      reg[v*Tl.RN + rc] = f[Idx(g, t.m, nvar, v, g.ks + t.ck0 - 1 + rz,
+                                g.js + t.cj0 - 1 - Tl.SY + ry, g.is + t.ci0 - 1 + rx)];
+    }
+}
+
+// The tile scratch holds the regions row by row: one row of the scratch is the x1 rows (RX
+// values each) of the regions of all tiles of a block that share (tz, ty, component, rz, ry), in
+// tx order, so that the combine reads consecutive cells of a padded row from consecutive words.
+__host__ __device__ inline long long ScrRow(const Tiles &T, int m, int tz, int ty, int nv, int v,
+                                            int rz, int ry) {
+  return ((((((long long)m*T.ntz + tz)*T.nty + ty)*nv + v)*T.RZ + rz)*T.RY + ry)*
+         ((long long)T.ntx*T.RX);
 }
 
 template <typename T>
-__device__ inline void FlushRegion(const T *reg, T *scr, int tile, int n) {
+__device__ inline void FlushRegion(const Tiles &Tl, const TileInfo &t, const T *reg, T *scr,
+                                   int nv) {
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) scr[(long long)tile*n + i] = reg[i];
+  for (int v=0; v<nv; ++v)
+    for (int rc = threadIdx.x; rc < Tl.RN; rc += blockDim.x) {
+      const int rr = FDiv(rc, Tl.iRX);
+      const int rx = rc - rr*Tl.RX;
+      const int rz = FDiv(rr, Tl.iRY);
+      const int ry = rr - rz*Tl.RY;
+      scr[ScrRow(Tl, t.m, t.tz, t.ty, nv, v, rz, ry) + t.tx*Tl.RX + rx] = reg[v*Tl.RN + rc];
+    }
 }
 
 // active cell indices of tile-local cell l
 __device__ inline void CellIdx(const Tiles &Tl, const TileInfo &t, int l, int &ci, int &cj,
                                int &ck) {
-  ci = t.ci0 + l % Tl.TX;
-  cj = t.cj0 + (l/Tl.TX) % Tl.TY;
-  ck = t.ck0 + l/(Tl.TX*Tl.TY);
+  ci = t.ci0 + (l & (Tl.TX - 1));
+  cj = t.cj0 + ((l >> Tl.LX) & (Tl.TY - 1));
+  ck = t.ck0 + (l >> (Tl.LX + Tl.LY));
 }
 
-// the tile-local cell (clamped into the tile) and the x2 shift of a particle's deposit centre,
-// from its position (the cell it is sorted into)
+// Where a particle deposits (PMWeights of src/dust/dust.hpp:1088-1111 on each axis, x2 shifted
+// by sh x1, dust.hpp:1699-1702): the deposit cell clamped into the block, the x2 shift of the
+// centre (with SY: -1 or +1 when the shifted centre is one cell beyond the block's x2 face), del
+// about that centre on each axis, and the cell of the unshifted position (the centre of the
+// consistent split, dust_pm.cpp:649-659), which is an active cell when cen is true.
+struct Loc {
+  int ci, cj, ck, sj;
+  int jc;                   // x2 cell of the unshifted position
+  bool cen;                 // the unshifted position's cell is an active cell
+  double dx, dy, dz;
+};
+
 template <bool SY>
-__device__ inline void PartCell(const Geo &g, const Tiles &Tl, const TileInfo &t,
-                                const DevBlock &b, const double x[3], double sh, int &ci,
-                                int &cj, int &ck, int &sj) {
-  int nju;
-  DepositCell(g, b, x, sh, ci, cj, ck, nju);
-  ci = min(max(ci, t.ci0), t.ci0 + Tl.TX - 1);
-  cj = min(max(cj, t.cj0), t.cj0 + Tl.TY - 1);
-  ck = min(max(ck, t.ck0), t.ck0 + Tl.TZ - 1);
-  sj = 0;
-  if constexpr (SY) {
-    const int nj = min(max(nju, 0), g.nx2 - 1);
-    sj = min(max(nju - nj, -1), 1);
-  }
+__device__ inline Loc Locate(const Geo &g, const DevBlock &b, const double x[3], double sh) {
+#pragma clang fp contract(off)
+  Loc L;
+  const double cx = (x[0] - b.x1min)*b.sx1;
+  const double cy = ((x[1] + sh*x[0]) - b.x2min)*b.sx2;
+  const double cz = (x[2] - b.x3min)*b.sx3;
+  const double cu = (x[1] - b.x2min)*b.sx2;
+  const int iu = static_cast<int>(cx + 1.0) - 1;
+  const int ju = static_cast<int>(cy + 1.0) - 1;
+  const int ku = static_cast<int>(cz + 1.0) - 1;
+  L.jc = static_cast<int>(cu + 1.0) - 1;
+  L.ci = min(max(iu, 0), g.nx1 - 1);
+  L.cj = min(max(ju, 0), g.nx2 - 1);
+  L.ck = min(max(ku, 0), g.nx3 - 1);
+  L.sj = SY ? min(max(ju - L.cj, -1), 1) : 0;
+  L.cen = (iu == L.ci) && (ku == L.ck) && (L.jc >= 0) && (L.jc < g.nx2);
+  L.dx = cx - (static_cast<double>(L.ci) + 0.5);
+  L.dy = cy - (static_cast<double>(L.cj + L.sj) + 0.5);
+  L.dz = cz - (static_cast<double>(L.ck) + 0.5);
+  return L;
+}
+
+// the same, with the cell clamped into the tile (LDS indices stay inside the region whatever the
+// position; a particle outside its tile is an error the push pass has recorded)
+template <bool SY>
+__device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo &t,
+                                   const DevBlock &b, const double x[3], double sh) {
+  Loc L = Locate<SY>(g, b, x, sh);
+  L.ci = min(max(L.ci, t.ci0), t.ci0 + Tl.TX - 1);
+  L.cj = min(max(L.cj, t.cj0), t.cj0 + Tl.TY - 1);
+  L.ck = min(max(L.ck, t.ck0), t.ck0 + Tl.TZ - 1);
+  return L;
 }
 
 //----------------------------------------------------------------------------------------
@@ -584,7 +632,7 @@ k_key(KArgs A, int do_key) {
     const int q = c0 + tid;
     if (q < c1) {
       PState s;
-      LoadP(A.src, q, s, !A.stage1, false);
+      LoadP(A.src, q, s, !A.stage1, false, false);
       int code = 0;
       double dv[3] = {0.0, 0.0, 0.0};
       if (A.pending) {
@@ -616,7 +664,7 @@ k_key(KArgs A, int do_key) {
         }
       }
       if (do_key) {
-        if (!PushMigrate(A, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+        if (!PushMigrate<false>(A, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
         // a particle that failed to route keeps its block: its key stays valid
         int ni, nj, nk, nju;
         DepositCell(g, A.B[s.gid], s.x, A.sh, ni, nj, nk, nju);
@@ -781,41 +829,36 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
         A.dst.V(d, q) = s.v[d]; A.dst.V1(d, q) = s.v1[d];
       }
       A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = s.gid; A.dst.ORIG(q) = A.src.ORIG(p);
-      float w9[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      float wx[3] = {0.0f, 0.0f, 0.0f}, wy[3] = {0.0f, 0.0f, 0.0f}, wz[3] = {0.0f, 0.0f, 0.0f};
       float mu = 0.0f, pr[3] = {0.0f, 0.0f, 0.0f};
       int code = 1 << 2;   // no centre, sj = 0
       if (!ok) {
         dk::RecordError(A.err, DK_EROUTE, p);
       } else {
-        int ni, nj, nk, nju;
-        DepositCell(g, A.B[s.gid], s.x, A.sh, ni, nj, nk, nju);
+        const Loc L = Locate<SY>(g, A.B[s.gid], s.x, A.sh);
         // the key pass placed the particle by the same push: the cell must be its slot's
-        if (TileKey(Tl, s.gid, ni, nj, nk) != A.key[p]) {
+        if (TileKey(Tl, s.gid, L.ci, L.cj, L.ck) != A.key[p]) {
           dk::RecordError(A.err, DK_EHALO, p);
         } else {
           const float af = static_cast<float>(A.a_dt);
           mu = mass*(af/(ts + af))*static_cast<float>(ivol);
           for (int d=0; d<3; ++d)
             pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
-          int ic_, jc, kc_;
           int cen = 0;
-          if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
+          if (mass > 0.0f && L.cen) {
             // the centre of the unshifted position: this cell or one beside it in x2
-            if (ic_ != ni || kc_ != nk || jc - nj < -1 || jc - nj > 1)
-              dk::RecordError(A.err, DK_EHALO, p);
-            else
-              cen = jc - nj + 2;
+            if (L.jc - L.cj < -1 || L.jc - L.cj > 1) dk::RecordError(A.err, DK_EHALO, p);
+            else cen = L.jc - L.cj + 2;
           }
-          const int sj = SY ? min(max(nju - nj, -1), 1) : 0;
-          code = cen | ((sj + 1) << 2);
-          W3<S, float>(static_cast<float>(DelOf(s.x[0], b.x1min, b.sx1, ni)), w9);
-          W3<S, float>(static_cast<float>(DelOf(s.x[1] + A.sh*s.x[0], b.x2min, b.sx2, nj + sj)),
-                       w9 + 3);
-          W3<S, float>(static_cast<float>(DelOf(s.x[2], b.x3min, b.sx3, nk)), w9 + 6);
+          code = cen | ((L.sj + 1) << 2);
+          W3<S, float>(static_cast<float>(L.dx), wx);
+          W3<S, float>(static_cast<float>(L.dy), wy);
+          W3<S, float>(static_cast<float>(L.dz), wz);
         }
       }
-#pragma unroll
-      for (int i=0; i<9; ++i) rw[i*CH + r] = w9[i];
+      rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
+      rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
+      rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
       rv[r] = mu;
       rv[CH + r] = mu*s.v[0]; rv[2*CH + r] = mu*s.v[1]; rv[3*CH + r] = mu*s.v[2];
       rp[r] = pr[0]; rp[CH + r] = pr[1]; rp[2*CH + r] = pr[2];
@@ -850,7 +893,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
         q3[2] += (cen == 3) ? val[0] : 0.0f;
       }
     }
-    DepositPhases<float, 4, SY>(Tl, reg, l, slab, acc, spl, spdir);
+    DepositSlabs<float, 4, SY>(Tl, reg, rw, l, slab, acc, spl, spdir);
   }
   int ci, cj, ck;
   CellIdx(Tl, t, l, ci, cj, ck);
@@ -863,7 +906,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
     qt1[i0 + g.n1] = q3[2];
     qt2[i0 - g.n1] = q3[0];
   }
-  FlushRegion<float>(reg, scr, tile, 4*Tl.RN);
+  FlushRegion<float>(Tl, t, reg, scr, 4);
 }
 
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
@@ -902,13 +945,13 @@ k_p2(KArgs A, const float *us, float *scr) {
       const int q = c0 + r;
       if (q >= c1) break;
       const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
-      int ci, cj, ck, sj;
-      PartCell<SY>(g, Tl, t, b, x, A.sh, ci, cj, ck, sj);
+      const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
+      const int sj = L.sj;
       float wx[3], wy[3], wz[3];
-      W3<S, float>(static_cast<float>(DelOf(x[0], b.x1min, b.sx1, ci)), wx);
-      W3<S, float>(static_cast<float>(DelOf(x[1] + A.sh*x[0], b.x2min, b.sx2, cj + sj)), wy);
-      W3<S, float>(static_cast<float>(DelOf(x[2], b.x3min, b.sx3, ck)), wz);
-      const int rb = ((ck - t.ck0)*Tl.RY + (cj - t.cj0 + Tl.SY + sj))*Tl.RX + (ci - t.ci0);
+      W3<S, float>(static_cast<float>(L.dx), wx);
+      W3<S, float>(static_cast<float>(L.dy), wy);
+      W3<S, float>(static_cast<float>(L.dz), wz);
+      const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
       float gth[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int c=0; c<3; ++c)
@@ -952,9 +995,9 @@ k_p2(KArgs A, const float *us, float *scr) {
       const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
       AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
     }
-    DepositPhases<float, 3, SY>(Tl, dep, l, slab, acc, spl, spdir);
+    DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
   }
-  FlushRegion<float>(dep, scr, tile, 3*Tl.RN);
+  FlushRegion<float>(Tl, t, dep, scr, 3);
 }
 
 // P3: gather u*, kick, drag rate, and the FP64 deposit of the actual impulse
@@ -993,16 +1036,14 @@ k_p3(KArgs A, const float *us, double *scr) {
       const int q = c0 + r;
       if (q >= c1) break;
       const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
-      int ci, cj, ck, sj;
-      PartCell<SY>(g, Tl, t, b, x, A.sh, ci, cj, ck, sj);
-      const double dx = DelOf(x[0], b.x1min, b.sx1, ci);
-      const double dy = DelOf(x[1] + A.sh*x[0], b.x2min, b.sx2, cj + sj);
-      const double dz = DelOf(x[2], b.x3min, b.sx3, ck);
+      const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
+      const int sj = L.sj;
+      const double dx = L.dx, dy = L.dy, dz = L.dz;
       float wx[3], wy[3], wz[3];
       W3<S, float>(static_cast<float>(dx), wx);
       W3<S, float>(static_cast<float>(dy), wy);
       W3<S, float>(static_cast<float>(dz), wz);
-      const int rb = ((ck - t.ck0)*Tl.RY + (cj - t.cj0 + Tl.SY + sj))*Tl.RX + (ci - t.ci0);
+      const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
       float u[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int c=0; c<3; ++c)
@@ -1053,28 +1094,31 @@ k_p3(KArgs A, const float *us, double *scr) {
       const double val[3] = {rd[3*CH + r], rd[4*CH + r], rd[5*CH + r]};
       AddSlab<double, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
     }
-    DepositPhases<double, 3, SY>(Tl, dep, l, slab, acc, spl, spdir);
+    DepositSlabs<double, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
   }
-  FlushRegion<double>(dep, scr, tile, 3*Tl.RN);
+  FlushRegion<double>(Tl, t, dep, scr, 3);
 }
 
-// every padded cell sums the tile regions covering it, in ascending tile order
-template <typename T>
-__global__ void k_combine(Geo g, Tiles Tl, const T *scr, int nvar, T *out) {
-  const long long n = g.ncb*g.nmb;
-  GRID_STRIDE(c, n) {
-    const int m = static_cast<int>(c/g.ncb);
-    long long rr = c - (long long)m*g.ncb;
-    const int i = static_cast<int>(rr % g.n1); rr /= g.n1;
-    const int j = static_cast<int>(rr % g.n2);
-    const int k = static_cast<int>(rr/g.n2);
+// every padded cell sums the tile regions covering it, in ascending tile order (grid: x over
+// the cells of a block, y over the blocks)
+template <typename T, int NV>
+__global__ void k_combine(Geo g, Tiles Tl, const T *scr, T *out) {
+  const int m = blockIdx.y;
+  const int ncb = static_cast<int>(g.ncb);
+  for (int rr = blockIdx.x*blockDim.x + threadIdx.x; rr < ncb; rr += gridDim.x*blockDim.x) {
+    const int i = rr % g.n1;
+    const int t2 = rr/g.n1;
+    const int j = t2 % g.n2;
+    const int k = t2/g.n2;
     // region coordinates relative to tile 0: r = (i - is + 1) - tx*TX etc.
     const int ai = i - g.is + 1, aj = j - g.js + 1 + Tl.SY, ak = k - g.ks + 1;
-    T sum[5] = {T(0), T(0), T(0), T(0), T(0)};
-    const int tz0 = max(0, (ak - Tl.RZ + Tl.TZ)/Tl.TZ), tz1 = min(Tl.ntz - 1, ak/Tl.TZ);
-    const int ty0 = max(0, (aj - Tl.RY + Tl.TY)/Tl.TY), ty1 = min(Tl.nty - 1, aj/Tl.TY);
-    const int tx0 = max(0, (ai - Tl.RX + Tl.TX)/Tl.TX), tx1 = min(Tl.ntx - 1, ai/Tl.TX);
+    T sum[NV];
+#pragma unroll
+    for (int v=0; v<NV; ++v) sum[v] = T(0);
     if (ai >= 0 && aj >= 0 && ak >= 0) {
+      const int tz0 = max(0, (ak - Tl.RZ + Tl.TZ)/Tl.TZ), tz1 = min(Tl.ntz - 1, ak/Tl.TZ);
+      const int ty0 = max(0, (aj - Tl.RY + Tl.TY)/Tl.TY), ty1 = min(Tl.nty - 1, aj/Tl.TY);
+      const int tx0 = max(0, (ai - Tl.RX + Tl.TX)/Tl.TX), tx1 = min(Tl.ntx - 1, ai/Tl.TX);
       for (int tz=tz0; tz<=tz1; ++tz) {
         const int rz = ak - tz*Tl.TZ;
         if (rz < 0 || rz >= Tl.RZ) continue;
@@ -1084,14 +1128,16 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, int nvar, T *out) {
           for (int tx=tx0; tx<=tx1; ++tx) {
             const int rx = ai - tx*Tl.TX;
             if (rx < 0 || rx >= Tl.RX) continue;
-            const long long tile = (long long)m*Tl.ntb + (tz*Tl.nty + ty)*Tl.ntx + tx;
-            const long long base = tile*nvar*Tl.RN + (rz*Tl.RY + ry)*Tl.RX + rx;
-            for (int v=0; v<nvar; ++v) sum[v] += scr[base + (long long)v*Tl.RN];
+            const long long base = ScrRow(Tl, m, tz, ty, NV, 0, rz, ry) + tx*Tl.RX + rx;
+            const long long vstride = (long long)Tl.RZ*Tl.RY*Tl.ntx*Tl.RX;
+#pragma unroll
+            for (int v=0; v<NV; ++v) sum[v] += scr[base + v*vstride];
           }
         }
       }
     }
-    for (int v=0; v<nvar; ++v) out[Idx(g,m,nvar,v,k,j,i)] = sum[v];
+#pragma unroll
+    for (int v=0; v<NV; ++v) out[Idx(g,m,NV,v,k,j,i)] = sum[v];
   }
 }
 
@@ -1383,6 +1429,11 @@ void LaunchP3(dk_ctx *c, const KArgs &A, const float *us, double *scr) {
   SWITCH_SCHEME(k_p3, (A, us, scr));
 }
 
+dim3 CombineGrid(const dk_ctx *c) {
+  const long long b = (c->g.ncb + kBlock - 1)/kBlock;
+  return dim3(static_cast<unsigned>(std::min<long long>(b, 4096)), c->g.nmb);
+}
+
 #define TIMED(label, launch) \
   do { const int tk_ = dk::ProfBegin(c, label); launch; dk::ProfEnd(c, tk_); } while (0)
 
@@ -1421,6 +1472,15 @@ KArgs MakeArgs(dk_ctx *c) {
   A.sbox = c->P.shearing_box; A.strat = c->P.stratified;
   A.shear = (c->M.bc_x1 == DK_X1_SHEAR);
   A.qs = c->P.qshear; A.om = c->P.omega0;
+  if (A.sbox) {
+    A.c_fx = 2.0*A.om;
+    A.c_fy = -(2.0-A.qs)*A.om;
+    A.c_fz = A.strat ? -(A.om*A.om) : 0.0;
+    A.c_sh = A.qs*A.om;
+  }
+  A.lxm = c->M.x1max - c->M.x1min;
+  A.lym = c->M.x2max - c->M.x2min;
+  A.lzm = c->M.x3max - c->M.x3min;
   A.pending = f->pending; A.a_prev = f->a_prev;
   A.res = f->res; A.dc0 = f->dc[0]; A.dc1 = f->dc[1]; A.dc2 = f->dc[2];
   return A;
@@ -1472,9 +1532,9 @@ int FastCreate(dk_ctx *c) {
   Fast *f = new Fast();
   c->fast = f;
   Tiles &T = f->T;
-  // tile shape: 8 x 4 x 4 cells by default (or the largest divisors of the block size below
-  // those); DK_TILE=AxBxC selects another shape whose sides divide the block and whose cell
-  // count is at most kMaxTileCells
+  // tile shape: 8 x 4 x 4 cells by default (or the largest powers of two dividing the block
+  // below those); DK_TILE=AxBxC selects another shape whose sides divide the block, whose first
+  // two sides are powers of two and whose cell count is at most kMaxTileCells
   T.TX = Largest(c->g.nx1, 8);
   T.TY = Largest(c->g.nx2, 4);
   T.TZ = Largest(c->g.nx3, 4);
@@ -1482,7 +1542,7 @@ int FastCreate(dk_ctx *c) {
     int a = 0, b = 0, d = 0;
     if (std::sscanf(e, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 &&
         c->g.nx1 % a == 0 && c->g.nx2 % b == 0 && c->g.nx3 % d == 0 &&
-        a*b*d <= kMaxTileCells) {
+        a*b*d <= kMaxTileCells && (a & (a - 1)) == 0 && (b & (b - 1)) == 0) {
       T.TX = a; T.TY = b; T.TZ = d;
     }
   }
@@ -1490,8 +1550,20 @@ int FastCreate(dk_ctx *c) {
   T.SY = (c->P.shearing_box) ? 1 : 0;
   T.RX = T.TX + 2; T.RY = T.TY + 2 + 2*T.SY; T.RZ = T.TZ + 2;
   T.RN = T.RX*T.RY*T.RZ;
-  // every tile kernel's LDS within 64 KiB
+  T.LX = 0; while ((1 << T.LX) < T.TX) ++T.LX;
+  T.LY = 0; while ((1 << T.LY) < T.TY) ++T.LY;
+  T.iRX = 1.0f/static_cast<float>(T.RX);
+  T.iRY = 1.0f/static_cast<float>(T.RY);
+  // every tile kernel's LDS within 64 KiB, and the three slab copies of a region within the
+  // record area of its kernel
   if (std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)) > 65536 - 1024) return DK_EUNSUPPORTED;
+  {
+    const size_t ch = (size_t)kPPT*3*T.TC;
+    if (3*4*T.RN*sizeof(float) > ch*16*sizeof(float) ||
+        3*3*T.RN*sizeof(float) > ch*12*sizeof(float) ||
+        3*3*T.RN*sizeof(double) > ch*6*sizeof(double))
+      return DK_EUNSUPPORTED;
+  }
   // a block's first and last x2 rows must differ (cells centred one row off in x2)
   if (T.SY && c->g.nx2 < 2) return DK_EUNSUPPORTED;
   T.ntx = c->g.nx1/T.TX; T.nty = c->g.nx2/T.TY; T.ntz = c->g.nx3/T.TZ;
@@ -1669,8 +1741,8 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   f->cur = 1 - f->cur;
   A.src = f->s[f->cur];
   A.dst = f->s[1 - f->cur];
-  TIMED("combine", (k_combine<float><<<Grid(NCells(c)), kBlock, 0, c->stream>>>(
-      g, T, f->scr_f, 4, f->qdep)));
+  TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
+      g, T, f->scr_f, f->qdep)));
   if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
   // 5. local guess + dc1/dc2 sweeps
   TIMED("guess", (k_guess<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ustar)));
@@ -1678,16 +1750,16 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   for (int sw=0; sw<nsweeps; ++sw) {
     if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
     TIMED("p2_operator", LaunchP2(c, A, f->ustar, f->scr_f));
-    TIMED("combine", (k_combine<float><<<Grid(NCells(c)), kBlock, 0, c->stream>>>(
-        g, T, f->scr_f, 3, f->ap)));
+    TIMED("combine", (k_combine<float, 3><<<CombineGrid(c), kBlock, 0, c->stream>>>(
+        g, T, f->scr_f, f->ap)));
     if ((e = FoldT<float>(c, f->ap, 3, st->time))) return e;
     TIMED("dc", (k_dc<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ap, f->ustar)));
   }
   if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
   // 6. gather-kick + impulse deposit
   TIMED("p3_gather_kick", LaunchP3(c, A, f->ustar, f->scr_d));
-  TIMED("combine", (k_combine<double><<<Grid(NCells(c)), kBlock, 0, c->stream>>>(
-      g, T, f->scr_d, 3, f->dmom)));
+  TIMED("combine", (k_combine<double, 3><<<CombineGrid(c), kBlock, 0, c->stream>>>(
+      g, T, f->scr_d, f->dmom)));
   if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;
   // 7. consistent split: the dust's residual per unit coupled mass
   TIMED("split", (k_split<<<Grid(nact), kBlock, 0, c->stream>>>(

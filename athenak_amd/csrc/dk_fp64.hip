@@ -959,7 +959,13 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     d.sx2 = static_cast<double>(g.nx2)/(b.x2max - b.x2min);
     d.sx3 = static_cast<double>(g.nx3)/(b.x3max - b.x3min);
     d.ivol = 1.0/(d.dx1*d.dx2*d.dx3);
-    d.lx1 = b.lx1; d.lx2 = b.lx2; d.lx3 = b.lx3; d.pad = 0;
+    d.lx1 = b.lx1; d.lx2 = b.lx2; d.lx3 = b.lx3;
+    {
+      const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
+      d.face = ((b.x1min < mesh->x1min + 0.5*lx) ? 1 : 0) | ((b.x1max > mesh->x1max - 0.5*lx) ? 2 : 0) |
+               ((b.x2min < mesh->x2min + 0.5*ly) ? 4 : 0) | ((b.x2max > mesh->x2max - 0.5*ly) ? 8 : 0) |
+               ((b.x3min < mesh->x3min + 0.5*lz) ? 16 : 0) | ((b.x3max > mesh->x3max - 0.5*lz) ? 32 : 0);
+    }
     for (int q=0; q<27; ++q) d.nghbr[q] = b.nghbr[q];
   }
   // the same-level fold slots in NeighborIndex order (src/mesh/nghbr_index.hpp:19-54)
@@ -1018,6 +1024,8 @@ int dk_destroy(dk_ctx *c) {
   if (c == nullptr) return DK_OK;
   (void)hipStreamSynchronize(c->stream);
   if (c->fast) dk::FastDestroy(c);
+  for (auto &x : c->gexec)
+    if (x) {(void)hipGraphExecDestroy(x); x = nullptr;}
   dk::ProfDestroy(c);
   for (void *p : {(void*)c->d_blocks, (void*)c->d_slot_dir, (void*)c->d_err,
                   (void*)c->d_partial, (void*)c->d_scalar, (void*)c->qdep, (void*)c->ustar,
@@ -1260,8 +1268,9 @@ int dk_fill_gas_ghosts(dk_ctx *c, const dk_stage *st) {
   return FillGhosts(c, c->u0, DK_NGAS, false, st->time + st->dt);
 }
 
-int dk_stage_run(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
-  if (c == nullptr || st == nullptr) return DK_EINVAL;
+namespace {
+
+int StageBody(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   if (c->form == DK_FORM_FP32) return dk::FastStage(c, st, src);
   int e;
   if ((e = dk_first_two_imp_rk(c, st))) return e;
@@ -1278,6 +1287,53 @@ int dk_stage_run(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   if ((e = dk_apply_pmbr(c, st))) return e;
   if (ActiveStage(c, st) && (e = dk_fill_gas_ghosts(c, st))) return e;
   return DK_OK;
+}
+
+// The stage captured from the context's stream into a graph, which then updates the stage's
+// executable graph (same topology every time: only kernel arguments change) and is launched as
+// one unit.  Not used on the null stream, while the caller captures the stream itself, or while
+// per-kernel timing is on (its events would enter the graph).
+int StageGraph(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (c->stream == nullptr || dk::ProfOn(c) ||
+      hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return StageBody(c, st, src);
+  const int slot = std::min(std::max(st->stage, 1), 3) - 1 + (src ? 3 : 0);
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return DK_EHIP;
+  const int e = StageBody(c, st, src);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+  if (e || ce != hipSuccess || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    return e ? e : DK_EHIP;
+  }
+  hipGraphExec_t &x = c->gexec[slot];
+  if (x != nullptr) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+    if (hipGraphExecUpdate(x, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
+  }
+  int r = DK_OK;
+  if (x == nullptr && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+    x = nullptr;
+    r = DK_EHIP;
+  }
+  (void)hipGraphDestroy(g);
+  if (r) return r;
+  return dk::Check(hipGraphLaunch(x, c->stream));
+}
+
+}  // namespace
+
+int dk_stage_run(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
+  if (c == nullptr || st == nullptr) return DK_EINVAL;
+  if (c->graphs) return StageGraph(c, st, src);
+  return StageBody(c, st, src);
 }
 
 int dk_set_graphs(dk_ctx *c, int32_t enable) {

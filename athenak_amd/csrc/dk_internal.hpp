@@ -31,7 +31,9 @@ struct DevBlock {
   double dx1, dx2, dx3;
   double sx1, sx2, sx3;    // nx/(xmax - xmin) of each axis, as PMWeights forms it
   double ivol;             // 1/(dx1 dx2 dx3)
-  int lx1, lx2, lx3, pad;
+  int lx1, lx2, lx3;
+  int face;                // bits 1,2 / 4,8 / 16,32: lower/upper face within half a block of
+                           // the mesh face in x1 / x2 / x3 (the test of bvals_part.cpp)
   int nghbr[27];
 };
 
@@ -110,6 +112,7 @@ struct dk_ctx {
   // the FP32 form (dk_fp32.hip)
   struct Fast *fast = nullptr;
   int graphs = 0;
+  hipGraphExec_t gexec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-kernel event timing
   struct Prof *prof = nullptr;
 };
@@ -126,6 +129,7 @@ template <typename T> int FoldT(dk_ctx *c, T *a, int nvar, double t);
 
 // per-kernel event timing (no-ops unless enabled): Begin returns a token for End
 int ProfBegin(dk_ctx *c, const char *label);
+bool ProfOn(const dk_ctx *c);
 void ProfEnd(dk_ctx *c, int token);
 void ProfDestroy(dk_ctx *c);
 

@@ -43,6 +43,8 @@ struct Prof {
 
 namespace dk {
 
+bool ProfOn(const dk_ctx *c) { return c->prof != nullptr && c->prof->on; }
+
 int ProfBegin(dk_ctx *c, const char *label) {
   Prof *p = c->prof;
   if (p == nullptr || !p->on) return -1;

@@ -188,3 +188,20 @@ def test_halo_violation_is_reported_not_fatal():
     with pytest.raises(DustError, match="halo"):
         g.check_device_error()
     g.check_device_error()   # cleared
+
+
+def test_graph_replays_of_the_fp64_form_match_the_oracle():
+    # the FP64 form with every stage captured into a graph and replayed: still the oracle's
+    # result (its deposits use atomics, so replays and eager runs agree to round-off)
+    w = workloads.c1(n=16, nb=2, ppc=2)
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.3, 2e-3
+    w.name = "c1-graphs"
+    o, g = make_pair(w)
+    g.set_graphs(True)
+    for s in range(6):
+        o.step(w.dt, w.src)
+        g.step(w.dt, w.src)
+        g.check_device_error()
+        g.diag()                     # eager work and a host sync between the replays
+        compare(o, g, w, what="graph step %d" % s)
+    say("c1-graphs: 6 steps of graph replays match the oracle")
