@@ -81,7 +81,7 @@ struct Tiles {
   int TX, TY, TZ, TC, SY, RX, RY, RZ, RN;
   int LX, LY;                       // log2 TX, log2 TY (the tile sides are powers of 2)
   int ntx, nty, ntz, ntb, nt;
-  float iRX, iRY;                   // 1/RX, 1/RY for FDiv
+  float iRX, iRY, iRN;              // 1/RX, 1/RY, 1/RN for FDiv
 };
 
 }  // namespace
@@ -103,7 +103,8 @@ struct Fast {
   float *ustar = nullptr, *ap = nullptr, *res = nullptr;
   double *dmom = nullptr;           // D of the kick, then R_g
   double *dc[3] = {nullptr, nullptr, nullptr};  // actual dust correction: own, below, above
-  double *pres = nullptr;           // push rounding residue per cell (to the gas)
+  double *pres[2] = {nullptr, nullptr};   // push rounding residue per cell (to the gas):
+  int pres_rd = 0;                        // the buffer of the last push, read by the next stage
   float *scr_f = nullptr;
   double *scr_d = nullptr;
   int pending = 0;
@@ -369,9 +370,9 @@ __host__ __device__ inline long long TileKey(const Tiles &T, int m, int ci, int 
          (((lz << T.LY) + ly) << T.LX) + lx;
 }
 
-// i/d for 0 <= i < 4096 and a divisor d < 64, from the float reciprocal inv = 1/d: the error of
-// (i + 1/2) inv (below 2^-22 i) is far below the distance 1/(2 d) of (i + 1/2)/d from an
-// integer
+// i/d for 0 <= i < 4096 and a divisor d < 64, or 0 <= i < 4 d with any d < 2^20, from the
+// float reciprocal inv = 1/d: the error of (i + 1/2) inv (below 2^-22 (i + 1/2)/d) is below the
+// distance 1/(2 d) of (i + 1/2)/d from an integer
 __device__ inline int FDiv(int i, float inv) {
   return static_cast<int>((static_cast<float>(i) + 0.5f)*inv);
 }
@@ -509,20 +510,40 @@ __device__ inline void OrderCells(const int *st, int TC, int *lord, int *hb) {
   __syncthreads();
 }
 
-// the tile region of a padded cell field into LDS
+// the tile region of a padded cell field, fetched into registers (up to kRegionPer values per
+// thread) so that its latency overlaps the LDS set-up, then stored into LDS
+constexpr int kRegionPer = 6;
+
 template <typename T>
-__device__ inline void LoadRegion(const Geo &g, const Tiles &Tl, const TileInfo &t,
-                                  const T *f, int nvar, T *reg) {
-  for (int v=0; v<nvar; ++v)
-    for (int rc = threadIdx.x; rc < Tl.RN; rc += blockDim.x) {
-      const int rr = FDiv(rc, Tl.iRX);
-      const int rx = rc - rr*Tl.RX;
-      const int rz = FDiv(rr, Tl.iRY);
-      const int ry = rr - rz*Tl.RY;
-      reg[v*Tl.RN + rc] = f[Idx(g, t.m, nvar, v, g.ks + t.ck0 - 1 + rz,
-                                g.js + t.cj0 - 1 - Tl.SY + ry, g.is + t.ci0 - 1 + rx)];
+struct RegionFetch {
+  T val[kRegionPer];
+  __device__ void Load(const Geo &g, const Tiles &Tl, const TileInfo &t, const T *f, int nvar) {
+    const int n = nvar*Tl.RN;
+#pragma unroll
+    for (int k=0; k<kRegionPer; ++k) {
+      const int i = threadIdx.x + k*blockDim.x;
+      val[k] = T(0);
+      if (i < n) {
+        const int v = FDiv(i, Tl.iRN);
+        const int rc = i - v*Tl.RN;
+        const int rr = FDiv(rc, Tl.iRX);
+        const int rx = rc - rr*Tl.RX;
+        const int rz = FDiv(rr, Tl.iRY);
+        const int ry = rr - rz*Tl.RY;
+        val[k] = f[Idx(g, t.m, nvar, v, g.ks + t.ck0 - 1 + rz, g.js + t.cj0 - 1 - Tl.SY + ry,
+                       g.is + t.ci0 - 1 + rx)];
+      }
     }
-}
+  }
+  __device__ void Store(const Tiles &Tl, int nvar, T *reg) const {
+    const int n = nvar*Tl.RN;
+#pragma unroll
+    for (int k=0; k<kRegionPer; ++k) {
+      const int i = threadIdx.x + k*blockDim.x;
+      if (i < n) reg[i] = val[k];
+    }
+  }
+};
 
 // The tile scratch holds the regions row by row: one row of the scratch is the x1 rows (RX
 // values each) of the regions of all tiles of a block that share (tz, ty, component, rz, ry), in
@@ -603,13 +624,16 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
 }
 
 //----------------------------------------------------------------------------------------
-// key pass: the pending consistent-dust correction of the previous stage (dust_pm.cpp:686-696;
-// the actual change of every particle, summed per centre cell, is recorded for the gas), then
-// every particle's cell after this stage's push and migration and its rank: a particle that
-// stays in the tile takes its rank from a count in LDS (the tile owns those cells: stay[]), one
-// that leaves it from a global count of arrivals (count[])
+// key pass, one pass over the store in its present order: the pending consistent-dust
+// correction of the previous stage (dust_pm.cpp:686-696; the actual change of every particle,
+// summed per centre cell, is recorded for the gas), this stage's push and migration
+// (dust_push.cpp:249-297, bvals_part.cpp:164-429) written back in place, the push rounding
+// residue summed per cell for the gas, and every particle's new tile-major key and rank: a
+// particle that stays in its tile takes its rank from a count in LDS (the tile owns those
+// cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
+// only the pending correction is applied (written to v and R).
 __global__ void __launch_bounds__(kKeyThreads)
-k_key(KArgs A, int do_key) {
+k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -617,7 +641,8 @@ k_key(KArgs A, int do_key) {
   const TileInfo t = TileOf(Tl, tile);
   const int tid = threadIdx.x, CH = blockDim.x;
   double *rdc = lds_d;                                   // [3][CH] m delta/V
-  int *rcode = reinterpret_cast<int*>(rdc + 3*CH);       // [CH] centre row + 2 (0: none)
+  float *rpr = reinterpret_cast<float*>(rdc + 3*CH);     // [3][CH] m vres/V
+  int *rcode = reinterpret_cast<int*>(rpr + 3*CH);       // [CH] centre row + 2 (0: none)
   int *st = rcode + CH;                                  // [TC + 1]
   int *lcnt = st + Tl.TC + 1;                            // [TC]
   for (int i = tid; i <= Tl.TC; i += CH) st[i] = A.start[t.key0 + i];
@@ -627,16 +652,17 @@ k_key(KArgs A, int do_key) {
   const DevBlock &b = A.B[t.m];
   const double ivol = b.ivol;
   double dcs[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  float prs[3] = {0.0f, 0.0f, 0.0f};
   for (int c0 = P0; c0 < P1; c0 += CH) {
     const int c1 = min(c0 + CH, P1);
     const int q = c0 + tid;
     if (q < c1) {
       PState s;
-      LoadP(A.src, q, s, !A.stage1, false, false);
+      LoadP(A.src, q, s, !A.stage1, A.atw_dt != 0.0 || !do_key);
+      const float mass = A.src.M(q);
       int code = 0;
       double dv[3] = {0.0, 0.0, 0.0};
       if (A.pending) {
-        const float mass = A.src.M(q);
         int ic_, jc, kc_;
         if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
           int ci, cj, ck;
@@ -655,16 +681,24 @@ k_key(KArgs A, int do_key) {
               // the recorded rate takes the actual change, so that the history terms of dust
               // and gas stay opposite (dust_pm.cpp:694)
               const double delta = static_cast<double>(vn) - static_cast<double>(vo);
-              A.src.R(d, q) = A.src.R(d, q) + delta/A.a_prev;
-              A.src.V(d, q) = vn;
+              s.r[d] += delta/A.a_prev;
               dv[d] = static_cast<double>(mass)*delta*ivol;
             }
             code = dj + 2;
+            if (!do_key)
+              for (int d=0; d<3; ++d) {A.src.V(d, q) = s.v[d]; A.src.R(d, q) = s.r[d];}
           }
         }
       }
+      float pr[3] = {0.0f, 0.0f, 0.0f};
       if (do_key) {
-        if (!PushMigrate<false>(A, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+        if (!PushMigrate(A, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+        for (int d=0; d<3; ++d) {
+          A.src.X(d, q) = s.x[d]; A.src.X1(d, q) = s.x1[d];
+          A.src.V(d, q) = s.v[d]; A.src.V1(d, q) = s.v1[d];
+          pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
+        }
+        if (s.parked) A.src.M(q) = 0.0f;
         // a particle that failed to route keeps its block: its key stays valid
         int ni, nj, nk, nju;
         DepositCell(g, A.B[s.gid], s.x, A.sh, ni, nj, nk, nju);
@@ -674,40 +708,39 @@ k_key(KArgs A, int do_key) {
         A.rank[q] = (lk >= 0 && lk < Tl.TC) ? atomicAdd(&lcnt[lk], 1)
                                             : -1 - atomicAdd(&A.count[key], 1);
       }
-      if (A.pending) {
-        for (int d=0; d<3; ++d) rdc[d*CH + tid] = dv[d];
-        rcode[tid] = code;
-      }
+      for (int d=0; d<3; ++d) {rdc[d*CH + tid] = dv[d]; rpr[d*CH + tid] = pr[d];}
+      rcode[tid] = code;
     }
-    if (A.pending) {
-      __syncthreads();
-      if (tid < Tl.TC) {
-        const int s0 = max(st[tid], c0), s1 = min(st[tid + 1], c1);
-        for (int qq = s0; qq < s1; ++qq) {
-          const int r = qq - c0;
-          const int code = rcode[r];
-          const double d0 = rdc[r], d1 = rdc[CH + r], d2 = rdc[2*CH + r];
-          if (code == 1) {dcs[0][0] += d0; dcs[0][1] += d1; dcs[0][2] += d2;}
-          else if (code == 2) {dcs[1][0] += d0; dcs[1][1] += d1; dcs[1][2] += d2;}
-          else if (code == 3) {dcs[2][0] += d0; dcs[2][1] += d1; dcs[2][2] += d2;}
-        }
-      }
-      __syncthreads();
-    }
-  }
-  if (do_key) {
     __syncthreads();
-    for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
+    if (tid < Tl.TC) {
+      const int s0 = max(st[tid], c0), s1 = min(st[tid + 1], c1);
+      for (int qq = s0; qq < s1; ++qq) {
+        const int r = qq - c0;
+        const int code = rcode[r];
+        const double d0 = rdc[r], d1 = rdc[CH + r], d2 = rdc[2*CH + r];
+        if (code == 1) {dcs[0][0] += d0; dcs[0][1] += d1; dcs[0][2] += d2;}
+        else if (code == 2) {dcs[1][0] += d0; dcs[1][1] += d1; dcs[1][2] += d2;}
+        else if (code == 3) {dcs[2][0] += d0; dcs[2][1] += d1; dcs[2][2] += d2;}
+        prs[0] += rpr[r]; prs[1] += rpr[CH + r]; prs[2] += rpr[2*CH + r];
+      }
+    }
+    __syncthreads();
   }
-  if (A.pending && tid < Tl.TC) {
-    // own cell, and the cells beside it in x2 (every target written by exactly one thread)
+  if (do_key)
+    for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
+  if (tid < Tl.TC) {
     int ci, cj, ck;
     CellIdx(Tl, t, tid, ci, cj, ck);
     const long long i0 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
-    for (int d=0; d<3; ++d) {
-      A.dc0[i0 + d*g.ncb] = dcs[1][d];
-      A.dc1[i0 + g.n1 + d*g.ncb] = dcs[2][d];   // to the cell above: "from below" there
-      A.dc2[i0 - g.n1 + d*g.ncb] = dcs[0][d];   // to the cell below: "from above" there
+    if (do_key)
+      for (int d=0; d<3; ++d) pres[i0 + d*g.ncb] = static_cast<double>(prs[d]);
+    if (A.pending) {
+      // own cell, and the cells beside it in x2 (every target written by exactly one thread)
+      for (int d=0; d<3; ++d) {
+        A.dc0[i0 + d*g.ncb] = dcs[1][d];
+        A.dc1[i0 + g.n1 + d*g.ncb] = dcs[2][d];   // to the cell above: "from below" there
+        A.dc2[i0 - g.n1 + d*g.ncb] = dcs[0][d];   // to the cell below: "from above" there
+      }
     }
   }
 }
@@ -780,12 +813,12 @@ __global__ void __launch_bounds__(1024) k_bigsort(const int *start, int *perm, c
 }
 
 //----------------------------------------------------------------------------------------
-// P1: push + migration into the new order, and the deposit of Q = sum m c W/V and
-// P = sum m c W v/V (dust_pm.cpp:225-274), the centre mass m c/V (dust_pm.cpp:660-667) and
-// the push rounding residue of each cell
+// P1: the pushed particles (the key pass pushed them in place) gathered into the new order, and
+// the deposit of Q = sum m c W/V and P = sum m c W v/V (dust_pm.cpp:225-274) and of the centre
+// mass m c/V (dust_pm.cpp:660-667)
 template <int S, bool SY>
-__global__ void __launch_bounds__(kCellThreads)
-k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
+__global__ void __launch_bounds__(kCellThreads, 4)
+k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -794,8 +827,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
   const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
   float *rw = reinterpret_cast<float*>(lds_d);           // [9][CH] weights x0-2, y0-2, z0-2
   float *rv = rw + 9*CH;                                 // [4][CH] mu, mu v
-  float *rp = rv + 4*CH;                                 // [3][CH] m vres/V
-  int *rcode = reinterpret_cast<int*>(rp + 3*CH);        // [CH] centre | (sj + 1) << 2
+  int *rcode = reinterpret_cast<int*>(rv + 4*CH);        // [CH] centre | (sj + 1) << 2
   float *reg = reinterpret_cast<float*>(rcode + CH);     // [4][RN]
   int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
   int *lord = st + Tl.TC + 1;                            // [TC]
@@ -809,7 +841,6 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
   const double ivol = b.ivol;
   const int slab = tid/Tl.TC;
   const int l = lord[tid - slab*Tl.TC];
-  float prs[3] = {0.0f, 0.0f, 0.0f};
   float q3[3] = {0.0f, 0.0f, 0.0f};
   for (int c0 = P0; c0 < P1; c0 += CH) {
     const int c1 = min(c0 + CH, P1);
@@ -819,49 +850,41 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
       const int q = c0 + r;
       if (q >= c1) break;
       const int p = A.perm[q];
-      PState s;
-      LoadP(A.src, p, s, !A.stage1, A.atw_dt != 0.0);
-      const bool ok = PushMigrate(A, s, A.stage1 != 0);
-      const float ts = A.src.TS(p);
-      const float mass = s.parked ? 0.0f : A.src.M(p);
+      const double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
+      const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
+      const float ts = A.src.TS(p), mass = A.src.M(p);
+      const int kp = A.key[p];
       for (int d=0; d<3; ++d) {
-        A.dst.X(d, q) = s.x[d]; A.dst.X1(d, q) = s.x1[d];
-        A.dst.V(d, q) = s.v[d]; A.dst.V1(d, q) = s.v1[d];
+        A.dst.X(d, q) = x[d]; A.dst.X1(d, q) = A.src.X1(d, p);
+        A.dst.V(d, q) = v[d]; A.dst.V1(d, q) = A.src.V1(d, p);
       }
-      A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = s.gid; A.dst.ORIG(q) = A.src.ORIG(p);
+      A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = t.m; A.dst.ORIG(q) = A.src.ORIG(p);
       float wx[3] = {0.0f, 0.0f, 0.0f}, wy[3] = {0.0f, 0.0f, 0.0f}, wz[3] = {0.0f, 0.0f, 0.0f};
-      float mu = 0.0f, pr[3] = {0.0f, 0.0f, 0.0f};
+      float mu = 0.0f;
       int code = 1 << 2;   // no centre, sj = 0
-      if (!ok) {
-        dk::RecordError(A.err, DK_EROUTE, p);
+      const Loc L = Locate<SY>(g, b, x, A.sh);
+      // the key pass placed the particle by this position: the cell must be its slot's
+      if (TileKey(Tl, t.m, L.ci, L.cj, L.ck) != kp) {
+        dk::RecordError(A.err, DK_EHALO, p);
       } else {
-        const Loc L = Locate<SY>(g, A.B[s.gid], s.x, A.sh);
-        // the key pass placed the particle by the same push: the cell must be its slot's
-        if (TileKey(Tl, s.gid, L.ci, L.cj, L.ck) != A.key[p]) {
-          dk::RecordError(A.err, DK_EHALO, p);
-        } else {
-          const float af = static_cast<float>(A.a_dt);
-          mu = mass*(af/(ts + af))*static_cast<float>(ivol);
-          for (int d=0; d<3; ++d)
-            pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
-          int cen = 0;
-          if (mass > 0.0f && L.cen) {
-            // the centre of the unshifted position: this cell or one beside it in x2
-            if (L.jc - L.cj < -1 || L.jc - L.cj > 1) dk::RecordError(A.err, DK_EHALO, p);
-            else cen = L.jc - L.cj + 2;
-          }
-          code = cen | ((L.sj + 1) << 2);
-          W3<S, float>(static_cast<float>(L.dx), wx);
-          W3<S, float>(static_cast<float>(L.dy), wy);
-          W3<S, float>(static_cast<float>(L.dz), wz);
+        const float af = static_cast<float>(A.a_dt);
+        mu = mass*(af/(ts + af))*static_cast<float>(ivol);
+        int cen = 0;
+        if (mass > 0.0f && L.cen) {
+          // the centre of the unshifted position: this cell or one beside it in x2
+          if (L.jc - L.cj < -1 || L.jc - L.cj > 1) dk::RecordError(A.err, DK_EHALO, p);
+          else cen = L.jc - L.cj + 2;
         }
+        code = cen | ((L.sj + 1) << 2);
+        W3<S, float>(static_cast<float>(L.dx), wx);
+        W3<S, float>(static_cast<float>(L.dy), wy);
+        W3<S, float>(static_cast<float>(L.dz), wz);
       }
       rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
       rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
       rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
       rv[r] = mu;
-      rv[CH + r] = mu*s.v[0]; rv[2*CH + r] = mu*s.v[1]; rv[3*CH + r] = mu*s.v[2];
-      rp[r] = pr[0]; rp[CH + r] = pr[1]; rp[2*CH + r] = pr[2];
+      rv[CH + r] = mu*v[0]; rv[2*CH + r] = mu*v[1]; rv[3*CH + r] = mu*v[2];
       rcode[r] = code;
     }
     __syncthreads();
@@ -884,9 +907,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
       const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
       const int code = rcode[r];
       AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
-      if (slab == 0) {
-        prs[0] += rp[r]; prs[1] += rp[CH + r]; prs[2] += rp[2*CH + r];
-      } else if (slab == 1) {
+      if (slab == 1) {
         const int cen = code & 3;
         q3[0] += (cen == 1) ? val[0] : 0.0f;
         q3[1] += (cen == 2) ? val[0] : 0.0f;
@@ -895,12 +916,9 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
     }
     DepositSlabs<float, 4, SY>(Tl, reg, rw, l, slab, acc, spl, spdir);
   }
-  int ci, cj, ck;
-  CellIdx(Tl, t, l, ci, cj, ck);
-  if (slab == 0) {
-    const long long i3 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
-    for (int d=0; d<3; ++d) pres[i3 + d*g.ncb] = static_cast<double>(prs[d]);
-  } else if (slab == 1) {
+  if (slab == 1) {
+    int ci, cj, ck;
+    CellIdx(Tl, t, l, ci, cj, ck);
     const long long i0 = Idx(g, t.m, 1, 0, g.ks + ck, g.js + cj, g.is + ci);
     qt0[i0] = q3[1];
     qt1[i0 + g.n1] = q3[2];
@@ -911,7 +929,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, double *pres, float *scr) {
 
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
 template <int S, bool SY>
-__global__ void __launch_bounds__(kCellThreads)
+__global__ void __launch_bounds__(kCellThreads, 4)
 k_p2(KArgs A, const float *us, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -927,11 +945,14 @@ k_p2(KArgs A, const float *us, float *scr) {
   int *st = rcode + CH;                                  // [TC + 1]
   int *lord = st + Tl.TC + 1;                            // [TC]
   int *hb = lord + Tl.TC;                                // [64]
+  RegionFetch<float> uf;
+  uf.Load(g, Tl, t, us, 3);
   for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
   for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
-  LoadRegion<float>(g, Tl, t, us, 3, us_);
   __syncthreads();
   OrderCells(st, Tl.TC, lord, hb);
+  uf.Store(Tl, 3, us_);
+  __syncthreads();
   const int P0 = st[0], P1 = st[Tl.TC];
   const DevBlock &b = A.B[t.m];
   const double ivol = b.ivol;
@@ -945,6 +966,7 @@ k_p2(KArgs A, const float *us, float *scr) {
       const int q = c0 + r;
       if (q >= c1) break;
       const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
+      const float tsq = A.src.TS(q), mq = A.src.M(q);
       const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
       const int sj = L.sj;
       float wx[3], wy[3], wz[3];
@@ -968,7 +990,7 @@ k_p2(KArgs A, const float *us, float *scr) {
           }
         }
       const float af = static_cast<float>(A.a_dt);
-      const float fac = A.src.M(q)*(af/(A.src.TS(q) + af))*static_cast<float>(ivol);
+      const float fac = mq*(af/(tsq + af))*static_cast<float>(ivol);
       rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
       rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
       rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
@@ -1003,7 +1025,7 @@ k_p2(KArgs A, const float *us, float *scr) {
 // P3: gather u*, kick, drag rate, and the FP64 deposit of the actual impulse
 // (dust_pm.cpp:500-564)
 template <int S, bool SY>
-__global__ void __launch_bounds__(kCellThreads)
+__global__ void __launch_bounds__(kCellThreads, 4)
 k_p3(KArgs A, const float *us, double *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1018,11 +1040,14 @@ k_p3(KArgs A, const float *us, double *scr) {
   int *st = rcode + CH;                                  // [TC + 1]
   int *lord = st + Tl.TC + 1;                            // [TC]
   int *hb = lord + Tl.TC;                                // [64]
+  RegionFetch<float> uf;
+  uf.Load(g, Tl, t, us, 3);
   for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
   for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0;
-  LoadRegion<float>(g, Tl, t, us, 3, us_);
   __syncthreads();
   OrderCells(st, Tl.TC, lord, hb);
+  uf.Store(Tl, 3, us_);
+  __syncthreads();
   const int P0 = st[0], P1 = st[Tl.TC];
   const DevBlock &b = A.B[t.m];
   const double ivol = b.ivol;
@@ -1036,6 +1061,8 @@ k_p3(KArgs A, const float *us, double *scr) {
       const int q = c0 + r;
       if (q >= c1) break;
       const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
+      const float vk0[3] = {A.src.V(0, q), A.src.V(1, q), A.src.V(2, q)};
+      const float tsq = A.src.TS(q), mq = A.src.M(q);
       const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
       const int sj = L.sj;
       const double dx = L.dx, dy = L.dy, dz = L.dz;
@@ -1060,10 +1087,10 @@ k_p3(KArgs A, const float *us, double *scr) {
           }
         }
       const float af = static_cast<float>(A.a_dt);
-      const float cj32 = af/(A.src.TS(q) + af);
-      const double fac = -static_cast<double>(A.src.M(q))*ivol;
+      const float cj32 = af/(tsq + af);
+      const double fac = -static_cast<double>(mq)*ivol;
       for (int d=0; d<3; ++d) {
-        const float vo = A.src.V(d, q);
+        const float vo = vk0[d];
         const float vn = vo + cj32*(u[d] - vo);
         A.src.V(d, q) = vn;
         const double delta = static_cast<double>(vn) - static_cast<double>(vo);
@@ -1378,12 +1405,13 @@ int FillV(dk_ctx *c, T *a, long long n, T v) {
 
 // dynamic LDS of the tile kernels (NT = 3 TC threads, CH = kPPT NT records)
 size_t LdsKey(const Tiles &T) {
-  return (size_t)kKeyThreads*(3*sizeof(double) + sizeof(int)) + sizeof(int)*(2*T.TC + 1);
+  return (size_t)kKeyThreads*(3*sizeof(double) + 3*sizeof(float) + sizeof(int)) +
+         sizeof(int)*(2*T.TC + 1);
 }
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(2*T.TC + 1 + 64); }
 size_t LdsP1(const Tiles &T) {
   const size_t ch = (size_t)kPPT*3*T.TC;
-  return ch*(16*sizeof(float) + sizeof(int)) + sizeof(float)*4*T.RN + LdsTail(T);
+  return ch*(13*sizeof(float) + sizeof(int)) + sizeof(float)*4*T.RN + LdsTail(T);
 }
 size_t LdsP2(const Tiles &T) {
   const size_t ch = (size_t)kPPT*3*T.TC;
@@ -1407,12 +1435,11 @@ size_t LdsP3(const Tiles &T) {
     default: KERNEL<0, false><<<grid, blk, lds, c->stream>>> ARGS; break; \
   }
 
-void LaunchP1(dk_ctx *c, const KArgs &A, float *qt0, float *qt1, float *qt2, double *pres,
-              float *scr) {
+void LaunchP1(dk_ctx *c, const KArgs &A, float *qt0, float *qt1, float *qt2, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 grid(T.nt), blk(3*T.TC);
   const size_t lds = LdsP1(T);
-  SWITCH_SCHEME(k_p1, (A, qt0, qt1, qt2, pres, scr));
+  SWITCH_SCHEME(k_p1, (A, qt0, qt1, qt2, scr));
 }
 
 void LaunchP2(dk_ctx *c, const KArgs &A, const float *us, float *scr) {
@@ -1510,12 +1537,13 @@ int Flush(dk_ctx *c) {
   Fast *f = c->fast;
   if (!f->pending) return DK_OK;
   KArgs A = MakeArgs(c);
-  TIMED("key", (k_key<<<f->T.nt, kKeyThreads, LdsKey(f->T), c->stream>>>(A, 0)));
+  TIMED("key", (k_key<<<f->T.nt, kKeyThreads, LdsKey(f->T), c->stream>>>(A, 0, nullptr)));
   dk_gas_src z;
   std::memset(&z, 0, sizeof(z));
   const long long nact = (long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3;
   TIMED("cell_start", (k_cell_start<<<Grid(nact), kBlock, 0, c->stream>>>(
-      c->g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->pres, 1,
+      c->g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
+      f->pres[f->pres_rd], 1,
       1.0/f->a_prev, 0, 0, 1.0, 0.0, 0.0, 0, z, 0.0)));
   f->pending = 0;
   return dk::Check(hipGetLastError());
@@ -1554,12 +1582,15 @@ int FastCreate(dk_ctx *c) {
   T.LY = 0; while ((1 << T.LY) < T.TY) ++T.LY;
   T.iRX = 1.0f/static_cast<float>(T.RX);
   T.iRY = 1.0f/static_cast<float>(T.RY);
+  T.iRN = 1.0f/static_cast<float>(T.RN);
+  // the u* region of P2/P3 in kRegionPer values per thread
+  if (3*T.RN > kRegionPer*3*T.TC) return DK_EUNSUPPORTED;
   // every tile kernel's LDS within 64 KiB, and the three slab copies of a region within the
   // record area of its kernel
   if (std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)) > 65536 - 1024) return DK_EUNSUPPORTED;
   {
     const size_t ch = (size_t)kPPT*3*T.TC;
-    if (3*4*T.RN*sizeof(float) > ch*16*sizeof(float) ||
+    if (3*4*T.RN*sizeof(float) > ch*13*sizeof(float) ||
         3*3*T.RN*sizeof(float) > ch*12*sizeof(float) ||
         3*3*T.RN*sizeof(double) > ch*6*sizeof(double))
       return DK_EUNSUPPORTED;
@@ -1583,7 +1614,7 @@ int FastCreate(dk_ctx *c) {
   e = e ? e : Alloc((void**)&f->ap, sizeof(float)*3*nc);
   e = e ? e : Alloc((void**)&f->res, sizeof(float)*3*nc);
   e = e ? e : Alloc((void**)&f->dmom, sizeof(double)*3*nc);
-  e = e ? e : Alloc((void**)&f->pres, sizeof(double)*3*nc);
+  for (int q=0; q<2; ++q) e = e ? e : Alloc((void**)&f->pres[q], sizeof(double)*3*nc);
   for (int q=0; q<3; ++q) e = e ? e : Alloc((void**)&f->dc[q], sizeof(double)*3*nc);
   e = e ? e : Alloc((void**)&f->scr_f, sizeof(float)*4*(size_t)T.RN*T.nt);
   e = e ? e : Alloc((void**)&f->scr_d, sizeof(double)*3*(size_t)T.RN*T.nt);
@@ -1593,7 +1624,7 @@ int FastCreate(dk_ctx *c) {
   (void)hipMemset(f->qt[2], 0, sizeof(float)*nc);
   for (int q=0; q<3; ++q) (void)hipMemset(f->dc[q], 0, sizeof(double)*3*nc);
   (void)hipMemset(f->dmom, 0, sizeof(double)*3*nc);
-  (void)hipMemset(f->pres, 0, sizeof(double)*3*nc);
+  for (int q=0; q<2; ++q) (void)hipMemset(f->pres[q], 0, sizeof(double)*3*nc);
   (void)hipMemset(f->ustar, 0, sizeof(float)*3*nc);
   (void)hipMemset(f->res, 0, sizeof(float)*3*nc);
   if (hipcub::DeviceScan::ExclusiveSum(nullptr, f->scan_bytes, f->count, f->start,
@@ -1612,7 +1643,7 @@ void FastDestroy(dk_ctx *c) {
   }
   for (void *p : {(void*)f->key, (void*)f->rank, (void*)f->count, (void*)f->start,
                   (void*)f->perm, (void*)f->big, (void*)f->stay, f->scan_tmp, (void*)f->itag, (void*)f->qdep,
-                  (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom, (void*)f->pres,
+                  (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom, (void*)f->pres[0], (void*)f->pres[1],
                   (void*)f->scr_f, (void*)f->scr_d})
     if (p) (void)hipFree(p);
   for (int q=0; q<3; ++q) {
@@ -1720,7 +1751,9 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   }
   // 1. key pass (with the pending correction), histogram
   if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
-  TIMED("key", (k_key<<<T.nt, kKeyThreads, LdsKey(T), c->stream>>>(A, 1)));
+  // the push of this stage writes its residue into the buffer the next stage's start reads
+  TIMED("key", (k_key<<<T.nt, kKeyThreads, LdsKey(T), c->stream>>>(A, 1,
+                                                                  f->pres[1 - f->pres_rd])));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history
   {
     const int hook = 1;
@@ -1728,7 +1761,8 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     std::memset(&z, 0, sizeof(z));
     const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
     TIMED("cell_start", (k_cell_start<<<Grid(nact), kBlock, 0, c->stream>>>(
-        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->pres, f->pending,
+        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
+        f->pres[f->pres_rd], f->pending,
         f->pending ? 1.0/f->a_prev : 0.0, A.stage1, hook, gam0, gam1, bdt, src ? 1 : 0,
         src ? *src : z, (c->P.back_reaction && st->stage == 2) ? st->a_twid22*st->dt : 0.0)));
   }
@@ -1737,7 +1771,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   // 3. counting sort
   if ((e = SortKeys(c))) return e;
   // 4. push into the new order + deposit Q, P, qt
-  TIMED("p1_push_deposit", LaunchP1(c, A, f->qt[0], f->qt[1], f->qt[2], f->pres, f->scr_f));
+  TIMED("p1_push_deposit", LaunchP1(c, A, f->qt[0], f->qt[1], f->qt[2], f->scr_f));
   f->cur = 1 - f->cur;
   A.src = f->s[f->cur];
   A.dst = f->s[1 - f->cur];
@@ -1765,6 +1799,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   TIMED("split", (k_split<<<Grid(nact), kBlock, 0, c->stream>>>(
       g, c->u0, f->ustar, f->qdep, f->qt[0], f->qt[1], f->qt[2], f->dmom, f->res)));
   f->pending = 1;
+  f->pres_rd = 1 - f->pres_rd;
   f->a_prev = a_dt;
   return dk::Check(hipGetLastError());
 }
