@@ -331,10 +331,9 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
 // particle's work and leaves one record per particle in LDS; a cell phase (one thread per
 // (cell, stencil slab), the slab being the x3 offset) sums the records of its cell, in store
 // order, into its 9 stencil offsets; the sums are then added into the LDS copy of the tile
-// region in 27 phases, one offset per phase, so that no two threads of a phase touch the same
-// word.  The cells are dealt to the threads in order of decreasing particle count (so that the
-// lanes of a wave loop a similar number of times); which thread forms a cell's sums does not
-// change them.  At the end the region is written to the tile's scratch.
+// region in phases, so that no two threads of a phase touch the same word (DepositSlabs in
+// FP32, DepositPhases64 in FP64).  Thread (cell l, slab) forms cell l's sums.  At the end the
+// region is written to the tile's scratch.
 
 constexpr int kPPT = 2;             // particles per thread in the particle phase
 
@@ -493,21 +492,46 @@ __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int s
   __syncthreads();
 }
 
-// the cells of the tile in order of decreasing particle count (64 buckets; the order inside a
-// bucket is arbitrary).  hb: 64 ints of LDS.  Ends with a barrier.
-__device__ inline void OrderCells(const int *st, int TC, int *lord, int *hb) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < 64; i += nt) hb[i] = 0;
+// The FP64 deposit of one chunk's sums into the LDS tile region: 27 phases, one stencil offset
+// per phase, each thread adding its 3 values at its cell + offset through the LDS FP64 adder
+// (no read-back); no two threads of a phase touch one word, so every word's sum is formed in
+// phase order.  (The FP32 regions use DepositSlabs: the LDS FP32 adder is slower than the
+// per-slab copies on gfx950.)  Ends with a barrier.
+template <int NC, bool SY>
+__device__ inline void DepositPhases64(const Tiles &Tl, double *reg, int l, int slab,
+                                       const double (&acc)[3][3][NC],
+                                       const double (&spl)[3][NC], int spdir) {
+  const int lx = l & (Tl.TX - 1), ly = (l >> Tl.LX) & (Tl.TY - 1), lz = l >> (Tl.LX + Tl.LY);
+  const int rbase = ((lz + slab)*Tl.RY + (ly + Tl.SY))*Tl.RX + lx;
   __syncthreads();
-  for (int l = tid; l < TC; l += nt) atomicAdd(&hb[63 - min(st[l + 1] - st[l], 63)], 1);
-  __syncthreads();
-  if (tid == 0) {
-    int s = 0;
-    for (int i=0; i<64; ++i) {const int c = hb[i]; hb[i] = s; s += c;}
+#pragma unroll
+  for (int pc=0; pc<3; ++pc)
+#pragma unroll
+    for (int pb=0; pb<3; ++pb)
+#pragma unroll
+      for (int pa=0; pa<3; ++pa) {
+        if (slab == pc) {
+          const int rc = rbase + pb*Tl.RX + pa;
+#pragma unroll
+          for (int n=0; n<NC; ++n) atomicAdd(&reg[n*Tl.RN + rc], acc[pb][pa][n]);
+        }
+        __syncthreads();
+      }
+  if constexpr (SY) {
+    if (__syncthreads_or(spdir != 0)) {
+#pragma unroll
+      for (int pc=0; pc<3; ++pc)
+#pragma unroll
+        for (int pa=0; pa<3; ++pa) {
+          if (slab == pc && spdir != 0) {
+            const int rc = rbase + (1 + 2*spdir)*Tl.RX + pa;
+#pragma unroll
+            for (int n=0; n<NC; ++n) atomicAdd(&reg[n*Tl.RN + rc], spl[pa][n]);
+          }
+          __syncthreads();
+        }
+    }
   }
-  __syncthreads();
-  for (int l = tid; l < TC; l += nt) lord[atomicAdd(&hb[63 - min(st[l + 1] - st[l], 63)], 1)] = l;
-  __syncthreads();
 }
 
 // the tile region of a padded cell field, fetched into registers (up to kRegionPer values per
@@ -830,17 +854,14 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   int *rcode = reinterpret_cast<int*>(rv + 4*CH);        // [CH] centre | (sj + 1) << 2
   float *reg = reinterpret_cast<float*>(rcode + CH);     // [4][RN]
   int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
-  int *lord = st + Tl.TC + 1;                            // [TC]
-  int *hb = lord + Tl.TC;                                // [64]
   for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
   for (int i = tid; i < 4*Tl.RN; i += NT) reg[i] = 0.0f;
   __syncthreads();
-  OrderCells(st, Tl.TC, lord, hb);
   const int P0 = st[0], P1 = st[Tl.TC];
   const DevBlock &b = A.B[t.m];
   const double ivol = b.ivol;
   const int slab = tid/Tl.TC;
-  const int l = lord[tid - slab*Tl.TC];
+  const int l = tid - slab*Tl.TC;
   float q3[3] = {0.0f, 0.0f, 0.0f};
   for (int c0 = P0; c0 < P1; c0 += CH) {
     const int c1 = min(c0 + CH, P1);
@@ -943,21 +964,18 @@ k_p2(KArgs A, const float *us, float *scr) {
   float *rg = rw + 9*CH;                                 // [3][CH] (m c/V) G u*
   int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1
   int *st = rcode + CH;                                  // [TC + 1]
-  int *lord = st + Tl.TC + 1;                            // [TC]
-  int *hb = lord + Tl.TC;                                // [64]
   RegionFetch<float> uf;
   uf.Load(g, Tl, t, us, 3);
   for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
   for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
   __syncthreads();
-  OrderCells(st, Tl.TC, lord, hb);
   uf.Store(Tl, 3, us_);
   __syncthreads();
   const int P0 = st[0], P1 = st[Tl.TC];
   const DevBlock &b = A.B[t.m];
   const double ivol = b.ivol;
   const int slab = tid/Tl.TC;
-  const int l = lord[tid - slab*Tl.TC];
+  const int l = tid - slab*Tl.TC;
   for (int c0 = P0; c0 < P1; c0 += CH) {
     const int c1 = min(c0 + CH, P1);
 #pragma unroll 1
@@ -1038,21 +1056,18 @@ k_p3(KArgs A, const float *us, double *scr) {
   float *us_ = reinterpret_cast<float*>(rd + 6*CH);      // [3][RN] u*
   int *rcode = reinterpret_cast<int*>(us_ + 3*Tl.RN);    // [CH] sj + 1
   int *st = rcode + CH;                                  // [TC + 1]
-  int *lord = st + Tl.TC + 1;                            // [TC]
-  int *hb = lord + Tl.TC;                                // [64]
   RegionFetch<float> uf;
   uf.Load(g, Tl, t, us, 3);
   for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
   for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0;
   __syncthreads();
-  OrderCells(st, Tl.TC, lord, hb);
   uf.Store(Tl, 3, us_);
   __syncthreads();
   const int P0 = st[0], P1 = st[Tl.TC];
   const DevBlock &b = A.B[t.m];
   const double ivol = b.ivol;
   const int slab = tid/Tl.TC;
-  const int l = lord[tid - slab*Tl.TC];
+  const int l = tid - slab*Tl.TC;
   for (int c0 = P0; c0 < P1; c0 += CH) {
     const int c1 = min(c0 + CH, P1);
 #pragma unroll 1
@@ -1121,7 +1136,7 @@ k_p3(KArgs A, const float *us, double *scr) {
       const double val[3] = {rd[3*CH + r], rd[4*CH + r], rd[5*CH + r]};
       AddSlab<double, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
     }
-    DepositSlabs<double, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
+    DepositPhases64<3, SY>(Tl, dep, l, slab, acc, spl, spdir);
   }
   FlushRegion<double>(Tl, t, dep, scr, 3);
 }
@@ -1408,7 +1423,7 @@ size_t LdsKey(const Tiles &T) {
   return (size_t)kKeyThreads*(3*sizeof(double) + 3*sizeof(float) + sizeof(int)) +
          sizeof(int)*(2*T.TC + 1);
 }
-size_t LdsTail(const Tiles &T) { return sizeof(int)*(2*T.TC + 1 + 64); }
+size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1); }
 size_t LdsP1(const Tiles &T) {
   const size_t ch = (size_t)kPPT*3*T.TC;
   return ch*(13*sizeof(float) + sizeof(int)) + sizeof(float)*4*T.RN + LdsTail(T);
