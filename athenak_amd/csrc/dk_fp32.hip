@@ -4,9 +4,11 @@
 // exchange D and every global sum stay FP64; velocities, registers, drag rates, stopping times,
 // masses and the mesh work fields (Q, P, u*, the dc1 operator result, the split residual) are
 // FP32.  The particle impulse a kick hands to the gas is the exact difference of the FP32
-// velocities before and after (exact in FP64), deposited with FP64 weights, and the consistent
-// gas/dust split gives the gas D minus the dust's actual change: total momentum is conserved to
-// FP64 round-off.
+// velocities before and after (exact in FP64); it is deposited in FP32, and every tile's defect
+// (its particles' FP64 impulse sum minus the FP64 sum of its FP32 deposit) is added to one of
+// its cells, so that the FP64 field D holds the particles' total impulse to FP64 round-off; the
+// consistent gas/dust split gives the gas D minus the dust's actual change: total momentum is
+// conserved to FP64 round-off.
 //
 // Order: particles are kept in a store sorted by (block, cell of the deposit position).  Every
 // stage a key pass computes each particle's new cell, a counting sort (atomic histogram,
@@ -106,9 +108,10 @@ struct Fast {
   double *pres[2] = {nullptr, nullptr};   // push rounding residue per cell (to the gas):
   int pres_rd = 0;                        // the buffer of the last push, read by the next stage
   float *scr_f = nullptr;
-  double *scr_d = nullptr;
+  double *defect = nullptr;    // [nt][3] the FP64 defect of every tile's impulse deposit
   int pending = 0;
   double a_prev = 0.0;
+  int regs_dead = 0;                // the store holds no registers (after a step's last stage)
 };
 
 namespace {
@@ -141,8 +144,12 @@ struct KArgs {
   dk::ErrWord *err;
   // stage numbers
   double gam0, gam1, beta_dt, atw_dt, a_dt, sh, dt;
+  double inv_a_dt, inv_a_prev;      // 1/a_dt, 1/a_prev (the drag rates R = delta/a)
   double ysh, dsh;                  // shear crossing offsets (bvals_part.cpp, dust.cpp)
   int stage1, sbox, strat, shear;
+  // the stage registers x1, v1 and R of this stage are read by a later stage (0 at the last
+  // active stage: the next step's stage 1 sets them before any use, so they are not kept)
+  int regs_live;
   double qs, om;
   double c_fx, c_fy, c_fz, c_sh;    // 2 om, -(2 - q) om, -om^2 (stratified), q om; 0 without
   double lxm, lym, lzm;             // mesh lengths
@@ -331,9 +338,9 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
 // particle's work and leaves one record per particle in LDS; a cell phase (one thread per
 // (cell, stencil slab), the slab being the x3 offset) sums the records of its cell, in store
 // order, into its 9 stencil offsets; the sums are then added into the LDS copy of the tile
-// region in phases, so that no two threads of a phase touch the same word (DepositSlabs in
-// FP32, DepositPhases64 in FP64).  Thread (cell l, slab) forms cell l's sums.  At the end the
-// region is written to the tile's scratch.
+// region in phases, so that no two threads of a phase touch the same word (DepositSlabs).
+// Thread (cell l, slab) forms cell l's sums.  At the end the region is written to the tile's
+// scratch.
 
 constexpr int kPPT = 2;             // particles per thread in the particle phase
 
@@ -374,6 +381,19 @@ __host__ __device__ inline long long TileKey(const Tiles &T, int m, int ci, int 
 // distance 1/(2 d) of (i + 1/2)/d from an integer
 __device__ inline int FDiv(int i, float inv) {
   return static_cast<int>((static_cast<float>(i) + 0.5f)*inv);
+}
+
+// the sum of v over the na active lanes of a wavefront (the first na), in a fixed order: a
+// butterfly for a full wavefront, else lane order; the result is lane 0's
+__device__ inline double WaveSum(double v, int na) {
+  if (na >= 64) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+  }
+  double s = 0.0;
+  for (int j = 0; j < na; ++j) s += __shfl(v, j, 64);
+  return s;
 }
 
 // the tile-local cell of store slot q: the last c with st[c] <= q (st[0] <= q < st[TC])
@@ -492,48 +512,6 @@ __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int s
   __syncthreads();
 }
 
-// The FP64 deposit of one chunk's sums into the LDS tile region: 27 phases, one stencil offset
-// per phase, each thread adding its 3 values at its cell + offset through the LDS FP64 adder
-// (no read-back); no two threads of a phase touch one word, so every word's sum is formed in
-// phase order.  (The FP32 regions use DepositSlabs: the LDS FP32 adder is slower than the
-// per-slab copies on gfx950.)  Ends with a barrier.
-template <int NC, bool SY>
-__device__ inline void DepositPhases64(const Tiles &Tl, double *reg, int l, int slab,
-                                       const double (&acc)[3][3][NC],
-                                       const double (&spl)[3][NC], int spdir) {
-  const int lx = l & (Tl.TX - 1), ly = (l >> Tl.LX) & (Tl.TY - 1), lz = l >> (Tl.LX + Tl.LY);
-  const int rbase = ((lz + slab)*Tl.RY + (ly + Tl.SY))*Tl.RX + lx;
-  __syncthreads();
-#pragma unroll
-  for (int pc=0; pc<3; ++pc)
-#pragma unroll
-    for (int pb=0; pb<3; ++pb)
-#pragma unroll
-      for (int pa=0; pa<3; ++pa) {
-        if (slab == pc) {
-          const int rc = rbase + pb*Tl.RX + pa;
-#pragma unroll
-          for (int n=0; n<NC; ++n) atomicAdd(&reg[n*Tl.RN + rc], acc[pb][pa][n]);
-        }
-        __syncthreads();
-      }
-  if constexpr (SY) {
-    if (__syncthreads_or(spdir != 0)) {
-#pragma unroll
-      for (int pc=0; pc<3; ++pc)
-#pragma unroll
-        for (int pa=0; pa<3; ++pa) {
-          if (slab == pc && spdir != 0) {
-            const int rc = rbase + (1 + 2*spdir)*Tl.RX + pa;
-#pragma unroll
-            for (int n=0; n<NC; ++n) atomicAdd(&reg[n*Tl.RN + rc], spl[pa][n]);
-          }
-          __syncthreads();
-        }
-    }
-  }
-}
-
 // the tile region of a padded cell field, fetched into registers (up to kRegionPer values per
 // thread) so that its latency overlaps the LDS set-up, then stored into LDS
 constexpr int kRegionPer = 6;
@@ -578,18 +556,25 @@ __host__ __device__ inline long long ScrRow(const Tiles &T, int m, int tz, int t
          ((long long)T.ntx*T.RX);
 }
 
-template <typename T>
+// (sum: also the FP64 sum of the words this thread writes, per component)
+template <typename T, int NV>
 __device__ inline void FlushRegion(const Tiles &Tl, const TileInfo &t, const T *reg, T *scr,
-                                   int nv) {
+                                   double *sum = nullptr) {
   __syncthreads();
-  for (int v=0; v<nv; ++v)
+#pragma unroll 1
+  for (int v=0; v<NV; ++v) {
+    double s = 0.0;
     for (int rc = threadIdx.x; rc < Tl.RN; rc += blockDim.x) {
       const int rr = FDiv(rc, Tl.iRX);
       const int rx = rc - rr*Tl.RX;
       const int rz = FDiv(rr, Tl.iRY);
       const int ry = rr - rz*Tl.RY;
-      scr[ScrRow(Tl, t.m, t.tz, t.ty, nv, v, rz, ry) + t.tx*Tl.RX + rx] = reg[v*Tl.RN + rc];
+      const T val = reg[v*Tl.RN + rc];
+      scr[ScrRow(Tl, t.m, t.tz, t.ty, NV, v, rz, ry) + t.tx*Tl.RX + rx] = val;
+      s += static_cast<double>(val);
     }
+    if (sum != nullptr) sum[v] = s;
+  }
 }
 
 // active cell indices of tile-local cell l
@@ -705,12 +690,15 @@ k_key(KArgs A, int do_key, double *pres) {
               // the recorded rate takes the actual change, so that the history terms of dust
               // and gas stay opposite (dust_pm.cpp:694)
               const double delta = static_cast<double>(vn) - static_cast<double>(vo);
-              s.r[d] += delta/A.a_prev;
+              s.r[d] += delta*A.inv_a_prev;
               dv[d] = static_cast<double>(mass)*delta*ivol;
             }
             code = dj + 2;
             if (!do_key)
-              for (int d=0; d<3; ++d) {A.src.V(d, q) = s.v[d]; A.src.R(d, q) = s.r[d];}
+              for (int d=0; d<3; ++d) {
+                A.src.V(d, q) = s.v[d];
+                if (A.regs_live) A.src.R(d, q) = s.r[d];
+              }
           }
         }
       }
@@ -718,8 +706,9 @@ k_key(KArgs A, int do_key, double *pres) {
       if (do_key) {
         if (!PushMigrate(A, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
         for (int d=0; d<3; ++d) {
-          A.src.X(d, q) = s.x[d]; A.src.X1(d, q) = s.x1[d];
-          A.src.V(d, q) = s.v[d]; A.src.V1(d, q) = s.v1[d];
+          A.src.X(d, q) = s.x[d];
+          A.src.V(d, q) = s.v[d];
+          if (A.regs_live) {A.src.X1(d, q) = s.x1[d]; A.src.V1(d, q) = s.v1[d];}
           pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
         }
         if (s.parked) A.src.M(q) = 0.0f;
@@ -875,10 +864,9 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
       const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
       const float ts = A.src.TS(p), mass = A.src.M(p);
       const int kp = A.key[p];
-      for (int d=0; d<3; ++d) {
-        A.dst.X(d, q) = x[d]; A.dst.X1(d, q) = A.src.X1(d, p);
-        A.dst.V(d, q) = v[d]; A.dst.V1(d, q) = A.src.V1(d, p);
-      }
+      for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d];}
+      if (A.regs_live)
+        for (int d=0; d<3; ++d) {A.dst.X1(d, q) = A.src.X1(d, p); A.dst.V1(d, q) = A.src.V1(d, p);}
       A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = t.m; A.dst.ORIG(q) = A.src.ORIG(p);
       float wx[3] = {0.0f, 0.0f, 0.0f}, wy[3] = {0.0f, 0.0f, 0.0f}, wz[3] = {0.0f, 0.0f, 0.0f};
       float mu = 0.0f;
@@ -945,7 +933,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
     qt1[i0 + g.n1] = q3[2];
     qt2[i0 - g.n1] = q3[0];
   }
-  FlushRegion<float>(Tl, t, reg, scr, 4);
+  FlushRegion<float, 4>(Tl, t, reg, scr);
 }
 
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
@@ -1037,29 +1025,33 @@ k_p2(KArgs A, const float *us, float *scr) {
     }
     DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
   }
-  FlushRegion<float>(Tl, t, dep, scr, 3);
+  FlushRegion<float, 3>(Tl, t, dep, scr);
 }
 
-// P3: gather u*, kick, drag rate, and the FP64 deposit of the actual impulse
-// (dust_pm.cpp:500-564)
-template <int S, bool SY>
-__global__ void __launch_bounds__(kCellThreads, 4)
-k_p3(KArgs A, const float *us, double *scr) {
+// P3: gather u*, kick, drag rate, and the deposit of the actual impulse (dust_pm.cpp:500-564).
+// The impulse is deposited in FP32; the tile's defect, the FP64 sum of its particles' impulses
+// minus the FP64 sum of its FP32 region, goes to the combine, which adds it to the tile's first
+// cell: the FP64 field then holds exactly (to FP64 round-off) the particles' total impulse.
+// (REGS: the drag rate R is written, for a later stage; a template argument, as a run-time test
+// around the store costs the kernel registers)
+template <int S, bool SY, bool REGS>
+__device__ inline void P3Body(const KArgs &A, const float *us, float *scr, double *defect) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
   const int tile = TileIndex(Tl);
   const TileInfo t = TileOf(Tl, tile);
   const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
-  double *dep = lds_d;                                   // [3][RN] the deposit (FP64)
-  double *rd = dep + 3*Tl.RN;                            // [6][CH] del x,y,z; -m delta/V
-  float *us_ = reinterpret_cast<float*>(rd + 6*CH);      // [3][RN] u*
-  int *rcode = reinterpret_cast<int*>(us_ + 3*Tl.RN);    // [CH] sj + 1
+  float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
+  float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
+  float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
+  float *rv = rw + 9*CH;                                 // [3][CH] -m delta/V
+  int *rcode = reinterpret_cast<int*>(rv + 3*CH);        // [CH] sj + 1
   int *st = rcode + CH;                                  // [TC + 1]
   RegionFetch<float> uf;
   uf.Load(g, Tl, t, us, 3);
   for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
-  for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0;
+  for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
   __syncthreads();
   uf.Store(Tl, 3, us_);
   __syncthreads();
@@ -1068,6 +1060,8 @@ k_p3(KArgs A, const float *us, double *scr) {
   const double ivol = b.ivol;
   const int slab = tid/Tl.TC;
   const int l = tid - slab*Tl.TC;
+  // this thread's particles' impulses in FP64, in its fixed particle order
+  double ex[3] = {0.0, 0.0, 0.0};
   for (int c0 = P0; c0 < P1; c0 += CH) {
     const int c1 = min(c0 + CH, P1);
 #pragma unroll 1
@@ -1080,11 +1074,10 @@ k_p3(KArgs A, const float *us, double *scr) {
       const float tsq = A.src.TS(q), mq = A.src.M(q);
       const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
       const int sj = L.sj;
-      const double dx = L.dx, dy = L.dy, dz = L.dz;
       float wx[3], wy[3], wz[3];
-      W3<S, float>(static_cast<float>(dx), wx);
-      W3<S, float>(static_cast<float>(dy), wy);
-      W3<S, float>(static_cast<float>(dz), wz);
+      W3<S, float>(static_cast<float>(L.dx), wx);
+      W3<S, float>(static_cast<float>(L.dy), wy);
+      W3<S, float>(static_cast<float>(L.dz), wz);
       const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
       float u[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -1109,42 +1102,74 @@ k_p3(KArgs A, const float *us, double *scr) {
         const float vn = vo + cj32*(u[d] - vo);
         A.src.V(d, q) = vn;
         const double delta = static_cast<double>(vn) - static_cast<double>(vo);
-        A.src.R(d, q) = delta/A.a_dt;
-        rd[(3 + d)*CH + r] = fac*delta;
+        if constexpr (REGS) A.src.R(d, q) = delta*A.inv_a_dt;
+        const double imp = fac*delta;
+        ex[d] += imp;
+        rv[d*CH + r] = static_cast<float>(imp);
       }
-      rd[r] = dx; rd[CH + r] = dy; rd[2*CH + r] = dz;
+      rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
+      rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
+      rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
       rcode[r] = sj + 1;
     }
     __syncthreads();
-    double acc[3][3][3], spl[3][3];
+    float acc[3][3][3], spl[3][3];
 #pragma unroll
     for (int a=0; a<3; ++a)
 #pragma unroll
       for (int n=0; n<3; ++n) {
-        spl[a][n] = 0.0;
+        spl[a][n] = 0.0f;
 #pragma unroll
-        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0;
+        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
       }
     int spdir = 0;
     const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
     for (int qq = s0; qq < s1; ++qq) {
       const int r = qq - c0;
-      double wy[3], wx[3];
-      W3<S, double>(rd[CH + r], wy);
-      W3<S, double>(rd[r], wx);
-      const double wz = W1<S, double>(slab, rd[2*CH + r]);
-      const double val[3] = {rd[3*CH + r], rd[4*CH + r], rd[5*CH + r]};
-      AddSlab<double, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+      const float wz = rw[(6 + slab)*CH + r];
+      const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
+      const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+      const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
+      AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
     }
-    DepositPhases64<3, SY>(Tl, dep, l, slab, acc, spl, spdir);
+    DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
   }
-  FlushRegion<double>(Tl, t, dep, scr, 3);
+  // the flush, every thread summing (FP64) the words it writes; the defect: every thread's
+  // impulses minus those words, summed over each wavefront, then over the wavefronts in order
+  // (the record area is free after the last deposit's closing barrier)
+  double ws[3];
+  FlushRegion<float, 3>(Tl, t, dep, scr, ws);
+  double *red = reinterpret_cast<double*>(rw);           // [NT/64][3]
+  const int wv = tid >> 6;
+  for (int d=0; d<3; ++d) {
+    const double a = WaveSum(ex[d] - ws[d], min(64, NT - (wv << 6)));
+    if ((tid & 63) == 0) red[3*wv + d] = a;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double a = 0.0;
+    for (int w = 0; w < (NT + 63) >> 6; ++w) a += red[3*w + tid];
+    defect[3*(t.key0/Tl.TC) + tid] = a;
+  }
+}
+
+template <int S, bool SY>
+__global__ void __launch_bounds__(kCellThreads, 4)
+k_p3(KArgs A, const float *us, float *scr, double *defect) {
+  P3Body<S, SY, true>(A, us, scr, defect);
+}
+
+template <int S, bool SY>
+__global__ void __launch_bounds__(kCellThreads, 4)
+k_p3n(KArgs A, const float *us, float *scr, double *defect) {
+  P3Body<S, SY, false>(A, us, scr, defect);
 }
 
 // every padded cell sums the tile regions covering it, in ascending tile order (grid: x over
-// the cells of a block, y over the blocks)
-template <typename T, int NV>
-__global__ void k_combine(Geo g, Tiles Tl, const T *scr, T *out) {
+// the cells of a block, y over the blocks), in the output type TO; with defect, the first cell of
+// every tile then adds the tile's defect (NV values per tile, tile-major)
+template <typename T, int NV, typename TO = T>
+__global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *defect) {
   const int m = blockIdx.y;
   const int ncb = static_cast<int>(g.ncb);
   for (int rr = blockIdx.x*blockDim.x + threadIdx.x; rr < ncb; rr += gridDim.x*blockDim.x) {
@@ -1154,9 +1179,9 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, T *out) {
     const int k = t2/g.n2;
     // region coordinates relative to tile 0: r = (i - is + 1) - tx*TX etc.
     const int ai = i - g.is + 1, aj = j - g.js + 1 + Tl.SY, ak = k - g.ks + 1;
-    T sum[NV];
+    TO sum[NV];
 #pragma unroll
-    for (int v=0; v<NV; ++v) sum[v] = T(0);
+    for (int v=0; v<NV; ++v) sum[v] = TO(0);
     if (ai >= 0 && aj >= 0 && ak >= 0) {
       const int tz0 = max(0, (ak - Tl.RZ + Tl.TZ)/Tl.TZ), tz1 = min(Tl.ntz - 1, ak/Tl.TZ);
       const int ty0 = max(0, (aj - Tl.RY + Tl.TY)/Tl.TY), ty1 = min(Tl.nty - 1, aj/Tl.TY);
@@ -1173,9 +1198,19 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, T *out) {
             const long long base = ScrRow(Tl, m, tz, ty, NV, 0, rz, ry) + tx*Tl.RX + rx;
             const long long vstride = (long long)Tl.RZ*Tl.RY*Tl.ntx*Tl.RX;
 #pragma unroll
-            for (int v=0; v<NV; ++v) sum[v] += scr[base + v*vstride];
+            for (int v=0; v<NV; ++v) sum[v] += static_cast<TO>(scr[base + v*vstride]);
           }
         }
+      }
+      // the tile whose first cell this is (active cells only)
+      const int li = ai - 1, lj = aj - 1 - Tl.SY, lk = ak - 1;
+      if (defect != nullptr && li < Tl.ntx*Tl.TX && lj >= 0 && lj < Tl.nty*Tl.TY &&
+          lk < Tl.ntz*Tl.TZ && (li & (Tl.TX - 1)) == 0 && (lj & (Tl.TY - 1)) == 0 &&
+          lk % Tl.TZ == 0 && li >= 0 && lk >= 0) {
+        const long long ct = (long long)m*Tl.ntb +
+                             ((lk/Tl.TZ)*Tl.nty + (lj >> Tl.LY))*Tl.ntx + (li >> Tl.LX);
+#pragma unroll
+        for (int v=0; v<NV; ++v) sum[v] += defect[NV*ct + v];
       }
     }
 #pragma unroll
@@ -1346,15 +1381,18 @@ __global__ void k_permute_all(Store src, Store dst, const int *perm, long long n
   }
 }
 
+// (regs_dead: the store keeps no registers; they are written as x1 = x, v1 = v, R = 0, the values
+// the next stage 1 gives them before any use)
 __global__ void k_export(Store S, long long np, const int32_t *itag, long long np0, double *r,
-                         int32_t *iv, long long cap) {
+                         int32_t *iv, long long cap, int regs_dead) {
   GRID_STRIDE(q, np) {
     for (int d=0; d<3; ++d) {
-      r[(long long)(DK_IPX + 2*d)*cap + q] = S.X(d, q);
-      r[(long long)(DK_IPVX + 2*d)*cap + q] = static_cast<double>(S.V(d, q));
-      r[(long long)(DK_IPX1 + 2*d)*cap + q] = S.X1(d, q);
-      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = static_cast<double>(S.V1(d, q));
-      r[(long long)(DK_IPRX + d)*cap + q] = S.R(d, q);
+      const double x = S.X(d, q), v = static_cast<double>(S.V(d, q));
+      r[(long long)(DK_IPX + 2*d)*cap + q] = x;
+      r[(long long)(DK_IPVX + 2*d)*cap + q] = v;
+      r[(long long)(DK_IPX1 + 2*d)*cap + q] = regs_dead ? x : S.X1(d, q);
+      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = regs_dead ? v : static_cast<double>(S.V1(d, q));
+      r[(long long)(DK_IPRX + d)*cap + q] = regs_dead ? 0.0 : S.R(d, q);
     }
     r[(long long)DK_IPTS*cap + q] = static_cast<double>(S.TS(q));
     r[(long long)DK_IPM*cap + q] = static_cast<double>(S.M(q));
@@ -1432,11 +1470,7 @@ size_t LdsP2(const Tiles &T) {
   const size_t ch = (size_t)kPPT*3*T.TC;
   return sizeof(float)*6*T.RN + ch*(12*sizeof(float) + sizeof(int)) + LdsTail(T);
 }
-size_t LdsP3(const Tiles &T) {
-  const size_t ch = (size_t)kPPT*3*T.TC;
-  return sizeof(double)*3*T.RN + ch*(6*sizeof(double) + sizeof(int)) +
-         sizeof(float)*3*T.RN + LdsTail(T);
-}
+size_t LdsP3(const Tiles &T) { return LdsP2(T); }   // the same areas
 
 // a tile kernel instantiated for the deposit scheme and the frame-shift flag of the context,
 // launched with grid, blk, lds in scope
@@ -1464,11 +1498,12 @@ void LaunchP2(dk_ctx *c, const KArgs &A, const float *us, float *scr) {
   SWITCH_SCHEME(k_p2, (A, us, scr));
 }
 
-void LaunchP3(dk_ctx *c, const KArgs &A, const float *us, double *scr) {
+void LaunchP3(dk_ctx *c, const KArgs &A, const float *us, float *scr, double *defect) {
   const Tiles &T = c->fast->T;
   const dim3 grid(T.nt), blk(3*T.TC);
   const size_t lds = LdsP3(T);
-  SWITCH_SCHEME(k_p3, (A, us, scr));
+  if (A.regs_live) SWITCH_SCHEME(k_p3, (A, us, scr, defect))
+  else SWITCH_SCHEME(k_p3n, (A, us, scr, defect))
 }
 
 dim3 CombineGrid(const dk_ctx *c) {
@@ -1524,6 +1559,8 @@ KArgs MakeArgs(dk_ctx *c) {
   A.lym = c->M.x2max - c->M.x2min;
   A.lzm = c->M.x3max - c->M.x3min;
   A.pending = f->pending; A.a_prev = f->a_prev;
+  A.inv_a_prev = f->pending ? 1.0/f->a_prev : 0.0;
+  A.regs_live = !f->regs_dead;
   A.res = f->res; A.dc0 = f->dc[0]; A.dc1 = f->dc[1]; A.dc2 = f->dc[2];
   return A;
 }
@@ -1576,8 +1613,8 @@ int FastCreate(dk_ctx *c) {
   c->fast = f;
   Tiles &T = f->T;
   // tile shape: 8 x 4 x 4 cells by default (or the largest powers of two dividing the block
-  // below those); DK_TILE=AxBxC selects another shape whose sides divide the block, whose first
-  // two sides are powers of two and whose cell count is at most kMaxTileCells
+  // below those); DK_TILE=AxBxC selects another shape whose sides are powers of two dividing the
+  // block and whose cell count is at most kMaxTileCells
   T.TX = Largest(c->g.nx1, 8);
   T.TY = Largest(c->g.nx2, 4);
   T.TZ = Largest(c->g.nx3, 4);
@@ -1585,7 +1622,8 @@ int FastCreate(dk_ctx *c) {
     int a = 0, b = 0, d = 0;
     if (std::sscanf(e, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 &&
         c->g.nx1 % a == 0 && c->g.nx2 % b == 0 && c->g.nx3 % d == 0 &&
-        a*b*d <= kMaxTileCells && (a & (a - 1)) == 0 && (b & (b - 1)) == 0) {
+        a*b*d <= kMaxTileCells && (a & (a - 1)) == 0 && (b & (b - 1)) == 0 &&
+        (d & (d - 1)) == 0) {
       T.TX = a; T.TY = b; T.TZ = d;
     }
   }
@@ -1607,7 +1645,7 @@ int FastCreate(dk_ctx *c) {
     const size_t ch = (size_t)kPPT*3*T.TC;
     if (3*4*T.RN*sizeof(float) > ch*13*sizeof(float) ||
         3*3*T.RN*sizeof(float) > ch*12*sizeof(float) ||
-        3*3*T.RN*sizeof(double) > ch*6*sizeof(double))
+        3*3*T.RN*sizeof(float) > ch*12*sizeof(float))
       return DK_EUNSUPPORTED;
   }
   // a block's first and last x2 rows must differ (cells centred one row off in x2)
@@ -1632,7 +1670,7 @@ int FastCreate(dk_ctx *c) {
   for (int q=0; q<2; ++q) e = e ? e : Alloc((void**)&f->pres[q], sizeof(double)*3*nc);
   for (int q=0; q<3; ++q) e = e ? e : Alloc((void**)&f->dc[q], sizeof(double)*3*nc);
   e = e ? e : Alloc((void**)&f->scr_f, sizeof(float)*4*(size_t)T.RN*T.nt);
-  e = e ? e : Alloc((void**)&f->scr_d, sizeof(double)*3*(size_t)T.RN*T.nt);
+  e = e ? e : Alloc((void**)&f->defect, sizeof(double)*3*(size_t)T.nt);
   if (e) return e;
   (void)hipMemset(f->qt[0], 0, sizeof(float)*nc);
   (void)hipMemset(f->qt[1], 0, sizeof(float)*nc);
@@ -1659,7 +1697,7 @@ void FastDestroy(dk_ctx *c) {
   for (void *p : {(void*)f->key, (void*)f->rank, (void*)f->count, (void*)f->start,
                   (void*)f->perm, (void*)f->big, (void*)f->stay, f->scan_tmp, (void*)f->itag, (void*)f->qdep,
                   (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom, (void*)f->pres[0], (void*)f->pres[1],
-                  (void*)f->scr_f, (void*)f->scr_d})
+                  (void*)f->scr_f, (void*)f->defect})
     if (p) (void)hipFree(p);
   for (int q=0; q<3; ++q) {
     if (f->qt[q]) (void)hipFree(f->qt[q]);
@@ -1699,6 +1737,7 @@ int FastBind(dk_ctx *c) {
   f->np = c->np;
   f->cur = 0;
   f->pending = 0;
+  f->regs_dead = 0;
   if (f->np == 0) return DK_OK;
   k_bind<<<Grid(f->np), kBlock, 0, c->stream>>>(c->rdata, c->idata, (long long)c->cap, f->np,
                                                  f->s[0], f->itag);
@@ -1719,7 +1758,8 @@ int FastExport(dk_ctx *c) {
   int e;
   if ((e = Flush(c))) return e;
   k_export<<<Grid(f->np), kBlock, 0, c->stream>>>(f->s[f->cur], f->np, f->itag, f->np,
-                                                   c->rdata, c->idata, (long long)c->cap);
+                                                   c->rdata, c->idata, (long long)c->cap,
+                                                   f->regs_dead);
   return dk::Check(hipGetLastError());
 }
 
@@ -1756,9 +1796,12 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   const long long nact = (long long)g.nmb*g.nx1*g.nx2*g.nx3;
   KArgs A = MakeArgs(c);
   A.stage1 = (st->stage == 1);
+  // the last active stage (stage 3 of a three-stage scheme is the identity)
+  A.regs_live = !(st->stage == st->nexp_stages || (st->nexp_stages == 3 && st->stage == 2));
   A.gam0 = st->gam0[s]; A.gam1 = st->gam1[s]; A.beta_dt = st->beta[s]*st->dt;
   A.atw_dt = (st->stage == 2) ? st->a_twid22*st->dt : 0.0;
   A.a_dt = a_dt; A.sh = sh; A.dt = st->dt;
+  A.inv_a_dt = 1.0/a_dt;
   if (A.shear) {
     const double lx = c->M.x1max - c->M.x1min, ly = c->M.x2max - c->M.x2min;
     A.ysh = std::fmod(c->P.qshear*c->P.omega0*lx*st->time, ly);
@@ -1791,7 +1834,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   A.src = f->s[f->cur];
   A.dst = f->s[1 - f->cur];
   TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
-      g, T, f->scr_f, f->qdep)));
+      g, T, f->scr_f, f->qdep, nullptr)));
   if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
   // 5. local guess + dc1/dc2 sweeps
   TIMED("guess", (k_guess<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ustar)));
@@ -1800,20 +1843,21 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
     TIMED("p2_operator", LaunchP2(c, A, f->ustar, f->scr_f));
     TIMED("combine", (k_combine<float, 3><<<CombineGrid(c), kBlock, 0, c->stream>>>(
-        g, T, f->scr_f, f->ap)));
+        g, T, f->scr_f, f->ap, nullptr)));
     if ((e = FoldT<float>(c, f->ap, 3, st->time))) return e;
     TIMED("dc", (k_dc<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ap, f->ustar)));
   }
   if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
   // 6. gather-kick + impulse deposit
-  TIMED("p3_gather_kick", LaunchP3(c, A, f->ustar, f->scr_d));
-  TIMED("combine", (k_combine<double, 3><<<CombineGrid(c), kBlock, 0, c->stream>>>(
-      g, T, f->scr_d, f->dmom)));
+  TIMED("p3_gather_kick", LaunchP3(c, A, f->ustar, f->scr_f, f->defect));
+  TIMED("combine", (k_combine<float, 3, double><<<CombineGrid(c), kBlock, 0, c->stream>>>(
+      g, T, f->scr_f, f->dmom, f->defect)));
   if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;
   // 7. consistent split: the dust's residual per unit coupled mass
   TIMED("split", (k_split<<<Grid(nact), kBlock, 0, c->stream>>>(
       g, c->u0, f->ustar, f->qdep, f->qt[0], f->qt[1], f->qt[2], f->dmom, f->res)));
   f->pending = 1;
+  f->regs_dead = !A.regs_live;
   f->pres_rd = 1 - f->pres_rd;
   f->a_prev = a_dt;
   return dk::Check(hipGetLastError());

@@ -206,3 +206,40 @@ def test_fp32_graph_replays_match_eager():
         say("graph replay step %d identical" % step)
     ge.close()
     gg.close()
+
+
+def test_fp32_registers_mid_step_and_at_step_end():
+    # after stage 1 the registers x1, v1, R are live and exported as the oracle's; after the
+    # step's last stage the store keeps none (the next stage 1 sets them before any use) and
+    # export writes x1 = x, v1 = v, R = 0
+    w = workloads.c1(n=16, nb=2, ppc=2)
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.3, 2e-3
+    o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
+    o.setup(w.setup)
+    g = fp32_ctx(w)
+    st = abi.imex2p_stages(0.0, w.dt)
+    o.stage(st[0], w.src)
+    g.stage(st[0], w.src)
+    g.check_device_error()
+    rg, ig, _ = exported(g)
+    n = o.npart
+    order = np.argsort(o.idata()[abi.PTAG, :n], kind="stable")
+    ro = o.rdata()[:, :n][:, order]
+    np.testing.assert_array_equal(o.idata()[abi.PTAG, :n][order], ig[abi.PTAG])
+    for c in (abi.IPX1, abi.IPY1, abi.IPZ1):
+        assert np.abs(ro[c] - rg[c]).max() <= 1e-12, c
+    for cols in ((abi.IPVX1, abi.IPVY1, abi.IPVZ1), (abi.IPRX, abi.IPRY, abi.IPRZ)):
+        scale = np.abs(ro[list(cols)]).max()
+        err = np.abs(ro[list(cols)] - rg[list(cols)]).max() / scale
+        say("stage 1 registers %s: max rel err %.3e" % (cols, err))
+        assert err < 1e-5, (cols, err)
+    for s in st[1:]:
+        g.stage(s, w.src)
+    g.check_device_error()
+    rg, _, _ = exported(g)
+    for a, b in ((abi.IPX1, abi.IPX), (abi.IPY1, abi.IPY), (abi.IPZ1, abi.IPZ),
+                 (abi.IPVX1, abi.IPVX), (abi.IPVY1, abi.IPVY), (abi.IPVZ1, abi.IPVZ)):
+        np.testing.assert_array_equal(rg[a], rg[b])
+    for c in (abi.IPRX, abi.IPRY, abi.IPRZ):
+        assert not rg[c].any()
+    g.close()
