@@ -1829,7 +1829,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   // 3. counting sort
   if ((e = SortKeys(c))) return e;
   // 4. push into the new order + deposit Q, P, qt
-  TIMED("p1_push_deposit", LaunchP1(c, A, f->qt[0], f->qt[1], f->qt[2], f->scr_f));
+  TIMED("p1_gather_deposit", LaunchP1(c, A, f->qt[0], f->qt[1], f->qt[2], f->scr_f));
   f->cur = 1 - f->cur;
   A.src = f->s[f->cur];
   A.dst = f->s[1 - f->cur];

@@ -36,11 +36,14 @@ def balg_step(form, ppc):
 # Algorithmic bytes of one launch of a kernel: (bytes per particle, bytes per active cell).
 # Each particle column or mesh value the kernel must read or write counts once.
 KERNEL_BYTES = {
-    # FP32 form (positions 8 B, other particle values 4 B; averaged over the two stages):
-    # P1 stage 1 reads x, v, ts, m and writes x, v, x1, v1; stage 2 reads x, v, x1, v1, R, ts, m
-    # and writes x, v; Q, P written per cell.  P2 reads x, ts, m; u* read, result written.
-    # P3 reads x, v, ts, m and writes v (and R at stage 1); u* read, D (FP64) written.
-    "p1_push_deposit": ((116 + 128) / 2.0, 4 * 4),
+    # FP32 form (positions 8 B, other particle values 4 B; averaged over the two stages).
+    # The key pass is the push: stage 1 reads x, v and writes x, v, x1, v1; stage 2 reads x, v,
+    # x1, v1, R and writes x, v.  P1 is the deposit: reads x, v, ts, m; Q, P and the three centre
+    # masses written per cell (moving the store into the new order is sort traffic, not counted).
+    # P2 reads x, ts, m; u* read, result written.  P3 reads x, v, ts, m and writes v (and R at
+    # stage 1); u* read, D (FP64) written.
+    "key": ((36 + 72 + 84 + 36) / 2.0, 0),
+    "p1_gather_deposit": (44, 7 * 4),
     "p2_operator": (32, 6 * 4),
     "p3_gather_kick": ((68 + 56) / 2.0, 3 * 4 + 3 * 8),
     # FP64 form
@@ -164,8 +167,8 @@ def cpu_baseline(name, budget_s):
 # known size) of the C2 FP32 run, committed under profiles/; timer label -> kernel name prefix
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "hbm_traffic_c2_fp32.md")
-TRAFFIC_KERNEL = {"p1_push_deposit": "k_p1", "p2_operator": "k_p2", "p3_gather_kick": "k_p3",
-                  "key": "k_key"}
+TRAFFIC_KERNEL = {"p1_gather_deposit": ["k_p1"], "p2_operator": ["k_p2"],
+                  "p3_gather_kick": ["k_p3", "k_p3n"], "key": ["k_key"]}
 
 
 def measured_traffic(label, w, form):
@@ -175,15 +178,18 @@ def measured_traffic(label, w, form):
         lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
         return None
-    want = TRAFFIC_KERNEL[label]
-    for ln in lines:
-        parts = ln.split()
-        if len(parts) < 3 or parts[0] != "traffic":
-            continue
-        name = " ".join(parts[1:-1])
-        if name == want or name.startswith(want + "<"):
-            return float(parts[-1])
-    return None
+    # the mean over the kernels the label times (one launch of each per step)
+    vals = []
+    for want in TRAFFIC_KERNEL[label]:
+        for ln in lines:
+            parts = ln.split()
+            if len(parts) < 3 or parts[0] != "traffic":
+                continue
+            name = " ".join(parts[1:-1])
+            if name == want or name.startswith(want + "<"):
+                vals.append(float(parts[-1]))
+                break
+    return sum(vals) / len(vals) if vals else None
 
 
 def kernel_roofline(r, w, form="fp32"):

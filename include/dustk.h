@@ -151,7 +151,9 @@ int dk_destroy(dk_ctx *ctx);
 
 /* Binds the caller's particle SoA (Particles::prtcl_rdata/idata) and gas registers
  * (Hydro::u0/u1).  In DK_FORM_FP32 the particles are copied into the library's
- * mixed-precision cell-sorted store; dk_export_particles writes them back. */
+ * mixed-precision cell-sorted store; dk_export_particles writes them back (in the store's
+ * order; at a step boundary the stage registers x1, v1, R, which no stage reads before the
+ * next stage 1 sets them, are written as x1 = x, v1 = v, R = 0). */
 int dk_bind_particles(dk_ctx *ctx, double *rdata, int32_t *idata, int64_t capacity,
                       int64_t npart);
 int dk_bind_gas(dk_ctx *ctx, double *u0, double *u1);
@@ -198,8 +200,9 @@ int dk_new_time_step(dk_ctx *ctx, double dt_cfl, double *dt_out);
 
 /* All dust tasks of one stage in task-graph order (dust_tasks.cpp:58-109), gas hook
  * included when src != NULL.  The only stage entry of DK_FORM_FP32, where it runs the
- * fused cell-sorted passes; with graphs enabled (dk_set_graphs) a stage is one
- * hipGraphLaunch. */
+ * fused cell-sorted passes; with graphs enabled (dk_set_graphs) a stage is captured on the
+ * context's stream into a HIP graph and launched (the executable graph of the same stage
+ * index is updated in place on later calls).  Not while the caller's stream is capturing. */
 int dk_stage_run(dk_ctx *ctx, const dk_stage *st, const dk_gas_src *src);
 int dk_set_graphs(dk_ctx *ctx, int32_t enable);
 
