@@ -47,7 +47,7 @@ class dk_physics(C.Structure):
                 ("back_reaction", C.c_int32), ("consistent_gas", C.c_int32),
                 ("shearing_box", C.c_int32), ("stratified", C.c_int32),
                 ("omega0", C.c_double), ("qshear", C.c_double),
-                ("pc2_damping", C.c_int32), ("stiffness", C.c_int32)]
+                ("pc2_damping", C.c_int32), ("stiffness", C.c_int32), ("clouds", C.c_int32)]
 
 
 class dk_stage(C.Structure):

@@ -926,6 +926,8 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     return DK_EINVAL;
   if (form != DK_FORM_FP64 && form != DK_FORM_FP32) return DK_EINVAL;
   if (phys->solver < DK_LOCAL || phys->solver > DK_DC2) return DK_EUNSUPPORTED;
+  if (phys->clouds && (!dk::CloudsSupported(mesh, phys) || form != DK_FORM_FP64))
+    return DK_EUNSUPPORTED;
   dk_ctx *c = new dk_ctx();
   c->M = *mesh;
   c->P = *phys;
@@ -1002,6 +1004,10 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     alloc((void**)&c->dmom, sizeof(double)*3*nc);
     alloc((void**)&c->ap, sizeof(double)*3*nc);
     alloc((void**)&c->resid, sizeof(double)*4*nc);
+    if (phys->clouds) {
+      alloc((void**)&c->gpack, sizeof(double)*12*nc);
+      alloc((void**)&c->cvel, sizeof(double)*3*nc);
+    }
   }
   if (!e) e = dk::Check(hipMemcpy(c->d_blocks, hb.data(), sizeof(DevBlock)*mesh->nmb,
                                   hipMemcpyHostToDevice));
@@ -1013,6 +1019,7 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
       const long long nv = (p == c->qdep) ? 5 : ((p == c->resid) ? 4 : 3);
       if (!e) e = dk::Check(hipMemset(p, 0, sizeof(double)*nv*nc));
     }
+    if (!e && phys->clouds) e = dk::Check(hipMemset(c->gpack, 0, sizeof(double)*12*nc));
   }
   if (!e && form == DK_FORM_FP32) e = dk::FastCreate(c);
   if (e) {dk_destroy(c); return e;}
@@ -1031,7 +1038,7 @@ int dk_destroy(dk_ctx *c) {
                   (void*)c->d_partial, (void*)c->d_scalar, (void*)c->qdep, (void*)c->ustar,
                   (void*)c->dmom, (void*)c->ap, (void*)c->resid, c->sort_tmp,
                   (void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
-                  (void*)c->sort_ibuf})
+                  (void*)c->sort_ibuf, (void*)c->gpack, (void*)c->cvel})
     if (p) (void)hipFree(p);
   delete c;
   return DK_OK;
@@ -1090,6 +1097,10 @@ int dk_explicit_push(dk_ctx *c, const dk_stage *st) {
   if (c->P.coupling == DK_PC2) {
     if (st->stage != 2) return DK_OK;
     if (c->P.back_reaction && (e = dk::Zero(c, c->dmom, 3*dk::NCells(c)))) return e;
+    if (c->P.clouds) {   // dust_push.cpp:124-129
+      if ((e = dk::CloudAdvance(c, st->dt))) return e;
+      return Launched(c);
+    }
     {const int _pt = dk::ProfBegin(c, "pc2_advance");
     k_pc2_advance<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->ustar, c->dmom, st->dt,
                              c->P.back_reaction, sbox, strat, qs, om, c->d_err);
@@ -1135,6 +1146,10 @@ int dk_deposit_drag(dk_ctx *c, const dk_stage *st) {
   if (c->P.coupling == DK_PC2) {
     if (st->stage != 1) return DK_OK;
     if ((e = dk::Zero(c, c->qdep, 5*dk::NCells(c)))) return e;
+    if (c->P.clouds) {   // dust_pm.cpp:130-137
+      if ((e = dk::CloudPredictor(c, st->dt, c->P.stiffness))) return e;
+      return Launched(c);
+    }
     {const int _pt = dk::ProfBegin(c, "pc2_predictor");
     k_pc2_predictor<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->u1, c->qdep, st->dt,
                                c->P.stiffness, c->d_err);

@@ -98,6 +98,8 @@ struct dk_ctx {
   double *u0 = nullptr, *u1 = nullptr;
   // library-owned work fields (layout of the gas, nvar components)
   double *qdep = nullptr, *ustar = nullptr, *dmom = nullptr, *ap = nullptr, *resid = nullptr;
+  // refinable clouds: the packed gather field (m, k, j, i, 12) and a velocity field (3)
+  double *gpack = nullptr, *cvel = nullptr;
   dk::ErrWord *d_err = nullptr;
   double *d_partial = nullptr;     // reduction partials
   double *d_scalar = nullptr;
@@ -135,6 +137,10 @@ void ProfDestroy(dk_ctx *c);
 
 // dk_fp32.hip
 int FastCreate(dk_ctx *c);
+// refinable clouds at the mesh level (dk_clouds.hip)
+bool CloudsSupported(const dk_mesh *mesh, const dk_physics *phys);
+int CloudPredictor(dk_ctx *c, double dt, int rate_dep);
+int CloudAdvance(dk_ctx *c, double dt);
 void FastDestroy(dk_ctx *c);
 int FastBind(dk_ctx *c);
 int FastExport(dk_ctx *c);

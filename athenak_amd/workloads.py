@@ -7,6 +7,9 @@ device memory.
       NSH equilibrium + 1e-4 noise, imex2+/dc1, dt 2e-4, 50 steps
   c3  stratified (0.2 H)^3 at 256^3, 7 log-uniform Stokes bins over [1e-3, 1],
       mass_slope 0.5, Z 0.01, 1 per bin per cell, H_p 0.02, x3 outflow, imex2+/dc1
+  c4  refinable clouds with pc2 at the root level of the SMR case: 64^3 periodic in blocks
+      of 16^3, tau_s 0.05, eps 1, ppc 4, a quarter of the particles in 16 Gaussian clumps of
+      width 2 cells (the nested refined boxes of the SMR case are not built: same-level clouds)
   c5  128^3 periodic in 8 blocks of 64^3, ppc 8, two species (tau_s 0.01 and 1, eps 0.5
       each), half the particles in 64 Gaussian clumps of width 2 cells, imex2+/dc1, 100 steps
 """
@@ -58,11 +61,11 @@ def _setup(seed=1, ppc=1, taus=(0.1,), eps=(1.0,), rho0=1.0, gas_noise=0.0, dust
 
 
 def _phys(coupling=abi.DK_IMEX, solver=abi.DK_DC1, sbox=False, strat=False, omega0=1.0,
-          qshear=1.5, deposit=abi.DK_TSC, back_reaction=1, consistent_gas=1):
+          qshear=1.5, deposit=abi.DK_TSC, back_reaction=1, consistent_gas=1, clouds=0):
     return abi.dk_physics(deposit=deposit, coupling=coupling, solver=solver,
                           back_reaction=back_reaction, consistent_gas=consistent_gas,
                           shearing_box=int(sbox), stratified=int(strat), omega0=omega0,
-                          qshear=qshear, pc2_damping=0, stiffness=0)
+                          qshear=qshear, pc2_damping=0, stiffness=0, clouds=int(clouds))
 
 
 def nsh_gas(omega0, qshear, ax, eps, taus):
@@ -132,4 +135,16 @@ def c5(n=128, nbx=2, ppc=8, steps=100, seed=1):
     return Workload("c5", mesh, blocks, _phys(), su, None, 1.0e-3, steps)
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c5": c5}
+def c4(n=64, nbx=4, ppc=4, steps=50, seed=1, clump_frac=0.25):
+    """pc2 with same-level refinable clouds (CloudPredictorT / CloudAdvanceT) on the root level
+    of the SMR case; clumps make the per-block particle counts uneven."""
+    nx = n // nbx
+    mesh, blocks = abi.uniform_mesh((nx, nx, nx), (nbx, nbx, nbx), ((0.0, 1.0),) * 3, ng=2)
+    su = _setup(seed=seed, ppc=ppc, taus=(0.05,), eps=(1.0,), gas_noise=1e-3, dust_noise=1e-3,
+                nclump=16 if clump_frac > 0 else 0, clump_frac=clump_frac,
+                clump_sigma=2.0 / n)
+    return Workload("c4", mesh, blocks, _phys(coupling=abi.DK_PC2, solver=abi.DK_LOCAL,
+                                              clouds=1), su, None, 1.0e-3, steps)
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}

@@ -115,6 +115,10 @@ typedef struct dk_physics {
   double omega0, qshear;
   int32_t pc2_damping;      /* pc2 feedback damping of stiff cells (dust_pm.cpp:389-398) */
   int32_t stiffness;        /* deposit the pc2 rate field qdep[4] (StiffnessActive) */
+  int32_t clouds;           /* pc2 with 3-D refinable clouds at the mesh level (plain codes:
+                               CloudPredictorT / CloudAdvanceT of dust_cloud3d.cpp:1369-1536);
+                               same-level meshes periodic in x1, x2, x3 only (DK_EUNSUPPORTED
+                               otherwise); 0: point particles */
 } dk_physics;
 
 /* What a task read from Driver and Mesh for one stage. */

@@ -111,6 +111,10 @@ struct or_sim {
   int err = 0;
   int64_t err_p = -1;
   double x3_removed_mass = 0.0, x3_removed_count = 0.0;
+  // refinable clouds: the root cell size (the cloud half-width at the mesh level) and the
+  // packed gather field (m, k, j, i, 12): u, and its central differences in x1, x2, x3
+  double root_dx[3] = {0.0, 0.0, 0.0}, iroot_dx[3] = {0.0, 0.0, 0.0};
+  std::vector<double> gpack;
   // fold order: the same-level neighbour slots in NeighborIndex order
   std::vector<int> slot_dir;   // dir index of each used slot, ordered
 
@@ -190,6 +194,299 @@ inline void MakeStencil(or_sim *s, int m, double x, double y, double z, Stencil 
 inline bool HaloOK(const or_sim *s, const Stencil &w) {   // src/dust/dust_pm.cpp:241-242
   return !(w.ip < (s->is-1) || w.ip > (s->is+s->nx1) || w.jp < (s->js-1) ||
            w.jp > (s->js+s->nx2) || w.kp < (s->ks-1) || w.kp > (s->ks+s->nx3));
+}
+
+//----------------------------------------------------------------------------------------
+// Refinable clouds at the mesh level (3-D, plain codes: no lobes), on a same-level mesh
+// periodic in every direction.  A cloud of half-width h = root dx is traversed in the block
+// that holds its centre, ghost cells included (it reaches at most one cell beyond that block):
+// its ghost deposits are folded like every deposit and its gathers read filled ghosts, which
+// sums what LeafTraversal3D::ForEach (src/dust/dust_leaf3d.hpp:368-517) visits on the
+// neighbour blocks themselves.
+
+// src/dust/dust.hpp:169-182 (CloudGeometry::HatCDF, HatMoment)
+inline double HatCDF(const double sv) {
+  if (sv <= -1.0) return 0.0;
+  if (sv >= 1.0) return 1.0;
+  return (sv <= 0.0) ? 0.5*SQR(1.0 + sv) : 1.0 - 0.5*SQR(1.0 - sv);
+}
+inline double HatMoment(const double sv) {
+  if (sv <= -1.0 || sv >= 1.0) return 1.0/6.0;
+  return (sv <= 0.0) ? 0.5*SQR(sv) + SQR(sv)*sv/3.0 : 0.5*SQR(sv) - SQR(sv)*sv/3.0;
+}
+// src/dust/dust.hpp:186-188
+inline double MinImage(const double d, const double length) {
+  return d - length*std::round(d/length);
+}
+
+// CloudShape1D::Shape / FaceAt / Cell of a plain cloud (src/dust/dust.hpp:240-298, lobe 0)
+struct CloudAxis { double x, ell, inv, lo, hi; };
+inline CloudAxis MakeAxis(const double x, const double ell, const double inv) {
+  CloudAxis a;
+  a.ell = ell; a.lo = x - ell; a.hi = x + ell; a.x = x; a.inv = inv;
+  return a;
+}
+struct CloudFace { double c = 0.0, h = 0.0; };
+inline CloudFace FaceAt(const CloudAxis &a, const double z, const bool moments) {
+  CloudFace f;
+  const double sz = (z - a.x)*a.inv;
+  f.c = HatCDF(sz);
+  if (moments) f.h = HatMoment(sz);
+  return f;
+}
+inline void CloudCell(const CloudAxis &a, const CloudFace &fa, const CloudFace &fb,
+                      const bool moments, double &w, double &g) {
+  w = fb.c - fa.c;
+  g = moments ? a.ell*(fb.h - fa.h) : 0.0;
+}
+
+// The block that holds x (block m or a same-level neighbour; x is moved by a mesh length
+// across a periodic face into that block's frame); -1 when x is farther away.
+int CloudAnchor(const or_sim *s, int m, double x[3]) {
+  const dk_block &b = s->B[m];
+  const double lo[3] = {b.x1min, b.x2min, b.x3min}, hi[3] = {b.x1max, b.x2max, b.x3max};
+  int o[3];
+  for (int d=0; d<3; ++d) o[d] = (x[d] < lo[d]) ? -1 : ((x[d] >= hi[d]) ? 1 : 0);
+  if (o[0] == 0 && o[1] == 0 && o[2] == 0) return m;
+  const int nb = b.nghbr[Dir(o[0], o[1], o[2])];
+  if (nb < 0) return -1;
+  const dk_block &n = s->B[nb];
+  const double nlo[3] = {n.x1min, n.x2min, n.x3min}, nhi[3] = {n.x1max, n.x2max, n.x3max};
+  const double len[3] = {s->M.x1max - s->M.x1min, s->M.x2max - s->M.x2min,
+                         s->M.x3max - s->M.x3min};
+  for (int d=0; d<3; ++d) {
+    if (o[d] == 0) continue;
+    if (x[d] < nlo[d]) x[d] += len[d];
+    else if (x[d] >= nhi[d]) x[d] -= len[d];
+  }
+  return nb;
+}
+
+// LeafTraversal3D::ForEach (src/dust/dust_leaf3d.hpp:383-517) for a plain cloud at the mesh
+// level centred in block m: function(k, j, i, w, m1x, m1y, m1z) (padded indices) with moments,
+// function(k, j, i, w) without, in the reference's loop order (x runs of 4, then x3, x2, x1).
+// Returns false when the cloud reaches beyond the first ghost layer.
+template <bool kMoments, class F>
+bool CloudForEach(const or_sim *s, const int m, const double x[3], const F &function) {
+  const CloudAxis s1 = MakeAxis(x[0], s->root_dx[0], s->iroot_dx[0]);
+  const CloudAxis s2 = MakeAxis(x[1], s->root_dx[1], s->iroot_dx[1]);
+  const CloudAxis s3 = MakeAxis(x[2], s->root_dx[2], s->iroot_dx[2]);
+  const dk_block &b = s->B[m];
+  const double dx = s->dx1[m], dy = s->dx2[m], dz = s->dx3[m];
+  const double bx0 = b.x1min, by0 = b.x2min, bz0 = b.x3min;
+  const double idx = 1.0/dx, idy = 1.0/dy, idz = 1.0/dz;
+  const int i0 = static_cast<int>(std::floor((s1.lo - bx0)*idx));
+  const int i1 = static_cast<int>(std::ceil((s1.hi - bx0)*idx)) - 1;
+  const int j0 = static_cast<int>(std::floor((s2.lo - by0)*idy));
+  const int j1 = static_cast<int>(std::ceil((s2.hi - by0)*idy)) - 1;
+  const int k0 = static_cast<int>(std::floor((s3.lo - bz0)*idz));
+  const int k1 = static_cast<int>(std::ceil((s3.hi - bz0)*idz)) - 1;
+  if (i0 < -1 || j0 < -1 || k0 < -1 || i1 > s->nx1 || j1 > s->nx2 || k1 > s->nx3) return false;
+  constexpr int kRun = 4;
+  for (int ic=i0; ic<=i1; ic+=kRun) {
+    const int nr = (i1 - ic + 1 < kRun) ? i1 - ic + 1 : kRun;
+    double w1[kRun], g1[kRun];
+    CloudFace f1 = FaceAt(s1, bx0 + ic*dx, kMoments);
+    for (int a=0; a<kRun; ++a) {
+      w1[a] = 0.0; g1[a] = 0.0;
+      if (a < nr) {
+        const CloudFace fn = FaceAt(s1, bx0 + (ic + a + 1)*dx, kMoments);
+        CloudCell(s1, f1, fn, kMoments, w1[a], g1[a]);
+        f1 = fn;
+      }
+    }
+    CloudFace f3 = FaceAt(s3, bz0 + k0*dz, kMoments);
+    for (int k=k0; k<=k1; ++k) {
+      const CloudFace fn3 = FaceAt(s3, bz0 + (k + 1)*dz, kMoments);
+      double w3, g3;
+      CloudCell(s3, f3, fn3, kMoments, w3, g3);
+      f3 = fn3;
+      if (w3 == 0.0 && (!kMoments || g3 == 0.0)) continue;
+      CloudFace f2 = FaceAt(s2, by0 + j0*dy, kMoments);
+      for (int j=j0; j<=j1; ++j) {
+        const CloudFace fn2 = FaceAt(s2, by0 + (j + 1)*dy, kMoments);
+        double w2, g2;
+        CloudCell(s2, f2, fn2, kMoments, w2, g2);
+        f2 = fn2;
+        if (w2 == 0.0 && (!kMoments || g2 == 0.0)) continue;
+        const double w23 = w2*w3;
+        for (int a=0; a<nr; ++a) {
+          const double wa = w1[a];
+          const double w = w23*wa;
+          if constexpr (kMoments) {
+            const double ga = g1[a];
+            const double m1x = w23*ga;
+            const double m1y = wa*g2*w3, m1z = wa*w2*g3;
+            if (w == 0.0 && m1x == 0.0 && m1y == 0.0 && m1z == 0.0) continue;
+            function(k + s->ks, j + s->js, ic + a + s->is, w, m1x, m1y, m1z);
+          } else {
+            if (w == 0.0) continue;
+            function(k + s->ks, j + s->js, ic + a + s->is, w);
+          }
+        }
+      }
+    }
+  }
+  return true;
+}
+
+// LeafApi<LeafTraversal3D>::GatherLinear (src/dust/dust_leaf_api.hpp:202-220): the cloud
+// integral of the cell's linear reconstruction (12 packed components, PackGatherField3D)
+inline void GatherLinear(const or_sim *s, const int m, const int k, const int j, const int i,
+                         const double w, const double m1x, const double m1y, const double m1z,
+                         const double x[3], double u[3]) {
+  const double dx1 = s->dx1[m], dx2 = s->dx2[m], dx3 = s->dx3[m];
+  const dk_block &b = s->B[m];
+  const double xc = b.x1min + (i - s->is + 0.5)*dx1;
+  const double yc = b.x2min + (j - s->js + 0.5)*dx2;
+  const double zc = b.x3min + (k - s->ks + 0.5)*dx3;
+  const double lx = s->M.x1max - s->M.x1min, ly = s->M.x2max - s->M.x2min;
+  const double lz = s->M.x3max - s->M.x3min;
+  const double cx = 0.5*(m1x + MinImage(x[0] - xc, lx)*w)/dx1;
+  const double cy = 0.5*(m1y + MinImage(x[1] - yc, ly)*w)/dx2;
+  const double cz = 0.5*(m1z + MinImage(x[2] - zc, lz)*w)/dx3;
+  const double *gp = &s->gpack[((((int64_t)m*s->n3 + k)*s->n2 + j)*s->n1 + i)*12];
+  u[0] += w*gp[0] + cx*gp[3] + cy*gp[6] + cz*gp[9];
+  u[1] += w*gp[1] + cx*gp[4] + cy*gp[7] + cz*gp[10];
+  u[2] += w*gp[2] + cx*gp[5] + cy*gp[8] + cz*gp[11];
+}
+
+// src/dust/dust_cloud3d.cpp:1319-1340 (PackGatherField3D): velocity and its central
+// differences of every cell with both neighbours in each direction; vel(m, c, k, j, i)
+void PackGatherField(or_sim *s, const std::vector<double> &vel) {
+  s->gpack.assign(s->ncb*s->M.nmb*12, 0.0);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<s->M.nmb; ++m)
+    for (int k=1; k<s->n3-1; ++k)
+      for (int j=1; j<s->n2-1; ++j)
+        for (int i=1; i<s->n1-1; ++i) {
+          double *gp = &s->gpack[((((int64_t)m*s->n3 + k)*s->n2 + j)*s->n1 + i)*12];
+          for (int c=0; c<3; ++c) {
+            gp[c] = vel[s->Idx(m,3,c,k,j,i)];
+            gp[3+c] = vel[s->Idx(m,3,c,k,j,i+1)] - vel[s->Idx(m,3,c,k,j,i-1)];
+            gp[6+c] = vel[s->Idx(m,3,c,k,j+1,i)] - vel[s->Idx(m,3,c,k,j-1,i)];
+            gp[9+c] = vel[s->Idx(m,3,c,k+1,j,i)] - vel[s->Idx(m,3,c,k-1,j,i)];
+          }
+        }
+}
+
+// src/dust/dust_cloud3d.cpp:1342-1360 (PackGasGather3D): u = M/rho of every cell, then packed
+void PackGasGather(or_sim *s, const std::vector<double> &u) {
+  std::vector<double> vel(3*s->ncb*s->M.nmb);
+  for (int m=0; m<s->M.nmb; ++m)
+    for (int64_t r=0; r<s->ncb; ++r) {
+      const double invrho = 1.0/u[((int64_t)m*NG_VARS + DK_IDN)*s->ncb + r];
+      for (int c=0; c<3; ++c)
+        vel[((int64_t)m*3 + c)*s->ncb + r] = u[((int64_t)m*NG_VARS + DK_IM1 + c)*s->ncb + r]*invrho;
+    }
+  PackGatherField(s, vel);
+}
+
+// src/dust/dust_cloud3d.cpp:1369-1437 (CloudPredictorT, 3-D, plain codes): the linear gather of
+// the step-start gas velocity over the cloud and the drag impulse deposited with the gather's
+// weights into qdep 1-3 (and 4)
+void CloudPredictor(or_sim *s, const double dt, const bool rate_dep) {
+  PackGasGather(s, s->u1);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t p=0; p<s->np; ++p) {
+    double x[3] = {s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p)};
+    const int am = CloudAnchor(s, s->I(DK_PGID,p), x);
+    double ug[3] = {0.0, 0.0, 0.0};
+    int nt = 0, tk[27], tj[27], ti[27];
+    double tw[27];
+    const bool ok = (am >= 0) && CloudForEach<true>(s, am, x,
+        [&](int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+          GatherLinear(s, am, k, j, i, w, m1x, m1y, m1z, x, ug);
+          if (nt < 27) {tk[nt] = k; tj[nt] = j; ti[nt] = i; tw[nt] = w;}
+          ++nt;
+        });
+    if (!ok || nt > 27) {s->Fail(DK_EHALO, p); continue;}
+    const double mass_rate = s->R(DK_IPM,p)/s->R(DK_IPTS,p);
+    const double gx = -dt*mass_rate*(ug[0] - s->R(DK_IPVX,p));
+    const double gy = -dt*mass_rate*(ug[1] - s->R(DK_IPVY,p));
+    const double gz = -dt*mass_rate*(ug[2] - s->R(DK_IPVZ,p));
+    const double inv_volume = 1.0/(s->dx1[am]*s->dx2[am]*s->dx3[am]);
+    for (int q=0; q<nt; ++q) {
+      if (tw[q] == 0.0) continue;
+      const double w = tw[q];
+      DepositAdd(s, &s->qdep[s->Idx(am,5,1,tk[q],tj[q],ti[q])], inv_volume*w*gx);
+      DepositAdd(s, &s->qdep[s->Idx(am,5,2,tk[q],tj[q],ti[q])], inv_volume*w*gy);
+      DepositAdd(s, &s->qdep[s->Idx(am,5,3,tk[q],tj[q],ti[q])], inv_volume*w*gz);
+      if (rate_dep) DepositAdd(s, &s->qdep[s->Idx(am,5,4,tk[q],tj[q],ti[q])], inv_volume*w*mass_rate);
+    }
+  }
+}
+
+// src/dust/dust_cloud3d.cpp:1449-1536 (CloudAdvanceT, 3-D, plain codes) with
+// CloudPC2Push<LeafTraversal3D> (:118-158): the linear gather of u* at the midpoint, the
+// implicit-midpoint kick and drift, the impulse deposited into dmom with the gather's weights
+void CloudAdvance(or_sim *s, const double dt) {
+  const bool br = s->P.back_reaction;
+  const bool sbox = s->P.shearing_box, strat = s->P.stratified;
+  const double qs = s->P.qshear, om = s->P.omega0;
+  std::vector<double> vel(s->ustar);
+  PackGatherField(s, vel);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int64_t p=0; p<s->np; ++p) {
+    const double x0[3] = {s->R(DK_IPX1,p), s->R(DK_IPY1,p), s->R(DK_IPZ1,p)};
+    const double v0[3] = {s->R(DK_IPVX1,p), s->R(DK_IPVY1,p), s->R(DK_IPVZ1,p)};
+    double xh[3], xg[3];
+    xh[0] = x0[0] + 0.5*dt*v0[0];
+    xh[1] = x0[1] + 0.5*dt*(v0[1] - (sbox ? qs*om*x0[0] : 0.0));
+    xh[2] = x0[2] + 0.5*dt*v0[2];
+    xg[0] = xh[0];
+    xg[1] = xh[1] + (sbox ? qs*om*xh[0]*0.5*dt : 0.0);
+    xg[2] = xh[2];
+    const int am = CloudAnchor(s, s->I(DK_PGID,p), xg);
+    double ug[3] = {0.0, 0.0, 0.0};
+    int nt = 0, tk[27], tj[27], ti[27];
+    double tw[27];
+    const bool ok = (am >= 0) && CloudForEach<true>(s, am, xg,
+        [&](int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+          GatherLinear(s, am, k, j, i, w, m1x, m1y, m1z, xg, ug);
+          if (nt < 27) {tk[nt] = k; tj[nt] = j; ti[nt] = i; tw[nt] = w;}
+          ++nt;
+        });
+    if (!ok || nt > 27) {s->Fail(DK_EHALO, p); continue;}
+    const double h = 0.5*dt;
+    const double lambda = h/s->R(DK_IPTS,p);
+    const double aa = 1.0 + lambda;
+    double r[3] = {v0[0] + lambda*ug[0], v0[1] + lambda*ug[1], v0[2] + lambda*ug[2]};
+    double vh[3];
+    if (sbox) {
+      const double cxy = 2.0*h*om;
+      const double cyx = h*(2.0-qs)*om;
+      const double det = aa*aa + cxy*cyx;
+      vh[0] = (aa*r[0] + cxy*r[1])/det;
+      vh[1] = (aa*r[1] - cyx*r[0])/det;
+      if (strat) r[2] -= h*SQR(om)*xh[2];
+      vh[2] = r[2]/aa;
+    } else {
+      vh[0] = r[0]/aa; vh[1] = r[1]/aa; vh[2] = r[2]/aa;
+    }
+    s->R(DK_IPVX,p) = 2.0*vh[0] - v0[0];
+    s->R(DK_IPVY,p) = 2.0*vh[1] - v0[1];
+    s->R(DK_IPVZ,p) = 2.0*vh[2] - v0[2];
+    s->R(DK_IPX,p) = x0[0] + dt*vh[0];
+    s->R(DK_IPY,p) = x0[1] + dt*(vh[1] - (sbox ? qs*om*xh[0] : 0.0));
+    s->R(DK_IPZ,p) = x0[2] + dt*vh[2];
+    s->R(DK_IPX1,p) = xh[0]; s->R(DK_IPY1,p) = xh[1]; s->R(DK_IPZ1,p) = xh[2];
+    const double ts = s->R(DK_IPTS,p);
+    const double dvx = dt*(ug[0]-vh[0])/ts;
+    const double dvy = dt*(ug[1]-vh[1])/ts;
+    const double dvz = dt*(ug[2]-vh[2])/ts;
+    s->R(DK_IPRX,p) = dvx; s->R(DK_IPRY,p) = dvy; s->R(DK_IPRZ,p) = dvz;
+    if (!br) continue;
+    const double mass = -s->R(DK_IPM,p);
+    const double measure = s->dx1[am]*s->dx2[am]*s->dx3[am];
+    for (int q=0; q<nt; ++q) {
+      if (tw[q] == 0.0) continue;
+      const double dw = mass*tw[q]/measure;
+      DepositAdd(s, &s->dmom[s->Idx(am,3,0,tk[q],tj[q],ti[q])], dw*dvx);
+      DepositAdd(s, &s->dmom[s->Idx(am,3,1,tk[q],tj[q],ti[q])], dw*dvy);
+      DepositAdd(s, &s->dmom[s->Idx(am,3,2,tk[q],tj[q],ti[q])], dw*dvz);
+    }
+  }
 }
 
 //----------------------------------------------------------------------------------------
@@ -463,6 +760,10 @@ or_sim *or_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics 
                   int64_t capacity) {
   if (mesh == nullptr || blocks == nullptr || phys == nullptr) return nullptr;
   if (mesh->nx3 < 2 || mesh->ng < 1) return nullptr;   // 3-D meshes only
+  // refinable clouds: pc2 on a same-level mesh periodic in x1, x2, x3, ghosts of 2
+  if (phys->clouds && (phys->coupling != DK_PC2 || mesh->bc_x1 != DK_X1_PERIODIC ||
+                       mesh->bc_x3 != DK_X3_PERIODIC || mesh->ng < 2))
+    return nullptr;
   or_sim *s = new or_sim();
   s->M = *mesh;
   s->P = *phys;
@@ -490,6 +791,11 @@ or_sim *or_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics 
     s->dx2[m] = (s->B[m].x2max - s->B[m].x2min)/static_cast<double>(s->nx2);
     s->dx3[m] = (s->B[m].x3max - s->B[m].x3min)/static_cast<double>(s->nx3);
   }
+  // the root cell size (src/dust/dust_cloud.cpp:135, dust_cloud3d.cpp:591-593)
+  s->root_dx[0] = (mesh->x1max - mesh->x1min)/static_cast<double>(mesh->nmbx1*mesh->nx1);
+  s->root_dx[1] = (mesh->x2max - mesh->x2min)/static_cast<double>(mesh->nmbx2*mesh->nx2);
+  s->root_dx[2] = (mesh->x3max - mesh->x3min)/static_cast<double>(mesh->nmbx3*mesh->nx3);
+  for (int d=0; d<3; ++d) s->iroot_dx[d] = 1.0/s->root_dx[d];
   // the 26 same-level slots in NeighborIndex order
   std::vector<std::pair<int,int>> order;
   for (int ox3=-1; ox3<=1; ++ox3)
@@ -737,6 +1043,10 @@ int or_explicit_push(or_sim *s, const dk_stage *st) {
     const double dt = st->dt;
     const bool br = s->P.back_reaction;
     if (br) Zero(s->dmom);
+    if (s->P.clouds) {   // src/dust/dust_push.cpp:124-129
+      CloudAdvance(s, dt);
+      return s->err;
+    }
 #pragma omp parallel for num_threads(s->nthreads) schedule(static)
     for (int64_t p=0; p<s->np; ++p) {
       const int m = s->I(DK_PGID,p);
@@ -990,6 +1300,10 @@ int or_deposit_drag(or_sim *s, const dk_stage *st) {
     if (st->stage != 1) return DK_OK;
     Zero(s->qdep);
     const double dt = st->dt;
+    if (s->P.clouds) {   // src/dust/dust_pm.cpp:130-137
+      CloudPredictor(s, dt, rate_dep);
+      return s->err;
+    }
 #pragma omp parallel for num_threads(s->nthreads) schedule(static)
     for (int64_t p=0; p<s->np; ++p) {
       const int m = s->I(DK_PGID,p);
@@ -1506,6 +1820,31 @@ int or_apply_operator(or_sim *s, double a_dt, double sh, const double *x, double
             y[id] = r[id] + s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)]*f[id];
           }
   return DK_OK;
+}
+
+int or_cloud_cells(or_sim *s, int m, const double x[3], int64_t *cell, double *w, double *m1) {
+  int n = 0;
+  const bool ok = CloudForEach<true>(s, m, x,
+      [&](int k, int j, int i, double ww, double a, double b, double c) {
+        if (n < 27) {
+          cell[n] = ((int64_t)k*s->n2 + j)*s->n1 + i;
+          w[n] = ww; m1[3*n] = a; m1[3*n+1] = b; m1[3*n+2] = c;
+        }
+        ++n;
+      });
+  return (ok && n <= 27) ? n : -1;
+}
+
+int or_cloud_gather(or_sim *s, const double *vel, int m, const double x[3], double out[3]) {
+  std::vector<double> v(vel, vel + 3*s->ncb*s->M.nmb);
+  PackGatherField(s, v);
+  double u[3] = {0.0, 0.0, 0.0};
+  const bool ok = CloudForEach<true>(s, m, x,
+      [&](int k, int j, int i, double ww, double a, double b, double c) {
+        GatherLinear(s, m, k, j, i, ww, a, b, c, x, u);
+      });
+  for (int d=0; d<3; ++d) out[d] = u[d];
+  return ok ? 0 : -1;
 }
 
 int or_diag_sums(or_sim *s, double out[9]) {

@@ -66,6 +66,14 @@ int or_new_time_step(or_sim *s, double dt_cfl, double *dt_out);
  * active cells (field layout of ustar, 3 components); uses qdep-free u0 densities. */
 int or_apply_operator(or_sim *s, double a_dt, double sh, const double *x, double *y);
 
+/* refinable-cloud pieces for closed-form tests (same-level plain clouds): the cells under the
+ * cloud at x centred in block m (padded cell index (k*n2 + j)*n1 + i, weight, first moments
+ * m1x, m1y, m1z; at most 27), returned count or -1; and the linear gather at x of the
+ * 3-component padded field vel (the layout of ustar). */
+int or_cloud_cells(or_sim *s, int m, const double x[3], int64_t *cell, double *w,
+                   double *m1 /* [27][3] */);
+int or_cloud_gather(or_sim *s, const double *vel, int m, const double x[3], double out[3]);
+
 int or_diag_sums(or_sim *s, double out[9]);
 int or_last_error(or_sim *s, int64_t *particle);
 

@@ -53,6 +53,9 @@ def lib():
                                         C.POINTER(C.c_double)]
         L.or_diag_sums.argtypes = [P, C.POINTER(C.c_double)]
         L.or_last_error.argtypes = [P, C.POINTER(C.c_int64)]
+        D = C.POINTER(C.c_double)
+        L.or_cloud_cells.argtypes = [P, C.c_int, D, C.POINTER(C.c_int64), D, D]
+        L.or_cloud_gather.argtypes = [P, D, C.c_int, D, D]
         _LIB = L
     return _LIB
 
@@ -142,6 +145,24 @@ class Oracle:
     def diag(self):
         out = (C.c_double * 9)()
         lib().or_diag_sums(self.h, out)
+        return np.array(out[:])
+
+    def cloud_cells(self, m, x):
+        """(padded cell indices, weights, first moments [n, 3]) under the cloud at x."""
+        xs = (C.c_double * 3)(*x)
+        cell = (C.c_int64 * 27)()
+        w = (C.c_double * 27)()
+        m1 = (C.c_double * 81)()
+        n = lib().or_cloud_cells(self.h, m, xs, cell, w, m1)
+        assert n >= 0, "cloud beyond the first ghost layer"
+        return (np.array(cell[:n]), np.array(w[:n]), np.array(m1[:3 * n]).reshape(n, 3))
+
+    def cloud_gather(self, vel, m, x):
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        xs = (C.c_double * 3)(*x)
+        out = (C.c_double * 3)()
+        assert lib().or_cloud_gather(self.h, vel.ctypes.data_as(C.POINTER(C.c_double)), m,
+                                     xs, out) == 0
         return np.array(out[:])
 
     def apply_operator(self, a_dt, x, sh=0.0):

@@ -60,3 +60,29 @@ def test_strerror_without_device():
         pytest.skip("libdustk.so not built")
     L = lib.lib()
     assert L.dk_strerror(4).decode().startswith("particle outside")
+
+
+def test_clouds_refused_where_not_built_without_device():
+    """dk_create checks its options before any HIP call: clouds need pc2, the FP64 form and a
+    mesh periodic in x1 and x3."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libdustk.so not built")
+    from athenak_amd import workloads
+    L = lib.lib()
+    out = ctypes.c_void_p()
+    nx = 8
+    cases = []
+    mesh, blocks = abi.uniform_mesh((nx,) * 3, (1, 1, 1), ((0.0, 1.0),) * 3, ng=2,
+                                    bc_x1=abi.DK_X1_SHEAR)
+    cases.append((mesh, blocks, abi.DK_PC2, abi.DK_FORM_FP64))
+    mesh, blocks = abi.uniform_mesh((nx,) * 3, (1, 1, 1), ((0.0, 1.0),) * 3, ng=2,
+                                    bc_x3=abi.DK_X3_OUTFLOW)
+    cases.append((mesh, blocks, abi.DK_PC2, abi.DK_FORM_FP64))
+    mesh, blocks = abi.uniform_mesh((nx,) * 3, (1, 1, 1), ((0.0, 1.0),) * 3, ng=2)
+    cases.append((mesh, blocks, abi.DK_IMEX, abi.DK_FORM_FP64))
+    cases.append((mesh, blocks, abi.DK_PC2, abi.DK_FORM_FP32))
+    for mesh, blocks, coupling, form in cases:
+        phys = workloads._phys(coupling=coupling, solver=abi.DK_LOCAL, clouds=1)
+        e = L.dk_create(ctypes.byref(mesh), blocks, ctypes.byref(phys), form, None,
+                        ctypes.byref(out))
+        assert e == 7, (coupling, form, e)   # DK_EUNSUPPORTED

@@ -205,3 +205,26 @@ def test_graph_replays_of_the_fp64_form_match_the_oracle():
         g.diag()                     # eager work and a host sync between the replays
         compare(o, g, w, what="graph step %d" % s)
     say("c1-graphs: 6 steps of graph replays match the oracle")
+
+
+def test_clouds_pc2_particles_crossing_blocks():
+    # same-level refinable clouds (CloudPredictorT / CloudAdvanceT): linear gathers and cloud
+    # weights, clouds reaching into neighbour blocks and across periodic faces
+    w = workloads.c4(n=16, nbx=2, ppc=2)
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 2.0, 0.5, 2e-3
+    w.name = "c4-clouds-cross"
+    run(w, 20, check_every=5)
+
+
+def test_clouds_pc2_with_the_stiffness_field_and_damping():
+    w = workloads.c4(n=16, nbx=2, ppc=4)
+    w.phys.stiffness, w.phys.pc2_damping = 1, 1
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.3, 5e-3
+    w.name = "c4-clouds-stiff"
+    o, g = run(w, 10, check_every=5)
+    # the predictor's rate field (qdep 4, active cells after the fold) is the oracle's
+    ng = w.mesh.ng
+    qo = o.field(0)[:, 4, ng:-ng, ng:-ng, ng:-ng]
+    qg = g.field(0)[:, 4, ng:-ng, ng:-ng, ng:-ng]
+    assert np.abs(qo).max() > 0
+    np.testing.assert_allclose(qg, qo, rtol=0, atol=1e-11 * np.abs(qo).max())
