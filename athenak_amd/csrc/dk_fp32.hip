@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "dk_internal.hpp"
 
@@ -47,7 +48,8 @@ namespace {
 
 constexpr int kSegSmall = 32;       // lists longer than this are ordered by k_bigsort
 constexpr int kBigMax = 16384;      // longest list of one cell in the FP32 form
-constexpr int kKeyThreads = 256;    // the key pass: one particle per thread per chunk
+constexpr int kKeyThreads = 256;    // the key pass: one particle per thread per chunk (default)
+constexpr int kKeyThreadsMax = 512;
 constexpr int kCellThreads = 384;   // P1-P3: one thread per (cell, stencil slab), <= 128 cells
 constexpr int kMaxTileCells = 128;
 
@@ -84,6 +86,7 @@ struct Tiles {
   int LX, LY;                       // log2 TX, log2 TY (the tile sides are powers of 2)
   int ntx, nty, ntz, ntb, nt;
   float iRX, iRY, iRN;              // 1/RX, 1/RY, 1/RN for FDiv
+  int xcd;                          // workgroups dealt to the 8 XCDs in runs of tiles (1) or not
 };
 
 }  // namespace
@@ -112,6 +115,14 @@ struct Fast {
   int pending = 0;
   double a_prev = 0.0;
   int regs_dead = 0;                // the store holds no registers (after a step's last stage)
+  // launch parameters that leave every result unchanged (the autotuner's choices)
+  int key_threads = kKeyThreads;
+  // online autotuning: state 0 fixed, 1 tuning, 2 tuned; candidates run one after another
+  // for kTuneStages active stages each, after as many warm-up stages
+  int tune_state = 0, tune_cand = -1, tune_stage = 0, ncand = 0;
+  int cand_xcd[8] = {0}, cand_kt[8] = {0};
+  float cand_ms[8] = {0.0f};
+  hipEvent_t tev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -353,7 +364,7 @@ struct TileInfo {
 // neighbouring tiles (whose regions overlap) share its L2
 __device__ inline int TileIndex(const Tiles &T) {
   const int b = blockIdx.x;
-  if (T.nt % 8 != 0) return b;
+  if (!T.xcd || T.nt % 8 != 0) return b;
   return (b % 8)*(T.nt/8) + b/8;
 }
 
@@ -641,7 +652,7 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
 // particle that stays in its tile takes its rank from a count in LDS (the tile owns those
 // cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
 // only the pending correction is applied (written to v and R).
-__global__ void __launch_bounds__(kKeyThreads)
+__global__ void __launch_bounds__(kKeyThreadsMax)
 k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1457,8 +1468,8 @@ int FillV(dk_ctx *c, T *a, long long n, T v) {
 }
 
 // dynamic LDS of the tile kernels (NT = 3 TC threads, CH = kPPT NT records)
-size_t LdsKey(const Tiles &T) {
-  return (size_t)kKeyThreads*(3*sizeof(double) + 3*sizeof(float) + sizeof(int)) +
+size_t LdsKey(const Tiles &T, int threads) {
+  return (size_t)threads*(3*sizeof(double) + 3*sizeof(float) + sizeof(int)) +
          sizeof(int)*(2*T.TC + 1);
 }
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1); }
@@ -1589,7 +1600,8 @@ int Flush(dk_ctx *c) {
   Fast *f = c->fast;
   if (!f->pending) return DK_OK;
   KArgs A = MakeArgs(c);
-  TIMED("key", (k_key<<<f->T.nt, kKeyThreads, LdsKey(f->T), c->stream>>>(A, 0, nullptr)));
+  TIMED("key", (k_key<<<f->T.nt, f->key_threads, LdsKey(f->T, f->key_threads),
+                        c->stream>>>(A, 0, nullptr)));
   dk_gas_src z;
   std::memset(&z, 0, sizeof(z));
   const long long nact = (long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3;
@@ -1629,6 +1641,7 @@ int FastCreate(dk_ctx *c) {
   }
   T.TC = T.TX*T.TY*T.TZ;
   T.SY = (c->P.shearing_box) ? 1 : 0;
+  T.xcd = 1;
   T.RX = T.TX + 2; T.RY = T.TY + 2 + 2*T.SY; T.RZ = T.TZ + 2;
   T.RN = T.RX*T.RY*T.RZ;
   T.LX = 0; while ((1 << T.LX) < T.TX) ++T.LX;
@@ -1689,6 +1702,8 @@ int FastCreate(dk_ctx *c) {
 void FastDestroy(dk_ctx *c) {
   Fast *f = c->fast;
   if (f == nullptr) return;
+  for (auto &ev : f->tev)
+    if (ev) {(void)hipEventDestroy(ev); ev = nullptr;}
   for (int b=0; b<2; ++b) {
     Store &S = f->s[b];
     for (void *p : {(void*)S.d, (void*)S.f, (void*)S.i})
@@ -1810,7 +1825,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   // 1. key pass (with the pending correction), histogram
   if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
   // the push of this stage writes its residue into the buffer the next stage's start reads
-  TIMED("key", (k_key<<<T.nt, kKeyThreads, LdsKey(T), c->stream>>>(A, 1,
+  TIMED("key", (k_key<<<T.nt, f->key_threads, LdsKey(T, f->key_threads), c->stream>>>(A, 1,
                                                                   f->pres[1 - f->pres_rd])));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history
   {
@@ -1864,3 +1879,197 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
 }
 
 }  // namespace dk
+
+//----------------------------------------------------------------------------------------
+// The autotuner.  Its choices are launch parameters that leave every result bit-identical: the
+// dealing of tiles to the XCDs (runs of tiles on one XCD, so that neighbouring tiles share its
+// L2, or the plain order) and the key pass's workgroup size (its per-cell sums run over each
+// cell's particles in store order whatever the chunk, and its ranks are re-sorted by source
+// slot).  The tile shape is not among them: it groups the cross-tile sums.  Tuning runs online,
+// on the stages the caller runs anyway: after kTuneStages warm-up stages each candidate runs
+// kTuneStages active stages, timed with events on the context's stream, and the fastest is
+// kept; with DK_TUNE_CACHE naming a JSON file the choice is stored there under a key of the
+// mesh, tile shape and particle count, and read back instead of tuning again.
+
+namespace {
+
+constexpr int kTuneStages = 2;
+
+std::string TuneKey(const dk_ctx *c) {
+  const Fast *f = c->fast;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "fp32 nmb=%d n=%dx%dx%d tile=%dx%dx%d sy=%d np=%lld",
+                c->g.nmb, c->g.nx1, c->g.nx2, c->g.nx3, f->T.TX, f->T.TY, f->T.TZ, f->T.SY,
+                (long long)f->np);
+  return buf;
+}
+
+bool ValidKeyThreads(const Fast *f, int kt) {
+  return (kt == 128 || kt == 256 || kt == 512) && kt >= f->T.TC;
+}
+
+// the entry of this configuration in the cache file (one entry per line)
+bool TuneCacheRead(const dk_ctx *c, int &xcd, int &kt) {
+  const char *path = std::getenv("DK_TUNE_CACHE");
+  if (path == nullptr || *path == 0) return false;
+  FILE *fp = std::fopen(path, "r");
+  if (fp == nullptr) return false;
+  const std::string want = "\"" + TuneKey(c) + "\"";
+  char line[1024];
+  bool found = false;
+  while (std::fgets(line, sizeof(line), fp)) {
+    const char *k = std::strstr(line, want.c_str());
+    if (k == nullptr) continue;
+    const char *a = std::strstr(k, "\"xcd\":"), *b = std::strstr(k, "\"key_threads\":");
+    if (a && b && std::sscanf(a, "\"xcd\": %d", &xcd) == 1 &&
+        std::sscanf(b, "\"key_threads\": %d", &kt) == 1 && (xcd == 0 || xcd == 1) &&
+        ValidKeyThreads(c->fast, kt))
+      found = true;
+    break;
+  }
+  std::fclose(fp);
+  return found;
+}
+
+void TuneCacheWrite(const dk_ctx *c) {
+  const char *path = std::getenv("DK_TUNE_CACHE");
+  if (path == nullptr || *path == 0) return;
+  const Fast *f = c->fast;
+  const std::string key = "\"" + TuneKey(c) + "\"";
+  std::vector<std::string> keep;
+  if (FILE *fp = std::fopen(path, "r")) {
+    char line[1024];
+    while (std::fgets(line, sizeof(line), fp)) {
+      std::string l(line);
+      while (!l.empty() && (l.back() == '\n' || l.back() == ',' || l.back() == ' ')) l.pop_back();
+      if (l.size() > 2 && l[0] == ' ' && l.find("\": {") != std::string::npos &&
+          l.find(key) == std::string::npos)
+        keep.push_back(l);
+    }
+    std::fclose(fp);
+  }
+  std::string e = "  " + key + ": {\"xcd\": " + std::to_string(f->T.xcd) +
+                  ", \"key_threads\": " + std::to_string(f->key_threads) + ", \"ms\": [";
+  for (int i=0; i<f->ncand; ++i) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "%s[%d, %d, %.4f]", i ? ", " : "", f->cand_xcd[i],
+                  f->cand_kt[i], f->cand_ms[i]);
+    e += b;
+  }
+  e += "]}";
+  keep.push_back(e);
+  if (FILE *fp = std::fopen(path, "w")) {
+    std::fputs("{\n", fp);
+    for (size_t i=0; i<keep.size(); ++i)
+      std::fprintf(fp, "%s%s\n", keep[i].c_str(), (i + 1 < keep.size()) ? "," : "");
+    std::fputs("}\n", fp);
+    std::fclose(fp);
+  }
+}
+
+void ApplyCand(Fast *f, int i) {
+  f->T.xcd = f->cand_xcd[i];
+  f->key_threads = f->cand_kt[i];
+}
+
+}  // namespace
+
+namespace dk {
+
+int FastTuneBegin(dk_ctx *c, const dk_stage *st) {
+  Fast *f = c->fast;
+  if (f == nullptr || f->tune_state != 1 || !ActiveStage(c, st)) return DK_OK;
+  if (f->tune_cand >= 0) return Check(hipEventRecord(f->tev[0], c->stream));
+  return DK_OK;
+}
+
+int FastTuneEnd(dk_ctx *c, const dk_stage *st) {
+  Fast *f = c->fast;
+  if (f == nullptr || f->tune_state != 1 || !ActiveStage(c, st)) return DK_OK;
+  int e = DK_OK;
+  if (f->tune_cand >= 0) {
+    float ms = 0.0f;
+    e = Check(hipEventRecord(f->tev[1], c->stream));
+    if (!e) e = Check(hipEventSynchronize(f->tev[1]));
+    if (!e) e = Check(hipEventElapsedTime(&ms, f->tev[0], f->tev[1]));
+    if (e) return e;
+    f->cand_ms[f->tune_cand] += ms;
+  }
+  if (++f->tune_stage < kTuneStages) return DK_OK;
+  f->tune_stage = 0;
+  ++f->tune_cand;
+  if (f->tune_cand < f->ncand) {
+    ApplyCand(f, f->tune_cand);
+    return DK_OK;
+  }
+  int best = 0;
+  for (int i=1; i<f->ncand; ++i)
+    if (f->cand_ms[i] < f->cand_ms[best]) best = i;
+  ApplyCand(f, best);
+  f->tune_state = 2;
+  TuneCacheWrite(c);
+  return DK_OK;
+}
+
+}  // namespace dk
+
+extern "C" {
+
+int dk_autotune(dk_ctx *c, int32_t mode) {
+  if (c == nullptr || mode < 0 || mode > 1) return DK_EINVAL;
+  Fast *f = c->fast;
+  if (f == nullptr) return DK_OK;   // the plain FP64 form has nothing to tune
+  if (mode == 0) {
+    if (f->tune_state == 1) {   // stopped early: back to the defaults
+      f->T.xcd = 1;
+      f->key_threads = kKeyThreads;
+    }
+    f->tune_state = 0;
+    return DK_OK;
+  }
+  int xcd = 1, kt = kKeyThreads;
+  if (TuneCacheRead(c, xcd, kt)) {
+    f->T.xcd = xcd;
+    f->key_threads = kt;
+    f->tune_state = 2;
+    return DK_OK;
+  }
+  f->ncand = 0;
+  for (int x : {1, 0})
+    for (int k : {256, 128, 512})
+      if (ValidKeyThreads(f, k) && f->ncand < 8) {
+        f->cand_xcd[f->ncand] = x;
+        f->cand_kt[f->ncand] = k;
+        f->cand_ms[f->ncand] = 0.0f;
+        ++f->ncand;
+      }
+  for (auto &ev : f->tev)
+    if (ev == nullptr && hipEventCreate(&ev) != hipSuccess) return DK_EHIP;
+  f->tune_state = 1;
+  f->tune_cand = -1;
+  f->tune_stage = 0;
+  return DK_OK;
+}
+
+int dk_tune_config(dk_ctx *c, int32_t *state, int32_t *xcd, int32_t *key_threads) {
+  if (c == nullptr || state == nullptr || xcd == nullptr || key_threads == nullptr)
+    return DK_EINVAL;
+  const Fast *f = c->fast;
+  *state = f ? f->tune_state : 0;
+  *xcd = f ? f->T.xcd : 0;
+  *key_threads = f ? f->key_threads : 0;
+  return DK_OK;
+}
+
+int dk_set_tune_config(dk_ctx *c, int32_t xcd, int32_t key_threads) {
+  if (c == nullptr) return DK_EINVAL;
+  Fast *f = c->fast;
+  if (f == nullptr) return DK_ESTATE;
+  if ((xcd != 0 && xcd != 1) || !ValidKeyThreads(f, key_threads)) return DK_EINVAL;
+  f->T.xcd = xcd;
+  f->key_threads = key_threads;
+  f->tune_state = 0;
+  return DK_OK;
+}
+
+}  // extern "C"

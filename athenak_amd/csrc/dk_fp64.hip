@@ -1347,8 +1347,10 @@ int StageGraph(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
 
 int dk_stage_run(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   if (c == nullptr || st == nullptr) return DK_EINVAL;
-  if (c->graphs) return StageGraph(c, st, src);
-  return StageBody(c, st, src);
+  int e = c->fast ? dk::FastTuneBegin(c, st) : DK_OK;
+  if (!e) e = c->graphs ? StageGraph(c, st, src) : StageBody(c, st, src);
+  if (!e && c->fast) e = dk::FastTuneEnd(c, st);
+  return e;
 }
 
 int dk_set_graphs(dk_ctx *c, int32_t enable) {

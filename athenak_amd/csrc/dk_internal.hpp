@@ -137,6 +137,9 @@ void ProfDestroy(dk_ctx *c);
 
 // dk_fp32.hip
 int FastCreate(dk_ctx *c);
+// the FP32 form's online autotuner around an active stage (no-ops when not tuning)
+int FastTuneBegin(dk_ctx *c, const dk_stage *st);
+int FastTuneEnd(dk_ctx *c, const dk_stage *st);
 // refinable clouds at the mesh level (dk_clouds.hip)
 bool CloudsSupported(const dk_mesh *mesh, const dk_physics *phys);
 int CloudPredictor(dk_ctx *c, double dt, int rate_dep);

@@ -21,7 +21,8 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_export_particles", "dk_npart", "dk_stage_run", "dk_set_graphs",
             "dk_gas_hook", "dk_sort_particles", "dk_new_time_step", "dk_diag_sums",
             "dk_error", "dk_field", "dk_strerror", "dk_setup_problem", "dk_prof_enable",
-            "dk_prof_reset", "dk_prof_read"] +
+            "dk_prof_reset", "dk_prof_read", "dk_autotune", "dk_tune_config",
+            "dk_set_tune_config"] +
            ["dk_" + t for t in TASKS])
 
 
@@ -51,6 +52,9 @@ def lib():
         L.dk_stage_run.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_gas_hook.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_set_graphs.argtypes = [P, C.c_int32]
+        L.dk_autotune.argtypes = [P, C.c_int32]
+        L.dk_tune_config.argtypes = [P] + [C.POINTER(C.c_int32)] * 3
+        L.dk_set_tune_config.argtypes = [P, C.c_int32, C.c_int32]
         L.dk_sort_particles.argtypes = [P]
         L.dk_new_time_step.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
         L.dk_diag_sums.argtypes = [P, C.POINTER(C.c_double)]
@@ -174,6 +178,21 @@ class DustContext:
 
     def set_graphs(self, on):
         self._check(lib().dk_set_graphs(self.h, 1 if on else 0), "dk_set_graphs")
+
+    def autotune(self, on=True):
+        """Tune the FP32 form's launch parameters online over the next stages (or take them
+        from the DK_TUNE_CACHE file); results are unchanged by any choice."""
+        self._check(lib().dk_autotune(self.h, 1 if on else 0), "dk_autotune")
+
+    def tune_config(self):
+        """(state 0 fixed / 1 tuning / 2 tuned, xcd dealing, key-pass workgroup size)."""
+        v = [C.c_int32(0) for _ in range(3)]
+        self._check(lib().dk_tune_config(self.h, *[C.byref(x) for x in v]), "dk_tune_config")
+        return tuple(x.value for x in v)
+
+    def set_tune_config(self, xcd, key_threads):
+        self._check(lib().dk_set_tune_config(self.h, int(xcd), int(key_threads)),
+                    "dk_set_tune_config")
 
     def sort(self):
         self._check(lib().dk_sort_particles(self.h), "dk_sort_particles")

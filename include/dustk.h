@@ -210,6 +210,20 @@ int dk_new_time_step(dk_ctx *ctx, double dt_cfl, double *dt_out);
 int dk_stage_run(dk_ctx *ctx, const dk_stage *st, const dk_gas_src *src);
 int dk_set_graphs(dk_ctx *ctx, int32_t enable);
 
+/* The autotuner (no counterpart in the reference; the "work-tile and block sizes picked by a
+ * project-written autotuner" of the FP32 form).  mode 1: take this configuration's entry from
+ * the JSON file named by DK_TUNE_CACHE, else tune online over the next dk_stage_run calls
+ * (2 warm-up stages, then 2 active stages per candidate, timed on the context's stream; the
+ * fastest is kept and stored in the cache file).  The candidates are launch parameters that
+ * leave every result bit-identical (the dealing of tiles to XCDs, the key pass's workgroup
+ * size), so tuning on the caller's own stages changes no result.  mode 0: stop tuning.  The
+ * plain FP64 form has nothing to tune (DK_OK). */
+int dk_autotune(dk_ctx *ctx, int32_t mode);
+/* state: 0 fixed, 1 tuning, 2 tuned; the current choices */
+int dk_tune_config(dk_ctx *ctx, int32_t *state, int32_t *xcd, int32_t *key_threads);
+/* sets the choices (xcd 0 or 1; key_threads 128, 256 or 512, at least the tile's cells) */
+int dk_set_tune_config(dk_ctx *ctx, int32_t xcd, int32_t key_threads);
+
 /* ---- gas stand-ins of the harness (not part of the reference's dust path) --------------- */
 
 /* Hydro::RKUpdate with zero fluxes + Hydro::HydroSrcTerms (see dk_gas_src). */

@@ -243,3 +243,63 @@ def test_fp32_registers_mid_step_and_at_step_end():
     for c in (abi.IPRX, abi.IPRY, abi.IPRZ):
         assert not rg[c].any()
     g.close()
+
+
+def test_fp32_autotuner_choices_are_bit_identical(tmp_path):
+    # every choice the autotuner makes (tile dealing to XCDs, key-pass workgroup size) leaves
+    # the results bit-identical; online tuning over the caller's stages and the cache file
+    # therefore change nothing either
+    import os
+    w = workloads.c5(n=32, nbx=2, ppc=8)
+    w.setup.dust_noise = 0.5
+    outs = []
+    for xcd, kt in ((1, 256), (0, 128), (1, 512)):
+        g = fp32_ctx(w)
+        g.set_tune_config(xcd, kt)
+        for _ in range(5):
+            g.step(w.dt, w.src)
+        g.check_device_error()
+        outs.append(exported(g))
+        g.close()
+        say("tune config xcd=%d key_threads=%d done" % (xcd, kt))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert a.tobytes() == b.tobytes()
+    cache = tmp_path / "tune.json"
+    os.environ["DK_TUNE_CACHE"] = str(cache)
+    try:
+        g = fp32_ctx(w)
+        g.autotune(True)
+        assert g.tune_config()[0] == 1
+        steps = 0
+        while g.tune_config()[0] == 1 and steps < 20:
+            g.step(w.dt, w.src)
+            steps += 1
+        state, xcd, kt = g.tune_config()
+        assert state == 2, state
+        for _ in range(5 - steps if steps < 5 else 0):
+            g.step(w.dt, w.src)
+        say("online tuning: %d steps, chose xcd=%d key_threads=%d" % (steps, xcd, kt))
+        assert cache.exists() and '"key_threads"' in cache.read_text()
+        g.close()
+        g2 = fp32_ctx(w)
+        g2.autotune(True)
+        assert g2.tune_config() == (2, xcd, kt)
+        g2.close()
+    finally:
+        del os.environ["DK_TUNE_CACHE"]
+    # the tuned run (its first steps ran under every candidate) against a plain run
+    g = fp32_ctx(w)
+    g.autotune(True)
+    for _ in range(8):
+        g.step(w.dt, w.src)
+    g.check_device_error()
+    tuned = exported(g)
+    g.close()
+    g = fp32_ctx(w)
+    for _ in range(8):
+        g.step(w.dt, w.src)
+    plain = exported(g)
+    g.close()
+    for a, b in zip(tuned, plain):
+        assert a.tobytes() == b.tobytes()
