@@ -1753,7 +1753,8 @@ int FastBind(dk_ctx *c) {
   f->cur = 0;
   f->pending = 0;
   f->regs_dead = 0;
-  if (f->np == 0) return DK_OK;
+  // (with no particles the passes below run over empty sets: they still write the cell
+  // offsets, start[] = 0, that every later pass reads)
   k_bind<<<Grid(f->np), kBlock, 0, c->stream>>>(c->rdata, c->idata, (long long)c->cap, f->np,
                                                  f->s[0], f->itag);
   int e;
