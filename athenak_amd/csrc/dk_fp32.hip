@@ -197,9 +197,12 @@ __device__ inline int BlockOff(double x, double lo, double hi, double l) {
   return static_cast<int>((x - lo + l)/l) - 1;
 }
 
-// VEL: also the new velocity (the key pass needs the position only)
+// VEL: also the new velocity (the key pass needs the position only).  b0: the particle's block
+// (s.gid) before the push, passed in so that a caller whose particles share one block reads it
+// once for all of them.
 template <bool VEL = true>
-__device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
+__device__ inline bool PushMigrate(const KArgs &A, const DevBlock &b0, PState &s,
+                                   bool write_regs) {
 #pragma clang fp contract(off)
   const Geo &g = A.g;
   const double xo = s.x[0], yo = s.x[1], zo = s.x[2];
@@ -230,7 +233,7 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
   // ShiftShearCrossings
   const DevBlock *B = A.B;
   if (A.shear && A.dsh != 0.0) {
-    const DevBlock &bo = B[s.gid];
+    const DevBlock &bo = b0;
     const int ix = BlockOff(s.x[0], bo.x1min, bo.x1max, bo.x1max - bo.x1min);
     const bool lo1 = (ix < 0) && (bo.face & 1);
     const bool hi1 = (ix > 0) && (bo.face & 2);
@@ -252,7 +255,7 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
     } else {
       // outflow: park the particle at rest in the boundary cell of its block with zero mass
       // (bvals_part.cpp:184-196); its momentum leaves with it
-      const DevBlock &bo = A.B[s.gid];
+      const DevBlock &bo = b0;
       const double z = lo ? (bo.x3min + 0.5*bo.dx3) : (bo.x3max - 0.5*bo.dx3);
       s.x[2] = z;
       for (int d=0; d<3; ++d) {s.v[d] = 0.0f; s.vres[d] = 0.0; s.v1[d] = 0.0f;}
@@ -260,7 +263,7 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
       s.parked = 1;
     }
   }
-  const DevBlock &b = B[s.gid];
+  const DevBlock &b = b0;
   const int ix = BlockOff(s.x[0], b.x1min, b.x1max, b.x1max - b.x1min);
   const int iy = BlockOff(s.x[1], b.x2min, b.x2max, b.x2max - b.x2min);
   const int iz = BlockOff(s.x[2], b.x3min, b.x3max, b.x3max - b.x3min);
@@ -307,15 +310,22 @@ __device__ inline bool PushMigrate(const KArgs &A, PState &s, bool write_regs) {
 
 // the deposit cell of a position (y shifted by sh x, the gas's t^n frame) clamped into the block
 // (cj is returned unclamped too: a centre one cell beyond the x2 face is handled by the caller)
-__device__ inline void DepositCell(const Geo &g, const DevBlock &b, const double x[3],
-                                   double sh, int &ci, int &cj, int &ck, int &cju) {
+// (the block's lower corner lo and cells per length sc)
+__device__ inline void DepositCellOf(const Geo &g, const double lo[3], const double sc[3],
+                                     const double x[3], double sh, int &ci, int &cj, int &ck,
+                                     int &cju) {
 #pragma clang fp contract(off)
-  ci = CellOf(x[0], b.x1min, b.sx1);
-  cju = CellOf(x[1] + sh*x[0], b.x2min, b.sx2);
-  ck = CellOf(x[2], b.x3min, b.sx3);
+  ci = CellOf(x[0], lo[0], sc[0]);
+  cju = CellOf(x[1] + sh*x[0], lo[1], sc[1]);
+  ck = CellOf(x[2], lo[2], sc[2]);
   ci = min(max(ci, 0), g.nx1 - 1);
   ck = min(max(ck, 0), g.nx3 - 1);
   cj = min(max(cju, 0), g.nx2 - 1);
+}
+__device__ inline void DepositCell(const Geo &g, const DevBlock &b, const double x[3],
+                                   double sh, int &ci, int &cj, int &ck, int &cju) {
+  const double lo[3] = {b.x1min, b.x2min, b.x3min}, sc[3] = {b.sx1, b.sx2, b.sx3};
+  DepositCellOf(g, lo, sc, x, sh, ci, cj, ck, cju);
 }
 
 // the centre cell of the unshifted position (dust_pm.cpp:649-659); false when it rounds onto
@@ -338,7 +348,7 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
     s.r[d] = rates ? S.R(d, p) : 0.0;
     s.vres[d] = 0.0;
   }
-  s.gid = S.GID(p);
+  s.gid = -1;   // set by the caller (the tile's block)
   s.parked = 0;
 }
 
@@ -679,6 +689,7 @@ k_key(KArgs A, int do_key, double *pres) {
     if (q < c1) {
       PState s;
       LoadP(A.src, q, s, !A.stage1, A.atw_dt != 0.0 || !do_key);
+      s.gid = t.m;   // (the tile's particles are in its block: the store is tile-major)
       const float mass = A.src.M(q);
       int code = 0;
       double dv[3] = {0.0, 0.0, 0.0};
@@ -715,7 +726,7 @@ k_key(KArgs A, int do_key, double *pres) {
       }
       float pr[3] = {0.0f, 0.0f, 0.0f};
       if (do_key) {
-        if (!PushMigrate(A, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+        if (!PushMigrate(A, b, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
         for (int d=0; d<3; ++d) {
           A.src.X(d, q) = s.x[d];
           A.src.V(d, q) = s.v[d];
@@ -725,7 +736,14 @@ k_key(KArgs A, int do_key, double *pres) {
         if (s.parked) A.src.M(q) = 0.0f;
         // a particle that failed to route keeps its block: its key stays valid
         int ni, nj, nk, nju;
-        DepositCell(g, A.B[s.gid], s.x, A.sh, ni, nj, nk, nju);
+        // the tile's block (read once by the workgroup), or the block a particle migrated to
+        double lo[3] = {b.x1min, b.x2min, b.x3min}, sc[3] = {b.sx1, b.sx2, b.sx3};
+        if (s.gid != t.m) {
+          const DevBlock &bn = A.B[s.gid];
+          lo[0] = bn.x1min; lo[1] = bn.x2min; lo[2] = bn.x3min;
+          sc[0] = bn.sx1; sc[1] = bn.sx2; sc[2] = bn.sx3;
+        }
+        DepositCellOf(g, lo, sc, s.x, A.sh, ni, nj, nk, nju);
         const long long key = TileKey(Tl, s.gid, ni, nj, nk);
         A.key[q] = static_cast<int>(key);
         const long long lk = key - t.key0;
