@@ -50,6 +50,7 @@ constexpr int kSegSmall = 32;       // lists longer than this are ordered by k_b
 constexpr int kBigMax = 16384;      // longest list of one cell in the FP32 form
 constexpr int kKeyThreads = 256;    // the key pass: one particle per thread per chunk (default)
 constexpr int kKeyThreadsMax = 512;
+constexpr int kOverGrid = 4096;     // workgroups of the launch over the extra parts of full tiles
 constexpr int kCellThreads = 384;   // P1-P3: one thread per (cell, stencil slab), <= 128 cells
 constexpr int kMaxTileCells = 128;
 
@@ -99,6 +100,13 @@ struct Fast {
   int *key = nullptr, *rank = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr;
   int *stay = nullptr;              // per cell: the particles that stayed in its tile
   int *big = nullptr;               // [0] = count of long lists, then their cells
+  // work items (KArgs): parts per tile, their scan, the tile of every item; overflow slots
+  int *nparts = nullptr, *item_start = nullptr, *item_tile = nullptr;
+  long long item_cap = 0, ov_cap = 0;
+  void *iscan_tmp = nullptr;
+  size_t iscan_bytes = 0;
+  float *ov_scr = nullptr, *ov_q = nullptr, *ov_pr = nullptr;
+  double *ov_def = nullptr, *ov_dc = nullptr;
   void *scan_tmp = nullptr;
   size_t scan_bytes = 0;
   int32_t *itag = nullptr;          // caller's PTAG, PSP, PLEV, PTAGH by original index
@@ -169,6 +177,16 @@ struct KArgs {
   double a_prev;
   const float *res;
   double *dc0, *dc1, *dc2;
+  // work items: a tile's particles in parts of at most W (item i: tile item_tile[i], part
+  // i - item_start[tile]); parts after the first leave their partial sums in overflow slots
+  // (slot of part p >= 1 of tile t: item_start[t] - t + p - 1)
+  const int *item_start, *item_tile;
+  int W;
+  float *ov_scr;                    // [slot][4][RN] regions
+  double *ov_def;                   // [slot][3] impulse defects
+  float *ov_q;                      // [slot][3][TC] centre masses (own, to above, to below)
+  double *ov_dc;                    // [slot][9][TC] dust corrections by centre row
+  float *ov_pr;                     // [slot][3][TC] push residues
 };
 
 // the cell (active index, unclamped) of position x along one axis, as PMWeights
@@ -598,6 +616,29 @@ __device__ inline void FlushRegion(const Tiles &Tl, const TileInfo &t, const T *
   }
 }
 
+// the overflow slot of part >= 1 of a tile
+__device__ inline long long OvSlot(const int *item_start, int tile, int part) {
+  return (long long)item_start[tile] - tile + part - 1;
+}
+
+// the region of a part after the first into its overflow slot, words in region order (sum: also
+// the FP64 sum of the words this thread writes)
+template <int NV>
+__device__ inline void FlushOverflow(const Tiles &Tl, const float *reg, float *ov,
+                                     double *sum = nullptr) {
+  __syncthreads();
+#pragma unroll 1
+  for (int v=0; v<NV; ++v) {
+    double s = 0.0;
+    for (int rc = threadIdx.x; rc < Tl.RN; rc += blockDim.x) {
+      const float val = reg[v*Tl.RN + rc];
+      ov[v*Tl.RN + rc] = val;
+      s += static_cast<double>(val);
+    }
+    if (sum != nullptr) sum[v] = s;
+  }
+}
+
 // active cell indices of tile-local cell l
 __device__ inline void CellIdx(const Tiles &Tl, const TileInfo &t, int l, int &ci, int &cj,
                                int &ck) {
@@ -662,128 +703,168 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
 // particle that stays in its tile takes its rank from a count in LDS (the tile owns those
 // cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
 // only the pending correction is applied (written to v and R).
+template <bool OV>
 __global__ void __launch_bounds__(kKeyThreadsMax)
 k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  const int tile = TileIndex(Tl);
-  const TileInfo t = TileOf(Tl, tile);
-  const int tid = threadIdx.x, CH = blockDim.x;
-  double *rdc = lds_d;                                   // [3][CH] m delta/V
-  float *rpr = reinterpret_cast<float*>(rdc + 3*CH);     // [3][CH] m vres/V
-  int *rcode = reinterpret_cast<int*>(rpr + 3*CH);       // [CH] centre row + 2 (0: none)
-  int *st = rcode + CH;                                  // [TC + 1]
-  int *lcnt = st + Tl.TC + 1;                            // [TC]
-  for (int i = tid; i <= Tl.TC; i += CH) st[i] = A.start[t.key0 + i];
-  for (int i = tid; i < Tl.TC; i += CH) lcnt[i] = 0;
-  __syncthreads();
-  const int P0 = st[0], P1 = st[Tl.TC];
-  const DevBlock &b = A.B[t.m];
-  const double ivol = b.ivol;
-  double dcs[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-  float prs[3] = {0.0f, 0.0f, 0.0f};
-  for (int c0 = P0; c0 < P1; c0 += CH) {
-    const int c1 = min(c0 + CH, P1);
-    const int q = c0 + tid;
-    if (q < c1) {
-      PState s;
-      LoadP(A.src, q, s, !A.stage1, A.atw_dt != 0.0 || !do_key);
-      s.gid = t.m;   // (the tile's particles are in its block: the store is tile-major)
-      const float mass = A.src.M(q);
-      int code = 0;
-      double dv[3] = {0.0, 0.0, 0.0};
-      if (A.pending) {
-        int ic_, jc, kc_;
-        if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
-          int ci, cj, ck;
-          CellIdx(Tl, t, SlotCell(st, Tl.TC, q), ci, cj, ck);
-          const int dj = jc - cj;   // the centre: this cell or one beside it in x2
-          if (ic_ != ci || kc_ != ck || dj < -1 || dj > 1) {
-            dk::RecordError(A.err, DK_EHALO, q);
-          } else {
-            const float apf = static_cast<float>(A.a_prev);
-            const float cjf = apf/(A.src.TS(q) + apf);
-            const long long ic = Idx(g, t.m, 3, 0, g.ks + ck, g.js + jc, g.is + ci);
-            for (int d=0; d<3; ++d) {
-              const float vo = s.v[d];
-              const float vn = vo + cjf*A.res[ic + d*g.ncb];
-              s.v[d] = vn;
-              // the recorded rate takes the actual change, so that the history terms of dust
-              // and gas stay opposite (dust_pm.cpp:694)
-              const double delta = static_cast<double>(vn) - static_cast<double>(vo);
-              s.r[d] += delta*A.inv_a_prev;
-              dv[d] = static_cast<double>(mass)*delta*ivol;
-            }
-            code = dj + 2;
-            if (!do_key)
+  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
+  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
+  const int nitem = A.item_start[Tl.nt];
+  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
+       it += gridDim.x) {
+    const int tile = A.item_tile[it];
+    const int part = it - A.item_start[tile];
+    const TileInfo t = TileOf(Tl, tile);
+    const int tid = threadIdx.x, CH = blockDim.x;
+    double *rdc = lds_d;                                   // [3][CH] m delta/V
+    float *rpr = reinterpret_cast<float*>(rdc + 3*CH);     // [3][CH] m vres/V
+    int *rcode = reinterpret_cast<int*>(rpr + 3*CH);       // [CH] centre row + 2 (0: none)
+    int *st = rcode + CH;                                  // [TC + 1]
+    int *lcnt = st + Tl.TC + 1;                            // [TC]
+    for (int i = tid; i <= Tl.TC; i += CH) st[i] = A.start[t.key0 + i];
+    for (int i = tid; i < Tl.TC; i += CH) lcnt[i] = 0;
+    __syncthreads();
+    // this part's particles: at most W from the tile's range
+    const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
+    const DevBlock &b = A.B[t.m];
+    const double ivol = b.ivol;
+    double dcs[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    float prs[3] = {0.0f, 0.0f, 0.0f};
+    for (int c0 = P0; c0 < P1; c0 += CH) {
+      const int c1 = min(c0 + CH, P1);
+      const int q = c0 + tid;
+      if (q < c1) {
+        PState s;
+        LoadP(A.src, q, s, !A.stage1, A.atw_dt != 0.0 || !do_key);
+        s.gid = t.m;   // (the tile's particles are in its block: the store is tile-major)
+        const float mass = A.src.M(q);
+        int code = 0;
+        double dv[3] = {0.0, 0.0, 0.0};
+        if (A.pending) {
+          int ic_, jc, kc_;
+          if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
+            int ci, cj, ck;
+            CellIdx(Tl, t, SlotCell(st, Tl.TC, q), ci, cj, ck);
+            const int dj = jc - cj;   // the centre: this cell or one beside it in x2
+            if (ic_ != ci || kc_ != ck || dj < -1 || dj > 1) {
+              dk::RecordError(A.err, DK_EHALO, q);
+            } else {
+              const float apf = static_cast<float>(A.a_prev);
+              const float cjf = apf/(A.src.TS(q) + apf);
+              const long long ic = Idx(g, t.m, 3, 0, g.ks + ck, g.js + jc, g.is + ci);
               for (int d=0; d<3; ++d) {
-                A.src.V(d, q) = s.v[d];
-                if (A.regs_live) A.src.R(d, q) = s.r[d];
+                const float vo = s.v[d];
+                const float vn = vo + cjf*A.res[ic + d*g.ncb];
+                s.v[d] = vn;
+                // the recorded rate takes the actual change, so that the history terms of dust
+                // and gas stay opposite (dust_pm.cpp:694)
+                const double delta = static_cast<double>(vn) - static_cast<double>(vo);
+                s.r[d] += delta*A.inv_a_prev;
+                dv[d] = static_cast<double>(mass)*delta*ivol;
               }
+              code = dj + 2;
+              if (!do_key)
+                for (int d=0; d<3; ++d) {
+                  A.src.V(d, q) = s.v[d];
+                  if (A.regs_live) A.src.R(d, q) = s.r[d];
+                }
+            }
           }
         }
+        float pr[3] = {0.0f, 0.0f, 0.0f};
+        if (do_key) {
+          if (!PushMigrate(A, b, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+          for (int d=0; d<3; ++d) {
+            A.src.X(d, q) = s.x[d];
+            A.src.V(d, q) = s.v[d];
+            if (A.regs_live) {A.src.X1(d, q) = s.x1[d]; A.src.V1(d, q) = s.v1[d];}
+            pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
+          }
+          if (s.parked) A.src.M(q) = 0.0f;
+          // a particle that failed to route keeps its block: its key stays valid
+          int ni, nj, nk, nju;
+          // the tile's block (read once by the workgroup), or the block a particle migrated to
+          double lo[3] = {b.x1min, b.x2min, b.x3min}, sc[3] = {b.sx1, b.sx2, b.sx3};
+          if (s.gid != t.m) {
+            const DevBlock &bn = A.B[s.gid];
+            lo[0] = bn.x1min; lo[1] = bn.x2min; lo[2] = bn.x3min;
+            sc[0] = bn.sx1; sc[1] = bn.sx2; sc[2] = bn.sx3;
+          }
+          DepositCellOf(g, lo, sc, s.x, A.sh, ni, nj, nk, nju);
+          const long long key = TileKey(Tl, s.gid, ni, nj, nk);
+          A.key[q] = static_cast<int>(key);
+          const long long lk = key - t.key0;
+          // (a part after the first counts all its particles as arrivals: the LDS counts
+          // number the first part's stayers)
+          A.rank[q] = (part == 0 && lk >= 0 && lk < Tl.TC) ? atomicAdd(&lcnt[lk], 1)
+                                                           : -1 - atomicAdd(&A.count[key], 1);
+        }
+        for (int d=0; d<3; ++d) {rdc[d*CH + tid] = dv[d]; rpr[d*CH + tid] = pr[d];}
+        rcode[tid] = code;
       }
-      float pr[3] = {0.0f, 0.0f, 0.0f};
-      if (do_key) {
-        if (!PushMigrate(A, b, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+      __syncthreads();
+      if (tid < Tl.TC) {
+        const int s0 = max(st[tid], c0), s1 = min(st[tid + 1], c1);
+        for (int qq = s0; qq < s1; ++qq) {
+          const int r = qq - c0;
+          const int code = rcode[r];
+          const double d0 = rdc[r], d1 = rdc[CH + r], d2 = rdc[2*CH + r];
+          if (code == 1) {dcs[0][0] += d0; dcs[0][1] += d1; dcs[0][2] += d2;}
+          else if (code == 2) {dcs[1][0] += d0; dcs[1][1] += d1; dcs[1][2] += d2;}
+          else if (code == 3) {dcs[2][0] += d0; dcs[2][1] += d1; dcs[2][2] += d2;}
+          prs[0] += rpr[r]; prs[1] += rpr[CH + r]; prs[2] += rpr[2*CH + r];
+        }
+      }
+      __syncthreads();
+    }
+    if (do_key && part == 0)
+      for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
+    if (tid < Tl.TC && part == 0) {
+      int ci, cj, ck;
+      CellIdx(Tl, t, tid, ci, cj, ck);
+      const long long i0 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+      if (do_key)
+        for (int d=0; d<3; ++d) pres[i0 + d*g.ncb] = static_cast<double>(prs[d]);
+      if (A.pending) {
+        // own cell, and the cells beside it in x2 (every target written by exactly one thread)
         for (int d=0; d<3; ++d) {
-          A.src.X(d, q) = s.x[d];
-          A.src.V(d, q) = s.v[d];
-          if (A.regs_live) {A.src.X1(d, q) = s.x1[d]; A.src.V1(d, q) = s.v1[d];}
-          pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
+          A.dc0[i0 + d*g.ncb] = dcs[1][d];
+          A.dc1[i0 + g.n1 + d*g.ncb] = dcs[2][d];   // to the cell above: "from below" there
+          A.dc2[i0 - g.n1 + d*g.ncb] = dcs[0][d];   // to the cell below: "from above" there
         }
-        if (s.parked) A.src.M(q) = 0.0f;
-        // a particle that failed to route keeps its block: its key stays valid
-        int ni, nj, nk, nju;
-        // the tile's block (read once by the workgroup), or the block a particle migrated to
-        double lo[3] = {b.x1min, b.x2min, b.x3min}, sc[3] = {b.sx1, b.sx2, b.sx3};
-        if (s.gid != t.m) {
-          const DevBlock &bn = A.B[s.gid];
-          lo[0] = bn.x1min; lo[1] = bn.x2min; lo[2] = bn.x3min;
-          sc[0] = bn.sx1; sc[1] = bn.sx2; sc[2] = bn.sx3;
-        }
-        DepositCellOf(g, lo, sc, s.x, A.sh, ni, nj, nk, nju);
-        const long long key = TileKey(Tl, s.gid, ni, nj, nk);
-        A.key[q] = static_cast<int>(key);
-        const long long lk = key - t.key0;
-        A.rank[q] = (lk >= 0 && lk < Tl.TC) ? atomicAdd(&lcnt[lk], 1)
-                                            : -1 - atomicAdd(&A.count[key], 1);
       }
-      for (int d=0; d<3; ++d) {rdc[d*CH + tid] = dv[d]; rpr[d*CH + tid] = pr[d];}
-      rcode[tid] = code;
+    } else if (tid < Tl.TC) {
+      // a later part: its partial sums to its overflow slot (added by k_fix_key)
+      const long long o = OvSlot(A.item_start, tile, part);
+      if (do_key)
+        for (int d=0; d<3; ++d) A.ov_pr[(o*3 + d)*Tl.TC + tid] = prs[d];
+      if (A.pending)
+        for (int k=0; k<3; ++k)
+          for (int d=0; d<3; ++d) A.ov_dc[(o*9 + 3*k + d)*Tl.TC + tid] = dcs[k][d];
     }
-    __syncthreads();
-    if (tid < Tl.TC) {
-      const int s0 = max(st[tid], c0), s1 = min(st[tid + 1], c1);
-      for (int qq = s0; qq < s1; ++qq) {
-        const int r = qq - c0;
-        const int code = rcode[r];
-        const double d0 = rdc[r], d1 = rdc[CH + r], d2 = rdc[2*CH + r];
-        if (code == 1) {dcs[0][0] += d0; dcs[0][1] += d1; dcs[0][2] += d2;}
-        else if (code == 2) {dcs[1][0] += d0; dcs[1][1] += d1; dcs[1][2] += d2;}
-        else if (code == 3) {dcs[2][0] += d0; dcs[2][1] += d1; dcs[2][2] += d2;}
-        prs[0] += rpr[r]; prs[1] += rpr[CH + r]; prs[2] += rpr[2*CH + r];
-      }
-    }
-    __syncthreads();
+    if constexpr (!OV) break;
+    __syncthreads();   // (the next item reuses LDS)
   }
-  if (do_key)
-    for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
-  if (tid < Tl.TC) {
-    int ci, cj, ck;
-    CellIdx(Tl, t, tid, ci, cj, ck);
-    const long long i0 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
-    if (do_key)
-      for (int d=0; d<3; ++d) pres[i0 + d*g.ncb] = static_cast<double>(prs[d]);
-    if (A.pending) {
-      // own cell, and the cells beside it in x2 (every target written by exactly one thread)
-      for (int d=0; d<3; ++d) {
-        A.dc0[i0 + d*g.ncb] = dcs[1][d];
-        A.dc1[i0 + g.n1 + d*g.ncb] = dcs[2][d];   // to the cell above: "from below" there
-        A.dc2[i0 - g.n1 + d*g.ncb] = dcs[0][d];   // to the cell below: "from above" there
-      }
-    }
+}
+
+// the parts of every tile: at most W particles each, at least one (the entry after the last tile
+// is 0, for the scan)
+__global__ void k_nparts(Tiles Tl, const int *start, int W, int *nparts) {
+  GRID_STRIDE(t, (long long)Tl.nt + 1) {
+    if (t == Tl.nt) {nparts[t] = 0; continue;}
+    const long long k0 = t*(long long)Tl.TC;
+    const int n = start[k0 + Tl.TC] - start[k0];
+    nparts[t] = max(1, (n + W - 1)/W);
+  }
+}
+
+// the tile of every work item
+__global__ void k_item_tile(Tiles Tl, const int *item_start, int *item_tile) {
+  GRID_STRIDE(t, (long long)Tl.nt) {
+    const int i0 = item_start[t], i1 = item_start[t + 1];
+    for (int i = i0; i < i1; ++i) item_tile[i] = static_cast<int>(t);
   }
 }
 
@@ -858,203 +939,229 @@ __global__ void __launch_bounds__(1024) k_bigsort(const int *start, int *perm, c
 // P1: the pushed particles (the key pass pushed them in place) gathered into the new order, and
 // the deposit of Q = sum m c W/V and P = sum m c W v/V (dust_pm.cpp:225-274) and of the centre
 // mass m c/V (dust_pm.cpp:660-667)
-template <int S, bool SY>
+template <int S, bool SY, bool OV>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  const int tile = TileIndex(Tl);
-  const TileInfo t = TileOf(Tl, tile);
-  const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
-  float *rw = reinterpret_cast<float*>(lds_d);           // [9][CH] weights x0-2, y0-2, z0-2
-  float *rv = rw + 9*CH;                                 // [4][CH] mu, mu v
-  int *rcode = reinterpret_cast<int*>(rv + 4*CH);        // [CH] centre | (sj + 1) << 2
-  float *reg = reinterpret_cast<float*>(rcode + CH);     // [4][RN]
-  int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
-  for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
-  for (int i = tid; i < 4*Tl.RN; i += NT) reg[i] = 0.0f;
-  __syncthreads();
-  const int P0 = st[0], P1 = st[Tl.TC];
-  const DevBlock &b = A.B[t.m];
-  const double ivol = b.ivol;
-  const int slab = tid/Tl.TC;
-  const int l = tid - slab*Tl.TC;
-  float q3[3] = {0.0f, 0.0f, 0.0f};
-  for (int c0 = P0; c0 < P1; c0 += CH) {
-    const int c1 = min(c0 + CH, P1);
-#pragma unroll 1
-    for (int k=0; k<kPPT; ++k) {
-      const int r = tid + k*NT;
-      const int q = c0 + r;
-      if (q >= c1) break;
-      const int p = A.perm[q];
-      const double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
-      const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
-      const float ts = A.src.TS(p), mass = A.src.M(p);
-      const int kp = A.key[p];
-      for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d];}
-      if (A.regs_live)
-        for (int d=0; d<3; ++d) {A.dst.X1(d, q) = A.src.X1(d, p); A.dst.V1(d, q) = A.src.V1(d, p);}
-      A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = t.m; A.dst.ORIG(q) = A.src.ORIG(p);
-      float wx[3] = {0.0f, 0.0f, 0.0f}, wy[3] = {0.0f, 0.0f, 0.0f}, wz[3] = {0.0f, 0.0f, 0.0f};
-      float mu = 0.0f;
-      int code = 1 << 2;   // no centre, sj = 0
-      const Loc L = Locate<SY>(g, b, x, A.sh);
-      // the key pass placed the particle by this position: the cell must be its slot's
-      if (TileKey(Tl, t.m, L.ci, L.cj, L.ck) != kp) {
-        dk::RecordError(A.err, DK_EHALO, p);
-      } else {
-        const float af = static_cast<float>(A.a_dt);
-        mu = mass*(af/(ts + af))*static_cast<float>(ivol);
-        int cen = 0;
-        if (mass > 0.0f && L.cen) {
-          // the centre of the unshifted position: this cell or one beside it in x2
-          if (L.jc - L.cj < -1 || L.jc - L.cj > 1) dk::RecordError(A.err, DK_EHALO, p);
-          else cen = L.jc - L.cj + 2;
-        }
-        code = cen | ((L.sj + 1) << 2);
-        W3<S, float>(static_cast<float>(L.dx), wx);
-        W3<S, float>(static_cast<float>(L.dy), wy);
-        W3<S, float>(static_cast<float>(L.dz), wz);
-      }
-      rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
-      rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
-      rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
-      rv[r] = mu;
-      rv[CH + r] = mu*v[0]; rv[2*CH + r] = mu*v[1]; rv[3*CH + r] = mu*v[2];
-      rcode[r] = code;
-    }
+  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
+  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
+  const int nitem = A.item_start[Tl.nt];
+  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
+       it += gridDim.x) {
+    const int tile = A.item_tile[it];
+    const int part = it - A.item_start[tile];
+    const TileInfo t = TileOf(Tl, tile);
+    const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+    float *rw = reinterpret_cast<float*>(lds_d);           // [9][CH] weights x0-2, y0-2, z0-2
+    float *rv = rw + 9*CH;                                 // [4][CH] mu, mu v
+    int *rcode = reinterpret_cast<int*>(rv + 4*CH);        // [CH] centre | (sj + 1) << 2
+    float *reg = reinterpret_cast<float*>(rcode + CH);     // [4][RN]
+    int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
+    for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+    for (int i = tid; i < 4*Tl.RN; i += NT) reg[i] = 0.0f;
     __syncthreads();
-    float acc[3][3][4], spl[3][4];
-#pragma unroll
-    for (int a=0; a<3; ++a)
-#pragma unroll
-      for (int n=0; n<4; ++n) {
-        spl[a][n] = 0.0f;
-#pragma unroll
-        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+    // this part's particles: at most W from the tile's range
+    const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
+    const DevBlock &b = A.B[t.m];
+    const double ivol = b.ivol;
+    const int slab = tid/Tl.TC;
+    const int l = tid - slab*Tl.TC;
+    float q3[3] = {0.0f, 0.0f, 0.0f};
+    for (int c0 = P0; c0 < P1; c0 += CH) {
+      const int c1 = min(c0 + CH, P1);
+  #pragma unroll 1
+      for (int k=0; k<kPPT; ++k) {
+        const int r = tid + k*NT;
+        const int q = c0 + r;
+        if (q >= c1) break;
+        const int p = A.perm[q];
+        const double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
+        const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
+        const float ts = A.src.TS(p), mass = A.src.M(p);
+        const int kp = A.key[p];
+        for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d];}
+        if (A.regs_live)
+          for (int d=0; d<3; ++d) {A.dst.X1(d, q) = A.src.X1(d, p); A.dst.V1(d, q) = A.src.V1(d, p);}
+        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = t.m; A.dst.ORIG(q) = A.src.ORIG(p);
+        float wx[3] = {0.0f, 0.0f, 0.0f}, wy[3] = {0.0f, 0.0f, 0.0f}, wz[3] = {0.0f, 0.0f, 0.0f};
+        float mu = 0.0f;
+        int code = 1 << 2;   // no centre, sj = 0
+        const Loc L = Locate<SY>(g, b, x, A.sh);
+        // the key pass placed the particle by this position: the cell must be its slot's
+        if (TileKey(Tl, t.m, L.ci, L.cj, L.ck) != kp) {
+          dk::RecordError(A.err, DK_EHALO, p);
+        } else {
+          const float af = static_cast<float>(A.a_dt);
+          mu = mass*(af/(ts + af))*static_cast<float>(ivol);
+          int cen = 0;
+          if (mass > 0.0f && L.cen) {
+            // the centre of the unshifted position: this cell or one beside it in x2
+            if (L.jc - L.cj < -1 || L.jc - L.cj > 1) dk::RecordError(A.err, DK_EHALO, p);
+            else cen = L.jc - L.cj + 2;
+          }
+          code = cen | ((L.sj + 1) << 2);
+          W3<S, float>(static_cast<float>(L.dx), wx);
+          W3<S, float>(static_cast<float>(L.dy), wy);
+          W3<S, float>(static_cast<float>(L.dz), wz);
+        }
+        rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
+        rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
+        rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
+        rv[r] = mu;
+        rv[CH + r] = mu*v[0]; rv[2*CH + r] = mu*v[1]; rv[3*CH + r] = mu*v[2];
+        rcode[r] = code;
       }
-    int spdir = 0;
-    const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-    for (int qq = s0; qq < s1; ++qq) {
-      const int r = qq - c0;
-      const float wz = rw[(6 + slab)*CH + r];
-      const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
-      const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
-      const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
-      const int code = rcode[r];
-      AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
-      if (slab == 1) {
-        const int cen = code & 3;
-        q3[0] += (cen == 1) ? val[0] : 0.0f;
-        q3[1] += (cen == 2) ? val[0] : 0.0f;
-        q3[2] += (cen == 3) ? val[0] : 0.0f;
+      __syncthreads();
+      float acc[3][3][4], spl[3][4];
+  #pragma unroll
+      for (int a=0; a<3; ++a)
+  #pragma unroll
+        for (int n=0; n<4; ++n) {
+          spl[a][n] = 0.0f;
+  #pragma unroll
+          for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+        }
+      int spdir = 0;
+      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+      for (int qq = s0; qq < s1; ++qq) {
+        const int r = qq - c0;
+        const float wz = rw[(6 + slab)*CH + r];
+        const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
+        const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+        const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
+        const int code = rcode[r];
+        AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
+        if (slab == 1) {
+          const int cen = code & 3;
+          q3[0] += (cen == 1) ? val[0] : 0.0f;
+          q3[1] += (cen == 2) ? val[0] : 0.0f;
+          q3[2] += (cen == 3) ? val[0] : 0.0f;
+        }
       }
+      DepositSlabs<float, 4, SY>(Tl, reg, rw, l, slab, acc, spl, spdir);
     }
-    DepositSlabs<float, 4, SY>(Tl, reg, rw, l, slab, acc, spl, spdir);
+    if (slab == 1 && part == 0) {
+      int ci, cj, ck;
+      CellIdx(Tl, t, l, ci, cj, ck);
+      const long long i0 = Idx(g, t.m, 1, 0, g.ks + ck, g.js + cj, g.is + ci);
+      qt0[i0] = q3[1];
+      qt1[i0 + g.n1] = q3[2];
+      qt2[i0 - g.n1] = q3[0];
+    } else if (slab == 1) {
+      // a later part: to its overflow slot (added by k_fix_qt)
+      float *o = A.ov_q + OvSlot(A.item_start, tile, part)*3*Tl.TC;
+      o[l] = q3[1]; o[Tl.TC + l] = q3[2]; o[2*Tl.TC + l] = q3[0];
+    }
+    if (part == 0) FlushRegion<float, 4>(Tl, t, reg, scr);
+    else FlushOverflow<4>(Tl, reg, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN);
+    if constexpr (!OV) break;
+    __syncthreads();   // (the next item reuses LDS)
   }
-  if (slab == 1) {
-    int ci, cj, ck;
-    CellIdx(Tl, t, l, ci, cj, ck);
-    const long long i0 = Idx(g, t.m, 1, 0, g.ks + ck, g.js + cj, g.is + ci);
-    qt0[i0] = q3[1];
-    qt1[i0 + g.n1] = q3[2];
-    qt2[i0 - g.n1] = q3[0];
-  }
-  FlushRegion<float, 4>(Tl, t, reg, scr);
 }
 
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
-template <int S, bool SY>
+template <int S, bool SY, bool OV>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_p2(KArgs A, const float *us, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  const int tile = TileIndex(Tl);
-  const TileInfo t = TileOf(Tl, tile);
-  const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
-  float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
-  float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
-  float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
-  float *rg = rw + 9*CH;                                 // [3][CH] (m c/V) G u*
-  int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1
-  int *st = rcode + CH;                                  // [TC + 1]
-  RegionFetch<float> uf;
-  uf.Load(g, Tl, t, us, 3);
-  for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
-  for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
-  __syncthreads();
-  uf.Store(Tl, 3, us_);
-  __syncthreads();
-  const int P0 = st[0], P1 = st[Tl.TC];
-  const DevBlock &b = A.B[t.m];
-  const double ivol = b.ivol;
-  const int slab = tid/Tl.TC;
-  const int l = tid - slab*Tl.TC;
-  for (int c0 = P0; c0 < P1; c0 += CH) {
-    const int c1 = min(c0 + CH, P1);
-#pragma unroll 1
-    for (int k=0; k<kPPT; ++k) {
-      const int r = tid + k*NT;
-      const int q = c0 + r;
-      if (q >= c1) break;
-      const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
-      const float tsq = A.src.TS(q), mq = A.src.M(q);
-      const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
-      const int sj = L.sj;
-      float wx[3], wy[3], wz[3];
-      W3<S, float>(static_cast<float>(L.dx), wx);
-      W3<S, float>(static_cast<float>(L.dy), wy);
-      W3<S, float>(static_cast<float>(L.dz), wz);
-      const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
-      float gth[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int c=0; c<3; ++c)
-#pragma unroll
-        for (int bb=0; bb<3; ++bb) {
-          const float wcb = wz[c]*wy[bb];
-          const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
-#pragma unroll
-          for (int a=0; a<3; ++a) {
-            const float w = wcb*wx[a];
-            gth[0] += w*us_[rr + a];
-            gth[1] += w*us_[Tl.RN + rr + a];
-            gth[2] += w*us_[2*Tl.RN + rr + a];
-          }
-        }
-      const float af = static_cast<float>(A.a_dt);
-      const float fac = mq*(af/(tsq + af))*static_cast<float>(ivol);
-      rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
-      rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
-      rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
-      rg[r] = fac*gth[0]; rg[CH + r] = fac*gth[1]; rg[2*CH + r] = fac*gth[2];
-      rcode[r] = sj + 1;
-    }
+  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
+  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
+  const int nitem = A.item_start[Tl.nt];
+  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
+       it += gridDim.x) {
+    const int tile = A.item_tile[it];
+    const int part = it - A.item_start[tile];
+    const TileInfo t = TileOf(Tl, tile);
+    const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+    float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
+    float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
+    float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
+    float *rg = rw + 9*CH;                                 // [3][CH] (m c/V) G u*
+    int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1
+    int *st = rcode + CH;                                  // [TC + 1]
+    RegionFetch<float> uf;
+    uf.Load(g, Tl, t, us, 3);
+    for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+    for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
     __syncthreads();
-    float acc[3][3][3], spl[3][3];
-#pragma unroll
-    for (int a=0; a<3; ++a)
-#pragma unroll
-      for (int n=0; n<3; ++n) {
-        spl[a][n] = 0.0f;
-#pragma unroll
-        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+    uf.Store(Tl, 3, us_);
+    __syncthreads();
+    // this part's particles: at most W from the tile's range
+    const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
+    const DevBlock &b = A.B[t.m];
+    const double ivol = b.ivol;
+    const int slab = tid/Tl.TC;
+    const int l = tid - slab*Tl.TC;
+    for (int c0 = P0; c0 < P1; c0 += CH) {
+      const int c1 = min(c0 + CH, P1);
+  #pragma unroll 1
+      for (int k=0; k<kPPT; ++k) {
+        const int r = tid + k*NT;
+        const int q = c0 + r;
+        if (q >= c1) break;
+        const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
+        const float tsq = A.src.TS(q), mq = A.src.M(q);
+        const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
+        const int sj = L.sj;
+        float wx[3], wy[3], wz[3];
+        W3<S, float>(static_cast<float>(L.dx), wx);
+        W3<S, float>(static_cast<float>(L.dy), wy);
+        W3<S, float>(static_cast<float>(L.dz), wz);
+        const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
+        float gth[3] = {0.0f, 0.0f, 0.0f};
+  #pragma unroll
+        for (int c=0; c<3; ++c)
+  #pragma unroll
+          for (int bb=0; bb<3; ++bb) {
+            const float wcb = wz[c]*wy[bb];
+            const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+  #pragma unroll
+            for (int a=0; a<3; ++a) {
+              const float w = wcb*wx[a];
+              gth[0] += w*us_[rr + a];
+              gth[1] += w*us_[Tl.RN + rr + a];
+              gth[2] += w*us_[2*Tl.RN + rr + a];
+            }
+          }
+        const float af = static_cast<float>(A.a_dt);
+        const float fac = mq*(af/(tsq + af))*static_cast<float>(ivol);
+        rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
+        rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
+        rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
+        rg[r] = fac*gth[0]; rg[CH + r] = fac*gth[1]; rg[2*CH + r] = fac*gth[2];
+        rcode[r] = sj + 1;
       }
-    int spdir = 0;
-    const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-    for (int qq = s0; qq < s1; ++qq) {
-      const int r = qq - c0;
-      const float wz = rw[(6 + slab)*CH + r];
-      const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
-      const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
-      const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
-      AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+      __syncthreads();
+      float acc[3][3][3], spl[3][3];
+  #pragma unroll
+      for (int a=0; a<3; ++a)
+  #pragma unroll
+        for (int n=0; n<3; ++n) {
+          spl[a][n] = 0.0f;
+  #pragma unroll
+          for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+        }
+      int spdir = 0;
+      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+      for (int qq = s0; qq < s1; ++qq) {
+        const int r = qq - c0;
+        const float wz = rw[(6 + slab)*CH + r];
+        const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
+        const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+        const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
+        AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+      }
+      DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
     }
-    DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
+    if (part == 0) FlushRegion<float, 3>(Tl, t, dep, scr);
+    else FlushOverflow<3>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN);
+    if constexpr (!OV) break;
+    __syncthreads();   // (the next item reuses LDS)
   }
-  FlushRegion<float, 3>(Tl, t, dep, scr);
 }
 
 // P3: gather u*, kick, drag rate, and the deposit of the actual impulse (dust_pm.cpp:500-564).
@@ -1063,142 +1170,212 @@ k_p2(KArgs A, const float *us, float *scr) {
 // cell: the FP64 field then holds exactly (to FP64 round-off) the particles' total impulse.
 // (REGS: the drag rate R is written, for a later stage; a template argument, as a run-time test
 // around the store costs the kernel registers)
-template <int S, bool SY, bool REGS>
+template <int S, bool SY, bool REGS, bool OV>
 __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, double *defect) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  const int tile = TileIndex(Tl);
-  const TileInfo t = TileOf(Tl, tile);
-  const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
-  float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
-  float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
-  float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
-  float *rv = rw + 9*CH;                                 // [3][CH] -m delta/V
-  int *rcode = reinterpret_cast<int*>(rv + 3*CH);        // [CH] sj + 1
-  int *st = rcode + CH;                                  // [TC + 1]
-  RegionFetch<float> uf;
-  uf.Load(g, Tl, t, us, 3);
-  for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
-  for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
-  __syncthreads();
-  uf.Store(Tl, 3, us_);
-  __syncthreads();
-  const int P0 = st[0], P1 = st[Tl.TC];
-  const DevBlock &b = A.B[t.m];
-  const double ivol = b.ivol;
-  const int slab = tid/Tl.TC;
-  const int l = tid - slab*Tl.TC;
-  // this thread's particles' impulses in FP64, in its fixed particle order
-  double ex[3] = {0.0, 0.0, 0.0};
-  for (int c0 = P0; c0 < P1; c0 += CH) {
-    const int c1 = min(c0 + CH, P1);
-#pragma unroll 1
-    for (int k=0; k<kPPT; ++k) {
-      const int r = tid + k*NT;
-      const int q = c0 + r;
-      if (q >= c1) break;
-      const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
-      const float vk0[3] = {A.src.V(0, q), A.src.V(1, q), A.src.V(2, q)};
-      const float tsq = A.src.TS(q), mq = A.src.M(q);
-      const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
-      const int sj = L.sj;
-      float wx[3], wy[3], wz[3];
-      W3<S, float>(static_cast<float>(L.dx), wx);
-      W3<S, float>(static_cast<float>(L.dy), wy);
-      W3<S, float>(static_cast<float>(L.dz), wz);
-      const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
-      float u[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int c=0; c<3; ++c)
-#pragma unroll
-        for (int bb=0; bb<3; ++bb) {
-          const float wcb = wz[c]*wy[bb];
-          const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
-#pragma unroll
-          for (int a=0; a<3; ++a) {
-            const float w = wcb*wx[a];
-            u[0] += w*us_[rr + a];
-            u[1] += w*us_[Tl.RN + rr + a];
-            u[2] += w*us_[2*Tl.RN + rr + a];
+  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
+  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
+  const int nitem = A.item_start[Tl.nt];
+  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
+       it += gridDim.x) {
+    const int tile = A.item_tile[it];
+    const int part = it - A.item_start[tile];
+    const TileInfo t = TileOf(Tl, tile);
+    const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+    float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
+    float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
+    float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
+    float *rv = rw + 9*CH;                                 // [3][CH] -m delta/V
+    int *rcode = reinterpret_cast<int*>(rv + 3*CH);        // [CH] sj + 1
+    int *st = rcode + CH;                                  // [TC + 1]
+    RegionFetch<float> uf;
+    uf.Load(g, Tl, t, us, 3);
+    for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+    for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
+    __syncthreads();
+    uf.Store(Tl, 3, us_);
+    __syncthreads();
+    // this part's particles: at most W from the tile's range
+    const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
+    const DevBlock &b = A.B[t.m];
+    const double ivol = b.ivol;
+    const int slab = tid/Tl.TC;
+    const int l = tid - slab*Tl.TC;
+    // this thread's particles' impulses in FP64, in its fixed particle order
+    double ex[3] = {0.0, 0.0, 0.0};
+    for (int c0 = P0; c0 < P1; c0 += CH) {
+      const int c1 = min(c0 + CH, P1);
+  #pragma unroll 1
+      for (int k=0; k<kPPT; ++k) {
+        const int r = tid + k*NT;
+        const int q = c0 + r;
+        if (q >= c1) break;
+        const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
+        const float vk0[3] = {A.src.V(0, q), A.src.V(1, q), A.src.V(2, q)};
+        const float tsq = A.src.TS(q), mq = A.src.M(q);
+        const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
+        const int sj = L.sj;
+        float wx[3], wy[3], wz[3];
+        W3<S, float>(static_cast<float>(L.dx), wx);
+        W3<S, float>(static_cast<float>(L.dy), wy);
+        W3<S, float>(static_cast<float>(L.dz), wz);
+        const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
+        float u[3] = {0.0f, 0.0f, 0.0f};
+  #pragma unroll
+        for (int c=0; c<3; ++c)
+  #pragma unroll
+          for (int bb=0; bb<3; ++bb) {
+            const float wcb = wz[c]*wy[bb];
+            const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+  #pragma unroll
+            for (int a=0; a<3; ++a) {
+              const float w = wcb*wx[a];
+              u[0] += w*us_[rr + a];
+              u[1] += w*us_[Tl.RN + rr + a];
+              u[2] += w*us_[2*Tl.RN + rr + a];
+            }
           }
+        const float af = static_cast<float>(A.a_dt);
+        const float cj32 = af/(tsq + af);
+        const double fac = -static_cast<double>(mq)*ivol;
+        for (int d=0; d<3; ++d) {
+          const float vo = vk0[d];
+          const float vn = vo + cj32*(u[d] - vo);
+          A.src.V(d, q) = vn;
+          const double delta = static_cast<double>(vn) - static_cast<double>(vo);
+          if constexpr (REGS) A.src.R(d, q) = delta*A.inv_a_dt;
+          const double imp = fac*delta;
+          ex[d] += imp;
+          rv[d*CH + r] = static_cast<float>(imp);
         }
-      const float af = static_cast<float>(A.a_dt);
-      const float cj32 = af/(tsq + af);
-      const double fac = -static_cast<double>(mq)*ivol;
-      for (int d=0; d<3; ++d) {
-        const float vo = vk0[d];
-        const float vn = vo + cj32*(u[d] - vo);
-        A.src.V(d, q) = vn;
-        const double delta = static_cast<double>(vn) - static_cast<double>(vo);
-        if constexpr (REGS) A.src.R(d, q) = delta*A.inv_a_dt;
-        const double imp = fac*delta;
-        ex[d] += imp;
-        rv[d*CH + r] = static_cast<float>(imp);
+        rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
+        rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
+        rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
+        rcode[r] = sj + 1;
       }
-      rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
-      rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
-      rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
-      rcode[r] = sj + 1;
+      __syncthreads();
+      float acc[3][3][3], spl[3][3];
+  #pragma unroll
+      for (int a=0; a<3; ++a)
+  #pragma unroll
+        for (int n=0; n<3; ++n) {
+          spl[a][n] = 0.0f;
+  #pragma unroll
+          for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+        }
+      int spdir = 0;
+      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+      for (int qq = s0; qq < s1; ++qq) {
+        const int r = qq - c0;
+        const float wz = rw[(6 + slab)*CH + r];
+        const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
+        const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+        const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
+        AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+      }
+      DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
+    }
+    // the flush, every thread summing (FP64) the words it writes; the defect: every thread's
+    // impulses minus those words, summed over each wavefront, then over the wavefronts in order
+    // (the record area is free after the last deposit's closing barrier)
+    double ws[3];
+    if (part == 0) FlushRegion<float, 3>(Tl, t, dep, scr, ws);
+    else FlushOverflow<3>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN, ws);
+    double *red = reinterpret_cast<double*>(rw);           // [NT/64][3]
+    const int wv = tid >> 6;
+    for (int d=0; d<3; ++d) {
+      const double a = WaveSum(ex[d] - ws[d], min(64, NT - (wv << 6)));
+      if ((tid & 63) == 0) red[3*wv + d] = a;
     }
     __syncthreads();
-    float acc[3][3][3], spl[3][3];
-#pragma unroll
-    for (int a=0; a<3; ++a)
-#pragma unroll
-      for (int n=0; n<3; ++n) {
-        spl[a][n] = 0.0f;
-#pragma unroll
-        for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
-      }
-    int spdir = 0;
-    const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-    for (int qq = s0; qq < s1; ++qq) {
-      const int r = qq - c0;
-      const float wz = rw[(6 + slab)*CH + r];
-      const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
-      const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
-      const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
-      AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+    if (tid < 3) {
+      double a = 0.0;
+      for (int w = 0; w < (NT + 63) >> 6; ++w) a += red[3*w + tid];
+      if (part == 0) defect[3*tile + tid] = a;
+      else A.ov_def[3*OvSlot(A.item_start, tile, part) + tid] = a;
     }
-    DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
-  }
-  // the flush, every thread summing (FP64) the words it writes; the defect: every thread's
-  // impulses minus those words, summed over each wavefront, then over the wavefronts in order
-  // (the record area is free after the last deposit's closing barrier)
-  double ws[3];
-  FlushRegion<float, 3>(Tl, t, dep, scr, ws);
-  double *red = reinterpret_cast<double*>(rw);           // [NT/64][3]
-  const int wv = tid >> 6;
-  for (int d=0; d<3; ++d) {
-    const double a = WaveSum(ex[d] - ws[d], min(64, NT - (wv << 6)));
-    if ((tid & 63) == 0) red[3*wv + d] = a;
-  }
-  __syncthreads();
-  if (tid < 3) {
-    double a = 0.0;
-    for (int w = 0; w < (NT + 63) >> 6; ++w) a += red[3*w + tid];
-    defect[3*(t.key0/Tl.TC) + tid] = a;
+    if constexpr (!OV) break;
+    __syncthreads();   // (the next item reuses LDS)
   }
 }
 
-template <int S, bool SY>
+template <int S, bool SY, bool OV>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_p3(KArgs A, const float *us, float *scr, double *defect) {
-  P3Body<S, SY, true>(A, us, scr, defect);
+  P3Body<S, SY, true, OV>(A, us, scr, defect);
 }
 
-template <int S, bool SY>
+template <int S, bool SY, bool OV>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_p3n(KArgs A, const float *us, float *scr, double *defect) {
-  P3Body<S, SY, false>(A, us, scr, defect);
+  P3Body<S, SY, false, OV>(A, us, scr, defect);
 }
 
-// every padded cell sums the tile regions covering it, in ascending tile order (grid: x over
-// the cells of a block, y over the blocks), in the output type TO; with defect, the first cell of
-// every tile then adds the tile's defect (NV values per tile, tile-major)
+// the centre masses of a tile's later parts added in part order (thread per cell of the tiles
+// with more than one part; every target has one writer: its own cell, the cell above, below)
+__global__ void k_fix_qt(KArgs A, float *qt0, float *qt1, float *qt2) {
+  const Geo &g = A.g;
+  const Tiles &Tl = A.T;
+  GRID_STRIDE(u, (long long)Tl.nt*Tl.TC) {
+    const int tile = static_cast<int>(u/Tl.TC), l = static_cast<int>(u - (long long)tile*Tl.TC);
+    const int i0 = A.item_start[tile], np = A.item_start[tile + 1] - i0;
+    if (np <= 1) continue;
+    const TileInfo t = TileOf(Tl, tile);
+    int ci, cj, ck;
+    CellIdx(Tl, t, l, ci, cj, ck);
+    const long long c = Idx(g, t.m, 1, 0, g.ks + ck, g.js + cj, g.is + ci);
+    float a0 = qt0[c], a1 = qt1[c + g.n1], a2 = qt2[c - g.n1];
+    for (int p = 1; p < np; ++p) {
+      const float *o = A.ov_q + ((long long)i0 - tile + p - 1)*3*Tl.TC;
+      a0 += o[l]; a1 += o[Tl.TC + l]; a2 += o[2*Tl.TC + l];
+    }
+    qt0[c] = a0; qt1[c + g.n1] = a1; qt2[c - g.n1] = a2;
+  }
+}
+
+// the push residues and dust corrections of a tile's later parts added in part order (as
+// k_fix_qt)
+__global__ void k_fix_key(KArgs A, int do_key, double *pres) {
+  const Geo &g = A.g;
+  const Tiles &Tl = A.T;
+  GRID_STRIDE(u, (long long)Tl.nt*Tl.TC) {
+    const int tile = static_cast<int>(u/Tl.TC), l = static_cast<int>(u - (long long)tile*Tl.TC);
+    const int i0 = A.item_start[tile], np = A.item_start[tile + 1] - i0;
+    if (np <= 1) continue;
+    const TileInfo t = TileOf(Tl, tile);
+    int ci, cj, ck;
+    CellIdx(Tl, t, l, ci, cj, ck);
+    const long long c = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+    for (int d=0; d<3; ++d) {
+      double pr = do_key ? pres[c + d*g.ncb] : 0.0;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      if (A.pending) {a0 = A.dc2[c - g.n1 + d*g.ncb]; a1 = A.dc0[c + d*g.ncb];
+                      a2 = A.dc1[c + g.n1 + d*g.ncb];}
+      for (int p = 1; p < np; ++p) {
+        const long long o = (long long)i0 - tile + p - 1;
+        if (do_key) pr += static_cast<double>(A.ov_pr[(o*3 + d)*Tl.TC + l]);
+        if (A.pending) {
+          a0 += A.ov_dc[(o*9 + d)*Tl.TC + l];
+          a1 += A.ov_dc[(o*9 + 3 + d)*Tl.TC + l];
+          a2 += A.ov_dc[(o*9 + 6 + d)*Tl.TC + l];
+        }
+      }
+      if (do_key) pres[c + d*g.ncb] = pr;
+      if (A.pending) {A.dc2[c - g.n1 + d*g.ncb] = a0; A.dc0[c + d*g.ncb] = a1;
+                      A.dc1[c + g.n1 + d*g.ncb] = a2;}
+    }
+  }
+}
+
+// every padded cell sums the tile regions covering it, in ascending tile order and, within a tile,
+// in part order (the later parts' regions in their overflow slots) (grid: x over the cells of a
+// block, y over the blocks), in the output type TO; with defect, the first cell of every tile
+// then adds the tile's defects (NV values per tile and part) in part order
 template <typename T, int NV, typename TO = T>
-__global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *defect) {
+__global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *defect,
+                          const int *item_start, const float *ov_scr, const double *ov_def) {
   const int m = blockIdx.y;
   const int ncb = static_cast<int>(g.ncb);
   for (int rr = blockIdx.x*blockDim.x + threadIdx.x; rr < ncb; rr += gridDim.x*blockDim.x) {
@@ -1228,6 +1405,14 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *
             const long long vstride = (long long)Tl.RZ*Tl.RY*Tl.ntx*Tl.RX;
 #pragma unroll
             for (int v=0; v<NV; ++v) sum[v] += static_cast<TO>(scr[base + v*vstride]);
+            const int tile = (m*Tl.ntz + tz)*Tl.nty*Tl.ntx + ty*Tl.ntx + tx;
+            const int i0 = item_start[tile], np = item_start[tile + 1] - i0;
+            for (int p = 1; p < np; ++p) {
+              const float *o = ov_scr + ((long long)(i0 - tile + p - 1)*4*Tl.RN +
+                                         (rz*Tl.RY + ry)*Tl.RX + rx);
+#pragma unroll
+              for (int v=0; v<NV; ++v) sum[v] += static_cast<TO>(o[v*Tl.RN]);
+            }
           }
         }
       }
@@ -1240,6 +1425,10 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *
                              ((lk/Tl.TZ)*Tl.nty + (lj >> Tl.LY))*Tl.ntx + (li >> Tl.LX);
 #pragma unroll
         for (int v=0; v<NV; ++v) sum[v] += defect[NV*ct + v];
+        const int i0 = item_start[ct], np = item_start[ct + 1] - i0;
+        for (int p = 1; p < np; ++p)
+#pragma unroll
+          for (int v=0; v<NV; ++v) sum[v] += ov_def[NV*((long long)i0 - ct + p - 1) + v];
       }
     }
 #pragma unroll
@@ -1503,36 +1692,56 @@ size_t LdsP3(const Tiles &T) { return LdsP2(T); }   // the same areas
 
 // a tile kernel instantiated for the deposit scheme and the frame-shift flag of the context,
 // launched with grid, blk, lds in scope
-#define SWITCH_SCHEME(KERNEL, ARGS) \
+#define SWITCH_SCHEME1(KERNEL, OV, grid, ARGS) \
   switch (c->g.deposit*2 + c->fast->T.SY) { \
-    case 5: KERNEL<2, true><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 4: KERNEL<2, false><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 3: KERNEL<1, true><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 2: KERNEL<1, false><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 1: KERNEL<0, true><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    default: KERNEL<0, false><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 5: KERNEL<2, true, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 4: KERNEL<2, false, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 3: KERNEL<1, true, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 2: KERNEL<1, false, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 1: KERNEL<0, true, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    default: KERNEL<0, false, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
   }
+
+// a tile kernel over the work items: the first nt (a workgroup each), then the extra parts
+#define SWITCH_SCHEME(KERNEL, ARGS) \
+  do { \
+    SWITCH_SCHEME1(KERNEL, false, dim3(T.nt), ARGS) \
+    SWITCH_SCHEME1(KERNEL, true, OverGrid(c), ARGS) \
+  } while (0)
+
+// workgroups of the launch over the extra parts (they loop over the items; all exit at once when
+// no tile has more than one part)
+dim3 OverGrid(const dk_ctx *c) {
+  return dim3(static_cast<unsigned>(std::min<long long>(c->fast->ov_cap, kOverGrid)));
+}
+
+void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres) {
+  const Fast *f = c->fast;
+  const size_t lds = LdsKey(f->T, f->key_threads);
+  k_key<false><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres);
+  k_key<true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres);
+}
 
 void LaunchP1(dk_ctx *c, const KArgs &A, float *qt0, float *qt1, float *qt2, float *scr) {
   const Tiles &T = c->fast->T;
-  const dim3 grid(T.nt), blk(3*T.TC);
+  const dim3 blk(3*T.TC);
   const size_t lds = LdsP1(T);
   SWITCH_SCHEME(k_p1, (A, qt0, qt1, qt2, scr));
 }
 
 void LaunchP2(dk_ctx *c, const KArgs &A, const float *us, float *scr) {
   const Tiles &T = c->fast->T;
-  const dim3 grid(T.nt), blk(3*T.TC);
+  const dim3 blk(3*T.TC);
   const size_t lds = LdsP2(T);
   SWITCH_SCHEME(k_p2, (A, us, scr));
 }
 
 void LaunchP3(dk_ctx *c, const KArgs &A, const float *us, float *scr, double *defect) {
   const Tiles &T = c->fast->T;
-  const dim3 grid(T.nt), blk(3*T.TC);
+  const dim3 blk(3*T.TC);
   const size_t lds = LdsP3(T);
-  if (A.regs_live) SWITCH_SCHEME(k_p3, (A, us, scr, defect))
-  else SWITCH_SCHEME(k_p3n, (A, us, scr, defect))
+  if (A.regs_live) SWITCH_SCHEME(k_p3, (A, us, scr, defect));
+  else SWITCH_SCHEME(k_p3n, (A, us, scr, defect));
 }
 
 dim3 CombineGrid(const dk_ctx *c) {
@@ -1591,6 +1800,10 @@ KArgs MakeArgs(dk_ctx *c) {
   A.inv_a_prev = f->pending ? 1.0/f->a_prev : 0.0;
   A.regs_live = !f->regs_dead;
   A.res = f->res; A.dc0 = f->dc[0]; A.dc1 = f->dc[1]; A.dc2 = f->dc[2];
+  A.item_start = f->item_start; A.item_tile = f->item_tile;
+  A.W = kPPT*3*f->T.TC;
+  A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
+  A.ov_pr = f->ov_pr;
   return A;
 }
 
@@ -1613,13 +1826,62 @@ int SortKeys(dk_ctx *c) {
   return dk::Check(hipGetLastError());
 }
 
+// the work items of the present order: parts per tile, their scan, the tile of every item
+int BuildItems(dk_ctx *c) {
+  Fast *f = c->fast;
+  const Tiles &T = f->T;
+  const int W = kPPT*3*T.TC;
+  TIMED("items", (k_nparts<<<Grid(T.nt + 1), kBlock, 0, c->stream>>>(T, f->start, W,
+                                                                      f->nparts)));
+  int e;
+  TIMED("items", e = dk::Check(hipcub::DeviceScan::ExclusiveSum(
+      f->iscan_tmp, f->iscan_bytes, f->nparts, f->item_start, T.nt + 1, c->stream)));
+  if (e) return e;
+  TIMED("items", (k_item_tile<<<Grid(T.nt), kBlock, 0, c->stream>>>(T, f->item_start,
+                                                                     f->item_tile)));
+  return dk::Check(hipGetLastError());
+}
+
+// the overflow slots and item list for a store of n particles
+int AllocItems(dk_ctx *c, long long n) {
+  Fast *f = c->fast;
+  const Tiles &T = f->T;
+  const long long W = (long long)kPPT*3*T.TC;
+  const long long ov = (n + W - 1)/W + 1;
+  if (ov <= f->ov_cap && f->item_start) return DK_OK;
+  for (void *p : {(void*)f->item_tile, (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr,
+                  (void*)f->ov_def, (void*)f->ov_dc})
+    if (p) (void)hipFree(p);
+  f->item_tile = nullptr; f->ov_scr = nullptr; f->ov_q = nullptr; f->ov_pr = nullptr;
+  f->ov_def = nullptr; f->ov_dc = nullptr;
+  if (f->nparts == nullptr) {
+    if (Alloc((void**)&f->nparts, sizeof(int)*(T.nt + 1))) return DK_ENOMEM;
+    if (Alloc((void**)&f->item_start, sizeof(int)*(T.nt + 1))) return DK_ENOMEM;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, f->iscan_bytes, f->nparts, f->item_start,
+                                         T.nt + 1) != hipSuccess)
+      return DK_EHIP;
+    if (Alloc(&f->iscan_tmp, std::max<size_t>(f->iscan_bytes, 16))) return DK_ENOMEM;
+  }
+  f->item_cap = T.nt + ov;
+  f->ov_cap = ov;
+  if (Alloc((void**)&f->item_tile, sizeof(int)*f->item_cap) ||
+      Alloc((void**)&f->ov_scr, sizeof(float)*4*(size_t)T.RN*ov) ||
+      Alloc((void**)&f->ov_q, sizeof(float)*3*(size_t)T.TC*ov) ||
+      Alloc((void**)&f->ov_pr, sizeof(float)*3*(size_t)T.TC*ov) ||
+      Alloc((void**)&f->ov_def, sizeof(double)*3*(size_t)ov) ||
+      Alloc((void**)&f->ov_dc, sizeof(double)*9*(size_t)T.TC*ov))
+    return DK_ENOMEM;
+  return DK_OK;
+}
+
 // the pending consistent-dust correction alone (before an export or a diagnostic)
 int Flush(dk_ctx *c) {
   Fast *f = c->fast;
   if (!f->pending) return DK_OK;
   KArgs A = MakeArgs(c);
-  TIMED("key", (k_key<<<f->T.nt, f->key_threads, LdsKey(f->T, f->key_threads),
-                        c->stream>>>(A, 0, nullptr)));
+  TIMED("key", LaunchKey(c, A, 0, nullptr));
+  TIMED("fix", (k_fix_key<<<Grid((long long)f->T.nt*f->T.TC), kBlock, 0, c->stream>>>(
+      A, 0, nullptr)));
   dk_gas_src z;
   std::memset(&z, 0, sizeof(z));
   const long long nact = (long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3;
@@ -1722,6 +1984,10 @@ void FastDestroy(dk_ctx *c) {
   if (f == nullptr) return;
   for (auto &ev : f->tev)
     if (ev) {(void)hipEventDestroy(ev); ev = nullptr;}
+  for (void *p : {(void*)f->nparts, (void*)f->item_start, (void*)f->item_tile, f->iscan_tmp,
+                  (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr, (void*)f->ov_def,
+                  (void*)f->ov_dc})
+    if (p) (void)hipFree(p);
   for (int b=0; b<2; ++b) {
     Store &S = f->s[b];
     for (void *p : {(void*)S.d, (void*)S.f, (void*)S.i})
@@ -1771,6 +2037,8 @@ int FastBind(dk_ctx *c) {
   f->cur = 0;
   f->pending = 0;
   f->regs_dead = 0;
+  int ea;
+  if ((ea = AllocItems(c, std::max<long long>(f->cap, f->np)))) return ea;
   // (with no particles the passes below run over empty sets: they still write the cell
   // offsets, start[] = 0, that every later pass reads)
   k_bind<<<Grid(f->np), kBlock, 0, c->stream>>>(c->rdata, c->idata, (long long)c->cap, f->np,
@@ -1781,6 +2049,7 @@ int FastBind(dk_ctx *c) {
   k_key_static<<<Grid(f->np), kBlock, 0, c->stream>>>(c->g, f->T, c->d_blocks, f->s[0], f->np,
                                                       0.0, f->key, f->rank, f->count);
   if ((e = SortKeys(c))) return e;
+  if ((e = BuildItems(c))) return e;
   k_permute_all<<<Grid(f->np), kBlock, 0, c->stream>>>(f->s[0], f->s[1], f->perm, f->np);
   f->cur = 1;
   return dk::Check(hipGetLastError());
@@ -1813,7 +2082,7 @@ int FastDiag(dk_ctx *c, double out[9]) {
 
 int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   Fast *f = c->fast;
-  if (f == nullptr || c->u0 == nullptr) return DK_ESTATE;
+  if (f == nullptr || c->u0 == nullptr || f->item_start == nullptr) return DK_ESTATE;
   if (!ActiveStage(c, st)) return DK_OK;   // the identity stage of imex2+
   const Geo &g = c->g;
   const Tiles &T = f->T;
@@ -1844,8 +2113,9 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   // 1. key pass (with the pending correction), histogram
   if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
   // the push of this stage writes its residue into the buffer the next stage's start reads
-  TIMED("key", (k_key<<<T.nt, f->key_threads, LdsKey(T, f->key_threads), c->stream>>>(A, 1,
-                                                                  f->pres[1 - f->pres_rd])));
+  TIMED("key", LaunchKey(c, A, 1, f->pres[1 - f->pres_rd]));
+  TIMED("fix", (k_fix_key<<<Grid((long long)T.nt*T.TC), kBlock, 0, c->stream>>>(
+      A, 1, f->pres[1 - f->pres_rd])));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history
   {
     const int hook = 1;
@@ -1862,13 +2132,16 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   A.pending = 0;
   // 3. counting sort
   if ((e = SortKeys(c))) return e;
+  if ((e = BuildItems(c))) return e;
   // 4. push into the new order + deposit Q, P, qt
   TIMED("p1_gather_deposit", LaunchP1(c, A, f->qt[0], f->qt[1], f->qt[2], f->scr_f));
+  TIMED("fix", (k_fix_qt<<<Grid((long long)T.nt*T.TC), kBlock, 0, c->stream>>>(
+      A, f->qt[0], f->qt[1], f->qt[2])));
   f->cur = 1 - f->cur;
   A.src = f->s[f->cur];
   A.dst = f->s[1 - f->cur];
   TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
-      g, T, f->scr_f, f->qdep, nullptr)));
+      g, T, f->scr_f, f->qdep, nullptr, f->item_start, f->ov_scr, f->ov_def)));
   if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
   // 5. local guess + dc1/dc2 sweeps
   TIMED("guess", (k_guess<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ustar)));
@@ -1877,7 +2150,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
     TIMED("p2_operator", LaunchP2(c, A, f->ustar, f->scr_f));
     TIMED("combine", (k_combine<float, 3><<<CombineGrid(c), kBlock, 0, c->stream>>>(
-        g, T, f->scr_f, f->ap, nullptr)));
+        g, T, f->scr_f, f->ap, nullptr, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<float>(c, f->ap, 3, st->time))) return e;
     TIMED("dc", (k_dc<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ap, f->ustar)));
   }
@@ -1885,7 +2158,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   // 6. gather-kick + impulse deposit
   TIMED("p3_gather_kick", LaunchP3(c, A, f->ustar, f->scr_f, f->defect));
   TIMED("combine", (k_combine<float, 3, double><<<CombineGrid(c), kBlock, 0, c->stream>>>(
-      g, T, f->scr_f, f->dmom, f->defect)));
+      g, T, f->scr_f, f->dmom, f->defect, f->item_start, f->ov_scr, f->ov_def)));
   if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;
   // 7. consistent split: the dust's residual per unit coupled mass
   TIMED("split", (k_split<<<Grid(nact), kBlock, 0, c->stream>>>(

@@ -125,7 +125,9 @@ def gpu_run(w, form, steps, warmup):
     ctx.prof(False)
     ctx.check_device_error()
     d1 = ctx.diag()
-    drift = float(abs((d1[0:3] + d1[3:6]) - (d0[0:3] + d0[3:6])).max() / d0[8])
+    # (relative to the larger of the two momentum scales: a run may start at rest)
+    drift = float(abs((d1[0:3] + d1[3:6]) - (d0[0:3] + d0[3:6])).max() /
+                  max(d0[8], d1[8], 1e-300))
     ctx.export()
     n = ctx.npart
     order = torch.argsort(ctx.idata[abi.PTAG, :n])
