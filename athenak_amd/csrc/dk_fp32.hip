@@ -46,8 +46,10 @@ using dk::kBlock;
 
 namespace {
 
-constexpr int kSegSmall = 32;       // lists longer than this are ordered by k_bigsort
+constexpr int kSegSmall = 32;       // lists longer than this are ordered by k_medsort
+constexpr int kMedMax = 512;        // lists longer than this are ordered by k_bigsort
 constexpr int kBigMax = 16384;      // longest list of one cell in the FP32 form
+constexpr int kPartRank = 1024;     // a stayer's rank: part*kPartRank + its rank in the part
 constexpr int kKeyThreads = 256;    // the key pass: one particle per thread per chunk (default)
 constexpr int kKeyThreadsMax = 512;
 constexpr int kOverGrid = 4096;     // workgroups of the launch over the extra parts of full tiles
@@ -99,7 +101,8 @@ struct Fast {
   long long np = 0, cap = 0, nkeys = 0;
   int *key = nullptr, *rank = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr;
   int *stay = nullptr;              // per cell: the particles that stayed in its tile
-  int *big = nullptr;               // [0] = count of long lists, then their cells
+  // [0], [1]: counts of medium and long lists; their cells from [2] up and from [1 + nkeys] down
+  int *big = nullptr;
   // work items (KArgs): parts per tile, their scan, the tile of every item; overflow slots
   int *nparts = nullptr, *item_start = nullptr, *item_tile = nullptr;
   long long item_cap = 0, ov_cap = 0;
@@ -107,6 +110,7 @@ struct Fast {
   size_t iscan_bytes = 0;
   float *ov_scr = nullptr, *ov_q = nullptr, *ov_pr = nullptr;
   double *ov_def = nullptr, *ov_dc = nullptr;
+  int *ov_cnt = nullptr;
   void *scan_tmp = nullptr;
   size_t scan_bytes = 0;
   int32_t *itag = nullptr;          // caller's PTAG, PSP, PLEV, PTAGH by original index
@@ -187,6 +191,7 @@ struct KArgs {
   float *ov_q;                      // [slot][3][TC] centre masses (own, to above, to below)
   double *ov_dc;                    // [slot][9][TC] dust corrections by centre row
   float *ov_pr;                     // [slot][3][TC] push residues
+  int *ov_cnt;                      // [slot][TC] stayers counted by the part, then the offsets
 };
 
 // the cell (active index, unclamped) of position x along one axis, as PMWeights
@@ -382,6 +387,7 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
 // scratch.
 
 constexpr int kPPT = 2;             // particles per thread in the particle phase
+static_assert(kPPT*kCellThreads <= kPartRank, "a part's stayer ranks must fit below kPartRank");
 
 struct TileInfo {
   int m, tx, ty, tz, ci0, cj0, ck0;
@@ -796,10 +802,10 @@ k_key(KArgs A, int do_key, double *pres) {
           const long long key = TileKey(Tl, s.gid, ni, nj, nk);
           A.key[q] = static_cast<int>(key);
           const long long lk = key - t.key0;
-          // (a part after the first counts all its particles as arrivals: the LDS counts
-          // number the first part's stayers)
-          A.rank[q] = (part == 0 && lk >= 0 && lk < Tl.TC) ? atomicAdd(&lcnt[lk], 1)
-                                                           : -1 - atomicAdd(&A.count[key], 1);
+          // (a stayer's rank is its rank among its part's stayers of the cell, tagged with the
+          // part: the placement adds the stayers of the earlier parts)
+          A.rank[q] = (lk >= 0 && lk < Tl.TC) ? part*kPartRank + atomicAdd(&lcnt[lk], 1)
+                                              : -1 - atomicAdd(&A.count[key], 1);
         }
         for (int d=0; d<3; ++d) {rdc[d*CH + tid] = dv[d]; rpr[d*CH + tid] = pr[d];}
         rcode[tid] = code;
@@ -821,6 +827,9 @@ k_key(KArgs A, int do_key, double *pres) {
     }
     if (do_key && part == 0)
       for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
+    else if (do_key)
+      for (int i = tid; i < Tl.TC; i += CH)
+        A.ov_cnt[OvSlot(A.item_start, tile, part)*Tl.TC + i] = lcnt[i];
     if (tid < Tl.TC && part == 0) {
       int ci, cj, ck;
       CellIdx(Tl, t, tid, ci, cj, ck);
@@ -873,23 +882,42 @@ __global__ void k_total(int *count, const int *stay, long long nkeys) {
   GRID_STRIDE(c, nkeys) count[c] += stay[c];
 }
 
-// placement: stayers first (their LDS rank), then arrivals (rank -1 - r)
+// placement: stayers first (part*kPartRank + their rank in the part: a later part's stayers after
+// the offset in its overflow slot), then arrivals (rank -1 - r); item_start: the items of the
+// order the ranks were taken in
 __global__ void k_place(const int *key, const int *rank, const int *start, const int *stay,
-                        long long np, int *perm) {
+                        long long np, int TC, const int *item_start, const int *ov_cnt,
+                        int *perm) {
   GRID_STRIDE(p, np) {
     const int k = key[p], r = rank[p];
-    perm[start[k] + ((r >= 0) ? r : (stay[k] - 1 - r))] = static_cast<int>(p);
+    int off;
+    if (r < 0) {
+      off = stay[k] - 1 - r;
+    } else if (r < kPartRank) {
+      off = r;
+    } else {
+      const int part = r/kPartRank, tile = k/TC;
+      off = ov_cnt[((long long)item_start[tile] - tile + part - 1)*TC + (k - tile*TC)] +
+            (r - part*kPartRank);
+    }
+    perm[start[k] + off] = static_cast<int>(p);
   }
 }
 
-// every cell's list in ascending source slot (the stable order); long lists are queued
+// every cell's list in ascending source slot (the stable order); medium and long lists are
+// queued
 __global__ void k_segsort(const int *start, long long nkeys, int *perm, int *big) {
   GRID_STRIDE(c, nkeys) {
     const int s0 = start[c], s1 = start[c + 1], n = s1 - s0;
     if (n <= 1) continue;
+    if (n > kMedMax) {
+      const int slot = atomicAdd(&big[1], 1);
+      big[1 + nkeys - slot] = static_cast<int>(c);
+      continue;
+    }
     if (n > kSegSmall) {
       const int slot = atomicAdd(&big[0], 1);
-      big[1 + slot] = static_cast<int>(c);
+      big[2 + slot] = static_cast<int>(c);
       continue;
     }
     for (int i=s0+1; i<s1; ++i) {
@@ -901,13 +929,44 @@ __global__ void k_segsort(const int *start, long long nkeys, int *perm, int *big
   }
 }
 
+// one wavefront per medium list: an element's place is the count of smaller source slots in the
+// list (the slots are distinct), the list staged in LDS
+__global__ void __launch_bounds__(64) k_medsort(const int *start, int *perm, const int *big) {
+  constexpr int kPer = kMedMax/64;
+  __shared__ int sh[kMedMax];
+  const int nmed = big[0];
+  const int lane = threadIdx.x;
+  for (int e = blockIdx.x; e < nmed; e += gridDim.x) {
+    const int c = big[2 + e];
+    const int s0 = start[c], n = start[c + 1] - s0;
+    int v[kPer], r[kPer];
+#pragma unroll
+    for (int t=0; t<kPer; ++t) {
+      const int i = t*64 + lane;
+      v[t] = (i < n) ? perm[s0 + i] : INT_MAX;
+      r[t] = 0;
+      sh[i] = v[t];
+    }
+    __syncthreads();
+    for (int j=0; j<n; ++j) {
+      const int x = sh[j];
+#pragma unroll
+      for (int t=0; t<kPer; ++t) r[t] += (x < v[t]) ? 1 : 0;
+    }
+#pragma unroll
+    for (int t=0; t<kPer; ++t)
+      if (t*64 + lane < n) perm[s0 + r[t]] = v[t];
+    __syncthreads();
+  }
+}
+
 // one workgroup per long list: bitonic sort in LDS (padded to a power of two)
 __global__ void __launch_bounds__(1024) k_bigsort(const int *start, int *perm, const int *big,
-                                                  dk::ErrWord *err) {
+                                                  long long nkeys, dk::ErrWord *err) {
   __shared__ int sh[kBigMax];
-  const int nbig = big[0];
+  const int nbig = big[1];
   for (int e = blockIdx.x; e < nbig; e += gridDim.x) {
-    const int c = big[1 + e];
+    const int c = big[1 + nkeys - e];
     const int s0 = start[c], n = start[c + 1] - s0;
     if (n > kBigMax) {
       if (threadIdx.x == 0) dk::RecordError(err, DK_ECAPACITY, c);
@@ -1366,6 +1425,19 @@ __global__ void k_fix_key(KArgs A, int do_key, double *pres) {
       if (A.pending) {A.dc2[c - g.n1 + d*g.ncb] = a0; A.dc0[c + d*g.ncb] = a1;
                       A.dc1[c + g.n1 + d*g.ncb] = a2;}
     }
+    if (do_key) {
+      // the stayers of each later part follow those of the parts before it: its count becomes
+      // its offset, and the cell's stay count the total
+      const long long k = (long long)tile*Tl.TC + l;
+      int base = A.stay[k];
+      for (int p = 1; p < np; ++p) {
+        int &o = A.ov_cnt[((long long)i0 - tile + p - 1)*Tl.TC + l];
+        const int n = o;
+        o = base;
+        base += n;
+      }
+      A.stay[k] = base;
+    }
   }
 }
 
@@ -1803,7 +1875,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.item_start = f->item_start; A.item_tile = f->item_tile;
   A.W = kPPT*3*f->T.TC;
   A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
-  A.ov_pr = f->ov_pr;
+  A.ov_pr = f->ov_pr; A.ov_cnt = f->ov_cnt;
   return A;
 }
 
@@ -1816,12 +1888,13 @@ int SortKeys(dk_ctx *c) {
   TIMED("scan", e = dk::Check(hipcub::DeviceScan::ExclusiveSum(
       f->scan_tmp, f->scan_bytes, f->count, f->start, (int)(f->nkeys + 1), c->stream)));
   if (e) return e;
-  TIMED("place", (k_place<<<Grid(f->np), kBlock, 0, c->stream>>>(f->key, f->rank, f->start,
-                                                                   f->stay, f->np, f->perm)));
-  if ((e = FillV<int>(c, f->big, 1, 0))) return e;
+  TIMED("place", (k_place<<<Grid(f->np), kBlock, 0, c->stream>>>(
+      f->key, f->rank, f->start, f->stay, f->np, f->T.TC, f->item_start, f->ov_cnt, f->perm)));
+  if ((e = FillV<int>(c, f->big, 2, 0))) return e;
   TIMED("segsort", (k_segsort<<<Grid(f->nkeys), kBlock, 0, c->stream>>>(f->start, f->nkeys,
                                                                          f->perm, f->big)));
-  TIMED("bigsort", (k_bigsort<<<256, 1024, 0, c->stream>>>(f->start, f->perm, f->big,
+  TIMED("medsort", (k_medsort<<<4096, 64, 0, c->stream>>>(f->start, f->perm, f->big)));
+  TIMED("bigsort", (k_bigsort<<<256, 1024, 0, c->stream>>>(f->start, f->perm, f->big, f->nkeys,
                                                             c->d_err)));
   return dk::Check(hipGetLastError());
 }
@@ -1850,10 +1923,10 @@ int AllocItems(dk_ctx *c, long long n) {
   const long long ov = (n + W - 1)/W + 1;
   if (ov <= f->ov_cap && f->item_start) return DK_OK;
   for (void *p : {(void*)f->item_tile, (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr,
-                  (void*)f->ov_def, (void*)f->ov_dc})
+                  (void*)f->ov_def, (void*)f->ov_dc, (void*)f->ov_cnt})
     if (p) (void)hipFree(p);
   f->item_tile = nullptr; f->ov_scr = nullptr; f->ov_q = nullptr; f->ov_pr = nullptr;
-  f->ov_def = nullptr; f->ov_dc = nullptr;
+  f->ov_def = nullptr; f->ov_dc = nullptr; f->ov_cnt = nullptr;
   if (f->nparts == nullptr) {
     if (Alloc((void**)&f->nparts, sizeof(int)*(T.nt + 1))) return DK_ENOMEM;
     if (Alloc((void**)&f->item_start, sizeof(int)*(T.nt + 1))) return DK_ENOMEM;
@@ -1869,7 +1942,8 @@ int AllocItems(dk_ctx *c, long long n) {
       Alloc((void**)&f->ov_q, sizeof(float)*3*(size_t)T.TC*ov) ||
       Alloc((void**)&f->ov_pr, sizeof(float)*3*(size_t)T.TC*ov) ||
       Alloc((void**)&f->ov_def, sizeof(double)*3*(size_t)ov) ||
-      Alloc((void**)&f->ov_dc, sizeof(double)*9*(size_t)T.TC*ov))
+      Alloc((void**)&f->ov_dc, sizeof(double)*9*(size_t)T.TC*ov) ||
+      Alloc((void**)&f->ov_cnt, sizeof(int)*(size_t)T.TC*ov))
     return DK_ENOMEM;
   return DK_OK;
 }
@@ -1952,7 +2026,7 @@ int FastCreate(dk_ctx *c) {
   int e = DK_OK;
   e = e ? e : Alloc((void**)&f->count, sizeof(int)*(f->nkeys + 1));
   e = e ? e : Alloc((void**)&f->start, sizeof(int)*(f->nkeys + 1));
-  e = e ? e : Alloc((void**)&f->big, sizeof(int)*(f->nkeys + 1));
+  e = e ? e : Alloc((void**)&f->big, sizeof(int)*(f->nkeys + 2));
   e = e ? e : Alloc((void**)&f->stay, sizeof(int)*(f->nkeys + 1));
   e = e ? e : Alloc((void**)&f->qdep, sizeof(float)*4*nc);
   for (int q=0; q<3; ++q) e = e ? e : Alloc((void**)&f->qt[q], sizeof(float)*nc);
@@ -1986,7 +2060,7 @@ void FastDestroy(dk_ctx *c) {
     if (ev) {(void)hipEventDestroy(ev); ev = nullptr;}
   for (void *p : {(void*)f->nparts, (void*)f->item_start, (void*)f->item_tile, f->iscan_tmp,
                   (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr, (void*)f->ov_def,
-                  (void*)f->ov_dc})
+                  (void*)f->ov_dc, (void*)f->ov_cnt})
     if (p) (void)hipFree(p);
   for (int b=0; b<2; ++b) {
     Store &S = f->s[b];
