@@ -67,6 +67,10 @@ def parse():
     ap.add_argument("--n", type=int, default=None, help="override the mesh size (tests)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="FP32 form: keep the default launch parameters (no autotuner steps)")
+    ap.add_argument("--prof-steps", type=int, default=5,
+                    help="eager steps timed kernel by kernel after the timed steps (>= 1)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run plain device copies of this many GiB (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -90,7 +94,11 @@ def describe(w):
             "shear_periodic_x1": bool(m.bc_x1), "solver": "dc1"}
 
 
-def gpu_run(w, form, steps, warmup):
+def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True):
+    """Runs one form: (FP32) the online autotuner's steps, `warmup` steps, `steps` timed steps
+    (FP32: every stage one HIP graph launch), then `prof_steps` eager steps timed kernel by kernel
+    with HIP events on the library's stream (events cannot enter the graph).  The FP64 form takes
+    `tune_steps` extra warm-up steps so that both forms end at the same step."""
     import torch
     from athenak_amd import abi
     from athenak_amd.lib import DustContext
@@ -103,13 +111,26 @@ def gpu_run(w, form, steps, warmup):
         ctx.sort()
     torch.cuda.synchronize()
     d0 = ctx.diag()
+    tuned = None
+    if form == "fp32" and tune:
+        # the autotuner times its candidates on these steps (no choice changes a result)
+        ctx.autotune(True)
+        tune_steps = 0
+        while ctx.tune_config()[0] == 1 and tune_steps < 16:
+            ctx.step(w.dt, w.src)
+            tune_steps += 1
+        st, xcd, kt = ctx.tune_config()
+        tuned = {"state": {0: "fixed", 1: "tuning", 2: "tuned"}[st], "xcd_dealing": xcd,
+                 "key_threads": kt, "steps": tune_steps}
+    else:
+        for _ in range(tune_steps):
+            ctx.step(w.dt, w.src)
     for _ in range(warmup):
         ctx.step(w.dt, w.src)
     ctx.check_device_error()
-    torch.cuda.synchronize()
-    ctx.prof(True)
-    ctx.prof_reset()
     stream = ctx.stream
+    graphs = form == "fp32"
+    ctx.set_graphs(graphs)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -121,6 +142,18 @@ def gpu_run(w, form, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
+    ctx.set_graphs(False)
+    ctx.check_device_error()
+    # per-kernel HIP events over eager steps
+    ctx.prof(True)
+    ctx.prof_reset()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(prof_steps):
+        ctx.step(w.dt, w.src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_prof = e0.elapsed_time(e1)
     prof = ctx.prof_read()
     ctx.prof(False)
     ctx.check_device_error()
@@ -135,7 +168,9 @@ def gpu_run(w, form, steps, warmup):
     ctx.close()
     del ctx
     torch.cuda.empty_cache()
-    return {"ms": ms, "wall_s": wall, "prof": prof, "drift": drift, "vel": vel}
+    return {"ms": ms, "wall_s": wall, "prof": prof, "drift": drift, "vel": vel,
+            "ms_prof": ms_prof, "prof_steps": prof_steps, "tune_steps": tune_steps,
+            "tuned": tuned, "graphs": graphs}
 
 
 def drift_or_none(w, drift):
@@ -176,11 +211,13 @@ def cpu_baseline(name, budget_s):
 
 
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of
-# known size) of the C2 FP32 run, committed under profiles/; timer label -> kernel name prefix
+# known size) of the C2 FP32 run, committed under profiles/.  A timer label covers one launch of
+# every kernel of a group (a tile kernel is two launches: its first items and its extra parts);
+# labels with two groups (P3 with and without the stage-2 rate store) take their mean.
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "hbm_traffic_c2_fp32.md")
-TRAFFIC_KERNEL = {"p1_gather_deposit": ["k_p1"], "p2_operator": ["k_p2"],
-                  "p3_gather_kick": ["k_p3", "k_p3n"], "key": ["k_key"]}
+TRAFFIC_KERNEL = {"p1_gather_deposit": [["k_p1<"]], "p2_operator": [["k_p2<"]],
+                  "p3_gather_kick": [["k_p3<"], ["k_p3n<"]], "key": [["k_key<"]]}
 
 
 def measured_traffic(label, w, form):
@@ -190,17 +227,16 @@ def measured_traffic(label, w, form):
         lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
         return None
-    # the mean over the kernels the label times (one launch of each per step)
+    per_kernel = {}
+    for ln in lines:
+        parts = ln.split()
+        if len(parts) >= 3 and parts[0] == "traffic":
+            per_kernel[" ".join(parts[1:-1])] = float(parts[-1])
     vals = []
-    for want in TRAFFIC_KERNEL[label]:
-        for ln in lines:
-            parts = ln.split()
-            if len(parts) < 3 or parts[0] != "traffic":
-                continue
-            name = " ".join(parts[1:-1])
-            if name == want or name.startswith(want + "<"):
-                vals.append(float(parts[-1]))
-                break
+    for group in TRAFFIC_KERNEL[label]:
+        hit = [v for name, v in per_kernel.items() if any(name.startswith(p) for p in group)]
+        if hit:
+            vals.append(sum(hit))
     return sum(vals) / len(vals) if vals else None
 
 
@@ -216,7 +252,7 @@ def kernel_roofline(r, w, form="fp32"):
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": measured_traffic(label, w, form),
             "avg_launch_ms": round(tot / cnt, 4), "alg_bytes_per_launch": alg,
-            "share_of_step": round(tot / r["ms"], 3)}
+            "share_of_step": round(tot / r["ms_prof"], 3)}
 
 
 def rate(r, w, steps):
@@ -249,14 +285,17 @@ def main():
               "not run (the path would shard by MeshBlock; see DESIGN.md)" % args.gpus)
         sys.exit(2)
     w = make_workload(args.workload, args.n)
-    forms = ["fp64", "fp32"] if args.form == "both" else [args.form]
+    forms = ["fp32", "fp64"] if args.form == "both" else [args.form]
     res = {}
+    tune_steps = 0
     for f in forms:
         print("[bench] %s form: %s, %d warm-up + %d timed steps" % (f, w.name, args.warmup,
                                                                      args.steps),
               file=sys.stderr, flush=True)
-        res[f] = gpu_run(w, f, args.steps, args.warmup)
-    main_form = forms[-1]
+        res[f] = gpu_run(w, f, args.steps, args.warmup, tune_steps=tune_steps,
+                         prof_steps=max(1, args.prof_steps), tune=not args.no_tune)
+        tune_steps = res[f]["tune_steps"]
+    main_form = forms[0]
     r = res[main_form]
     ms_step, mps = rate(r, w, args.steps)
     step_alg = balg_step(main_form, w.ppc)
@@ -273,8 +312,12 @@ def main():
                           "achieved": round(mps * 1e6 * step_alg / 1e9, 1), "unit": "GB/s",
                           "frac": round(mps * 1e6 * step_alg / 1e9 / HBM_PEAK_GBS, 4)},
         "momentum_drift": drift_or_none(w, r["drift"]),
-        "kernels_ms": {k: round(v[0], 3) for k, v in sorted(r["prof"].items(),
-                                                             key=lambda kv: -kv[1][0])},
+        "graphs": r["graphs"],
+        "autotune": r["tuned"],
+        "profiled_steps": {"steps": r["prof_steps"], "graphs": False,
+                           "ms_per_step": round(r["ms_prof"] / r["prof_steps"], 3)},
+        "kernels_ms_per_step": {k: round(v[0] / r["prof_steps"], 3)
+                                for k, v in sorted(r["prof"].items(), key=lambda kv: -kv[1][0])},
     }
     if "fp64" in res and "fp32" in res:
         ms64, mps64 = rate(res["fp64"], w, args.steps)
