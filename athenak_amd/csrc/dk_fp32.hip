@@ -622,6 +622,12 @@ __device__ inline void FlushRegion(const Tiles &Tl, const TileInfo &t, const T *
   }
 }
 
+// the record area of a tile kernel, in words: nrec words per particle of a chunk of ch, or the
+// three slab copies of an nc-component region that the deposit places there, if larger
+__host__ __device__ inline int RecArea(const Tiles &Tl, int ch, int nrec, int nc) {
+  return (nrec*ch > 3*nc*Tl.RN) ? nrec*ch : 3*nc*Tl.RN;
+}
+
 // the overflow slot of part >= 1 of a tile
 __device__ inline long long OvSlot(const int *item_start, int tile, int part) {
   return (long long)item_start[tile] - tile + part - 1;
@@ -1013,10 +1019,10 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
     const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
-    float *rw = reinterpret_cast<float*>(lds_d);           // [9][CH] weights x0-2, y0-2, z0-2
-    float *rv = rw + 9*CH;                                 // [4][CH] mu, mu v
+    float *rd = reinterpret_cast<float*>(lds_d);           // [3][CH] offsets in the cell x, y, z
+    float *rv = rd + 3*CH;                                 // [4][CH] mu, mu v
     int *rcode = reinterpret_cast<int*>(rv + 4*CH);        // [CH] centre | (sj + 1) << 2
-    float *reg = reinterpret_cast<float*>(rcode + CH);     // [4][RN]
+    float *reg = rd + RecArea(Tl, CH, 8, 4);               // [4][RN]
     int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
     for (int i = tid; i < 4*Tl.RN; i += NT) reg[i] = 0.0f;
@@ -1044,7 +1050,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
         if (A.regs_live)
           for (int d=0; d<3; ++d) {A.dst.X1(d, q) = A.src.X1(d, p); A.dst.V1(d, q) = A.src.V1(d, p);}
         A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = t.m; A.dst.ORIG(q) = A.src.ORIG(p);
-        float wx[3] = {0.0f, 0.0f, 0.0f}, wy[3] = {0.0f, 0.0f, 0.0f}, wz[3] = {0.0f, 0.0f, 0.0f};
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         float mu = 0.0f;
         int code = 1 << 2;   // no centre, sj = 0
         const Loc L = Locate<SY>(g, b, x, A.sh);
@@ -1061,13 +1067,12 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
             else cen = L.jc - L.cj + 2;
           }
           code = cen | ((L.sj + 1) << 2);
-          W3<S, float>(static_cast<float>(L.dx), wx);
-          W3<S, float>(static_cast<float>(L.dy), wy);
-          W3<S, float>(static_cast<float>(L.dz), wz);
+          ox = static_cast<float>(L.dx);
+          oy = static_cast<float>(L.dy);
+          oz = static_cast<float>(L.dz);
         }
-        rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
-        rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
-        rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
+        // (a record with mu = 0 deposits zeros whatever its weights)
+        rd[r] = ox; rd[CH + r] = oy; rd[2*CH + r] = oz;
         rv[r] = mu;
         rv[CH + r] = mu*v[0]; rv[2*CH + r] = mu*v[1]; rv[3*CH + r] = mu*v[2];
         rcode[r] = code;
@@ -1086,9 +1091,10 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
       const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
       for (int qq = s0; qq < s1; ++qq) {
         const int r = qq - c0;
-        const float wz = rw[(6 + slab)*CH + r];
-        const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
-        const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+        float wx[3], wy[3];
+        W3<S, float>(rd[r], wx);
+        W3<S, float>(rd[CH + r], wy);
+        const float wz = W1<S, float>(slab, rd[2*CH + r]);
         const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
         const int code = rcode[r];
         AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
@@ -1099,7 +1105,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
           q3[2] += (cen == 3) ? val[0] : 0.0f;
         }
       }
-      DepositSlabs<float, 4, SY>(Tl, reg, rw, l, slab, acc, spl, spdir);
+      DepositSlabs<float, 4, SY>(Tl, reg, rd, l, slab, acc, spl, spdir);
     }
     if (slab == 1 && part == 0) {
       int ci, cj, ck;
@@ -1138,10 +1144,10 @@ k_p2(KArgs A, const float *us, float *scr) {
     const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
     float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
     float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
-    float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
-    float *rg = rw + 9*CH;                                 // [3][CH] (m c/V) G u*
+    float *rd = us_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
+    float *rg = rd + 3*CH;                                 // [3][CH] (m c/V) G u*
     int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1
-    int *st = rcode + CH;                                  // [TC + 1]
+    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 7, 3));   // [TC + 1]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1188,9 +1194,8 @@ k_p2(KArgs A, const float *us, float *scr) {
           }
         const float af = static_cast<float>(A.a_dt);
         const float fac = mq*(af/(tsq + af))*static_cast<float>(ivol);
-        rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
-        rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
-        rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
+        rd[r] = static_cast<float>(L.dx); rd[CH + r] = static_cast<float>(L.dy);
+        rd[2*CH + r] = static_cast<float>(L.dz);
         rg[r] = fac*gth[0]; rg[CH + r] = fac*gth[1]; rg[2*CH + r] = fac*gth[2];
         rcode[r] = sj + 1;
       }
@@ -1208,13 +1213,14 @@ k_p2(KArgs A, const float *us, float *scr) {
       const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
       for (int qq = s0; qq < s1; ++qq) {
         const int r = qq - c0;
-        const float wz = rw[(6 + slab)*CH + r];
-        const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
-        const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+        float wx[3], wy[3];
+        W3<S, float>(rd[r], wx);
+        W3<S, float>(rd[CH + r], wy);
+        const float wz = W1<S, float>(slab, rd[2*CH + r]);
         const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
         AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
       }
-      DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
+      DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
     if (part == 0) FlushRegion<float, 3>(Tl, t, dep, scr);
     else FlushOverflow<3>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN);
@@ -1245,10 +1251,10 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
     const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
     float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
     float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
-    float *rw = us_ + 3*Tl.RN;                             // [9][CH] weights
-    float *rv = rw + 9*CH;                                 // [3][CH] -m delta/V
+    float *rd = us_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
+    float *rv = rd + 3*CH;                                 // [3][CH] -m delta/V
     int *rcode = reinterpret_cast<int*>(rv + 3*CH);        // [CH] sj + 1
-    int *st = rcode + CH;                                  // [TC + 1]
+    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 7, 3));   // [TC + 1]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1309,9 +1315,8 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
           ex[d] += imp;
           rv[d*CH + r] = static_cast<float>(imp);
         }
-        rw[r] = wx[0]; rw[CH + r] = wx[1]; rw[2*CH + r] = wx[2];
-        rw[3*CH + r] = wy[0]; rw[4*CH + r] = wy[1]; rw[5*CH + r] = wy[2];
-        rw[6*CH + r] = wz[0]; rw[7*CH + r] = wz[1]; rw[8*CH + r] = wz[2];
+        rd[r] = static_cast<float>(L.dx); rd[CH + r] = static_cast<float>(L.dy);
+        rd[2*CH + r] = static_cast<float>(L.dz);
         rcode[r] = sj + 1;
       }
       __syncthreads();
@@ -1328,13 +1333,14 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
       const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
       for (int qq = s0; qq < s1; ++qq) {
         const int r = qq - c0;
-        const float wz = rw[(6 + slab)*CH + r];
-        const float wy[3] = {rw[3*CH + r], rw[4*CH + r], rw[5*CH + r]};
-        const float wx[3] = {rw[r], rw[CH + r], rw[2*CH + r]};
+        float wx[3], wy[3];
+        W3<S, float>(rd[r], wx);
+        W3<S, float>(rd[CH + r], wy);
+        const float wz = W1<S, float>(slab, rd[2*CH + r]);
         const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
         AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
       }
-      DepositSlabs<float, 3, SY>(Tl, dep, rw, l, slab, acc, spl, spdir);
+      DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
     // the flush, every thread summing (FP64) the words it writes; the defect: every thread's
     // impulses minus those words, summed over each wavefront, then over the wavefronts in order
@@ -1342,7 +1348,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
     double ws[3];
     if (part == 0) FlushRegion<float, 3>(Tl, t, dep, scr, ws);
     else FlushOverflow<3>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN, ws);
-    double *red = reinterpret_cast<double*>(rw);           // [NT/64][3]
+    double *red = reinterpret_cast<double*>(rd);           // [NT/64][3]
     const int wv = tid >> 6;
     for (int d=0; d<3; ++d) {
       const double a = WaveSum(ex[d] - ws[d], min(64, NT - (wv << 6)));
@@ -1753,12 +1759,10 @@ size_t LdsKey(const Tiles &T, int threads) {
 }
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1); }
 size_t LdsP1(const Tiles &T) {
-  const size_t ch = (size_t)kPPT*3*T.TC;
-  return ch*(13*sizeof(float) + sizeof(int)) + sizeof(float)*4*T.RN + LdsTail(T);
+  return sizeof(float)*(RecArea(T, kPPT*3*T.TC, 8, 4) + 4*T.RN) + LdsTail(T);
 }
 size_t LdsP2(const Tiles &T) {
-  const size_t ch = (size_t)kPPT*3*T.TC;
-  return sizeof(float)*6*T.RN + ch*(12*sizeof(float) + sizeof(int)) + LdsTail(T);
+  return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, 7, 3)) + LdsTail(T);
 }
 size_t LdsP3(const Tiles &T) { return LdsP2(T); }   // the same areas
 
@@ -2005,16 +2009,8 @@ int FastCreate(dk_ctx *c) {
   T.iRN = 1.0f/static_cast<float>(T.RN);
   // the u* region of P2/P3 in kRegionPer values per thread
   if (3*T.RN > kRegionPer*3*T.TC) return DK_EUNSUPPORTED;
-  // every tile kernel's LDS within 64 KiB, and the three slab copies of a region within the
-  // record area of its kernel
+  // every tile kernel's LDS within 64 KiB
   if (std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)) > 65536 - 1024) return DK_EUNSUPPORTED;
-  {
-    const size_t ch = (size_t)kPPT*3*T.TC;
-    if (3*4*T.RN*sizeof(float) > ch*13*sizeof(float) ||
-        3*3*T.RN*sizeof(float) > ch*12*sizeof(float) ||
-        3*3*T.RN*sizeof(float) > ch*12*sizeof(float))
-      return DK_EUNSUPPORTED;
-  }
   // a block's first and last x2 rows must differ (cells centred one row off in x2)
   if (T.SY && c->g.nx2 < 2) return DK_EUNSUPPORTED;
   T.ntx = c->g.nx1/T.TX; T.nty = c->g.nx2/T.TY; T.ntz = c->g.nx3/T.TZ;
