@@ -715,8 +715,10 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
 // particle that stays in its tile takes its rank from a count in LDS (the tile owns those
 // cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
 // only the pending correction is applied (written to v and R).
+// (five waves per SIMD: the register cap spills a few values, and the key pass still runs 3 %
+// faster than at the four waves its 109 registers would allow)
 template <bool OV>
-__global__ void __launch_bounds__(kKeyThreadsMax)
+__global__ void __launch_bounds__(kKeyThreadsMax) __attribute__((amdgpu_waves_per_eu(5)))
 k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1036,7 +1038,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
     float q3[3] = {0.0f, 0.0f, 0.0f};
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
-  #pragma unroll 1
+#pragma unroll 1
       for (int k=0; k<kPPT; ++k) {
         const int r = tid + k*NT;
         const int q = c0 + r;
@@ -1079,12 +1081,12 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
       }
       __syncthreads();
       float acc[3][3][4], spl[3][4];
-  #pragma unroll
+#pragma unroll
       for (int a=0; a<3; ++a)
-  #pragma unroll
+#pragma unroll
         for (int n=0; n<4; ++n) {
           spl[a][n] = 0.0f;
-  #pragma unroll
+#pragma unroll
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
@@ -1163,7 +1165,7 @@ k_p2(KArgs A, const float *us, float *scr) {
     const int l = tid - slab*Tl.TC;
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
-  #pragma unroll 1
+#pragma unroll 1
       for (int k=0; k<kPPT; ++k) {
         const int r = tid + k*NT;
         const int q = c0 + r;
@@ -1178,13 +1180,13 @@ k_p2(KArgs A, const float *us, float *scr) {
         W3<S, float>(static_cast<float>(L.dz), wz);
         const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
         float gth[3] = {0.0f, 0.0f, 0.0f};
-  #pragma unroll
+#pragma unroll
         for (int c=0; c<3; ++c)
-  #pragma unroll
+#pragma unroll
           for (int bb=0; bb<3; ++bb) {
             const float wcb = wz[c]*wy[bb];
             const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
-  #pragma unroll
+#pragma unroll
             for (int a=0; a<3; ++a) {
               const float w = wcb*wx[a];
               gth[0] += w*us_[rr + a];
@@ -1201,12 +1203,12 @@ k_p2(KArgs A, const float *us, float *scr) {
       }
       __syncthreads();
       float acc[3][3][3], spl[3][3];
-  #pragma unroll
+#pragma unroll
       for (int a=0; a<3; ++a)
-  #pragma unroll
+#pragma unroll
         for (int n=0; n<3; ++n) {
           spl[a][n] = 0.0f;
-  #pragma unroll
+#pragma unroll
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
@@ -1272,7 +1274,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
     double ex[3] = {0.0, 0.0, 0.0};
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
-  #pragma unroll 1
+#pragma unroll 1
       for (int k=0; k<kPPT; ++k) {
         const int r = tid + k*NT;
         const int q = c0 + r;
@@ -1288,13 +1290,13 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
         W3<S, float>(static_cast<float>(L.dz), wz);
         const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0 + Tl.SY + sj))*Tl.RX + (L.ci - t.ci0);
         float u[3] = {0.0f, 0.0f, 0.0f};
-  #pragma unroll
+#pragma unroll
         for (int c=0; c<3; ++c)
-  #pragma unroll
+#pragma unroll
           for (int bb=0; bb<3; ++bb) {
             const float wcb = wz[c]*wy[bb];
             const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
-  #pragma unroll
+#pragma unroll
             for (int a=0; a<3; ++a) {
               const float w = wcb*wx[a];
               u[0] += w*us_[rr + a];
@@ -1321,12 +1323,12 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
       }
       __syncthreads();
       float acc[3][3][3], spl[3][3];
-  #pragma unroll
+#pragma unroll
       for (int a=0; a<3; ++a)
-  #pragma unroll
+#pragma unroll
         for (int n=0; n<3; ++n) {
           spl[a][n] = 0.0f;
-  #pragma unroll
+#pragma unroll
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
