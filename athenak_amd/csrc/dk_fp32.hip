@@ -715,10 +715,8 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
 // particle that stays in its tile takes its rank from a count in LDS (the tile owns those
 // cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
 // only the pending correction is applied (written to v and R).
-// (five waves per SIMD: the register cap spills a few values, and the key pass still runs 3 %
-// faster than at the four waves its 109 registers would allow)
 template <bool OV>
-__global__ void __launch_bounds__(kKeyThreadsMax) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(kKeyThreadsMax)
 k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1528,6 +1526,19 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *
     const int k = g.ks + static_cast<int>(rr_ % g.nx3); \
     const int m = static_cast<int>(rr_/g.nx3);
 
+// the same over a 2-D grid (CellGrid: x over the active cells of a block, y over the blocks), with
+// 32-bit index arithmetic
+#define BLOCK_CELLS(m, k, j, i) \
+  const int m = blockIdx.y; \
+  const unsigned ncell_ = static_cast<unsigned>(g.nx1)*g.nx2*g.nx3; \
+  for (unsigned t_ = blockIdx.x*blockDim.x + threadIdx.x; t_ < ncell_; \
+       t_ += gridDim.x*blockDim.x) { \
+    const unsigned q_ = t_/static_cast<unsigned>(g.nx1); \
+    const unsigned k_ = q_/static_cast<unsigned>(g.nx2); \
+    const int i = g.is + static_cast<int>(t_ - q_*g.nx1); \
+    const int j = g.js + static_cast<int>(q_ - k_*g.nx2); \
+    const int k = g.ks + static_cast<int>(k_);
+
 // stage start: the gas share of the previous split, u0 += D - (actual dust correction),
 // R_g = that/a (dust_pm.cpp:670-685); register copy u1 <- u0 (stage 1, dust_push.cpp:65);
 // the harness's gas update (zero-flux RK combination + source terms); the drag history
@@ -1538,7 +1549,7 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
                              int pending, double inv_aprev, int stage1, int hook, double gam0,
                              double gam1, double bdt, int use_src, dk_gas_src src,
                              double atw_dt) {
-  ACTIVE_CELLS(t, m, k, j, i)
+  BLOCK_CELLS(m, k, j, i)
     double u[4];
     for (int v=0; v<4; ++v) u[v] = u0[Idx(g,m,DK_NGAS,v,k,j,i)];
     double rg[3] = {0.0, 0.0, 0.0};
@@ -1585,7 +1596,7 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
 
 // the local guess u* = (M + P)/(rho + Q) (dust_solver.cpp:643-649)
 __global__ void k_guess(Geo g, const double *u0, const float *qd, float *us) {
-  ACTIVE_CELLS(t, m, k, j, i)
+  BLOCK_CELLS(m, k, j, i)
     const double pinv = 1.0/(u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)] +
                              static_cast<double>(qd[Idx(g,m,4,0,k,j,i)]));
     for (int c=0; c<3; ++c)
@@ -1596,7 +1607,7 @@ __global__ void k_guess(Geo g, const double *u0, const float *qd, float *us) {
 
 // one defect-correction sweep x += (rho + Q)^-1 (M + P - (Ax)) (dust_solver.cpp:243-252)
 __global__ void k_dc(Geo g, const double *u0, const float *qd, const float *ap, float *us) {
-  ACTIVE_CELLS(t, m, k, j, i)
+  BLOCK_CELLS(m, k, j, i)
     const double rho = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
     const double pinv = 1.0/(rho + static_cast<double>(qd[Idx(g,m,4,0,k,j,i)]));
     for (int d=0; d<3; ++d) {
@@ -1616,7 +1627,7 @@ __global__ void k_dc(Geo g, const double *u0, const float *qd, const float *ap, 
 __global__ void k_split(Geo g, const double *u0, const float *us, const float *qd,
                         const float *qt0, const float *qt1, const float *qt2, const double *dmom,
                         float *res) {
-  ACTIVE_CELLS(t, m, k, j, i)
+  BLOCK_CELLS(m, k, j, i)
     const double rho = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
     const long long i1 = Idx(g,m,1,0,k,j,i);
     const double qt = static_cast<double>(qt0[i1]) + static_cast<double>(qt1[i1]) +
@@ -1822,6 +1833,12 @@ void LaunchP3(dk_ctx *c, const KArgs &A, const float *us, float *scr, double *de
   else SWITCH_SCHEME(k_p3n, (A, us, scr, defect));
 }
 
+// the cell passes: x over the active cells of a block, y over the blocks
+dim3 CellGrid(const dk_ctx *c) {
+  const long long b = ((long long)c->g.nx1*c->g.nx2*c->g.nx3 + kBlock - 1)/kBlock;
+  return dim3(static_cast<unsigned>(std::min<long long>(b, 65535)), c->g.nmb);
+}
+
 dim3 CombineGrid(const dk_ctx *c) {
   const long long b = (c->g.ncb + kBlock - 1)/kBlock;
   return dim3(static_cast<unsigned>(std::min<long long>(b, 4096)), c->g.nmb);
@@ -1965,7 +1982,7 @@ int Flush(dk_ctx *c) {
   dk_gas_src z;
   std::memset(&z, 0, sizeof(z));
   const long long nact = (long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3;
-  TIMED("cell_start", (k_cell_start<<<Grid(nact), kBlock, 0, c->stream>>>(
+  TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
       c->g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
       f->pres[f->pres_rd], 1,
       1.0/f->a_prev, 0, 0, 1.0, 0.0, 0.0, 0, z, 0.0)));
@@ -2194,7 +2211,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     dk_gas_src z;
     std::memset(&z, 0, sizeof(z));
     const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
-    TIMED("cell_start", (k_cell_start<<<Grid(nact), kBlock, 0, c->stream>>>(
+    TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
         f->pres[f->pres_rd], f->pending,
         f->pending ? 1.0/f->a_prev : 0.0, A.stage1, hook, gam0, gam1, bdt, src ? 1 : 0,
@@ -2216,7 +2233,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
       g, T, f->scr_f, f->qdep, nullptr, f->item_start, f->ov_scr, f->ov_def)));
   if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
   // 5. local guess + dc1/dc2 sweeps
-  TIMED("guess", (k_guess<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ustar)));
+  TIMED("guess", (k_guess<<<CellGrid(c), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ustar)));
   const int nsweeps = (c->P.solver == DK_DC2) ? 2 : ((c->P.solver == DK_DC1) ? 1 : 0);
   for (int sw=0; sw<nsweeps; ++sw) {
     if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
@@ -2224,7 +2241,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     TIMED("combine", (k_combine<float, 3><<<CombineGrid(c), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, f->ap, nullptr, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<float>(c, f->ap, 3, st->time))) return e;
-    TIMED("dc", (k_dc<<<Grid(nact), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ap, f->ustar)));
+    TIMED("dc", (k_dc<<<CellGrid(c), kBlock, 0, c->stream>>>(g, c->u0, f->qdep, f->ap, f->ustar)));
   }
   if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
   // 6. gather-kick + impulse deposit
@@ -2233,7 +2250,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
       g, T, f->scr_f, f->dmom, f->defect, f->item_start, f->ov_scr, f->ov_def)));
   if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;
   // 7. consistent split: the dust's residual per unit coupled mass
-  TIMED("split", (k_split<<<Grid(nact), kBlock, 0, c->stream>>>(
+  TIMED("split", (k_split<<<CellGrid(c), kBlock, 0, c->stream>>>(
       g, c->u0, f->ustar, f->qdep, f->qt[0], f->qt[1], f->qt[2], f->dmom, f->res)));
   f->pending = 1;
   f->regs_dead = !A.regs_live;

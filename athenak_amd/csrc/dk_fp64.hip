@@ -925,6 +925,12 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
       mesh->nmb != mesh->nmbx1*mesh->nmbx2*mesh->nmbx3)
     return DK_EINVAL;
   if (form != DK_FORM_FP64 && form != DK_FORM_FP32) return DK_EINVAL;
+  // the mesh passes launch one grid row per block (at most 65535), and index a block's padded
+  // cells in 32 bits
+  if (mesh->nmb > 65535 ||
+      (long long)(mesh->nx1 + 2*mesh->ng)*(mesh->nx2 + 2*mesh->ng)*(mesh->nx3 + 2*mesh->ng) >
+          INT32_MAX)
+    return DK_ECAPACITY;
   if (phys->solver < DK_LOCAL || phys->solver > DK_DC2) return DK_EUNSUPPORTED;
   if (phys->clouds && (!dk::CloudsSupported(mesh, phys) || form != DK_FORM_FP64))
     return DK_EUNSUPPORTED;

@@ -32,19 +32,60 @@ inline int Grid(long long n) {
   return static_cast<int>(std::min<long long>(b, 65536));
 }
 
+// x over n cells of one block, y over the blocks (at most 65535: checked at creation)
+inline dim3 BlockGrid(long long n, int nmb) {
+  const long long b = std::max<long long>(1, (n + kBlock - 1)/kBlock);
+  return dim3(static_cast<unsigned>(std::min<long long>(b, 65535)), static_cast<unsigned>(nmb));
+}
+
+#define BLOCK_STRIDE(t, n) \
+  for (unsigned t = blockIdx.x*blockDim.x + threadIdx.x; t < (n); t += gridDim.x*blockDim.x)
+
+// the padded cells outside the active box of a block: GhostCount of them, the x3 ghost layers
+// (whole planes), then the x2 ghost rows of the active layers, then the x1 ghost cells of the
+// active rows (32-bit index arithmetic)
+__host__ __device__ inline unsigned GhostCount(const Geo &g) {
+  return 2u*g.ng*g.n2*g.n1 + static_cast<unsigned>(g.nx3)*2u*g.ng*g.n1 +
+         static_cast<unsigned>(g.nx3)*g.nx2*2u*g.ng;
+}
+
+__device__ inline void GhostCell(const Geo &g, unsigned s, int &i, int &j, int &k) {
+  const unsigned a = 2u*g.ng*g.n2*g.n1, b = static_cast<unsigned>(g.nx3)*2u*g.ng*g.n1;
+  const unsigned ng2 = 2u*g.ng;
+  if (s < a) {
+    const unsigned plane = static_cast<unsigned>(g.n2)*g.n1;
+    const unsigned t = s/plane, r = s - t*plane;
+    const unsigned jj = r/g.n1;
+    k = (t < static_cast<unsigned>(g.ng)) ? static_cast<int>(t) : g.ke + 1 + static_cast<int>(t) - g.ng;
+    j = static_cast<int>(jj);
+    i = static_cast<int>(r - jj*g.n1);
+  } else if (s < a + b) {
+    const unsigned r0 = s - a, rows = ng2*g.n1;
+    const unsigned t = r0/rows, r = r0 - t*rows;
+    const unsigned jj = r/g.n1;
+    k = g.ks + static_cast<int>(t);
+    j = (jj < static_cast<unsigned>(g.ng)) ? static_cast<int>(jj) : g.je + 1 + static_cast<int>(jj) - g.ng;
+    i = static_cast<int>(r - jj*g.n1);
+  } else {
+    const unsigned r0 = s - a - b, row = static_cast<unsigned>(g.nx2)*ng2;
+    const unsigned t = r0/row, r = r0 - t*row;
+    const unsigned jj = r/ng2, ii = r - jj*ng2;
+    k = g.ks + static_cast<int>(t);
+    j = g.js + static_cast<int>(jj);
+    i = (ii < static_cast<unsigned>(g.ng)) ? static_cast<int>(ii) : g.ie + 1 + static_cast<int>(ii) - g.ng;
+  }
+}
+
 template <typename T>
 __global__ void k_copy_ghosts(Geo g, const DevBlock *B, T *a, int nvar) {
-  const long long n = g.ncb*g.nmb;
-  GRID_STRIDE(c, n) {
-    const int m = static_cast<int>(c/g.ncb);
-    long long rr = c - (long long)m*g.ncb;
-    const int i = static_cast<int>(rr % g.n1); rr /= g.n1;
-    const int j = static_cast<int>(rr % g.n2);
-    const int k = static_cast<int>(rr/g.n2);
+  const unsigned n = GhostCount(g);
+  const int m = blockIdx.y;
+  BLOCK_STRIDE(c, n) {
+    int i, j, k;
+    GhostCell(g, c, i, j, k);
     const int ox1 = (i < g.is) ? -1 : ((i > g.ie) ? 1 : 0);
     const int ox2 = (j < g.js) ? -1 : ((j > g.je) ? 1 : 0);
     const int ox3 = (k < g.ks) ? -1 : ((k > g.ke) ? 1 : 0);
-    if (ox1 == 0 && ox2 == 0 && ox3 == 0) continue;
     const int nb = B[m].nghbr[Dir(ox1, ox2, ox3)];
     if (nb < 0) continue;
     for (int v=0; v<nvar; ++v)
@@ -172,30 +213,31 @@ __host__ __device__ inline long long ShellCount(const Geo &g) {
          (long long)(g.nx3 - e3)*(g.nx2 - e2)*e1;
 }
 
-__device__ inline void ShellCell(const Geo &g, long long s, int &i, int &j, int &k) {
+__device__ inline void ShellCell(const Geo &g, unsigned s, int &i, int &j, int &k) {
   const int e1 = min(2*g.ng, g.nx1), e2 = min(2*g.ng, g.nx2), e3 = min(2*g.ng, g.nx3);
-  const long long a = (long long)e3*g.nx2*g.nx1;
-  const long long b = (long long)(g.nx3 - e3)*e2*g.nx1;
+  const unsigned plane = static_cast<unsigned>(g.nx2)*g.nx1;
+  const unsigned a = static_cast<unsigned>(e3)*plane;
+  const unsigned b = static_cast<unsigned>(g.nx3 - e3)*e2*g.nx1;
   if (s < a) {
-    const int t = static_cast<int>(s/((long long)g.nx2*g.nx1));
-    const long long r = s - (long long)t*g.nx2*g.nx1;
-    k = EdgeLayer(t, g.nx3, g.ng, e3);
-    j = static_cast<int>(r/g.nx1);
-    i = static_cast<int>(r - (long long)j*g.nx1);
+    const unsigned t = s/plane, r = s - t*plane;
+    const unsigned jj = r/g.nx1;
+    k = EdgeLayer(static_cast<int>(t), g.nx3, g.ng, e3);
+    j = static_cast<int>(jj);
+    i = static_cast<int>(r - jj*g.nx1);
   } else if (s < a + b) {
-    const long long r0 = s - a;
-    const int t = static_cast<int>(r0/((long long)e2*g.nx1));
-    const long long r = r0 - (long long)t*e2*g.nx1;
-    k = g.ng + t;
-    j = EdgeLayer(static_cast<int>(r/g.nx1), g.nx2, g.ng, e2);
-    i = static_cast<int>(r % g.nx1);
+    const unsigned r0 = s - a, rows = static_cast<unsigned>(e2)*g.nx1;
+    const unsigned t = r0/rows, r = r0 - t*rows;
+    const unsigned jj = r/g.nx1;
+    k = g.ng + static_cast<int>(t);
+    j = EdgeLayer(static_cast<int>(jj), g.nx2, g.ng, e2);
+    i = static_cast<int>(r - jj*g.nx1);
   } else {
-    const long long r0 = s - a - b;
-    const int t = static_cast<int>(r0/((long long)(g.nx2 - e2)*e1));
-    const long long r = r0 - (long long)t*(g.nx2 - e2)*e1;
-    k = g.ng + t;
-    j = g.ng + static_cast<int>(r/e1);
-    i = EdgeLayer(static_cast<int>(r % e1), g.nx1, g.ng, e1);
+    const unsigned r0 = s - a - b, row = static_cast<unsigned>(g.nx2 - e2)*e1;
+    const unsigned t = r0/row, r = r0 - t*row;
+    const unsigned jj = r/e1;
+    k = g.ng + static_cast<int>(t);
+    j = g.ng + static_cast<int>(jj);
+    i = EdgeLayer(static_cast<int>(r - jj*e1), g.nx1, g.ng, e1);
   }
   i += g.is; j += g.js; k += g.ks;
 }
@@ -210,13 +252,12 @@ __global__ void k_fold(Geo g, const DevBlock *B, const int *slot_dir, T *a, int 
   __shared__ int sdir[26];
   if (threadIdx.x < 26) sdir[threadIdx.x] = slot_dir[threadIdx.x];
   __syncthreads();
-  const long long nsh = ShellCount(g);
-  const long long n = nsh*g.nmb;
+  const unsigned nsh = static_cast<unsigned>(ShellCount(g));
   const bool shear = (g.bc_x1 == DK_X1_SHEAR);
-  GRID_STRIDE(t, n) {
-    const int m = static_cast<int>(t/nsh);
+  const int m = blockIdx.y;
+  BLOCK_STRIDE(t, nsh) {
     int i, j, k;
-    ShellCell(g, t - (long long)m*nsh, i, j, k);
+    ShellCell(g, t, i, j, k);
     const DevBlock &bm = B[m];
     // the allowed offsets per axis: bit (o + 1) set when offset o covers this cell
     const int a1 = 2 | ((i < g.is + g.ng) ? 1 : 0) | ((i > g.ie - g.ng) ? 4 : 0);
@@ -275,15 +316,19 @@ template <typename T>
 __global__ void k_fold_shear(Geo g, const DevBlock *B, T *a, int nvar, double shc) {
   const int nface = g.nmbx2*g.nmbx3;
   const int ny = g.nx2*g.nmbx2;
-  const long long n = 2LL*nface*g.nx3*g.nx2*g.ng;
+  const unsigned n = 2u*nface*g.nx3*g.nx2*g.ng;
   const bool perz = (g.bc_x3 == DK_X3_PERIODIC);
-  GRID_STRIDE(t, n) {
-    long long rr = t;
-    const int gg = static_cast<int>(rr % g.ng); rr /= g.ng;
-    const int jt = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
-    const int kt = g.ks + static_cast<int>(rr % g.nx3); rr /= g.nx3;
-    const int f = static_cast<int>(rr % nface);
-    const int tside = static_cast<int>(rr/nface);   // 0: target on the inner face
+  BLOCK_STRIDE(t, n) {
+    unsigned rr = t;
+    unsigned q = rr/g.ng;
+    const int gg = static_cast<int>(rr - q*g.ng); rr = q;
+    q = rr/g.nx2;
+    const int jt = g.js + static_cast<int>(rr - q*g.nx2); rr = q;
+    q = rr/g.nx3;
+    const int kt = g.ks + static_cast<int>(rr - q*g.nx3); rr = q;
+    q = rr/nface;
+    const int f = static_cast<int>(rr - q*nface);
+    const int tside = static_cast<int>(q);   // 0: target on the inner face
     const int byt = f % g.nmbx2, bzt = f/g.nmbx2;
     const int mt = ((tside == 0) ? 0 : g.nmbx1 - 1) + g.nmbx1*(byt + g.nmbx2*bzt);
     const int it = (tside == 0) ? (g.is + gg) : (g.ie - gg);
@@ -332,7 +377,8 @@ template <typename T>
 int FillGhostsT(dk_ctx *c, T *a, int nvar, bool velocity, double t) {
   const Geo &g = c->g;
   {const int tk = ProfBegin(c, "copy_ghosts");
-   k_copy_ghosts<T><<<Grid(NCells(c)), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar);
+   k_copy_ghosts<T><<<BlockGrid(GhostCount(g), g.nmb), kBlock, 0, c->stream>>>(
+       g, c->d_blocks, a, nvar);
    ProfEnd(c, tk);}
   if (c->M.bc_x1 == DK_X1_SHEAR) {
     const double yshear = (c->P.qshear*c->P.omega0)*(c->M.x1max - c->M.x1min)*t;
@@ -356,7 +402,7 @@ int FoldT(dk_ctx *c, T *a, int nvar, double t) {
     k_fold_x3<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar);
   }
   {const int tk = ProfBegin(c, "fold");
-   k_fold<T><<<Grid(ShellCount(g)*g.nmb), kBlock, 0, c->stream>>>(
+   k_fold<T><<<BlockGrid(ShellCount(g), g.nmb), kBlock, 0, c->stream>>>(
        g, c->d_blocks, c->d_slot_dir, a, nvar);
    ProfEnd(c, tk);}
   if (c->M.bc_x1 == DK_X1_SHEAR) {
@@ -365,7 +411,7 @@ int FoldT(dk_ctx *c, T *a, int nvar, double t) {
     const double dy = ly/(g.nx2*g.nmbx2);
     const long long n = 2LL*g.nmbx2*g.nmbx3*g.nx3*g.nx2*g.ng;
     const int tk = ProfBegin(c, "fold_shear");
-    k_fold_shear<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar, ysh/dy);
+    k_fold_shear<T><<<BlockGrid(n, 1), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar, ysh/dy);
     ProfEnd(c, tk);
   }
   return Check(hipGetLastError());
