@@ -111,6 +111,12 @@ struct Fast {
   float *ov_scr = nullptr, *ov_q = nullptr, *ov_pr = nullptr;
   double *ov_def = nullptr, *ov_dc = nullptr;
   int *ov_cnt = nullptr;
+  // shearing box: the items of the tiles inside a block's x2 range ([0]) and of the tiles on its
+  // x2 faces ([1]), in item order, their counts (ilist_n), and the scan that splits them
+  int *ilist[2] = {nullptr, nullptr};
+  int *ilist_n = nullptr, *iflag = nullptr, *iflag_scan = nullptr;
+  void *lscan_tmp = nullptr;
+  size_t lscan_bytes = 0;
   void *scan_tmp = nullptr;
   size_t scan_bytes = 0;
   int32_t *itag = nullptr;          // caller's PTAG, PSP, PLEV, PTAGH by original index
@@ -192,6 +198,10 @@ struct KArgs {
   double *ov_dc;                    // [slot][9][TC] dust corrections by centre row
   float *ov_pr;                     // [slot][3][TC] push residues
   int *ov_cnt;                      // [slot][TC] stayers counted by the part, then the offsets
+  // the launch's items: ilist[0 .. *ilist_n) (all items when ilist is null), the first nfirst of
+  // them one workgroup each
+  const int *ilist, *ilist_n;
+  int nfirst;
 };
 
 // the cell (active index, unclamped) of position x along one axis, as PMWeights
@@ -400,6 +410,20 @@ __device__ inline int TileIndex(const Tiles &T) {
   const int b = blockIdx.x;
   if (!T.xcd || T.nt % 8 != 0) return b;
   return (b % 8)*(T.nt/8) + b/8;
+}
+
+// the same for a launch of n workgroups
+__device__ inline int DealIndex(const Tiles &T, int n) {
+  const int b = blockIdx.x;
+  if (!T.xcd || n % 8 != 0) return b;
+  return (b % 8)*(n/8) + b/8;
+}
+
+// a tile on an x2 face of its block (in the shearing box the only tiles whose records can sit
+// one row off in x2: the deposit centre beyond the face in the shifted frame)
+__host__ __device__ inline bool EdgeTile(const Tiles &T, int tile) {
+  const int ty = ((tile % T.ntb)/T.ntx) % T.nty;
+  return ty == 0 || ty == T.nty - 1;
 }
 
 __device__ inline TileInfo TileOf(const Tiles &T, int tile) {
@@ -721,11 +745,13 @@ k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
-  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
-  const int nitem = A.item_start[Tl.nt];
-  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
-       it += gridDim.x) {
+  // the workgroup's work items in the launch's list: one of the first nfirst (a launch of
+  // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
+  // extra parts of full tiles)
+  const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+       li < nlist; li += gridDim.x) {
+    const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
@@ -883,6 +909,27 @@ __global__ void k_item_tile(Tiles Tl, const int *item_start, int *item_tile) {
   }
 }
 
+// the class of every item slot (1: a tile on a block's x2 face; 0 beyond the last item)
+__global__ void k_item_flags(Tiles Tl, const int *item_start, const int *item_tile, long long n,
+                             int *flag) {
+  const int nitem = item_start[Tl.nt];
+  GRID_STRIDE(i, n) flag[i] = (i < nitem && EdgeTile(Tl, item_tile[i])) ? 1 : 0;
+}
+
+// the two item lists in item order (e: the exclusive scan of the flags over n + 1 slots)
+__global__ void k_item_lists(Tiles Tl, const int *item_start, const int *flag, const int *e,
+                             long long n, int *list0, int *list1, int *count) {
+  const int nitem = item_start[Tl.nt];
+  GRID_STRIDE(i, (long long)nitem) {
+    if (flag[i]) list1[e[i]] = static_cast<int>(i);
+    else list0[i - e[i]] = static_cast<int>(i);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    count[1] = e[n];
+    count[0] = nitem - e[n];
+  }
+}
+
 // the count of every cell: the particles that stay in its tile plus the arrivals
 __global__ void k_total(int *count, const int *stay, long long nkeys) {
   GRID_STRIDE(c, nkeys) count[c] += stay[c];
@@ -1010,11 +1057,13 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
-  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
-  const int nitem = A.item_start[Tl.nt];
-  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
-       it += gridDim.x) {
+  // the workgroup's work items in the launch's list: one of the first nfirst (a launch of
+  // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
+  // extra parts of full tiles)
+  const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+       li < nlist; li += gridDim.x) {
+    const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
@@ -1133,11 +1182,13 @@ k_p2(KArgs A, const float *us, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
-  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
-  const int nitem = A.item_start[Tl.nt];
-  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
-       it += gridDim.x) {
+  // the workgroup's work items in the launch's list: one of the first nfirst (a launch of
+  // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
+  // extra parts of full tiles)
+  const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+       li < nlist; li += gridDim.x) {
+    const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
@@ -1240,11 +1291,13 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  // the workgroup's work items: one of the first nt (a launch of nt workgroups, !OV), or every
-  // gridDim.x-th after those (OV: a smaller launch over the extra parts of full tiles)
-  const int nitem = A.item_start[Tl.nt];
-  for (int it = OV ? Tl.nt + static_cast<int>(blockIdx.x) : TileIndex(Tl); it < nitem;
-       it += gridDim.x) {
+  // the workgroup's work items in the launch's list: one of the first nfirst (a launch of
+  // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
+  // extra parts of full tiles)
+  const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+       li < nlist; li += gridDim.x) {
+    const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
@@ -1779,25 +1832,6 @@ size_t LdsP2(const Tiles &T) {
 }
 size_t LdsP3(const Tiles &T) { return LdsP2(T); }   // the same areas
 
-// a tile kernel instantiated for the deposit scheme and the frame-shift flag of the context,
-// launched with grid, blk, lds in scope
-#define SWITCH_SCHEME1(KERNEL, OV, grid, ARGS) \
-  switch (c->g.deposit*2 + c->fast->T.SY) { \
-    case 5: KERNEL<2, true, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 4: KERNEL<2, false, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 3: KERNEL<1, true, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 2: KERNEL<1, false, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 1: KERNEL<0, true, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    default: KERNEL<0, false, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-  }
-
-// a tile kernel over the work items: the first nt (a workgroup each), then the extra parts
-#define SWITCH_SCHEME(KERNEL, ARGS) \
-  do { \
-    SWITCH_SCHEME1(KERNEL, false, dim3(T.nt), ARGS) \
-    SWITCH_SCHEME1(KERNEL, true, OverGrid(c), ARGS) \
-  } while (0)
-
 // workgroups of the launch over the extra parts (they loop over the items; all exit at once when
 // no tile has more than one part)
 dim3 OverGrid(const dk_ctx *c) {
@@ -1811,26 +1845,70 @@ void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres) {
   k_key<true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres);
 }
 
-void LaunchP1(dk_ctx *c, const KArgs &A, float *qt0, float *qt1, float *qt2, float *scr) {
+// a deposit kernel of the scheme of the context with the x2-shift code SYV
+#define SWITCH_S(KERNEL, SYV, OV, grid, ARGS) \
+  switch (c->g.deposit) { \
+    case 2: KERNEL<2, SYV, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 1: KERNEL<1, SYV, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    default: KERNEL<0, SYV, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+  }
+
+// The tile kernels P1-P3 run over their items in classes: without the shearing box one class,
+// every item; in the shearing box the tiles inside a block's x2 range (their records never sit
+// one row off in x2: the plain code, fewer registers) and the tiles on its x2 faces (the SY code),
+// each over its own list.  Both codes give the same sums for a tile inside the range.
+int TileClasses(const dk_ctx *c) { return c->fast->T.SY ? 2 : 1; }
+
+KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
+  const Fast *f = c->fast;
+  const Tiles &T = f->T;
+  KArgs A = A0;
+  if (!T.SY) {
+    sy = false;
+    A.ilist = nullptr; A.ilist_n = nullptr; A.nfirst = T.nt;
+    return A;
+  }
+  const int nedge = (T.nt/T.ntb)*T.ntz*T.ntx*((T.nty > 1) ? 2 : 1);
+  sy = (cls == 1);
+  A.ilist = f->ilist[cls]; A.ilist_n = f->ilist_n + cls;
+  A.nfirst = sy ? nedge : T.nt - nedge;
+  return A;
+}
+
+#define LAUNCH_CLASSES(KERNEL, ARGS) \
+  for (int cls = 0; cls < TileClasses(c); ++cls) { \
+    bool sy; \
+    const KArgs A = ClassArgs(c, A0, cls, sy); \
+    if (A.nfirst == 0) continue; \
+    if (sy) { \
+      SWITCH_S(KERNEL, true, false, dim3(A.nfirst), ARGS) \
+      SWITCH_S(KERNEL, true, true, OverGrid(c), ARGS) \
+    } else { \
+      SWITCH_S(KERNEL, false, false, dim3(A.nfirst), ARGS) \
+      SWITCH_S(KERNEL, false, true, OverGrid(c), ARGS) \
+    } \
+  }
+
+void LaunchP1(dk_ctx *c, const KArgs &A0, float *qt0, float *qt1, float *qt2, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
   const size_t lds = LdsP1(T);
-  SWITCH_SCHEME(k_p1, (A, qt0, qt1, qt2, scr));
+  LAUNCH_CLASSES(k_p1, (A, qt0, qt1, qt2, scr));
 }
 
-void LaunchP2(dk_ctx *c, const KArgs &A, const float *us, float *scr) {
+void LaunchP2(dk_ctx *c, const KArgs &A0, const float *us, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
   const size_t lds = LdsP2(T);
-  SWITCH_SCHEME(k_p2, (A, us, scr));
+  LAUNCH_CLASSES(k_p2, (A, us, scr));
 }
 
-void LaunchP3(dk_ctx *c, const KArgs &A, const float *us, float *scr, double *defect) {
+void LaunchP3(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
   const size_t lds = LdsP3(T);
-  if (A.regs_live) SWITCH_SCHEME(k_p3, (A, us, scr, defect));
-  else SWITCH_SCHEME(k_p3n, (A, us, scr, defect));
+  if (A0.regs_live) {LAUNCH_CLASSES(k_p3, (A, us, scr, defect));}
+  else {LAUNCH_CLASSES(k_p3n, (A, us, scr, defect));}
 }
 
 // the cell passes: x over the active cells of a block, y over the blocks
@@ -1899,6 +1977,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.W = kPPT*3*f->T.TC;
   A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
   A.ov_pr = f->ov_pr; A.ov_cnt = f->ov_cnt;
+  A.ilist = nullptr; A.ilist_n = nullptr; A.nfirst = f->T.nt;
   return A;
 }
 
@@ -1935,6 +2014,17 @@ int BuildItems(dk_ctx *c) {
   if (e) return e;
   TIMED("items", (k_item_tile<<<Grid(T.nt), kBlock, 0, c->stream>>>(T, f->item_start,
                                                                      f->item_tile)));
+  if (T.SY) {
+    TIMED("items", (k_item_flags<<<Grid(f->item_cap + 1), kBlock, 0, c->stream>>>(
+        T, f->item_start, f->item_tile, f->item_cap + 1, f->iflag)));
+    TIMED("items", e = dk::Check(hipcub::DeviceScan::ExclusiveSum(
+        f->lscan_tmp, f->lscan_bytes, f->iflag, f->iflag_scan, (int)(f->item_cap + 1),
+        c->stream)));
+    if (e) return e;
+    TIMED("items", (k_item_lists<<<Grid(f->item_cap), kBlock, 0, c->stream>>>(
+        T, f->item_start, f->iflag, f->iflag_scan, f->item_cap, f->ilist[0], f->ilist[1],
+        f->ilist_n)));
+  }
   return dk::Check(hipGetLastError());
 }
 
@@ -1946,10 +2036,13 @@ int AllocItems(dk_ctx *c, long long n) {
   const long long ov = (n + W - 1)/W + 1;
   if (ov <= f->ov_cap && f->item_start) return DK_OK;
   for (void *p : {(void*)f->item_tile, (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr,
-                  (void*)f->ov_def, (void*)f->ov_dc, (void*)f->ov_cnt})
+                  (void*)f->ov_def, (void*)f->ov_dc, (void*)f->ov_cnt, (void*)f->ilist[0],
+                  (void*)f->ilist[1], (void*)f->iflag, (void*)f->iflag_scan, f->lscan_tmp})
     if (p) (void)hipFree(p);
   f->item_tile = nullptr; f->ov_scr = nullptr; f->ov_q = nullptr; f->ov_pr = nullptr;
   f->ov_def = nullptr; f->ov_dc = nullptr; f->ov_cnt = nullptr;
+  f->ilist[0] = nullptr; f->ilist[1] = nullptr; f->iflag = nullptr; f->iflag_scan = nullptr;
+  f->lscan_tmp = nullptr;
   if (f->nparts == nullptr) {
     if (Alloc((void**)&f->nparts, sizeof(int)*(T.nt + 1))) return DK_ENOMEM;
     if (Alloc((void**)&f->item_start, sizeof(int)*(T.nt + 1))) return DK_ENOMEM;
@@ -1968,6 +2061,19 @@ int AllocItems(dk_ctx *c, long long n) {
       Alloc((void**)&f->ov_dc, sizeof(double)*9*(size_t)T.TC*ov) ||
       Alloc((void**)&f->ov_cnt, sizeof(int)*(size_t)T.TC*ov))
     return DK_ENOMEM;
+  if (T.SY) {
+    if (f->ilist_n == nullptr && Alloc((void**)&f->ilist_n, sizeof(int)*2)) return DK_ENOMEM;
+    if (Alloc((void**)&f->ilist[0], sizeof(int)*f->item_cap) ||
+        Alloc((void**)&f->ilist[1], sizeof(int)*f->item_cap) ||
+        Alloc((void**)&f->iflag, sizeof(int)*(f->item_cap + 1)) ||
+        Alloc((void**)&f->iflag_scan, sizeof(int)*(f->item_cap + 1)))
+      return DK_ENOMEM;
+    f->lscan_bytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, f->lscan_bytes, f->iflag, f->iflag_scan,
+                                         (int)(f->item_cap + 1)) != hipSuccess)
+      return DK_EHIP;
+    if (Alloc(&f->lscan_tmp, std::max<size_t>(f->lscan_bytes, 16))) return DK_ENOMEM;
+  }
   return DK_OK;
 }
 
@@ -2075,7 +2181,8 @@ void FastDestroy(dk_ctx *c) {
     if (ev) {(void)hipEventDestroy(ev); ev = nullptr;}
   for (void *p : {(void*)f->nparts, (void*)f->item_start, (void*)f->item_tile, f->iscan_tmp,
                   (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr, (void*)f->ov_def,
-                  (void*)f->ov_dc, (void*)f->ov_cnt})
+                  (void*)f->ov_dc, (void*)f->ov_cnt, (void*)f->ilist[0], (void*)f->ilist[1],
+                  (void*)f->ilist_n, (void*)f->iflag, (void*)f->iflag_scan, f->lscan_tmp})
     if (p) (void)hipFree(p);
   for (int b=0; b<2; ++b) {
     Store &S = f->s[b];

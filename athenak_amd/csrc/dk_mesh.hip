@@ -341,12 +341,14 @@ __global__ void k_fold_shear(Geo g, const DevBlock *B, T *a, int nvar, double sh
     const int GKt = bzt*g.nx3 + (kt - g.ks);
     int kb[3], kl[3];
     const int nk = RowCopies(GKt, g.nx3, g.nmbx3, g.ng, g.ks, perz, kb, kl);
-    const int nd = (ny < 5) ? ny : 5;
+    // rows base + d: a source row r sends to floor(r + sh) and the row above, so with ny >= 5
+    // only d = 1 and d = 2 can reach the target (the other rows add nothing)
+    const int d0 = (ny < 5) ? 0 : 1, d1 = (ny < 5) ? ny : 3;
     const int base = GJt - static_cast<int>(floor(sh)) - 2;
     for (int v=0; v<nvar; ++v) {
       T sum = a[Idx(g,mt,nvar,v,kt,jt,it)];
       for (int q=0; q<nk; ++q) {
-        for (int d=0; d<nd; ++d) {
+        for (int d=d0; d<d1; ++d) {
           const int r = (((base + d) % ny) + ny) % ny;
           int jb[3], jl[3];
           const int nj = RowCopies(r, g.nx2, g.nmbx2, g.ng, g.js, true, jb, jl);
