@@ -1197,8 +1197,8 @@ k_p2(KArgs A, const float *us, float *scr) {
     float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
     float *rd = us_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
     float *rg = rd + 3*CH;                                 // [3][CH] (m c/V) G u*
-    int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1
-    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 7, 3));   // [TC + 1]
+    int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1 (SY only)
+    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, SY ? 7 : 6, 3));   // [TC + 1]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1248,7 +1248,7 @@ k_p2(KArgs A, const float *us, float *scr) {
         rd[r] = static_cast<float>(L.dx); rd[CH + r] = static_cast<float>(L.dy);
         rd[2*CH + r] = static_cast<float>(L.dz);
         rg[r] = fac*gth[0]; rg[CH + r] = fac*gth[1]; rg[2*CH + r] = fac*gth[2];
-        rcode[r] = sj + 1;
+        if constexpr (SY) rcode[r] = sj + 1;
       }
       __syncthreads();
       float acc[3][3][3], spl[3][3];
@@ -1269,7 +1269,7 @@ k_p2(KArgs A, const float *us, float *scr) {
         W3<S, float>(rd[CH + r], wy);
         const float wz = W1<S, float>(slab, rd[2*CH + r]);
         const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
-        AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+        AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
       }
       DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
@@ -1306,8 +1306,8 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
     float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
     float *rd = us_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
     float *rv = rd + 3*CH;                                 // [3][CH] -m delta/V
-    int *rcode = reinterpret_cast<int*>(rv + 3*CH);        // [CH] sj + 1
-    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 7, 3));   // [TC + 1]
+    int *rcode = reinterpret_cast<int*>(rv + 3*CH);        // [CH] sj + 1 (SY only)
+    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, SY ? 7 : 6, 3));   // [TC + 1]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1370,7 +1370,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
         }
         rd[r] = static_cast<float>(L.dx); rd[CH + r] = static_cast<float>(L.dy);
         rd[2*CH + r] = static_cast<float>(L.dz);
-        rcode[r] = sj + 1;
+        if constexpr (SY) rcode[r] = sj + 1;
       }
       __syncthreads();
       float acc[3][3][3], spl[3][3];
@@ -1391,7 +1391,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
         W3<S, float>(rd[CH + r], wy);
         const float wz = W1<S, float>(slab, rd[2*CH + r]);
         const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
-        AddSlab<float, 3, SY>(acc, spl, spdir, rcode[r] - 1, wz, wy, wx, val);
+        AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
       }
       DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
@@ -1420,13 +1420,13 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
 }
 
 template <int S, bool SY, bool OV>
-__global__ void __launch_bounds__(kCellThreads, 4)
+__global__ void __launch_bounds__(kCellThreads, (SY || OV) ? 4 : 8)
 k_p3(KArgs A, const float *us, float *scr, double *defect) {
   P3Body<S, SY, true, OV>(A, us, scr, defect);
 }
 
 template <int S, bool SY, bool OV>
-__global__ void __launch_bounds__(kCellThreads, 4)
+__global__ void __launch_bounds__(kCellThreads, (SY || OV) ? 4 : 8)
 k_p3n(KArgs A, const float *us, float *scr, double *defect) {
   P3Body<S, SY, false, OV>(A, us, scr, defect);
 }
@@ -1824,13 +1824,14 @@ size_t LdsKey(const Tiles &T, int threads) {
          sizeof(int)*(2*T.TC + 1);
 }
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1); }
-size_t LdsP1(const Tiles &T) {
+// (sy: the x2-shift code, whose P2/P3 records carry the shift in one more word)
+size_t LdsP1(const Tiles &T, bool = true) {
   return sizeof(float)*(RecArea(T, kPPT*3*T.TC, 8, 4) + 4*T.RN) + LdsTail(T);
 }
-size_t LdsP2(const Tiles &T) {
-  return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, 7, 3)) + LdsTail(T);
+size_t LdsP2(const Tiles &T, bool sy = true) {
+  return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, sy ? 7 : 6, 3)) + LdsTail(T);
 }
-size_t LdsP3(const Tiles &T) { return LdsP2(T); }   // the same areas
+size_t LdsP3(const Tiles &T, bool sy = true) { return LdsP2(T, sy); }   // the same areas
 
 // workgroups of the launch over the extra parts (they loop over the items; all exit at once when
 // no tile has more than one part)
@@ -1875,11 +1876,12 @@ KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
   return A;
 }
 
-#define LAUNCH_CLASSES(KERNEL, ARGS) \
+#define LAUNCH_CLASSES(KERNEL, LDS, ARGS) \
   for (int cls = 0; cls < TileClasses(c); ++cls) { \
     bool sy; \
     const KArgs A = ClassArgs(c, A0, cls, sy); \
     if (A.nfirst == 0) continue; \
+    const size_t lds = LDS(T, sy); \
     if (sy) { \
       SWITCH_S(KERNEL, true, false, dim3(A.nfirst), ARGS) \
       SWITCH_S(KERNEL, true, true, OverGrid(c), ARGS) \
@@ -1892,23 +1894,20 @@ KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
 void LaunchP1(dk_ctx *c, const KArgs &A0, float *qt0, float *qt1, float *qt2, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  const size_t lds = LdsP1(T);
-  LAUNCH_CLASSES(k_p1, (A, qt0, qt1, qt2, scr));
+  LAUNCH_CLASSES(k_p1, LdsP1, (A, qt0, qt1, qt2, scr));
 }
 
 void LaunchP2(dk_ctx *c, const KArgs &A0, const float *us, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  const size_t lds = LdsP2(T);
-  LAUNCH_CLASSES(k_p2, (A, us, scr));
+  LAUNCH_CLASSES(k_p2, LdsP2, (A, us, scr));
 }
 
 void LaunchP3(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  const size_t lds = LdsP3(T);
-  if (A0.regs_live) {LAUNCH_CLASSES(k_p3, (A, us, scr, defect));}
-  else {LAUNCH_CLASSES(k_p3n, (A, us, scr, defect));}
+  if (A0.regs_live) {LAUNCH_CLASSES(k_p3, LdsP3, (A, us, scr, defect));}
+  else {LAUNCH_CLASSES(k_p3n, LdsP3, (A, us, scr, defect));}
 }
 
 // the cell passes: x over the active cells of a block, y over the blocks
