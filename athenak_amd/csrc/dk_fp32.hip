@@ -90,6 +90,7 @@ struct Tiles {
   int ntx, nty, ntz, ntb, nt;
   float iRX, iRY, iRN;              // 1/RX, 1/RY, 1/RN for FDiv
   int xcd;                          // workgroups dealt to the 8 XCDs in runs of tiles (1) or not
+  int wrows;                        // every wavefront of a tile kernel holds whole x1-x2 layers
 };
 
 }  // namespace
@@ -537,7 +538,11 @@ __device__ inline void AddSlab(T (&acc)[3][3][NC], T (&spl)[3][NC], int &spdir, 
 // 9 phases, one (x1, x2) offset per phase, all slabs at once, no two threads of a slab touching
 // one word in a phase; the spill terms of cells centred one row off in x2 take 3 more phases.
 // The copies are then added into the region in slab order.  Every word's sum has a fixed order.
-// Starts and ends with a barrier.
+// A thread writes only the x3 row (its layer + its slab) of its slab's copy, so when every
+// wavefront holds whole x1-x2 layers of one slab (Tl.wrows) no two wavefronts touch one word in
+// the 9 phases: the phases then need no workgroup barrier, only program order within the
+// wavefront (its LDS operations complete in issue order; the empty asm keeps the compiler from
+// moving one phase's LDS accesses across the next).  Starts and ends with a barrier.
 template <typename T, int NC, bool SY>
 __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int slab,
                                     const T (&acc)[3][3][NC], const T (&spl)[3][NC], int spdir) {
@@ -548,6 +553,7 @@ __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int s
   const int lx = l & (Tl.TX - 1), ly = (l >> Tl.LX) & (Tl.TY - 1), lz = l >> (Tl.LX + Tl.LY);
   const int rbase = ((lz + slab)*Tl.RY + (ly + Tl.SY))*Tl.RX + lx;
   T *mine = cp + slab*NC*Tl.RN;
+  const bool wrows = Tl.wrows != 0;
 #pragma unroll
   for (int pb=0; pb<3; ++pb)
 #pragma unroll
@@ -555,8 +561,10 @@ __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int s
       const int rc = rbase + pb*Tl.RX + pa;
 #pragma unroll
       for (int n=0; n<NC; ++n) mine[n*Tl.RN + rc] += acc[pb][pa][n];
-      __syncthreads();
+      if (wrows) asm volatile("" ::: "memory");
+      else __syncthreads();
     }
+  if (wrows) __syncthreads();
   if constexpr (SY) {
     if (__syncthreads_or(spdir != 0)) {
 #pragma unroll
@@ -2124,6 +2132,9 @@ int FastCreate(dk_ctx *c) {
   T.TC = T.TX*T.TY*T.TZ;
   T.SY = (c->P.shearing_box) ? 1 : 0;
   T.xcd = 1;
+  // 3 TC threads, TC a power of 2 <= 128: a wavefront of 64 holds whole x1-x2 layers of one slab
+  // (or whole slabs) unless a layer has more than 64 cells
+  T.wrows = (T.TX*T.TY <= 64) ? 1 : 0;
   T.RX = T.TX + 2; T.RY = T.TY + 2 + 2*T.SY; T.RZ = T.TZ + 2;
   T.RN = T.RX*T.RY*T.RZ;
   T.LX = 0; while ((1 << T.LX) < T.TX) ++T.LX;
