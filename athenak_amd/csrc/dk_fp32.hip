@@ -782,6 +782,18 @@ k_key(KArgs A, int do_key, double *pres) {
       const int c1 = min(c0 + CH, P1);
       const int q = c0 + tid;
       if (q < c1) {
+        // the correction's residual rows (the slot's cell and the cells beside it in x2),
+        // fetched alongside the particle: the centre is one of them
+        int ci = 0, cj = 0, ck = 0;
+        float rrow[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        if (A.pending) {
+          CellIdx(Tl, t, SlotCell(st, Tl.TC, q), ci, cj, ck);
+          const long long i0 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+#pragma unroll
+          for (int r=0; r<3; ++r)
+#pragma unroll
+            for (int d=0; d<3; ++d) rrow[r][d] = A.res[i0 + (r - 1)*g.n1 + d*g.ncb];
+        }
         PState s;
         LoadP(A.src, q, s, !A.stage1, A.atw_dt != 0.0 || !do_key);
         s.gid = t.m;   // (the tile's particles are in its block: the store is tile-major)
@@ -791,18 +803,16 @@ k_key(KArgs A, int do_key, double *pres) {
         if (A.pending) {
           int ic_, jc, kc_;
           if (mass > 0.0f && CenterCell(g, b, s.x, ic_, jc, kc_)) {
-            int ci, cj, ck;
-            CellIdx(Tl, t, SlotCell(st, Tl.TC, q), ci, cj, ck);
             const int dj = jc - cj;   // the centre: this cell or one beside it in x2
             if (ic_ != ci || kc_ != ck || dj < -1 || dj > 1) {
               dk::RecordError(A.err, DK_EHALO, q);
             } else {
               const float apf = static_cast<float>(A.a_prev);
               const float cjf = apf/(A.src.TS(q) + apf);
-              const long long ic = Idx(g, t.m, 3, 0, g.ks + ck, g.js + jc, g.is + ci);
               for (int d=0; d<3; ++d) {
                 const float vo = s.v[d];
-                const float vn = vo + cjf*A.res[ic + d*g.ncb];
+                const float rd = (dj < 0) ? rrow[0][d] : ((dj > 0) ? rrow[2][d] : rrow[1][d]);
+                const float vn = vo + cjf*rd;
                 s.v[d] = vn;
                 // the recorded rate takes the actual change, so that the history terms of dust
                 // and gas stay opposite (dust_pm.cpp:694)
