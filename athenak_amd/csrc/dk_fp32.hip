@@ -86,7 +86,7 @@ struct Store {
 // a block's x2 face: the orbital-advection frame shift of the 3-D shearing box).
 struct Tiles {
   int TX, TY, TZ, TC, SY, RX, RY, RZ, RN;
-  int LX, LY;                       // log2 TX, log2 TY (the tile sides are powers of 2)
+  int LX, LY, LZ;                   // log2 TX, TY, TZ (the tile sides are powers of 2)
   int ntx, nty, ntz, ntb, nt;
   float iRX, iRY, iRN;              // 1/RX, 1/RY, 1/RN for FDiv
   int xcd;                          // workgroups dealt to the 8 XCDs in runs of tiles (1) or not
@@ -1454,6 +1454,7 @@ k_p3n(KArgs A, const float *us, float *scr, double *defect) {
 __global__ void k_fix_qt(KArgs A, float *qt0, float *qt1, float *qt2) {
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
+  if (A.item_start[Tl.nt] == Tl.nt) return;   // every tile is one part
   GRID_STRIDE(u, (long long)Tl.nt*Tl.TC) {
     const int tile = static_cast<int>(u/Tl.TC), l = static_cast<int>(u - (long long)tile*Tl.TC);
     const int i0 = A.item_start[tile], np = A.item_start[tile + 1] - i0;
@@ -1476,6 +1477,7 @@ __global__ void k_fix_qt(KArgs A, float *qt0, float *qt1, float *qt2) {
 __global__ void k_fix_key(KArgs A, int do_key, double *pres) {
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
+  if (A.item_start[Tl.nt] == Tl.nt) return;   // every tile is one part
   GRID_STRIDE(u, (long long)Tl.nt*Tl.TC) {
     const int tile = static_cast<int>(u/Tl.TC), l = static_cast<int>(u - (long long)tile*Tl.TC);
     const int i0 = A.item_start[tile], np = A.item_start[tile + 1] - i0;
@@ -1526,21 +1528,26 @@ template <typename T, int NV, typename TO = T>
 __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *defect,
                           const int *item_start, const float *ov_scr, const double *ov_def) {
   const int m = blockIdx.y;
-  const int ncb = static_cast<int>(g.ncb);
-  for (int rr = blockIdx.x*blockDim.x + threadIdx.x; rr < ncb; rr += gridDim.x*blockDim.x) {
-    const int i = rr % g.n1;
-    const int t2 = rr/g.n1;
-    const int j = t2 % g.n2;
-    const int k = t2/g.n2;
+  const unsigned ncb = static_cast<unsigned>(g.ncb);
+  // (any tile of more than one part: only then are the overflow slots read)
+  const bool multi = item_start[Tl.nt] > Tl.nt;
+  for (unsigned rr = blockIdx.x*blockDim.x + threadIdx.x; rr < ncb;
+       rr += gridDim.x*blockDim.x) {
+    const unsigned t2 = rr/static_cast<unsigned>(g.n1);
+    const unsigned k_ = t2/static_cast<unsigned>(g.n2);
+    const int i = static_cast<int>(rr - t2*g.n1);
+    const int j = static_cast<int>(t2 - k_*g.n2);
+    const int k = static_cast<int>(k_);
     // region coordinates relative to tile 0: r = (i - is + 1) - tx*TX etc.
     const int ai = i - g.is + 1, aj = j - g.js + 1 + Tl.SY, ak = k - g.ks + 1;
     TO sum[NV];
 #pragma unroll
     for (int v=0; v<NV; ++v) sum[v] = TO(0);
     if (ai >= 0 && aj >= 0 && ak >= 0) {
-      const int tz0 = max(0, (ak - Tl.RZ + Tl.TZ)/Tl.TZ), tz1 = min(Tl.ntz - 1, ak/Tl.TZ);
-      const int ty0 = max(0, (aj - Tl.RY + Tl.TY)/Tl.TY), ty1 = min(Tl.nty - 1, aj/Tl.TY);
-      const int tx0 = max(0, (ai - Tl.RX + Tl.TX)/Tl.TX), tx1 = min(Tl.ntx - 1, ai/Tl.TX);
+      // (arithmetic shifts: a negative lower bound clamps to 0 either way)
+      const int tz0 = max(0, (ak - Tl.RZ + Tl.TZ) >> Tl.LZ), tz1 = min(Tl.ntz - 1, ak >> Tl.LZ);
+      const int ty0 = max(0, (aj - Tl.RY + Tl.TY) >> Tl.LY), ty1 = min(Tl.nty - 1, aj >> Tl.LY);
+      const int tx0 = max(0, (ai - Tl.RX + Tl.TX) >> Tl.LX), tx1 = min(Tl.ntx - 1, ai >> Tl.LX);
       for (int tz=tz0; tz<=tz1; ++tz) {
         const int rz = ak - tz*Tl.TZ;
         if (rz < 0 || rz >= Tl.RZ) continue;
@@ -1554,6 +1561,7 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *
             const long long vstride = (long long)Tl.RZ*Tl.RY*Tl.ntx*Tl.RX;
 #pragma unroll
             for (int v=0; v<NV; ++v) sum[v] += static_cast<TO>(scr[base + v*vstride]);
+            if (!multi) continue;
             const int tile = (m*Tl.ntz + tz)*Tl.nty*Tl.ntx + ty*Tl.ntx + tx;
             const int i0 = item_start[tile], np = item_start[tile + 1] - i0;
             for (int p = 1; p < np; ++p) {
@@ -1569,12 +1577,12 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *
       const int li = ai - 1, lj = aj - 1 - Tl.SY, lk = ak - 1;
       if (defect != nullptr && li < Tl.ntx*Tl.TX && lj >= 0 && lj < Tl.nty*Tl.TY &&
           lk < Tl.ntz*Tl.TZ && (li & (Tl.TX - 1)) == 0 && (lj & (Tl.TY - 1)) == 0 &&
-          lk % Tl.TZ == 0 && li >= 0 && lk >= 0) {
+          (lk & (Tl.TZ - 1)) == 0 && li >= 0 && lk >= 0) {
         const long long ct = (long long)m*Tl.ntb +
-                             ((lk/Tl.TZ)*Tl.nty + (lj >> Tl.LY))*Tl.ntx + (li >> Tl.LX);
+                             ((lk >> Tl.LZ)*Tl.nty + (lj >> Tl.LY))*Tl.ntx + (li >> Tl.LX);
 #pragma unroll
         for (int v=0; v<NV; ++v) sum[v] += defect[NV*ct + v];
-        const int i0 = item_start[ct], np = item_start[ct + 1] - i0;
+        const int i0 = item_start[ct], np = multi ? item_start[ct + 1] - i0 : 1;
         for (int p = 1; p < np; ++p)
 #pragma unroll
           for (int v=0; v<NV; ++v) sum[v] += ov_def[NV*((long long)i0 - ct + p - 1) + v];
@@ -2149,6 +2157,7 @@ int FastCreate(dk_ctx *c) {
   T.RN = T.RX*T.RY*T.RZ;
   T.LX = 0; while ((1 << T.LX) < T.TX) ++T.LX;
   T.LY = 0; while ((1 << T.LY) < T.TY) ++T.LY;
+  T.LZ = 0; while ((1 << T.LZ) < T.TZ) ++T.LZ;
   T.iRX = 1.0f/static_cast<float>(T.RX);
   T.iRY = 1.0f/static_cast<float>(T.RY);
   T.iRN = 1.0f/static_cast<float>(T.RN);
