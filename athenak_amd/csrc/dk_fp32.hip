@@ -795,7 +795,13 @@ k_key(KArgs A, int do_key, double *pres) {
             for (int d=0; d<3; ++d) rrow[r][d] = A.res[i0 + (r - 1)*g.n1 + d*g.ncb];
         }
         PState s;
-        LoadP(A.src, q, s, !A.stage1, A.atw_dt != 0.0 || !do_key);
+        LoadP(A.src, q, s, false, A.atw_dt != 0.0 || !do_key);
+        if (do_key && !A.stage1) {
+          // the stage registers x1, v1 where the stage-1 key pass wrote them: the store before
+          // the last sort (the other store), at the slot the permutation gives
+          const int pq = A.perm[q];
+          for (int d=0; d<3; ++d) {s.x1[d] = A.dst.X1(d, pq); s.v1[d] = A.dst.V1(d, pq);}
+        }
         s.gid = t.m;   // (the tile's particles are in its block: the store is tile-major)
         const float mass = A.src.M(q);
         int code = 0;
@@ -1114,8 +1120,8 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
         const float ts = A.src.TS(p), mass = A.src.M(p);
         const int kp = A.key[p];
         for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d];}
-        if (A.regs_live)
-          for (int d=0; d<3; ++d) {A.dst.X1(d, q) = A.src.X1(d, p); A.dst.V1(d, q) = A.src.V1(d, p);}
+        // (x1, v1 stay in the source order: the next stage's key pass reads them there through
+        // this sort's permutation)
         A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = t.m; A.dst.ORIG(q) = A.src.ORIG(p);
         float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         float mu = 0.0f;
@@ -1770,16 +1776,18 @@ __global__ void k_permute_all(Store src, Store dst, const int *perm, long long n
 }
 
 // (regs_dead: the store keeps no registers; they are written as x1 = x, v1 = v, R = 0, the values
-// the next stage 1 gives them before any use)
-__global__ void k_export(Store S, long long np, const int32_t *itag, long long np0, double *r,
-                         int32_t *iv, long long cap, int regs_dead) {
+// the next stage 1 gives them before any use.  Otherwise x1, v1 are read where the stage-1 key
+// pass wrote them: the other store O, at the slot of the last sort's permutation)
+__global__ void k_export(Store S, Store O, const int *perm, long long np, const int32_t *itag,
+                         long long np0, double *r, int32_t *iv, long long cap, int regs_dead) {
   GRID_STRIDE(q, np) {
+    const int pq = regs_dead ? 0 : perm[q];
     for (int d=0; d<3; ++d) {
       const double x = S.X(d, q), v = static_cast<double>(S.V(d, q));
       r[(long long)(DK_IPX + 2*d)*cap + q] = x;
       r[(long long)(DK_IPVX + 2*d)*cap + q] = v;
-      r[(long long)(DK_IPX1 + 2*d)*cap + q] = regs_dead ? x : S.X1(d, q);
-      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = regs_dead ? v : static_cast<double>(S.V1(d, q));
+      r[(long long)(DK_IPX1 + 2*d)*cap + q] = regs_dead ? x : O.X1(d, pq);
+      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = regs_dead ? v : static_cast<double>(O.V1(d, pq));
       r[(long long)(DK_IPRX + d)*cap + q] = regs_dead ? 0.0 : S.R(d, q);
     }
     r[(long long)DK_IPTS*cap + q] = static_cast<double>(S.TS(q));
@@ -2285,9 +2293,9 @@ int FastExport(dk_ctx *c) {
   if (f == nullptr) return DK_ESTATE;
   int e;
   if ((e = Flush(c))) return e;
-  k_export<<<Grid(f->np), kBlock, 0, c->stream>>>(f->s[f->cur], f->np, f->itag, f->np,
-                                                   c->rdata, c->idata, (long long)c->cap,
-                                                   f->regs_dead);
+  k_export<<<Grid(f->np), kBlock, 0, c->stream>>>(f->s[f->cur], f->s[1 - f->cur], f->perm,
+                                                   f->np, f->itag, f->np, c->rdata, c->idata,
+                                                   (long long)c->cap, f->regs_dead);
   return dk::Check(hipGetLastError());
 }
 
