@@ -1122,7 +1122,9 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
         for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d];}
         // (x1, v1 stay in the source order: the next stage's key pass reads them there through
         // this sort's permutation)
-        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.GID(q) = t.m; A.dst.ORIG(q) = A.src.ORIG(p);
+        // (no block index: every particle of a tile is in the tile's block; an export reads it
+        // from the key)
+        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.ORIG(q) = A.src.ORIG(p);
         float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         float mu = 0.0f;
         int code = 1 << 2;   // no centre, sj = 0
@@ -1778,10 +1780,11 @@ __global__ void k_permute_all(Store src, Store dst, const int *perm, long long n
 // (regs_dead: the store keeps no registers; they are written as x1 = x, v1 = v, R = 0, the values
 // the next stage 1 gives them before any use.  Otherwise x1, v1 are read where the stage-1 key
 // pass wrote them: the other store O, at the slot of the last sort's permutation)
-__global__ void k_export(Store S, Store O, const int *perm, long long np, const int32_t *itag,
-                         long long np0, double *r, int32_t *iv, long long cap, int regs_dead) {
+__global__ void k_export(Store S, Store O, const int *perm, const int *key, Tiles Tl,
+                         long long np, const int32_t *itag, long long np0, double *r,
+                         int32_t *iv, long long cap, int regs_dead) {
   GRID_STRIDE(q, np) {
-    const int pq = regs_dead ? 0 : perm[q];
+    const int pq = perm[q];
     for (int d=0; d<3; ++d) {
       const double x = S.X(d, q), v = static_cast<double>(S.V(d, q));
       r[(long long)(DK_IPX + 2*d)*cap + q] = x;
@@ -1792,7 +1795,8 @@ __global__ void k_export(Store S, Store O, const int *perm, long long np, const 
     }
     r[(long long)DK_IPTS*cap + q] = static_cast<double>(S.TS(q));
     r[(long long)DK_IPM*cap + q] = static_cast<double>(S.M(q));
-    iv[(long long)DK_PGID*cap + q] = S.GID(q);
+    // the block: the store is tile-major by the keys of the last sort
+    iv[(long long)DK_PGID*cap + q] = key[pq]/Tl.TC/Tl.ntb;
     const int o = S.ORIG(q);
     for (int c=1; c<DK_NIDATA; ++c) iv[(long long)c*cap + q] = itag[(long long)(c - 1)*np0 + o];
   }
@@ -2294,8 +2298,9 @@ int FastExport(dk_ctx *c) {
   int e;
   if ((e = Flush(c))) return e;
   k_export<<<Grid(f->np), kBlock, 0, c->stream>>>(f->s[f->cur], f->s[1 - f->cur], f->perm,
-                                                   f->np, f->itag, f->np, c->rdata, c->idata,
-                                                   (long long)c->cap, f->regs_dead);
+                                                   f->key, f->T, f->np, f->itag, f->np,
+                                                   c->rdata, c->idata, (long long)c->cap,
+                                                   f->regs_dead);
   return dk::Check(hipGetLastError());
 }
 
