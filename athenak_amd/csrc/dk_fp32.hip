@@ -660,6 +660,12 @@ __host__ __device__ inline int RecArea(const Tiles &Tl, int ch, int nrec, int nc
   return (nrec*ch > 3*nc*Tl.RN) ? nrec*ch : 3*nc*Tl.RN;
 }
 
+// the key pass's LDS before its per-cell sums: records, cell offsets, stayer counts
+__host__ __device__ inline size_t LdsKeyBase(const Tiles &T, int threads) {
+  return (size_t)threads*(3*sizeof(double) + 3*sizeof(float) + sizeof(int)) +
+         sizeof(int)*(2*T.TC + 1);
+}
+
 // the overflow slot of part >= 1 of a tile
 __device__ inline long long OvSlot(const int *item_start, int tile, int part) {
   return (long long)item_start[tile] - tile + part - 1;
@@ -769,6 +775,14 @@ k_key(KArgs A, int do_key, double *pres) {
     int *rcode = reinterpret_cast<int*>(rpr + 3*CH);       // [CH] centre row + 2 (0: none)
     int *st = rcode + CH;                                  // [TC + 1]
     int *lcnt = st + Tl.TC + 1;                            // [TC]
+    // every cell's sums (the correction by centre row, the push residue) in LDS slots that only
+    // the cell's thread touches: they stay out of registers during the particle phase
+    double *cacc = reinterpret_cast<double*>(lds_d) + (LdsKeyBase(Tl, CH) + 7)/8;   // [9][TC]
+    float *pacc = reinterpret_cast<float*>(cacc + 9*Tl.TC);                   // [3][TC]
+    if (tid < Tl.TC) {
+      for (int k=0; k<9; ++k) cacc[k*Tl.TC + tid] = 0.0;
+      for (int k=0; k<3; ++k) pacc[k*Tl.TC + tid] = 0.0f;
+    }
     for (int i = tid; i <= Tl.TC; i += CH) st[i] = A.start[t.key0 + i];
     for (int i = tid; i < Tl.TC; i += CH) lcnt[i] = 0;
     __syncthreads();
@@ -776,8 +790,6 @@ k_key(KArgs A, int do_key, double *pres) {
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
     const DevBlock &b = A.B[t.m];
     const double ivol = b.ivol;
-    double dcs[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-    float prs[3] = {0.0f, 0.0f, 0.0f};
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
       const int q = c0 + tid;
@@ -868,6 +880,10 @@ k_key(KArgs A, int do_key, double *pres) {
       }
       __syncthreads();
       if (tid < Tl.TC) {
+        double dcs[3][3];
+        float prs[3];
+        for (int k=0; k<9; ++k) dcs[k/3][k%3] = cacc[k*Tl.TC + tid];
+        for (int k=0; k<3; ++k) prs[k] = pacc[k*Tl.TC + tid];
         const int s0 = max(st[tid], c0), s1 = min(st[tid + 1], c1);
         for (int qq = s0; qq < s1; ++qq) {
           const int r = qq - c0;
@@ -878,8 +894,16 @@ k_key(KArgs A, int do_key, double *pres) {
           else if (code == 3) {dcs[2][0] += d0; dcs[2][1] += d1; dcs[2][2] += d2;}
           prs[0] += rpr[r]; prs[1] += rpr[CH + r]; prs[2] += rpr[2*CH + r];
         }
+        for (int k=0; k<9; ++k) cacc[k*Tl.TC + tid] = dcs[k/3][k%3];
+        for (int k=0; k<3; ++k) pacc[k*Tl.TC + tid] = prs[k];
       }
       __syncthreads();
+    }
+    double dcs[3][3];
+    float prs[3];
+    if (tid < Tl.TC) {
+      for (int k=0; k<9; ++k) dcs[k/3][k%3] = cacc[k*Tl.TC + tid];
+      for (int k=0; k<3; ++k) prs[k] = pacc[k*Tl.TC + tid];
     }
     if (do_key && part == 0)
       for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
@@ -1858,8 +1882,7 @@ int FillV(dk_ctx *c, T *a, long long n, T v) {
 
 // dynamic LDS of the tile kernels (NT = 3 TC threads, CH = kPPT NT records)
 size_t LdsKey(const Tiles &T, int threads) {
-  return (size_t)threads*(3*sizeof(double) + 3*sizeof(float) + sizeof(int)) +
-         sizeof(int)*(2*T.TC + 1);
+  return (LdsKeyBase(T, threads) + 7)/8*8 + (size_t)T.TC*(9*sizeof(double) + 3*sizeof(float));
 }
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1); }
 // (sy: the x2-shift code, whose P2/P3 records carry the shift in one more word)
