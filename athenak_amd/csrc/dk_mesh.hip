@@ -247,6 +247,8 @@ __device__ inline void ShellCell(const Geo &g, unsigned s, int &i, int &j, int &
 // cell.  A strip of direction (ox1, ox2, ox3) covers the cell when each non-zero ox puts the cell
 // in the first (ox < 0) or last (ox > 0) ng layers of that axis.  The strips that cross a
 // shear-periodic x1 face are left to k_fold_shear.
+constexpr int kFoldVars = 5;
+
 template <typename T>
 __global__ void k_fold(Geo g, const DevBlock *B, const int *slot_dir, T *a, int nvar) {
   __shared__ int sdir[26];
@@ -263,24 +265,29 @@ __global__ void k_fold(Geo g, const DevBlock *B, const int *slot_dir, T *a, int 
     const int a1 = 2 | ((i < g.is + g.ng) ? 1 : 0) | ((i > g.ie - g.ng) ? 4 : 0);
     const int a2 = 2 | ((j < g.js + g.ng) ? 1 : 0) | ((j > g.je - g.ng) ? 4 : 0);
     const int a3 = 2 | ((k < g.ks + g.ng) ? 1 : 0) | ((k > g.ke - g.ng) ? 4 : 0);
-    for (int v=0; v<nvar; ++v) {
-      const long long it = Idx(g, m, nvar, v, k, j, i);
-      T sum = a[it];
-      for (int s=0; s<26; ++s) {
-        const int d = sdir[s];
-        const int o1 = d % 3, o2 = (d/3) % 3, o3 = d/9;
-        if (!((a1 >> o1) & 1) || !((a2 >> o2) & 1) || !((a3 >> o3) & 1)) continue;
-        const int nb = bm.nghbr[d];
-        if (nb < 0) continue;
-        if (shear && o1 != 1) {
-          const bool crosses = (o1 == 0) ? (bm.lx1 == 0) : (bm.lx1 == g.nmbx1 - 1);
-          if (crosses) continue;
-        }
-        sum += a[Idx(g, nb, nvar, v, k - (o3 - 1)*g.nx3, j - (o2 - 1)*g.nx2,
-                     i - (o1 - 1)*g.nx1)];
+    // every variable's sum in slot order (at most kFoldVars variables: the deposits fold 3-5)
+    T sum[kFoldVars];
+#pragma unroll
+    for (int v=0; v<kFoldVars; ++v) sum[v] = (v < nvar) ? a[Idx(g, m, nvar, v, k, j, i)] : T(0);
+    for (int s=0; s<26; ++s) {
+      const int d = sdir[s];
+      const int o1 = d % 3, o2 = (d/3) % 3, o3 = d/9;
+      if (!((a1 >> o1) & 1) || !((a2 >> o2) & 1) || !((a3 >> o3) & 1)) continue;
+      const int nb = bm.nghbr[d];
+      if (nb < 0) continue;
+      if (shear && o1 != 1) {
+        const bool crosses = (o1 == 0) ? (bm.lx1 == 0) : (bm.lx1 == g.nmbx1 - 1);
+        if (crosses) continue;
       }
-      a[it] = sum;
+      const long long src = Idx(g, nb, nvar, 0, k - (o3 - 1)*g.nx3, j - (o2 - 1)*g.nx2,
+                                i - (o1 - 1)*g.nx1);
+#pragma unroll
+      for (int v=0; v<kFoldVars; ++v)
+        if (v < nvar) sum[v] += a[src + v*g.ncb];
     }
+#pragma unroll
+    for (int v=0; v<kFoldVars; ++v)
+      if (v < nvar) a[Idx(g, m, nvar, v, k, j, i)] = sum[v];
   }
 }
 
@@ -399,6 +406,7 @@ int FillGhostsT(dk_ctx *c, T *a, int nvar, bool velocity, double t) {
 template <typename T>
 int FoldT(dk_ctx *c, T *a, int nvar, double t) {
   const Geo &g = c->g;
+  if (nvar < 1 || nvar > kFoldVars) return DK_EINVAL;
   if (c->M.bc_x3 != DK_X3_PERIODIC && !c->M.x3_dep_drop) {
     const long long n = (long long)g.nmb*nvar*g.ng*g.n2*g.n1;
     k_fold_x3<T><<<Grid(n), kBlock, 0, c->stream>>>(g, c->d_blocks, a, nvar);
