@@ -291,25 +291,28 @@ __global__ void k_fold(Geo g, const DevBlock *B, const int *slot_dir, T *a, int 
   }
 }
 
-// The cells of a padded block field that hold global row r (0 <= r < nx*nb) along one axis:
-// the active row of block r/nx and the ghost copies in the adjacent blocks (periodic wrap, or
-// none beyond a physical face).  Returns the count; blk/loc receive block row and padded index.
+// The cells of a padded block field that hold global row r (0 <= r < nx*nb) along one axis, in
+// fixed slots: [0] the active row of block r/nx, [1] the upper ghost copy in block r/nx - 1,
+// [2] the lower ghost copy in block r/nx + 1 (periodic wrap, or none beyond a physical face).
+// blk/loc receive block row and padded index; the bit s of the result is set when slot s holds a
+// copy (fixed slots, so that callers' loops keep the arrays in registers).
 __device__ inline int RowCopies(int r, int nx, int nb, int ng, int start, bool periodic,
                                 int blk[3], int loc[3]) {
-  int n = 0;
+  int mask = 1;
   const int b = r/nx, l = r - b*nx;
-  blk[n] = b; loc[n] = start + l; ++n;
+  blk[0] = b; loc[0] = start + l;
+  blk[1] = 0; loc[1] = 0; blk[2] = 0; loc[2] = 0;
   if (l < ng) {                 // the upper ghost rows of block b - 1
     int b2 = b - 1;
     if (b2 < 0) b2 = periodic ? b2 + nb : -1;
-    if (b2 >= 0) {blk[n] = b2; loc[n] = start + nx + l; ++n;}
+    if (b2 >= 0) {blk[1] = b2; loc[1] = start + nx + l; mask |= 2;}
   }
   if (l >= nx - ng) {           // the lower ghost rows of block b + 1
     int b2 = b + 1;
     if (b2 >= nb) b2 = periodic ? b2 - nb : -1;
-    if (b2 >= 0) {blk[n] = b2; loc[n] = start - (nx - l); ++n;}
+    if (b2 >= 0) {blk[2] = b2; loc[2] = start - (nx - l); mask |= 4;}
   }
-  return n;
+  return mask;
 }
 
 // The shear part as a gather: a ghost cell beyond the outer x1 face deposits into the inner
@@ -347,19 +350,26 @@ __global__ void k_fold_shear(Geo g, const DevBlock *B, T *a, int nvar, double sh
     const int GJt = byt*g.nx2 + (jt - g.js);
     const int GKt = bzt*g.nx3 + (kt - g.ks);
     int kb[3], kl[3];
-    const int nk = RowCopies(GKt, g.nx3, g.nmbx3, g.ng, g.ks, perz, kb, kl);
+    const int kmask = RowCopies(GKt, g.nx3, g.nmbx3, g.ng, g.ks, perz, kb, kl);
     // rows base + d: a source row r sends to floor(r + sh) and the row above, so with ny >= 5
     // only d = 1 and d = 2 can reach the target (the other rows add nothing)
     const int d0 = (ny < 5) ? 0 : 1, d1 = (ny < 5) ? ny : 3;
     const int base = GJt - static_cast<int>(floor(sh)) - 2;
     for (int v=0; v<nvar; ++v) {
       T sum = a[Idx(g,mt,nvar,v,kt,jt,it)];
-      for (int q=0; q<nk; ++q) {
-        for (int d=d0; d<d1; ++d) {
+      // (fixed trip counts with predicates: the copy arrays stay in registers)
+#pragma unroll
+      for (int q=0; q<3; ++q) {
+        if (!((kmask >> q) & 1)) continue;
+#pragma unroll
+        for (int d=0; d<5; ++d) {
+          if (d < d0 || d >= d1) continue;
           const int r = (((base + d) % ny) + ny) % ny;
           int jb[3], jl[3];
-          const int nj = RowCopies(r, g.nx2, g.nmbx2, g.ng, g.js, true, jb, jl);
-          for (int p=0; p<nj; ++p) {
+          const int jmask = RowCopies(r, g.nx2, g.nmbx2, g.ng, g.js, true, jb, jl);
+#pragma unroll
+          for (int p=0; p<3; ++p) {
+            if (!((jmask >> p) & 1)) continue;
             const int ms = bxs + g.nmbx1*(jb[p] + g.nmbx2*kb[q]);
             const int gj = jb[p]*g.nx2 + (jl[p] - g.js);    // the source's unwrapped row
             const double lo = gj + sh;
