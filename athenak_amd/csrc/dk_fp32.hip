@@ -2327,6 +2327,44 @@ int FastExport(dk_ctx *c) {
   return dk::Check(hipGetLastError());
 }
 
+// NewTimeStep over the store (dust_newdt.cpp:135-145, as the plain form's k_newdt): min over
+// particles of dx/max(|v|, 1e-30), the azimuthal velocity without the background shear in the
+// 3-D shearing box, every stopping time validated (dust_newdt.cpp:34-99); the block of slot q
+// from the last sort's key
+__global__ void k_fnewdt(const DevBlock *B, Store S, const int *perm, const int *key, Tiles Tl,
+                         long long np, double qo, double *partial, dk::ErrWord *err) {
+  __shared__ double sh[kBlock];
+  double dtmin = 3.4028234663852886e38;   // the largest float, the reference's start value
+  GRID_STRIDE(q, np) {
+    const int m = key[perm[q]]/Tl.TC/Tl.ntb;
+    const double ts = static_cast<double>(S.TS(q));
+    if (!isfinite(ts) || !(ts > 0.0)) dk::RecordError(err, DK_ESTOP, q);
+    const double vy = static_cast<double>(S.V(1, q)) - qo*S.X(0, q);
+    dtmin = fmin(B[m].dx1/fmax(fabs(static_cast<double>(S.V(0, q))), 1.0e-30), dtmin);
+    dtmin = fmin(B[m].dx2/fmax(fabs(vy), 1.0e-30), dtmin);
+    dtmin = fmin(B[m].dx3/fmax(fabs(static_cast<double>(S.V(2, q))), 1.0e-30), dtmin);
+  }
+  sh[threadIdx.x] = dtmin;
+  __syncthreads();
+  for (int s=kBlock/2; s>0; s>>=1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// the smallest particle time step over the store, after the pending correction (the velocities a
+// step boundary holds); partial receives nb per-workgroup minima
+int FastNewDt(dk_ctx *c, double qo, double *partial, int nb) {
+  Fast *f = c->fast;
+  if (f == nullptr || c->u0 == nullptr) return DK_ESTATE;
+  int e;
+  if ((e = Flush(c))) return e;
+  k_fnewdt<<<nb, kBlock, 0, c->stream>>>(c->d_blocks, f->s[f->cur], f->perm, f->key, f->T,
+                                          f->np, qo, partial, c->d_err);
+  return dk::Check(hipGetLastError());
+}
+
 int FastDiag(dk_ctx *c, double out[9]) {
   Fast *f = c->fast;
   if (f == nullptr || c->u0 == nullptr) return DK_ESTATE;

@@ -1365,8 +1365,10 @@ int dk_set_graphs(dk_ctx *c, int32_t enable) {
   return DK_OK;
 }
 
-// src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell)
+// src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell).  The FP32 form
+// keeps its store sorted by (block, tile, cell) every stage, so it has nothing to do here.
 int dk_sort_particles(dk_ctx *c) {
+  if (c != nullptr && c->form == DK_FORM_FP32) return c->fast ? DK_OK : DK_ESTATE;
   int e = Ready(c);
   if (e) return e;
   const long long n = c->np;
@@ -1414,12 +1416,17 @@ int dk_sort_particles(dk_ctx *c) {
 }
 
 int dk_new_time_step(dk_ctx *c, double dt_cfl, double *dt_out) {
-  int e = Ready(c);
-  if (e) return e;
   const int nb = 1024;
+  if (c == nullptr) return DK_EINVAL;
   const double qo = c->P.shearing_box ? c->P.qshear*c->P.omega0 : 0.0;
-  k_newdt<<<nb, kBlock, 0, c->stream>>>(c->d_blocks, c->rdata, c->idata, (long long)c->cap,
-                                        (long long)c->np, qo, c->d_partial, c->d_err);
+  int e;
+  if (c->form == DK_FORM_FP32) {
+    if ((e = dk::FastNewDt(c, qo, c->d_partial, nb))) return e;
+  } else {
+    if ((e = Ready(c))) return e;
+    k_newdt<<<nb, kBlock, 0, c->stream>>>(c->d_blocks, c->rdata, c->idata, (long long)c->cap,
+                                          (long long)c->np, qo, c->d_partial, c->d_err);
+  }
   std::vector<double> h(nb);
   if ((e = dk::Check(hipMemcpyAsync(h.data(), c->d_partial, sizeof(double)*nb,
                                     hipMemcpyDeviceToHost, c->stream))))

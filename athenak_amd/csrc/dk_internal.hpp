@@ -149,6 +149,7 @@ int FastBind(dk_ctx *c);
 int FastExport(dk_ctx *c);
 int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);
 int FastDiag(dk_ctx *c, double out[9]);
+int FastNewDt(dk_ctx *c, double qo, double *partial, int nb);
 
 // dk_fp64.hip pieces reused by the FP32 form
 int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);

@@ -195,9 +195,11 @@ int dk_apply_pmbr(dk_ctx *ctx, const dk_stage *st);
 /* DustGasDrag::AddDragHistoryGas, src/dust/dust_pm.cpp:721-742 */
 int dk_add_drag_history_gas(dk_ctx *ctx, const dk_stage *st);
 /* DustGasDrag::SortParticles (cell sort of point particles),
- * src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell), all columns */
+ * src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell), all columns.  The
+ * FP32 form keeps its particles sorted by (block, tile, cell) every stage: a no-op there. */
 int dk_sort_particles(dk_ctx *ctx);
-/* DustGasDrag::NewTimeStep, src/dust/dust_newdt.cpp:108-149: dt_out = dt_cfl min dx/|v| */
+/* DustGasDrag::NewTimeStep, src/dust/dust_newdt.cpp:108-149: dt_out = dt_cfl min dx/|v|
+ * (both forms; the FP32 form applies a pending correction first, from its FP32 velocities) */
 int dk_new_time_step(dk_ctx *ctx, double dt_cfl, double *dt_out);
 
 /* ---- whole stage ------------------------------------------------------------------- */

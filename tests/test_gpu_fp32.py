@@ -303,3 +303,28 @@ def test_fp32_autotuner_choices_are_bit_identical(tmp_path):
     g.close()
     for a, b in zip(tuned, plain):
         assert a.tobytes() == b.tobytes()
+
+
+def test_fp32_new_time_step_and_sort():
+    # NewTimeStep over the FP32 store (shearing box: the azimuthal velocity without the
+    # background shear) against the oracle from the same state, at the start and after steps:
+    # the FP32 velocities put it within 1e-5 relative; the cell sort is a no-op that succeeds
+    import ctypes
+    from oracle.pyoracle import lib as olib
+    w = workloads.c2(n=16, nbx=2, ppc=2)
+    w.setup.dust_noise, w.dt = 0.5, 1e-3
+    o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16, threads=4)
+    o.setup(w.setup)
+    g = fp32_ctx(w)
+    for steps in (0, 3):
+        for _ in range(steps):
+            o.step(w.dt, w.src)
+            g.step(w.dt, w.src)
+        out = ctypes.c_double(0.0)
+        olib().or_new_time_step(o.h, 0.5, ctypes.byref(out))
+        dt = g.new_time_step(0.5)
+        say("new time step after %d steps: %.9e (oracle %.9e)" % (steps, dt, out.value))
+        assert abs(dt - out.value) <= 1e-5 * out.value
+    g.sort()
+    g.check_device_error()
+    g.close()
