@@ -13,7 +13,7 @@ DK_NGAS = 4
 PGID, PTAG, PSP, PLEV, PTAGH = range(5)
 DK_NGP, DK_CIC, DK_TSC = 0, 1, 2
 DK_IMEX, DK_PC2 = 0, 1
-DK_LOCAL, DK_APPLYA, DK_DC1, DK_DC2 = 0, 1, 2, 3
+DK_LOCAL, DK_APPLYA, DK_DC1, DK_DC2, DK_PCG, DK_ADAPTIVE = 0, 1, 2, 3, 4, 5
 DK_X1_PERIODIC, DK_X1_SHEAR = 0, 1
 DK_X3_PERIODIC, DK_X3_REFLECT, DK_X3_OUTFLOW = 0, 1, 2
 DK_FORM_FP64, DK_FORM_FP32 = 0, 1
@@ -21,7 +21,8 @@ DK_MAXSPEC = 8
 
 ERRORS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory",
           4: "particle outside deposit halo", 5: "particle routing", 6: "stopping time <= 0",
-          7: "unsupported option", 8: "capacity", 9: "call order", 10: "non-finite value"}
+          7: "unsupported option", 8: "capacity", 9: "call order", 10: "non-finite value",
+          11: "drag solver failed"}
 
 
 class dk_block(C.Structure):

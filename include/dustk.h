@@ -48,7 +48,7 @@ enum { DK_NGP = 0, DK_CIC = 1, DK_TSC = 2 };
 /* src/dust/dust.hpp:74 */
 enum { DK_IMEX = 0, DK_PC2 = 1 };
 /* src/dust/dust.hpp:70 (pcg and adaptive are not provided: DK_EUNSUPPORTED) */
-enum { DK_LOCAL = 0, DK_APPLYA = 1, DK_DC1 = 2, DK_DC2 = 3 };
+enum { DK_LOCAL = 0, DK_APPLYA = 1, DK_DC1 = 2, DK_DC2 = 3, DK_PCG = 4, DK_ADAPTIVE = 5 };
 /* mesh boundaries handled by the drag path */
 enum { DK_X1_PERIODIC = 0, DK_X1_SHEAR = 1 };
 enum { DK_X3_PERIODIC = 0, DK_X3_REFLECT = 1, DK_X3_OUTFLOW = 2 };
@@ -75,7 +75,9 @@ enum {
   DK_EUNSUPPORTED = 7,  /* option not provided by this build */
   DK_ECAPACITY = 8,     /* a size beyond a fixed maximum */
   DK_ESTATE = 9,        /* call order violated (e.g. particles not bound) */
-  DK_ENOTFINITE = 10    /* a non-finite value met by a guarded kernel */
+  DK_ENOTFINITE = 10,   /* a non-finite value met by a guarded kernel */
+  DK_ESOLVER = 11       /* the strict drag solver lost positive definiteness or did not reach
+                           its tolerance (SolverFatal, src/dust/dust_solver.cpp:468-526) */
 };
 
 /* A MeshBlock: its box (RegionSize, src/mesh/mesh.hpp:25-29), logical location and level,

@@ -106,6 +106,15 @@ struct or_sim {
   std::vector<int32_t> pi;
   int64_t cap = 0, np = 0;
   std::vector<double> u0, u1, qdep, ustar, dmom, ap, resid;
+  // the strict solver's fields and parameters (src/dust/dust.cpp:467-474 defaults) and the last
+  // solve's figures: iterations, fast accept, true residual, error bound, target, state scale,
+  // max Q/rho
+  std::vector<double> sr, sp;
+  double drag_rtol = 1.0e-11, drag_atol = 1.0e-14;
+  int drag_iter_max = 200;
+  double adaptive_order_c = 0.25, adaptive_rtol_max = 1.0e-3, adaptive_tref = 1.0,
+         adaptive_state_floor = 1.0;
+  double solver_last[7] = {0, 0, -1, -1, -1, -1, -1};
   std::vector<double> dx1, dx2, dx3;   // per block (meshblock.cpp:113-118)
   int nthreads = 1;
   int err = 0;
@@ -785,6 +794,10 @@ or_sim *or_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics 
   s->dmom.assign(3*nc, 0.0);
   s->ap.assign(3*nc, 0.0);
   s->resid.assign(4*nc, 0.0);
+  if (phys->solver == DK_PCG || phys->solver == DK_ADAPTIVE) {
+    s->sr.assign(3*nc, 0.0);
+    s->sp.assign(3*nc, 0.0);
+  }
   s->dx1.resize(mesh->nmb); s->dx2.resize(mesh->nmb); s->dx3.resize(mesh->nmb);
   for (int m=0; m<mesh->nmb; ++m) {
     s->dx1[m] = (s->B[m].x1max - s->B[m].x1min)/static_cast<double>(s->nx1);
@@ -1460,6 +1473,194 @@ void ApplyCoupledOperatorImpl(or_sim *s, std::vector<double> &field, std::vector
         }
 }
 
+// ---- the strict solver (src/dust/dust_solver.cpp:268-625) -------------------------------
+extern "C++" {   // (templates, inside the C-linkage region)
+// Sums over the active cells, serial and in cell order (m, k, j, i), of a quantity per velocity
+// component; b = M + P is the right-hand side, P = diag(rho + Q) the row-sum preconditioner.
+
+double CellVolume(const or_sim *s, int m) { return s->dx1[m]*s->dx2[m]*s->dx3[m]; }
+double RhsB(const or_sim *s, int m, int d, int k, int j, int i) {
+  return s->u0[s->Idx(m,NG_VARS,DK_IM1+d,k,j,i)] + s->qdep[s->Idx(m,5,1+d,k,j,i)];
+}
+double PrecondDiag(const or_sim *s, int m, int k, int j, int i) {
+  return s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)] + s->qdep[s->Idx(m,5,0,k,j,i)];
+}
+
+template <class F>
+void CellSums3(const or_sim *s, F f, double out[3]) {
+  out[0] = out[1] = out[2] = 0.0;
+  for (int m=0; m<s->M.nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i)
+          for (int d=0; d<3; ++d) out[d] += f(m, d, k, j, i);
+}
+
+template <class F>
+void ForActive3(or_sim *s, F f) {
+  for (int m=0; m<s->M.nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i)
+          for (int d=0; d<3; ++d) f(m, d, k, j, i);
+}
+
+// GlobalDot: sum of vol a b per component
+void VolDot(const or_sim *s, const std::vector<double> &a, const std::vector<double> &b,
+            double out[3]) {
+  CellSums3(s, [&](int m, int d, int k, int j, int i) {
+    const int64_t id = s->Idx(m,3,d,k,j,i);
+    return CellVolume(s, m)*a[id]*b[id];
+  }, out);
+}
+
+// GlobalPreconditionedNorm: sum of vol r^2/(rho + Q) per component
+void PrecondNorm(const or_sim *s, const std::vector<double> &r, double out[3]) {
+  CellSums3(s, [&](int m, int d, int k, int j, int i) {
+    const double v = r[s->Idx(m,3,d,k,j,i)];
+    return CellVolume(s, m)/PrecondDiag(s, m, k, j, i)*v*v;
+  }, out);
+}
+
+// StrictPCG: three independent preconditioned CG recurrences sharing the operator passes;
+// returns the completed updates, or -1 (the reference's SolverFatal)
+int StrictPCG(or_sim *s, double a_dt, double sh, double t, bool residual_is_current) {
+  std::vector<double> &x = s->ustar, &r = s->sr, &p = s->sp, &ap = s->ap;
+  if (!residual_is_current) {
+    ApplyCoupledOperatorImpl(s, x, ap, a_dt, sh, false, t);
+    ForActive3(s, [&](int m, int d, int k, int j, int i) {
+      const int64_t id = s->Idx(m,3,d,k,j,i);
+      r[id] = RhsB(s, m, d, k, j, i) - ap[id];
+      p[id] = (1.0/PrecondDiag(s, m, k, j, i))*r[id];
+    });
+  } else {
+    ForActive3(s, [&](int m, int d, int k, int j, int i) {
+      const int64_t id = s->Idx(m,3,d,k,j,i);
+      p[id] = (1.0/PrecondDiag(s, m, k, j, i))*r[id];
+    });
+  }
+  double rz[3], bn2[3], target[3];
+  bool conv[3];
+  VolDot(s, r, p, rz);
+  CellSums3(s, [&](int m, int d, int k, int j, int i) {
+    const double b = RhsB(s, m, d, k, j, i);
+    return CellVolume(s, m)/PrecondDiag(s, m, k, j, i)*b*b;
+  }, bn2);
+  for (int d=0; d<3; ++d) {
+    target[d] = s->drag_atol + s->drag_rtol*std::sqrt(std::max(bn2[d], 0.0));
+    conv[d] = std::sqrt(std::max(rz[d], 0.0)) <= target[d];
+  }
+  for (int it=0; it<=s->drag_iter_max; ++it) {
+    if (conv[0] && conv[1] && conv[2]) {
+      // the recursive residual says converged: verify with the true residual
+      ApplyCoupledOperatorImpl(s, x, ap, a_dt, sh, false, t);
+      ForActive3(s, [&](int m, int d, int k, int j, int i) {
+        const int64_t id = s->Idx(m,3,d,k,j,i);
+        r[id] = RhsB(s, m, d, k, j, i) - ap[id];
+      });
+      double tn2[3];
+      PrecondNorm(s, r, tn2);
+      bool all = true;
+      s->solver_last[2] = 0.0;
+      for (int d=0; d<3; ++d) {
+        const double nrm = std::sqrt(std::max(tn2[d], 0.0));
+        s->solver_last[2] = std::max(s->solver_last[2], nrm);
+        conv[d] = nrm <= target[d];
+        all = all && conv[d];
+        rz[d] = tn2[d];
+      }
+      if (all) return it;
+      ForActive3(s, [&](int m, int d, int k, int j, int i) {
+        const int64_t id = s->Idx(m,3,d,k,j,i);
+        p[id] = (1.0/PrecondDiag(s, m, k, j, i))*r[id];
+      });
+    }
+    if (it == s->drag_iter_max) break;
+    ApplyCoupledOperatorImpl(s, p, ap, a_dt, sh, false, t);
+    double pap[3], alpha[3];
+    VolDot(s, p, ap, pap);
+    for (int d=0; d<3; ++d) {
+      if (!conv[d] && (!(pap[d] > 0.0) || !std::isfinite(pap[d]))) return -1;
+      alpha[d] = conv[d] ? 0.0 : rz[d]/pap[d];
+    }
+    ForActive3(s, [&](int m, int d, int k, int j, int i) {
+      const int64_t id = s->Idx(m,3,d,k,j,i);
+      x[id] += alpha[d]*p[id];
+      r[id] -= alpha[d]*ap[id];
+    });
+    double rzn[3], beta[3];
+    bool now[3];
+    PrecondNorm(s, r, rzn);
+    for (int d=0; d<3; ++d) {
+      now[d] = std::sqrt(std::max(rzn[d], 0.0)) <= target[d];
+      beta[d] = (conv[d] || now[d]) ? 0.0 : rzn[d]/rz[d];
+    }
+    ForActive3(s, [&](int m, int d, int k, int j, int i) {
+      const int64_t id = s->Idx(m,3,d,k,j,i);
+      const double pinv = 1.0/PrecondDiag(s, m, k, j, i);
+      p[id] = now[d] ? 0.0 : pinv*r[id] + beta[d]*p[id];
+    });
+    for (int d=0; d<3; ++d) {conv[d] = now[d]; rz[d] = rzn[d];}
+  }
+  return -1;
+}
+
+// AdaptiveErrorBound: the conservative bound response*residual on the error of the DC1 iterate,
+// and the acceptance target from the mass-weighted state scale
+double AdaptiveBound(or_sim *s, double a_dt, double outer_dt, double residual,
+                     double &state_scale, double &target) {
+  double eps = 0.0;
+  for (int m=0; m<s->M.nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i)
+          eps = std::fmax(eps, s->qdep[s->Idx(m,5,0,k,j,i)]/s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)]);
+  double cmax = 0.0;
+  for (int64_t p=0; p<s->np; ++p) cmax = std::fmax(cmax, a_dt/(s->R(DK_IPTS,p) + a_dt));
+  s->solver_last[6] = eps;
+  const std::vector<double> &x = s->ustar;
+  double gas = 0.0;
+  for (int m=0; m<s->M.nmb; ++m)
+    for (int k=s->ks; k<=s->ke; ++k)
+      for (int j=s->js; j<=s->je; ++j)
+        for (int i=s->is; i<=s->ie; ++i) {
+          const double u2 = x[s->Idx(m,3,0,k,j,i)]*x[s->Idx(m,3,0,k,j,i)] +
+                            x[s->Idx(m,3,1,k,j,i)]*x[s->Idx(m,3,1,k,j,i)] +
+                            x[s->Idx(m,3,2,k,j,i)]*x[s->Idx(m,3,2,k,j,i)];
+          gas += CellVolume(s, m)*s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)]*u2;
+        }
+  double part = 0.0;
+  for (int64_t p=0; p<s->np; ++p) {
+    const int m = s->I(DK_PGID,p);
+    Stencil w;
+    MakeStencil(s, m, s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p), w);
+    double gu[3] = {0.0, 0.0, 0.0};
+    for (int c=0; c<3; ++c)
+      for (int b=0; b<3; ++b)
+        for (int a=0; a<3; ++a) {
+          const double ww = w.wz[c]*w.wy[b]*w.wx[a];
+          for (int d=0; d<3; ++d)
+            gu[d] += ww*x[s->Idx(m,3,d,w.kp+c-1,w.jp+b-1,w.ip+a-1)];
+        }
+    const double cj = a_dt/(s->R(DK_IPTS,p) + a_dt);
+    double v2 = 0.0;
+    for (int d=0; d<3; ++d) {
+      const double v = s->R(DK_IPVX + 2*d,p) + cj*(gu[d] - s->R(DK_IPVX + 2*d,p));
+      v2 += v*v;
+    }
+    part += s->R(DK_IPM,p)*v2;
+  }
+  state_scale = std::sqrt(std::max(gas + part, 0.0));
+  const double theta = eps/(1.0 + eps);
+  const double response = (1.0 + eps)*std::sqrt((1.0 + eps)*theta*theta + cmax*theta);
+  const double rtol_step = std::min(s->adaptive_rtol_max,
+                                    s->adaptive_order_c*std::pow(outer_dt/s->adaptive_tref, 3));
+  target = s->drag_atol + rtol_step*std::max(state_scale, s->adaptive_state_floor);
+  return response*residual;
+}
+
+}  // extern "C++"
+
 }  // namespace
 
 // src/dust/dust_pm.cpp:372-442 and src/dust/dust_solver.cpp:630-724
@@ -1507,14 +1708,52 @@ int or_gas_implicit_solve(or_sim *s, const dk_stage *st) {
         }
   if (!coupled) return DK_OK;
   const double sh = ImplicitShearShift(s, st);
+  for (int q=0; q<7; ++q) s->solver_last[q] = (q < 2) ? 0.0 : -1.0;
   if (s->P.solver == DK_APPLYA) {
     ApplyCoupledOperatorImpl(s, s->ustar, s->ap, a_dt, sh, false, st->time);
+  } else if (s->P.solver == DK_PCG) {
+    const int it = StrictPCG(s, a_dt, sh, st->time, false);
+    if (it < 0) return DK_ESOLVER;
+    s->solver_last[0] = it;
   } else {
     const int nsweeps = (s->P.solver == DK_DC2) ? 2 : 1;
     for (int sw=0; sw<nsweeps; ++sw)
       ApplyCoupledOperatorImpl(s, s->ustar, s->ap, a_dt, sh, true, st->time);
+    if (s->P.solver == DK_ADAPTIVE) {
+      // the true residual of the DC1 iterate, its error bound, and PCG from it if too large
+      ApplyCoupledOperatorImpl(s, s->ustar, s->ap, a_dt, sh, false, st->time);
+      ForActive3(s, [&](int m, int d, int k, int j, int i) {
+        const int64_t id = s->Idx(m,3,d,k,j,i);
+        s->sr[id] = RhsB(s, m, d, k, j, i) - s->ap[id];
+      });
+      double n2[3];
+      PrecondNorm(s, s->sr, n2);
+      const double res = std::sqrt(std::max(n2[0] + n2[1] + n2[2], 0.0));
+      double scale, target;
+      const double bound = AdaptiveBound(s, a_dt, st->dt, res, scale, target);
+      s->solver_last[2] = res; s->solver_last[3] = bound; s->solver_last[4] = target;
+      s->solver_last[5] = scale;
+      if (bound <= target) {
+        s->solver_last[1] = 1.0;
+      } else {
+        const int it = StrictPCG(s, a_dt, sh, st->time, true);
+        if (it < 0) return DK_ESOLVER;
+        s->solver_last[0] = it;
+      }
+    }
   }
   return DK_OK;
+}
+
+void or_set_solver(or_sim *s, double rtol, double atol, int iter_max, double order_c,
+                   double rtol_max, double tref, double state_floor) {
+  s->drag_rtol = rtol; s->drag_atol = atol; s->drag_iter_max = iter_max;
+  s->adaptive_order_c = order_c; s->adaptive_rtol_max = rtol_max; s->adaptive_tref = tref;
+  s->adaptive_state_floor = state_floor;
+}
+
+void or_solver_stats(or_sim *s, double out[7]) {
+  for (int q=0; q<7; ++q) out[q] = s->solver_last[q];
 }
 
 // src/dust/dust_tasks.cpp:265-312

@@ -75,6 +75,13 @@ int or_cloud_cells(or_sim *s, int m, const double x[3], int64_t *cell, double *w
 int or_cloud_gather(or_sim *s, const double *vel, int m, const double x[3], double out[3]);
 
 int or_diag_sums(or_sim *s, double out[9]);
+/* the strict solver's parameters (<dust> drag_rtol, drag_atol, drag_iter_max,
+ * drag_adaptive_order_c, drag_adaptive_rtol_max, drag_adaptive_tref, drag_adaptive_state_floor)
+ * and the last implicit solve's iterations, fast accept (0/1), true residual, error bound,
+ * acceptance target, state scale and max Q/rho (-1 where the path did not form them) */
+void or_set_solver(or_sim *s, double rtol, double atol, int iter_max, double order_c,
+                   double rtol_max, double tref, double state_floor);
+void or_solver_stats(or_sim *s, double out[7]);
 int or_last_error(or_sim *s, int64_t *particle);
 
 /* imex2+ (gamma = 1+1/sqrt(2)) and rk2 tableaux of the reference driver */

@@ -56,6 +56,9 @@ def lib():
         D = C.POINTER(C.c_double)
         L.or_cloud_cells.argtypes = [P, C.c_int, D, C.POINTER(C.c_int64), D, D]
         L.or_cloud_gather.argtypes = [P, D, C.c_int, D, D]
+        L.or_set_solver.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_double,
+                                    C.c_double, C.c_double, C.c_double]
+        L.or_solver_stats.argtypes = [P, D]
         _LIB = L
     return _LIB
 
@@ -163,6 +166,18 @@ class Oracle:
         out = (C.c_double * 3)()
         assert lib().or_cloud_gather(self.h, vel.ctypes.data_as(C.POINTER(C.c_double)), m,
                                      xs, out) == 0
+        return np.array(out[:])
+
+    def set_solver(self, rtol=1e-11, atol=1e-14, iter_max=200, order_c=0.25, rtol_max=1e-3,
+                   tref=1.0, state_floor=1.0):
+        """The strict solver's parameters (the reference's <dust> defaults)."""
+        lib().or_set_solver(self.h, rtol, atol, iter_max, order_c, rtol_max, tref, state_floor)
+
+    def solver_stats(self):
+        """The last implicit solve: iterations, fast accept, true residual, error bound,
+        acceptance target, state scale, max Q/rho (-1 where not formed)."""
+        out = (C.c_double * 7)()
+        lib().or_solver_stats(self.h, out)
         return np.array(out[:])
 
     def apply_operator(self, a_dt, x, sh=0.0):
