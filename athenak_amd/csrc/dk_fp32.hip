@@ -2162,7 +2162,8 @@ namespace dk {
 
 int FastCreate(dk_ctx *c) {
   if (c->P.coupling != DK_IMEX) return DK_EUNSUPPORTED;          // pc2: the FP64 form
-  if (c->P.solver == DK_APPLYA) return DK_EUNSUPPORTED;
+  if (c->P.solver == DK_APPLYA || c->P.solver == DK_PCG || c->P.solver == DK_ADAPTIVE)
+    return DK_EUNSUPPORTED;                                          // the FP64 form
   if (c->g.ng < 2 || !c->P.back_reaction) return DK_EUNSUPPORTED;
   Fast *f = new Fast();
   c->fast = f;

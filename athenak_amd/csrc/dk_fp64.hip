@@ -861,19 +861,296 @@ int FillGhosts(dk_ctx *c, double *a, int nvar, bool velocity, double t) {
 
 int Fold(dk_ctx *c, double *a, int nvar, double t) { return dk::FoldT<double>(c, a, nvar, t); }
 
-int ApplyCoupledOperator(dk_ctx *c, double a_dt, double sh, bool correct, double t) {
+// ApplyCoupledOperator (dust_solver.cpp:218-263) on field into out: out = A field, or with
+// correct one defect-correction sweep of field
+int ApplyOperatorTo(dk_ctx *c, double *field, double *out, double a_dt, double sh, bool correct,
+                    double t) {
   int e;
-  if ((e = FillGhosts(c, c->ustar, 3, true, t))) return e;
-  if ((e = dk::Zero(c, c->ap, 3*dk::NCells(c)))) return e;
+  if ((e = FillGhosts(c, field, 3, true, t))) return e;
+  if ((e = dk::Zero(c, out, 3*dk::NCells(c)))) return e;
   {const int _pt = dk::ProfBegin(c, "applya_gtsg");
-  k_applya_gtsg<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->ustar, c->ap, a_dt, sh, c->d_err);
+  k_applya_gtsg<<<PGRID>>>(c->g, c->d_blocks, PARGS, field, out, a_dt, sh, c->d_err);
    dk::ProfEnd(c, _pt);}
-  if ((e = Fold(c, c->ap, 3, t))) return e;
+  if ((e = Fold(c, out, 3, t))) return e;
   {const int _pt = dk::ProfBegin(c, "applya_cell");
-  k_applya_cell<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, c->ustar, c->ap,
-                                           correct ? 1 : 0);
+  k_applya_cell<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, field, out, correct ? 1 : 0);
    dk::ProfEnd(c, _pt);}
   return Launched(c);
+}
+
+int ApplyCoupledOperator(dk_ctx *c, double a_dt, double sh, bool correct, double t) {
+  return ApplyOperatorTo(c, c->ustar, c->ap, a_dt, sh, correct, t);
+}
+
+// ---- the strict solver (dust_solver.cpp:268-625) ------------------------------------------
+// Per-workgroup partial sums over the active cells (three per workgroup, in a fixed tree order);
+// the host adds the kDiagGrid partials in order.  mode 0: vol a.b (GlobalDot), 1: vol a^2/(rho
+// + Q) (GlobalPreconditionedNorm), 2: the same of b = M + P (the right-hand side), 3: [0] max
+// Q/rho, [1] vol rho |a|^2 (the gas state scale)
+__global__ void k_solver_sums(Geo g, const DevBlock *B, const double *u0, const double *qdep,
+                              const double *a, const double *b, int mode, double *partial) {
+  __shared__ double sh[3][kBlock];
+  double acc[3] = {0.0, 0.0, 0.0};
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double vol = (B[m].dx1*B[m].dx2)*B[m].dx3;
+    const double rho = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+    const double pd = rho + qdep[Idx(g,m,5,0,k,j,i)];
+    if (mode == 3) {
+      acc[0] = fmax(acc[0], qdep[Idx(g,m,5,0,k,j,i)]/rho);
+      double u2 = 0.0;
+      for (int d=0; d<3; ++d) {const double v = a[Idx(g,m,3,d,k,j,i)]; u2 += v*v;}
+      acc[1] += vol*rho*u2;
+      continue;
+    }
+    for (int d=0; d<3; ++d) {
+      const long long id = Idx(g,m,3,d,k,j,i);
+      if (mode == 0) {
+        acc[d] += vol*a[id]*b[id];
+      } else {
+        const double v = (mode == 1) ? a[id] : u0[Idx(g,m,DK_NGAS,DK_IM1+d,k,j,i)] +
+                                                qdep[Idx(g,m,5,1+d,k,j,i)];
+        acc[d] += vol/pd*v*v;
+      }
+    }
+  }
+  for (int d=0; d<3; ++d) sh[d][threadIdx.x] = acc[d];
+  __syncthreads();
+  for (int s=kBlock/2; s>0; s>>=1) {
+    if (threadIdx.x < s)
+      for (int d=0; d<3; ++d)
+        sh[d][threadIdx.x] = (mode == 3 && d == 0) ? fmax(sh[d][threadIdx.x], sh[d][threadIdx.x + s])
+                                                   : sh[d][threadIdx.x] + sh[d][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partial[3*blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+#define PR(c, p) r[(long long)(c)*cap + (p)]
+#define PI(c, p) iv[(long long)(c)*cap + (p)]
+// the particle side of the adaptive bound: [0] max a_dt/(t_s + a_dt), [1] sum m |v + c (G x -
+// v)|^2 with the gather at the particle's position (dust_solver.cpp:548-591)
+__global__ void k_solver_particles(Geo g, const DevBlock *B, const double *r, const int32_t *iv,
+                                   long long cap, long long np, const double *x, double a_dt,
+                                   double *partial) {
+  __shared__ double sh[2][kBlock];
+  double acc[2] = {0.0, 0.0};
+  GRID_STRIDE(p, np) {
+    const double ts = PR(DK_IPTS,p);
+    acc[0] = fmax(acc[0], a_dt/(ts + a_dt));
+    const int m = PI(DK_PGID,p);
+    Stencil w;
+    MakeStencil(g, B[m], PR(DK_IPX,p), PR(DK_IPY,p), PR(DK_IPZ,p), w);
+    double gu[3] = {0.0, 0.0, 0.0};
+    if (InPad(g, w))
+      for (int cc=0; cc<3; ++cc)
+        for (int bb=0; bb<3; ++bb)
+          for (int a=0; a<3; ++a) {
+            const double ww = w.wz[cc]*w.wy[bb]*w.wx[a];
+            for (int d=0; d<3; ++d)
+              gu[d] += ww*x[Idx(g,m,3,d,w.kp+cc-1,w.jp+bb-1,w.ip+a-1)];
+          }
+    const double cj = a_dt/(ts + a_dt);
+    double v2 = 0.0;
+    for (int d=0; d<3; ++d) {
+      const double v = PR(DK_IPVX + 2*d,p) + cj*(gu[d] - PR(DK_IPVX + 2*d,p));
+      v2 += v*v;
+    }
+    acc[1] += PR(DK_IPM,p)*v2;
+  }
+  sh[0][threadIdx.x] = acc[0];
+  sh[1][threadIdx.x] = acc[1];
+  __syncthreads();
+  for (int s=kBlock/2; s>0; s>>=1) {
+    if (threadIdx.x < s) {
+      sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + s]);
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) partial[2*blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+#undef PR
+#undef PI
+
+// mode 0: r = b - ap, p = r/(rho + Q); 1: p = r/(rho + Q); 2: r = b - ap
+__global__ void k_pcg_residual(Geo g, const double *u0, const double *qdep, const double *ap,
+                               double *r, double *p, int mode) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double pinv = 1.0/(u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)] + qdep[Idx(g,m,5,0,k,j,i)]);
+    for (int d=0; d<3; ++d) {
+      const long long id = Idx(g,m,3,d,k,j,i);
+      if (mode != 1)
+        r[id] = u0[Idx(g,m,DK_NGAS,DK_IM1+d,k,j,i)] + qdep[Idx(g,m,5,1+d,k,j,i)] - ap[id];
+      if (mode != 2) p[id] = pinv*r[id];
+    }
+  }
+}
+
+// x += alpha p, r -= alpha ap (per component)
+__global__ void k_pcg_xr(Geo g, double *x, double *r, const double *p, const double *ap,
+                         double a0, double a1, double a2) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double al[3] = {a0, a1, a2};
+    for (int d=0; d<3; ++d) {
+      const long long id = Idx(g,m,3,d,k,j,i);
+      x[id] += al[d]*p[id];
+      r[id] -= al[d]*ap[id];
+    }
+  }
+}
+
+// p = r/(rho + Q) + beta p, or 0 for a converged component
+__global__ void k_pcg_p(Geo g, const double *u0, const double *qdep, const double *r, double *p,
+                        double b0, double b1, double b2, int done) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const double pinv = 1.0/(u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)] + qdep[Idx(g,m,5,0,k,j,i)]);
+    const double be[3] = {b0, b1, b2};
+    for (int d=0; d<3; ++d) {
+      const long long id = Idx(g,m,3,d,k,j,i);
+      p[id] = ((done >> d) & 1) ? 0.0 : pinv*r[id] + be[d]*p[id];
+    }
+  }
+}
+
+// the kDiagGrid partials of k_solver_sums summed on the host in order
+int SolverSums(dk_ctx *c, const double *a, const double *b, int mode, double out[3]) {
+  k_solver_sums<<<kDiagGrid, kBlock, 0, c->stream>>>(c->g, c->d_blocks, c->u0, c->qdep, a, b,
+                                                     mode, c->d_partial);
+  std::vector<double> h(3*kDiagGrid);
+  int e = dk::Check(hipMemcpyAsync(h.data(), c->d_partial, sizeof(double)*h.size(),
+                                   hipMemcpyDeviceToHost, c->stream));
+  if (!e) e = dk::Check(hipStreamSynchronize(c->stream));
+  if (e) return e;
+  out[0] = out[1] = out[2] = 0.0;
+  for (int q=0; q<kDiagGrid; ++q)
+    for (int d=0; d<3; ++d)
+      out[d] = (mode == 3 && d == 0) ? std::fmax(out[d], h[3*q + d]) : out[d] + h[3*q + d];
+  return DK_OK;
+}
+
+// StrictPCG (dust_solver.cpp:350-526): the completed updates in *iters, DK_ESOLVER for the
+// reference's SolverFatal
+int StrictPCG(dk_ctx *c, double a_dt, double sh, double t, bool residual_is_current,
+              int *iters) {
+  double *x = c->ustar, *r = c->solver_r, *p = c->solver_p, *ap = c->ap;
+  int e;
+  if (!residual_is_current) {
+    if ((e = ApplyOperatorTo(c, x, ap, a_dt, sh, false, t))) return e;
+    k_pcg_residual<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, ap, r, p, 0);
+  } else {
+    k_pcg_residual<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, ap, r, p, 1);
+  }
+  double rz[3], bn2[3], target[3];
+  bool conv[3];
+  if ((e = SolverSums(c, r, p, 0, rz))) return e;
+  if ((e = SolverSums(c, nullptr, nullptr, 2, bn2))) return e;
+  for (int d=0; d<3; ++d) {
+    target[d] = c->drag_atol + c->drag_rtol*std::sqrt(std::max(bn2[d], 0.0));
+    conv[d] = std::sqrt(std::max(rz[d], 0.0)) <= target[d];
+  }
+  for (int it=0; it<=c->drag_iter_max; ++it) {
+    if (conv[0] && conv[1] && conv[2]) {
+      if ((e = ApplyOperatorTo(c, x, ap, a_dt, sh, false, t))) return e;
+      k_pcg_residual<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, ap, r, p, 2);
+      double tn2[3];
+      if ((e = SolverSums(c, r, nullptr, 1, tn2))) return e;
+      bool all = true;
+      c->solver_last[2] = 0.0;
+      for (int d=0; d<3; ++d) {
+        const double nrm = std::sqrt(std::max(tn2[d], 0.0));
+        c->solver_last[2] = std::max(c->solver_last[2], nrm);
+        conv[d] = nrm <= target[d];
+        all = all && conv[d];
+        rz[d] = tn2[d];
+      }
+      if (all) {*iters = it; return DK_OK;}
+      k_pcg_residual<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, ap, r, p, 1);
+    }
+    if (it == c->drag_iter_max) break;
+    if ((e = ApplyOperatorTo(c, p, ap, a_dt, sh, false, t))) return e;
+    double pap[3], alpha[3];
+    if ((e = SolverSums(c, p, ap, 0, pap))) return e;
+    for (int d=0; d<3; ++d) {
+      if (!conv[d] && (!(pap[d] > 0.0) || !std::isfinite(pap[d]))) return DK_ESOLVER;
+      alpha[d] = conv[d] ? 0.0 : rz[d]/pap[d];
+    }
+    k_pcg_xr<<<CGRID(ActiveCells(c))>>>(c->g, x, r, p, ap, alpha[0], alpha[1], alpha[2]);
+    double rzn[3], beta[3];
+    int done = 0;
+    if ((e = SolverSums(c, r, nullptr, 1, rzn))) return e;
+    bool now[3];
+    for (int d=0; d<3; ++d) {
+      now[d] = std::sqrt(std::max(rzn[d], 0.0)) <= target[d];
+      beta[d] = (conv[d] || now[d]) ? 0.0 : rzn[d]/rz[d];
+      if (now[d]) done |= 1 << d;
+    }
+    k_pcg_p<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, r, p, beta[0], beta[1], beta[2],
+                                       done);
+    for (int d=0; d<3; ++d) {conv[d] = now[d]; rz[d] = rzn[d];}
+  }
+  return DK_ESOLVER;
+}
+
+// the adaptive path after its DC1 sweep (dust_solver.cpp:664-686, 529-624): the true residual,
+// the error bound and the acceptance target; PCG from the residual when the bound exceeds it
+int AdaptiveSolve(dk_ctx *c, double a_dt, double sh, double outer_dt, double t) {
+  int e;
+  if ((e = ApplyOperatorTo(c, c->ustar, c->ap, a_dt, sh, false, t))) return e;
+  k_pcg_residual<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, c->ap, c->solver_r,
+                                            c->solver_p, 2);
+  double n2[3];
+  if ((e = SolverSums(c, c->solver_r, nullptr, 1, n2))) return e;
+  const double res = std::sqrt(std::max(n2[0] + n2[1] + n2[2], 0.0));
+  double gas[3];
+  if ((e = SolverSums(c, c->ustar, nullptr, 3, gas))) return e;
+  const double eps = gas[0];
+  k_solver_particles<<<kDiagGrid, kBlock, 0, c->stream>>>(c->g, c->d_blocks, PARGS, c->ustar,
+                                                          a_dt, c->d_partial);
+  std::vector<double> h(2*kDiagGrid);
+  if ((e = dk::Check(hipMemcpyAsync(h.data(), c->d_partial, sizeof(double)*h.size(),
+                                    hipMemcpyDeviceToHost, c->stream))))
+    return e;
+  if ((e = dk::Check(hipStreamSynchronize(c->stream)))) return e;
+  double cmax = 0.0, part = 0.0;
+  for (int q=0; q<kDiagGrid; ++q) {cmax = std::fmax(cmax, h[2*q]); part += h[2*q + 1];}
+  const double scale = std::sqrt(std::max(gas[1] + part, 0.0));
+  const double theta = eps/(1.0 + eps);
+  const double response = (1.0 + eps)*std::sqrt((1.0 + eps)*theta*theta + cmax*theta);
+  const double rtol_step = std::min(c->adaptive_rtol_max,
+                                    c->adaptive_order_c*std::pow(outer_dt/c->adaptive_tref, 3));
+  const double target = c->drag_atol + rtol_step*std::max(scale, c->adaptive_state_floor);
+  const double bound = response*res;
+  c->solver_last[2] = res; c->solver_last[3] = bound; c->solver_last[4] = target;
+  c->solver_last[5] = scale; c->solver_last[6] = eps;
+  if (bound <= target) {c->solver_last[1] = 1.0; return DK_OK;}
+  int it = 0;
+  if ((e = StrictPCG(c, a_dt, sh, t, true, &it))) return e;
+  c->solver_last[0] = it;
+  return DK_OK;
 }
 
 }  // namespace
@@ -913,6 +1190,7 @@ const char *dk_strerror(int code) {
     case DK_ECAPACITY: return "size beyond a fixed maximum";
     case DK_ESTATE: return "call order violated";
     case DK_ENOTFINITE: return "non-finite value";
+    case DK_ESOLVER: return "drag solver failed";
     default: return "unknown error";
   }
 }
@@ -931,7 +1209,7 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
       (long long)(mesh->nx1 + 2*mesh->ng)*(mesh->nx2 + 2*mesh->ng)*(mesh->nx3 + 2*mesh->ng) >
           INT32_MAX)
     return DK_ECAPACITY;
-  if (phys->solver < DK_LOCAL || phys->solver > DK_DC2) return DK_EUNSUPPORTED;
+  if (phys->solver < DK_LOCAL || phys->solver > DK_ADAPTIVE) return DK_EUNSUPPORTED;
   if (phys->clouds && (!dk::CloudsSupported(mesh, phys) || form != DK_FORM_FP64))
     return DK_EUNSUPPORTED;
   dk_ctx *c = new dk_ctx();
@@ -1014,6 +1292,10 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
       alloc((void**)&c->gpack, sizeof(double)*12*nc);
       alloc((void**)&c->cvel, sizeof(double)*3*nc);
     }
+    if (phys->solver == DK_PCG || phys->solver == DK_ADAPTIVE) {
+      alloc((void**)&c->solver_r, sizeof(double)*3*nc);
+      alloc((void**)&c->solver_p, sizeof(double)*3*nc);
+    }
   }
   if (!e) e = dk::Check(hipMemcpy(c->d_blocks, hb.data(), sizeof(DevBlock)*mesh->nmb,
                                   hipMemcpyHostToDevice));
@@ -1026,6 +1308,8 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
       if (!e) e = dk::Check(hipMemset(p, 0, sizeof(double)*nv*nc));
     }
     if (!e && phys->clouds) e = dk::Check(hipMemset(c->gpack, 0, sizeof(double)*12*nc));
+    for (double *p : {c->solver_r, c->solver_p})
+      if (!e && p) e = dk::Check(hipMemset(p, 0, sizeof(double)*3*nc));
   }
   if (!e && form == DK_FORM_FP32) e = dk::FastCreate(c);
   if (e) {dk_destroy(c); return e;}
@@ -1044,7 +1328,8 @@ int dk_destroy(dk_ctx *c) {
                   (void*)c->d_partial, (void*)c->d_scalar, (void*)c->qdep, (void*)c->ustar,
                   (void*)c->dmom, (void*)c->ap, (void*)c->resid, c->sort_tmp,
                   (void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
-                  (void*)c->sort_ibuf, (void*)c->gpack, (void*)c->cvel})
+                  (void*)c->sort_ibuf, (void*)c->gpack, (void*)c->cvel, (void*)c->solver_r,
+                  (void*)c->solver_p})
     if (p) (void)hipFree(p);
   delete c;
   return DK_OK;
@@ -1199,10 +1484,18 @@ int dk_gas_implicit_solve(dk_ctx *c, const dk_stage *st) {
   if ((e = Launched(c))) return e;
   if (!br || c->P.solver == DK_LOCAL) return DK_OK;
   const double a_dt = DragStep(c, st), sh = ImplicitShearShift(c, st);
+  for (int q=0; q<7; ++q) c->solver_last[q] = (q < 2) ? 0.0 : -1.0;
   if (c->P.solver == DK_APPLYA) return ApplyCoupledOperator(c, a_dt, sh, false, st->time);
+  if (c->P.solver == DK_PCG) {
+    int it = 0;
+    if ((e = StrictPCG(c, a_dt, sh, st->time, false, &it))) return e;
+    c->solver_last[0] = it;
+    return DK_OK;
+  }
   const int nsweeps = (c->P.solver == DK_DC2) ? 2 : 1;
   for (int s=0; s<nsweeps; ++s)
     if ((e = ApplyCoupledOperator(c, a_dt, sh, true, st->time))) return e;
+  if (c->P.solver == DK_ADAPTIVE) return AdaptiveSolve(c, a_dt, sh, st->dt, st->time);
   return DK_OK;
 }
 
@@ -1357,6 +1650,24 @@ int dk_stage_run(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   if (!e) e = c->graphs ? StageGraph(c, st, src) : StageBody(c, st, src);
   if (!e && c->fast) e = dk::FastTuneEnd(c, st);
   return e;
+}
+
+// the strict solver's parameters (src/dust/dust.cpp:467-474)
+int dk_set_solver(dk_ctx *c, double rtol, double atol, int32_t iter_max, double order_c,
+                  double rtol_max, double tref, double state_floor) {
+  if (c == nullptr || !(rtol > 0.0) || atol < 0.0 || iter_max < 1 || !(order_c > 0.0) ||
+      !(rtol_max > 0.0) || !(tref > 0.0) || !(state_floor > 0.0))
+    return DK_EINVAL;
+  c->drag_rtol = rtol; c->drag_atol = atol; c->drag_iter_max = iter_max;
+  c->adaptive_order_c = order_c; c->adaptive_rtol_max = rtol_max; c->adaptive_tref = tref;
+  c->adaptive_state_floor = state_floor;
+  return DK_OK;
+}
+
+int dk_solver_stats(dk_ctx *c, double out[7]) {
+  if (c == nullptr || out == nullptr) return DK_EINVAL;
+  for (int q=0; q<7; ++q) out[q] = c->solver_last[q];
+  return DK_OK;
 }
 
 int dk_set_graphs(dk_ctx *c, int32_t enable) {

@@ -100,6 +100,15 @@ struct dk_ctx {
   double *qdep = nullptr, *ustar = nullptr, *dmom = nullptr, *ap = nullptr, *resid = nullptr;
   // refinable clouds: the packed gather field (m, k, j, i, 12) and a velocity field (3)
   double *gpack = nullptr, *cvel = nullptr;
+  // the strict solver (pcg, adaptive): residual and search direction (3 components), the
+  // parameters (the reference's <dust> defaults, dust.cpp:467-474) and the last solve's
+  // iterations, fast accept, true residual, error bound, target, state scale, max Q/rho
+  double *solver_r = nullptr, *solver_p = nullptr;
+  double drag_rtol = 1.0e-11, drag_atol = 1.0e-14;
+  int drag_iter_max = 200;
+  double adaptive_order_c = 0.25, adaptive_rtol_max = 1.0e-3, adaptive_tref = 1.0,
+         adaptive_state_floor = 1.0;
+  double solver_last[7] = {0, 0, -1, -1, -1, -1, -1};
   dk::ErrWord *d_err = nullptr;
   double *d_partial = nullptr;     // reduction partials
   double *d_scalar = nullptr;

@@ -22,6 +22,7 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_gas_hook", "dk_sort_particles", "dk_new_time_step", "dk_diag_sums",
             "dk_error", "dk_field", "dk_strerror", "dk_setup_problem", "dk_prof_enable",
             "dk_prof_reset", "dk_prof_read", "dk_autotune", "dk_tune_config",
+            "dk_set_solver", "dk_solver_stats",
             "dk_set_tune_config"] +
            ["dk_" + t for t in TASKS])
 
@@ -53,6 +54,9 @@ def lib():
         L.dk_gas_hook.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_set_graphs.argtypes = [P, C.c_int32]
         L.dk_autotune.argtypes = [P, C.c_int32]
+        L.dk_set_solver.argtypes = [P, C.c_double, C.c_double, C.c_int32, C.c_double,
+                                    C.c_double, C.c_double, C.c_double]
+        L.dk_solver_stats.argtypes = [P, C.POINTER(C.c_double)]
         L.dk_tune_config.argtypes = [P] + [C.POINTER(C.c_int32)] * 3
         L.dk_set_tune_config.argtypes = [P, C.c_int32, C.c_int32]
         L.dk_sort_particles.argtypes = [P]
@@ -193,6 +197,19 @@ class DustContext:
     def set_tune_config(self, xcd, key_threads):
         self._check(lib().dk_set_tune_config(self.h, int(xcd), int(key_threads)),
                     "dk_set_tune_config")
+
+    def set_solver(self, rtol=1e-11, atol=1e-14, iter_max=200, order_c=0.25, rtol_max=1e-3,
+                   tref=1.0, state_floor=1.0):
+        """The strict drag solver's parameters (the reference's <dust> defaults)."""
+        self._check(lib().dk_set_solver(self.h, rtol, atol, iter_max, order_c, rtol_max, tref,
+                                        state_floor), "dk_set_solver")
+
+    def solver_stats(self):
+        """The last implicit solve: iterations, fast accept, true residual, error bound,
+        acceptance target, state scale, max Q/rho (-1 where not formed)."""
+        out = (C.c_double * 7)()
+        self._check(lib().dk_solver_stats(self.h, out), "dk_solver_stats")
+        return list(out[:])
 
     def sort(self):
         self._check(lib().dk_sort_particles(self.h), "dk_sort_particles")

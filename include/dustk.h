@@ -204,6 +204,19 @@ int dk_sort_particles(dk_ctx *ctx);
  * (both forms; the FP32 form applies a pending correction first, from its FP32 velocities) */
 int dk_new_time_step(dk_ctx *ctx, double dt_cfl, double *dt_out);
 
+/* The strict drag solver (dk_physics.solver = DK_PCG or DK_ADAPTIVE, FP64 form): its
+ * parameters, the <dust> inputs drag_rtol, drag_atol, drag_iter_max, drag_adaptive_order_c,
+ * drag_adaptive_rtol_max, drag_adaptive_tref, drag_adaptive_state_floor
+ * (src/dust/dust.cpp:467-474; defaults 1e-11, 1e-14, 200, 0.25, 1e-3, 1, 1), and the last
+ * implicit solve's figures (the DUST_SOLVER_DIAG line, src/dust/dust_solver.cpp:709-718):
+ * out[0] PCG iterations, [1] adaptive fast accept (0/1), [2] true residual, [3] error bound,
+ * [4] acceptance target, [5] state scale, [6] max Q/rho (-1 where the path did not form it).
+ * A solve that loses positive definiteness or exhausts drag_iter_max returns DK_ESOLVER from
+ * dk_gas_implicit_solve (the reference aborts, dust_solver.cpp:468-526). */
+int dk_set_solver(dk_ctx *ctx, double rtol, double atol, int32_t iter_max, double order_c,
+                  double rtol_max, double tref, double state_floor);
+int dk_solver_stats(dk_ctx *ctx, double out[7]);
+
 /* ---- whole stage ------------------------------------------------------------------- */
 
 /* All dust tasks of one stage in task-graph order (dust_tasks.cpp:58-109), gas hook

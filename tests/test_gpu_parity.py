@@ -118,6 +118,41 @@ def test_imex_particles_crossing_blocks(solver):
     run(w, 20, check_every=5)
 
 
+@pytest.mark.parametrize("solver,params", [
+    (abi.DK_PCG, {}),
+    (abi.DK_ADAPTIVE, {}),                                   # the default target: PCG fallback
+    (abi.DK_ADAPTIVE, {"order_c": 1e12, "rtol_max": 1.0})])  # a loose target: DC1 accepted
+def test_strict_solver_matches_oracle(solver, params):
+    """StrictPCG and the adaptive path (dust_solver.cpp:268-625) against the oracle: the same
+    path every stage (fast accept or PCG, iterations within one: the reductions sum in another
+    order) and the same state to 1e-11 after 10 steps of particles crossing blocks."""
+    w = workloads.c1(n=16, nb=2, ppc=2, solver=solver)
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 2.0, 0.5, 2e-3
+    w.name = "strict-%d" % solver
+    o, g = make_pair(w)
+    o.set_solver(**params)
+    g.set_solver(**params)
+    for s in range(10):
+        o.step(w.dt, w.src)
+        g.step(w.dt, w.src)
+        so, sg = o.solver_stats(), g.solver_stats()
+        say("step %d: oracle %s, gpu %s" % (s, so[:2], sg[:2]))
+        assert so[1] == sg[1] and abs(so[0] - sg[0]) <= 1, (so, sg)
+    g.check_device_error()
+    compare(o, g, w, vtol=1e-11, what="strict solver")
+
+
+def test_strict_solver_budget_is_reported():
+    from athenak_amd.lib import DustError
+    w = workloads.c1(n=16, nb=2, ppc=2, solver=abi.DK_PCG)
+    w.setup.dust_noise, w.setup.gas_noise = 0.5, 0.3
+    _, g = make_pair(w)
+    g.set_solver(rtol=1e-16, atol=0.0, iter_max=1)
+    with pytest.raises(DustError, match="drag solver failed"):
+        g.step(w.dt, w.src)
+    g.close()
+
+
 def test_pc2_particles_crossing_blocks():
     w = workloads.c1(n=16, nb=2, ppc=2, coupling=abi.DK_PC2, solver=abi.DK_LOCAL)
     w.setup.dust_noise, w.setup.gas_noise, w.dt = 2.0, 0.5, 2e-3
