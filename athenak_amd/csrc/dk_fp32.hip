@@ -140,7 +140,8 @@ struct Fast {
   // for kTuneStages active stages each, after as many warm-up stages
   int tune_state = 0, tune_cand = -1, tune_stage = 0, ncand = 0;
   int cand_xcd[8] = {0}, cand_kt[8] = {0};
-  float cand_ms[8] = {0.0f};
+  float cand_ms[8] = {0.0f};        // the fastest visit of each candidate
+  float visit_ms = 0.0f;
   hipEvent_t tev[2] = {nullptr, nullptr};
 };
 
@@ -2479,13 +2480,15 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
 // cell's particles in store order whatever the chunk, and its ranks are re-sorted by source
 // slot).  The tile shape is not among them: it groups the cross-tile sums.  Tuning runs online,
 // on the stages the caller runs anyway: after kTuneStages warm-up stages each candidate runs
-// kTuneStages active stages, timed with events on the context's stream, and the fastest is
-// kept; with DK_TUNE_CACHE naming a JSON file the choice is stored there under a key of the
+// kTuneStages active stages, timed with events on the context's stream, in kTuneRounds rounds
+// over the candidates, and the one with the fastest visit is kept (the smallest of its visits:
+// a stage's time varies by about as much as the candidates differ); with DK_TUNE_CACHE naming a JSON file the choice is stored there under a key of the
 // mesh, tile shape and particle count, and read back instead of tuning again.
 
 namespace {
 
 constexpr int kTuneStages = 2;
+constexpr int kTuneRounds = 2;
 
 std::string TuneKey(const dk_ctx *c) {
   const Fast *f = c->fast;
@@ -2585,13 +2588,18 @@ int FastTuneEnd(dk_ctx *c, const dk_stage *st) {
     if (!e) e = Check(hipEventSynchronize(f->tev[1]));
     if (!e) e = Check(hipEventElapsedTime(&ms, f->tev[0], f->tev[1]));
     if (e) return e;
-    f->cand_ms[f->tune_cand] += ms;
+    f->visit_ms += ms;
   }
   if (++f->tune_stage < kTuneStages) return DK_OK;
   f->tune_stage = 0;
+  if (f->tune_cand >= 0) {
+    float &best_visit = f->cand_ms[f->tune_cand % f->ncand];
+    if (f->tune_cand < f->ncand || f->visit_ms < best_visit) best_visit = f->visit_ms;
+  }
+  f->visit_ms = 0.0f;
   ++f->tune_cand;
-  if (f->tune_cand < f->ncand) {
-    ApplyCand(f, f->tune_cand);
+  if (f->tune_cand < kTuneRounds*f->ncand) {
+    ApplyCand(f, f->tune_cand % f->ncand);
     return DK_OK;
   }
   int best = 0;
@@ -2640,6 +2648,7 @@ int dk_autotune(dk_ctx *c, int32_t mode) {
   f->tune_state = 1;
   f->tune_cand = -1;
   f->tune_stage = 0;
+  f->visit_ms = 0.0f;
   return DK_OK;
 }
 
