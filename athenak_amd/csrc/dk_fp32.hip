@@ -134,6 +134,9 @@ struct Fast {
   int pending = 0;
   double a_prev = 0.0;
   int regs_dead = 0;                // the store holds no registers (after a step's last stage)
+  // pc2: R lives in the other store (after a predictor); the positions await the migration
+  // that the next key pass applies (after an advance)
+  int r_other = 0, unmigrated = 0;
   // launch parameters that leave every result unchanged (the autotuner's choices)
   int key_threads = kKeyThreads;
   // online autotuning: state 0 fixed, 1 tuning, 2 tuned; candidates run one after another
@@ -178,6 +181,7 @@ struct KArgs {
   double inv_a_dt, inv_a_prev;      // 1/a_dt, 1/a_prev (the drag rates R = delta/a)
   double ysh, dsh;                  // shear crossing offsets (bvals_part.cpp, dust.cpp)
   int stage1, sbox, strat, shear;
+  int rate_dep;                     // pc2: the predictor also deposits the rate m/(t_s V)
   // the stage registers x1, v1 and R of this stage are read by a later stage (0 at the last
   // active stage: the next step's stage 1 sets them before any use, so they are not kept)
   int regs_live;
@@ -361,6 +365,42 @@ __device__ inline void DepositCell(const Geo &g, const DevBlock &b, const double
                                    double sh, int &ci, int &cj, int &ck, int &cju) {
   const double lo[3] = {b.x1min, b.x2min, b.x3min}, sc[3] = {b.sx1, b.sx2, b.sx3};
   DepositCellOf(g, lo, sc, x, sh, ci, cj, ck, cju);
+}
+
+// pc2: the half-step position xh = x0 + dt v0/2 (dust_push.cpp:150-154), where the advance gathers
+// and deposits (periodic meshes without the shearing box: no frame terms)
+__device__ inline void Pc2Mid(const KArgs &A, const double x0[3], const float v0[3],
+                              double xh[3]) {
+#pragma clang fp contract(off)
+  const double h = 0.5*A.dt;
+  for (int d=0; d<3; ++d) xh[d] = x0[d] + h*static_cast<double>(v0[d]);
+}
+
+// pc2 stage 2: the particle goes to the block of its half-step position for the advance, its
+// position moved by a mesh length across a periodic face with it (the advance recomputes xh from
+// the moved position, the same arithmetic as here); x1 <- xh, v1 <- v0 (dust_push.cpp:240-242).
+// The block of the advanced position is found by the next stage's key pass (or an export).
+// Returns false on a routing error (the particle keeps its block); xh: the position keyed.
+__device__ inline bool Pc2Route(const KArgs &A, const DevBlock &b, PState &s, double xh[3]) {
+  Pc2Mid(A, s.x, s.v, xh);
+  const int ix = BlockOff(xh[0], b.x1min, b.x1max, b.x1max - b.x1min);
+  const int iy = BlockOff(xh[1], b.x2min, b.x2max, b.x2max - b.x2min);
+  const int iz = BlockOff(xh[2], b.x3min, b.x3max, b.x3max - b.x3min);
+  bool ok = true;
+  if (ix != 0 || iy != 0 || iz != 0) {
+    const int nb = (abs(ix) > 1 || abs(iy) > 1 || abs(iz) > 1) ? -1 : b.nghbr[Dir(ix, iy, iz)];
+    if (nb < 0) {
+      ok = false;
+    } else {
+      s.gid = nb;
+      if ((ix < 0) && (b.face & 1)) s.x[0] += A.lxm; else if ((ix > 0) && (b.face & 2)) s.x[0] -= A.lxm;
+      if ((iy < 0) && (b.face & 4)) s.x[1] += A.lym; else if ((iy > 0) && (b.face & 8)) s.x[1] -= A.lym;
+      if ((iz < 0) && (b.face & 16)) s.x[2] += A.lzm; else if ((iz > 0) && (b.face & 32)) s.x[2] -= A.lzm;
+      Pc2Mid(A, s.x, s.v, xh);
+    }
+  }
+  for (int d=0; d<3; ++d) {s.x1[d] = xh[d]; s.v1[d] = s.v[d];}
+  return ok;
 }
 
 // the centre cell of the unshifted position (dust_pm.cpp:649-659); false when it rounds onto
@@ -753,8 +793,9 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
 // residue summed per cell for the gas, and every particle's new tile-major key and rank: a
 // particle that stays in its tile takes its rank from a count in LDS (the tile owns those
 // cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
-// only the pending correction is applied (written to v and R).
-template <bool OV>
+// only the pending correction is applied (written to v and R).  MID: the pc2 stage-2 form, keyed
+// by the half-step position (Pc2Route) instead of pushed.
+template <bool OV, bool MID = false>
 __global__ void __launch_bounds__(kKeyThreadsMax)
 k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
@@ -809,7 +850,7 @@ k_key(KArgs A, int do_key, double *pres) {
         }
         PState s;
         LoadP(A.src, q, s, false, A.atw_dt != 0.0 || !do_key);
-        if (do_key && !A.stage1) {
+        if (!MID && do_key && !A.stage1) {
           // the stage registers x1, v1 where the stage-1 key pass wrote them: the store before
           // the last sort (the other store), at the slot the permutation gives
           const int pq = A.perm[q];
@@ -850,7 +891,12 @@ k_key(KArgs A, int do_key, double *pres) {
         }
         float pr[3] = {0.0f, 0.0f, 0.0f};
         if (do_key) {
-          if (!PushMigrate(A, b, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+          double xh[3];   // (MID: the half-step position, which gives the key)
+          if constexpr (MID) {
+            if (!Pc2Route(A, b, s, xh)) dk::RecordError(A.err, DK_EROUTE, q);
+          } else {
+            if (!PushMigrate(A, b, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+          }
           for (int d=0; d<3; ++d) {
             A.src.X(d, q) = s.x[d];
             A.src.V(d, q) = s.v[d];
@@ -867,7 +913,7 @@ k_key(KArgs A, int do_key, double *pres) {
             lo[0] = bn.x1min; lo[1] = bn.x2min; lo[2] = bn.x3min;
             sc[0] = bn.sx1; sc[1] = bn.sx2; sc[2] = bn.sx3;
           }
-          DepositCellOf(g, lo, sc, s.x, A.sh, ni, nj, nk, nju);
+          DepositCellOf(g, lo, sc, MID ? xh : s.x, A.sh, ni, nj, nk, nju);
           const long long key = TileKey(Tl, s.gid, ni, nj, nk);
           A.key[q] = static_cast<int>(key);
           const long long lk = key - t.key0;
@@ -1482,6 +1528,273 @@ k_p3n(KArgs A, const float *us, float *scr, double *defect) {
   P3Body<S, SY, false, OV>(A, us, scr, defect);
 }
 
+// pc2 predictor (dust_pm.cpp:141-193): the particles (routed in place by the key pass) moved into
+// the new order, the gas velocity u = M/rho of u1 (uv, with ghosts) gathered at each particle,
+// and the deposit of the rate m/(t_s V) (A.rate_dep) and of g = -dt (m/(t_s V)) (u - v) with the
+// same weights: qdep components 0 (rate), 1-3 (g).  The registers x1, v1 and the rate R stay in
+// the source order (the next key pass and an export read them through this sort's permutation).
+template <int S, bool OV>
+__global__ void __launch_bounds__(kCellThreads, 4)
+k_pp(KArgs A, const float *uv, float *scr) {
+  extern __shared__ double lds_d[];
+  const Geo &g = A.g;
+  const Tiles &Tl = A.T;
+  const int nlist = A.item_start[Tl.nt];
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+       li < nlist; li += gridDim.x) {
+    const int tile = A.item_tile[li];
+    const int part = li - A.item_start[tile];
+    const TileInfo t = TileOf(Tl, tile);
+    const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+    float *dep = reinterpret_cast<float*>(lds_d);          // [4][RN] the deposit
+    float *uv_ = dep + 4*Tl.RN;                            // [3][RN] u = M/rho
+    float *rd = uv_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
+    float *rv = rd + 3*CH;                                 // [4][CH] rate, g
+    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 7, 4));   // [TC + 1]
+    RegionFetch<float> uf;
+    uf.Load(g, Tl, t, uv, 3);
+    for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+    for (int i = tid; i < 4*Tl.RN; i += NT) dep[i] = 0.0f;
+    __syncthreads();
+    uf.Store(Tl, 3, uv_);
+    __syncthreads();
+    const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
+    const DevBlock &b = A.B[t.m];
+    const float ivolf = static_cast<float>(b.ivol);
+    const float dtf = static_cast<float>(A.dt);
+    const int slab = tid/Tl.TC;
+    const int l = tid - slab*Tl.TC;
+    for (int c0 = P0; c0 < P1; c0 += CH) {
+      const int c1 = min(c0 + CH, P1);
+#pragma unroll 1
+      for (int k=0; k<kPPT; ++k) {
+        const int r = tid + k*NT;
+        const int q = c0 + r;
+        if (q >= c1) break;
+        const int p = A.perm[q];
+        const double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
+        const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
+        const float ts = A.src.TS(p), mass = A.src.M(p);
+        const int kp = A.key[p];
+        for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d];}
+        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.ORIG(q) = A.src.ORIG(p);
+        const Loc L = Locate<false>(g, b, x, 0.0);
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+        float val[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (TileKey(Tl, t.m, L.ci, L.cj, L.ck) != kp) {
+          dk::RecordError(A.err, DK_EHALO, p);
+        } else {
+          ox = static_cast<float>(L.dx);
+          oy = static_cast<float>(L.dy);
+          oz = static_cast<float>(L.dz);
+          float wx[3], wy[3], wz[3];
+          W3<S, float>(ox, wx);
+          W3<S, float>(oy, wy);
+          W3<S, float>(oz, wz);
+          const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0))*Tl.RX + (L.ci - t.ci0);
+          float u[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int c=0; c<3; ++c)
+#pragma unroll
+            for (int bb=0; bb<3; ++bb) {
+              const float wcb = wz[c]*wy[bb];
+              const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+#pragma unroll
+              for (int a=0; a<3; ++a) {
+                const float w = wcb*wx[a];
+                u[0] += w*uv_[rr + a];
+                u[1] += w*uv_[Tl.RN + rr + a];
+                u[2] += w*uv_[2*Tl.RN + rr + a];
+              }
+            }
+          const float rate = mass*ivolf/ts;
+          val[0] = A.rate_dep ? rate : 0.0f;
+          for (int d=0; d<3; ++d) val[1 + d] = -dtf*rate*(u[d] - v[d]);
+        }
+        rd[r] = ox; rd[CH + r] = oy; rd[2*CH + r] = oz;
+        for (int n=0; n<4; ++n) rv[n*CH + r] = val[n];
+      }
+      __syncthreads();
+      float acc[3][3][4], spl[3][4];
+#pragma unroll
+      for (int a=0; a<3; ++a)
+#pragma unroll
+        for (int n=0; n<4; ++n) {
+          spl[a][n] = 0.0f;
+#pragma unroll
+          for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+        }
+      int spdir = 0;
+      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+      for (int qq = s0; qq < s1; ++qq) {
+        const int r = qq - c0;
+        float wx[3], wy[3];
+        W3<S, float>(rd[r], wx);
+        W3<S, float>(rd[CH + r], wy);
+        const float wz = W1<S, float>(slab, rd[2*CH + r]);
+        const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
+        AddSlab<float, 4, false>(acc, spl, spdir, 0, wz, wy, wx, val);
+      }
+      DepositSlabs<float, 4, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
+    }
+    if (part == 0) FlushRegion<float, 4>(Tl, t, dep, scr);
+    else FlushOverflow<4>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN);
+    if constexpr (!OV) break;
+    __syncthreads();   // (the next item reuses LDS)
+  }
+}
+
+// pc2 advance (dust_push.cpp:132-245, no frame terms): the particles moved into the new order,
+// u* gathered at the half-step position xh (the key pass placed them by it), the implicit
+// midpoint kick (1 + lambda) vh = v0 + lambda u*, lambda = dt/(2 t_s), v = 2 vh - v0,
+// x = x0 + dt vh, and the deposit of the impulse -m (v - v0)/V at xh (dmom, dust_push.cpp:
+// 226-244).  v - v0 = dt (u* - vh)/t_s in exact arithmetic; the impulse and R take the exact
+// difference of the FP32 velocities, so that the gas receives the dust's actual change.  As P3:
+// the impulse is deposited in FP32 and every tile's FP64 defect goes to the combine.
+template <int S, bool OV>
+__global__ void __launch_bounds__(kCellThreads, 4)
+k_pa(KArgs A, const float *us, float *scr, double *defect) {
+  extern __shared__ double lds_d[];
+  const Geo &g = A.g;
+  const Tiles &Tl = A.T;
+  const int nlist = A.item_start[Tl.nt];
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+       li < nlist; li += gridDim.x) {
+    const int tile = A.item_tile[li];
+    const int part = li - A.item_start[tile];
+    const TileInfo t = TileOf(Tl, tile);
+    const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+    float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
+    float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
+    float *rd = us_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
+    float *rv = rd + 3*CH;                                 // [3][CH] -m (v - v0)/V
+    int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 6, 3));   // [TC + 1]
+    RegionFetch<float> uf;
+    uf.Load(g, Tl, t, us, 3);
+    for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
+    for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
+    __syncthreads();
+    uf.Store(Tl, 3, us_);
+    __syncthreads();
+    const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
+    const DevBlock &b = A.B[t.m];
+    const double ivol = b.ivol;
+    const float hf = static_cast<float>(0.5*A.dt);
+    const int slab = tid/Tl.TC;
+    const int l = tid - slab*Tl.TC;
+    double ex[3] = {0.0, 0.0, 0.0};
+    for (int c0 = P0; c0 < P1; c0 += CH) {
+      const int c1 = min(c0 + CH, P1);
+#pragma unroll 1
+      for (int k=0; k<kPPT; ++k) {
+        const int r = tid + k*NT;
+        const int q = c0 + r;
+        if (q >= c1) break;
+        const int p = A.perm[q];
+        double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
+        float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
+        const float ts = A.src.TS(p), mass = A.src.M(p);
+        const int kp = A.key[p];
+        double xh[3];
+        Pc2Mid(A, x, v, xh);
+        const Loc L = Locate<false>(g, b, xh, 0.0);
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+        double dv[3] = {0.0, 0.0, 0.0};
+        if (TileKey(Tl, t.m, L.ci, L.cj, L.ck) != kp) {
+          dk::RecordError(A.err, DK_EHALO, p);
+        } else {
+          ox = static_cast<float>(L.dx);
+          oy = static_cast<float>(L.dy);
+          oz = static_cast<float>(L.dz);
+          float wx[3], wy[3], wz[3];
+          W3<S, float>(ox, wx);
+          W3<S, float>(oy, wy);
+          W3<S, float>(oz, wz);
+          const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0))*Tl.RX + (L.ci - t.ci0);
+          float u[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int c=0; c<3; ++c)
+#pragma unroll
+            for (int bb=0; bb<3; ++bb) {
+              const float wcb = wz[c]*wy[bb];
+              const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+#pragma unroll
+              for (int a=0; a<3; ++a) {
+                const float w = wcb*wx[a];
+                u[0] += w*us_[rr + a];
+                u[1] += w*us_[Tl.RN + rr + a];
+                u[2] += w*us_[2*Tl.RN + rr + a];
+              }
+            }
+          const float lam = hf/ts;
+          const float aa = 1.0f + lam;
+          for (int d=0; d<3; ++d) {
+            const float vo = v[d];
+            const float vh = (vo + lam*u[d])/aa;
+            const float vn = 2.0f*vh - vo;
+            x[d] = x[d] + A.dt*static_cast<double>(vh);
+            v[d] = vn;
+            dv[d] = static_cast<double>(vn) - static_cast<double>(vo);
+          }
+        }
+        for (int d=0; d<3; ++d) {
+          A.dst.X(d, q) = x[d];
+          A.dst.V(d, q) = v[d];
+          A.dst.R(d, q) = dv[d];
+        }
+        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.ORIG(q) = A.src.ORIG(p);
+        const double fac = -static_cast<double>(mass)*ivol;
+        for (int d=0; d<3; ++d) {
+          const double imp = fac*dv[d];
+          ex[d] += imp;
+          rv[d*CH + r] = static_cast<float>(imp);
+        }
+        rd[r] = ox; rd[CH + r] = oy; rd[2*CH + r] = oz;
+      }
+      __syncthreads();
+      float acc[3][3][3], spl[3][3];
+#pragma unroll
+      for (int a=0; a<3; ++a)
+#pragma unroll
+        for (int n=0; n<3; ++n) {
+          spl[a][n] = 0.0f;
+#pragma unroll
+          for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
+        }
+      int spdir = 0;
+      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+      for (int qq = s0; qq < s1; ++qq) {
+        const int r = qq - c0;
+        float wx[3], wy[3];
+        W3<S, float>(rd[r], wx);
+        W3<S, float>(rd[CH + r], wy);
+        const float wz = W1<S, float>(slab, rd[2*CH + r]);
+        const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
+        AddSlab<float, 3, false>(acc, spl, spdir, 0, wz, wy, wx, val);
+      }
+      DepositSlabs<float, 3, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
+    }
+    double ws[3];
+    if (part == 0) FlushRegion<float, 3>(Tl, t, dep, scr, ws);
+    else FlushOverflow<3>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN, ws);
+    double *red = reinterpret_cast<double*>(rd);           // [NT/64][3]
+    const int wv = tid >> 6;
+    for (int d=0; d<3; ++d) {
+      const double a = WaveSum(ex[d] - ws[d], min(64, NT - (wv << 6)));
+      if ((tid & 63) == 0) red[3*wv + d] = a;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      double a = 0.0;
+      for (int w = 0; w < (NT + 63) >> 6; ++w) a += red[3*w + tid];
+      if (part == 0) defect[3*tile + tid] = a;
+      else A.ov_def[3*OvSlot(A.item_start, tile, part) + tid] = a;
+    }
+    if constexpr (!OV) break;
+    __syncthreads();   // (the next item reuses LDS)
+  }
+}
+
 // the centre masses of a tile's later parts added in part order (thread per cell of the tiles
 // with more than one part; every target has one writer: its own cell, the cell above, below)
 __global__ void k_fix_qt(KArgs A, float *qt0, float *qt1, float *qt2) {
@@ -1654,16 +1967,27 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *
 // stage start: the gas share of the previous split, u0 += D - (actual dust correction),
 // R_g = that/a (dust_pm.cpp:670-685); register copy u1 <- u0 (stage 1, dust_push.cpp:65);
 // the harness's gas update (zero-flux RK combination + source terms); the drag history
-// (stage 2, dust_pm.cpp:734-739)
+// (stage 2, dust_pm.cpp:734-739).  pc2 (fout, before the gas update): fmode 1 the gas velocity
+// M/rho of u1 (= u0 here, stage 1) that the predictor gathers (dust_pm.cpp:163-172), fmode 2
+// u* = (M1 + M0)/(rho1 + rho0) that the advance gathers (dust_push.cpp:45-58)
 __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, double *dmom,
                              const double *dc0, const double *dc1, const double *dc2,
                              const double *pres,
                              int pending, double inv_aprev, int stage1, int hook, double gam0,
                              double gam1, double bdt, int use_src, dk_gas_src src,
-                             double atw_dt) {
+                             double atw_dt, float *fout = nullptr, int fmode = 0) {
   BLOCK_CELLS(m, k, j, i)
     double u[4];
     for (int v=0; v<4; ++v) u[v] = u0[Idx(g,m,DK_NGAS,v,k,j,i)];
+    if (fmode == 1) {
+      const double irho = 1.0/u[0];
+      for (int c=0; c<3; ++c) fout[Idx(g,m,3,c,k,j,i)] = static_cast<float>(u[1+c]*irho);
+    } else if (fmode == 2) {
+      const double irho = 1.0/(u1[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)] + u[0]);
+      for (int c=0; c<3; ++c)
+        fout[Idx(g,m,3,c,k,j,i)] =
+            static_cast<float>((u1[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)] + u[1+c])*irho);
+    }
     double rg[3] = {0.0, 0.0, 0.0};
     if (pending) {
       for (int c=0; c<3; ++c) {
@@ -1703,6 +2027,38 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
     }
     if (atw_dt != 0.0) for (int c=0; c<3; ++c) u[1+c] += atw_dt*rg[c];
     for (int v=0; v<4; ++v) u0[Idx(g,m,DK_NGAS,v,k,j,i)] = u[v];
+  }
+}
+
+// pc2 apply-predictor (dust_pm.cpp:387-402): with damping, the cells where chi = dt rate/rho
+// exceeds 1 scale their g by 1/chi (kept: the commit removes the same values); u0 += g
+__global__ void k_pc2_apply(Geo g, double *u0, float *qd, int damp, double dt) {
+  BLOCK_CELLS(m, k, j, i)
+    float q[3];
+    for (int c=0; c<3; ++c) q[c] = qd[Idx(g,m,4,1+c,k,j,i)];
+    if (damp) {
+      const double chi = dt*static_cast<double>(qd[Idx(g,m,4,0,k,j,i)])/
+                         u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+      if (chi > 1.0) {
+        const double sc = 1.0/chi;
+        for (int c=0; c<3; ++c) {
+          q[c] = static_cast<float>(static_cast<double>(q[c])*sc);
+          qd[Idx(g,m,4,1+c,k,j,i)] = q[c];
+        }
+      }
+    }
+    for (int c=0; c<3; ++c) u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)] += static_cast<double>(q[c]);
+  }
+}
+
+// pc2 commit (dust_pm.cpp:596-601): u0 += dmom - gam1 g (the predictor's share of the stage
+// combination taken back, the actual impulse given)
+__global__ void k_pc2_commit(Geo g, double *u0, const double *dmom, const float *qd,
+                             double gam1) {
+  BLOCK_CELLS(m, k, j, i)
+    for (int c=0; c<3; ++c)
+      u0[Idx(g,m,DK_NGAS,DK_IM1+c,k,j,i)] +=
+          dmom[Idx(g,m,3,c,k,j,i)] - gam1*static_cast<double>(qd[Idx(g,m,4,1+c,k,j,i)]);
   }
 }
 
@@ -1804,24 +2160,40 @@ __global__ void k_permute_all(Store src, Store dst, const int *perm, long long n
 
 // (regs_dead: the store keeps no registers; they are written as x1 = x, v1 = v, R = 0, the values
 // the next stage 1 gives them before any use.  Otherwise x1, v1 are read where the stage-1 key
-// pass wrote them: the other store O, at the slot of the last sort's permutation)
-__global__ void k_export(Store S, Store O, const int *perm, const int *key, Tiles Tl,
+// pass wrote them: the other store O, at the slot of the last sort's permutation; r_other: R
+// too (after a pc2 predictor, which leaves it there).  migrate: after a pc2 advance, whose
+// positions the next key pass routes, the export applies that migration (bvals_part.cpp:164-429)
+// to x, x1 and the block, as the reference's state holds them; A: its mesh numbers, with
+// gam0 = 1 and gam1 = beta_dt = 0 so that PushMigrate only routes)
+__global__ void k_export(KArgs A, Store S, Store O, const int *perm, const int *key, Tiles Tl,
                          long long np, const int32_t *itag, long long np0, double *r,
-                         int32_t *iv, long long cap, int regs_dead) {
+                         int32_t *iv, long long cap, int regs_dead, int r_other, int migrate) {
   GRID_STRIDE(q, np) {
     const int pq = perm[q];
+    // the block: the store is tile-major by the keys of the last sort
+    PState s;
+    s.gid = key[pq]/Tl.TC/Tl.ntb;
     for (int d=0; d<3; ++d) {
-      const double x = S.X(d, q), v = static_cast<double>(S.V(d, q));
-      r[(long long)(DK_IPX + 2*d)*cap + q] = x;
-      r[(long long)(DK_IPVX + 2*d)*cap + q] = v;
-      r[(long long)(DK_IPX1 + 2*d)*cap + q] = regs_dead ? x : O.X1(d, pq);
-      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = regs_dead ? v : static_cast<double>(O.V1(d, pq));
-      r[(long long)(DK_IPRX + d)*cap + q] = regs_dead ? 0.0 : S.R(d, q);
+      s.x[d] = S.X(d, q);
+      s.v[d] = S.V(d, q);
+      s.x1[d] = regs_dead ? s.x[d] : O.X1(d, pq);
+      s.v1[d] = regs_dead ? s.v[d] : O.V1(d, pq);
+      s.r[d] = regs_dead ? 0.0 : (r_other ? O.R(d, pq) : S.R(d, q));
+      s.vres[d] = 0.0;
+    }
+    s.parked = 0;
+    if (migrate && !PushMigrate<false>(A, A.B[s.gid], s, false))
+      dk::RecordError(A.err, DK_EROUTE, q);
+    for (int d=0; d<3; ++d) {
+      r[(long long)(DK_IPX + 2*d)*cap + q] = s.x[d];
+      r[(long long)(DK_IPVX + 2*d)*cap + q] = static_cast<double>(s.v[d]);
+      r[(long long)(DK_IPX1 + 2*d)*cap + q] = s.x1[d];
+      r[(long long)(DK_IPVX1 + 2*d)*cap + q] = static_cast<double>(s.v1[d]);
+      r[(long long)(DK_IPRX + d)*cap + q] = s.r[d];
     }
     r[(long long)DK_IPTS*cap + q] = static_cast<double>(S.TS(q));
     r[(long long)DK_IPM*cap + q] = static_cast<double>(S.M(q));
-    // the block: the store is tile-major by the keys of the last sort
-    iv[(long long)DK_PGID*cap + q] = key[pq]/Tl.TC/Tl.ntb;
+    iv[(long long)DK_PGID*cap + q] = s.gid;
     const int o = S.ORIG(q);
     for (int c=1; c<DK_NIDATA; ++c) iv[(long long)c*cap + q] = itag[(long long)(c - 1)*np0 + o];
   }
@@ -1894,6 +2266,12 @@ size_t LdsP2(const Tiles &T, bool sy = true) {
   return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, sy ? 7 : 6, 3)) + LdsTail(T);
 }
 size_t LdsP3(const Tiles &T, bool sy = true) { return LdsP2(T, sy); }   // the same areas
+size_t LdsPP(const Tiles &T) {
+  return sizeof(float)*(7*T.RN + RecArea(T, kPPT*3*T.TC, 7, 4)) + LdsTail(T);
+}
+size_t LdsPA(const Tiles &T) {
+  return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, 6, 3)) + LdsTail(T);
+}
 
 // workgroups of the launch over the extra parts (they loop over the items; all exit at once when
 // no tile has more than one part)
@@ -1901,9 +2279,15 @@ dim3 OverGrid(const dk_ctx *c) {
   return dim3(static_cast<unsigned>(std::min<long long>(c->fast->ov_cap, kOverGrid)));
 }
 
-void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres) {
+// (mid: the pc2 stage-2 form)
+void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres, bool mid = false) {
   const Fast *f = c->fast;
   const size_t lds = LdsKey(f->T, f->key_threads);
+  if (mid) {
+    k_key<false, true><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres);
+    k_key<true, true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres);
+    return;
+  }
   k_key<false><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres);
   k_key<true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres);
 }
@@ -1970,6 +2354,30 @@ void LaunchP3(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *d
   const dim3 blk(3*T.TC);
   if (A0.regs_live) {LAUNCH_CLASSES(k_p3, LdsP3, (A, us, scr, defect));}
   else {LAUNCH_CLASSES(k_p3n, LdsP3, (A, us, scr, defect));}
+}
+
+// the pc2 tile kernels (one class: no shearing box), first parts then the extra parts
+#define SWITCH_S2(KERNEL, OV, grid, ARGS) \
+  switch (c->g.deposit) { \
+    case 2: KERNEL<2, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 1: KERNEL<1, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    default: KERNEL<0, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+  }
+
+void LaunchPP(dk_ctx *c, const KArgs &A, const float *uv, float *scr) {
+  const Tiles &T = c->fast->T;
+  const dim3 blk(3*T.TC);
+  const size_t lds = LdsPP(T);
+  SWITCH_S2(k_pp, false, dim3(T.nt), (A, uv, scr))
+  SWITCH_S2(k_pp, true, OverGrid(c), (A, uv, scr))
+}
+
+void LaunchPA(dk_ctx *c, const KArgs &A, const float *us, float *scr, double *defect) {
+  const Tiles &T = c->fast->T;
+  const dim3 blk(3*T.TC);
+  const size_t lds = LdsPA(T);
+  SWITCH_S2(k_pa, false, dim3(T.nt), (A, us, scr, defect))
+  SWITCH_S2(k_pa, true, OverGrid(c), (A, us, scr, defect))
 }
 
 // the cell passes: x over the active cells of a block, y over the blocks
@@ -2162,9 +2570,17 @@ int Flush(dk_ctx *c) {
 namespace dk {
 
 int FastCreate(dk_ctx *c) {
-  if (c->P.coupling != DK_IMEX) return DK_EUNSUPPORTED;          // pc2: the FP64 form
-  if (c->P.solver == DK_APPLYA || c->P.solver == DK_PCG || c->P.solver == DK_ADAPTIVE)
+  if (c->P.coupling == DK_PC2) {
+    // pc2 of point particles on periodic meshes without the shearing box (the half-step
+    // position is keyed in the block that holds it; clouds: the FP64 form)
+    if (c->P.clouds || c->P.shearing_box || c->M.bc_x1 != DK_X1_PERIODIC ||
+        c->g.bc_x3 != DK_X3_PERIODIC)
+      return DK_EUNSUPPORTED;
+  } else if (c->P.coupling != DK_IMEX) {
+    return DK_EUNSUPPORTED;
+  } else if (c->P.solver == DK_APPLYA || c->P.solver == DK_PCG || c->P.solver == DK_ADAPTIVE) {
     return DK_EUNSUPPORTED;                                          // the FP64 form
+  }
   if (c->g.ng < 2 || !c->P.back_reaction) return DK_EUNSUPPORTED;
   Fast *f = new Fast();
   c->fast = f;
@@ -2201,7 +2617,9 @@ int FastCreate(dk_ctx *c) {
   // the u* region of P2/P3 in kRegionPer values per thread
   if (3*T.RN > kRegionPer*3*T.TC) return DK_EUNSUPPORTED;
   // every tile kernel's LDS within 64 KiB
-  if (std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)) > 65536 - 1024) return DK_EUNSUPPORTED;
+  if (std::max(std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)), std::max(LdsPP(T), LdsPA(T))) >
+      65536 - 1024)
+    return DK_EUNSUPPORTED;
   // a block's first and last x2 rows must differ (cells centred one row off in x2)
   if (T.SY && c->g.nx2 < 2) return DK_EUNSUPPORTED;
   T.ntx = c->g.nx1/T.TX; T.nty = c->g.nx2/T.TY; T.ntz = c->g.nx3/T.TZ;
@@ -2299,6 +2717,8 @@ int FastBind(dk_ctx *c) {
   f->cur = 0;
   f->pending = 0;
   f->regs_dead = 0;
+  f->r_other = 0;
+  f->unmigrated = 0;
   int ea;
   if ((ea = AllocItems(c, std::max<long long>(f->cap, f->np)))) return ea;
   // (with no particles the passes below run over empty sets: they still write the cell
@@ -2322,10 +2742,12 @@ int FastExport(dk_ctx *c) {
   if (f == nullptr) return DK_ESTATE;
   int e;
   if ((e = Flush(c))) return e;
-  k_export<<<Grid(f->np), kBlock, 0, c->stream>>>(f->s[f->cur], f->s[1 - f->cur], f->perm,
+  KArgs A = MakeArgs(c);
+  A.gam0 = 1.0; A.gam1 = 0.0; A.beta_dt = 0.0;
+  k_export<<<Grid(f->np), kBlock, 0, c->stream>>>(A, f->s[f->cur], f->s[1 - f->cur], f->perm,
                                                    f->key, f->T, f->np, f->itag, f->np,
                                                    c->rdata, c->idata, (long long)c->cap,
-                                                   f->regs_dead);
+                                                   f->regs_dead, f->r_other, f->unmigrated);
   return dk::Check(hipGetLastError());
 }
 
@@ -2381,9 +2803,70 @@ int FastDiag(dk_ctx *c, double out[9]) {
   return e;
 }
 
+// One rk2 stage of the pc2 coupling (dust_tasks.cpp:58-109 with coupling = pc2).  Stage 1: u1 <- u0
+// and the gas velocity M/rho for the predictor, the gas update, the key pass routing the last
+// advance's positions (UpdateParticleGIDs of that stage, bvals_part.cpp:164-429, with x1 <- x,
+// v1 <- v, dust_push.cpp:65-77), the sort, the predictor deposit, its fold and the predictor's
+// application to the gas (dust_pm.cpp:387-402).  Stage 2: u* = (M1 + M0)/(rho1 + rho0), the gas
+// update, the key pass placing every particle by its half-step position, the sort, the advance
+// with its impulse deposit, the fold and the commit (dust_pm.cpp:596-601).
+static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
+  Fast *f = c->fast;
+  const Geo &g = c->g;
+  const Tiles &T = f->T;
+  const int s = st->stage - 1;
+  const bool s1 = (st->stage == 1);
+  int e;
+  KArgs A = MakeArgs(c);
+  A.pending = 0;
+  A.regs_live = 1;
+  A.stage1 = s1 ? 1 : 0;
+  // (the key pass's push with these coefficients leaves x and v as they are: it only routes)
+  A.gam0 = 1.0; A.gam1 = 0.0; A.beta_dt = 0.0; A.atw_dt = 0.0;
+  A.sh = 0.0; A.dt = st->dt;
+  A.a_dt = st->dt; A.inv_a_dt = 1.0/st->dt;
+  A.rate_dep = c->P.stiffness;
+  {
+    dk_gas_src z;
+    std::memset(&z, 0, sizeof(z));
+    TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
+        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
+        f->pres[f->pres_rd], 0, 0.0, A.stage1, 1, st->gam0[s], st->gam1[s],
+        st->beta[s]*st->dt, src ? 1 : 0, src ? *src : z, 0.0, f->ustar, s1 ? 1 : 2)));
+  }
+  if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
+  if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
+  TIMED("key", LaunchKey(c, A, 1, f->pres[1 - f->pres_rd], !s1));
+  TIMED("fix", (k_fix_key<<<Grid((long long)T.nt*T.TC), kBlock, 0, c->stream>>>(
+      A, 1, f->pres[1 - f->pres_rd])));
+  if ((e = SortKeys(c))) return e;
+  if ((e = BuildItems(c))) return e;
+  if (s1) {
+    TIMED("pc2_predictor", LaunchPP(c, A, f->ustar, f->scr_f));
+    TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
+        g, T, f->scr_f, f->qdep, nullptr, f->item_start, f->ov_scr, f->ov_def)));
+    if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
+    TIMED("pc2_apply", (k_pc2_apply<<<CellGrid(c), kBlock, 0, c->stream>>>(
+        g, c->u0, f->qdep, c->P.pc2_damping, st->dt)));
+  } else {
+    TIMED("pc2_advance", LaunchPA(c, A, f->ustar, f->scr_f, f->defect));
+    TIMED("combine", (k_combine<float, 3, double><<<CombineGrid(c), kBlock, 0, c->stream>>>(
+        g, T, f->scr_f, f->dmom, f->defect, f->item_start, f->ov_scr, f->ov_def)));
+    if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;
+    TIMED("pc2_commit", (k_pc2_commit<<<CellGrid(c), kBlock, 0, c->stream>>>(
+        g, c->u0, f->dmom, f->qdep, st->gam1[st->nexp_stages - 1])));
+  }
+  f->cur = 1 - f->cur;
+  f->regs_dead = 0;
+  f->r_other = s1 ? 1 : 0;
+  f->unmigrated = s1 ? 0 : 1;
+  return dk::Check(hipGetLastError());
+}
+
 int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   Fast *f = c->fast;
   if (f == nullptr || c->u0 == nullptr || f->item_start == nullptr) return DK_ESTATE;
+  if (c->P.coupling == DK_PC2) return FastStagePc2(c, st, src);
   if (!ActiveStage(c, st)) return DK_OK;   // the identity stage of imex2+
   const Geo &g = c->g;
   const Tiles &T = f->T;

@@ -135,16 +135,18 @@ def c5(n=128, nbx=2, ppc=8, steps=100, seed=1):
     return Workload("c5", mesh, blocks, _phys(), su, None, 1.0e-3, steps)
 
 
-def c4(n=64, nbx=4, ppc=4, steps=50, seed=1, clump_frac=0.25):
+def c4(n=64, nbx=4, ppc=4, steps=50, seed=1, clump_frac=0.25, clouds=True):
     """pc2 with same-level refinable clouds (CloudPredictorT / CloudAdvanceT) on the root level
-    of the SMR case; clumps make the per-block particle counts uneven."""
+    of the SMR case; clumps make the per-block particle counts uneven.  clouds=False: the same
+    case with point particles (dust_pc2_predictor / dust_pc2_advance)."""
     nx = n // nbx
     mesh, blocks = abi.uniform_mesh((nx, nx, nx), (nbx, nbx, nbx), ((0.0, 1.0),) * 3, ng=2)
     su = _setup(seed=seed, ppc=ppc, taus=(0.05,), eps=(1.0,), gas_noise=1e-3, dust_noise=1e-3,
                 nclump=16 if clump_frac > 0 else 0, clump_frac=clump_frac,
                 clump_sigma=2.0 / n)
-    return Workload("c4", mesh, blocks, _phys(coupling=abi.DK_PC2, solver=abi.DK_LOCAL,
-                                              clouds=1), su, None, 1.0e-3, steps)
+    return Workload("c4" if clouds else "c4p", mesh, blocks,
+                    _phys(coupling=abi.DK_PC2, solver=abi.DK_LOCAL, clouds=int(clouds)), su,
+                    None, 1.0e-3, steps)
 
 
 WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
