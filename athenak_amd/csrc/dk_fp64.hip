@@ -1639,7 +1639,9 @@ int StageGraph(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   }
   (void)hipGraphDestroy(g);
   if (r) return r;
-  return dk::Check(hipGraphLaunch(x, c->stream));
+  r = dk::Check(hipGraphLaunch(x, c->stream));
+  if (!r) ++c->graph_launches;
+  return r;
 }
 
 }  // namespace
@@ -1675,6 +1677,8 @@ int dk_set_graphs(dk_ctx *c, int32_t enable) {
   c->graphs = enable ? 1 : 0;
   return DK_OK;
 }
+
+int64_t dk_graph_launches(const dk_ctx *c) { return c ? c->graph_launches : 0; }
 
 // src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell).  The FP32 form
 // keeps its store sorted by (block, tile, cell) every stage, so it has nothing to do here.

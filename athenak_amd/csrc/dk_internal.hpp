@@ -123,6 +123,7 @@ struct dk_ctx {
   // the FP32 form (dk_fp32.hip)
   struct Fast *fast = nullptr;
   int graphs = 0;
+  long long graph_launches = 0;    // stage graphs launched (dk_graph_launches)
   hipGraphExec_t gexec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-kernel event timing
   struct Prof *prof = nullptr;

@@ -23,7 +23,7 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_error", "dk_field", "dk_strerror", "dk_setup_problem", "dk_prof_enable",
             "dk_prof_reset", "dk_prof_read", "dk_autotune", "dk_tune_config",
             "dk_set_solver", "dk_solver_stats",
-            "dk_set_tune_config"] +
+            "dk_set_tune_config", "dk_graph_launches"] +
            ["dk_" + t for t in TASKS])
 
 
@@ -53,6 +53,8 @@ def lib():
         L.dk_stage_run.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_gas_hook.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_set_graphs.argtypes = [P, C.c_int32]
+        L.dk_graph_launches.restype = C.c_int64
+        L.dk_graph_launches.argtypes = [P]
         L.dk_autotune.argtypes = [P, C.c_int32]
         L.dk_set_solver.argtypes = [P, C.c_double, C.c_double, C.c_int32, C.c_double,
                                     C.c_double, C.c_double, C.c_double]
@@ -101,6 +103,8 @@ class DustContext:
                  mesh.nx1 + 2 * mesh.ng)
         self.u0 = torch.zeros(shape, dtype=torch.float64, device=dev)
         self.u1 = torch.zeros(shape, dtype=torch.float64, device=dev)
+        # (torch filled the tensors on its current stream; the library works on self.stream)
+        torch.cuda.synchronize()
         h = C.c_void_p()
         e = L.dk_create(C.byref(mesh), blocks, C.byref(phys), int(form),
                         C.c_void_p(self.stream.cuda_stream), C.byref(h))
@@ -150,6 +154,7 @@ class DustContext:
         self.idata[:, :n] = torch.from_numpy(idata).to(self.idata.device)
         self.u0.copy_(torch.from_numpy(u0))
         self.u1.copy_(torch.from_numpy(u1 if u1 is not None else u0))
+        torch.cuda.synchronize()
         self.bind_particles(n)
 
     def setup(self, su):
@@ -160,7 +165,10 @@ class DustContext:
         self.npart = lib().dk_npart(self.h)
 
     def export(self):
+        """Writes the particles into rdata/idata; returns once they are there (the library's
+        stream synchronised, so that torch work on any stream may read them)."""
         self._check(lib().dk_export_particles(self.h), "dk_export_particles")
+        self.stream.synchronize()
 
     # ---- stepping ----
     def task(self, name, st):
@@ -182,6 +190,10 @@ class DustContext:
 
     def set_graphs(self, on):
         self._check(lib().dk_set_graphs(self.h, 1 if on else 0), "dk_set_graphs")
+
+    def graphs_captured(self):
+        """Stage graphs launched so far (0 on the null stream, where stages run eagerly)."""
+        return lib().dk_graph_launches(self.h)
 
     def autotune(self, on=True):
         """Tune the FP32 form's launch parameters online over the next stages (or take them

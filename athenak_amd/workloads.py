@@ -149,4 +149,15 @@ def c4(n=64, nbx=4, ppc=4, steps=50, seed=1, clump_frac=0.25, clouds=True):
                     None, 1.0e-3, steps)
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}
+def c4p(n=128, nbx=8, ppc=4, steps=50, seed=1, clump_frac=0.25):
+    """The c4 case with point particles (pc2 without clouds) at the SMR case's scale: 128^3 in
+    512 blocks of 16^3, clumps making the per-block counts uneven (8,388,608 particles).  The
+    clump centres reach dust-to-gas ratios of ~260 (chi = dt eps/t_s ~ 5), where the explicit pc2
+    predictor is unstable: the rate field and the reference's damping of stiff cells are on
+    (StiffnessActive, dust_pm.cpp:288-402)."""
+    w = c4(n=n, nbx=nbx, ppc=ppc, steps=steps, seed=seed, clump_frac=clump_frac, clouds=False)
+    w.phys.stiffness, w.phys.pc2_damping = 1, 1
+    return w
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c4p": c4p, "c5": c5}

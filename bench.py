@@ -5,7 +5,8 @@ A step is one imex2+ cycle (two working stages of push, deposit, dc1 solve, gath
 back-reaction apply, plus the harness's gas source-term update) over every particle of the
 workload, with all inputs resident in HBM.  The default workload is the unstratified shearing
 box of 256^3 cells in 64 blocks of 64^3 with 4 particles per cell (67,108,864 particles), St 0.1,
-eps 3, dc1.
+eps 3, dc1.  `--workload c4p`: a step is one rk2 cycle of the pc2 coupling (predictor, advance)
+on 128^3 cells in 512 blocks of 16^3 with clumps (8,388,608 point particles, stiff cells damped).
 
 Prints one JSON line: value = Mparticle-steps/s of the measured form, the roofline of the
 dominant kernel (algorithmic bytes per launch / measured average launch time, HIP events on the
@@ -26,8 +27,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes of the minimal-pass schedule per particle-step (ppc = particles per cell):
-# imex2+/dc1 point TSC, FP64: 648 + 1176/ppc; FP32 form (positions and u0 in FP64): 456 + 732/ppc
-def balg_step(form, ppc):
+# imex2+/dc1 point TSC, FP64: 648 + 1176/ppc; FP32 form (positions and u0 in FP64): 456 + 732/ppc;
+# pc2 point TSC (two particle passes per cycle), FP64: 280 + 384/ppc; FP32 form: 204 + 292/ppc
+def balg_step(form, ppc, pc2=False):
+    if pc2:
+        return (280.0 + 384.0 / ppc) if form == "fp64" else (204.0 + 292.0 / ppc)
     if form == "fp64":
         return 648.0 + 1176.0 / ppc
     return 456.0 + 732.0 / ppc
@@ -55,6 +59,25 @@ KERNEL_BYTES = {
     "center_mass": (5 * 8 + 4, 8),
     "part_update": (3 * 8 + 4, 0),
 }
+
+# pc2 (rk2) kernels, per form.  FP32: the key pass reads x, v and writes x, x1, v1 (stage 1: the
+# routing of the advanced positions with x1 <- x, v1 <- v; stage 2: the half-step placement with
+# x1 <- xh, v1 <- v0); the predictor reads x, v, ts, m (u = M/rho read, Q, g written per cell);
+# the advance reads x, v, ts, m and writes x, v, R (u* read, D (FP64) written per cell).
+# FP64: the predictor reads x, v, ts, m, gid (u1 read, qdep written); the advance reads x1, v1,
+# ts, m, gid and writes x, v, x1, R (u* read, dmom written).
+KERNEL_BYTES_PC2 = {
+    "fp32": {"key": (36 + 60, 0), "pc2_predictor": (44, 3 * 4 + 4 * 4),
+             "pc2_advance": (44 + 60, 3 * 4 + 3 * 8)},
+    "fp64": {"pc2_predictor": (8 * 8 + 4, 4 * 8 + 4 * 8),
+             "pc2_advance": (8 * 8 + 4 + 12 * 8, 3 * 8 + 3 * 8)},
+}
+
+
+def kernel_bytes(label, form, w):
+    if w.phys.coupling == 1 and label in KERNEL_BYTES_PC2[form]:
+        return KERNEL_BYTES_PC2[form][label]
+    return KERNEL_BYTES.get(label, (0, 0))
 
 
 def parse():
@@ -85,13 +108,16 @@ def make_workload(name, n=None):
 
 def describe(w):
     m = w.mesh
+    pc2 = w.phys.coupling == 1
+    solver = {0: "local", 1: "applya", 2: "dc1", 3: "dc2", 4: "pcg", 5: "adaptive"}[w.phys.solver]
+    scheme = "rk2/pc2" if pc2 else "imex2+/" + solver
     return {"workload": "%s: %dx%dx%d cells in %d blocks of %dx%dx%d, ppc %d, %d particles, "
-                        "imex2+/%s, dt %g" % (
+                        "%s, dt %g" % (
                             w.name, m.nx1 * m.nmbx1, m.nx2 * m.nmbx2, m.nx3 * m.nmbx3, m.nmb,
-                            m.nx1, m.nx2, m.nx3, w.ppc, w.npart,
-                            {0: "local", 1: "applya", 2: "dc1", 3: "dc2"}[w.phys.solver], w.dt),
+                            m.nx1, m.nx2, m.nx3, w.ppc, w.npart, scheme, w.dt),
             "particles": w.npart, "cells": w.ncells, "ppc": w.ppc, "blocks": m.nmb,
-            "shear_periodic_x1": bool(m.bc_x1), "solver": "dc1"}
+            "shear_periodic_x1": bool(m.bc_x1), "coupling": "pc2" if pc2 else "imex",
+            "solver": "local" if pc2 else solver}
 
 
 def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True):
@@ -103,7 +129,9 @@ def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True):
     from athenak_amd import abi
     from athenak_amd.lib import DustContext
     fid = abi.DK_FORM_FP64 if form == "fp64" else abi.DK_FORM_FP32
-    ctx = DustContext(w.mesh, w.blocks, w.phys, w.npart, form=fid)
+    # a stream of the library's own (default priority): graph capture needs a stream other than
+    # the null stream, which is torch's current stream here
+    ctx = DustContext(w.mesh, w.blocks, w.phys, w.npart, form=fid, stream=torch.cuda.Stream())
     ctx.setup(w.setup)
     if form == "fp64":
         # the reference sorts point particles by cell at cycle 0 and every 50 cycles
@@ -190,7 +218,13 @@ def cpu_baseline(name, budget_s):
     from oracle.pyoracle import Oracle
     full = make_workload(name)
     nx = full.mesh.nx1
-    w = workloads.WORKLOADS[name](n=nx, nbx=1) if name != "c1" else full
+    if name == "c1":
+        w = full
+    elif nx >= 32:
+        w = workloads.WORKLOADS[name](n=nx, nbx=1)
+    else:
+        # small blocks (c4, c4p: 16^3): the 64^3 root level in blocks of the same size
+        w = workloads.WORKLOADS[name](n=64, nbx=64 // nx)
     cores = min(16, os.cpu_count() or 1)
     o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16, threads=cores)
     o.setup(w.setup)
@@ -207,7 +241,7 @@ def cpu_baseline(name, budget_s):
     return {"value": round(rate, 4), "unit": "Mparticle-steps/s", "cores": cores, "kind": "port",
             "sample": "CPU FP64 oracle (reference kernels restated, OpenMP atomics) on %s physics "
                       "with %d blocks of %d^3 (%d particles), %d steps after 1 warm-up, %.1f s"
-                      % (name, w.mesh.nmb, nx, w.npart, steps, el)}
+                      % (name, w.mesh.nmb, w.mesh.nx1, w.npart, steps, el)}
 
 
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of
@@ -242,7 +276,7 @@ def measured_traffic(label, w, form):
 
 def kernel_roofline(r, w, form="fp32"):
     label, (tot, cnt) = max(r["prof"].items(), key=lambda kv: kv[1][0])
-    per_p, per_c = KERNEL_BYTES.get(label, (0, 0))
+    per_p, per_c = kernel_bytes(label, form, w)
     alg = per_p * w.npart + per_c * w.ncells
     avg_s = tot / cnt * 1e-3
     achieved = alg / avg_s / 1e9 if alg else None
@@ -298,7 +332,8 @@ def main():
     main_form = forms[0]
     r = res[main_form]
     ms_step, mps = rate(r, w, args.steps)
-    step_alg = balg_step(main_form, w.ppc)
+    pc2 = w.phys.coupling == 1
+    step_alg = balg_step(main_form, w.ppc, pc2)
     out = {
         "metric": "Mparticle-steps/s & %HBM peak, FP32+batched vs FP64 GPU at 1e-5 rel err",
         "value": round(mps, 3), "unit": "Mparticle-steps/s", "n_gpus": 1,
@@ -328,7 +363,8 @@ def main():
         nbad = int((dp > 1e-5).sum())
         out["fp64_form"] = {"value": round(mps64, 3), "ms_per_step": round(ms64, 3),
                             "step_roofline_frac": round(
-                                mps64 * 1e6 * balg_step("fp64", w.ppc) / 1e9 / HBM_PEAK_GBS, 4),
+                                mps64 * 1e6 * balg_step("fp64", w.ppc, pc2) / 1e9 /
+                                HBM_PEAK_GBS, 4),
                             "roofline": kernel_roofline(res["fp64"], w, "fp64"),
                             "momentum_drift": drift_or_none(w, res["fp64"]["drift"])}
         out["speedup_vs_fp64"] = round(mps / mps64, 3)

@@ -47,7 +47,7 @@ enum { DK_IDN = 0, DK_IM1 = 1, DK_IM2 = 2, DK_IM3 = 3 };
 enum { DK_NGP = 0, DK_CIC = 1, DK_TSC = 2 };
 /* src/dust/dust.hpp:74 */
 enum { DK_IMEX = 0, DK_PC2 = 1 };
-/* src/dust/dust.hpp:70 (pcg and adaptive are not provided: DK_EUNSUPPORTED) */
+/* src/dust/dust.hpp:70 (pcg and adaptive: DK_FORM_FP64 only) */
 enum { DK_LOCAL = 0, DK_APPLYA = 1, DK_DC1 = 2, DK_DC2 = 3, DK_PCG = 4, DK_ADAPTIVE = 5 };
 /* mesh boundaries handled by the drag path */
 enum { DK_X1_PERIODIC = 0, DK_X1_SHEAR = 1 };
@@ -58,7 +58,10 @@ enum {
   DK_FORM_FP64 = 0,   /* plain FP64: one kernel per reference kernel, atomic deposits,
                          operating directly on the caller's arrays */
   DK_FORM_FP32 = 1    /* FP32 bulk, FP64 positions and sums: cell-sorted particles,
-                         atomic-free deposits, fused stage passes (dk_stage_run only) */
+                         atomic-free deposits, fused stage passes (dk_stage_run only).
+                         imex with local/dc1/dc2 on any mesh of the FP64 form; pc2 with
+                         point particles on periodic meshes without the shearing box;
+                         otherwise dk_create returns DK_EUNSUPPORTED */
 };
 
 /* error codes */
@@ -226,6 +229,9 @@ int dk_solver_stats(dk_ctx *ctx, double out[7]);
  * index is updated in place on later calls).  Not while the caller's stream is capturing. */
 int dk_stage_run(dk_ctx *ctx, const dk_stage *st, const dk_gas_src *src);
 int dk_set_graphs(dk_ctx *ctx, int32_t enable);
+/* The number of stage graphs launched so far (no counterpart in the reference: it tells a
+ * caller whether its stages ran as graphs, which needs a stream other than the null stream). */
+int64_t dk_graph_launches(const dk_ctx *ctx);
 
 /* The autotuner (no counterpart in the reference; the "work-tile and block sizes picked by a
  * project-written autotuner" of the FP32 form).  mode 1: take this configuration's entry from

@@ -21,9 +21,13 @@ def say(msg):
     sys.stdout.flush()
 
 
-def fp32_ctx(w):
+def fp32_ctx(w, own_stream=False):
+    """(own_stream: on a stream of its own, default priority; graphs are captured only off the
+    null stream, which is torch's current stream here)"""
+    import torch
     from athenak_amd.lib import DustContext
-    g = DustContext(w.mesh, w.blocks, w.phys, w.npart + 16, form=abi.DK_FORM_FP32)
+    g = DustContext(w.mesh, w.blocks, w.phys, w.npart + 16, form=abi.DK_FORM_FP32,
+                    stream=torch.cuda.Stream() if own_stream else None)
     g.setup(w.setup)
     return g
 
@@ -278,9 +282,13 @@ def test_fp32_graph_replays_match_eager():
     # each stage captured into a graph and replayed (the executable graph updated every stage),
     # with eager work and host synchronisations between the replays: every step identical to
     # the eager run
-    w = workloads.c5(n=32, nbx=2, ppc=4)
-    w.setup.dust_noise = 0.5
-    ge, gg = fp32_ctx(w), fp32_ctx(w)
+    for w in (workloads.c5(n=32, nbx=2, ppc=4), workloads.c4(n=32, nbx=2, ppc=4, clouds=False)):
+        w.setup.dust_noise = 0.5
+        graph_replays(w)
+
+
+def graph_replays(w):
+    ge, gg = fp32_ctx(w), fp32_ctx(w, own_stream=True)
     gg.set_graphs(True)
     for step in range(4):
         ge.step(w.dt, w.src)
@@ -292,7 +300,8 @@ def test_fp32_graph_replays_match_eager():
         a, b = exported(ge), exported(gg)
         for x, y in zip(a, b):
             assert x.tobytes() == y.tobytes(), step
-        say("graph replay step %d identical" % step)
+        say("%s: graph replay step %d identical" % (w.name, step))
+    assert gg.graphs_captured() > 0
     ge.close()
     gg.close()
 

@@ -23,16 +23,21 @@ def say(msg):
     sys.stdout.flush()
 
 
-def make_pair(w, form=abi.DK_FORM_FP64):
+def make_pair(w, form=abi.DK_FORM_FP64, own_stream=False):
+    """(own_stream: the context on a stream of its own, default priority: graphs are captured
+    only off the null stream, which is torch's current stream here)"""
+    import torch
     from athenak_amd.lib import DustContext
     o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
     o.setup(w.setup)
-    g = DustContext(w.mesh, w.blocks, w.phys, w.npart + 16, form=form)
+    g = DustContext(w.mesh, w.blocks, w.phys, w.npart + 16, form=form,
+                    stream=torch.cuda.Stream() if own_stream else None)
     g.setup(w.setup)
     return o, g
 
 
 def gpu_state(g):
+    g.stream.synchronize()
     n = g.npart
     r = g.rdata[:, :n].cpu().numpy()
     i = g.idata[:, :n].cpu().numpy()
@@ -228,18 +233,22 @@ def test_halo_violation_is_reported_not_fatal():
 def test_graph_replays_of_the_fp64_form_match_the_oracle():
     # the FP64 form with every stage captured into a graph and replayed: still the oracle's
     # result (its deposits use atomics, so replays and eager runs agree to round-off)
-    w = workloads.c1(n=16, nb=2, ppc=2)
-    w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.3, 2e-3
-    w.name = "c1-graphs"
-    o, g = make_pair(w)
-    g.set_graphs(True)
-    for s in range(6):
-        o.step(w.dt, w.src)
-        g.step(w.dt, w.src)
-        g.check_device_error()
-        g.diag()                     # eager work and a host sync between the replays
-        compare(o, g, w, what="graph step %d" % s)
-    say("c1-graphs: 6 steps of graph replays match the oracle")
+    for coupling, solver in ((abi.DK_IMEX, abi.DK_DC1), (abi.DK_PC2, abi.DK_LOCAL)):
+        w = workloads.c1(n=16, nb=2, ppc=2, coupling=coupling, solver=solver)
+        w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.3, 2e-3
+        w.name = "c1-graphs-%s" % ("pc2" if coupling == abi.DK_PC2 else "imex")
+        o, g = make_pair(w, own_stream=True)
+        g.set_graphs(True)
+        for s in range(6):
+            o.step(w.dt, w.src)
+            g.step(w.dt, w.src)
+            g.check_device_error()
+            g.diag()                     # eager work and a host sync between the replays
+            compare(o, g, w, what="graph step %d" % s)
+        assert g.graphs_captured() > 0
+        say("%s: 6 steps of graph replays (%d graph launches) match the oracle"
+            % (w.name, g.graphs_captured()))
+        g.close()
 
 
 def test_clouds_pc2_particles_crossing_blocks():
