@@ -182,6 +182,8 @@ struct KArgs {
   double ysh, dsh;                  // shear crossing offsets (bvals_part.cpp, dust.cpp)
   int stage1, sbox, strat, shear;
   int rate_dep;                     // pc2: the predictor also deposits the rate m/(t_s V)
+  int cell_sums;                    // the key pass sums corrections and push residues per cell
+                                    // (0 with pc2, where both are zero)
   // the stage registers x1, v1 and R of this stage are read by a later stage (0 at the last
   // active stage: the next step's stage 1 sets them before any use, so they are not kept)
   int regs_live;
@@ -630,6 +632,125 @@ __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int s
   __syncthreads();
 }
 
+// Crowded cells.  The cell phase gives each cell's records to one thread per slab, in store
+// order; a clump cell holding hundreds of a chunk's records would serialise its workgroup.  When
+// a cell of the chunk holds more than kSplitMin records, its records are cut into runs of T (the
+// smallest power of two >= 16 for which the runs fit the idle threads and the record area), the
+// first run stays with the cell's own thread and every further run goes to the thread of an
+// empty cell (a helper, in cell order); the helpers' partial sums are then added to the owner's
+// in run order.  Every sum has a fixed order (the split depends only on the counts).
+constexpr int kSplitMin = 16;
+
+// the plan area of a tile kernel, in ints after st[TC + 1]: 8 counters, 2 wave totals, and
+// hcell, hsub, lhelp, hbase [TC] each
+__host__ __device__ inline int PlanInts(int TC) { return 12 + 4*TC; }
+
+// inclusive prefix sum over the 64 lanes of a wavefront
+__device__ inline int WaveInclScan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// Sums the chunk's records of this thread's cell l (chunk slots [s0, s1)) into acc (and ex),
+// splitting crowded cells over helper threads as above.  add(r, acc, ex) adds record r (chunk
+// index) to the accumulators.  area: the record area (free once every record is summed), of
+// area_words floats; plan: PlanInts(TC) ints.  All threads of the workgroup call it.
+template <int NC, int NE, class Add>
+__device__ inline void CellSums(const Tiles &Tl, const int *st, int c0, int c1, int l, int slab,
+                                float *area, int area_words, int *plan, const Add &add,
+                                float (&acc)[3][3][NC], float (&ex)[NE]) {
+  const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+  const int n = max(s1 - s0, 0);
+  if (!__syncthreads_or(n > kSplitMin)) {
+    for (int qq = s0; qq < s1; ++qq) add(qq - c0, acc, ex);
+    return;
+  }
+  const int TC = Tl.TC, tid = threadIdx.x;
+  int *cnt = plan;                 // [0, 7): helpers for T = 16 << i; [7]: empty cells
+  int *wtot = plan + 8;            // [2] wave totals of the scans
+  int *hcell = plan + 12, *hsub = hcell + TC, *lhelp = hsub + TC, *hbase = lhelp + TC;
+  if (tid < 8) cnt[tid] = 0;
+  __syncthreads();
+  if (slab == 0) {
+    for (int i = 0; i < 7; ++i) {
+      const int T = kSplitMin << i;
+      const int h = (n > 0) ? (n + T - 1)/T - 1 : 0;
+      if (h) atomicAdd(&cnt[i], h);
+    }
+    if (n == 0) atomicAdd(&cnt[7], 1);
+  }
+  __syncthreads();
+  const int slot_words = 9*NC + NE;
+  const int hmax = min(cnt[7], area_words/(3*slot_words));
+  int T = kSplitMin << 6, H = cnt[6];
+  for (int i = 5; i >= 0; --i)
+    if (cnt[i] <= hmax) {T = kSplitMin << i; H = cnt[i];}
+  if (H == 0 || H > hmax) {   // (T = 1024 exceeds any chunk: no helpers)
+    for (int qq = s0; qq < s1; ++qq) add(qq - c0, acc, ex);
+    return;
+  }
+  // helpers per cell and their first slot, empty cells' ranks (exclusive scans over the cells,
+  // run by the threads of the first ceil(TC/64) wavefronts)
+  const int nh = (n > 0) ? (n + T - 1)/T - 1 : 0;
+  int hb = 0, er = 0;
+  if (tid < ((TC + 63) & ~63)) {
+    const bool mine = tid < TC;
+    const int a = WaveInclScan(mine ? nh : 0), e = WaveInclScan((mine && n == 0) ? 1 : 0);
+    if ((tid & 63) == 63) wtot[tid >> 6] = a | (e << 16);
+    hb = a - (mine ? nh : 0);
+    er = e - ((mine && n == 0) ? 1 : 0);
+  }
+  __syncthreads();
+  if (tid >= 64 && tid < TC) {hb += wtot[0] & 0xffff; er += wtot[0] >> 16;}
+  if (slab == 0) {
+    hbase[l] = hb;
+    for (int k = 0; k < nh; ++k) {hcell[hb + k] = l; hsub[hb + k] = k + 1;}
+    lhelp[l] = (n == 0 && er < H) ? er : -1;
+  }
+  __syncthreads();
+  // this thread's run: the first of its own cell, or (a helper) run hsub of cell hcell
+  const int h = lhelp[l];
+  float hex[NE > 0 ? NE : 1];
+  for (int e = 0; e < NE; ++e) hex[e] = 0.0f;
+  if (h >= 0) {
+    const int L = hcell[h], k = hsub[h];
+    const int a0 = max(st[L], c0) + k*T, a1 = min(a0 + T, min(st[L + 1], c1));
+    for (int qq = a0; qq < a1; ++qq) add(qq - c0, acc, hex);
+  } else {
+    for (int qq = s0; qq < min(s0 + T, s1); ++qq) add(qq - c0, acc, ex);
+  }
+  __syncthreads();   // every record summed: the record area takes the helpers' partials
+  if (h >= 0) {
+    float *o = area + (slab*H + h)*slot_words;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {o[(b*3 + a)*NC + c] = acc[b][a][c]; acc[b][a][c] = 0.0f;}
+    for (int e = 0; e < NE; ++e) o[9*NC + e] = hex[e];
+  }
+  __syncthreads();
+  if (nh > 0) {
+    const int hb0 = hbase[l];
+    for (int k = 0; k < nh; ++k) {
+      const float *o = area + (slab*H + hb0 + k)*slot_words;
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[b][a][c] += o[(b*3 + a)*NC + c];
+      for (int e = 0; e < NE; ++e) ex[e] += o[9*NC + e];
+    }
+  }
+}
+
 // the tile region of a padded cell field, fetched into registers (up to kRegionPer values per
 // thread) so that its latency overlaps the LDS set-up, then stored into LDS
 constexpr int kRegionPer = 6;
@@ -926,7 +1047,7 @@ k_key(KArgs A, int do_key, double *pres) {
         rcode[tid] = code;
       }
       __syncthreads();
-      if (tid < Tl.TC) {
+      if (A.cell_sums && tid < Tl.TC) {
         double dcs[3][3];
         float prs[3];
         for (int k=0; k<9; ++k) dcs[k/3][k%3] = cacc[k*Tl.TC + tid];
@@ -1005,10 +1126,18 @@ __global__ void k_item_tile(Tiles Tl, const int *item_start, int *item_tile) {
 }
 
 // the class of every item slot (1: a tile on a block's x2 face; 0 beyond the last item)
+// (multi: the items of tiles of more than one part, where crowded cells are, instead)
 __global__ void k_item_flags(Tiles Tl, const int *item_start, const int *item_tile, long long n,
-                             int *flag) {
+                             int *flag, int multi) {
   const int nitem = item_start[Tl.nt];
-  GRID_STRIDE(i, n) flag[i] = (i < nitem && EdgeTile(Tl, item_tile[i])) ? 1 : 0;
+  GRID_STRIDE(i, n) {
+    int f = 0;
+    if (i < nitem) {
+      const int t = item_tile[i];
+      f = multi ? ((item_start[t + 1] - item_start[t] > 1) ? 1 : 0) : (EdgeTile(Tl, t) ? 1 : 0);
+    }
+    flag[i] = f;
+  }
 }
 
 // the two item lists in item order (e: the exclusive scan of the flags over n + 1 slots)
@@ -1168,6 +1297,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
     int *rcode = reinterpret_cast<int*>(rv + 4*CH);        // [CH] centre | (sj + 1) << 2
     float *reg = rd + RecArea(Tl, CH, 8, 4);               // [4][RN]
     int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
+    [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
     for (int i = tid; i < 4*Tl.RN; i += NT) reg[i] = 0.0f;
     __syncthreads();
@@ -1234,21 +1364,40 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-      for (int qq = s0; qq < s1; ++qq) {
-        const int r = qq - c0;
-        float wx[3], wy[3];
-        W3<S, float>(rd[r], wx);
-        W3<S, float>(rd[CH + r], wy);
-        const float wz = W1<S, float>(slab, rd[2*CH + r]);
-        const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
-        const int code = rcode[r];
-        AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
-        if (slab == 1) {
-          const int cen = code & 3;
-          q3[0] += (cen == 1) ? val[0] : 0.0f;
-          q3[1] += (cen == 2) ? val[0] : 0.0f;
-          q3[2] += (cen == 3) ? val[0] : 0.0f;
+      if constexpr (!SY && OV) {
+        const auto add = [&](int r, float (&ac)[3][3][4], float (&e)[3]) {
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
+          const int code = rcode[r];
+          AddSlab<float, 4, SY>(ac, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
+          if (slab == 1) {
+            const int cen = code & 3;
+            e[0] += (cen == 1) ? val[0] : 0.0f;
+            e[1] += (cen == 2) ? val[0] : 0.0f;
+            e[2] += (cen == 3) ? val[0] : 0.0f;
+          }
+        };
+        CellSums<4, 3>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 8, 4), plan, add, acc, q3);
+      } else {
+        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+        for (int qq = s0; qq < s1; ++qq) {
+          const int r = qq - c0;
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
+          const int code = rcode[r];
+          AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
+          if (slab == 1) {
+            const int cen = code & 3;
+            q3[0] += (cen == 1) ? val[0] : 0.0f;
+            q3[1] += (cen == 2) ? val[0] : 0.0f;
+            q3[2] += (cen == 3) ? val[0] : 0.0f;
+          }
         }
       }
       DepositSlabs<float, 4, SY>(Tl, reg, rd, l, slab, acc, spl, spdir);
@@ -1296,6 +1445,7 @@ k_p2(KArgs A, const float *us, float *scr) {
     float *rg = rd + 3*CH;                                 // [3][CH] (m c/V) G u*
     int *rcode = reinterpret_cast<int*>(rg + 3*CH);        // [CH] sj + 1 (SY only)
     int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, SY ? 7 : 6, 3));   // [TC + 1]
+    [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1358,15 +1508,28 @@ k_p2(KArgs A, const float *us, float *scr) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-      for (int qq = s0; qq < s1; ++qq) {
-        const int r = qq - c0;
-        float wx[3], wy[3];
-        W3<S, float>(rd[r], wx);
-        W3<S, float>(rd[CH + r], wy);
-        const float wz = W1<S, float>(slab, rd[2*CH + r]);
-        const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
-        AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+      if constexpr (!SY && OV) {
+        float nex[1] = {0.0f};
+        const auto add = [&](int r, float (&ac)[3][3][3], float (&)[1]) {
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
+          AddSlab<float, 3, SY>(ac, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+        };
+        CellSums<3, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 6, 3), plan, add, acc, nex);
+      } else {
+        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+        for (int qq = s0; qq < s1; ++qq) {
+          const int r = qq - c0;
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
+          AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+        }
       }
       DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
@@ -1405,6 +1568,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
     float *rv = rd + 3*CH;                                 // [3][CH] -m delta/V
     int *rcode = reinterpret_cast<int*>(rv + 3*CH);        // [CH] sj + 1 (SY only)
     int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, SY ? 7 : 6, 3));   // [TC + 1]
+    [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1480,15 +1644,28 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-      for (int qq = s0; qq < s1; ++qq) {
-        const int r = qq - c0;
-        float wx[3], wy[3];
-        W3<S, float>(rd[r], wx);
-        W3<S, float>(rd[CH + r], wy);
-        const float wz = W1<S, float>(slab, rd[2*CH + r]);
-        const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
-        AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+      if constexpr (!SY && OV) {
+        float nex[1] = {0.0f};
+        const auto add = [&](int r, float (&ac)[3][3][3], float (&)[1]) {
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
+          AddSlab<float, 3, SY>(ac, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+        };
+        CellSums<3, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 6, 3), plan, add, acc, nex);
+      } else {
+        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+        for (int qq = s0; qq < s1; ++qq) {
+          const int r = qq - c0;
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
+          AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+        }
       }
       DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
@@ -1539,11 +1716,12 @@ k_pp(KArgs A, const float *uv, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  const int nlist = A.item_start[Tl.nt];
+  const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
   for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
        li < nlist; li += gridDim.x) {
-    const int tile = A.item_tile[li];
-    const int part = li - A.item_start[tile];
+    const int it = A.ilist ? A.ilist[li] : li;
+    const int tile = A.item_tile[it];
+    const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
     const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
     float *dep = reinterpret_cast<float*>(lds_d);          // [4][RN] the deposit
@@ -1551,6 +1729,7 @@ k_pp(KArgs A, const float *uv, float *scr) {
     float *rd = uv_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
     float *rv = rd + 3*CH;                                 // [4][CH] rate, g
     int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 7, 4));   // [TC + 1]
+    [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, uv, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1625,15 +1804,28 @@ k_pp(KArgs A, const float *uv, float *scr) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-      for (int qq = s0; qq < s1; ++qq) {
-        const int r = qq - c0;
-        float wx[3], wy[3];
-        W3<S, float>(rd[r], wx);
-        W3<S, float>(rd[CH + r], wy);
-        const float wz = W1<S, float>(slab, rd[2*CH + r]);
-        const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
-        AddSlab<float, 4, false>(acc, spl, spdir, 0, wz, wy, wx, val);
+      if constexpr (OV) {
+        float nex[1] = {0.0f};
+        const auto add = [&](int r, float (&ac)[3][3][4], float (&)[1]) {
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
+          AddSlab<float, 4, false>(ac, spl, spdir, 0, wz, wy, wx, val);
+        };
+        CellSums<4, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 7, 4), plan, add, acc, nex);
+      } else {
+        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+        for (int qq = s0; qq < s1; ++qq) {
+          const int r = qq - c0;
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
+          AddSlab<float, 4, false>(acc, spl, spdir, 0, wz, wy, wx, val);
+        }
       }
       DepositSlabs<float, 4, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
@@ -1657,11 +1849,12 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
-  const int nlist = A.item_start[Tl.nt];
+  const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
   for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
        li < nlist; li += gridDim.x) {
-    const int tile = A.item_tile[li];
-    const int part = li - A.item_start[tile];
+    const int it = A.ilist ? A.ilist[li] : li;
+    const int tile = A.item_tile[it];
+    const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
     const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
     float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
@@ -1669,6 +1862,7 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
     float *rd = us_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
     float *rv = rd + 3*CH;                                 // [3][CH] -m (v - v0)/V
     int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 6, 3));   // [TC + 1]
+    [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
     RegionFetch<float> uf;
     uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
@@ -1762,15 +1956,28 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-      for (int qq = s0; qq < s1; ++qq) {
-        const int r = qq - c0;
-        float wx[3], wy[3];
-        W3<S, float>(rd[r], wx);
-        W3<S, float>(rd[CH + r], wy);
-        const float wz = W1<S, float>(slab, rd[2*CH + r]);
-        const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
-        AddSlab<float, 3, false>(acc, spl, spdir, 0, wz, wy, wx, val);
+      if constexpr (OV) {
+        float nex[1] = {0.0f};
+        const auto add = [&](int r, float (&ac)[3][3][3], float (&)[1]) {
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
+          AddSlab<float, 3, false>(ac, spl, spdir, 0, wz, wy, wx, val);
+        };
+        CellSums<3, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 6, 3), plan, add, acc, nex);
+      } else {
+        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
+        for (int qq = s0; qq < s1; ++qq) {
+          const int r = qq - c0;
+          float wx[3], wy[3];
+          W3<S, float>(rd[r], wx);
+          W3<S, float>(rd[CH + r], wy);
+          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
+          AddSlab<float, 3, false>(acc, spl, spdir, 0, wz, wy, wx, val);
+        }
       }
       DepositSlabs<float, 3, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
@@ -2257,7 +2464,7 @@ int FillV(dk_ctx *c, T *a, long long n, T v) {
 size_t LdsKey(const Tiles &T, int threads) {
   return (LdsKeyBase(T, threads) + 7)/8*8 + (size_t)T.TC*(9*sizeof(double) + 3*sizeof(float));
 }
-size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1); }
+size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1 + PlanInts(T.TC)); }
 // (sy: the x2-shift code, whose P2/P3 records carry the shift in one more word)
 size_t LdsP1(const Tiles &T, bool = true) {
   return sizeof(float)*(RecArea(T, kPPT*3*T.TC, 8, 4) + 4*T.RN) + LdsTail(T);
@@ -2300,24 +2507,26 @@ void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres, bool mid = f
     default: KERNEL<0, SYV, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
   }
 
-// The tile kernels P1-P3 run over their items in classes: without the shearing box one class,
-// every item; in the shearing box the tiles inside a block's x2 range (their records never sit
-// one row off in x2: the plain code, fewer registers) and the tiles on its x2 faces (the SY code),
-// each over its own list.  Both codes give the same sums for a tile inside the range.
-int TileClasses(const dk_ctx *c) { return c->fast->T.SY ? 2 : 1; }
+// The tile kernels P1-P3 run over their items in two classes, each over its own list.  Without
+// the shearing box: the items of single-part tiles (one workgroup each, the lean code) and the
+// items of tiles of several parts, where crowded cells are (the extra-parts launch, whose code
+// splits crowded cells: CellSums).  In the shearing box: the tiles inside a block's x2 range
+// (their records never sit one row off in x2: the plain code, fewer registers) and the tiles on
+// its x2 faces (the SY code).  Both codes give the same sums for a tile inside the range.
+int TileClasses(const dk_ctx *) { return 2; }
 
 KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
   const Fast *f = c->fast;
   const Tiles &T = f->T;
   KArgs A = A0;
+  A.ilist = f->ilist[cls]; A.ilist_n = f->ilist_n + cls;
   if (!T.SY) {
     sy = false;
-    A.ilist = nullptr; A.ilist_n = nullptr; A.nfirst = T.nt;
+    A.nfirst = (cls == 0) ? T.nt : 0;
     return A;
   }
   const int nedge = (T.nt/T.ntb)*T.ntz*T.ntx*((T.nty > 1) ? 2 : 1);
   sy = (cls == 1);
-  A.ilist = f->ilist[cls]; A.ilist_n = f->ilist_n + cls;
   A.nfirst = sy ? nedge : T.nt - nedge;
   return A;
 }
@@ -2326,8 +2535,13 @@ KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
   for (int cls = 0; cls < TileClasses(c); ++cls) { \
     bool sy; \
     const KArgs A = ClassArgs(c, A0, cls, sy); \
-    if (A.nfirst == 0) continue; \
     const size_t lds = LDS(T, sy); \
+    if (!T.SY) { \
+      if (cls == 0) {SWITCH_S(KERNEL, false, false, dim3(A.nfirst), ARGS)} \
+      else {SWITCH_S(KERNEL, false, true, OverGrid(c), ARGS)} \
+      continue; \
+    } \
+    if (A.nfirst == 0) continue; \
     if (sy) { \
       SWITCH_S(KERNEL, true, false, dim3(A.nfirst), ARGS) \
       SWITCH_S(KERNEL, true, true, OverGrid(c), ARGS) \
@@ -2364,19 +2578,26 @@ void LaunchP3(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *d
     default: KERNEL<0, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
   }
 
-void LaunchPP(dk_ctx *c, const KArgs &A, const float *uv, float *scr) {
+// (the two classes of items as for P1-P3 without the shearing box)
+void LaunchPP(dk_ctx *c, const KArgs &A0, const float *uv, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
   const size_t lds = LdsPP(T);
+  bool sy;
+  KArgs A = ClassArgs(c, A0, 0, sy);
   SWITCH_S2(k_pp, false, dim3(T.nt), (A, uv, scr))
+  A = ClassArgs(c, A0, 1, sy);
   SWITCH_S2(k_pp, true, OverGrid(c), (A, uv, scr))
 }
 
-void LaunchPA(dk_ctx *c, const KArgs &A, const float *us, float *scr, double *defect) {
+void LaunchPA(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
   const size_t lds = LdsPA(T);
+  bool sy;
+  KArgs A = ClassArgs(c, A0, 0, sy);
   SWITCH_S2(k_pa, false, dim3(T.nt), (A, us, scr, defect))
+  A = ClassArgs(c, A0, 1, sy);
   SWITCH_S2(k_pa, true, OverGrid(c), (A, us, scr, defect))
 }
 
@@ -2447,6 +2668,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
   A.ov_pr = f->ov_pr; A.ov_cnt = f->ov_cnt;
   A.ilist = nullptr; A.ilist_n = nullptr; A.nfirst = f->T.nt;
+  A.cell_sums = 1;
   return A;
 }
 
@@ -2483,9 +2705,9 @@ int BuildItems(dk_ctx *c) {
   if (e) return e;
   TIMED("items", (k_item_tile<<<Grid(T.nt), kBlock, 0, c->stream>>>(T, f->item_start,
                                                                      f->item_tile)));
-  if (T.SY) {
+  {
     TIMED("items", (k_item_flags<<<Grid(f->item_cap + 1), kBlock, 0, c->stream>>>(
-        T, f->item_start, f->item_tile, f->item_cap + 1, f->iflag)));
+        T, f->item_start, f->item_tile, f->item_cap + 1, f->iflag, T.SY ? 0 : 1)));
     TIMED("items", e = dk::Check(hipcub::DeviceScan::ExclusiveSum(
         f->lscan_tmp, f->lscan_bytes, f->iflag, f->iflag_scan, (int)(f->item_cap + 1),
         c->stream)));
@@ -2530,7 +2752,7 @@ int AllocItems(dk_ctx *c, long long n) {
       Alloc((void**)&f->ov_dc, sizeof(double)*9*(size_t)T.TC*ov) ||
       Alloc((void**)&f->ov_cnt, sizeof(int)*(size_t)T.TC*ov))
     return DK_ENOMEM;
-  if (T.SY) {
+  {
     if (f->ilist_n == nullptr && Alloc((void**)&f->ilist_n, sizeof(int)*2)) return DK_ENOMEM;
     if (Alloc((void**)&f->ilist[0], sizeof(int)*f->item_cap) ||
         Alloc((void**)&f->ilist[1], sizeof(int)*f->item_cap) ||
@@ -2826,6 +3048,7 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   A.sh = 0.0; A.dt = st->dt;
   A.a_dt = st->dt; A.inv_a_dt = 1.0/st->dt;
   A.rate_dep = c->P.stiffness;
+  A.cell_sums = 0;
   {
     dk_gas_src z;
     std::memset(&z, 0, sizeof(z));
