@@ -741,11 +741,14 @@ __device__ inline void CellSums(const Tiles &Tl, const int *st, int c0, int c1, 
     for (int k = 0; k < nh; ++k) {
       const float *o = area + (slab*H + hb0 + k)*slot_words;
 #pragma unroll
-      for (int b = 0; b < 3; ++b)
+      for (int b = 0; b < 3; ++b) {
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
           for (int c = 0; c < NC; ++c) acc[b][a][c] += o[(b*3 + a)*NC + c];
+        // (one row's loads at a time: the compiler would otherwise hoist all 9 NC of them)
+        asm volatile("" ::: "memory");
+      }
       for (int e = 0; e < NE; ++e) ex[e] += o[9*NC + e];
     }
   }
@@ -1275,7 +1278,7 @@ __global__ void __launch_bounds__(1024) k_bigsort(const int *start, int *perm, c
 // P1: the pushed particles (the key pass pushed them in place) gathered into the new order, and
 // the deposit of Q = sum m c W/V and P = sum m c W v/V (dust_pm.cpp:225-274) and of the centre
 // mass m c/V (dust_pm.cpp:660-667)
-template <int S, bool SY, bool OV>
+template <int S, bool SY, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   extern __shared__ double lds_d[];
@@ -1364,7 +1367,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      if constexpr (!SY && OV) {
+      if constexpr (SPLIT) {
         const auto add = [&](int r, float (&ac)[3][3][4], float (&e)[3]) {
           float wx[3], wy[3];
           W3<S, float>(rd[r], wx);
@@ -1422,7 +1425,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
 }
 
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
-template <int S, bool SY, bool OV>
+template <int S, bool SY, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_p2(KArgs A, const float *us, float *scr) {
   extern __shared__ double lds_d[];
@@ -1508,7 +1511,7 @@ k_p2(KArgs A, const float *us, float *scr) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      if constexpr (!SY && OV) {
+      if constexpr (SPLIT) {
         float nex[1] = {0.0f};
         const auto add = [&](int r, float (&ac)[3][3][3], float (&)[1]) {
           float wx[3], wy[3];
@@ -1546,7 +1549,7 @@ k_p2(KArgs A, const float *us, float *scr) {
 // cell: the FP64 field then holds exactly (to FP64 round-off) the particles' total impulse.
 // (REGS: the drag rate R is written, for a later stage; a template argument, as a run-time test
 // around the store costs the kernel registers)
-template <int S, bool SY, bool REGS, bool OV>
+template <int S, bool SY, bool REGS, bool OV, bool SPLIT = false>
 __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, double *defect) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1644,7 +1647,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      if constexpr (!SY && OV) {
+      if constexpr (SPLIT) {
         float nex[1] = {0.0f};
         const auto add = [&](int r, float (&ac)[3][3][3], float (&)[1]) {
           float wx[3], wy[3];
@@ -1693,16 +1696,16 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
   }
 }
 
-template <int S, bool SY, bool OV>
-__global__ void __launch_bounds__(kCellThreads, (SY || OV) ? 4 : 8)
+template <int S, bool SY, bool OV, bool SPLIT = false>
+__global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 8)
 k_p3(KArgs A, const float *us, float *scr, double *defect) {
-  P3Body<S, SY, true, OV>(A, us, scr, defect);
+  P3Body<S, SY, true, OV, SPLIT>(A, us, scr, defect);
 }
 
-template <int S, bool SY, bool OV>
-__global__ void __launch_bounds__(kCellThreads, (SY || OV) ? 4 : 8)
+template <int S, bool SY, bool OV, bool SPLIT = false>
+__global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 8)
 k_p3n(KArgs A, const float *us, float *scr, double *defect) {
-  P3Body<S, SY, false, OV>(A, us, scr, defect);
+  P3Body<S, SY, false, OV, SPLIT>(A, us, scr, defect);
 }
 
 // pc2 predictor (dust_pm.cpp:141-193): the particles (routed in place by the key pass) moved into
@@ -1710,7 +1713,7 @@ k_p3n(KArgs A, const float *us, float *scr, double *defect) {
 // and the deposit of the rate m/(t_s V) (A.rate_dep) and of g = -dt (m/(t_s V)) (u - v) with the
 // same weights: qdep components 0 (rate), 1-3 (g).  The registers x1, v1 and the rate R stay in
 // the source order (the next key pass and an export read them through this sort's permutation).
-template <int S, bool OV>
+template <int S, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_pp(KArgs A, const float *uv, float *scr) {
   extern __shared__ double lds_d[];
@@ -1804,7 +1807,7 @@ k_pp(KArgs A, const float *uv, float *scr) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      if constexpr (OV) {
+      if constexpr (SPLIT) {
         float nex[1] = {0.0f};
         const auto add = [&](int r, float (&ac)[3][3][4], float (&)[1]) {
           float wx[3], wy[3];
@@ -1843,7 +1846,7 @@ k_pp(KArgs A, const float *uv, float *scr) {
 // 226-244).  v - v0 = dt (u* - vh)/t_s in exact arithmetic; the impulse and R take the exact
 // difference of the FP32 velocities, so that the gas receives the dust's actual change.  As P3:
 // the impulse is deposited in FP32 and every tile's FP64 defect goes to the combine.
-template <int S, bool OV>
+template <int S, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_pa(KArgs A, const float *us, float *scr, double *defect) {
   extern __shared__ double lds_d[];
@@ -1956,7 +1959,7 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
           for (int bb=0; bb<3; ++bb) acc[bb][a][n] = 0.0f;
         }
       int spdir = 0;
-      if constexpr (OV) {
+      if constexpr (SPLIT) {
         float nex[1] = {0.0f};
         const auto add = [&](int r, float (&ac)[3][3][3], float (&)[1]) {
           float wx[3], wy[3];
@@ -2500,11 +2503,11 @@ void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres, bool mid = f
 }
 
 // a deposit kernel of the scheme of the context with the x2-shift code SYV
-#define SWITCH_S(KERNEL, SYV, OV, grid, ARGS) \
+#define SWITCH_S(KERNEL, SYV, OV, SPL, grid, ARGS) \
   switch (c->g.deposit) { \
-    case 2: KERNEL<2, SYV, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 1: KERNEL<1, SYV, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    default: KERNEL<0, SYV, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 2: KERNEL<2, SYV, OV, SPL><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 1: KERNEL<1, SYV, OV, SPL><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    default: KERNEL<0, SYV, OV, SPL><<<grid, blk, lds, c->stream>>> ARGS; break; \
   }
 
 // The tile kernels P1-P3 run over their items in two classes, each over its own list.  Without
@@ -2521,8 +2524,10 @@ KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
   KArgs A = A0;
   A.ilist = f->ilist[cls]; A.ilist_n = f->ilist_n + cls;
   if (!T.SY) {
+    // (class 1: one workgroup per item over a bound of its count, every multi-part tile having
+    // at least two parts and at most n/W extra ones; the surplus workgroups exit at once)
     sy = false;
-    A.nfirst = (cls == 0) ? T.nt : 0;
+    A.nfirst = (cls == 0) ? T.nt : static_cast<int>(2*f->ov_cap);
     return A;
   }
   const int nedge = (T.nt/T.ntb)*T.ntz*T.ntx*((T.nty > 1) ? 2 : 1);
@@ -2537,17 +2542,17 @@ KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
     const KArgs A = ClassArgs(c, A0, cls, sy); \
     const size_t lds = LDS(T, sy); \
     if (!T.SY) { \
-      if (cls == 0) {SWITCH_S(KERNEL, false, false, dim3(A.nfirst), ARGS)} \
-      else {SWITCH_S(KERNEL, false, true, OverGrid(c), ARGS)} \
+      if (cls == 0) {SWITCH_S(KERNEL, false, false, false, dim3(A.nfirst), ARGS)} \
+      else {SWITCH_S(KERNEL, false, false, true, dim3(A.nfirst), ARGS)} \
       continue; \
     } \
     if (A.nfirst == 0) continue; \
     if (sy) { \
-      SWITCH_S(KERNEL, true, false, dim3(A.nfirst), ARGS) \
-      SWITCH_S(KERNEL, true, true, OverGrid(c), ARGS) \
+      SWITCH_S(KERNEL, true, false, false, dim3(A.nfirst), ARGS) \
+      SWITCH_S(KERNEL, true, true, false, OverGrid(c), ARGS) \
     } else { \
-      SWITCH_S(KERNEL, false, false, dim3(A.nfirst), ARGS) \
-      SWITCH_S(KERNEL, false, true, OverGrid(c), ARGS) \
+      SWITCH_S(KERNEL, false, false, false, dim3(A.nfirst), ARGS) \
+      SWITCH_S(KERNEL, false, true, false, OverGrid(c), ARGS) \
     } \
   }
 
@@ -2571,11 +2576,11 @@ void LaunchP3(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *d
 }
 
 // the pc2 tile kernels (one class: no shearing box), first parts then the extra parts
-#define SWITCH_S2(KERNEL, OV, grid, ARGS) \
+#define SWITCH_S2(KERNEL, OV, SPL, grid, ARGS) \
   switch (c->g.deposit) { \
-    case 2: KERNEL<2, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    case 1: KERNEL<1, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
-    default: KERNEL<0, OV><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 2: KERNEL<2, OV, SPL><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    case 1: KERNEL<1, OV, SPL><<<grid, blk, lds, c->stream>>> ARGS; break; \
+    default: KERNEL<0, OV, SPL><<<grid, blk, lds, c->stream>>> ARGS; break; \
   }
 
 // (the two classes of items as for P1-P3 without the shearing box)
@@ -2585,9 +2590,9 @@ void LaunchPP(dk_ctx *c, const KArgs &A0, const float *uv, float *scr) {
   const size_t lds = LdsPP(T);
   bool sy;
   KArgs A = ClassArgs(c, A0, 0, sy);
-  SWITCH_S2(k_pp, false, dim3(T.nt), (A, uv, scr))
+  SWITCH_S2(k_pp, false, false, dim3(A.nfirst), (A, uv, scr))
   A = ClassArgs(c, A0, 1, sy);
-  SWITCH_S2(k_pp, true, OverGrid(c), (A, uv, scr))
+  SWITCH_S2(k_pp, false, true, dim3(A.nfirst), (A, uv, scr))
 }
 
 void LaunchPA(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect) {
@@ -2596,9 +2601,9 @@ void LaunchPA(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *d
   const size_t lds = LdsPA(T);
   bool sy;
   KArgs A = ClassArgs(c, A0, 0, sy);
-  SWITCH_S2(k_pa, false, dim3(T.nt), (A, us, scr, defect))
+  SWITCH_S2(k_pa, false, false, dim3(A.nfirst), (A, us, scr, defect))
   A = ClassArgs(c, A0, 1, sy);
-  SWITCH_S2(k_pa, true, OverGrid(c), (A, us, scr, defect))
+  SWITCH_S2(k_pa, false, true, dim3(A.nfirst), (A, us, scr, defect))
 }
 
 // the cell passes: x over the active cells of a block, y over the blocks
