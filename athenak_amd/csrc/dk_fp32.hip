@@ -2489,10 +2489,23 @@ dim3 OverGrid(const dk_ctx *c) {
   return dim3(static_cast<unsigned>(std::min<long long>(c->fast->ov_cap, kOverGrid)));
 }
 
-// (mid: the pc2 stage-2 form)
+KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy);
+
+// (mid: the pc2 stage-2 form).  Without the shearing box over the two item classes of the tile
+// kernels, one workgroup per item each (the lists of the last sort describe the present order);
+// in the shearing box over all items, the extra parts in a looping launch.
 void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres, bool mid = false) {
   const Fast *f = c->fast;
   const size_t lds = LdsKey(f->T, f->key_threads);
+  if (!f->T.SY) {
+    bool sy;
+    for (int cls = 0; cls < 2; ++cls) {
+      const KArgs Ac = ClassArgs(c, A, cls, sy);
+      if (mid) k_key<false, true><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
+      else k_key<false><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
+    }
+    return;
+  }
   if (mid) {
     k_key<false, true><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres);
     k_key<true, true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres);
