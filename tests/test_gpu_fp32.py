@@ -205,6 +205,30 @@ def test_fp32_pc2_bit_identical_reruns():
         assert a.tobytes() == b.tobytes()
 
 
+def test_fp32_crowded_cells_split_over_helpers():
+    # clumps a quarter cell wide: ~2000 particles in one cell, so that every chunk of a clump tile
+    # is one cell and its records are split over up to 47 helper threads (runs of 16); imex2+/dc1
+    # and pc2 against the oracle, and two runs bit-identical
+    for coupling in (abi.DK_IMEX, abi.DK_PC2):
+        w = workloads.c5(n=32, nbx=2, ppc=8)
+        w.setup.clump_sigma = 0.25 / 32
+        if coupling == abi.DK_PC2:
+            w.phys.coupling, w.phys.solver = abi.DK_PC2, abi.DK_LOCAL
+            w.phys.stiffness, w.phys.pc2_damping = 1, 1
+        w.name = "fp32-crowded-%s" % ("pc2" if coupling == abi.DK_PC2 else "imex")
+        g = check_against_oracle(w, 5)
+        a = exported(g)
+        g.close()
+        g = fp32_ctx(w)
+        for _ in range(5):
+            g.step(w.dt, w.src)
+        g.check_device_error()
+        b = exported(g)
+        g.close()
+        for x, y in zip(a, b):
+            assert x.tobytes() == y.tobytes()
+
+
 def test_fp32_bit_identical_reruns():
     w = workloads.c5(n=32, nbx=2, ppc=8)
     w.setup.dust_noise = 0.5
