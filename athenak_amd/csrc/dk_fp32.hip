@@ -182,6 +182,7 @@ struct KArgs {
   double ysh, dsh;                  // shear crossing offsets (bvals_part.cpp, dust.cpp)
   int stage1, sbox, strat, shear;
   int rate_dep;                     // pc2: the predictor also deposits the rate m/(t_s V)
+  int deal;                         // one-item launches deal their items to the XCDs (FirstItem)
   int cell_sums;                    // the key pass sums corrections and push residues per cell
                                     // (0 with pc2, where both are zero)
   // the stage registers x1, v1 and R of this stage are read by a later stage (0 at the last
@@ -461,6 +462,13 @@ __device__ inline int DealIndex(const Tiles &T, int n) {
   const int b = blockIdx.x;
   if (!T.xcd || n % 8 != 0) return b;
   return (b % 8)*(n/8) + b/8;
+}
+
+// the list index of a one-item workgroup: dealt to the XCDs in runs (DealIndex), except in a
+// launch whose grid only bounds its list (A.deal = 0: the items of multi-part tiles fill a
+// prefix of the grid, which runs of the grid would give to one XCD)
+__device__ inline int FirstItem(const Tiles &T, const KArgs &A) {
+  return A.deal ? DealIndex(T, A.nfirst) : static_cast<int>(blockIdx.x);
 }
 
 // a tile on an x2 face of its block (in the shearing box the only tiles whose records can sit
@@ -929,7 +937,7 @@ k_key(KArgs A, int do_key, double *pres) {
   // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
   // extra parts of full tiles)
   const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
-  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : FirstItem(Tl, A);
        li < nlist; li += gridDim.x) {
     const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
@@ -1288,7 +1296,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
   // extra parts of full tiles)
   const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
-  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : FirstItem(Tl, A);
        li < nlist; li += gridDim.x) {
     const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
@@ -1435,7 +1443,7 @@ k_p2(KArgs A, const float *us, float *scr) {
   // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
   // extra parts of full tiles)
   const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
-  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : FirstItem(Tl, A);
        li < nlist; li += gridDim.x) {
     const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
@@ -1558,7 +1566,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
   // nfirst workgroups, !OV), or every gridDim.x-th after those (OV: a smaller launch over the
   // extra parts of full tiles)
   const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
-  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : FirstItem(Tl, A);
        li < nlist; li += gridDim.x) {
     const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
@@ -1720,7 +1728,7 @@ k_pp(KArgs A, const float *uv, float *scr) {
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
   const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
-  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : FirstItem(Tl, A);
        li < nlist; li += gridDim.x) {
     const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
@@ -1853,7 +1861,7 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
   const int nlist = A.ilist ? *A.ilist_n : A.item_start[Tl.nt];
-  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : DealIndex(Tl, A.nfirst);
+  for (int li = OV ? A.nfirst + static_cast<int>(blockIdx.x) : FirstItem(Tl, A);
        li < nlist; li += gridDim.x) {
     const int it = A.ilist ? A.ilist[li] : li;
     const int tile = A.item_tile[it];
@@ -2541,6 +2549,7 @@ KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
     // at least two parts and at most n/W extra ones; the surplus workgroups exit at once)
     sy = false;
     A.nfirst = (cls == 0) ? T.nt : static_cast<int>(2*f->ov_cap);
+    A.deal = (cls == 0) ? 1 : 0;
     return A;
   }
   const int nedge = (T.nt/T.ntb)*T.ntz*T.ntx*((T.nty > 1) ? 2 : 1);
@@ -2686,6 +2695,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
   A.ov_pr = f->ov_pr; A.ov_cnt = f->ov_cnt;
   A.ilist = nullptr; A.ilist_n = nullptr; A.nfirst = f->T.nt;
+  A.deal = 1;
   A.cell_sums = 1;
   return A;
 }

@@ -92,6 +92,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-tune", action="store_true",
                     help="FP32 form: keep the default launch parameters (no autotuner steps)")
+    ap.add_argument("--tune-config", default=None, metavar="XCD,KT",
+                    help="FP32 form: fix the launch parameters the autotuner would choose (XCD "
+                         "dealing 0/1, key-pass workgroup size), e.g. for a profiled run")
     ap.add_argument("--prof-steps", type=int, default=5,
                     help="eager steps timed kernel by kernel after the timed steps (>= 1)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -120,7 +123,7 @@ def describe(w):
             "solver": "local" if pc2 else solver}
 
 
-def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True):
+def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True, tune_config=None):
     """Runs one form: (FP32) the online autotuner's steps, `warmup` steps, `steps` timed steps
     (FP32: every stage one HIP graph launch), then `prof_steps` eager steps timed kernel by kernel
     with HIP events on the library's stream (events cannot enter the graph).  The FP64 form takes
@@ -140,7 +143,11 @@ def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True):
     torch.cuda.synchronize()
     d0 = ctx.diag()
     tuned = None
-    if form == "fp32" and tune:
+    if form == "fp32" and tune_config is not None:
+        xcd, kt = (int(v) for v in tune_config.split(","))
+        ctx.set_tune_config(xcd, kt)
+        tuned = {"state": "fixed", "xcd_dealing": xcd, "key_threads": kt, "steps": 0}
+    elif form == "fp32" and tune:
         # the autotuner times its candidates on these steps (no choice changes a result)
         ctx.autotune(True)
         tune_steps = 0
@@ -327,7 +334,8 @@ def main():
                                                                      args.steps),
               file=sys.stderr, flush=True)
         res[f] = gpu_run(w, f, args.steps, args.warmup, tune_steps=tune_steps,
-                         prof_steps=max(1, args.prof_steps), tune=not args.no_tune)
+                         prof_steps=max(1, args.prof_steps), tune=not args.no_tune,
+                         tune_config=args.tune_config)
         tune_steps = res[f]["tune_steps"]
     main_form = forms[0]
     r = res[main_form]
