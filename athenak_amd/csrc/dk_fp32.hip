@@ -3097,6 +3097,7 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, f->qdep, nullptr, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
+    if (c->P.stiffness && (e = StiffnessRecord<float>(c, f->qdep, 4, 0, st->dt))) return e;
     TIMED("pc2_apply", (k_pc2_apply<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->u0, f->qdep, c->P.pc2_damping, st->dt)));
   } else {

@@ -502,6 +502,41 @@ __global__ void k_applya_cell(Geo g, const double *u0, const double *qdep, doubl
   }
 }
 
+// UpdateStiffness (dust_pm.cpp:288-342): the largest pc2 rate/rho over the active cells (q: the
+// rate component comp of an nv-component deposit field), one partial per workgroup
+template <typename T>
+__global__ void k_stiff_partial(Geo g, const T *q, int nv, int comp, const double *u0,
+                                double *partial) {
+  __shared__ double sh[kBlock];
+  double mx = 0.0;
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    mx = fmax(mx, static_cast<double>(q[Idx(g,m,nv,comp,k,j,i)])/u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)]);
+  }
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int h = kBlock/2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// chi = dt max, and the running maximum (rec[0], rec[1]; dust_pm.cpp:315-320)
+__global__ void k_stiff_record(const double *partial, int nblk, double dt, double *rec) {
+  if (threadIdx.x != 0) return;
+  double r = 0.0;
+  for (int b=0; b<nblk; ++b) r = fmax(r, partial[b]);
+  const double chi = dt*r;
+  rec[0] = chi;
+  rec[1] = (rec[1] < chi) ? chi : rec[1];
+}
+
 // dust_pm.cpp:387-402 (dust_pc2_apply_predictor)
 __global__ void k_pc2_apply_predictor(Geo g, double *u0, double *qdep, int damp, double dt) {
   const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
@@ -1170,6 +1205,18 @@ int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   return Launched(c);
 }
 
+template <typename T>
+int StiffnessRecord(dk_ctx *c, const T *q, int nv, int comp, double dt) {
+  const int t = ProfBegin(c, "stiffness");
+  k_stiff_partial<T><<<kDiagGrid, kBlock, 0, c->stream>>>(c->g, q, nv, comp, c->u0,
+                                                          c->d_partial);
+  k_stiff_record<<<1, 64, 0, c->stream>>>(c->d_partial, kDiagGrid, dt, c->d_scalar + 14);
+  ProfEnd(c, t);
+  return Check(hipGetLastError());
+}
+template int StiffnessRecord<float>(dk_ctx *, const float *, int, int, double);
+template int StiffnessRecord<double>(dk_ctx *, const double *, int, int, double);
+
 }  // namespace dk
 
 //========================================================================================
@@ -1469,6 +1516,7 @@ int dk_gas_implicit_solve(dk_ctx *c, const dk_stage *st) {
   if (e || !ActiveStage(c, st)) return e;
   if (c->P.coupling == DK_PC2) {
     if (st->stage == 1 && c->P.back_reaction) {
+      if (c->P.stiffness && (e = dk::StiffnessRecord<double>(c, c->qdep, 5, 4, st->dt))) return e;
       {const int _pt = dk::ProfBegin(c, "pc2_apply_predictor");
       k_pc2_apply_predictor<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, c->P.pc2_damping,
                                                        st->dt);
@@ -1767,6 +1815,15 @@ int dk_diag_sums(dk_ctx *c, double out[9]) {
   k_reduce_partials<<<1, 64, 0, c->stream>>>(c->d_partial, kDiagGrid, 9, c->d_scalar);
   int e = dk::Check(hipMemcpyAsync(out, c->d_scalar, sizeof(double)*9, hipMemcpyDeviceToHost,
                                    c->stream));
+  if (!e) e = dk::Check(hipStreamSynchronize(c->stream));
+  return e;
+}
+
+// the UpdateStiffness records of the last pc2 cycle (dust_pm.cpp:288-342), both forms
+int dk_stiffness(dk_ctx *c, double out[2]) {
+  if (c == nullptr || out == nullptr) return DK_EINVAL;
+  int e = dk::Check(hipMemcpyAsync(out, c->d_scalar + 14, sizeof(double)*2,
+                                   hipMemcpyDeviceToHost, c->stream));
   if (!e) e = dk::Check(hipStreamSynchronize(c->stream));
   return e;
 }

@@ -23,7 +23,7 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_error", "dk_field", "dk_strerror", "dk_setup_problem", "dk_prof_enable",
             "dk_prof_reset", "dk_prof_read", "dk_autotune", "dk_tune_config",
             "dk_set_solver", "dk_solver_stats",
-            "dk_set_tune_config", "dk_graph_launches"] +
+            "dk_set_tune_config", "dk_graph_launches", "dk_stiffness"] +
            ["dk_" + t for t in TASKS])
 
 
@@ -53,6 +53,7 @@ def lib():
         L.dk_stage_run.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_gas_hook.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_set_graphs.argtypes = [P, C.c_int32]
+        L.dk_stiffness.argtypes = [P, C.POINTER(C.c_double)]
         L.dk_graph_launches.restype = C.c_int64
         L.dk_graph_launches.argtypes = [P]
         L.dk_autotune.argtypes = [P, C.c_int32]
@@ -190,6 +191,13 @@ class DustContext:
 
     def set_graphs(self, on):
         self._check(lib().dk_set_graphs(self.h, 1 if on else 0), "dk_set_graphs")
+
+    def stiffness(self):
+        """UpdateStiffness records: chi of the last pc2 cycle and its running maximum."""
+        import numpy as np
+        out = (C.c_double * 2)()
+        self._check(lib().dk_stiffness(self.h, out), "dk_stiffness")
+        return np.array(out[:])
 
     def graphs_captured(self):
         """Stage graphs launched so far (0 on the null stream, where stages run eagerly)."""

@@ -269,6 +269,11 @@ int dk_prof_enable(dk_ctx *ctx, int32_t on);
 int dk_prof_reset(dk_ctx *ctx);
 int dk_prof_read(dk_ctx *ctx, int32_t idx, char *label, int32_t label_len, double *ms,
                  int64_t *count);
+/* UpdateStiffness / SyncStiffness (src/dust/dust_pm.cpp:288-359; dust.hpp stiff_chi_last,
+ * stiff_chi_run_max): with the stiffness field on (dk_physics.stiffness) every pc2 cycle records
+ * chi = dt max_cells qdep[4]/rho before the predictor is applied; out[0] = chi of the last
+ * cycle, out[1] = the running maximum (one rank: no reduction over ranks).  Both forms. */
+int dk_stiffness(dk_ctx *ctx, double out[2]);
 /* The first device-side error (code, particle index); clears it. */
 int dk_error(dk_ctx *ctx, int32_t *code, int64_t *particle);
 /* Device pointer to a work field: 0 qdep(5), 1 ustar(3), 2 dmom(3), 3 solver ap(3),

@@ -115,6 +115,8 @@ struct or_sim {
   double adaptive_order_c = 0.25, adaptive_rtol_max = 1.0e-3, adaptive_tref = 1.0,
          adaptive_state_floor = 1.0;
   double solver_last[7] = {0, 0, -1, -1, -1, -1, -1};
+  // UpdateStiffness records (dust_pm.cpp:288-342): chi of the last pc2 cycle, its running max
+  double stiff_chi_last = 0.0, stiff_chi_run_max = 0.0;
   std::vector<double> dx1, dx2, dx3;   // per block (meshblock.cpp:113-118)
   int nthreads = 1;
   int err = 0;
@@ -1671,6 +1673,19 @@ int or_gas_implicit_solve(or_sim *s, const dk_stage *st) {
     if (st->stage == 1 && s->P.back_reaction) {
       const bool damp = s->P.pc2_damping;
       const double dt = st->dt;
+      // UpdateStiffness (dust_pm.cpp:288-342): chi = dt max_cells qdep[4]/rho over the active
+      // cells, before the predictor is applied; the running maximum kept beside it
+      if (s->P.stiffness) {
+        double rate = 0.0;
+        for (int m=0; m<nmb; ++m)
+          for (int k=s->ks; k<=s->ke; ++k)
+            for (int j=s->js; j<=s->je; ++j)
+              for (int i=s->is; i<=s->ie; ++i)
+                rate = std::fmax(rate, s->qdep[s->Idx(m,5,4,k,j,i)]/
+                                       s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)]);
+        s->stiff_chi_last = dt*rate;
+        s->stiff_chi_run_max = std::max(s->stiff_chi_run_max, s->stiff_chi_last);
+      }
 #pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
       for (int m=0; m<nmb; ++m)
         for (int k=s->ks; k<=s->ke; ++k)
@@ -1750,6 +1765,11 @@ void or_set_solver(or_sim *s, double rtol, double atol, int iter_max, double ord
   s->drag_rtol = rtol; s->drag_atol = atol; s->drag_iter_max = iter_max;
   s->adaptive_order_c = order_c; s->adaptive_rtol_max = rtol_max; s->adaptive_tref = tref;
   s->adaptive_state_floor = state_floor;
+}
+
+void or_stiffness(or_sim *s, double out[2]) {
+  out[0] = s->stiff_chi_last;
+  out[1] = s->stiff_chi_run_max;
 }
 
 void or_solver_stats(or_sim *s, double out[7]) {

@@ -59,6 +59,7 @@ def lib():
         L.or_set_solver.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_double,
                                     C.c_double, C.c_double, C.c_double]
         L.or_solver_stats.argtypes = [P, D]
+        L.or_stiffness.argtypes = [P, D]
         _LIB = L
     return _LIB
 
@@ -178,6 +179,12 @@ class Oracle:
         acceptance target, state scale, max Q/rho (-1 where not formed)."""
         out = (C.c_double * 7)()
         lib().or_solver_stats(self.h, out)
+        return np.array(out[:])
+
+    def stiffness(self):
+        """UpdateStiffness records: chi of the last pc2 cycle and its running maximum."""
+        out = (C.c_double * 2)()
+        lib().or_stiffness(self.h, out)
         return np.array(out[:])
 
     def apply_operator(self, a_dt, x, sh=0.0):

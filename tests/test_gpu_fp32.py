@@ -450,3 +450,29 @@ def test_fp32_new_time_step_and_sort():
     g.sort()
     g.check_device_error()
     g.close()
+
+
+def test_pc2_stiffness_record_both_forms():
+    # UpdateStiffness (dust_pm.cpp:288-342): chi of the last cycle and its running maximum, in
+    # the FP64 form to the oracle's round-off and in the FP32 form to FP32 round-off
+    from athenak_amd.lib import DustContext
+    w = workloads.c1(n=16, nb=2, ppc=2, coupling=abi.DK_PC2, solver=abi.DK_LOCAL)
+    w.phys.stiffness, w.phys.pc2_damping = 1, 1
+    w.setup.taus[0], w.setup.dust_noise, w.dt = 2e-3, 0.5, 4e-3
+    o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
+    o.setup(w.setup)
+    g64 = DustContext(w.mesh, w.blocks, w.phys, w.npart + 16, form=abi.DK_FORM_FP64)
+    g64.setup(w.setup)
+    g32 = fp32_ctx(w)
+    for step in range(3):
+        for x in (o, g64, g32):
+            x.step(w.dt, w.src)
+        so, s64, s32 = o.stiffness(), g64.stiffness(), g32.stiffness()
+        say("stiffness step %d: oracle %s, fp64 %s, fp32 %s" % (step, so, s64, s32))
+        assert so[0] > 1.0
+        assert np.all(np.abs(s64 - so) <= 1e-12 * so), (s64, so)
+        assert np.all(np.abs(s32 - so) <= 1e-5 * so), (s32, so)
+    g64.check_device_error()
+    g32.check_device_error()
+    g64.close()
+    g32.close()

@@ -115,3 +115,30 @@ def test_adaptive_falls_back_to_pcg_and_meets_the_tolerance():
 def test_pcg_budget_too_small_is_reported():
     with pytest.raises(RuntimeError, match="drag solver failed"):
         stage_up_to_solve(abi.DK_PCG, rtol=1e-16, atol=0.0, iter_max=1)
+
+
+def test_pc2_stiffness_record_is_the_max_rate_over_rho():
+    # UpdateStiffness (dust_pm.cpp:288-342): chi = dt max over the active cells of qdep[4]/rho,
+    # taken before the predictor is applied (rho is not changed by the cycle: zero fluxes), and
+    # its running maximum; the stiffness field off records nothing
+    w = workloads.c1(n=16, nb=2, ppc=2, coupling=abi.DK_PC2, solver=abi.DK_LOCAL)
+    w.phys.stiffness, w.phys.pc2_damping = 1, 1
+    w.setup.taus[0], w.setup.dust_noise, w.dt = 2e-3, 0.5, 4e-3
+    o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
+    o.setup(w.setup)
+    ng = w.mesh.ng
+    chis = []
+    for _ in range(3):
+        o.step(w.dt, w.src)
+        q4 = active(o.field(0), ng)[:, 4]
+        rho = active(o.u0(), ng)[:, 0]
+        chis.append(w.dt * (q4 / rho).max())
+        s = o.stiffness()
+        assert s[0] == chis[-1], (s, chis)
+        assert s[1] == max(chis), (s, chis)
+    assert chis[0] > 1.0          # (the damping case: stiff cells exist)
+    w.phys.stiffness = 0
+    o0 = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
+    o0.setup(w.setup)
+    o0.step(w.dt, w.src)
+    assert np.array_equal(o0.stiffness(), [0.0, 0.0])
