@@ -148,6 +148,15 @@ def imex2p_stages(time, dt, gamma=IMEX2P_GAMMA):
     return out
 
 
+def gamma_switch(dt, taus_max, coupling=DK_IMEX):
+    """GammaSwitch (src/dust/dust_newdt.cpp:160-173, Krapp et al. 2024 Eq. 18): the imex2+
+    diagonal coefficient 1/2 when dt exceeds the largest stopping time (stiff regime), else
+    1 + 1/sqrt(2); pc2 has none (None)."""
+    if coupling == DK_PC2:
+        return None
+    return 0.5 if dt > taus_max else IMEX2P_GAMMA
+
+
 def rk2_stages(time, dt):
     """Heun / SSPRK(2,2) (src/driver/driver.cpp:106-118) as two dk_stage records."""
     out = []

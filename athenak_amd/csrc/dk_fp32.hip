@@ -3001,6 +3001,32 @@ int FastExport(dk_ctx *c) {
   return dk::Check(hipGetLastError());
 }
 
+// RefreshStoppingTimeMaximum over the store (dust_newdt.cpp:34-98, as the plain form's
+// k_stop_partial): per workgroup the largest and smallest (FP32) stopping time, an invalid one
+// recorded
+__global__ void k_fstop(Store S, long long np, const int32_t *itag, double *partial,
+                        dk::ErrWord *err) {
+  __shared__ double smax[kBlock], smin[kBlock];
+  double mx = 0.0, mn = 3.4028234663852886e38;
+  GRID_STRIDE(q, np) {
+    const double ts = static_cast<double>(S.TS(q));
+    const int sp = itag[(long long)(DK_PSP - 1)*np + S.ORIG(q)];
+    if (!isfinite(ts) || !(ts > 0.0) || sp < 0) {dk::RecordError(err, DK_ESTOP, q); continue;}
+    mx = fmax(mx, ts);
+    mn = fmin(mn, ts);
+  }
+  smax[threadIdx.x] = mx; smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int h = kBlock/2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + h]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {partial[2*blockIdx.x] = smax[0]; partial[2*blockIdx.x + 1] = smin[0];}
+}
+
 // NewTimeStep over the store (dust_newdt.cpp:135-145, as the plain form's k_newdt): min over
 // particles of dx/max(|v|, 1e-30), the azimuthal velocity without the background shear in the
 // 3-D shearing box, every stopping time validated (dust_newdt.cpp:34-99); the block of slot q
@@ -3029,6 +3055,13 @@ __global__ void k_fnewdt(const DevBlock *B, Store S, const int *perm, const int 
 
 // the smallest particle time step over the store, after the pending correction (the velocities a
 // step boundary holds); partial receives nb per-workgroup minima
+int FastStoppingTimes(dk_ctx *c, double *partial, int nb) {
+  Fast *f = c->fast;
+  if (f == nullptr) return DK_ESTATE;
+  k_fstop<<<nb, kBlock, 0, c->stream>>>(f->s[f->cur], f->np, f->itag, partial, c->d_err);
+  return dk::Check(hipGetLastError());
+}
+
 int FastNewDt(dk_ctx *c, double qo, double *partial, int nb) {
   Fast *f = c->fast;
   if (f == nullptr || c->u0 == nullptr) return DK_ESTATE;

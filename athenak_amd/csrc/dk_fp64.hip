@@ -527,7 +527,7 @@ __global__ void k_stiff_partial(Geo g, const T *q, int nv, int comp, const doubl
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-// chi = dt max, and the running maximum (rec[0], rec[1]; dust_pm.cpp:315-320)
+// chi = dt max, the running maximum and the rate maximum (rec[0..2]; dust_pm.cpp:315-331)
 __global__ void k_stiff_record(const double *partial, int nblk, double dt, double *rec) {
   if (threadIdx.x != 0) return;
   double r = 0.0;
@@ -535,6 +535,31 @@ __global__ void k_stiff_record(const double *partial, int nblk, double dt, doubl
   const double chi = dt*r;
   rec[0] = chi;
   rec[1] = (rec[1] < chi) ? chi : rec[1];
+  rec[2] = r;
+}
+
+// RefreshStoppingTimeMaximum (dust_newdt.cpp:34-98): per workgroup the largest and smallest
+// stopping time, an invalid one (not finite, not positive, species index negative) recorded
+__global__ void k_stop_partial(const double *r, const int32_t *iv, long long cap, long long np,
+                               double *partial, dk::ErrWord *err) {
+  __shared__ double smax[kBlock], smin[kBlock];
+  double mx = 0.0, mn = 3.4028234663852886e38;
+  GRID_STRIDE(p, np) {
+    const double ts = PR(DK_IPTS,p);
+    if (!isfinite(ts) || !(ts > 0.0) || PI(DK_PSP,p) < 0) {dk::RecordError(err, DK_ESTOP, p); continue;}
+    mx = fmax(mx, ts);
+    mn = fmin(mn, ts);
+  }
+  smax[threadIdx.x] = mx; smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int h = kBlock/2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + h]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {partial[2*blockIdx.x] = smax[0]; partial[2*blockIdx.x + 1] = smin[0];}
 }
 
 // dust_pm.cpp:387-402 (dust_pc2_apply_predictor)
@@ -1210,7 +1235,7 @@ int StiffnessRecord(dk_ctx *c, const T *q, int nv, int comp, double dt) {
   const int t = ProfBegin(c, "stiffness");
   k_stiff_partial<T><<<kDiagGrid, kBlock, 0, c->stream>>>(c->g, q, nv, comp, c->u0,
                                                           c->d_partial);
-  k_stiff_record<<<1, 64, 0, c->stream>>>(c->d_partial, kDiagGrid, dt, c->d_scalar + 14);
+  k_stiff_record<<<1, 64, 0, c->stream>>>(c->d_partial, kDiagGrid, dt, c->d_scalar + 9);
   ProfEnd(c, t);
   return Check(hipGetLastError());
 }
@@ -1820,12 +1845,40 @@ int dk_diag_sums(dk_ctx *c, double out[9]) {
 }
 
 // the UpdateStiffness records of the last pc2 cycle (dust_pm.cpp:288-342), both forms
-int dk_stiffness(dk_ctx *c, double out[2]) {
+int dk_stiffness(dk_ctx *c, double out[3]) {
   if (c == nullptr || out == nullptr) return DK_EINVAL;
-  int e = dk::Check(hipMemcpyAsync(out, c->d_scalar + 14, sizeof(double)*2,
+  int e = dk::Check(hipMemcpyAsync(out, c->d_scalar + 9, sizeof(double)*3,
                                    hipMemcpyDeviceToHost, c->stream));
   if (!e) e = dk::Check(hipStreamSynchronize(c->stream));
   return e;
+}
+
+// RefreshStoppingTimeMaximum (src/dust/dust_newdt.cpp:34-98), both forms
+int dk_stopping_times(dk_ctx *c, double out[2]) {
+  if (c == nullptr || out == nullptr) return DK_EINVAL;
+  int e = DK_OK;
+  if (c->form == DK_FORM_FP32) {
+    if ((e = dk::FastStoppingTimes(c, c->d_partial, kDiagGrid))) return e;
+  } else {
+    if ((e = Ready(c))) return e;
+    k_stop_partial<<<kDiagGrid, kBlock, 0, c->stream>>>(c->rdata, c->idata, (long long)c->cap,
+                                                        (long long)c->np, c->d_partial,
+                                                        c->d_err);
+    if ((e = dk::Check(hipGetLastError()))) return e;
+  }
+  std::vector<double> h(2*kDiagGrid);
+  if ((e = dk::Check(hipMemcpyAsync(h.data(), c->d_partial, sizeof(double)*h.size(),
+                                    hipMemcpyDeviceToHost, c->stream)))) return e;
+  if ((e = dk::Check(hipStreamSynchronize(c->stream)))) return e;
+  int32_t code = 0;
+  int64_t part = -1;
+  if ((e = dk_error(c, &code, &part))) return e;
+  if (code == DK_ESTOP) return DK_ESTOP;
+  double mx = 0.0, mn = 3.4028234663852886e38;
+  for (int b=0; b<kDiagGrid; ++b) {mx = std::max(mx, h[2*b]); mn = std::min(mn, h[2*b + 1]);}
+  out[0] = mx;
+  out[1] = mn;
+  return DK_OK;
 }
 
 int dk_error(dk_ctx *c, int32_t *code, int64_t *particle) {

@@ -160,8 +160,11 @@ int FastExport(dk_ctx *c);
 int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);
 int FastDiag(dk_ctx *c, double out[9]);
 int FastNewDt(dk_ctx *c, double qo, double *partial, int nb);
+// RefreshStoppingTimeMaximum over the FP32 store: nb partial (max, min) pairs
+int FastStoppingTimes(dk_ctx *c, double *partial, int nb);
 // UpdateStiffness (dust_pm.cpp:288-342): chi = dt max_cells q[comp]/rho and its running maximum,
-// kept on the device (d_scalar[14], [15]); q: the pc2 rate component of a deposit field
+// and the rate maximum, kept on the device (d_scalar[9..11]); q: the pc2 rate component of a
+// deposit field
 template <typename T> int StiffnessRecord(dk_ctx *c, const T *q, int nv, int comp, double dt);
 
 // dk_fp64.hip pieces reused by the FP32 form

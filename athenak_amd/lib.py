@@ -23,7 +23,8 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_error", "dk_field", "dk_strerror", "dk_setup_problem", "dk_prof_enable",
             "dk_prof_reset", "dk_prof_read", "dk_autotune", "dk_tune_config",
             "dk_set_solver", "dk_solver_stats",
-            "dk_set_tune_config", "dk_graph_launches", "dk_stiffness"] +
+            "dk_set_tune_config", "dk_graph_launches", "dk_stiffness",
+            "dk_stopping_times"] +
            ["dk_" + t for t in TASKS])
 
 
@@ -54,6 +55,7 @@ def lib():
         L.dk_gas_hook.argtypes = [P, st, C.POINTER(abi.dk_gas_src)]
         L.dk_set_graphs.argtypes = [P, C.c_int32]
         L.dk_stiffness.argtypes = [P, C.POINTER(C.c_double)]
+        L.dk_stopping_times.argtypes = [P, C.POINTER(C.c_double)]
         L.dk_graph_launches.restype = C.c_int64
         L.dk_graph_launches.argtypes = [P]
         L.dk_autotune.argtypes = [P, C.c_int32]
@@ -193,11 +195,27 @@ class DustContext:
         self._check(lib().dk_set_graphs(self.h, 1 if on else 0), "dk_set_graphs")
 
     def stiffness(self):
-        """UpdateStiffness records: chi of the last pc2 cycle and its running maximum."""
+        """UpdateStiffness records: chi of the last pc2 cycle, its running maximum, the rate
+        maximum."""
         import numpy as np
-        out = (C.c_double * 2)()
+        out = (C.c_double * 3)()
         self._check(lib().dk_stiffness(self.h, out), "dk_stiffness")
         return np.array(out[:])
+
+    def stopping_times(self):
+        """RefreshStoppingTimeMaximum: (largest, smallest) stopping time, validated."""
+        out = (C.c_double * 2)()
+        self._check(lib().dk_stopping_times(self.h, out), "dk_stopping_times")
+        return out[0], out[1]
+
+    def new_time_step_pc2(self, dt_cfl, stiff_thresh=0.0):
+        """NewTimeStep (dust_newdt.cpp:108-149) with the pc2 stiffness limit: min(dt_cfl min
+        dx/|v|, stiff_thresh/stiff_rate_max) when stiff_thresh > 0 and a rate was recorded."""
+        dt = self.new_time_step(dt_cfl)
+        rate = self.stiffness()[2]
+        if stiff_thresh > 0.0 and rate > 0.0 and self.phys.coupling == abi.DK_PC2:
+            dt = min(dt, stiff_thresh / rate)
+        return dt
 
     def graphs_captured(self):
         """Stage graphs launched so far (0 on the null stream, where stages run eagerly)."""

@@ -270,10 +270,18 @@ int dk_prof_reset(dk_ctx *ctx);
 int dk_prof_read(dk_ctx *ctx, int32_t idx, char *label, int32_t label_len, double *ms,
                  int64_t *count);
 /* UpdateStiffness / SyncStiffness (src/dust/dust_pm.cpp:288-359; dust.hpp stiff_chi_last,
- * stiff_chi_run_max): with the stiffness field on (dk_physics.stiffness) every pc2 cycle records
- * chi = dt max_cells qdep[4]/rho before the predictor is applied; out[0] = chi of the last
- * cycle, out[1] = the running maximum (one rank: no reduction over ranks).  Both forms. */
-int dk_stiffness(dk_ctx *ctx, double out[2]);
+ * stiff_chi_run_max, stiff_rate_max): with the stiffness field on (dk_physics.stiffness) every
+ * pc2 cycle records chi = dt max_cells qdep[4]/rho before the predictor is applied; out[0] = chi
+ * of the last cycle, out[1] = the running maximum, out[2] = the rate maximum max qdep[4]/rho
+ * (NewTimeStep's stiff_thresh/stiff_rate_max limit, dust_newdt.cpp:114-119).  One rank: no
+ * reduction over ranks.  Both forms. */
+int dk_stiffness(dk_ctx *ctx, double out[3]);
+/* RefreshStoppingTimeMaximum (src/dust/dust_newdt.cpp:34-98): validates every particle's
+ * stopping time (finite, positive, species index not negative; DK_ESTOP otherwise) and returns
+ * out[0] = the largest, out[1] = the smallest (taus_max, taus_min), which GammaSwitch
+ * (dust_newdt.cpp:160-173) compares with dt on the host.  The FP32 form reduces its FP32 copies.
+ * Waits for the stream. */
+int dk_stopping_times(dk_ctx *ctx, double out[2]);
 /* The first device-side error (code, particle index); clears it. */
 int dk_error(dk_ctx *ctx, int32_t *code, int64_t *particle);
 /* Device pointer to a work field: 0 qdep(5), 1 ustar(3), 2 dmom(3), 3 solver ap(3),

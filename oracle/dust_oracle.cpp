@@ -116,7 +116,7 @@ struct or_sim {
          adaptive_state_floor = 1.0;
   double solver_last[7] = {0, 0, -1, -1, -1, -1, -1};
   // UpdateStiffness records (dust_pm.cpp:288-342): chi of the last pc2 cycle, its running max
-  double stiff_chi_last = 0.0, stiff_chi_run_max = 0.0;
+  double stiff_chi_last = 0.0, stiff_chi_run_max = 0.0, stiff_rate_max = 0.0;
   std::vector<double> dx1, dx2, dx3;   // per block (meshblock.cpp:113-118)
   int nthreads = 1;
   int err = 0;
@@ -1683,6 +1683,7 @@ int or_gas_implicit_solve(or_sim *s, const dk_stage *st) {
               for (int i=s->is; i<=s->ie; ++i)
                 rate = std::fmax(rate, s->qdep[s->Idx(m,5,4,k,j,i)]/
                                        s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)]);
+        s->stiff_rate_max = rate;
         s->stiff_chi_last = dt*rate;
         s->stiff_chi_run_max = std::max(s->stiff_chi_run_max, s->stiff_chi_last);
       }
@@ -1767,9 +1768,10 @@ void or_set_solver(or_sim *s, double rtol, double atol, int iter_max, double ord
   s->adaptive_state_floor = state_floor;
 }
 
-void or_stiffness(or_sim *s, double out[2]) {
+void or_stiffness(or_sim *s, double out[3]) {
   out[0] = s->stiff_chi_last;
   out[1] = s->stiff_chi_run_max;
+  out[2] = s->stiff_rate_max;
 }
 
 void or_solver_stats(or_sim *s, double out[7]) {

@@ -82,8 +82,9 @@ int or_diag_sums(or_sim *s, double out[9]);
 void or_set_solver(or_sim *s, double rtol, double atol, int iter_max, double order_c,
                    double rtol_max, double tref, double state_floor);
 void or_solver_stats(or_sim *s, double out[7]);
-/* UpdateStiffness records (dust_pm.cpp:288-342): chi of the last pc2 cycle, its running max */
-void or_stiffness(or_sim *s, double out[2]);
+/* UpdateStiffness records (dust_pm.cpp:288-342): chi of the last pc2 cycle, its running max,
+ * the rate max */
+void or_stiffness(or_sim *s, double out[3]);
 int or_last_error(or_sim *s, int64_t *particle);
 
 /* imex2+ (gamma = 1+1/sqrt(2)) and rk2 tableaux of the reference driver */

@@ -182,8 +182,9 @@ class Oracle:
         return np.array(out[:])
 
     def stiffness(self):
-        """UpdateStiffness records: chi of the last pc2 cycle and its running maximum."""
-        out = (C.c_double * 2)()
+        """UpdateStiffness records: chi of the last pc2 cycle, its running maximum, the rate
+        maximum."""
+        out = (C.c_double * 3)()
         lib().or_stiffness(self.h, out)
         return np.array(out[:])
 

@@ -469,10 +469,39 @@ def test_pc2_stiffness_record_both_forms():
             x.step(w.dt, w.src)
         so, s64, s32 = o.stiffness(), g64.stiffness(), g32.stiffness()
         say("stiffness step %d: oracle %s, fp64 %s, fp32 %s" % (step, so, s64, s32))
-        assert so[0] > 1.0
+        assert so[0] > 1.0 and abs(so[2] * w.dt - so[0]) <= 1e-15 * so[0]
         assert np.all(np.abs(s64 - so) <= 1e-12 * so), (s64, so)
         assert np.all(np.abs(s32 - so) <= 1e-5 * so), (s32, so)
+    # NewTimeStep's pc2 limit: stiff_thresh over the recorded rate
+    dt_lim = g64.new_time_step_pc2(0.3, stiff_thresh=0.5)
+    assert dt_lim == min(g64.new_time_step(0.3), 0.5 / s64[2])
     g64.check_device_error()
     g32.check_device_error()
+    g64.close()
+    g32.close()
+
+
+def test_stopping_time_range_and_validation():
+    # RefreshStoppingTimeMaximum: the largest and smallest stopping time of a continuous size
+    # distribution (the FP32 form: of its FP32 copies), and an invalid one reported (DK_ESTOP);
+    # GammaSwitch picks 1/2 only when dt exceeds the largest
+    from athenak_amd.lib import DustContext, DustError
+    w = workloads.c3(n=32, nbx=2)
+    g64 = DustContext(w.mesh, w.blocks, w.phys, w.npart + 16, form=abi.DK_FORM_FP64)
+    g64.setup(w.setup)
+    g32 = fp32_ctx(w)
+    ts = g64.rdata[abi.IPTS, :g64.npart].cpu().numpy()
+    mx, mn = g64.stopping_times()
+    say("stopping times: fp64 (%.6e, %.6e), numpy (%.6e, %.6e)" % (mx, mn, ts.max(), ts.min()))
+    assert (mx, mn) == (ts.max(), ts.min())
+    t32 = ts.astype(np.float32).astype(np.float64)
+    assert g32.stopping_times() == (t32.max(), t32.min())
+    assert abi.gamma_switch(2.0 * mx, mx) == 0.5
+    assert abi.gamma_switch(mx, mx) == abi.IMEX2P_GAMMA
+    assert abi.gamma_switch(2.0 * mx, mx, abi.DK_PC2) is None
+    g64.rdata[abi.IPTS, 5] = -1.0
+    g64.stream.synchronize()
+    with pytest.raises(DustError, match="stopping"):
+        g64.stopping_times()
     g64.close()
     g32.close()

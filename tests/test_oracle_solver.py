@@ -1,4 +1,4 @@
-"""Pins the oracle's strict drag solver (StrictPCG, GlobalDot, GlobalPreconditionedNorm,
+"""Pins the oracle's strict drag solver and the pc2 stiffness record (StrictPCG, GlobalDot, GlobalPreconditionedNorm,
 AdaptiveErrorBound; src/dust/dust_solver.cpp:268-625), no GPU.
 
 - PCG's iterate satisfies the coupled system to the reference's tolerance: the residual
@@ -135,10 +135,11 @@ def test_pc2_stiffness_record_is_the_max_rate_over_rho():
         chis.append(w.dt * (q4 / rho).max())
         s = o.stiffness()
         assert s[0] == chis[-1], (s, chis)
+        assert s[2] == (q4 / rho).max(), s
         assert s[1] == max(chis), (s, chis)
     assert chis[0] > 1.0          # (the damping case: stiff cells exist)
     w.phys.stiffness = 0
     o0 = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
     o0.setup(w.setup)
     o0.step(w.dt, w.src)
-    assert np.array_equal(o0.stiffness(), [0.0, 0.0])
+    assert np.array_equal(o0.stiffness(), [0.0, 0.0, 0.0])
