@@ -1,5 +1,6 @@
-"""Pins the oracle's strict drag solver and the pc2 stiffness record (StrictPCG, GlobalDot, GlobalPreconditionedNorm,
-AdaptiveErrorBound; src/dust/dust_solver.cpp:268-625), no GPU.
+"""Pins the oracle's strict drag solver (StrictPCG, GlobalDot, GlobalPreconditionedNorm,
+AdaptiveErrorBound; src/dust/dust_solver.cpp:268-625) and the pc2 stiffness record
+(UpdateStiffness, src/dust/dust_pm.cpp:288-342), no GPU.
 
 - PCG's iterate satisfies the coupled system to the reference's tolerance: the residual
   b - A x, formed with the operator applied on its own (or_apply_operator), has a
