@@ -130,6 +130,7 @@ struct Fast {
   double *pres[2] = {nullptr, nullptr};   // push rounding residue per cell (to the gas):
   int pres_rd = 0;                        // the buffer of the last push, read by the next stage
   float *scr_f = nullptr;
+  float *cpack = nullptr;           // pc2 clouds: the 12-component gather pack (PackGatherField3D)
   double *defect = nullptr;    // [nt][3] the FP64 defect of every tile's impulse deposit
   int pending = 0;
   double a_prev = 0.0;
@@ -796,6 +797,75 @@ struct RegionFetch {
     }
   }
 };
+
+// the whole tile region of an nvar-component padded cell field straight into LDS (for fields
+// larger than RegionFetch holds)
+template <typename T>
+__device__ inline void RegionLoad(const Geo &g, const Tiles &Tl, const TileInfo &t, const T *f,
+                                  int nvar, T *reg) {
+  const int n = nvar*Tl.RN;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int v = FDiv(i, Tl.iRN);
+    const int rc = i - v*Tl.RN;
+    const int rr = FDiv(rc, Tl.iRX);
+    const int rx = rc - rr*Tl.RX;
+    const int rz = FDiv(rr, Tl.iRY);
+    const int ry = rr - rz*Tl.RY;
+    reg[i] = f[Idx(g, t.m, nvar, v, g.ks + t.ck0 - 1 + rz, g.js + t.cj0 - 1 - Tl.SY + ry,
+                   g.is + t.ci0 - 1 + rx)];
+  }
+}
+
+// Refinable clouds at the mesh level (half-width h = one cell): the first moments, in cells, of
+// the hat over the three cells of the support about the particle, whose offset from the centre
+// of its cell is del (CloudShape1D with HatMoment, src/dust/dust.hpp:169-182, 240-298: the
+// faces sit at s = a - 3/2 - del in units of h); the weights are the TSC weights (the hat's CDF
+// differences over the same cells)
+__device__ inline float HatMomentF(float s) {
+  if (s <= -1.0f || s >= 1.0f) return 1.0f/6.0f;
+  const float s2 = s*s;
+  return (s <= 0.0f) ? 0.5f*s2 + s2*s/3.0f : 0.5f*s2 - s2*s/3.0f;
+}
+__device__ inline void CloudMoments(float del, float hm[3]) {
+  float f0 = HatMomentF(-1.5f - del);
+#pragma unroll
+  for (int a=0; a<3; ++a) {
+    const float f1 = HatMomentF(-0.5f - del + static_cast<float>(a));
+    hm[a] = f1 - f0;
+    f0 = f1;
+  }
+}
+
+// The linear gather of a cloud (LeafApi<3D>::GatherLinear, src/dust/dust_leaf_api.hpp:202-220)
+// from the tile region of the 12-component pack (velocity, then its central differences along
+// x1, x2, x3; PackGatherField3D): u += w q + cx dq/dx1 + cy dq/dx2 + cz dq/dx3 per cell, with
+// c = (first moment + (x - x_cell) w)/(2 dx) in cells
+__device__ inline void CloudGather(const Tiles &Tl, const float *pk, int rb, const float o[3],
+                                   const float wx[3], const float wy[3], const float wz[3],
+                                   float u[3]) {
+  float hx[3], hy[3], hz[3];
+  CloudMoments(o[0], hx);
+  CloudMoments(o[1], hy);
+  CloudMoments(o[2], hz);
+#pragma unroll
+  for (int c=0; c<3; ++c)
+#pragma unroll
+    for (int bb=0; bb<3; ++bb) {
+      const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+#pragma unroll
+      for (int a=0; a<3; ++a) {
+        const float w = wz[c]*wy[bb]*wx[a];
+        const float cx = 0.5f*(wy[bb]*wz[c]*hx[a] + (o[0] + static_cast<float>(1 - a))*w);
+        const float cy = 0.5f*(wx[a]*wz[c]*hy[bb] + (o[1] + static_cast<float>(1 - bb))*w);
+        const float cz = 0.5f*(wx[a]*wy[bb]*hz[c] + (o[2] + static_cast<float>(1 - c))*w);
+        const int q = rr + a;
+#pragma unroll
+        for (int d=0; d<3; ++d)
+          u[d] += w*pk[d*Tl.RN + q] + cx*pk[(3 + d)*Tl.RN + q] + cy*pk[(6 + d)*Tl.RN + q] +
+                  cz*pk[(9 + d)*Tl.RN + q];
+      }
+    }
+}
 
 // The tile scratch holds the regions row by row: one row of the scratch is the x1 rows (RX
 // values each) of the regions of all tiles of a block that share (tz, ty, component, rz, ry), in
@@ -1721,7 +1791,8 @@ k_p3n(KArgs A, const float *us, float *scr, double *defect) {
 // and the deposit of the rate m/(t_s V) (A.rate_dep) and of g = -dt (m/(t_s V)) (u - v) with the
 // same weights: qdep components 0 (rate), 1-3 (g).  The registers x1, v1 and the rate R stay in
 // the source order (the next key pass and an export read them through this sort's permutation).
-template <int S, bool OV, bool SPLIT = false>
+// (CLOUD: refinable clouds at the mesh level, the linear gather from the 12-component pack uv)
+template <int S, bool OV, bool SPLIT = false, bool CLOUD = false>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_pp(KArgs A, const float *uv, float *scr) {
   extern __shared__ double lds_d[];
@@ -1735,18 +1806,20 @@ k_pp(KArgs A, const float *uv, float *scr) {
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
     const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+    constexpr int NR = CLOUD ? 12 : 3;
     float *dep = reinterpret_cast<float*>(lds_d);          // [4][RN] the deposit
-    float *uv_ = dep + 4*Tl.RN;                            // [3][RN] u = M/rho
-    float *rd = uv_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
+    float *uv_ = dep + 4*Tl.RN;                            // [NR][RN] u = M/rho (or its pack)
+    float *rd = uv_ + NR*Tl.RN;                            // [3][CH] offsets in the cell x, y, z
     float *rv = rd + 3*CH;                                 // [4][CH] rate, g
     int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 7, 4));   // [TC + 1]
     [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
     RegionFetch<float> uf;
-    uf.Load(g, Tl, t, uv, 3);
+    if constexpr (!CLOUD) uf.Load(g, Tl, t, uv, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
     for (int i = tid; i < 4*Tl.RN; i += NT) dep[i] = 0.0f;
+    if constexpr (CLOUD) RegionLoad(g, Tl, t, uv, NR, uv_);
     __syncthreads();
-    uf.Store(Tl, 3, uv_);
+    if constexpr (!CLOUD) uf.Store(Tl, 3, uv_);
     __syncthreads();
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
     const DevBlock &b = A.B[t.m];
@@ -1783,20 +1856,25 @@ k_pp(KArgs A, const float *uv, float *scr) {
           W3<S, float>(oz, wz);
           const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0))*Tl.RX + (L.ci - t.ci0);
           float u[3] = {0.0f, 0.0f, 0.0f};
+          if constexpr (CLOUD) {
+            const float o[3] = {ox, oy, oz};
+            CloudGather(Tl, uv_, rb, o, wx, wy, wz, u);
+          } else {
 #pragma unroll
-          for (int c=0; c<3; ++c)
+            for (int c=0; c<3; ++c)
 #pragma unroll
-            for (int bb=0; bb<3; ++bb) {
-              const float wcb = wz[c]*wy[bb];
-              const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+              for (int bb=0; bb<3; ++bb) {
+                const float wcb = wz[c]*wy[bb];
+                const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
 #pragma unroll
-              for (int a=0; a<3; ++a) {
-                const float w = wcb*wx[a];
-                u[0] += w*uv_[rr + a];
-                u[1] += w*uv_[Tl.RN + rr + a];
-                u[2] += w*uv_[2*Tl.RN + rr + a];
+                for (int a=0; a<3; ++a) {
+                  const float w = wcb*wx[a];
+                  u[0] += w*uv_[rr + a];
+                  u[1] += w*uv_[Tl.RN + rr + a];
+                  u[2] += w*uv_[2*Tl.RN + rr + a];
+                }
               }
-            }
+          }
           const float rate = mass*ivolf/ts;
           val[0] = A.rate_dep ? rate : 0.0f;
           for (int d=0; d<3; ++d) val[1 + d] = -dtf*rate*(u[d] - v[d]);
@@ -1854,7 +1932,8 @@ k_pp(KArgs A, const float *uv, float *scr) {
 // 226-244).  v - v0 = dt (u* - vh)/t_s in exact arithmetic; the impulse and R take the exact
 // difference of the FP32 velocities, so that the gas receives the dust's actual change.  As P3:
 // the impulse is deposited in FP32 and every tile's FP64 defect goes to the combine.
-template <int S, bool OV, bool SPLIT = false>
+// (CLOUD: as k_pp, the linear gather from the 12-component pack us of u*)
+template <int S, bool OV, bool SPLIT = false, bool CLOUD = false>
 __global__ void __launch_bounds__(kCellThreads, 4)
 k_pa(KArgs A, const float *us, float *scr, double *defect) {
   extern __shared__ double lds_d[];
@@ -1868,18 +1947,20 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
     const int tid = threadIdx.x, NT = blockDim.x, CH = kPPT*NT;
+    constexpr int NR = CLOUD ? 12 : 3;
     float *dep = reinterpret_cast<float*>(lds_d);          // [3][RN] the deposit
-    float *us_ = dep + 3*Tl.RN;                            // [3][RN] u*
-    float *rd = us_ + 3*Tl.RN;                             // [3][CH] offsets in the cell x, y, z
+    float *us_ = dep + 3*Tl.RN;                            // [NR][RN] u* (or its pack)
+    float *rd = us_ + NR*Tl.RN;                            // [3][CH] offsets in the cell x, y, z
     float *rv = rd + 3*CH;                                 // [3][CH] -m (v - v0)/V
     int *st = reinterpret_cast<int*>(rd + RecArea(Tl, CH, 6, 3));   // [TC + 1]
     [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
     RegionFetch<float> uf;
-    uf.Load(g, Tl, t, us, 3);
+    if constexpr (!CLOUD) uf.Load(g, Tl, t, us, 3);
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
     for (int i = tid; i < 3*Tl.RN; i += NT) dep[i] = 0.0f;
+    if constexpr (CLOUD) RegionLoad(g, Tl, t, us, NR, us_);
     __syncthreads();
-    uf.Store(Tl, 3, us_);
+    if constexpr (!CLOUD) uf.Store(Tl, 3, us_);
     __syncthreads();
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
     const DevBlock &b = A.B[t.m];
@@ -1917,20 +1998,25 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
           W3<S, float>(oz, wz);
           const int rb = ((L.ck - t.ck0)*Tl.RY + (L.cj - t.cj0))*Tl.RX + (L.ci - t.ci0);
           float u[3] = {0.0f, 0.0f, 0.0f};
+          if constexpr (CLOUD) {
+            const float o[3] = {ox, oy, oz};
+            CloudGather(Tl, us_, rb, o, wx, wy, wz, u);
+          } else {
 #pragma unroll
-          for (int c=0; c<3; ++c)
+            for (int c=0; c<3; ++c)
 #pragma unroll
-            for (int bb=0; bb<3; ++bb) {
-              const float wcb = wz[c]*wy[bb];
-              const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
+              for (int bb=0; bb<3; ++bb) {
+                const float wcb = wz[c]*wy[bb];
+                const int rr = rb + (c*Tl.RY + bb)*Tl.RX;
 #pragma unroll
-              for (int a=0; a<3; ++a) {
-                const float w = wcb*wx[a];
-                u[0] += w*us_[rr + a];
-                u[1] += w*us_[Tl.RN + rr + a];
-                u[2] += w*us_[2*Tl.RN + rr + a];
+                for (int a=0; a<3; ++a) {
+                  const float w = wcb*wx[a];
+                  u[0] += w*us_[rr + a];
+                  u[1] += w*us_[Tl.RN + rr + a];
+                  u[2] += w*us_[2*Tl.RN + rr + a];
+                }
               }
-            }
+          }
           const float lam = hf/ts;
           const float aa = 1.0f + lam;
           for (int d=0; d<3; ++d) {
@@ -2248,6 +2334,28 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
   }
 }
 
+// PackGatherField3D (src/dust/dust_cloud3d.cpp:1319-1340) in FP32: the velocity and its central
+// differences along x1, x2, x3 of every cell with both neighbours in each direction (the
+// active cells and the first ghost layer, which the tile regions read)
+__global__ void k_fpack(Geo g, const float *vel, float *gp) {
+  const long long ni = g.n1 - 2, nj = g.n2 - 2, nk = g.n3 - 2;
+  const long long n = (long long)g.nmb*nk*nj*ni;
+  GRID_STRIDE(t, n) {
+    const int i = 1 + static_cast<int>(t % ni);
+    long long r = t/ni;
+    const int j = 1 + static_cast<int>(r % nj);
+    r /= nj;
+    const int k = 1 + static_cast<int>(r % nk);
+    const int m = static_cast<int>(r/nk);
+    for (int c=0; c<3; ++c) {
+      gp[Idx(g,m,12,c,k,j,i)] = vel[Idx(g,m,3,c,k,j,i)];
+      gp[Idx(g,m,12,3+c,k,j,i)] = vel[Idx(g,m,3,c,k,j,i+1)] - vel[Idx(g,m,3,c,k,j,i-1)];
+      gp[Idx(g,m,12,6+c,k,j,i)] = vel[Idx(g,m,3,c,k,j+1,i)] - vel[Idx(g,m,3,c,k,j-1,i)];
+      gp[Idx(g,m,12,9+c,k,j,i)] = vel[Idx(g,m,3,c,k+1,j,i)] - vel[Idx(g,m,3,c,k-1,j,i)];
+    }
+  }
+}
+
 // pc2 apply-predictor (dust_pm.cpp:387-402): with damping, the cells where chi = dt rate/rho
 // exceeds 1 scale their g by 1/chi (kept: the commit removes the same values); u0 += g
 __global__ void k_pc2_apply(Geo g, double *u0, float *qd, int damp, double dt) {
@@ -2484,11 +2592,12 @@ size_t LdsP2(const Tiles &T, bool sy = true) {
   return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, sy ? 7 : 6, 3)) + LdsTail(T);
 }
 size_t LdsP3(const Tiles &T, bool sy = true) { return LdsP2(T, sy); }   // the same areas
-size_t LdsPP(const Tiles &T) {
-  return sizeof(float)*(7*T.RN + RecArea(T, kPPT*3*T.TC, 7, 4)) + LdsTail(T);
+// (cloud: the region holds the 12-component pack instead of the 3 velocity components)
+size_t LdsPP(const Tiles &T, bool cloud = false) {
+  return sizeof(float)*((cloud ? 16 : 7)*T.RN + RecArea(T, kPPT*3*T.TC, 7, 4)) + LdsTail(T);
 }
-size_t LdsPA(const Tiles &T) {
-  return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, 6, 3)) + LdsTail(T);
+size_t LdsPA(const Tiles &T, bool cloud = false) {
+  return sizeof(float)*((cloud ? 15 : 6)*T.RN + RecArea(T, kPPT*3*T.TC, 6, 3)) + LdsTail(T);
 }
 
 // workgroups of the launch over the extra parts (they loop over the items; all exit at once when
@@ -2605,27 +2714,39 @@ void LaunchP3(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *d
     default: KERNEL<0, OV, SPL><<<grid, blk, lds, c->stream>>> ARGS; break; \
   }
 
-// (the two classes of items as for P1-P3 without the shearing box)
-void LaunchPP(dk_ctx *c, const KArgs &A0, const float *uv, float *scr) {
+// (the two classes of items as for P1-P3 without the shearing box; clouds: the cloud weights
+// are the TSC weights, uv/us the pack)
+void LaunchPP(dk_ctx *c, const KArgs &A0, const float *uv, float *scr, bool cloud = false) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  const size_t lds = LdsPP(T);
+  const size_t lds = LdsPP(T, cloud);
   bool sy;
   KArgs A = ClassArgs(c, A0, 0, sy);
-  SWITCH_S2(k_pp, false, false, dim3(A.nfirst), (A, uv, scr))
+  if (cloud) k_pp<2, false, false, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, uv, scr);
+  else {SWITCH_S2(k_pp, false, false, dim3(A.nfirst), (A, uv, scr))}
   A = ClassArgs(c, A0, 1, sy);
-  SWITCH_S2(k_pp, false, true, dim3(A.nfirst), (A, uv, scr))
+  if (cloud) k_pp<2, false, true, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, uv, scr);
+  else {SWITCH_S2(k_pp, false, true, dim3(A.nfirst), (A, uv, scr))}
 }
 
-void LaunchPA(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect) {
+void LaunchPA(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect,
+              bool cloud = false) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  const size_t lds = LdsPA(T);
+  const size_t lds = LdsPA(T, cloud);
   bool sy;
   KArgs A = ClassArgs(c, A0, 0, sy);
-  SWITCH_S2(k_pa, false, false, dim3(A.nfirst), (A, us, scr, defect))
+  if (cloud) {
+    k_pa<2, false, false, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, us, scr, defect);
+  } else {
+    SWITCH_S2(k_pa, false, false, dim3(A.nfirst), (A, us, scr, defect))
+  }
   A = ClassArgs(c, A0, 1, sy);
-  SWITCH_S2(k_pa, false, true, dim3(A.nfirst), (A, us, scr, defect))
+  if (cloud) {
+    k_pa<2, false, true, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, us, scr, defect);
+  } else {
+    SWITCH_S2(k_pa, false, true, dim3(A.nfirst), (A, us, scr, defect))
+  }
 }
 
 // the cell passes: x over the active cells of a block, y over the blocks
@@ -2821,10 +2942,9 @@ namespace dk {
 
 int FastCreate(dk_ctx *c) {
   if (c->P.coupling == DK_PC2) {
-    // pc2 of point particles on periodic meshes without the shearing box (the half-step
-    // position is keyed in the block that holds it; clouds: the FP64 form)
-    if (c->P.clouds || c->P.shearing_box || c->M.bc_x1 != DK_X1_PERIODIC ||
-        c->g.bc_x3 != DK_X3_PERIODIC)
+    // pc2 of point particles and of clouds at the mesh level on periodic meshes without the
+    // shearing box (the half-step position is keyed in the block that holds it)
+    if (c->P.shearing_box || c->M.bc_x1 != DK_X1_PERIODIC || c->g.bc_x3 != DK_X3_PERIODIC)
       return DK_EUNSUPPORTED;
   } else if (c->P.coupling != DK_IMEX) {
     return DK_EUNSUPPORTED;
@@ -2867,8 +2987,9 @@ int FastCreate(dk_ctx *c) {
   // the u* region of P2/P3 in kRegionPer values per thread
   if (3*T.RN > kRegionPer*3*T.TC) return DK_EUNSUPPORTED;
   // every tile kernel's LDS within 64 KiB
-  if (std::max(std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)), std::max(LdsPP(T), LdsPA(T))) >
-      65536 - 1024)
+  const bool cl = c->P.clouds != 0;
+  if (std::max(std::max(std::max(LdsP1(T), LdsP2(T)), LdsP3(T)),
+               std::max(LdsPP(T, cl), LdsPA(T, cl))) > 65536 - 1024)
     return DK_EUNSUPPORTED;
   // a block's first and last x2 rows must differ (cells centred one row off in x2)
   if (T.SY && c->g.nx2 < 2) return DK_EUNSUPPORTED;
@@ -2893,7 +3014,9 @@ int FastCreate(dk_ctx *c) {
   for (int q=0; q<3; ++q) e = e ? e : Alloc((void**)&f->dc[q], sizeof(double)*3*nc);
   e = e ? e : Alloc((void**)&f->scr_f, sizeof(float)*4*(size_t)T.RN*T.nt);
   e = e ? e : Alloc((void**)&f->defect, sizeof(double)*3*(size_t)T.nt);
+  if (cl) e = e ? e : Alloc((void**)&f->cpack, sizeof(float)*12*nc);
   if (e) return e;
+  if (cl) (void)hipMemset(f->cpack, 0, sizeof(float)*12*nc);
   (void)hipMemset(f->qt[0], 0, sizeof(float)*nc);
   (void)hipMemset(f->qt[1], 0, sizeof(float)*nc);
   (void)hipMemset(f->qt[2], 0, sizeof(float)*nc);
@@ -2926,7 +3049,7 @@ void FastDestroy(dk_ctx *c) {
   for (void *p : {(void*)f->key, (void*)f->rank, (void*)f->count, (void*)f->start,
                   (void*)f->perm, (void*)f->big, (void*)f->stay, f->scan_tmp, (void*)f->itag, (void*)f->qdep,
                   (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom, (void*)f->pres[0], (void*)f->pres[1],
-                  (void*)f->scr_f, (void*)f->defect})
+                  (void*)f->scr_f, (void*)f->defect, (void*)f->cpack})
     if (p) (void)hipFree(p);
   for (int q=0; q<3; ++q) {
     if (f->qt[q]) (void)hipFree(f->qt[q]);
@@ -3119,6 +3242,14 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
         st->beta[s]*st->dt, src ? 1 : 0, src ? *src : z, 0.0, f->ustar, s1 ? 1 : 2)));
   }
   if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
+  // clouds: the gather pack of the velocity (PackGatherField3D)
+  const bool cl = c->P.clouds != 0;
+  const float *gf = cl ? f->cpack : f->ustar;
+  if (cl) {
+    const Geo &gg = c->g;
+    const long long nin = (long long)gg.nmb*(gg.n1 - 2)*(gg.n2 - 2)*(gg.n3 - 2);
+    TIMED("cloud_pack", (k_fpack<<<Grid(nin), kBlock, 0, c->stream>>>(gg, f->ustar, f->cpack)));
+  }
   if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
   TIMED("key", LaunchKey(c, A, 1, f->pres[1 - f->pres_rd], !s1));
   TIMED("fix", (k_fix_key<<<Grid((long long)T.nt*T.TC), kBlock, 0, c->stream>>>(
@@ -3126,7 +3257,7 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   if ((e = SortKeys(c))) return e;
   if ((e = BuildItems(c))) return e;
   if (s1) {
-    TIMED("pc2_predictor", LaunchPP(c, A, f->ustar, f->scr_f));
+    TIMED("pc2_predictor", LaunchPP(c, A, gf, f->scr_f, cl));
     TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, f->qdep, nullptr, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
@@ -3134,7 +3265,7 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     TIMED("pc2_apply", (k_pc2_apply<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->u0, f->qdep, c->P.pc2_damping, st->dt)));
   } else {
-    TIMED("pc2_advance", LaunchPA(c, A, f->ustar, f->scr_f, f->defect));
+    TIMED("pc2_advance", LaunchPA(c, A, gf, f->scr_f, f->defect, cl));
     TIMED("combine", (k_combine<float, 3, double><<<CombineGrid(c), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, f->dmom, f->defect, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;

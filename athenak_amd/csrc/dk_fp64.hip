@@ -1282,7 +1282,7 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
           INT32_MAX)
     return DK_ECAPACITY;
   if (phys->solver < DK_LOCAL || phys->solver > DK_ADAPTIVE) return DK_EUNSUPPORTED;
-  if (phys->clouds && (!dk::CloudsSupported(mesh, phys) || form != DK_FORM_FP64))
+  if (phys->clouds && !dk::CloudsSupported(mesh, phys))
     return DK_EUNSUPPORTED;
   dk_ctx *c = new dk_ctx();
   c->M = *mesh;

@@ -60,8 +60,9 @@ enum {
   DK_FORM_FP32 = 1    /* FP32 bulk, FP64 positions and sums: cell-sorted particles,
                          atomic-free deposits, fused stage passes (dk_stage_run only).
                          imex with local/dc1/dc2 on any mesh of the FP64 form; pc2 with
-                         point particles on periodic meshes without the shearing box;
-                         otherwise dk_create returns DK_EUNSUPPORTED */
+                         point particles or clouds at the mesh level on periodic meshes
+                         without the shearing box; otherwise dk_create returns
+                         DK_EUNSUPPORTED */
 };
 
 /* error codes */

@@ -189,6 +189,22 @@ def test_fp32_pc2_clumped_and_schemes():
         pc2_check(w, 6)
 
 
+def test_fp32_pc2_clouds_against_the_oracle():
+    # refinable clouds at the mesh level (CloudPredictorT / CloudAdvanceT): the TSC weights and
+    # the linear gather from the velocity pack, clouds reaching into neighbour blocks and across
+    # periodic faces; then clumps with the stiffness field and damping (tiles of several parts)
+    w = workloads.c4(n=16, nbx=2, ppc=2)
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 2.0, 0.5, 2e-3
+    w.name = "fp32-clouds-cross"
+    pc2_check(w, 20)
+    pc2_check(w, 5, stage1=True)
+    w = workloads.c4(n=32, nbx=2, ppc=8)
+    w.phys.stiffness, w.phys.pc2_damping = 1, 1
+    w.setup.dust_noise = 0.3
+    w.name = "fp32-clouds-clumped"
+    pc2_check(w, 6)
+
+
 def test_fp32_pc2_bit_identical_reruns():
     w = workloads.c4(n=32, nbx=2, ppc=8, clouds=False)
     w.setup.dust_noise = 0.5
