@@ -63,8 +63,8 @@ def test_strerror_without_device():
 
 
 def test_clouds_refused_where_not_built_without_device():
-    """dk_create checks its options before any HIP call: clouds need pc2, the FP64 form and a
-    mesh periodic in x1 and x3."""
+    """dk_create checks its options before any HIP call: clouds need pc2 and a mesh periodic in
+    x1 and x3 (in either form)."""
     if not os.path.exists(lib.LIB_PATH):
         pytest.skip("libdustk.so not built")
     from athenak_amd import workloads
@@ -80,9 +80,27 @@ def test_clouds_refused_where_not_built_without_device():
     cases.append((mesh, blocks, abi.DK_PC2, abi.DK_FORM_FP64))
     mesh, blocks = abi.uniform_mesh((nx,) * 3, (1, 1, 1), ((0.0, 1.0),) * 3, ng=2)
     cases.append((mesh, blocks, abi.DK_IMEX, abi.DK_FORM_FP64))
+    cases.append((mesh, blocks, abi.DK_IMEX, abi.DK_FORM_FP32))
+    mesh, blocks = abi.uniform_mesh((nx,) * 3, (1, 1, 1), ((0.0, 1.0),) * 3, ng=2,
+                                    bc_x1=abi.DK_X1_SHEAR)
     cases.append((mesh, blocks, abi.DK_PC2, abi.DK_FORM_FP32))
     for mesh, blocks, coupling, form in cases:
         phys = workloads._phys(coupling=coupling, solver=abi.DK_LOCAL, clouds=1)
         e = L.dk_create(ctypes.byref(mesh), blocks, ctypes.byref(phys), form, None,
                         ctypes.byref(out))
         assert e == 7, (coupling, form, e)   # DK_EUNSUPPORTED
+
+
+def test_gamma_switch_follows_the_reference_rule():
+    # GammaSwitch (src/dust/dust_newdt.cpp:160-173): 1/2 only when dt exceeds the largest
+    # stopping time (strictly), 1 + 1/sqrt(2) otherwise, nothing for pc2; the imex2+ records
+    # built with either coefficient satisfy the tableau's identities
+    assert abi.gamma_switch(0.2, 0.1) == 0.5
+    assert abi.gamma_switch(0.1, 0.1) == abi.IMEX2P_GAMMA
+    assert abi.gamma_switch(0.05, 0.1) == abi.IMEX2P_GAMMA
+    assert abi.gamma_switch(0.2, 0.1, abi.DK_PC2) is None
+    for gam in (0.5, abi.IMEX2P_GAMMA):
+        st = abi.imex2p_stages(0.0, 1.0, gam)[1]
+        assert abs(st.gam0[1] + st.gam1[1] - 1.0) < 1e-15
+        assert st.a_impl == gam and st.beta[0] == gam
+        assert abs(st.a_twid22 - (1.0 - 2.0 * gam * gam) / (2.0 * gam)) < 1e-15
