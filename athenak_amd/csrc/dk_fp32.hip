@@ -49,7 +49,7 @@ namespace {
 constexpr int kSegSmall = 32;       // lists longer than this are ordered by k_medsort
 constexpr int kMedMax = 512;        // lists longer than this are ordered by k_bigsort
 constexpr int kBigMax = 16384;      // longest list of one cell in the FP32 form
-constexpr int kPartRank = 1024;     // a stayer's rank: part*kPartRank + its rank in the part
+constexpr int kPartRank = 2048;     // a stayer's rank: part*kPartRank + its rank in the part
 constexpr int kKeyThreads = 256;    // the key pass: one particle per thread per chunk (default)
 constexpr int kKeyThreadsMax = 512;
 constexpr int kOverGrid = 4096;     // workgroups of the launch over the extra parts of full tiles
@@ -443,7 +443,10 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
 // scratch.
 
 constexpr int kPPT = 2;             // particles per thread in the particle phase
-static_assert(kPPT*kCellThreads <= kPartRank, "a part's stayer ranks must fit below kPartRank");
+// a work item (part) is at most kPartChunks chunks of a tile's particles
+constexpr int kPartChunks = 2;
+static_assert(kPartChunks*kPPT*kCellThreads <= kPartRank,
+              "a part's stayer ranks must fit below kPartRank");
 
 struct TileInfo {
   int m, tx, ty, tz, ci0, cj0, ck0;
@@ -2812,7 +2815,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.regs_live = !f->regs_dead;
   A.res = f->res; A.dc0 = f->dc[0]; A.dc1 = f->dc[1]; A.dc2 = f->dc[2];
   A.item_start = f->item_start; A.item_tile = f->item_tile;
-  A.W = kPPT*3*f->T.TC;
+  A.W = kPartChunks*kPPT*3*f->T.TC;
   A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
   A.ov_pr = f->ov_pr; A.ov_cnt = f->ov_cnt;
   A.ilist = nullptr; A.ilist_n = nullptr; A.nfirst = f->T.nt;
@@ -2845,7 +2848,7 @@ int SortKeys(dk_ctx *c) {
 int BuildItems(dk_ctx *c) {
   Fast *f = c->fast;
   const Tiles &T = f->T;
-  const int W = kPPT*3*T.TC;
+  const int W = kPartChunks*kPPT*3*T.TC;
   TIMED("items", (k_nparts<<<Grid(T.nt + 1), kBlock, 0, c->stream>>>(T, f->start, W,
                                                                       f->nparts)));
   int e;
@@ -2872,7 +2875,7 @@ int BuildItems(dk_ctx *c) {
 int AllocItems(dk_ctx *c, long long n) {
   Fast *f = c->fast;
   const Tiles &T = f->T;
-  const long long W = (long long)kPPT*3*T.TC;
+  const long long W = (long long)kPartChunks*kPPT*3*T.TC;
   const long long ov = (n + W - 1)/W + 1;
   if (ov <= f->ov_cap && f->item_start) return DK_OK;
   for (void *p : {(void*)f->item_tile, (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr,
