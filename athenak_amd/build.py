@@ -54,5 +54,22 @@ def build(verbose=False, jobs=3):
     return LIB
 
 
+def build_capi_driver(verbose=False):
+    """The C++ caller of the C-ABI that the tests run (tests/capi/drive_step), linked against
+    the library built above."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "capi", "drive_step.cpp")
+    out = os.path.join(root, "tests", "capi", "drive_step")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(root, "include"), src,
+           "-L" + HERE, "-ldustk", "-Wl,-rpath,$ORIGIN/../../athenak_amd", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("drive_step build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

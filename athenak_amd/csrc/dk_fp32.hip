@@ -132,6 +132,7 @@ struct Fast {
   float *scr_f = nullptr;
   float *cpack = nullptr;           // pc2 clouds: the 12-component gather pack (PackGatherField3D)
   double *defect = nullptr;    // [nt][3] the FP64 defect of every tile's impulse deposit
+  double *x3p = nullptr;       // [item_cap][2] outflow partials of the key pass (outflow x3 faces)
   int pending = 0;
   double a_prev = 0.0;
   int regs_dead = 0;                // the store holds no registers (after a step's last stage)
@@ -208,6 +209,9 @@ struct KArgs {
   double *ov_dc;                    // [slot][9][TC] dust corrections by centre row
   float *ov_pr;                     // [slot][3][TC] push residues
   int *ov_cnt;                      // [slot][TC] stayers counted by the part, then the offsets
+  // outflow x3 faces: every work item's (mass, count) of the particles it parked this key pass
+  // (bvals_part.cpp:184-186), summed in item order by k_x3sum (null without outflow faces)
+  double *x3p;
   // the launch's items: ilist[0 .. *ilist_n) (all items when ilist is null), the first nfirst of
   // them one workgroup each
   const int *ilist, *ilist_n;
@@ -1037,6 +1041,8 @@ k_key(KArgs A, int do_key, double *pres) {
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
     const DevBlock &b = A.B[t.m];
     const double ivol = b.ivol;
+    double pmass = 0.0;   // (outflow faces: the mass and count this thread parked, chunk order)
+    int pcnt = 0;
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
       const int q = c0 + tid;
@@ -1108,7 +1114,11 @@ k_key(KArgs A, int do_key, double *pres) {
             if (A.regs_live) {A.src.X1(d, q) = s.x1[d]; A.src.V1(d, q) = s.v1[d];}
             pr[d] = static_cast<float>(static_cast<double>(mass)*s.vres[d]*ivol);
           }
-          if (s.parked) A.src.M(q) = 0.0f;
+          if (s.parked) {
+            pmass += static_cast<double>(mass);
+            ++pcnt;
+            A.src.M(q) = 0.0f;
+          }
           // a particle that failed to route keeps its block: its key stays valid
           int ni, nj, nk, nju;
           // the tile's block (read once by the workgroup), or the block a particle migrated to
@@ -1185,9 +1195,43 @@ k_key(KArgs A, int do_key, double *pres) {
         for (int k=0; k<3; ++k)
           for (int d=0; d<3; ++d) A.ov_dc[(o*9 + 3*k + d)*Tl.TC + tid] = dcs[k][d];
     }
+    if (A.x3p != nullptr && do_key) {
+      // the item's parked mass and count in a fixed order: lanes by a butterfly, the waves in
+      // order (rdc is free once the chunks are done)
+      if (__syncthreads_or(pcnt)) {
+        const double wm = WaveSum(pmass, 64);
+        const double wc = WaveSum(static_cast<double>(pcnt), 64);
+        if ((tid & 63) == 0) {rdc[2*(tid >> 6)] = wm; rdc[2*(tid >> 6) + 1] = wc;}
+        __syncthreads();
+        if (tid == 0) {
+          double sm = 0.0, sc = 0.0;
+          for (int w = 0; w < CH/64; ++w) {sm += rdc[2*w]; sc += rdc[2*w + 1];}
+          A.x3p[2*it] = sm; A.x3p[2*it + 1] = sc;
+        }
+      } else if (tid == 0) {
+        A.x3p[2*it] = 0.0; A.x3p[2*it + 1] = 0.0;
+      }
+    }
     if constexpr (!OV) break;
     __syncthreads();   // (the next item reuses LDS)
   }
+}
+
+// the key pass's outflow partials summed over its items in a fixed order (a strided sum per
+// thread, then a tree) into the ledger (mass, count)
+__global__ void __launch_bounds__(256) k_x3sum(const int *item_start, int nt, const double *x3p,
+                                               double *ledger) {
+  __shared__ double sm[256], sc[256];
+  const int n = item_start[nt];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {a += x3p[2*i]; b += x3p[2*i + 1];}
+  sm[threadIdx.x] = a; sc[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {sm[threadIdx.x] += sm[threadIdx.x + h]; sc[threadIdx.x] += sc[threadIdx.x + h];}
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {ledger[0] += sm[0]; ledger[1] += sc[0];}
 }
 
 // the parts of every tile: at most W particles each, at least one (the entry after the last tile
@@ -2818,6 +2862,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.W = kPartChunks*kPPT*3*f->T.TC;
   A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
   A.ov_pr = f->ov_pr; A.ov_cnt = f->ov_cnt;
+  A.x3p = f->x3p;
   A.ilist = nullptr; A.ilist_n = nullptr; A.nfirst = f->T.nt;
   A.deal = 1;
   A.cell_sums = 1;
@@ -2880,8 +2925,10 @@ int AllocItems(dk_ctx *c, long long n) {
   if (ov <= f->ov_cap && f->item_start) return DK_OK;
   for (void *p : {(void*)f->item_tile, (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr,
                   (void*)f->ov_def, (void*)f->ov_dc, (void*)f->ov_cnt, (void*)f->ilist[0],
-                  (void*)f->ilist[1], (void*)f->iflag, (void*)f->iflag_scan, f->lscan_tmp})
+                  (void*)f->ilist[1], (void*)f->iflag, (void*)f->iflag_scan, f->lscan_tmp,
+                  (void*)f->x3p})
     if (p) (void)hipFree(p);
+  f->x3p = nullptr;
   f->item_tile = nullptr; f->ov_scr = nullptr; f->ov_q = nullptr; f->ov_pr = nullptr;
   f->ov_def = nullptr; f->ov_dc = nullptr; f->ov_cnt = nullptr;
   f->ilist[0] = nullptr; f->ilist[1] = nullptr; f->iflag = nullptr; f->iflag_scan = nullptr;
@@ -2903,6 +2950,8 @@ int AllocItems(dk_ctx *c, long long n) {
       Alloc((void**)&f->ov_def, sizeof(double)*3*(size_t)ov) ||
       Alloc((void**)&f->ov_dc, sizeof(double)*9*(size_t)T.TC*ov) ||
       Alloc((void**)&f->ov_cnt, sizeof(int)*(size_t)T.TC*ov))
+    return DK_ENOMEM;
+  if (c->g.bc_x3 == DK_X3_OUTFLOW && Alloc((void**)&f->x3p, sizeof(double)*2*f->item_cap))
     return DK_ENOMEM;
   {
     if (f->ilist_n == nullptr && Alloc((void**)&f->ilist_n, sizeof(int)*2)) return DK_ENOMEM;
@@ -3042,7 +3091,8 @@ void FastDestroy(dk_ctx *c) {
   for (void *p : {(void*)f->nparts, (void*)f->item_start, (void*)f->item_tile, f->iscan_tmp,
                   (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr, (void*)f->ov_def,
                   (void*)f->ov_dc, (void*)f->ov_cnt, (void*)f->ilist[0], (void*)f->ilist[1],
-                  (void*)f->ilist_n, (void*)f->iflag, (void*)f->iflag_scan, f->lscan_tmp})
+                  (void*)f->ilist_n, (void*)f->iflag, (void*)f->iflag_scan, f->lscan_tmp,
+                  (void*)f->x3p})
     if (p) (void)hipFree(p);
   for (int b=0; b<2; ++b) {
     Store &S = f->s[b];
@@ -3198,6 +3248,22 @@ int FastNewDt(dk_ctx *c, double qo, double *partial, int nb) {
   return dk::Check(hipGetLastError());
 }
 
+// the last sort's integer output (keys in the order before it, the permutation, the cell starts)
+int FastSortState(dk_ctx *c, int which, int32_t *host, long long n) {
+  Fast *f = c->fast;
+  if (f == nullptr) return DK_ESTATE;
+  const int *src = nullptr;
+  long long want = 0;
+  if (which == 0) {src = f->key; want = f->np;}
+  else if (which == 1) {src = f->perm; want = f->np;}
+  else if (which == 2) {src = f->start; want = f->nkeys + 1;}
+  else return DK_EINVAL;
+  if (n != want) return DK_EINVAL;
+  if (n == 0) return DK_OK;
+  int e = dk::Check(hipMemcpyAsync(host, src, sizeof(int)*n, hipMemcpyDeviceToHost, c->stream));
+  return e ? e : dk::Check(hipStreamSynchronize(c->stream));
+}
+
 int FastDiag(dk_ctx *c, double out[9]) {
   Fast *f = c->fast;
   if (f == nullptr || c->u0 == nullptr) return DK_ESTATE;
@@ -3234,7 +3300,7 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   A.gam0 = 1.0; A.gam1 = 0.0; A.beta_dt = 0.0; A.atw_dt = 0.0;
   A.sh = 0.0; A.dt = st->dt;
   A.a_dt = st->dt; A.inv_a_dt = 1.0/st->dt;
-  A.rate_dep = c->P.stiffness;
+  A.rate_dep = dk::StiffnessActive(c->P);
   A.cell_sums = 0;
   {
     dk_gas_src z;
@@ -3264,7 +3330,7 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, f->qdep, nullptr, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
-    if (c->P.stiffness && (e = StiffnessRecord<float>(c, f->qdep, 4, 0, st->dt))) return e;
+    if (dk::StiffnessActive(c->P) && (e = StiffnessRecord<float>(c, f->qdep, 4, 0, st->dt))) return e;
     TIMED("pc2_apply", (k_pc2_apply<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->u0, f->qdep, c->P.pc2_damping, st->dt)));
   } else {
@@ -3317,6 +3383,9 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   if ((e = FillV<int>(c, f->count, f->nkeys + 1, 0))) return e;
   // the push of this stage writes its residue into the buffer the next stage's start reads
   TIMED("key", LaunchKey(c, A, 1, f->pres[1 - f->pres_rd]));
+  if (f->x3p != nullptr)   // the outflow ledger (bvals_part.cpp:184-186) of the items just run
+    TIMED("x3_ledger", (k_x3sum<<<1, 256, 0, c->stream>>>(f->item_start, T.nt, f->x3p,
+                                                          c->d_scalar + 12)));
   TIMED("fix", (k_fix_key<<<Grid((long long)T.nt*T.TC), kBlock, 0, c->stream>>>(
       A, 1, f->pres[1 - f->pres_rd])));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history
@@ -3562,6 +3631,12 @@ int dk_tune_config(dk_ctx *c, int32_t *state, int32_t *xcd, int32_t *key_threads
   *xcd = f ? f->T.xcd : 0;
   *key_threads = f ? f->key_threads : 0;
   return DK_OK;
+}
+
+int dk_sort_state(dk_ctx *c, int32_t which, int32_t *host, int64_t n) {
+  if (c == nullptr || host == nullptr) return DK_EINVAL;
+  if (c->form != DK_FORM_FP32) return DK_EUNSUPPORTED;
+  return dk::FastSortState(c, which, host, n);
 }
 
 int dk_set_tune_config(dk_ctx *c, int32_t xcd, int32_t key_threads) {

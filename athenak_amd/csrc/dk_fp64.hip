@@ -1510,12 +1510,12 @@ int dk_deposit_drag(dk_ctx *c, const dk_stage *st) {
     if (st->stage != 1) return DK_OK;
     if ((e = dk::Zero(c, c->qdep, 5*dk::NCells(c)))) return e;
     if (c->P.clouds) {   // dust_pm.cpp:130-137
-      if ((e = dk::CloudPredictor(c, st->dt, c->P.stiffness))) return e;
+      if ((e = dk::CloudPredictor(c, st->dt, dk::StiffnessActive(c->P)))) return e;
       return Launched(c);
     }
     {const int _pt = dk::ProfBegin(c, "pc2_predictor");
     k_pc2_predictor<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->u1, c->qdep, st->dt,
-                               c->P.stiffness, c->d_err);
+                               dk::StiffnessActive(c->P) ? 1 : 0, c->d_err);
      dk::ProfEnd(c, _pt);}
     return Launched(c);
   }
@@ -1541,7 +1541,7 @@ int dk_gas_implicit_solve(dk_ctx *c, const dk_stage *st) {
   if (e || !ActiveStage(c, st)) return e;
   if (c->P.coupling == DK_PC2) {
     if (st->stage == 1 && c->P.back_reaction) {
-      if (c->P.stiffness && (e = dk::StiffnessRecord<double>(c, c->qdep, 5, 4, st->dt))) return e;
+      if (dk::StiffnessActive(c->P) && (e = dk::StiffnessRecord<double>(c, c->qdep, 5, 4, st->dt))) return e;
       {const int _pt = dk::ProfBegin(c, "pc2_apply_predictor");
       k_pc2_apply_predictor<<<CGRID(ActiveCells(c))>>>(c->g, c->u0, c->qdep, c->P.pc2_damping,
                                                        st->dt);
@@ -1755,25 +1755,17 @@ int64_t dk_graph_launches(const dk_ctx *c) { return c ? c->graph_launches : 0; }
 
 // src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell).  The FP32 form
 // keeps its store sorted by (block, tile, cell) every stage, so it has nothing to do here.
+namespace {
+int SortScratch(dk_ctx *c, long long n);
+}  // namespace
+
 int dk_sort_particles(dk_ctx *c) {
   if (c != nullptr && c->form == DK_FORM_FP32) return c->fast ? DK_OK : DK_ESTATE;
   int e = Ready(c);
   if (e) return e;
   const long long n = c->np;
   if (n == 0) return DK_OK;
-  if (c->sort_cap < n) {
-    for (void *p : {(void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
-                    (void*)c->sort_ibuf})
-      if (p) (void)hipFree(p);
-    c->sort_keys = nullptr; c->sort_idx = nullptr; c->sort_rbuf = nullptr;
-    c->sort_ibuf = nullptr;
-    if (hipMalloc((void**)&c->sort_keys, sizeof(unsigned long long)*2*n) != hipSuccess ||
-        hipMalloc((void**)&c->sort_idx, sizeof(int)*2*n) != hipSuccess ||
-        hipMalloc((void**)&c->sort_rbuf, sizeof(double)*DK_NRDATA*n) != hipSuccess ||
-        hipMalloc((void**)&c->sort_ibuf, sizeof(int32_t)*DK_NIDATA*n) != hipSuccess)
-      return DK_ENOMEM;
-    c->sort_cap = n;
-  }
+  if ((e = SortScratch(c, n))) return e;
   {const int _pt = dk::ProfBegin(c, "sort_keys");
   k_sort_keys<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->sort_keys, c->sort_idx);
    dk::ProfEnd(c, _pt);}
@@ -1801,6 +1793,122 @@ int dk_sort_particles(dk_ctx *c) {
                               c->sort_ibuf);
    dk::ProfEnd(c, _pt);}
   return Launched(c);
+}
+
+// RemoveOutflowParticles (src/dust/dust_tasks.cpp:365-420): the keep flags (mass > 0), their
+// exclusive scan, the kept particles copied in order into the scratch columns, then back
+__global__ void k_keep_flags(const double *r, long long cap, long long np, int *flag) {
+  GRID_STRIDE(p, np) flag[p] = (r[(long long)DK_IPM*cap + p] > 0.0) ? 1 : 0;
+}
+__global__ void k_compact_out(const double *r, const int32_t *iv, long long cap, long long np,
+                              const int *flag, const int *pos, double *rb, int32_t *ib) {
+  GRID_STRIDE(p, np) {
+    if (!flag[p]) continue;
+    const long long k = pos[p];
+    for (int c=0; c<DK_NRDATA; ++c) rb[c*np + k] = r[c*cap + p];
+    for (int c=0; c<DK_NIDATA; ++c) ib[c*np + k] = iv[c*cap + p];
+  }
+}
+__global__ void k_compact_back(double *r, int32_t *iv, long long cap, long long np,
+                               long long nkeep, const double *rb, const int32_t *ib) {
+  GRID_STRIDE(k, nkeep) {
+    for (int c=0; c<DK_NRDATA; ++c) r[c*cap + k] = rb[c*np + k];
+    for (int c=0; c<DK_NIDATA; ++c) iv[c*cap + k] = ib[c*np + k];
+  }
+}
+
+namespace {
+
+// the sort scratch of the context for n particles (grown on demand)
+int SortScratch(dk_ctx *c, long long n) {
+  if (c->sort_cap >= n) return DK_OK;
+  for (void *p : {(void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
+                  (void*)c->sort_ibuf})
+    if (p) (void)hipFree(p);
+  c->sort_keys = nullptr; c->sort_idx = nullptr; c->sort_rbuf = nullptr;
+  c->sort_ibuf = nullptr;
+  if (hipMalloc((void**)&c->sort_keys, sizeof(unsigned long long)*2*n) != hipSuccess ||
+      hipMalloc((void**)&c->sort_idx, sizeof(int)*2*n) != hipSuccess ||
+      hipMalloc((void**)&c->sort_rbuf, sizeof(double)*DK_NRDATA*n) != hipSuccess ||
+      hipMalloc((void**)&c->sort_ibuf, sizeof(int32_t)*DK_NIDATA*n) != hipSuccess)
+    return DK_ENOMEM;
+  c->sort_cap = n;
+  return DK_OK;
+}
+
+// the caller's particle columns without the particles of zero mass, in order; c->np updated
+int CompactMassless(dk_ctx *c) {
+  const long long n = c->np;
+  if (n == 0) return DK_OK;
+  int e;
+  if ((e = SortScratch(c, n))) return e;
+  int *flag = c->sort_idx, *pos = c->sort_idx + n;
+  k_keep_flags<<<PGRID>>>(c->rdata, (long long)c->cap, n, flag);
+  size_t bytes = 0;
+  if ((e = dk::Check(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag, pos, (int)n,
+                                                      c->stream))))
+    return e;
+  if (bytes > c->sort_tmp_bytes) {
+    if (c->sort_tmp) (void)hipFree(c->sort_tmp);
+    if (hipMalloc(&c->sort_tmp, bytes) != hipSuccess) {c->sort_tmp = nullptr; return DK_ENOMEM;}
+    c->sort_tmp_bytes = bytes;
+  }
+  if ((e = dk::Check(hipcub::DeviceScan::ExclusiveSum(c->sort_tmp, bytes, flag, pos, (int)n,
+                                                      c->stream))))
+    return e;
+  k_compact_out<<<PGRID>>>(c->rdata, c->idata, (long long)c->cap, n, flag, pos, c->sort_rbuf,
+                           c->sort_ibuf);
+  int last[2] = {0, 0};
+  if ((e = dk::Check(hipMemcpyAsync(&last[0], pos + n - 1, sizeof(int), hipMemcpyDeviceToHost,
+                                    c->stream))) ||
+      (e = dk::Check(hipMemcpyAsync(&last[1], flag + n - 1, sizeof(int), hipMemcpyDeviceToHost,
+                                    c->stream))) ||
+      (e = dk::Check(hipStreamSynchronize(c->stream))))
+    return e;
+  const long long nkeep = (long long)last[0] + last[1];
+  k_compact_back<<<Grid(nkeep), kBlock, 0, c->stream>>>(c->rdata, c->idata, (long long)c->cap,
+                                                        n, nkeep, c->sort_rbuf, c->sort_ibuf);
+  c->np = nkeep;
+  return Launched(c);
+}
+
+}  // namespace
+
+int dk_remove_outflow_particles(dk_ctx *c, double out[4]) {
+  if (c == nullptr || out == nullptr) return DK_EINVAL;
+  if (c->rdata == nullptr || c->u0 == nullptr) return DK_ESTATE;
+  int e;
+  out[0] = out[1] = 0.0;
+  if (c->M.bc_x3 == DK_X3_OUTFLOW) {
+    double led[2] = {0.0, 0.0};
+    if ((e = dk::Check(hipMemcpyAsync(led, c->d_scalar + 12, sizeof(led),
+                                      hipMemcpyDeviceToHost, c->stream))) ||
+        (e = dk::Check(hipStreamSynchronize(c->stream))))
+      return e;
+    if (led[1] != 0.0) {
+      // the FP32 form: its store back into the caller's columns, compacted there, bound again
+      if (c->form == DK_FORM_FP32 && (e = dk::FastExport(c))) return e;
+      if ((e = CompactMassless(c))) return e;
+      if (c->form == DK_FORM_FP32 && (e = dk::FastBind(c))) return e;
+      out[0] = led[0]; out[1] = led[1];
+      c->x3_out_mass += led[0];
+      c->x3_out_count += led[1];
+      if ((e = dk::Check(hipMemsetAsync(c->d_scalar + 12, 0, 2*sizeof(double), c->stream))))
+        return e;
+    }
+  }
+  out[2] = c->x3_out_mass;
+  out[3] = c->x3_out_count;
+  return dk::Check(hipStreamSynchronize(c->stream));
+}
+
+int dk_x3_removed(dk_ctx *c, double out[2]) {
+  if (c == nullptr || out == nullptr) return DK_EINVAL;
+  int e;
+  if ((e = dk::Check(hipMemcpyAsync(out, c->d_scalar + 12, 2*sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream))))
+    return e;
+  return dk::Check(hipStreamSynchronize(c->stream));
 }
 
 int dk_new_time_step(dk_ctx *c, double dt_cfl, double *dt_out) {

@@ -67,6 +67,11 @@ __device__ inline void PMWeights(const double x, const double xmin, const double
   }
 }
 
+// StiffnessActive (src/dust/dust.hpp:1150-1152): the pc2 predictor deposits the rate field
+// qdep[4] and UpdateStiffness records chi whenever the stiffness record is asked for or the
+// feedback damping of stiff cells needs it (phst_track, the particle history, is not built)
+inline bool StiffnessActive(const dk_physics &P) { return P.stiffness != 0 || P.pc2_damping != 0; }
+
 // First error wins: code and particle index.
 struct ErrWord {
   int code;
@@ -124,6 +129,9 @@ struct dk_ctx {
   struct Fast *fast = nullptr;
   int graphs = 0;
   long long graph_launches = 0;    // stage graphs launched (dk_graph_launches)
+  // RemoveOutflowParticles run totals (x3_outflow_mass, x3_outflow_count, dust.hpp); the ledger
+  // since the last removal is d_scalar[12..13] (bvals_part.cpp:184-186)
+  double x3_out_mass = 0.0, x3_out_count = 0.0;
   hipGraphExec_t gexec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-kernel event timing
   struct Prof *prof = nullptr;
@@ -159,6 +167,7 @@ int FastBind(dk_ctx *c);
 int FastExport(dk_ctx *c);
 int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);
 int FastDiag(dk_ctx *c, double out[9]);
+int FastSortState(dk_ctx *c, int which, int32_t *host, long long n);
 int FastNewDt(dk_ctx *c, double qo, double *partial, int nb);
 // RefreshStoppingTimeMaximum over the FP32 store: nb partial (max, min) pairs
 int FastStoppingTimes(dk_ctx *c, double *partial, int nb);

@@ -24,7 +24,8 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_prof_reset", "dk_prof_read", "dk_autotune", "dk_tune_config",
             "dk_set_solver", "dk_solver_stats",
             "dk_set_tune_config", "dk_graph_launches", "dk_stiffness",
-            "dk_stopping_times"] +
+            "dk_stopping_times", "dk_remove_outflow_particles", "dk_x3_removed",
+            "dk_sort_state"] +
            ["dk_" + t for t in TASKS])
 
 
@@ -56,6 +57,9 @@ def lib():
         L.dk_set_graphs.argtypes = [P, C.c_int32]
         L.dk_stiffness.argtypes = [P, C.POINTER(C.c_double)]
         L.dk_stopping_times.argtypes = [P, C.POINTER(C.c_double)]
+        L.dk_remove_outflow_particles.argtypes = [P, C.POINTER(C.c_double)]
+        L.dk_x3_removed.argtypes = [P, C.POINTER(C.c_double)]
+        L.dk_sort_state.argtypes = [P, C.c_int32, P, C.c_int64]
         L.dk_graph_launches.restype = C.c_int64
         L.dk_graph_launches.argtypes = [P]
         L.dk_autotune.argtypes = [P, C.c_int32]
@@ -207,6 +211,37 @@ class DustContext:
         out = (C.c_double * 2)()
         self._check(lib().dk_stopping_times(self.h, out), "dk_stopping_times")
         return out[0], out[1]
+
+    def remove_outflow_particles(self):
+        """RemoveOutflowParticles: (mass, count) removed now, then the run totals; the
+        particle count drops by the count."""
+        import numpy as np
+        out = (C.c_double * 4)()
+        self._check(lib().dk_remove_outflow_particles(self.h, out),
+                    "dk_remove_outflow_particles")
+        self.npart = lib().dk_npart(self.h)
+        return np.array(out[:])
+
+    def sort_state(self):
+        """The FP32 form's last sort: (keys in the order before it, permutation new -> old
+        slot, the first slot of every key), int32 numpy arrays."""
+        import numpy as np
+        m = self.mesh
+        nk = m.nmb * m.nx1 * m.nx2 * m.nx3
+        out = []
+        for which, n in ((0, self.npart), (1, self.npart), (2, nk + 1)):
+            a = np.zeros(max(n, 1), dtype=np.int32)
+            self._check(lib().dk_sort_state(self.h, which, C.c_void_p(a.ctypes.data), n),
+                        "dk_sort_state")
+            out.append(a[:n])
+        return tuple(out)
+
+    def x3_removed(self):
+        """The outflow ledger since the last removal: (mass, count)."""
+        import numpy as np
+        out = (C.c_double * 2)()
+        self._check(lib().dk_x3_removed(self.h, out), "dk_x3_removed")
+        return np.array(out[:])
 
     def new_time_step_pc2(self, dt_cfl, stiff_thresh=0.0):
         """NewTimeStep (dust_newdt.cpp:108-149) with the pc2 stiffness limit: min(dt_cfl min

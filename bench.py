@@ -3,10 +3,12 @@
 
 A step is one imex2+ cycle (two working stages of push, deposit, dc1 solve, gather-kick and
 back-reaction apply, plus the harness's gas source-term update) over every particle of the
-workload, with all inputs resident in HBM.  The default workload is the unstratified shearing
-box of 256^3 cells in 64 blocks of 64^3 with 4 particles per cell (67,108,864 particles), St 0.1,
-eps 3, dc1.  `--workload c4p`: a step is one rk2 cycle of the pc2 coupling (predictor, advance)
-on 128^3 cells in 512 blocks of 16^3 with clumps (8,388,608 point particles, stiff cells damped).
+workload, with all inputs resident in HBM.  The default workload is the stratified shearing box
+with a continuous size distribution: 256^3 cells in 64 blocks of 64^3, 7 log-uniform Stokes bins
+over [1e-3, 1] with one particle per bin per cell (117,440,512 particles), x3 outflow faces,
+imex2+/dc1.  `--workload c2`: the unstratified shearing box at ppc 4 (67,108,864 particles).
+`--workload c4p`: a step is one rk2 cycle of the pc2 coupling (predictor, advance) on 128^3
+cells in 512 blocks of 16^3 with clumps (8,388,608 point particles, stiff cells damped).
 
 Prints one JSON line: value = Mparticle-steps/s of the measured form, the roofline of the
 dominant kernel (algorithmic bytes per launch / measured average launch time, HIP events on the
@@ -17,6 +19,8 @@ bounded sample of the same workload).
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -85,11 +89,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c3")
     ap.add_argument("--form", default="both", choices=["fp64", "fp32", "both"])
     ap.add_argument("--n", type=int, default=None, help="override the mesh size (tests)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-ledger", action="store_true",
+                    help="skip the drag-exchange conservation run of workloads with sources")
     ap.add_argument("--no-tune", action="store_true",
                     help="FP32 form: keep the default launch parameters (no autotuner steps)")
     ap.add_argument("--tune-config", default=None, metavar="XCD,KT",
@@ -162,10 +168,19 @@ def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True, tune_
             ctx.step(w.dt, w.src)
     for _ in range(warmup):
         ctx.step(w.dt, w.src)
+    if w.mesh.bc_x3 == abi.DK_X3_OUTFLOW:
+        # the reference removes the particles parked at an outflow face before every cycle
+        # (RemoveOutflowParticles, before_timeintegrator); here once, outside the timed steps
+        ctx.remove_outflow_particles()
     ctx.check_device_error()
+    n_timed = ctx.npart
     stream = ctx.stream
     graphs = form == "fp32"
     ctx.set_graphs(graphs)
+    if graphs:
+        # the first graph-launched step captures and instantiates every stage graph: untimed
+        ctx.step(w.dt, w.src)
+        ctx.check_device_error()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -200,22 +215,61 @@ def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True, tune_
     n = ctx.npart
     order = torch.argsort(ctx.idata[abi.PTAG, :n])
     vel = ctx.rdata[[abi.IPVX, abi.IPVY, abi.IPVZ], :n][:, order].clone()
+    tags = ctx.idata[abi.PTAG, :n][order].clone()
     ctx.close()
     del ctx
     torch.cuda.empty_cache()
-    return {"ms": ms, "wall_s": wall, "prof": prof, "drift": drift, "vel": vel,
+    return {"ms": ms, "wall_s": wall, "prof": prof, "drift": drift, "vel": vel, "npart": n,
+            "tags": tags, "n_timed": n_timed,
             "ms_prof": ms_prof, "prof_steps": prof_steps, "tune_steps": tune_steps,
             "tuned": tuned, "graphs": graphs}
+
+
+def has_sources(w):
+    return bool(w.phys.shearing_box or w.phys.stratified or
+                (w.src is not None and (w.src.shearing_box or w.src.stratified or
+                                        w.src.const_accel != 0.0)))
+
+
+def exchange_drift(name, form, steps=3):
+    """The drag exchange alone at the workload's full size: the same mesh (shear-periodic x1 and
+    the x3 faces included), particles, stopping times, masses, gas and dt, with the momentum
+    sources switched off (no shearing-box Coriolis and tidal terms, no vertical gravity, no gas
+    source terms), so that total momentum must be conserved; its drift over `steps` steps
+    relative to sum m|v| + sum V|M|.  None for a workload without sources (its run reports its own
+    drift)."""
+    import torch
+    from athenak_amd import abi
+    from athenak_amd.lib import DustContext
+    w = make_workload(name)
+    if not has_sources(w):
+        return None
+    w.phys.shearing_box, w.phys.stratified = 0, 0
+    if w.setup.dust_noise == 0.0:
+        w.setup.dust_noise = 1e-3     # (a dust velocity to exchange)
+    fid = abi.DK_FORM_FP64 if form == "fp64" else abi.DK_FORM_FP32
+    ctx = DustContext(w.mesh, w.blocks, w.phys, w.npart, form=fid, stream=torch.cuda.Stream())
+    ctx.setup(w.setup)
+    d0 = ctx.diag()
+    for _ in range(steps):
+        ctx.step(w.dt, None)
+    ctx.check_device_error()
+    d1 = ctx.diag()
+    ctx.close()
+    del ctx
+    torch.cuda.empty_cache()
+    return {"value": float(abs((d1[0:3] + d1[3:6]) - (d0[0:3] + d0[3:6])).max() /
+                           max(d0[8], d1[8], 1e-300)),
+            "steps": steps,
+            "how": "the same mesh, particles and gas at full size with the momentum sources "
+                   "off (drag exchange alone)"}
 
 
 def drift_or_none(w, drift):
     """The total-momentum drift where total momentum is conserved; None where the workload has
     momentum sources (shearing-box Coriolis and tidal terms, vertical gravity, the radial
     forcing), whose change is physics, not a conservation error."""
-    sources = (w.phys.shearing_box or w.phys.stratified or
-               (w.src is not None and (w.src.shearing_box or w.src.stratified or
-                                       w.src.const_accel != 0.0)))
-    return None if sources else drift
+    return None if has_sources(w) else drift
 
 
 def cpu_baseline(name, budget_s):
@@ -245,27 +299,36 @@ def cpu_baseline(name, budget_s):
         if el > budget_s or steps >= 20:
             break
     rate = w.npart * steps / el / 1e6
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": round(rate, 4), "unit": "Mparticle-steps/s", "cores": cores, "kind": "port",
+            "cpu_model": model,
             "sample": "CPU FP64 oracle (reference kernels restated, OpenMP atomics) on %s physics "
                       "with %d blocks of %d^3 (%d particles), %d steps after 1 warm-up, %.1f s"
                       % (name, w.mesh.nmb, w.mesh.nx1, w.npart, steps, el)}
 
 
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of
-# known size) of the C2 FP32 run, committed under profiles/.  A timer label covers one launch of
-# every kernel of a group (a tile kernel is two launches: its first items and its extra parts);
-# labels with two groups (P3 with and without the stage-2 rate store) take their mean.
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "hbm_traffic_c2_fp32.md")
+# known size) of the workload's FP32 run, committed under profiles/ as hbm_traffic_<name>_fp32.md.
+# A timer label covers one launch of every kernel of a group (a tile kernel is two launches: its
+# first items and its extra parts); labels with two groups (P3 with and without the stage-2 rate
+# store) take their mean.
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 TRAFFIC_KERNEL = {"p1_gather_deposit": [["k_p1<"]], "p2_operator": [["k_p2<"]],
                   "p3_gather_kick": [["k_p3<"], ["k_p3n<"]], "key": [["k_key<"]]}
 
 
 def measured_traffic(label, w, form):
-    if w.name != "c2" or form != "fp32" or label not in TRAFFIC_KERNEL:
+    if form != "fp32" or label not in TRAFFIC_KERNEL:
         return None
     try:
-        lines = open(TRAFFIC_FILE).read().splitlines()
+        lines = open(os.path.join(PROFILES, "hbm_traffic_%s_fp32.md" % w.name)).read().splitlines()
     except OSError:
         return None
     per_kernel = {}
@@ -297,8 +360,10 @@ def kernel_roofline(r, w, form="fp32"):
 
 
 def rate(r, w, steps):
+    """(ms per step, Mparticle-steps/s) over the particles the timed steps carried (after an
+    outflow removal, fewer than the set-up's)."""
     ms_step = r["ms"] / steps
-    return ms_step, w.npart / (ms_step * 1e-3) / 1e6
+    return ms_step, r["n_timed"] / (ms_step * 1e-3) / 1e6
 
 
 def copy_calibration(gib):
@@ -355,6 +420,7 @@ def main():
                           "achieved": round(mps * 1e6 * step_alg / 1e9, 1), "unit": "GB/s",
                           "frac": round(mps * 1e6 * step_alg / 1e9 / HBM_PEAK_GBS, 4)},
         "momentum_drift": drift_or_none(w, r["drift"]),
+        "particles_timed": r["n_timed"],
         "graphs": r["graphs"],
         "autotune": r["tuned"],
         "profiled_steps": {"steps": r["prof_steps"], "graphs": False,
@@ -365,6 +431,14 @@ def main():
     if "fp64" in res and "fp32" in res:
         ms64, mps64 = rate(res["fp64"], w, args.steps)
         v64, v32 = res["fp64"]["vel"], res["fp32"]["vel"]
+        t64, t32 = res["fp64"]["tags"], res["fp32"]["tags"]
+        if not (len(t64) == len(t32) and bool((t64 == t32).all())):
+            # (the particles both forms still hold, by tag)
+            import torch
+            common = torch.from_numpy(np.intersect1d(t64.cpu().numpy(), t32.cpu().numpy()))
+            common = common.to(t64.device)
+            v64 = v64[:, torch.searchsorted(t64, common)]
+            v32 = v32[:, torch.searchsorted(t32, common)]
         # per particle: the largest component difference over the largest velocity
         dp = (v32 - v64).abs().amax(dim=0) / v64.abs().max()
         verr = float(dp.max())
@@ -381,6 +455,13 @@ def main():
         # so a particle whose FP32 and FP64 positions (1e-12..1e-9 apart) straddle a cell face
         # gets a different correction; over many steps a few of millions do
         out["particles_beyond_1e-5_vs_fp64"] = {"count": nbad, "of": int(dp.numel())}
+    if out["momentum_drift"] is None and not args.no_ledger:
+        print("[bench] drag-exchange conservation run (%s, sources off)" % main_form,
+              file=sys.stderr, flush=True)
+        ex = exchange_drift(args.workload, main_form)
+        if ex is not None:
+            out["momentum_drift"] = ex["value"]
+            out["momentum_drift_how"] = "%s, %d steps" % (ex["how"], ex["steps"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -120,7 +120,9 @@ typedef struct dk_physics {
   int32_t stratified;       /* vertical gravity -Omega^2 z */
   double omega0, qshear;
   int32_t pc2_damping;      /* pc2 feedback damping of stiff cells (dust_pm.cpp:389-398) */
-  int32_t stiffness;        /* deposit the pc2 rate field qdep[4] (StiffnessActive) */
+  int32_t stiffness;        /* record chi (UpdateStiffness); the pc2 rate field qdep[4] is
+                               deposited and chi recorded when stiffness or pc2_damping is set
+                               (StiffnessActive, src/dust/dust.hpp:1150-1152) */
   int32_t clouds;           /* pc2 with 3-D refinable clouds at the mesh level (plain codes:
                                CloudPredictorT / CloudAdvanceT of dust_cloud3d.cpp:1369-1536);
                                same-level meshes periodic in x1, x2, x3 only (DK_EUNSUPPORTED
@@ -204,6 +206,19 @@ int dk_add_drag_history_gas(dk_ctx *ctx, const dk_stage *st);
  * src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell), all columns.  The
  * FP32 form keeps its particles sorted by (block, tile, cell) every stage: a no-op there. */
 int dk_sort_particles(dk_ctx *ctx);
+/* DustGasDrag::RemoveOutflowParticles, src/dust/dust_tasks.cpp:365-420 (both forms): with an
+ * outflow x3 face (DK_X3_OUTFLOW) and a non-zero ledger of particles parked at the faces since
+ * the last call (bvals_part.cpp:184-186), every particle of zero mass is removed from the
+ * caller's arrays, the others keep their order, and dk_npart drops accordingly (the FP32 form
+ * exports its store, compacts the caller's arrays and binds them again).  out[0], out[1]: the
+ * mass and count removed by this call; out[2], out[3]: the run totals (x3_outflow_mass,
+ * x3_outflow_count).  Call it between steps, as the reference's before_timeintegrator task;
+ * waits for the stream. */
+int dk_remove_outflow_particles(dk_ctx *ctx, double out[4]);
+/* The outflow ledger since the last removal: out[0] mass, out[1] count (bvals_part.cpp:184-186,
+ * BoundaryValuesPart::x3_removed).  The FP32 form sums it over its work items in a fixed order.
+ * Waits for the stream. */
+int dk_x3_removed(dk_ctx *ctx, double out[2]);
 /* DustGasDrag::NewTimeStep, src/dust/dust_newdt.cpp:108-149: dt_out = dt_cfl min dx/|v|
  * (both forms; the FP32 form applies a pending correction first, from its FP32 velocities) */
 int dk_new_time_step(dk_ctx *ctx, double dt_cfl, double *dt_out);
@@ -256,6 +271,15 @@ int dk_gas_hook(dk_ctx *ctx, const dk_stage *st, const dk_gas_src *src);
 int dk_fill_gas_ghosts(dk_ctx *ctx, const dk_stage *st);
 
 /* ---- diagnostics ------------------------------------------------------------------- */
+
+/* The FP32 form's counting sort of its last stage (no counterpart in the reference: it lets a
+ * caller check the sort's integer output against an independent stable sort).  which 0: the
+ * tile-major key of every particle in the order before the sort (n = dk_npart entries; key =
+ * (block*tiles_per_block + tile)*cells_per_tile + cell in tile), 1: the permutation (slot of the
+ * new order -> slot of the old order, n entries), 2: the first slot of every key in the new order
+ * (nmb*nx1*nx2*nx3 + 1 entries).  host: n int32 of host memory; DK_EINVAL when n is not that size,
+ * DK_EUNSUPPORTED in the FP64 form.  Waits for the stream. */
+int dk_sort_state(dk_ctx *ctx, int32_t which, int32_t *host, int64_t n);
 
 /* FP64 sums over particles and active cells (waits for the stream):
  * out[0..2] dust momentum sum m v, out[3..5] gas momentum sum V M, out[6] dust mass,

@@ -121,7 +121,10 @@ struct or_sim {
   int nthreads = 1;
   int err = 0;
   int64_t err_p = -1;
+  // the x3 outflow ledger since the last removal (bvals_part.cpp:184-186) and the run totals of
+  // RemoveOutflowParticles (dust_tasks.cpp:407-408)
   double x3_removed_mass = 0.0, x3_removed_count = 0.0;
+  double x3_outflow_mass = 0.0, x3_outflow_count = 0.0;
   // refinable clouds: the root cell size (the cloud half-width at the mesh level) and the
   // packed gather field (m, k, j, i, 12): u, and its central differences in x1, x2, x3
   double root_dx[3] = {0.0, 0.0, 0.0}, iroot_dx[3] = {0.0, 0.0, 0.0};
@@ -1310,7 +1313,8 @@ int or_update_particle_gids(or_sim *s, const dk_stage *st) {
 // src/dust/dust_pm.cpp:104-277
 int or_deposit_drag(or_sim *s, const dk_stage *st) {
   if (!ActiveStage(s, st) || !s->P.back_reaction) return DK_OK;
-  const bool rate_dep = s->P.stiffness;
+  // StiffnessActive (src/dust/dust.hpp:1150-1152): the stiffness record or the damping
+  const bool rate_dep = s->P.stiffness || s->P.pc2_damping;
   if (s->P.coupling == DK_PC2) {
     if (st->stage != 1) return DK_OK;
     Zero(s->qdep);
@@ -1675,7 +1679,7 @@ int or_gas_implicit_solve(or_sim *s, const dk_stage *st) {
       const double dt = st->dt;
       // UpdateStiffness (dust_pm.cpp:288-342): chi = dt max_cells qdep[4]/rho over the active
       // cells, before the predictor is applied; the running maximum kept beside it
-      if (s->P.stiffness) {
+      if (s->P.stiffness || s->P.pc2_damping) {   // StiffnessActive (dust_pm.cpp:377)
         double rate = 0.0;
         for (int m=0; m<nmb; ++m)
           for (int k=s->ks; k<=s->ke; ++k)
@@ -2023,6 +2027,41 @@ int or_stage_run(or_sim *s, const dk_stage *st, const dk_gas_src *src) {
 }
 
 // src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell of the position)
+// DustGasDrag::RemoveOutflowParticles (src/dust/dust_tasks.cpp:365-420): with an outflow x3
+// face and a non-zero ledger, every particle of zero mass is removed, the others keep their order
+// (the exclusive scan of the keep flags); the ledger goes to the run totals and is cleared.
+// out: this call's removed mass and count, then the run totals
+int or_remove_outflow_particles(or_sim *s, double out[4]) {
+  out[0] = out[1] = 0.0;
+  if (s->M.bc_x3 == DK_X3_OUTFLOW && s->x3_removed_count != 0.0) {
+    int64_t k = 0;
+    for (int64_t p=0; p<s->np; ++p) {
+      if (!(s->R(DK_IPM,p) > 0.0)) continue;
+      if (k != p) {
+        for (int c=0; c<DK_NRDATA; ++c) s->R(c,k) = s->R(c,p);
+        for (int c=0; c<DK_NIDATA; ++c) s->I(c,k) = s->I(c,p);
+      }
+      ++k;
+    }
+    s->np = k;
+    out[0] = s->x3_removed_mass;
+    out[1] = s->x3_removed_count;
+    s->x3_outflow_mass += s->x3_removed_mass;
+    s->x3_outflow_count += s->x3_removed_count;
+    s->x3_removed_mass = 0.0;
+    s->x3_removed_count = 0.0;
+  }
+  out[2] = s->x3_outflow_mass;
+  out[3] = s->x3_outflow_count;
+  return DK_OK;
+}
+
+// the ledger since the last removal (mass, count)
+void or_x3_removed(or_sim *s, double out[2]) {
+  out[0] = s->x3_removed_mass;
+  out[1] = s->x3_removed_count;
+}
+
 int or_sort_particles(or_sim *s) {
   const int64_t n = s->np;
   std::vector<int64_t> key(n), ord(n);

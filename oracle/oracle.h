@@ -86,6 +86,10 @@ void or_solver_stats(or_sim *s, double out[7]);
  * the rate max */
 void or_stiffness(or_sim *s, double out[3]);
 int or_last_error(or_sim *s, int64_t *particle);
+/* RemoveOutflowParticles (dust_tasks.cpp:365-420): out[0..1] mass and count removed now,
+ * out[2..3] the run totals; or_x3_removed: the ledger since the last removal */
+int or_remove_outflow_particles(or_sim *s, double out[4]);
+void or_x3_removed(or_sim *s, double out[2]);
 
 /* imex2+ (gamma = 1+1/sqrt(2)) and rk2 tableaux of the reference driver */
 void or_imex2p_coef(double time, double dt, dk_stage *st_out /* [3] */);

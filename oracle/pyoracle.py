@@ -60,6 +60,8 @@ def lib():
                                     C.c_double, C.c_double, C.c_double]
         L.or_solver_stats.argtypes = [P, D]
         L.or_stiffness.argtypes = [P, D]
+        L.or_remove_outflow_particles.argtypes = [P, D]
+        L.or_x3_removed.argtypes = [P, D]
         _LIB = L
     return _LIB
 
@@ -186,6 +188,18 @@ class Oracle:
         maximum."""
         out = (C.c_double * 3)()
         lib().or_stiffness(self.h, out)
+        return np.array(out[:])
+
+    def remove_outflow_particles(self):
+        """RemoveOutflowParticles: (mass, count) removed now, (mass, count) run totals."""
+        out = (C.c_double * 4)()
+        lib().or_remove_outflow_particles(self.h, out)
+        return np.array(out[:])
+
+    def x3_removed(self):
+        """The outflow ledger since the last removal: (mass, count)."""
+        out = (C.c_double * 2)()
+        lib().or_x3_removed(self.h, out)
         return np.array(out[:])
 
     def apply_operator(self, a_dt, x, sh=0.0):

@@ -121,7 +121,8 @@ def test_pcg_budget_too_small_is_reported():
 def test_pc2_stiffness_record_is_the_max_rate_over_rho():
     # UpdateStiffness (dust_pm.cpp:288-342): chi = dt max over the active cells of qdep[4]/rho,
     # taken before the predictor is applied (rho is not changed by the cycle: zero fluxes), and
-    # its running maximum; the stiffness field off records nothing
+    # its running maximum; with neither the record nor the damping asked for (StiffnessActive,
+    # dust.hpp:1150-1152) nothing is recorded
     w = workloads.c1(n=16, nb=2, ppc=2, coupling=abi.DK_PC2, solver=abi.DK_LOCAL)
     w.phys.stiffness, w.phys.pc2_damping = 1, 1
     w.setup.taus[0], w.setup.dust_noise, w.dt = 2e-3, 0.5, 4e-3
@@ -139,7 +140,7 @@ def test_pc2_stiffness_record_is_the_max_rate_over_rho():
         assert s[2] == (q4 / rho).max(), s
         assert s[1] == max(chis), (s, chis)
     assert chis[0] > 1.0          # (the damping case: stiff cells exist)
-    w.phys.stiffness = 0
+    w.phys.stiffness, w.phys.pc2_damping = 0, 0
     o0 = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
     o0.setup(w.setup)
     o0.step(w.dt, w.src)
