@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <utility>
 
 #include "dk_internal.hpp"
@@ -1267,6 +1268,10 @@ const char *dk_strerror(int code) {
   }
 }
 
+namespace {
+int ScratchInit(int device);
+}  // namespace
+
 int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phys,
               int32_t form, void *stream, dk_ctx **out) {
   if (mesh == nullptr || blocks == nullptr || phys == nullptr || out == nullptr)
@@ -1291,6 +1296,9 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
   c->form = form;
   c->stream = static_cast<hipStream_t>(stream);
   (void)hipGetDevice(&c->device);
+  // the device address of this GPU's sort scratch (dk_sort_particles_noalloc); a failure is
+  // kept for that entry point, which returns it, and leaves every other entry point usable
+  c->noalloc_err = ScratchInit(c->device);
   Geo &g = c->g;
   g.nmb = mesh->nmb; g.nx1 = mesh->nx1; g.nx2 = mesh->nx2; g.nx3 = mesh->nx3; g.ng = mesh->ng;
   g.n1 = g.nx1 + 2*g.ng; g.n2 = g.nx2 + 2*g.ng; g.n3 = g.nx3 + 2*g.ng;
@@ -1759,6 +1767,54 @@ namespace {
 int SortScratch(dk_ctx *c, long long n);
 }  // namespace
 
+namespace {
+
+// The buffers of one cell sort of n particles: keys and ranks twice (radix sort input and
+// output), every column once (the permuted copy), the radix sort's temporary storage.
+struct SortBufs {
+  unsigned long long *keys;
+  int *idx;
+  double *rbuf;
+  int32_t *ibuf;
+  void *tmp;
+  size_t tmp_bytes;
+};
+
+int SortRadixBits(const dk_ctx *c) {
+  return 64 - __builtin_clzll((unsigned long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3);
+}
+
+// the radix sort's temporary bytes for n pairs (a size query: no work, no allocation)
+int SortTmpBytes(const dk_ctx *c, const SortBufs &b, long long n, size_t &bytes) {
+  bytes = 0;
+  return dk::Check(hipcub::DeviceRadixSort::SortPairs(
+      nullptr, bytes, b.keys, b.keys + n, b.idx, b.idx + n, (int)n, 0, SortRadixBits(c),
+      c->stream));
+}
+
+// keys of (block, cell), their stable radix sort, the columns gathered into the sorted order
+// and copied back (timed: per-kernel event timing, which the allocation-free entry leaves out)
+int SortInto(dk_ctx *c, const SortBufs &b, long long n, bool timed) {
+  int e;
+  {const int _pt = timed ? dk::ProfBegin(c, "sort_keys") : -1;
+  k_sort_keys<<<PGRID>>>(c->g, c->d_blocks, PARGS, b.keys, b.idx);
+   if (timed) dk::ProfEnd(c, _pt);}
+  size_t bytes = b.tmp_bytes;
+  if ((e = dk::Check(hipcub::DeviceRadixSort::SortPairs(
+           b.tmp, bytes, b.keys, b.keys + n, b.idx, b.idx + n, (int)n, 0, SortRadixBits(c),
+           c->stream))))
+    return e;
+  {const int _pt = timed ? dk::ProfBegin(c, "permute") : -1;
+  k_permute<<<PGRID>>>(c->rdata, c->idata, (long long)c->cap, n, b.idx + n, b.rbuf, b.ibuf);
+   if (timed) dk::ProfEnd(c, _pt);}
+  {const int _pt = timed ? dk::ProfBegin(c, "unpermute_copy") : -1;
+  k_unpermute_copy<<<PGRID>>>(c->rdata, c->idata, (long long)c->cap, n, b.rbuf, b.ibuf);
+   if (timed) dk::ProfEnd(c, _pt);}
+  return Launched(c);
+}
+
+}  // namespace
+
 int dk_sort_particles(dk_ctx *c) {
   if (c != nullptr && c->form == DK_FORM_FP32) return c->fast ? DK_OK : DK_ESTATE;
   int e = Ready(c);
@@ -1766,33 +1822,85 @@ int dk_sort_particles(dk_ctx *c) {
   const long long n = c->np;
   if (n == 0) return DK_OK;
   if ((e = SortScratch(c, n))) return e;
-  {const int _pt = dk::ProfBegin(c, "sort_keys");
-  k_sort_keys<<<PGRID>>>(c->g, c->d_blocks, PARGS, c->sort_keys, c->sort_idx);
-   dk::ProfEnd(c, _pt);}
+  SortBufs b{c->sort_keys, c->sort_idx, c->sort_rbuf, c->sort_ibuf, nullptr, 0};
   size_t bytes = 0;
-  const int bits = 64 - __builtin_clzll((unsigned long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3);
-  if ((e = dk::Check(hipcub::DeviceRadixSort::SortPairs(
-           nullptr, bytes, c->sort_keys, c->sort_keys + n, c->sort_idx, c->sort_idx + n, (int)n,
-           0, bits, c->stream))))
-    return e;
+  if ((e = SortTmpBytes(c, b, n, bytes))) return e;
   if (bytes > c->sort_tmp_bytes) {
     if (c->sort_tmp) (void)hipFree(c->sort_tmp);
     if (hipMalloc(&c->sort_tmp, bytes) != hipSuccess) {c->sort_tmp = nullptr; return DK_ENOMEM;}
     c->sort_tmp_bytes = bytes;
   }
-  if ((e = dk::Check(hipcub::DeviceRadixSort::SortPairs(
-           c->sort_tmp, bytes, c->sort_keys, c->sort_keys + n, c->sort_idx, c->sort_idx + n,
-           (int)n, 0, bits, c->stream))))
-    return e;
-  {const int _pt = dk::ProfBegin(c, "permute");
-  k_permute<<<PGRID>>>(c->rdata, c->idata, (long long)c->cap, n, c->sort_idx + n, c->sort_rbuf,
-                       c->sort_ibuf);
-   dk::ProfEnd(c, _pt);}
-  {const int _pt = dk::ProfBegin(c, "unpermute_copy");
-  k_unpermute_copy<<<PGRID>>>(c->rdata, c->idata, (long long)c->cap, n, c->sort_rbuf,
-                              c->sort_ibuf);
-   dk::ProfEnd(c, _pt);}
-  return Launched(c);
+  b.tmp = c->sort_tmp;
+  b.tmp_bytes = bytes;
+  return SortInto(c, b, n, true);
+}
+
+//----------------------------------------------------------------------------------------
+// The allocation-free cell sort (dk_sort_particles_noalloc).  Its buffers live in one
+// file-scope device array per GPU, held until the process ends: at most DK_SORT_NOALLOC_MAX
+// particles (keys and ranks twice, the 17 + 5 columns once: 180 B per particle) and the radix
+// sort's temporary storage in the rest.  In host code the array's name is not a device address:
+// every host use goes through the pointer hipGetSymbolAddress returned for the device, taken
+// once per device when a context is created on it.
+
+namespace {
+constexpr long long kSortNoAllocMax = DK_SORT_NOALLOC_MAX;
+constexpr size_t kSortNoAllocTmp = size_t(8) << 20;
+constexpr size_t kSortNoAllocBytes =
+    (size_t)kSortNoAllocMax*(2*sizeof(unsigned long long) + 2*sizeof(int) +
+                             DK_NRDATA*sizeof(double) + DK_NIDATA*sizeof(int32_t)) +
+    4*256 + kSortNoAllocTmp;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kSortNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// with `device` current: the device address of its copy of scratch (once per device)
+int ScratchInit(int device) {
+  if (device < 0 || device >= kMaxDevices) return DK_ECAPACITY;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] != nullptr) return DK_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr)
+    return DK_EHIP;
+  g_scratch_ptr[device] = p;
+  return DK_OK;
+}
+
+void *ScratchPtr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
+int dk_sort_particles_noalloc(dk_ctx *c) {
+  if (c != nullptr && c->form == DK_FORM_FP32) return c->fast ? DK_OK : DK_ESTATE;
+  int e = Ready(c);
+  if (e) return e;
+  const long long n = c->np;
+  if (n > kSortNoAllocMax) return DK_ECAPACITY;
+  if (n == 0) return DK_OK;
+  if (c->noalloc_err) return c->noalloc_err;   // (no address obtained at dk_create)
+  char *base = static_cast<char*>(ScratchPtr(c->device));
+  if (base == nullptr) return DK_EHIP;
+  char *p = base;
+  auto take = [&p](size_t bytes) {char *q = p; p += (bytes + 255)/256*256; return q;};
+  SortBufs b;
+  b.keys = reinterpret_cast<unsigned long long*>(take(2*sizeof(unsigned long long)*n));
+  b.idx = reinterpret_cast<int*>(take(2*sizeof(int)*n));
+  b.rbuf = reinterpret_cast<double*>(take(DK_NRDATA*sizeof(double)*n));
+  b.ibuf = reinterpret_cast<int32_t*>(take(DK_NIDATA*sizeof(int32_t)*n));
+  b.tmp = p;
+  b.tmp_bytes = kSortNoAllocBytes - static_cast<size_t>(p - base);
+  size_t bytes = 0;
+  if ((e = SortTmpBytes(c, b, n, bytes))) return e;
+  if (bytes > b.tmp_bytes) return DK_ECAPACITY;
+  return SortInto(c, b, n, false);
 }
 
 // RemoveOutflowParticles (src/dust/dust_tasks.cpp:365-420): the keep flags (mass > 0), their

@@ -125,6 +125,7 @@ struct dk_ctx {
   double *sort_rbuf = nullptr;
   int32_t *sort_ibuf = nullptr;
   int64_t sort_cap = 0;
+  int noalloc_err = DK_OK;         // set up of the allocation-free sort's scratch (dk_create)
   // the FP32 form (dk_fp32.hip)
   struct Fast *fast = nullptr;
   int graphs = 0;

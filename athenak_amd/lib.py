@@ -25,7 +25,7 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_set_solver", "dk_solver_stats",
             "dk_set_tune_config", "dk_graph_launches", "dk_stiffness",
             "dk_stopping_times", "dk_remove_outflow_particles", "dk_x3_removed",
-            "dk_sort_state"] +
+            "dk_sort_state", "dk_sort_particles_noalloc"] +
            ["dk_" + t for t in TASKS])
 
 
@@ -69,6 +69,7 @@ def lib():
         L.dk_tune_config.argtypes = [P] + [C.POINTER(C.c_int32)] * 3
         L.dk_set_tune_config.argtypes = [P, C.c_int32, C.c_int32]
         L.dk_sort_particles.argtypes = [P]
+        L.dk_sort_particles_noalloc.argtypes = [P]
         L.dk_new_time_step.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
         L.dk_diag_sums.argtypes = [P, C.POINTER(C.c_double)]
         L.dk_error.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]

@@ -206,6 +206,16 @@ int dk_add_drag_history_gas(dk_ctx *ctx, const dk_stage *st);
  * src/dust/dust_cloud_device.cpp:1881-1980: stable order by (block, cell), all columns.  The
  * FP32 form keeps its particles sorted by (block, tile, cell) every stage: a no-op there. */
 int dk_sort_particles(dk_ctx *ctx);
+/* No counterpart in the reference.  dk_sort_particles without any device allocation, for
+ * callers that cannot allocate: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  The same stable (block, cell) order, bit for bit, with the sort's
+ * buffers in a static device array of the library, one per GPU (about 53 MiB, held until the
+ * process ends, its address taken at dk_create).  At most DK_SORT_NOALLOC_MAX particles
+ * (DK_ECAPACITY beyond, nothing done).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (two contexts, two streams) race.  FP32 form: a no-op, as
+ * dk_sort_particles. */
+#define DK_SORT_NOALLOC_MAX 262144
+int dk_sort_particles_noalloc(dk_ctx *ctx);
 /* DustGasDrag::RemoveOutflowParticles, src/dust/dust_tasks.cpp:365-420 (both forms): with an
  * outflow x3 face (DK_X3_OUTFLOW) and a non-zero ledger of particles parked at the faces since
  * the last call (bvals_part.cpp:184-186), every particle of zero mass is removed from the
