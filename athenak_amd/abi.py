@@ -76,7 +76,8 @@ class dk_setup(C.Structure):
                 ("nsh", C.c_int32), ("omega0", C.c_double), ("qshear", C.c_double),
                 ("gauss_z", C.c_int32), ("hp", C.c_double), ("gas_strat", C.c_int32),
                 ("nclump", C.c_int32), ("clump_frac", C.c_double),
-                ("clump_sigma", C.c_double), ("zmetal", C.c_double)]
+                ("clump_sigma", C.c_double), ("zmetal", C.c_double),
+                ("block_lognormal", C.c_int32), ("block_sigma", C.c_double)]
 
 
 def _left_edge(ith, n, xmin, xmax):
@@ -125,6 +126,95 @@ def uniform_mesh(nx, nb, box, ng=2, bc_x1=DK_X1_PERIODIC, bc_x3=DK_X3_PERIODIC,
                                 continue
                             gx, gy, gz = (bx + o1) % nbx, (by + o2) % nby, z % nbz
                             b.nghbr[d] = gx + nbx * (gy + nby * gz)
+    return mesh, blocks
+
+
+def _edges(l, n, a, b):
+    lo = a if l == 0 else _left_edge(l, n, a, b)
+    hi = b if l == n - 1 else _left_edge(l + 1, n, a, b)
+    return lo, hi
+
+
+def smr_mesh(nx, nb, box, regions, ng=2):
+    """A statically refined mesh, periodic in every direction: root blocks nb[0]*nb[1]*nb[2] of
+    nx[0]*nx[1]*nx[2] cells over box, and for each level L = 1, 2, ... the region regions[L-1]
+    = ((a1, b1), (a2, b2), (a3, b3)) in units of root blocks: every level-(L-1) leaf whose extent
+    overlaps it with positive volume is split into its 8 children (<mesh_refinement> static
+    regions of src/mesh/build_tree.cpp).  Leaves are listed depth first, children in x1-fastest
+    order (the reference's Z-order of the block tree); each block's box is LeftEdgeX at its level
+    (src/mesh/meshblock.cpp:40-120).  Neighbour levels may differ by at most one (checked).
+    nghbr[d] holds the leaf that contains the centre of the same-level region beside the block
+    in direction d when that leaf is of the block's level or coarser, else -2 (finer leaves
+    there).  Returns (dk_mesh, array of dk_block)."""
+    (x1a, x1b), (x2a, x2b), (x3a, x3b) = box
+    nbx, nby, nbz = nb
+    leaves = [(0, bx, by, bz) for bz in range(nbz) for by in range(nby) for bx in range(nbx)]
+    for L, reg in enumerate(regions, start=1):
+        out = []
+        for (lv, l1, l2, l3) in leaves:
+            s = 1.0 / (1 << lv)      # block length in root-block units
+            lo, hi = (l1 * s, l2 * s, l3 * s), ((l1 + 1) * s, (l2 + 1) * s, (l3 + 1) * s)
+            hit = lv == L - 1 and all(min(hi[d], reg[d][1]) > max(lo[d], reg[d][0])
+                                      for d in range(3))
+            if not hit:
+                out.append((lv, l1, l2, l3))
+                continue
+            for o3 in (0, 1):
+                for o2 in (0, 1):
+                    for o1 in (0, 1):
+                        out.append((L, 2 * l1 + o1, 2 * l2 + o2, 2 * l3 + o3))
+        leaves = out
+    maxlev = max(lv for lv, *_ in leaves)
+    F = (nbx << maxlev, nby << maxlev, nbz << maxlev)
+    slot = {}
+    for g, (lv, l1, l2, l3) in enumerate(leaves):
+        r = 1 << (maxlev - lv)
+        for s3 in range(l3 * r, (l3 + 1) * r):
+            for s2 in range(l2 * r, (l2 + 1) * r):
+                for s1 in range(l1 * r, (l1 + 1) * r):
+                    slot[(s1, s2, s3)] = g
+
+    def leaf_at(f):   # f: a position in finest-slot units (periodic)
+        return slot[tuple(int(f[d] // 1) % F[d] for d in range(3))]
+
+    mesh = dk_mesh(nmb=len(leaves), nx1=nx[0], nx2=nx[1], nx3=nx[2], ng=ng,
+                   x1min=x1a, x1max=x1b, x2min=x2a, x2max=x2b, x3min=x3a, x3max=x3b,
+                   nmbx1=nbx, nmbx2=nby, nmbx3=nbz, bc_x1=DK_X1_PERIODIC,
+                   bc_x3=DK_X3_PERIODIC, x3_dep_drop=0)
+    blocks = (dk_block * mesh.nmb)()
+    for g, (lv, l1, l2, l3) in enumerate(leaves):
+        b = blocks[g]
+        b.x1min, b.x1max = _edges(l1, nbx << lv, x1a, x1b)
+        b.x2min, b.x2max = _edges(l2, nby << lv, x2a, x2b)
+        b.x3min, b.x3max = _edges(l3, nbz << lv, x3a, x3b)
+        b.lx1, b.lx2, b.lx3, b.level = l1, l2, l3, lv
+        r = 1 << (maxlev - lv)
+        for o3 in (-1, 0, 1):
+            for o2 in (-1, 0, 1):
+                for o1 in (-1, 0, 1):
+                    d = (o1 + 1) + 3 * (o2 + 1) + 9 * (o3 + 1)
+                    c = ((l1 + o1 + 0.5) * r, (l2 + o2 + 0.5) * r, (l3 + o3 + 0.5) * r)
+                    n = leaf_at(c)
+                    b.nghbr[d] = n if leaves[n][0] <= lv else -2
+                    # 2:1 balance: the leaves just beyond this face, edge or corner (sampled
+                    # at the quarter points along the directions with o = 0) differ by one level
+                    # at most
+                    if (o1, o2, o3) == (0, 0, 0):
+                        continue
+                    for f3 in (0.25, 0.75):
+                        for f2 in (0.25, 0.75):
+                            for f1 in (0.25, 0.75):
+                                q = []
+                                for l_, o_, f_ in ((l1, o1, f1), (l2, o2, f2), (l3, o3, f3)):
+                                    if o_ < 0:
+                                        q.append(l_ * r - 0.5)
+                                    elif o_ > 0:
+                                        q.append((l_ + 1) * r + 0.5)
+                                    else:
+                                        q.append((l_ + f_) * r)
+                                if abs(leaves[leaf_at(q)][0] - lv) > 1:
+                                    raise ValueError("refinement regions break the 2:1 balance "
+                                                     "at block %d" % g)
     return mesh, blocks
 
 

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include "dk_internal.hpp"
+#include "dk_smr.hpp"
 
 using dk::DevBlock;
 using dk::Dir;
@@ -183,6 +184,123 @@ __device__ inline void GatherLinear(const Geo &g, const DevBlock &b, const Cloud
   u[2] += w*q[2] + cx*q[5] + cy*q[8] + cz*q[11];
 }
 
+// LeafTraversal3D::ForEach (src/dust/dust_leaf3d.hpp:383-517) on a statically refined mesh:
+// the actual leaves under the cloud of level lev at x (half-width root dx 2^-lev) at their own
+// resolution, f(b, k, j, i, w, m1x, m1y, m1z) (padded indices of leaf b) with moments,
+// f(b, k, j, i, w) without, per candidate leaf the reference's loop order (x runs of 4, then
+// x3, x2, x1) and skips.  The candidates are the leaves holding the support's 8 corners (a leaf
+// at least as long as the support in every direction meets it only if it holds a corner), in
+// the order of the corners that found them (x1 fastest, low side first); false when a corner
+// lies in no leaf or a candidate is shorter than the support.
+template <bool kMoments, class F>
+__device__ bool SmrCloudForEach(const Geo &g, const DevBlock *B, const dk::SmrTree &t,
+                                const CloudGeo &cg, const double x[3], const int lev,
+                                const F &f) {
+  double h[3], ih[3], lo[3], hi[3];
+#pragma unroll
+  for (int d=0; d<3; ++d) {
+    h[d] = ldexp(cg.h[d], -lev);
+    ih[d] = ldexp(cg.ih[d], lev);
+    lo[d] = x[d] - h[d];
+    hi[d] = x[d] + h[d];
+  }
+  int cb[8], cs[8][3], nc = 0;
+  for (int c=0; c<8; ++c) {
+    double p[3];
+    int k[3];
+#pragma unroll
+    for (int d=0; d<3; ++d) p[d] = ((c >> d) & 1) ? nextafter(hi[d], -INFINITY) : lo[d];
+    dk::SmrWrap(t, p, k);
+    const int b = dk::SmrLeafAt(B, t, p);
+    if (b < 0) return false;
+    bool dup = false;
+    for (int q=0; q<nc; ++q)
+      if (cb[q] == b && cs[q][0] == k[0] && cs[q][1] == k[1] && cs[q][2] == k[2]) dup = true;
+    if (dup) continue;
+    cb[nc] = b; cs[nc][0] = k[0]; cs[nc][1] = k[1]; cs[nc][2] = k[2]; ++nc;
+  }
+  for (int q=0; q<nc; ++q) {
+    const DevBlock &bb = B[cb[q]];
+    if (bb.x1max - bb.x1min < 2.0*h[0] || bb.x2max - bb.x2min < 2.0*h[1] ||
+        bb.x3max - bb.x3min < 2.0*h[2])
+      return false;
+  }
+  constexpr int kRun = 4;
+  for (int q=0; q<nc; ++q) {
+    const int b = cb[q];
+    const DevBlock &bb = B[b];
+    const double dx = bb.dx1, dy = bb.dx2, dz = bb.dx3;
+    const double bx0 = bb.x1min + cs[q][0]*cg.len[0], by0 = bb.x2min + cs[q][1]*cg.len[1];
+    const double bz0 = bb.x3min + cs[q][2]*cg.len[2];
+    const double idx = 1.0/dx, idy = 1.0/dy, idz = 1.0/dz;
+    const int i0 = max(static_cast<int>(floor((lo[0] - bx0)*idx)), 0);
+    const int i1 = min(static_cast<int>(ceil((hi[0] - bx0)*idx)) - 1, g.nx1 - 1);
+    const int j0 = max(static_cast<int>(floor((lo[1] - by0)*idy)), 0);
+    const int j1 = min(static_cast<int>(ceil((hi[1] - by0)*idy)) - 1, g.nx2 - 1);
+    const int k0 = max(static_cast<int>(floor((lo[2] - bz0)*idz)), 0);
+    const int k1 = min(static_cast<int>(ceil((hi[2] - bz0)*idz)) - 1, g.nx3 - 1);
+    for (int ic=i0; ic<=i1; ic+=kRun) {
+      const int nr = (i1 - ic + 1 < kRun) ? i1 - ic + 1 : kRun;
+      double w1[kRun], g1[kRun];
+      double fc = HatCDF((bx0 + ic*dx - x[0])*ih[0]);
+      double fh = kMoments ? HatMoment((bx0 + ic*dx - x[0])*ih[0]) : 0.0;
+#pragma unroll
+      for (int a=0; a<kRun; ++a) {
+        w1[a] = 0.0; g1[a] = 0.0;
+        if (a < nr) {
+          const double sz = (bx0 + (ic + a + 1)*dx - x[0])*ih[0];
+          const double nc1 = HatCDF(sz);
+          w1[a] = nc1 - fc;
+          fc = nc1;
+          if (kMoments) {
+            const double nh = HatMoment(sz);
+            g1[a] = h[0]*(nh - fh);
+            fh = nh;
+          }
+        }
+      }
+      double f3c = HatCDF((bz0 + k0*dz - x[2])*ih[2]);
+      double f3h = kMoments ? HatMoment((bz0 + k0*dz - x[2])*ih[2]) : 0.0;
+      for (int k=k0; k<=k1; ++k) {
+        const double sz3 = (bz0 + (k + 1)*dz - x[2])*ih[2];
+        const double n3c = HatCDF(sz3);
+        const double w3 = n3c - f3c;
+        double g3 = 0.0;
+        f3c = n3c;
+        if (kMoments) {const double n3h = HatMoment(sz3); g3 = h[2]*(n3h - f3h); f3h = n3h;}
+        if (w3 == 0.0 && (!kMoments || g3 == 0.0)) continue;
+        double f2c = HatCDF((by0 + j0*dy - x[1])*ih[1]);
+        double f2h = kMoments ? HatMoment((by0 + j0*dy - x[1])*ih[1]) : 0.0;
+        for (int j=j0; j<=j1; ++j) {
+          const double sz2 = (by0 + (j + 1)*dy - x[1])*ih[1];
+          const double n2c = HatCDF(sz2);
+          const double w2 = n2c - f2c;
+          double g2 = 0.0;
+          f2c = n2c;
+          if (kMoments) {const double n2h = HatMoment(sz2); g2 = h[1]*(n2h - f2h); f2h = n2h;}
+          if (w2 == 0.0 && (!kMoments || g2 == 0.0)) continue;
+          const double w23 = w2*w3;
+          for (int a=0; a<nr; ++a) {
+            const double wa = w1[a];
+            const double w = w23*wa;
+            if constexpr (kMoments) {
+              const double ga = g1[a];
+              const double m1x = w23*ga;
+              const double m1y = wa*g2*w3, m1z = wa*w2*g3;
+              if (w == 0.0 && m1x == 0.0 && m1y == 0.0 && m1z == 0.0) continue;
+              f(b, k + g.ks, j + g.js, ic + a + g.is, w, m1x, m1y, m1z);
+            } else {
+              if (w == 0.0) continue;
+              f(b, k + g.ks, j + g.js, ic + a + g.is, w);
+            }
+          }
+        }
+      }
+    }
+  }
+  return true;
+}
+
 // src/dust/dust_cloud3d.cpp:1342-1360 (PackGasGather3D, dust_pc2_predictor_gasvel3d)
 __global__ void k_gas_velocity(Geo g, const double *u, double *vel) {
   const long long n = g.ncb*g.nmb;
@@ -317,6 +435,85 @@ __global__ void __launch_bounds__(kBlock) k_cloud_advance(Geo g, const DevBlock 
   }
 }
 
+// CloudPredictorT on a statically refined mesh: the gather and the deposit over the leaves of
+// every level (the deposit walks the cloud again without moments: the same cells and weights in
+// the same order), each leaf cell with its own volume
+__global__ void __launch_bounds__(kBlock) k_smr_cloud_predictor(
+    Geo g, const DevBlock *B, dk::SmrTree t, CloudGeo cg, const double *r, const int32_t *iv,
+    long long cap, long long np, const double *gp, double *qdep, double dt, int rate_dep,
+    dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const double x[3] = {PR(DK_IPX,p), PR(DK_IPY,p), PR(DK_IPZ,p)};
+    const int lev = PI(DK_PLEV,p);
+    double ug[3] = {0.0, 0.0, 0.0};
+    const bool ok = SmrCloudForEach<true>(g, B, t, cg, x, lev,
+        [&](int b, int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+          GatherLinear(g, B[b], cg, gp, b, k, j, i, w, m1x, m1y, m1z, x, ug);
+        });
+    if (!ok) {dk::RecordError(err, DK_EHALO, p); continue;}
+    const double mass_rate = PR(DK_IPM,p)/PR(DK_IPTS,p);
+    const double gx = -dt*mass_rate*(ug[0] - PR(DK_IPVX,p));
+    const double gy = -dt*mass_rate*(ug[1] - PR(DK_IPVY,p));
+    const double gz = -dt*mass_rate*(ug[2] - PR(DK_IPVZ,p));
+    SmrCloudForEach<false>(g, B, t, cg, x, lev, [&](int b, int k, int j, int i, double w) {
+      const DevBlock &bb = B[b];
+      const double inv_volume = 1.0/(bb.dx1*bb.dx2*bb.dx3);
+      unsafeAtomicAdd(&qdep[Idx(g,b,5,1,k,j,i)], inv_volume*w*gx);
+      unsafeAtomicAdd(&qdep[Idx(g,b,5,2,k,j,i)], inv_volume*w*gy);
+      unsafeAtomicAdd(&qdep[Idx(g,b,5,3,k,j,i)], inv_volume*w*gz);
+      if (rate_dep) unsafeAtomicAdd(&qdep[Idx(g,b,5,4,k,j,i)], inv_volume*w*mass_rate);
+    });
+  }
+}
+
+// CloudAdvanceT on a statically refined mesh (periodic, no shearing box)
+__global__ void __launch_bounds__(kBlock) k_smr_cloud_advance(
+    Geo g, const DevBlock *B, dk::SmrTree t, CloudGeo cg, double *r, const int32_t *iv,
+    long long cap, long long np, const double *gp, double *dmom, double dt, int br,
+    dk::ErrWord *err) {
+  GRID_STRIDE(p, np) {
+    const double x0[3] = {PR(DK_IPX1,p), PR(DK_IPY1,p), PR(DK_IPZ1,p)};
+    const double v0[3] = {PR(DK_IPVX1,p), PR(DK_IPVY1,p), PR(DK_IPVZ1,p)};
+    double xh[3];
+    xh[0] = x0[0] + 0.5*dt*v0[0];
+    xh[1] = x0[1] + 0.5*dt*v0[1];
+    xh[2] = x0[2] + 0.5*dt*v0[2];
+    const int lev = PI(DK_PLEV,p);
+    double ug[3] = {0.0, 0.0, 0.0};
+    const bool ok = SmrCloudForEach<true>(g, B, t, cg, xh, lev,
+        [&](int b, int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+          GatherLinear(g, B[b], cg, gp, b, k, j, i, w, m1x, m1y, m1z, xh, ug);
+        });
+    if (!ok) {dk::RecordError(err, DK_EHALO, p); continue;}
+    const double h = 0.5*dt;
+    const double lambda = h/PR(DK_IPTS,p);
+    const double aa = 1.0 + lambda;
+    const double rr[3] = {v0[0] + lambda*ug[0], v0[1] + lambda*ug[1], v0[2] + lambda*ug[2]};
+    const double vh[3] = {rr[0]/aa, rr[1]/aa, rr[2]/aa};
+    PR(DK_IPVX,p) = 2.0*vh[0] - v0[0];
+    PR(DK_IPVY,p) = 2.0*vh[1] - v0[1];
+    PR(DK_IPVZ,p) = 2.0*vh[2] - v0[2];
+    PR(DK_IPX,p) = x0[0] + dt*vh[0];
+    PR(DK_IPY,p) = x0[1] + dt*vh[1];
+    PR(DK_IPZ,p) = x0[2] + dt*vh[2];
+    PR(DK_IPX1,p) = xh[0]; PR(DK_IPY1,p) = xh[1]; PR(DK_IPZ1,p) = xh[2];
+    const double ts = PR(DK_IPTS,p);
+    const double dvx = dt*(ug[0]-vh[0])/ts;
+    const double dvy = dt*(ug[1]-vh[1])/ts;
+    const double dvz = dt*(ug[2]-vh[2])/ts;
+    PR(DK_IPRX,p) = dvx; PR(DK_IPRY,p) = dvy; PR(DK_IPRZ,p) = dvz;
+    if (!br) continue;
+    const double mass = -PR(DK_IPM,p);
+    SmrCloudForEach<false>(g, B, t, cg, xh, lev, [&](int b, int k, int j, int i, double w) {
+      const DevBlock &bb = B[b];
+      const double dw = mass*w/(bb.dx1*bb.dx2*bb.dx3);
+      unsafeAtomicAdd(&dmom[Idx(g,b,3,0,k,j,i)], dw*dvx);
+      unsafeAtomicAdd(&dmom[Idx(g,b,3,1,k,j,i)], dw*dvy);
+      unsafeAtomicAdd(&dmom[Idx(g,b,3,2,k,j,i)], dw*dvz);
+    });
+  }
+}
+
 CloudGeo MakeCloudGeo(const dk_ctx *c) {
   CloudGeo cg;
   const dk_mesh &M = c->M;
@@ -349,6 +546,14 @@ int CloudPredictor(dk_ctx *c, double dt, int rate_dep) {
   k_pack_gather<<<Grid(nin), kBlock, 0, c->stream>>>(g, c->cvel, c->gpack);
   ProfEnd(c, t);}
   if (c->np == 0) return Check(hipGetLastError());
+  if (c->smr) {
+    const int t = ProfBegin(c, "cloud_predictor");
+    k_smr_cloud_predictor<<<Grid(c->np), kBlock, 0, c->stream>>>(
+        g, c->d_blocks, c->tree, MakeCloudGeo(c), c->rdata, c->idata, (long long)c->cap,
+        (long long)c->np, c->gpack, c->qdep, dt, rate_dep, c->d_err);
+    ProfEnd(c, t);
+    return Check(hipGetLastError());
+  }
   {const int t = ProfBegin(c, "cloud_predictor");
   k_cloud_predictor<<<Grid(c->np), kBlock, 0, c->stream>>>(
       g, c->d_blocks, MakeCloudGeo(c), c->rdata, c->idata, (long long)c->cap,
@@ -364,6 +569,14 @@ int CloudAdvance(dk_ctx *c, double dt) {
   k_pack_gather<<<Grid(nin), kBlock, 0, c->stream>>>(g, c->ustar, c->gpack);
   ProfEnd(c, t);}
   if (c->np == 0) return Check(hipGetLastError());
+  if (c->smr) {
+    const int t = ProfBegin(c, "cloud_advance");
+    k_smr_cloud_advance<<<Grid(c->np), kBlock, 0, c->stream>>>(
+        g, c->d_blocks, c->tree, MakeCloudGeo(c), c->rdata, c->idata, (long long)c->cap,
+        (long long)c->np, c->gpack, c->dmom, dt, c->P.back_reaction, c->d_err);
+    ProfEnd(c, t);
+    return Check(hipGetLastError());
+  }
   {const int t = ProfBegin(c, "cloud_advance");
   k_cloud_advance<<<Grid(c->np), kBlock, 0, c->stream>>>(
       g, c->d_blocks, MakeCloudGeo(c), c->rdata, c->idata, (long long)c->cap,

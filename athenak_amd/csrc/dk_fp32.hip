@@ -1217,17 +1217,33 @@ k_key(KArgs A, int do_key, double *pres) {
   }
 }
 
-// the key pass's outflow partials summed over its items in a fixed order (a strided sum per
-// thread, then a tree) into the ledger (mass, count)
-__global__ void __launch_bounds__(256) k_x3sum(const int *item_start, int nt, const double *x3p,
-                                               double *ledger) {
+// the key pass's outflow partials summed over its items in a fixed order into the ledger (mass,
+// count): kX3Groups workgroups each sum a contiguous run of items (a strided sum per thread,
+// then a tree), one workgroup then adds the runs' sums by a tree
+constexpr int kX3Groups = 128;
+__global__ void __launch_bounds__(256) k_x3part(const int *item_start, int nt, const double *x3p,
+                                                double *part) {
   __shared__ double sm[256], sc[256];
   const int n = item_start[nt];
+  const int per = (n + kX3Groups - 1)/kX3Groups;
+  const int i0 = blockIdx.x*per, i1 = min(n, i0 + per);
   double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {a += x3p[2*i]; b += x3p[2*i + 1];}
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {a += x3p[2*i]; b += x3p[2*i + 1];}
   sm[threadIdx.x] = a; sc[threadIdx.x] = b;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {sm[threadIdx.x] += sm[threadIdx.x + h]; sc[threadIdx.x] += sc[threadIdx.x + h];}
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {part[2*blockIdx.x] = sm[0]; part[2*blockIdx.x + 1] = sc[0];}
+}
+
+__global__ void __launch_bounds__(kX3Groups) k_x3sum(const double *part, double *ledger) {
+  __shared__ double sm[kX3Groups], sc[kX3Groups];
+  sm[threadIdx.x] = part[2*threadIdx.x];
+  sc[threadIdx.x] = part[2*threadIdx.x + 1];
+  __syncthreads();
+  for (int h = kX3Groups/2; h > 0; h >>= 1) {
     if (threadIdx.x < h) {sm[threadIdx.x] += sm[threadIdx.x + h]; sc[threadIdx.x] += sc[threadIdx.x + h];}
     __syncthreads();
   }
@@ -2951,7 +2967,9 @@ int AllocItems(dk_ctx *c, long long n) {
       Alloc((void**)&f->ov_dc, sizeof(double)*9*(size_t)T.TC*ov) ||
       Alloc((void**)&f->ov_cnt, sizeof(int)*(size_t)T.TC*ov))
     return DK_ENOMEM;
-  if (c->g.bc_x3 == DK_X3_OUTFLOW && Alloc((void**)&f->x3p, sizeof(double)*2*f->item_cap))
+  // (the items' partials, then the kX3Groups run sums of k_x3part)
+  if (c->g.bc_x3 == DK_X3_OUTFLOW &&
+      Alloc((void**)&f->x3p, sizeof(double)*2*((size_t)f->item_cap + kX3Groups)))
     return DK_ENOMEM;
   {
     if (f->ilist_n == nullptr && Alloc((void**)&f->ilist_n, sizeof(int)*2)) return DK_ENOMEM;
@@ -3384,8 +3402,10 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   // the push of this stage writes its residue into the buffer the next stage's start reads
   TIMED("key", LaunchKey(c, A, 1, f->pres[1 - f->pres_rd]));
   if (f->x3p != nullptr)   // the outflow ledger (bvals_part.cpp:184-186) of the items just run
-    TIMED("x3_ledger", (k_x3sum<<<1, 256, 0, c->stream>>>(f->item_start, T.nt, f->x3p,
-                                                          c->d_scalar + 12)));
+    TIMED("x3_ledger", (k_x3part<<<kX3Groups, 256, 0, c->stream>>>(
+                            f->item_start, T.nt, f->x3p, f->x3p + 2*(size_t)f->item_cap),
+                        k_x3sum<<<1, kX3Groups, 0, c->stream>>>(
+                            f->x3p + 2*(size_t)f->item_cap, c->d_scalar + 12)));
   TIMED("fix", (k_fix_key<<<Grid((long long)T.nt*T.TC), kBlock, 0, c->stream>>>(
       A, 1, f->pres[1 - f->pres_rd])));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history

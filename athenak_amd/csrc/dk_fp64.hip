@@ -15,6 +15,7 @@
 #include <utility>
 
 #include "dk_internal.hpp"
+#include "dk_smr.hpp"
 
 using dk::Geo;
 using dk::DevBlock;
@@ -242,7 +243,7 @@ __global__ void k_shift_shear_crossings(const DevBlock *B, double *r, const int3
 // bvals_part.cpp:164-429 (part_update) on a uniform mesh of one rank
 __global__ void k_part_update(Geo g, const DevBlock *B, double *r, int32_t *iv, long long cap,
                               long long np, int shear, double ysh, double *x3rem,
-                              dk::ErrWord *err) {
+                              dk::SmrTree tree, dk::ErrWord *err) {
   GRID_STRIDE(p, np) {
     const int m = PI(DK_PGID,p);
     const DevBlock &b = B[m];
@@ -331,7 +332,14 @@ __global__ void k_part_update(Geo g, const DevBlock *B, double *r, int32_t *iv, 
       PI(DK_PGID,p) = b1 + g.nmbx1*(b2 + g.nmbx2*b3);
       continue;
     }
-    const int nb = b.nghbr[Dir(ix, iy, iz)];
+    int nb;
+    if (tree.leaf != nullptr) {   // a refined mesh: the neighbour of any level
+      const int off[3] = {ix, iy, iz};
+      const double xs[3] = {x1, x2, x3};
+      nb = dk::SmrDest(B, tree, m, off, xs);
+    } else {
+      nb = b.nghbr[Dir(ix, iy, iz)];
+    }
     if (nb < 0) {dk::RecordError(err, DK_EROUTE, p); continue;}
     PI(DK_PGID,p) = nb;
     const double lxm = g.x1max - g.x1min, lym = g.x2max - g.x2min, lzm = g.x3max - g.x3min;
@@ -1277,9 +1285,16 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
   if (mesh == nullptr || blocks == nullptr || phys == nullptr || out == nullptr)
     return DK_EINVAL;
   if (mesh->nmb < 1 || mesh->nx1 < 2 || mesh->nx2 < 2 || mesh->nx3 < 2 || mesh->ng < 1 ||
-      mesh->nmb != mesh->nmbx1*mesh->nmbx2*mesh->nmbx3)
+      mesh->nmbx1 < 1 || mesh->nmbx2 < 1 || mesh->nmbx3 < 1)
     return DK_EINVAL;
   if (form != DK_FORM_FP64 && form != DK_FORM_FP32) return DK_EINVAL;
+  // a statically refined mesh: refinable clouds with pc2 in the plain FP64 form (their
+  // traversal of the actual leaves; point-particle deposits would need the cross-level fold)
+  int maxlev = 0;
+  for (int m=0; m<mesh->nmb; ++m) maxlev = std::max(maxlev, blocks[m].level);
+  const bool smr = maxlev > 0 || mesh->nmb != mesh->nmbx1*mesh->nmbx2*mesh->nmbx3;
+  if (smr && (form != DK_FORM_FP64 || !phys->clouds)) return DK_EUNSUPPORTED;
+  if (maxlev > 20) return DK_EINVAL;
   // the mesh passes launch one grid row per block (at most 65535), and index a block's padded
   // cells in 32 bits
   if (mesh->nmb > 65535 ||
@@ -1326,6 +1341,7 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     d.sx3 = static_cast<double>(g.nx3)/(b.x3max - b.x3min);
     d.ivol = 1.0/(d.dx1*d.dx2*d.dx3);
     d.lx1 = b.lx1; d.lx2 = b.lx2; d.lx3 = b.lx3;
+    d.level = b.level;
     {
       const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
       d.face = ((b.x1min < mesh->x1min + 0.5*lx) ? 1 : 0) | ((b.x1max > mesh->x1max - 0.5*lx) ? 2 : 0) |
@@ -1391,6 +1407,41 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     for (double *p : {c->solver_r, c->solver_p})
       if (!e && p) e = dk::Check(hipMemset(p, 0, sizeof(double)*3*nc));
   }
+  if (!e && smr) {
+    // the finest block grid: every slot covered by exactly one leaf
+    const int F[3] = {mesh->nmbx1 << maxlev, mesh->nmbx2 << maxlev, mesh->nmbx3 << maxlev};
+    std::vector<int> leaf((size_t)F[0]*F[1]*F[2], -1);
+    for (int m=0; m<mesh->nmb && !e; ++m) {
+      const dk_block &b = blocks[m];
+      if (b.level < 0) {e = DK_EINVAL; break;}
+      const int r = 1 << (maxlev - b.level);
+      const int l[3] = {b.lx1, b.lx2, b.lx3};
+      for (int d=0; d<3; ++d)
+        if (l[d] < 0 || (long long)(l[d] + 1)*r > F[d]) e = DK_EINVAL;
+      for (int k=b.lx3*r; k<(b.lx3 + 1)*r && !e; ++k)
+        for (int j=b.lx2*r; j<(b.lx2 + 1)*r && !e; ++j)
+          for (int i=b.lx1*r; i<(b.lx1 + 1)*r; ++i) {
+            int &slot = leaf[((size_t)k*F[1] + j)*F[0] + i];
+            if (slot >= 0) {e = DK_EINVAL; break;}
+            slot = m;
+          }
+    }
+    for (size_t q=0; q<leaf.size() && !e; ++q)
+      if (leaf[q] < 0) e = DK_EINVAL;
+    alloc((void**)&c->d_leafmap, sizeof(int)*leaf.size());
+    if (!e) e = dk::Check(hipMemcpy(c->d_leafmap, leaf.data(), sizeof(int)*leaf.size(),
+                                    hipMemcpyHostToDevice));
+    if (!e) {
+      c->smr = 1;
+      dk::SmrTree &t = c->tree;
+      for (int d=0; d<3; ++d) t.F[d] = F[d];
+      t.maxlev = maxlev;
+      t.leaf = c->d_leafmap;
+      t.lo[0] = mesh->x1min; t.lo[1] = mesh->x2min; t.lo[2] = mesh->x3min;
+      t.len[0] = mesh->x1max - mesh->x1min; t.len[1] = mesh->x2max - mesh->x2min;
+      t.len[2] = mesh->x3max - mesh->x3min;
+    }
+  }
   if (!e && form == DK_FORM_FP32) e = dk::FastCreate(c);
   if (e) {dk_destroy(c); return e;}
   *out = c;
@@ -1409,7 +1460,7 @@ int dk_destroy(dk_ctx *c) {
                   (void*)c->dmom, (void*)c->ap, (void*)c->resid, c->sort_tmp,
                   (void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
                   (void*)c->sort_ibuf, (void*)c->gpack, (void*)c->cvel, (void*)c->solver_r,
-                  (void*)c->solver_p})
+                  (void*)c->solver_p, (void*)c->d_leafmap, (void*)c->d_block_off})
     if (p) (void)hipFree(p);
   delete c;
   return DK_OK;
@@ -1505,7 +1556,7 @@ int dk_update_particle_gids(dk_ctx *c, const dk_stage *st) {
   }
   {const int _pt = dk::ProfBegin(c, "part_update");
   k_part_update<<<PGRID>>>(c->g, c->d_blocks, PARGS, shear ? 1 : 0, ysh, c->d_scalar + 12,
-                           c->d_err);
+                           c->tree, c->d_err);
    dk::ProfEnd(c, _pt);}
   return Launched(c);
 }
@@ -1539,7 +1590,8 @@ int dk_deposit_drag(dk_ctx *c, const dk_stage *st) {
 int dk_fold_deposit_qp(dk_ctx *c, const dk_stage *st) {
   int e = Ready(c, false);
   const bool use = (c && c->P.coupling == DK_IMEX) || st->stage == 1;
-  if (e || !ActiveStage(c, st) || !c->P.back_reaction || !use) return e;
+  // (a refined mesh: the clouds deposited into the leaves themselves, no ghost holds any)
+  if (e || !ActiveStage(c, st) || !c->P.back_reaction || !use || c->smr) return e;
   return Fold(c, c->qdep, 5, st->time);
 }
 
@@ -1603,7 +1655,7 @@ int dk_gather_kick_pmbr(dk_ctx *c, const dk_stage *st) {
 int dk_fold_pmbr(dk_ctx *c, const dk_stage *st) {
   int e = Ready(c, false);
   const bool use = (c && c->P.coupling == DK_IMEX) || st->stage == 2;
-  if (e || !ActiveStage(c, st) || !c->P.back_reaction || !use) return e;
+  if (e || !ActiveStage(c, st) || !c->P.back_reaction || !use || c->smr) return e;
   return Fold(c, c->dmom, 3, st->time);
 }
 

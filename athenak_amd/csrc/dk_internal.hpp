@@ -32,9 +32,20 @@ struct DevBlock {
   double sx1, sx2, sx3;    // nx/(xmax - xmin) of each axis, as PMWeights forms it
   double ivol;             // 1/(dx1 dx2 dx3)
   int lx1, lx2, lx3;
+  int level;               // refinement level (0 on a uniform mesh)
   int face;                // bits 1,2 / 4,8 / 16,32: lower/upper face within half a block of
                            // the mesh face in x1 / x2 / x3 (the test of bvals_part.cpp)
   int nghbr[27];
+};
+
+// A statically refined mesh by value for kernels: F[d] finest-level block slots per direction,
+// leaf[(s3*F[1] + s2)*F[0] + s1] the leaf covering slot (s1, s2, s3); the mesh's lower corner
+// and lengths.  leaf is null on a uniform mesh.
+struct SmrTree {
+  int F[3];
+  int maxlev;
+  const int *leaf;
+  double lo[3], len[3];
 };
 
 __host__ __device__ inline long long Idx(const Geo &g, int m, int nvar, int v, int k, int j,
@@ -126,6 +137,11 @@ struct dk_ctx {
   int32_t *sort_ibuf = nullptr;
   int64_t sort_cap = 0;
   int noalloc_err = DK_OK;         // set up of the allocation-free sort's scratch (dk_create)
+  // a statically refined mesh (levels above 0): the finest block grid on the device
+  int smr = 0;
+  dk::SmrTree tree{};
+  int *d_leafmap = nullptr;
+  long long *d_block_off = nullptr;   // uneven set-ups: particle offsets of the blocks
   // the FP32 form (dk_fp32.hip)
   struct Fast *fast = nullptr;
   int graphs = 0;

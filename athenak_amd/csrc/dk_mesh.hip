@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "dk_internal.hpp"
+#include "dk_smr.hpp"
 
 using dk::DevBlock;
 using dk::Dir;
@@ -90,6 +91,47 @@ __global__ void k_copy_ghosts(Geo g, const DevBlock *B, T *a, int nvar) {
     if (nb < 0) continue;
     for (int v=0; v<nvar; ++v)
       a[Idx(g,m,nvar,v,k,j,i)] = a[Idx(g,nb,nvar,v,k-ox3*g.nx3,j-ox2*g.nx2,i-ox1*g.nx1)];
+  }
+}
+
+// The harness's gas ghost values on a statically refined mesh (a stand-in for the hydro
+// boundary exchange of src/bvals/bvals_cc.cpp, which is not part of the drag path): every ghost
+// cell takes the cell of the leaf holding its centre when that leaf is of the block's level (a
+// copy) or coarser (the coarse value, injected), or the mean of the 8 cells of a finer leaf
+// that make it up (the restriction of src/bvals/prolongation.cpp:104-111, same order).  Reads
+// active cells only, writes ghost cells only.
+template <typename T>
+__global__ void k_smr_ghosts(Geo g, const DevBlock *B, dk::SmrTree tree, T *a, int nvar,
+                             dk::ErrWord *err) {
+  const unsigned n = GhostCount(g);
+  const int m = blockIdx.y;
+  const DevBlock &b = B[m];
+  BLOCK_STRIDE(s, n) {
+    int i, j, k;
+    GhostCell(g, s, i, j, k);
+    double c[3] = {b.x1min + (i - g.is + 0.5)*b.dx1, b.x2min + (j - g.js + 0.5)*b.dx2,
+                   b.x3min + (k - g.ks + 0.5)*b.dx3};
+    int sh[3];
+    dk::SmrWrap(tree, c, sh);
+    const int nb = dk::SmrLeafAt(B, tree, c);
+    if (nb < 0) {dk::RecordError(err, DK_EROUTE, -1); continue;}
+    const DevBlock &q = B[nb];
+    if (q.level <= b.level) {
+      const int in = static_cast<int>(floor((c[0] - q.x1min)/q.dx1)) + g.is;
+      const int jn = static_cast<int>(floor((c[1] - q.x2min)/q.dx2)) + g.js;
+      const int kn = static_cast<int>(floor((c[2] - q.x3min)/q.dx3)) + g.ks;
+      for (int v=0; v<nvar; ++v) a[Idx(g,m,nvar,v,k,j,i)] = a[Idx(g,nb,nvar,v,kn,jn,in)];
+    } else {
+      const int fi = static_cast<int>(floor((c[0] - 0.5*b.dx1 - q.x1min)/q.dx1 + 0.5)) + g.is;
+      const int fj = static_cast<int>(floor((c[1] - 0.5*b.dx2 - q.x2min)/q.dx2 + 0.5)) + g.js;
+      const int fk = static_cast<int>(floor((c[2] - 0.5*b.dx3 - q.x3min)/q.dx3 + 0.5)) + g.ks;
+      for (int v=0; v<nvar; ++v)
+        a[Idx(g,m,nvar,v,k,j,i)] = 0.125*(
+            a[Idx(g,nb,nvar,v,fk  ,fj  ,fi)] + a[Idx(g,nb,nvar,v,fk  ,fj  ,fi+1)]
+          + a[Idx(g,nb,nvar,v,fk  ,fj+1,fi)] + a[Idx(g,nb,nvar,v,fk  ,fj+1,fi+1)]
+          + a[Idx(g,nb,nvar,v,fk+1,fj  ,fi)] + a[Idx(g,nb,nvar,v,fk+1,fj  ,fi+1)]
+          + a[Idx(g,nb,nvar,v,fk+1,fj+1,fi)] + a[Idx(g,nb,nvar,v,fk+1,fj+1,fi+1)]);
+    }
   }
 }
 
@@ -395,6 +437,13 @@ namespace dk {
 template <typename T>
 int FillGhostsT(dk_ctx *c, T *a, int nvar, bool velocity, double t) {
   const Geo &g = c->g;
+  if (c->smr) {   // (refined meshes are periodic in every direction)
+    const int tk = ProfBegin(c, "smr_ghosts");
+    k_smr_ghosts<T><<<BlockGrid(GhostCount(g), g.nmb), kBlock, 0, c->stream>>>(
+        g, c->d_blocks, c->tree, a, nvar, c->d_err);
+    ProfEnd(c, tk);
+    return Check(hipGetLastError());
+  }
   {const int tk = ProfBegin(c, "copy_ghosts");
    k_copy_ghosts<T><<<BlockGrid(GhostCount(g), g.nmb), kBlock, 0, c->stream>>>(
        g, c->d_blocks, a, nvar);

@@ -21,7 +21,7 @@ using dk::kBlock;
 namespace {
 
 // src/pgen/dust_amr/dust_setup.hpp:35-50
-__device__ inline double HashUniform01(unsigned long long key) {
+__host__ __device__ inline double HashUniform01(unsigned long long key) {
   key += 0x9e3779b97f4a7c15ULL;
   key = (key ^ (key >> 30)) * 0xbf58476d1ce4e5b9ULL;
   key = (key ^ (key >> 27)) * 0x94d049bb133111ebULL;
@@ -35,12 +35,30 @@ __device__ inline double Gauss(unsigned long long k) {
   return sqrt(-2.0*log(u1))*cos(2.0*M_PI*u2);
 }
 
+// the host's Gaussian draw of a block's count (the oracle draws it with the same host library)
+double HostGauss(unsigned long long k) {
+  const double u1 = std::fmax(HashUniform01(k), 1.0e-300);
+  const double u2 = HashUniform01(k + 0x1d8e4e27c47d124fULL);
+  return std::sqrt(-2.0*std::log(u1))*std::cos(2.0*M_PI*u2);
+}
+
 struct SetupArgs {
   dk_setup su;
   double wmean[DK_MAXSPEC];
   double gas_mass;
   long long npb, ntot;
+  const long long *boff;   // uneven blocks: the first particle of every block (nmb + 1)
 };
+
+// the block of particle q from the blocks' offsets (the last block whose offset is <= q)
+__device__ inline int BlockOf(const long long *off, int nmb, long long q) {
+  int a = 0, b = nmb;   // off[a] <= q < off[b]
+  while (b - a > 1) {
+    const int mid = (a + b)/2;
+    if (off[mid] <= q) a = mid; else b = mid;
+  }
+  return a;
+}
 
 __global__ void k_setup_particles(Geo g, const DevBlock *B, double *r, int32_t *iv,
                                   long long cap, SetupArgs a) {
@@ -52,7 +70,7 @@ __global__ void k_setup_particles(Geo g, const DevBlock *B, double *r, int32_t *
   for (long long q = blockIdx.x*(long long)blockDim.x + threadIdx.x; q < a.ntot;
        q += (long long)gridDim.x*blockDim.x) {
     const unsigned long long key = static_cast<unsigned long long>(q) + base;
-    int m = static_cast<int>(q/a.npb);
+    int m = a.boff ? BlockOf(a.boff, g.nmb, q) : static_cast<int>(q/a.npb);
     const int sp = static_cast<int>(q % su.nspecies);
     double x, y, z;
     if (su.nclump > 0) {
@@ -139,7 +157,8 @@ __global__ void k_setup_particles(Geo g, const DevBlock *B, double *r, int32_t *
     r[DK_IPRX*cap + q] = 0.0; r[DK_IPRY*cap + q] = 0.0; r[DK_IPRZ*cap + q] = 0.0;
     r[DK_IPTS*cap + q] = ts; r[DK_IPM*cap + q] = mass;
     iv[DK_PGID*cap + q] = m; iv[DK_PTAG*cap + q] = static_cast<int32_t>(q);
-    iv[DK_PSP*cap + q] = sp; iv[DK_PLEV*cap + q] = 0; iv[DK_PTAGH*cap + q] = 0;
+    // (a refinable cloud of its block's level: half-width one cell of that level)
+    iv[DK_PSP*cap + q] = sp; iv[DK_PLEV*cap + q] = b.level; iv[DK_PTAGH*cap + q] = 0;
   }
 }
 
@@ -147,7 +166,6 @@ __global__ void k_setup_gas(Geo g, const DevBlock *B, double *u0, dk_setup su) {
   const unsigned long long base = su.seed*0x9e3779b97f4a7c15ULL;
   const unsigned long long saltg[3] = {DK_SALT_G0, DK_SALT_G1, DK_SALT_G2};
   const long long n = (long long)g.nmb*g.ncb;
-  const int ny = g.nx2*g.nmbx2, nxg = g.nx1*g.nmbx1;
   for (long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x*blockDim.x) {
     const int m = static_cast<int>(t/g.ncb);
@@ -162,9 +180,13 @@ __global__ void k_setup_gas(Geo g, const DevBlock *B, double *u0, dk_setup su) {
       continue;
     }
     const DevBlock &b = B[m];
+    // the global cell index of the block's level, plus level * 2^40
+    const long long ny = (long long)g.nx2*(g.nmbx2 << b.level);
+    const long long nxg = (long long)g.nx1*(g.nmbx1 << b.level);
     const long long gi = b.lx1*g.nx1 + (i - g.is), gj = b.lx2*g.nx2 + (j - g.js);
     const long long gk = b.lx3*g.nx3 + (k - g.ks);
-    const unsigned long long ck = static_cast<unsigned long long>((gk*ny + gj)*nxg + gi) + base;
+    const unsigned long long ck = static_cast<unsigned long long>((gk*ny + gj)*nxg + gi) + base +
+                                  (static_cast<unsigned long long>(b.level) << 40);
     const double z = b.x3min + (k - g.ks + 0.5)*b.dx3;
     const double rho = su.gas_strat ? su.rho0*exp(-0.5*z*z) : su.rho0;
     const double um[3] = {su.gas_vx, su.gas_vy, su.gas_vz};
@@ -185,11 +207,36 @@ int dk_setup_problem(dk_ctx *c, const dk_setup *su) {
   if (c->rdata == nullptr || c->u0 == nullptr) return DK_ESTATE;
   if (su->nspecies < 1 || su->nspecies > DK_MAXSPEC || su->ppc < 1) return DK_EINVAL;
   const Geo &g = c->g;
+  if ((su->block_lognormal || c->smr) && (su->nclump > 0 || su->gauss_z)) return DK_EINVAL;
   SetupArgs a;
   a.su = *su;
   a.npb = (long long)su->ppc*g.nx1*g.nx2*g.nx3;
   a.ntot = a.npb*g.nmb;
+  a.boff = nullptr;
+  std::vector<long long> off;
+  if (su->block_lognormal) {
+    // every block's count, drawn on the host: round(ppc ncell exp(sigma z - sigma^2/2))
+    off.assign(g.nmb + 1, 0);
+    const unsigned long long base = su->seed*0x9e3779b97f4a7c15ULL;
+    const double sg = su->block_sigma;
+    for (int m=0; m<g.nmb; ++m) {
+      const double z = HostGauss(base + static_cast<unsigned long long>(m)*0x2545f4914f6cdd1dULL +
+                                 DK_SALT_BLOCK);
+      off[m + 1] = off[m] + std::llround(static_cast<double>(a.npb)*std::exp(sg*z - 0.5*sg*sg));
+    }
+    a.ntot = off[g.nmb];
+  }
   if (a.ntot > c->cap) return DK_ECAPACITY;
+  if (su->block_lognormal) {
+    if (c->d_block_off) (void)hipFree(c->d_block_off);
+    c->d_block_off = nullptr;
+    if (hipMalloc((void**)&c->d_block_off, sizeof(long long)*(g.nmb + 1)) != hipSuccess)
+      return DK_ENOMEM;
+    const int e0 = dk::Check(hipMemcpy(c->d_block_off, off.data(), sizeof(long long)*(g.nmb + 1),
+                                       hipMemcpyHostToDevice));
+    if (e0) return e0;
+    a.boff = c->d_block_off;
+  }
   // total gas mass in the oracle's order (blocks, k; every row alike)
   a.gas_mass = 0.0;
   for (int m=0; m<g.nmb; ++m) {

@@ -7,9 +7,12 @@ device memory.
       NSH equilibrium + 1e-4 noise, imex2+/dc1, dt 2e-4, 50 steps
   c3  stratified (0.2 H)^3 at 256^3, 7 log-uniform Stokes bins over [1e-3, 1],
       mass_slope 0.5, Z 0.01, 1 per bin per cell, H_p 0.02, x3 outflow, imex2+/dc1
-  c4  refinable clouds with pc2 at the root level of the SMR case: 64^3 periodic in blocks
-      of 16^3, tau_s 0.05, eps 1, ppc 4, a quarter of the particles in 16 Gaussian clumps of
-      width 2 cells (the nested refined boxes of the SMR case are not built: same-level clouds)
+  c4  batching over uneven blocks: a statically refined mesh (64^3 root in 16^3 blocks, levels
+      1 and 2 in nested regions, 792 blocks), lognormal particle counts per block (sigma 1.5,
+      mean 4 per cell), refinable clouds of their block's level, tau_s 0.05, eps 1, pc2
+  c4u the root level of c4 alone (64^3 in 16^3 blocks, same-level clouds) with a quarter of the
+      particles in 16 Gaussian clumps of width 2 cells; c4p: 128^3 in 512 blocks with point
+      particles
   c5  128^3 periodic in 8 blocks of 64^3, ppc 8, two species (tau_s 0.01 and 1, eps 0.5
       each), half the particles in 64 Gaussian clumps of width 2 cells, imex2+/dc1, 100 steps
 """
@@ -38,11 +41,43 @@ class Workload:
 
     @property
     def npart(self):
+        if self.setup.block_lognormal:
+            return int(sum(block_counts(self.mesh, self.setup)))
         return self.setup.ppc * self.ncells
 
     @property
     def ppc(self):
         return self.setup.ppc
+
+
+_M64 = (1 << 64) - 1
+
+
+def _hash_uniform01(key):
+    """HashUniform01 (src/pgen/dust_amr/dust_setup.hpp:35-50): the SplitMix64 finaliser."""
+    key = (key + 0x9e3779b97f4a7c15) & _M64
+    key = ((key ^ (key >> 30)) * 0xbf58476d1ce4e5b9) & _M64
+    key = ((key ^ (key >> 27)) * 0x94d049bb133111eb) & _M64
+    key ^= key >> 31
+    return (key >> 11) * (1.0 / 9007199254740992.0)
+
+
+def block_counts(mesh, su):
+    """The particle count of every block of an uneven set-up (dk_setup.block_lognormal), as the
+    library and the oracle draw it on the host."""
+    ncell = mesh.nx1 * mesh.nx2 * mesh.nx3
+    if not su.block_lognormal:
+        return [su.ppc * ncell] * mesh.nmb
+    base = (su.seed * 0x9e3779b97f4a7c15) & _M64
+    out = []
+    for m in range(mesh.nmb):
+        k = (base + m * 0x2545f4914f6cdd1d + 0x7137449123ef65cd) & _M64
+        u1 = max(_hash_uniform01(k), 1e-300)
+        u2 = _hash_uniform01((k + 0x1d8e4e27c47d124f) & _M64)
+        z = math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+        sg = su.block_sigma
+        out.append(int(math.floor(su.ppc * ncell * math.exp(sg * z - 0.5 * sg * sg) + 0.5)))
+    return out
 
 
 def _setup(seed=1, ppc=1, taus=(0.1,), eps=(1.0,), rho0=1.0, gas_noise=0.0, dust_noise=0.0,
@@ -135,16 +170,16 @@ def c5(n=128, nbx=2, ppc=8, steps=100, seed=1):
     return Workload("c5", mesh, blocks, _phys(), su, None, 1.0e-3, steps)
 
 
-def c4(n=64, nbx=4, ppc=4, steps=50, seed=1, clump_frac=0.25, clouds=True):
-    """pc2 with same-level refinable clouds (CloudPredictorT / CloudAdvanceT) on the root level
-    of the SMR case; clumps make the per-block particle counts uneven.  clouds=False: the same
-    case with point particles (dust_pc2_predictor / dust_pc2_advance)."""
+def c4u(n=64, nbx=4, ppc=4, steps=50, seed=1, clump_frac=0.25, clouds=True):
+    """pc2 with refinable clouds (CloudPredictorT / CloudAdvanceT) on a uniform mesh (the root
+    level of the SMR case alone); clumps make the per-block particle counts uneven.
+    clouds=False: the same case with point particles (dust_pc2_predictor / dust_pc2_advance)."""
     nx = n // nbx
     mesh, blocks = abi.uniform_mesh((nx, nx, nx), (nbx, nbx, nbx), ((0.0, 1.0),) * 3, ng=2)
     su = _setup(seed=seed, ppc=ppc, taus=(0.05,), eps=(1.0,), gas_noise=1e-3, dust_noise=1e-3,
                 nclump=16 if clump_frac > 0 else 0, clump_frac=clump_frac,
                 clump_sigma=2.0 / n)
-    return Workload("c4" if clouds else "c4p", mesh, blocks,
+    return Workload("c4u" if clouds else "c4p", mesh, blocks,
                     _phys(coupling=abi.DK_PC2, solver=abi.DK_LOCAL, clouds=int(clouds)), su,
                     None, 1.0e-3, steps)
 
@@ -155,9 +190,34 @@ def c4p(n=128, nbx=8, ppc=4, steps=50, seed=1, clump_frac=0.25):
     clump centres reach dust-to-gas ratios of ~260 (chi = dt eps/t_s ~ 5), where the explicit pc2
     predictor is unstable: the rate field and the reference's damping of stiff cells are on
     (StiffnessActive, dust_pm.cpp:288-402)."""
-    w = c4(n=n, nbx=nbx, ppc=ppc, steps=steps, seed=seed, clump_frac=clump_frac, clouds=False)
+    w = c4u(n=n, nbx=nbx, ppc=ppc, steps=steps, seed=seed, clump_frac=clump_frac, clouds=False)
     w.phys.stiffness, w.phys.pc2_damping = 1, 1
     return w
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c4p": c4p, "c5": c5}
+# the refined regions of the SMR case in root-block units (root 4^3 blocks): level 1 over the
+# middle half in x3 (all of x1, x2: periodic), level 2 inside it, one level-1 block from every
+# level-1 face (the 2:1 balance)
+C4_REGIONS = (((0.0, 4.0), (0.0, 4.0), (1.0, 3.0)), ((0.5, 3.5), (0.5, 3.5), (1.5, 2.5)))
+
+
+def c4(n=64, nbx=4, ppc=4, steps=50, seed=1, sigma=1.5, regions=C4_REGIONS):
+    """Batching over uneven blocks: a statically refined mesh, root n^3 in nbx^3 blocks of
+    (n/nbx)^3 with levels 1 and 2 over C4_REGIONS (792 blocks at the defaults), periodic.
+    Block m holds round(ppc (n/nbx)^3 exp(sigma z_m - sigma^2/2)) particles, z_m ~ N(0, 1) keyed
+    by the block (lognormal, mean ppc per cell), uniform in the block, each a refinable cloud of
+    its block's level (half-width one cell of that level); eps 1 (the mass of a cloud of level
+    L is eps rho V_L / ppc), tau_s 0.05, pc2 with the rate field and the damping of stiff cells
+    (the densest blocks reach eps ~ 30)."""
+    nx = n // nbx
+    mesh, blocks = abi.smr_mesh((nx, nx, nx), (nbx, nbx, nbx), ((0.0, 1.0),) * 3, regions)
+    su = _setup(seed=seed, ppc=ppc, taus=(0.05,), eps=(1.0,), gas_noise=1e-3, dust_noise=1e-3,
+                block_lognormal=1, block_sigma=sigma)
+    w = Workload("c4", mesh, blocks,
+                 _phys(coupling=abi.DK_PC2, solver=abi.DK_LOCAL, clouds=1), su, None,
+                 1.0e-3, steps)
+    w.phys.stiffness, w.phys.pc2_damping = 1, 1
+    return w
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c4u": c4u, "c4p": c4p, "c5": c5}

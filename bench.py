@@ -33,7 +33,11 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of the minimal-pass schedule per particle-step (ppc = particles per cell):
 # imex2+/dc1 point TSC, FP64: 648 + 1176/ppc; FP32 form (positions and u0 in FP64): 456 + 732/ppc;
 # pc2 point TSC (two particle passes per cycle), FP64: 280 + 384/ppc; FP32 form: 204 + 292/ppc
-def balg_step(form, ppc, pc2=False):
+# pc2 with refinable clouds (the packed 12-component gather field twice per cycle): FP64
+# 280 + 760/ppc, FP32 form 204 + 480/ppc
+def balg_step(form, ppc, pc2=False, clouds=False):
+    if pc2 and clouds:
+        return (280.0 + 760.0 / ppc) if form == "fp64" else (204.0 + 480.0 / ppc)
     if pc2:
         return (280.0 + 384.0 / ppc) if form == "fp64" else (204.0 + 292.0 / ppc)
     if form == "fp64":
@@ -74,7 +78,11 @@ KERNEL_BYTES_PC2 = {
     "fp32": {"key": (36 + 60, 0), "pc2_predictor": (44, 3 * 4 + 4 * 4),
              "pc2_advance": (44 + 60, 3 * 4 + 3 * 8)},
     "fp64": {"pc2_predictor": (8 * 8 + 4, 4 * 8 + 4 * 8),
-             "pc2_advance": (8 * 8 + 4 + 12 * 8, 3 * 8 + 3 * 8)},
+             "pc2_advance": (8 * 8 + 4 + 12 * 8, 3 * 8 + 3 * 8),
+             # refinable clouds: the same particle columns plus the level; per cell the packed
+             # gather field (12 values) read and the deposit written
+             "cloud_predictor": (8 * 8 + 4, 12 * 8 + 4 * 8),
+             "cloud_advance": (8 * 8 + 4 + 12 * 8, 12 * 8 + 3 * 8)},
 }
 
 
@@ -120,10 +128,19 @@ def describe(w):
     pc2 = w.phys.coupling == 1
     solver = {0: "local", 1: "applya", 2: "dc1", 3: "dc2", 4: "pcg", 5: "adaptive"}[w.phys.solver]
     scheme = "rk2/pc2" if pc2 else "imex2+/" + solver
-    return {"workload": "%s: %dx%dx%d cells in %d blocks of %dx%dx%d, ppc %d, %d particles, "
-                        "%s, dt %g" % (
-                            w.name, m.nx1 * m.nmbx1, m.nx2 * m.nmbx2, m.nx3 * m.nmbx3, m.nmb,
-                            m.nx1, m.nx2, m.nx3, w.ppc, w.npart, scheme, w.dt),
+    if w.phys.clouds:
+        scheme += " refinable clouds"
+    levels = sorted({w.blocks[b].level for b in range(m.nmb)})
+    mesh = "%dx%dx%d cells in %d blocks of %dx%dx%d" % (
+        m.nx1 * m.nmbx1, m.nx2 * m.nmbx2, m.nx3 * m.nmbx3, m.nmb, m.nx1, m.nx2, m.nx3)
+    if levels[-1] > 0:
+        mesh = ("statically refined, root %dx%dx%d, levels 0-%d, %d blocks of %dx%dx%d "
+                "(%d cells)" % (m.nx1 * m.nmbx1, m.nx2 * m.nmbx2, m.nx3 * m.nmbx3, levels[-1],
+                                m.nmb, m.nx1, m.nx2, m.nx3, w.ncells))
+    per = ("lognormal particle counts per block (sigma %g), mean ppc %d" %
+           (w.setup.block_sigma, w.ppc) if w.setup.block_lognormal else "ppc %d" % w.ppc)
+    return {"workload": "%s: %s, %s, %d particles, %s, dt %g" % (
+                            w.name, mesh, per, w.npart, scheme, w.dt),
             "particles": w.npart, "cells": w.ncells, "ppc": w.ppc, "blocks": m.nmb,
             "shear_periodic_x1": bool(m.bc_x1), "coupling": "pc2" if pc2 else "imex",
             "solver": "local" if pc2 else solver}
@@ -166,7 +183,10 @@ def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True, tune_
     else:
         for _ in range(tune_steps):
             ctx.step(w.dt, w.src)
-    for _ in range(warmup):
+    # at least one warm-up step: with graphs its last one captures and instantiates every stage
+    # graph, so that no capture falls into the timed steps (both forms take the same steps)
+    n_warm = max(warmup, 1)
+    for _ in range(n_warm - 1):
         ctx.step(w.dt, w.src)
     if w.mesh.bc_x3 == abi.DK_X3_OUTFLOW:
         # the reference removes the particles parked at an outflow face before every cycle
@@ -177,10 +197,8 @@ def gpu_run(w, form, steps, warmup, tune_steps=0, prof_steps=5, tune=True, tune_
     stream = ctx.stream
     graphs = form == "fp32"
     ctx.set_graphs(graphs)
-    if graphs:
-        # the first graph-launched step captures and instantiates every stage graph: untimed
-        ctx.step(w.dt, w.src)
-        ctx.check_device_error()
+    ctx.step(w.dt, w.src)
+    ctx.check_device_error()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -391,7 +409,11 @@ def main():
               "not run (the path would shard by MeshBlock; see DESIGN.md)" % args.gpus)
         sys.exit(2)
     w = make_workload(args.workload, args.n)
+    smr = any(w.blocks[m].level > 0 for m in range(w.mesh.nmb))
     forms = ["fp32", "fp64"] if args.form == "both" else [args.form]
+    if smr:
+        # refined meshes run in the plain FP64 form (the FP32 form refuses them)
+        forms = ["fp64"]
     res = {}
     tune_steps = 0
     for f in forms:
@@ -406,7 +428,7 @@ def main():
     r = res[main_form]
     ms_step, mps = rate(r, w, args.steps)
     pc2 = w.phys.coupling == 1
-    step_alg = balg_step(main_form, w.ppc, pc2)
+    step_alg = balg_step(main_form, w.ppc, pc2, bool(w.phys.clouds))
     out = {
         "metric": "Mparticle-steps/s & %HBM peak, FP32+batched vs FP64 GPU at 1e-5 rel err",
         "value": round(mps, 3), "unit": "Mparticle-steps/s", "n_gpus": 1,
@@ -445,8 +467,8 @@ def main():
         nbad = int((dp > 1e-5).sum())
         out["fp64_form"] = {"value": round(mps64, 3), "ms_per_step": round(ms64, 3),
                             "step_roofline_frac": round(
-                                mps64 * 1e6 * balg_step("fp64", w.ppc, pc2) / 1e9 /
-                                HBM_PEAK_GBS, 4),
+                                mps64 * 1e6 * balg_step("fp64", w.ppc, pc2, bool(w.phys.clouds)) /
+                                1e9 / HBM_PEAK_GBS, 4),
                             "roofline": kernel_roofline(res["fp64"], w, "fp64"),
                             "momentum_drift": drift_or_none(w, res["fp64"]["drift"])}
         out["speedup_vs_fp64"] = round(mps / mps64, 3)
@@ -462,6 +484,9 @@ def main():
         if ex is not None:
             out["momentum_drift"] = ex["value"]
             out["momentum_drift_how"] = "%s, %d steps" % (ex["how"], ex["steps"])
+    if smr:
+        out["fp32_form"] = ("not run: the FP32 form refuses refined meshes (the cross-level "
+                            "cloud traversal runs in the plain FP64 form)")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     print(json.dumps(out), flush=True)

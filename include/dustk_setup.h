@@ -5,8 +5,9 @@
  * src/pgen/dust_amr/dust_setup.cpp:735-750, src/dust/dust.cpp:1716-1778) so that the HIP
  * library and the CPU oracle start from identical, index-keyed particle sets.
  *
- * Particle q of block m (q = m*npb + r, npb = ppc * nx1*nx2*nx3) gets PGID = m, PTAG = q and
- * the draws below, each a function of (seed, q) only:
+ * Particle q of block m (q = m*npb + r, npb = ppc * nx1*nx2*nx3; with block_lognormal q runs
+ * over the blocks' prefix offsets instead) gets PGID = m, PTAG = q and the draws below, each a
+ * function of (seed, q) only:
  *   key  = q + seed*0x9e3779b97f4a7c15
  *   x    = x1min_m + U(key + 0x632be59bd9b4e019) * lx_m        (RootPlacement::Base3 mode 0)
  *   y    = x2min_m + U(key + 0x8cb92baa3f3d8dd7) * ly_m
@@ -19,7 +20,10 @@
  * eps_s*rho0*V_cell*nspecies/ppc; stopping time taus_s, or for continuous bins a log-uniform
  * draw in [tau_lo_s, tau_hi_s] with mass weight tau^mass_slope / <tau^mass_slope>_bin.
  * Gas: rho = rho0 (times exp(-z^2/2) if stratified), momentum rho*(u_mean + N(gas_noise))
- * with the cell key (global cell index) + seed*0x9e3779b97f4a7c15 + DK_SALT_G[c].
+ * with the cell key (global cell index) + seed*0x9e3779b97f4a7c15 + DK_SALT_G[c]; on a
+ * statically refined mesh the global cell index is that of the block's level, plus
+ * level * 2^40.  Block counts (block_lognormal): z_m = N with the key
+ * seed*0x9e3779b97f4a7c15 + m*0x2545f4914f6cdd1d + DK_SALT_BLOCK, evaluated on the host.
  */
 #ifndef DUSTK_SETUP_H_
 #define DUSTK_SETUP_H_
@@ -56,6 +60,12 @@ typedef struct dk_setup {
                                     the block holding the position */
   double zmetal;               /* > 0: masses scaled so that total dust mass = zmetal * total
                                   gas mass, equal per species (stratified problems) */
+  int32_t block_lognormal;     /* 1: uneven blocks: block m holds n_m = round(ppc * nx1*nx2*nx3 *
+                                  exp(block_sigma z_m - block_sigma^2/2)) particles, z_m = N(1)
+                                  keyed by (seed, m), uniform in the block (the particles of
+                                  block m follow those of blocks 0..m-1); with refinable clouds
+                                  PLEV is the block's level.  No clumps, no Gaussian heights. */
+  double block_sigma;
 } dk_setup;
 
 /* salts of the velocity and gas draws */
@@ -67,6 +77,7 @@ typedef struct dk_setup {
 #define DK_SALT_G2 0x5be0cd19137e2179ULL
 #define DK_SALT_TAU 0xcbbb9d5dc1059ed8ULL
 #define DK_SALT_CLUMP 0x629a292a367cd507ULL
+#define DK_SALT_BLOCK 0x7137449123ef65cdULL
 
 struct dk_ctx;
 

@@ -131,6 +131,12 @@ struct or_sim {
   std::vector<double> gpack;
   // fold order: the same-level neighbour slots in NeighborIndex order
   std::vector<int> slot_dir;   // dir index of each used slot, ordered
+  // a statically refined mesh: every block a leaf of level 0..maxlev; F[d] finest-level block
+  // slots per direction, leafmap the leaf covering each slot (x1 fastest)
+  bool smr = false;
+  int maxlev = 0;
+  int F[3] = {0, 0, 0};
+  std::vector<int> leafmap;
 
   int64_t Idx(int m, int nvar, int v, int k, int j, int i) const {
     return ((((int64_t)m*nvar + v)*n3 + k)*n2 + j)*(int64_t)n1 + i;
@@ -365,6 +371,228 @@ inline void GatherLinear(const or_sim *s, const int m, const int k, const int j,
   u[2] += w*gp[2] + cx*gp[5] + cy*gp[8] + cz*gp[11];
 }
 
+//----------------------------------------------------------------------------------------
+// Statically refined meshes (every block a leaf; levels differ by one at most between
+// neighbours).  A leaf is found from a position through the finest-level block grid; the
+// traversal of a cloud visits the actual leaves under its support at their own resolution, as
+// LeafTraversal3D::ForEach (src/dust/dust_leaf3d.hpp:383-517) does, with the candidate blocks
+// taken from the leaves holding the support's 8 corners (a leaf is at least as long as any
+// support in each direction, so every leaf that meets the support holds one of its corners)
+// instead of the anchor's neighbour list.
+
+// p moved by whole mesh lengths into [min, max) of the mesh; k: the lengths taken off
+void WrapPoint(const or_sim *s, double p[3], int k[3]) {
+  const double lo[3] = {s->M.x1min, s->M.x2min, s->M.x3min};
+  const double hi[3] = {s->M.x1max, s->M.x2max, s->M.x3max};
+  for (int d=0; d<3; ++d) {
+    const double len = hi[d] - lo[d];
+    k[d] = 0;
+    while (p[d] < lo[d]) {p[d] += len; --k[d];}
+    while (p[d] >= hi[d]) {p[d] -= len; ++k[d];}
+  }
+}
+
+// the leaf holding p (inside the mesh): its finest block slot, then one slot over in each
+// direction where p lies outside that leaf's box by round-off; -1 if none holds it
+int LeafAt(const or_sim *s, const double p[3]) {
+  const double lo[3] = {s->M.x1min, s->M.x2min, s->M.x3min};
+  const double len[3] = {s->M.x1max - s->M.x1min, s->M.x2max - s->M.x2min,
+                         s->M.x3max - s->M.x3min};
+  int sl[3];
+  for (int d=0; d<3; ++d) {
+    sl[d] = static_cast<int>(std::floor((p[d] - lo[d])/len[d]*s->F[d]));
+    sl[d] = std::min(std::max(sl[d], 0), s->F[d] - 1);
+  }
+  for (int it=0; it<4; ++it) {
+    const int b = s->leafmap[((int64_t)sl[2]*s->F[1] + sl[1])*s->F[0] + sl[0]];
+    const dk_block &B = s->B[b];
+    const double bmin[3] = {B.x1min, B.x2min, B.x3min}, bmax[3] = {B.x1max, B.x2max, B.x3max};
+    const int r = 1 << (s->maxlev - B.level);
+    const int l[3] = {B.lx1, B.lx2, B.lx3};
+    bool moved = false;
+    for (int d=0; d<3; ++d) {
+      if (p[d] < bmin[d] && l[d]*r > 0) {sl[d] = l[d]*r - 1; moved = true;}
+      else if (p[d] >= bmax[d] && (l[d] + 1)*r < s->F[d]) {sl[d] = (l[d] + 1)*r; moved = true;}
+    }
+    if (!moved) return b;
+  }
+  return -1;
+}
+
+// SetNewPrtclGID's choice of the neighbour slot (src/bvals/bvals_part.cpp:308-390) on a
+// statically refined mesh: the leaf holding the same-level region beside block m in the
+// direction (ix, iy, iz) of the crossing when it is of m's level or coarser (a coarser leaf
+// also covers an empty edge or corner slot, coarse_cover), else the finer leaf beside that
+// face, edge or corner in the half of m that x occupies along the other directions (the
+// sub-slot of fx, fy, fz, :224-229); -1 when none (a mesh that breaks the 2:1 balance)
+int SmrDest(const or_sim *s, const int m, const int off[3], const double x[3]) {
+  const dk_block &b = s->B[m];
+  const double bmin[3] = {b.x1min, b.x2min, b.x3min}, bmax[3] = {b.x1max, b.x2max, b.x3max};
+  double q[3];
+  int k[3];
+  for (int d=0; d<3; ++d) q[d] = bmin[d] + (off[d] + 0.5)*(bmax[d] - bmin[d]);
+  WrapPoint(s, q, k);
+  const int n0 = LeafAt(s, q);
+  if (n0 < 0 || s->B[n0].level <= b.level) return n0;
+  for (int d=0; d<3; ++d) {
+    const double l = bmax[d] - bmin[d];
+    const int f = (x[d] < 0.5*(bmin[d] + bmax[d])) ? 0 : 1;
+    q[d] = (off[d] < 0) ? bmin[d] - 0.25*l
+         : ((off[d] > 0) ? bmax[d] + 0.25*l : bmin[d] + (f ? 0.75 : 0.25)*l);
+  }
+  WrapPoint(s, q, k);
+  const int n = LeafAt(s, q);
+  return (n >= 0 && s->B[n].level == b.level + 1) ? n : -1;
+}
+
+// LeafTraversal3D::ForEach for a plain cloud of level lev at x (half-width root dx 2^-lev):
+// function(b, k, j, i, w, m1x, m1y, m1z) (padded indices of leaf b) with moments,
+// function(b, k, j, i, w) without; per candidate leaf the reference's loop order (x runs of 4,
+// then x3, x2, x1) and skips.  Candidates in the order of the support corners that found them
+// (x1 fastest, low side first).  False when a corner lies in no leaf.
+template <bool kMoments, class F>
+bool LeafForEach(const or_sim *s, const double x[3], const int lev, const F &function) {
+  CloudAxis sa[3];
+  for (int d=0; d<3; ++d)
+    sa[d] = MakeAxis(x[d], std::ldexp(s->root_dx[d], -lev), std::ldexp(s->iroot_dx[d], lev));
+  int cb[8], cs[8][3], nc = 0;
+  for (int c=0; c<8; ++c) {
+    double p[3];
+    int k[3];
+    for (int d=0; d<3; ++d)
+      p[d] = ((c >> d) & 1) ? std::nextafter(sa[d].hi, -std::numeric_limits<double>::infinity())
+                            : sa[d].lo;
+    WrapPoint(s, p, k);
+    const int b = LeafAt(s, p);
+    if (b < 0) return false;
+    bool dup = false;
+    for (int q=0; q<nc; ++q)
+      if (cb[q] == b && cs[q][0] == k[0] && cs[q][1] == k[1] && cs[q][2] == k[2]) dup = true;
+    if (dup) continue;
+    cb[nc] = b; cs[nc][0] = k[0]; cs[nc][1] = k[1]; cs[nc][2] = k[2]; ++nc;
+  }
+  const double len[3] = {s->M.x1max - s->M.x1min, s->M.x2max - s->M.x2min,
+                         s->M.x3max - s->M.x3min};
+  // (a leaf shorter than the support in some direction might meet it without holding a
+  // corner: refused)
+  for (int q=0; q<nc; ++q) {
+    const dk_block &B = s->B[cb[q]];
+    if (B.x1max - B.x1min < 2.0*sa[0].ell || B.x2max - B.x2min < 2.0*sa[1].ell ||
+        B.x3max - B.x3min < 2.0*sa[2].ell)
+      return false;
+  }
+  constexpr int kRun = 4;
+  for (int q=0; q<nc; ++q) {
+    const int b = cb[q];
+    const dk_block &B = s->B[b];
+    const double dx = s->dx1[b], dy = s->dx2[b], dz = s->dx3[b];
+    const double bx0 = B.x1min + cs[q][0]*len[0], by0 = B.x2min + cs[q][1]*len[1];
+    const double bz0 = B.x3min + cs[q][2]*len[2];
+    const double idx = 1.0/dx, idy = 1.0/dy, idz = 1.0/dz;
+    int i0 = static_cast<int>(std::floor((sa[0].lo - bx0)*idx));
+    int i1 = static_cast<int>(std::ceil((sa[0].hi - bx0)*idx)) - 1;
+    int j0 = static_cast<int>(std::floor((sa[1].lo - by0)*idy));
+    int j1 = static_cast<int>(std::ceil((sa[1].hi - by0)*idy)) - 1;
+    int k0 = static_cast<int>(std::floor((sa[2].lo - bz0)*idz));
+    int k1 = static_cast<int>(std::ceil((sa[2].hi - bz0)*idz)) - 1;
+    i0 = std::max(i0, 0); i1 = std::min(i1, s->nx1 - 1);
+    j0 = std::max(j0, 0); j1 = std::min(j1, s->nx2 - 1);
+    k0 = std::max(k0, 0); k1 = std::min(k1, s->nx3 - 1);
+    for (int ic=i0; ic<=i1; ic+=kRun) {
+      const int nr = (i1 - ic + 1 < kRun) ? i1 - ic + 1 : kRun;
+      double w1[kRun], g1[kRun];
+      CloudFace f1 = FaceAt(sa[0], bx0 + ic*dx, kMoments);
+      for (int a=0; a<kRun; ++a) {
+        w1[a] = 0.0; g1[a] = 0.0;
+        if (a < nr) {
+          const CloudFace fn = FaceAt(sa[0], bx0 + (ic + a + 1)*dx, kMoments);
+          CloudCell(sa[0], f1, fn, kMoments, w1[a], g1[a]);
+          f1 = fn;
+        }
+      }
+      CloudFace f3 = FaceAt(sa[2], bz0 + k0*dz, kMoments);
+      for (int k=k0; k<=k1; ++k) {
+        const CloudFace fn3 = FaceAt(sa[2], bz0 + (k + 1)*dz, kMoments);
+        double w3, g3;
+        CloudCell(sa[2], f3, fn3, kMoments, w3, g3);
+        f3 = fn3;
+        if (w3 == 0.0 && (!kMoments || g3 == 0.0)) continue;
+        CloudFace f2 = FaceAt(sa[1], by0 + j0*dy, kMoments);
+        for (int j=j0; j<=j1; ++j) {
+          const CloudFace fn2 = FaceAt(sa[1], by0 + (j + 1)*dy, kMoments);
+          double w2, g2;
+          CloudCell(sa[1], f2, fn2, kMoments, w2, g2);
+          f2 = fn2;
+          if (w2 == 0.0 && (!kMoments || g2 == 0.0)) continue;
+          const double w23 = w2*w3;
+          for (int a=0; a<nr; ++a) {
+            const double wa = w1[a];
+            const double w = w23*wa;
+            if constexpr (kMoments) {
+              const double ga = g1[a];
+              const double m1x = w23*ga;
+              const double m1y = wa*g2*w3, m1z = wa*w2*g3;
+              if (w == 0.0 && m1x == 0.0 && m1y == 0.0 && m1z == 0.0) continue;
+              function(b, k + s->ks, j + s->js, ic + a + s->is, w, m1x, m1y, m1z);
+            } else {
+              if (w == 0.0) continue;
+              function(b, k + s->ks, j + s->js, ic + a + s->is, w);
+            }
+          }
+        }
+      }
+    }
+  }
+  return true;
+}
+
+// The harness's gas ghost values on a statically refined mesh (a stand-in for the hydro
+// boundary exchange of src/bvals/bvals_cc.cpp, which is not part of the drag path): every ghost
+// cell takes the cell of the leaf holding its centre when that leaf is of the block's level
+// (a copy) or coarser (the coarse value, injected), or the mean of the 8 cells of a finer leaf
+// that make it up (the restriction of src/bvals/prolongation.cpp:104-111, same order)
+void SmrGhosts(or_sim *s, std::vector<double> &a, int nvar) {
+  const int nmb = s->M.nmb;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int m=0; m<nmb; ++m) {
+    const dk_block &b = s->B[m];
+    for (int k=0; k<s->n3; ++k)
+      for (int j=0; j<s->n2; ++j)
+        for (int i=0; i<s->n1; ++i) {
+          if (i >= s->is && i <= s->ie && j >= s->js && j <= s->je && k >= s->ks &&
+              k <= s->ke)
+            continue;
+          double c[3] = {b.x1min + (i - s->is + 0.5)*s->dx1[m],
+                         b.x2min + (j - s->js + 0.5)*s->dx2[m],
+                         b.x3min + (k - s->ks + 0.5)*s->dx3[m]};
+          int sh[3];
+          WrapPoint(s, c, sh);
+          const int n = LeafAt(s, c);
+          if (n < 0) {s->Fail(DK_EROUTE, -1); continue;}
+          const dk_block &nb = s->B[n];
+          if (nb.level <= b.level) {
+            const int in = static_cast<int>(std::floor((c[0] - nb.x1min)/s->dx1[n])) + s->is;
+            const int jn = static_cast<int>(std::floor((c[1] - nb.x2min)/s->dx2[n])) + s->js;
+            const int kn = static_cast<int>(std::floor((c[2] - nb.x3min)/s->dx3[n])) + s->ks;
+            for (int v=0; v<nvar; ++v) a[s->Idx(m,nvar,v,k,j,i)] = a[s->Idx(n,nvar,v,kn,jn,in)];
+          } else {
+            const int fi = static_cast<int>(std::floor((c[0] - 0.5*s->dx1[m] - nb.x1min)/
+                                                       s->dx1[n] + 0.5)) + s->is;
+            const int fj = static_cast<int>(std::floor((c[1] - 0.5*s->dx2[m] - nb.x2min)/
+                                                       s->dx2[n] + 0.5)) + s->js;
+            const int fk = static_cast<int>(std::floor((c[2] - 0.5*s->dx3[m] - nb.x3min)/
+                                                       s->dx3[n] + 0.5)) + s->ks;
+            for (int v=0; v<nvar; ++v)
+              a[s->Idx(m,nvar,v,k,j,i)] = 0.125*(
+                  a[s->Idx(n,nvar,v,fk  ,fj  ,fi)] + a[s->Idx(n,nvar,v,fk  ,fj  ,fi+1)]
+                + a[s->Idx(n,nvar,v,fk  ,fj+1,fi)] + a[s->Idx(n,nvar,v,fk  ,fj+1,fi+1)]
+                + a[s->Idx(n,nvar,v,fk+1,fj  ,fi)] + a[s->Idx(n,nvar,v,fk+1,fj  ,fi+1)]
+                + a[s->Idx(n,nvar,v,fk+1,fj+1,fi)] + a[s->Idx(n,nvar,v,fk+1,fj+1,fi+1)]);
+          }
+        }
+  }
+}
+
 // src/dust/dust_cloud3d.cpp:1319-1340 (PackGatherField3D): velocity and its central
 // differences of every cell with both neighbours in each direction; vel(m, c, k, j, i)
 void PackGatherField(or_sim *s, const std::vector<double> &vel) {
@@ -401,6 +629,33 @@ void PackGasGather(or_sim *s, const std::vector<double> &u) {
 // weights into qdep 1-3 (and 4)
 void CloudPredictor(or_sim *s, const double dt, const bool rate_dep) {
   PackGasGather(s, s->u1);
+  if (s->smr) {
+    // the actual leaves under the cloud at their own resolution; the deposit walks the cloud
+    // again without moments (the same cells and weights in the same order)
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+    for (int64_t p=0; p<s->np; ++p) {
+      const double x[3] = {s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p)};
+      const int lev = s->I(DK_PLEV,p);
+      double ug[3] = {0.0, 0.0, 0.0};
+      const bool ok = LeafForEach<true>(s, x, lev,
+          [&](int b, int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+            GatherLinear(s, b, k, j, i, w, m1x, m1y, m1z, x, ug);
+          });
+      if (!ok) {s->Fail(DK_EHALO, p); continue;}
+      const double mass_rate = s->R(DK_IPM,p)/s->R(DK_IPTS,p);
+      const double gx = -dt*mass_rate*(ug[0] - s->R(DK_IPVX,p));
+      const double gy = -dt*mass_rate*(ug[1] - s->R(DK_IPVY,p));
+      const double gz = -dt*mass_rate*(ug[2] - s->R(DK_IPVZ,p));
+      LeafForEach<false>(s, x, lev, [&](int b, int k, int j, int i, double w) {
+        const double inv_volume = 1.0/(s->dx1[b]*s->dx2[b]*s->dx3[b]);
+        DepositAdd(s, &s->qdep[s->Idx(b,5,1,k,j,i)], inv_volume*w*gx);
+        DepositAdd(s, &s->qdep[s->Idx(b,5,2,k,j,i)], inv_volume*w*gy);
+        DepositAdd(s, &s->qdep[s->Idx(b,5,3,k,j,i)], inv_volume*w*gz);
+        if (rate_dep) DepositAdd(s, &s->qdep[s->Idx(b,5,4,k,j,i)], inv_volume*w*mass_rate);
+      });
+    }
+    return;
+  }
 #pragma omp parallel for num_threads(s->nthreads) schedule(static)
   for (int64_t p=0; p<s->np; ++p) {
     double x[3] = {s->R(DK_IPX,p), s->R(DK_IPY,p), s->R(DK_IPZ,p)};
@@ -451,17 +706,27 @@ void CloudAdvance(or_sim *s, const double dt) {
     xg[0] = xh[0];
     xg[1] = xh[1] + (sbox ? qs*om*xh[0]*0.5*dt : 0.0);
     xg[2] = xh[2];
-    const int am = CloudAnchor(s, s->I(DK_PGID,p), xg);
+    const int lev = s->I(DK_PLEV,p);
+    const int am = s->smr ? -1 : CloudAnchor(s, s->I(DK_PGID,p), xg);
     double ug[3] = {0.0, 0.0, 0.0};
     int nt = 0, tk[27], tj[27], ti[27];
     double tw[27];
-    const bool ok = (am >= 0) && CloudForEach<true>(s, am, xg,
-        [&](int k, int j, int i, double w, double m1x, double m1y, double m1z) {
-          GatherLinear(s, am, k, j, i, w, m1x, m1y, m1z, xg, ug);
-          if (nt < 27) {tk[nt] = k; tj[nt] = j; ti[nt] = i; tw[nt] = w;}
-          ++nt;
-        });
-    if (!ok || nt > 27) {s->Fail(DK_EHALO, p); continue;}
+    bool ok;
+    if (s->smr) {
+      ok = LeafForEach<true>(s, xg, lev,
+          [&](int b, int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+            GatherLinear(s, b, k, j, i, w, m1x, m1y, m1z, xg, ug);
+          });
+    } else {
+      ok = (am >= 0) && CloudForEach<true>(s, am, xg,
+          [&](int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+            GatherLinear(s, am, k, j, i, w, m1x, m1y, m1z, xg, ug);
+            if (nt < 27) {tk[nt] = k; tj[nt] = j; ti[nt] = i; tw[nt] = w;}
+            ++nt;
+          });
+      ok = ok && nt <= 27;
+    }
+    if (!ok) {s->Fail(DK_EHALO, p); continue;}
     const double h = 0.5*dt;
     const double lambda = h/s->R(DK_IPTS,p);
     const double aa = 1.0 + lambda;
@@ -492,6 +757,15 @@ void CloudAdvance(or_sim *s, const double dt) {
     s->R(DK_IPRX,p) = dvx; s->R(DK_IPRY,p) = dvy; s->R(DK_IPRZ,p) = dvz;
     if (!br) continue;
     const double mass = -s->R(DK_IPM,p);
+    if (s->smr) {
+      LeafForEach<false>(s, xg, lev, [&](int b, int k, int j, int i, double w) {
+        const double dw = mass*w/(s->dx1[b]*s->dx2[b]*s->dx3[b]);
+        DepositAdd(s, &s->dmom[s->Idx(b,3,0,k,j,i)], dw*dvx);
+        DepositAdd(s, &s->dmom[s->Idx(b,3,1,k,j,i)], dw*dvy);
+        DepositAdd(s, &s->dmom[s->Idx(b,3,2,k,j,i)], dw*dvz);
+      });
+      continue;
+    }
     const double measure = s->dx1[am]*s->dx2[am]*s->dx3[am];
     for (int q=0; q<nt; ++q) {
       if (tw[q] == 0.0) continue;
@@ -507,7 +781,10 @@ void CloudAdvance(or_sim *s, const double dt) {
 // ghost copy exchange of a cell field (periodic same-level), src/bvals/bvals_cc.cpp:53-296;
 // with shear-periodic x1 the x1 ghosts of the blocks on the mesh x1 faces are then replaced
 // by the shear remap, and at physical x3 faces by UstarX3BCs (src/dust/dust_tasks.cpp:445-469)
+void SmrGhosts(or_sim *s, std::vector<double> &a, int nvar);
+
 void CopyGhosts(or_sim *s, std::vector<double> &a, int nvar) {
+  if (s->smr) {SmrGhosts(s, a, nvar); return;}
   const int nmb = s->M.nmb;
 #pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
   for (int m=0; m<nmb; ++m) {
@@ -774,10 +1051,16 @@ or_sim *or_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics 
                   int64_t capacity) {
   if (mesh == nullptr || blocks == nullptr || phys == nullptr) return nullptr;
   if (mesh->nx3 < 2 || mesh->ng < 1) return nullptr;   // 3-D meshes only
-  // refinable clouds: pc2 on a same-level mesh periodic in x1, x2, x3, ghosts of 2
+  // refinable clouds: pc2 on a mesh periodic in x1, x2, x3, ghosts of 2
   if (phys->clouds && (phys->coupling != DK_PC2 || mesh->bc_x1 != DK_X1_PERIODIC ||
                        mesh->bc_x3 != DK_X3_PERIODIC || mesh->ng < 2))
     return nullptr;
+  // a statically refined mesh (levels above 0): refinable clouds only (their traversal of the
+  // actual leaves; point-particle deposits would need the cross-level fold)
+  int maxlev = 0;
+  for (int m=0; m<mesh->nmb; ++m) maxlev = std::max(maxlev, blocks[m].level);
+  const bool smr = maxlev > 0 || mesh->nmb != mesh->nmbx1*mesh->nmbx2*mesh->nmbx3;
+  if (smr && !phys->clouds) return nullptr;
   or_sim *s = new or_sim();
   s->M = *mesh;
   s->P = *phys;
@@ -824,6 +1107,31 @@ or_sim *or_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics 
       }
   std::sort(order.begin(), order.end());
   for (auto &o : order) s->slot_dir.push_back(o.second);
+  if (smr) {
+    // the finest-level block grid: every slot covered by exactly one leaf
+    s->smr = true;
+    s->maxlev = maxlev;
+    s->F[0] = mesh->nmbx1 << maxlev; s->F[1] = mesh->nmbx2 << maxlev;
+    s->F[2] = mesh->nmbx3 << maxlev;
+    s->leafmap.assign((size_t)s->F[0]*s->F[1]*s->F[2], -1);
+    for (int m=0; m<mesh->nmb; ++m) {
+      const dk_block &b = blocks[m];
+      if (b.level < 0 || b.level > maxlev) {delete s; return nullptr;}
+      const int r = 1 << (maxlev - b.level);
+      const int l[3] = {b.lx1, b.lx2, b.lx3};
+      for (int d=0; d<3; ++d)
+        if (l[d] < 0 || (l[d] + 1)*r > s->F[d]) {delete s; return nullptr;}
+      for (int k=b.lx3*r; k<(b.lx3 + 1)*r; ++k)
+        for (int j=b.lx2*r; j<(b.lx2 + 1)*r; ++j)
+          for (int i=b.lx1*r; i<(b.lx1 + 1)*r; ++i) {
+            int &slot = s->leafmap[((size_t)k*s->F[1] + j)*s->F[0] + i];
+            if (slot >= 0) {delete s; return nullptr;}
+            slot = m;
+          }
+    }
+    for (int v : s->leafmap)
+      if (v < 0) {delete s; return nullptr;}
+  }
   return s;
 }
 
@@ -859,12 +1167,35 @@ int or_last_error(or_sim *s, int64_t *particle) {
 //========================================================================================
 // problem set-up (dustk_setup.h)
 
+// the particle count of every block of an uneven set-up (dk_setup.block_lognormal) and their
+// prefix offsets (nmb + 1 entries)
+std::vector<int64_t> BlockOffsets(const or_sim *s, const dk_setup *su) {
+  const int nmb = s->M.nmb;
+  const int64_t ncell_b = (int64_t)s->nx1*s->nx2*s->nx3;
+  const uint64_t base = su->seed*UINT64_C(0x9e3779b97f4a7c15);
+  std::vector<int64_t> off(nmb + 1, 0);
+  for (int m=0; m<nmb; ++m) {
+    int64_t n = (int64_t)su->ppc*ncell_b;
+    if (su->block_lognormal) {
+      const double z = Gauss(base + static_cast<uint64_t>(m)*UINT64_C(0x2545f4914f6cdd1d) +
+                             DK_SALT_BLOCK);
+      const double sg = su->block_sigma;
+      n = std::llround(static_cast<double>(su->ppc*ncell_b)*std::exp(sg*z - 0.5*sg*sg));
+    }
+    off[m + 1] = off[m] + n;
+  }
+  return off;
+}
+
 int or_setup(or_sim *s, const dk_setup *su) {
   const int nmb = s->M.nmb;
   const int64_t ncell_b = (int64_t)s->nx1*s->nx2*s->nx3;
   const int64_t npb = (int64_t)su->ppc*ncell_b;
-  const int64_t ntot = npb*nmb;
-  if (ntot > s->cap || su->nspecies < 1 || su->nspecies > DK_MAXSPEC) return DK_ECAPACITY;
+  if (su->nspecies < 1 || su->nspecies > DK_MAXSPEC) return DK_ECAPACITY;
+  if ((su->block_lognormal || s->smr) && (su->nclump > 0 || su->gauss_z)) return DK_EINVAL;
+  const std::vector<int64_t> boff = BlockOffsets(s, su);
+  const int64_t ntot = boff[nmb];
+  if (ntot > s->cap) return DK_ECAPACITY;
   s->np = ntot;
   const uint64_t base = su->seed*UINT64_C(0x9e3779b97f4a7c15);
   const double lxm = s->M.x1max - s->M.x1min, lym = s->M.x2max - s->M.x2min;
@@ -894,7 +1225,9 @@ int or_setup(or_sim *s, const dk_setup *su) {
 #pragma omp parallel for num_threads(s->nthreads) schedule(static)
   for (int64_t q=0; q<ntot; ++q) {
     const uint64_t key = static_cast<uint64_t>(q) + base;
-    int m = static_cast<int>(q/npb);
+    int m = su->block_lognormal
+          ? static_cast<int>(std::upper_bound(boff.begin(), boff.end(), q) - boff.begin()) - 1
+          : static_cast<int>(q/npb);
     const int sp = static_cast<int>(q % su->nspecies);
     double x, y, z;
     if (su->nclump > 0) {
@@ -986,20 +1319,23 @@ int or_setup(or_sim *s, const dk_setup *su) {
     s->R(DK_IPRX,q) = 0.0; s->R(DK_IPRY,q) = 0.0; s->R(DK_IPRZ,q) = 0.0;
     s->R(DK_IPTS,q) = ts; s->R(DK_IPM,q) = mass;
     s->I(DK_PGID,q) = m; s->I(DK_PTAG,q) = static_cast<int32_t>(q);
-    s->I(DK_PSP,q) = sp; s->I(DK_PLEV,q) = 0; s->I(DK_PTAGH,q) = 0;
+    // (a refinable cloud of its block's level: half-width one cell of that level)
+    s->I(DK_PSP,q) = sp; s->I(DK_PLEV,q) = s->B[m].level; s->I(DK_PTAGH,q) = 0;
   }
-  // gas
-  const int ny = s->nx2*s->M.nmbx2, nxg = s->nx1*s->M.nmbx1;
+  // gas (the global cell index of the block's level, plus level * 2^40)
   Zero(s->u0);
 #pragma omp parallel for num_threads(s->nthreads) schedule(static)
   for (int m=0; m<nmb; ++m) {
     const dk_block &b = s->B[m];
+    const int64_t ny = (int64_t)s->nx2*(s->M.nmbx2 << b.level);
+    const int64_t nxg = (int64_t)s->nx1*(s->M.nmbx1 << b.level);
     for (int k=s->ks; k<=s->ke; ++k)
       for (int j=s->js; j<=s->je; ++j)
         for (int i=s->is; i<=s->ie; ++i) {
           const int64_t gi = b.lx1*s->nx1 + (i - s->is), gj = b.lx2*s->nx2 + (j - s->js);
           const int64_t gk = b.lx3*s->nx3 + (k - s->ks);
-          const uint64_t ck = static_cast<uint64_t>((gk*ny + gj)*nxg + gi) + base;
+          const uint64_t ck = static_cast<uint64_t>((gk*ny + gj)*nxg + gi) + base +
+                              (static_cast<uint64_t>(b.level) << 40);
           const double z = b.x3min + (k - s->ks + 0.5)*s->dx3[m];
           const double rho = su->gas_strat ? su->rho0*std::exp(-0.5*z*z) : su->rho0;
           const double um[3] = {su->gas_vx, su->gas_vy, su->gas_vz};
@@ -1281,7 +1617,14 @@ int or_update_particle_gids(or_sim *s, const dk_stage *st) {
         const int b1 = cross_in ? (ms.nmbx1 - 1) : 0;   // the boundary block on the other side
         s->I(DK_PGID,p) = b1 + ms.nmbx1*(b2 + ms.nmbx2*b3);
       } else {
-        const int nb = b.nghbr[Dir(ix, iy, iz)];
+        int nb;
+        if (s->smr) {
+          const int off[3] = {ix, iy, iz};
+          const double xs[3] = {x1, x2, x3};
+          nb = SmrDest(s, m, off, xs);
+        } else {
+          nb = b.nghbr[Dir(ix, iy, iz)];
+        }
         if (nb < 0) {
           s->Fail(DK_EROUTE, p);
           continue;
@@ -1397,7 +1740,8 @@ int or_deposit_drag(or_sim *s, const dk_stage *st) {
 // dust_tasks.cpp:223-225)
 int or_fold_deposit_qp(or_sim *s, const dk_stage *st) {
   const bool use = (s->P.coupling == DK_IMEX) || st->stage == 1;
-  if (!ActiveStage(s, st) || !s->P.back_reaction || !use) return DK_OK;
+  // (a refined mesh: the clouds deposited into the leaves themselves, no ghost holds any)
+  if (!ActiveStage(s, st) || !s->P.back_reaction || !use || s->smr) return DK_OK;
   Fold(s, s->qdep, 5, st->time);
   return DK_OK;
 }
@@ -1847,7 +2191,7 @@ int or_gather_kick_pmbr(or_sim *s, const dk_stage *st) {
 // src/dust/dust_tasks.cpp:314-328
 int or_fold_pmbr(or_sim *s, const dk_stage *st) {
   const bool use = (s->P.coupling == DK_IMEX) || st->stage == 2;
-  if (!ActiveStage(s, st) || !s->P.back_reaction || !use) return DK_OK;
+  if (!ActiveStage(s, st) || !s->P.back_reaction || !use || s->smr) return DK_OK;
   Fold(s, s->dmom, 3, st->time);
   return DK_OK;
 }
@@ -2145,6 +2489,41 @@ int or_cloud_gather(or_sim *s, const double *vel, int m, const double x[3], doub
       });
   for (int d=0; d<3; ++d) out[d] = u[d];
   return ok ? 0 : -1;
+}
+
+int or_leaf_cells(or_sim *s, const double x[3], int lev, int32_t *blk, int64_t *cell,
+                  double *w, double *m1, int cap) {
+  int n = 0;
+  const bool ok = LeafForEach<true>(s, x, lev,
+      [&](int b, int k, int j, int i, double ww, double a, double bb, double c) {
+        if (n < cap) {
+          blk[n] = b;
+          cell[n] = ((int64_t)k*s->n2 + j)*s->n1 + i;
+          w[n] = ww; m1[3*n] = a; m1[3*n+1] = bb; m1[3*n+2] = c;
+        }
+        ++n;
+      });
+  return (ok && n <= cap) ? n : -1;
+}
+
+int or_leaf_gather(or_sim *s, const double *vel, const double x[3], int lev, double out[3]) {
+  std::vector<double> v(vel, vel + 3*s->ncb*s->M.nmb);
+  PackGatherField(s, v);
+  double u[3] = {0.0, 0.0, 0.0};
+  const bool ok = LeafForEach<true>(s, x, lev,
+      [&](int b, int k, int j, int i, double ww, double a, double bb, double c) {
+        GatherLinear(s, b, k, j, i, ww, a, bb, c, x, u);
+      });
+  for (int d=0; d<3; ++d) out[d] = u[d];
+  return ok ? 0 : -1;
+}
+
+int or_leaf_at(or_sim *s, const double x[3]) {
+  if (!s->smr) return -1;
+  double p[3] = {x[0], x[1], x[2]};
+  int k[3];
+  WrapPoint(s, p, k);
+  return LeafAt(s, p);
 }
 
 int or_diag_sums(or_sim *s, double out[9]) {

@@ -73,6 +73,13 @@ int or_apply_operator(or_sim *s, double a_dt, double sh, const double *x, double
 int or_cloud_cells(or_sim *s, int m, const double x[3], int64_t *cell, double *w,
                    double *m1 /* [27][3] */);
 int or_cloud_gather(or_sim *s, const double *vel, int m, const double x[3], double out[3]);
+/* the same on a statically refined mesh (every block a leaf): the leaf cells under the cloud of
+ * level lev at x (block, padded cell index, weight, first moments; at most cap), returned count
+ * or -1; the linear gather there; the leaf holding x (-1 on a uniform mesh). */
+int or_leaf_cells(or_sim *s, const double x[3], int lev, int32_t *blk, int64_t *cell,
+                  double *w, double *m1, int cap);
+int or_leaf_gather(or_sim *s, const double *vel, const double x[3], int lev, double out[3]);
+int or_leaf_at(or_sim *s, const double x[3]);
 
 int or_diag_sums(or_sim *s, double out[9]);
 /* the strict solver's parameters (<dust> drag_rtol, drag_atol, drag_iter_max,

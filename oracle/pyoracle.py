@@ -56,6 +56,10 @@ def lib():
         D = C.POINTER(C.c_double)
         L.or_cloud_cells.argtypes = [P, C.c_int, D, C.POINTER(C.c_int64), D, D]
         L.or_cloud_gather.argtypes = [P, D, C.c_int, D, D]
+        L.or_leaf_cells.argtypes = [P, D, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                    D, D, C.c_int]
+        L.or_leaf_gather.argtypes = [P, D, D, C.c_int, D]
+        L.or_leaf_at.argtypes = [P, D]
         L.or_set_solver.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_double,
                                     C.c_double, C.c_double, C.c_double]
         L.or_solver_stats.argtypes = [P, D]
@@ -170,6 +174,30 @@ class Oracle:
         assert lib().or_cloud_gather(self.h, vel.ctypes.data_as(C.POINTER(C.c_double)), m,
                                      xs, out) == 0
         return np.array(out[:])
+
+    def leaf_cells(self, x, lev, cap=4096):
+        """On a refined mesh: (blocks, padded cell indices, weights, first moments [n, 3]) of the
+        leaf cells under the cloud of level lev at x."""
+        xs = (C.c_double * 3)(*x)
+        blk = (C.c_int32 * cap)()
+        cell = (C.c_int64 * cap)()
+        w = (C.c_double * cap)()
+        m1 = (C.c_double * (3 * cap))()
+        n = lib().or_leaf_cells(self.h, xs, int(lev), blk, cell, w, m1, cap)
+        assert n >= 0, "cloud support not covered"
+        return (np.array(blk[:n]), np.array(cell[:n]), np.array(w[:n]),
+                np.array(m1[:3 * n]).reshape(n, 3))
+
+    def leaf_gather(self, vel, x, lev):
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        xs = (C.c_double * 3)(*x)
+        out = (C.c_double * 3)()
+        assert lib().or_leaf_gather(self.h, vel.ctypes.data_as(C.POINTER(C.c_double)), xs,
+                                    int(lev), out) == 0
+        return np.array(out[:])
+
+    def leaf_at(self, x):
+        return lib().or_leaf_at(self.h, (C.c_double * 3)(*x))
 
     def set_solver(self, rtol=1e-11, atol=1e-14, iter_max=200, order_c=0.25, rtol_max=1e-3,
                    tref=1.0, state_floor=1.0):

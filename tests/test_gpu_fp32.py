@@ -182,7 +182,7 @@ def test_fp32_pc2_stiffness_and_damping():
 def test_fp32_pc2_clumped_and_schemes():
     # clumps: tiles of more than one part; NGP and CIC as well as TSC
     for dep in (abi.DK_TSC, abi.DK_CIC, abi.DK_NGP):
-        w = workloads.c4(n=32, nbx=2, ppc=8, clouds=False)
+        w = workloads.c4u(n=32, nbx=2, ppc=8, clouds=False)
         w.phys.deposit = dep
         w.setup.dust_noise = 0.3
         w.name = "fp32-pc2-c4p-dep%d" % dep
@@ -193,12 +193,12 @@ def test_fp32_pc2_clouds_against_the_oracle():
     # refinable clouds at the mesh level (CloudPredictorT / CloudAdvanceT): the TSC weights and
     # the linear gather from the velocity pack, clouds reaching into neighbour blocks and across
     # periodic faces; then clumps with the stiffness field and damping (tiles of several parts)
-    w = workloads.c4(n=16, nbx=2, ppc=2)
+    w = workloads.c4u(n=16, nbx=2, ppc=2)
     w.setup.dust_noise, w.setup.gas_noise, w.dt = 2.0, 0.5, 2e-3
     w.name = "fp32-clouds-cross"
     pc2_check(w, 20)
     pc2_check(w, 5, stage1=True)
-    w = workloads.c4(n=32, nbx=2, ppc=8)
+    w = workloads.c4u(n=32, nbx=2, ppc=8)
     w.phys.stiffness, w.phys.pc2_damping = 1, 1
     w.setup.dust_noise = 0.3
     w.name = "fp32-clouds-clumped"
@@ -206,7 +206,7 @@ def test_fp32_pc2_clouds_against_the_oracle():
 
 
 def test_fp32_pc2_bit_identical_reruns():
-    w = workloads.c4(n=32, nbx=2, ppc=8, clouds=False)
+    w = workloads.c4u(n=32, nbx=2, ppc=8, clouds=False)
     w.setup.dust_noise = 0.5
     outs = []
     for run in range(2):
@@ -322,7 +322,7 @@ def test_fp32_graph_replays_match_eager():
     # each stage captured into a graph and replayed (the executable graph updated every stage),
     # with eager work and host synchronisations between the replays: every step identical to
     # the eager run
-    for w in (workloads.c5(n=32, nbx=2, ppc=4), workloads.c4(n=32, nbx=2, ppc=4, clouds=False)):
+    for w in (workloads.c5(n=32, nbx=2, ppc=4), workloads.c4u(n=32, nbx=2, ppc=4, clouds=False)):
         w.setup.dust_noise = 0.5
         graph_replays(w)
 

@@ -87,7 +87,7 @@ def test_full_size_c5_reruns_identical_conserving_and_near_fp64():
 
 def test_full_size_c4_clouds_against_the_oracle():
     from oracle.pyoracle import Oracle
-    w = workloads.c4()
+    w = workloads.c4u()
     steps = 3
     say("c4: %d particles, oracle %d steps" % (w.npart, steps))
     o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16, threads=16)

@@ -254,14 +254,14 @@ def test_graph_replays_of_the_fp64_form_match_the_oracle():
 def test_clouds_pc2_particles_crossing_blocks():
     # same-level refinable clouds (CloudPredictorT / CloudAdvanceT): linear gathers and cloud
     # weights, clouds reaching into neighbour blocks and across periodic faces
-    w = workloads.c4(n=16, nbx=2, ppc=2)
+    w = workloads.c4u(n=16, nbx=2, ppc=2)
     w.setup.dust_noise, w.setup.gas_noise, w.dt = 2.0, 0.5, 2e-3
     w.name = "c4-clouds-cross"
     run(w, 20, check_every=5)
 
 
 def test_clouds_pc2_with_the_stiffness_field_and_damping():
-    w = workloads.c4(n=16, nbx=2, ppc=4)
+    w = workloads.c4u(n=16, nbx=2, ppc=4)
     w.phys.stiffness, w.phys.pc2_damping = 1, 1
     w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.3, 5e-3
     w.name = "c4-clouds-stiff"

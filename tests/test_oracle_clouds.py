@@ -136,7 +136,7 @@ def test_gather_of_a_quadratic_field_differs_from_the_point_value():
 
 
 def test_cloud_pc2_conserves_momentum_across_blocks():
-    w = workloads.c4(n=16, nbx=2, ppc=2)
+    w = workloads.c4u(n=16, nbx=2, ppc=2)
     w.setup.dust_noise, w.setup.gas_noise, w.dt = 2.0, 0.5, 2e-3
     o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
     o.setup(w.setup)
@@ -151,7 +151,7 @@ def test_cloud_pc2_conserves_momentum_across_blocks():
 
 
 def test_clouds_in_a_uniform_gas_move_as_point_particles():
-    w = workloads.c4(n=16, nbx=2, ppc=2, clump_frac=0.0)
+    w = workloads.c4u(n=16, nbx=2, ppc=2, clump_frac=0.0)
     w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.0, 2e-3
     w.setup.gas_vx, w.setup.gas_vy, w.setup.gas_vz = 0.3, -0.2, 0.1
     outs = []
