@@ -49,7 +49,7 @@ namespace {
 constexpr int kSegSmall = 32;       // lists longer than this are ordered by k_medsort
 constexpr int kMedMax = 512;        // lists longer than this are ordered by k_bigsort
 constexpr int kBigMax = 16384;      // longest list of one cell in the FP32 form
-constexpr int kPartRank = 2048;     // a stayer's rank: part*kPartRank + its rank in the part
+constexpr int kPartRank = 8192;     // a stayer's rank: part*kPartRank + its rank in the part
 constexpr int kKeyThreads = 256;    // the key pass: one particle per thread per chunk (default)
 constexpr int kKeyThreadsMax = 512;
 constexpr int kOverGrid = 4096;     // workgroups of the launch over the extra parts of full tiles
@@ -447,10 +447,18 @@ __device__ inline void LoadP(const Store &S, long long p, PState &s, bool regs, 
 // scratch.
 
 constexpr int kPPT = 2;             // particles per thread in the particle phase
-// a work item (part) is at most kPartChunks chunks of a tile's particles
-constexpr int kPartChunks = 2;
-static_assert(kPartChunks*kPPT*kCellThreads <= kPartRank,
+// a work item (part) is at most PartChunks chunks of a tile's particles.  Without the shearing
+// box the items of multi-part tiles run one workgroup each in their own launch (the crowded-cell
+// split): small parts (2 chunks) spread a clump over many workgroups.  In the shearing box a
+// tile's extra parts run in the looping launch, whose loop-carried state costs registers: large
+// parts (8 chunks) keep the dense midplane tiles of a stratified box one item (C3: 64.1 -> 55.9
+// ms per step; the clumped cases without the shearing box lose with 8: C5 12.0 -> 18.6 ms).  The
+// part size is part of the summation order (a later part's partial sums are added in part
+// order), so it is fixed by the mesh, not tuned.
+constexpr int kPartChunksMax = 8;
+static_assert(kPartChunksMax*kPPT*kCellThreads <= kPartRank,
               "a part's stayer ranks must fit below kPartRank");
+__host__ inline int PartChunks(bool shear) { return shear ? 8 : 2; }
 
 struct TileInfo {
   int m, tx, ty, tz, ci0, cj0, ck0;
@@ -1326,26 +1334,88 @@ __global__ void k_place(const int *key, const int *rank, const int *start, const
 }
 
 // every cell's list in ascending source slot (the stable order); medium and long lists are
-// queued
-__global__ void k_segsort(const int *start, long long nkeys, int *perm, int *big) {
-  GRID_STRIDE(c, nkeys) {
-    const int s0 = start[c], s1 = start[c + 1], n = s1 - s0;
-    if (n <= 1) continue;
-    if (n > kMedMax) {
-      const int slot = atomicAdd(&big[1], 1);
-      big[1 + nkeys - slot] = static_cast<int>(c);
-      continue;
-    }
-    if (n > kSegSmall) {
-      const int slot = atomicAdd(&big[0], 1);
-      big[2 + slot] = static_cast<int>(c);
-      continue;
-    }
-    for (int i=s0+1; i<s1; ++i) {
-      const int v = perm[i];
+// queued.  A workgroup takes kBlock consecutive keys at a time, or, when their lists do not fit
+// in kSegStage slots, the same keys in runs of kSegKeys.  A run whose lists fit is staged in LDS
+// (one coalesced read, one coalesced write) and every element of a list of up to kSegRank is
+// placed by its rank (the count of smaller source slots in its list: the slots are distinct), a
+// thread per element; otherwise each short list is ordered in place in global memory by its
+// key's thread (insertion sort).  Longer lists are queued for the medium and long sorts, with
+// one global atomic per run and class.
+constexpr int kSegKeys = 64;
+constexpr int kSegStage = 4096;
+constexpr int kSegRank = 128;       // longest list a staged run ranks itself
+
+__device__ inline void SegSortRun(const int *start, long long nkeys, int *perm, int *big,
+                                  long long c0, int nk, int *sh, int *out, int *cs, int *qn,
+                                  int *qbase) {
+  const int t = threadIdx.x;
+  const long long c = c0 + t;
+  const long long ce = (c0 + nk < nkeys) ? c0 + nk : nkeys;
+  const int nc = static_cast<int>(ce - c0);
+  const int r0 = start[c0], r1 = start[ce];
+  const bool staged = (r1 - r0) <= kSegStage;   // (the same for the whole workgroup)
+  for (int q = t; q <= nc; q += kBlock) cs[q] = start[c0 + q] - r0;   // (nc + 1 entries)
+  if (t < 2) qn[t] = 0;
+  if (staged)
+    for (int i = r0 + t; i < r1; i += kBlock) sh[i - r0] = perm[i];
+  __syncthreads();
+  const int n = (t < nc) ? cs[t + 1] - cs[t] : 0;
+  // (a staged run ranks its lists of up to kSegRank elements: longer medium ones, and the
+  // medium ones of a run that is not staged, are queued)
+  const int cls = (n > kMedMax) ? 1 : ((n > (staged ? kSegRank : kSegSmall)) ? 0 : -1);
+  const int qs = (cls >= 0) ? atomicAdd(&qn[cls], 1) : 0;
+  __syncthreads();
+  if (t < 2) qbase[t] = qn[t] ? atomicAdd(&big[t], qn[t]) : 0;
+  __syncthreads();
+  if (cls == 1) big[1 + nkeys - (qbase[1] + qs)] = static_cast<int>(c);
+  else if (cls == 0) big[2 + qbase[0] + qs] = static_cast<int>(c);
+  if (!staged && n > 1 && n <= kSegSmall) {
+    int *a = perm + r0 + cs[t];
+    for (int i=1; i<n; ++i) {
+      const int v = a[i];
       int j = i - 1;
-      while (j >= s0 && perm[j] > v) {perm[j + 1] = perm[j]; --j;}
-      perm[j + 1] = v;
+      while (j >= 0 && a[j] > v) {a[j + 1] = a[j]; --j;}
+      a[j + 1] = v;
+    }
+  }
+  if (staged) {
+    const int ne = r1 - r0;
+    for (int e = t; e < ne; e += kBlock) {
+      // the element's key: the last list start at or below it
+      int lo = 0, hi = nc;   // cs[lo] <= e < cs[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cs[mid] <= e) lo = mid; else hi = mid;
+      }
+      const int s0 = cs[lo], s1 = cs[lo + 1], nl = s1 - s0;
+      const int v = sh[e];
+      if (nl <= kSegRank) {
+        int rank = 0;
+        for (int q = s0; q < s1; ++q) rank += (sh[q] < v) ? 1 : 0;
+        out[s0 + rank] = v;
+      } else {
+        out[e] = v;   // (ordered later by the medium or long sort)
+      }
+    }
+    __syncthreads();
+    for (int i = r0 + t; i < r1; i += kBlock) perm[i] = out[i - r0];
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) k_segsort(const int *start, long long nkeys, int *perm,
+                                                    int *big) {
+  __shared__ int sh[kSegStage], out[kSegStage];
+  __shared__ int cs[kBlock + 1];
+  __shared__ int qn[2], qbase[2];   // the run's medium / long lists and their queue base
+  for (long long c0 = (long long)blockIdx.x*kBlock; c0 < nkeys;
+       c0 += (long long)gridDim.x*kBlock) {
+    const long long ce = (c0 + kBlock < nkeys) ? c0 + kBlock : nkeys;
+    if (start[ce] - start[c0] <= kSegStage) {
+      SegSortRun(start, nkeys, perm, big, c0, kBlock, sh, out, cs, qn, qbase);
+    } else {
+      for (long long r = c0; r < ce; r += kSegKeys)
+        SegSortRun(start, nkeys, perm, big, r, kSegKeys, sh, out, cs, qn, qbase);
     }
   }
 }
@@ -2875,7 +2945,7 @@ KArgs MakeArgs(dk_ctx *c) {
   A.regs_live = !f->regs_dead;
   A.res = f->res; A.dc0 = f->dc[0]; A.dc1 = f->dc[1]; A.dc2 = f->dc[2];
   A.item_start = f->item_start; A.item_tile = f->item_tile;
-  A.W = kPartChunks*kPPT*3*f->T.TC;
+  A.W = PartChunks(f->T.SY)*kPPT*3*f->T.TC;
   A.ov_scr = f->ov_scr; A.ov_def = f->ov_def; A.ov_q = f->ov_q; A.ov_dc = f->ov_dc;
   A.ov_pr = f->ov_pr; A.ov_cnt = f->ov_cnt;
   A.x3p = f->x3p;
@@ -2897,8 +2967,8 @@ int SortKeys(dk_ctx *c) {
   TIMED("place", (k_place<<<Grid(f->np), kBlock, 0, c->stream>>>(
       f->key, f->rank, f->start, f->stay, f->np, f->T.TC, f->item_start, f->ov_cnt, f->perm)));
   if ((e = FillV<int>(c, f->big, 2, 0))) return e;
-  TIMED("segsort", (k_segsort<<<Grid(f->nkeys), kBlock, 0, c->stream>>>(f->start, f->nkeys,
-                                                                         f->perm, f->big)));
+  TIMED("segsort", (k_segsort<<<Grid(f->nkeys), kBlock, 0, c->stream>>>(
+                        f->start, f->nkeys, f->perm, f->big)));
   TIMED("medsort", (k_medsort<<<4096, 64, 0, c->stream>>>(f->start, f->perm, f->big)));
   TIMED("bigsort", (k_bigsort<<<256, 1024, 0, c->stream>>>(f->start, f->perm, f->big, f->nkeys,
                                                             c->d_err)));
@@ -2909,7 +2979,7 @@ int SortKeys(dk_ctx *c) {
 int BuildItems(dk_ctx *c) {
   Fast *f = c->fast;
   const Tiles &T = f->T;
-  const int W = kPartChunks*kPPT*3*T.TC;
+  const int W = PartChunks(T.SY)*kPPT*3*T.TC;
   TIMED("items", (k_nparts<<<Grid(T.nt + 1), kBlock, 0, c->stream>>>(T, f->start, W,
                                                                       f->nparts)));
   int e;
@@ -2936,7 +3006,7 @@ int BuildItems(dk_ctx *c) {
 int AllocItems(dk_ctx *c, long long n) {
   Fast *f = c->fast;
   const Tiles &T = f->T;
-  const long long W = (long long)kPartChunks*kPPT*3*T.TC;
+  const long long W = (long long)PartChunks(T.SY)*kPPT*3*T.TC;
   const long long ov = (n + W - 1)/W + 1;
   if (ov <= f->ov_cap && f->item_start) return DK_OK;
   for (void *p : {(void*)f->item_tile, (void*)f->ov_scr, (void*)f->ov_q, (void*)f->ov_pr,
