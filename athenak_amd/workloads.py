@@ -12,7 +12,8 @@ device memory.
       mean 4 per cell), refinable clouds of their block's level, tau_s 0.05, eps 1, pc2
   c4u the root level of c4 alone (64^3 in 16^3 blocks, same-level clouds) with a quarter of the
       particles in 16 Gaussian clumps of width 2 cells; c4p: 128^3 in 512 blocks with point
-      particles
+      particles; c4b: 128^3 in 512 blocks of 16^3 with c4's lognormal block counts, point
+      particles (the uneven-block case in a form both layouts run)
   c5  128^3 periodic in 8 blocks of 64^3, ppc 8, two species (tau_s 0.01 and 1, eps 0.5
       each), half the particles in 64 Gaussian clumps of width 2 cells, imex2+/dc1, 100 steps
 """
@@ -220,4 +221,20 @@ def c4(n=64, nbx=4, ppc=4, steps=50, seed=1, sigma=1.5, regions=C4_REGIONS):
     return w
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c4u": c4u, "c4p": c4p, "c5": c5}
+def c4b(n=128, nbx=8, ppc=4, steps=50, seed=1, sigma=1.5):
+    """Uneven blocks in one launch for both forms: pc2 point particles on 128^3 in 512 blocks of
+    16^3 with the lognormal per-block counts of c4 (sigma 1.5, mean ppc per cell, uniform in each
+    block), tau_s 0.05, eps 1; the densest blocks reach eps ~ 30, so the rate field and the
+    damping of stiff cells are on (as c4p)."""
+    nx = n // nbx
+    mesh, blocks = abi.uniform_mesh((nx, nx, nx), (nbx, nbx, nbx), ((0.0, 1.0),) * 3, ng=2)
+    su = _setup(seed=seed, ppc=ppc, taus=(0.05,), eps=(1.0,), gas_noise=1e-3, dust_noise=1e-3,
+                block_lognormal=1, block_sigma=sigma)
+    w = Workload("c4b", mesh, blocks, _phys(coupling=abi.DK_PC2, solver=abi.DK_LOCAL), su,
+                 None, 1.0e-3, steps)
+    w.phys.stiffness, w.phys.pc2_damping = 1, 1
+    return w
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c4u": c4u, "c4p": c4p, "c4b": c4b,
+             "c5": c5}

@@ -205,6 +205,17 @@ def test_fp32_pc2_clouds_against_the_oracle():
     pc2_check(w, 6)
 
 
+def test_fp32_pc2_uneven_blocks_against_the_oracle():
+    # lognormal particle counts per block (the batching case: blocks of a few particles next to
+    # blocks of thousands, one launch over all of them), point particles and clouds
+    for clouds in (0, 1):
+        w = workloads.c4b(n=32, nbx=4, ppc=4)
+        w.phys.clouds = clouds
+        w.setup.dust_noise = 0.3
+        w.name = "fp32-pc2-uneven-clouds%d" % clouds
+        pc2_check(w, 6)
+
+
 def test_fp32_pc2_bit_identical_reruns():
     w = workloads.c4u(n=32, nbx=2, ppc=8, clouds=False)
     w.setup.dust_noise = 0.5
