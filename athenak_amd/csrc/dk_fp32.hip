@@ -37,6 +37,8 @@
 #include <string>
 
 #include "dk_internal.hpp"
+#include "dk_smr.hpp"
+#include "dk_cloudwalk.hpp"
 
 using dk::DevBlock;
 using dk::Dir;
@@ -131,6 +133,8 @@ struct Fast {
   int pres_rd = 0;                        // the buffer of the last push, read by the next stage
   float *scr_f = nullptr;
   float *cpack = nullptr;           // pc2 clouds: the 12-component gather pack (PackGatherField3D)
+  int *xl_list = nullptr;           // refined meshes: the slots of the cross-level clouds and
+  int *xl_n = nullptr;              // their count
   double *defect = nullptr;    // [nt][3] the FP64 defect of every tile's impulse deposit
   double *x3p = nullptr;       // [item_cap][2] outflow partials of the key pass (outflow x3 faces)
   int pending = 0;
@@ -216,6 +220,12 @@ struct KArgs {
   // them one workgroup each
   const int *ilist, *ilist_n;
   int nfirst;
+  // refined meshes (tree.leaf null otherwise): the routing across levels, every particle's cloud
+  // level by original index, and the list of the store slots whose clouds the tile kernels
+  // leave to the leaf traversal (xl_n: its length)
+  dk::SmrTree tree;
+  const int32_t *plev;
+  int *xl_list, *xl_n;
 };
 
 // the cell (active index, unclamped) of position x along one axis, as PMWeights
@@ -246,8 +256,9 @@ __device__ inline int BlockOff(double x, double lo, double hi, double l) {
 
 // VEL: also the new velocity (the key pass needs the position only).  b0: the particle's block
 // (s.gid) before the push, passed in so that a caller whose particles share one block reads it
-// once for all of them.
-template <bool VEL = true>
+// once for all of them.  SMR: a refined mesh (the routing across levels, a separate
+// instantiation: inlined into the uniform mesh's key pass it would cost registers there).
+template <bool VEL = true, bool SMR = false>
 __device__ inline bool PushMigrate(const KArgs &A, const DevBlock &b0, PState &s,
                                    bool write_regs) {
 #pragma clang fp contract(off)
@@ -346,7 +357,11 @@ __device__ inline bool PushMigrate(const KArgs &A, const DevBlock &b0, PState &s
     s.gid = (cin ? (g.nmbx1 - 1) : 0) + g.nmbx1*(b2 + g.nmbx2*b3);
     return true;
   }
-  const int nb = b.nghbr[Dir(ix, iy, iz)];
+  // (a refined mesh: the neighbour of any level, from the block and the position, s.gid
+  // holding the block the particle leaves)
+  int nb;
+  if constexpr (SMR) nb = dk::SmrDest(A.B, A.tree, s.gid, off, s.x);
+  else nb = b.nghbr[Dir(ix, iy, iz)];
   if (nb < 0) return false;
   s.gid = nb;
   if (lo1) {s.x[0] += lxm; s.x1[0] += lxm;} else if (hi1) {s.x[0] -= lxm; s.x1[0] -= lxm;}
@@ -389,8 +404,16 @@ __device__ inline void Pc2Mid(const KArgs &A, const double x0[3], const float v0
 // the moved position, the same arithmetic as here); x1 <- xh, v1 <- v0 (dust_push.cpp:240-242).
 // The block of the advanced position is found by the next stage's key pass (or an export).
 // Returns false on a routing error (the particle keeps its block); xh: the position keyed.
+// On a refined mesh the particle keeps its block, as the reference's does until the migration
+// after the advance (the routing across levels depends on the block left, so routing through
+// the half-step block could end in another leaf); a half-step position outside the block is
+// keyed into the block's edge cells and its cloud goes to the leaf walk (CrossLevel).
 __device__ inline bool Pc2Route(const KArgs &A, const DevBlock &b, PState &s, double xh[3]) {
   Pc2Mid(A, s.x, s.v, xh);
+  if (A.tree.leaf != nullptr) {
+    for (int d=0; d<3; ++d) {s.x1[d] = xh[d]; s.v1[d] = s.v[d];}
+    return true;
+  }
   const int ix = BlockOff(xh[0], b.x1min, b.x1max, b.x1max - b.x1min);
   const int iy = BlockOff(xh[1], b.x2min, b.x2max, b.x2max - b.x2min);
   const int iz = BlockOff(xh[2], b.x3min, b.x3max, b.x3max - b.x3min);
@@ -1002,6 +1025,35 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
   return L;
 }
 
+// Refined meshes: a cloud that the tile kernels leave to the leaf traversal.  The tile kernels
+// deposit and gather a cloud as the TSC stencil of its block's cells over the tile region
+// (ghost layer included, folded to and filled from same-level neighbours), which is the cloud
+// of half-width one cell of its level when that level is its block's and the cells it reaches
+// beyond the block belong to one leaf of the same level.  Otherwise (its level is not its
+// block's, or it lies in the first or last cell layer of an axis toward a region beside the
+// block that xmask marks as not one same-level leaf, or it lies outside its block) it goes to
+// the leaf traversal.  x: the cloud's centre, (ci, cj, ck): its active cell.
+__device__ inline bool CrossLevel(const Geo &g, const DevBlock &b, int lev, const double x[3],
+                                  int ci, int cj, int ck) {
+  if (lev != b.level) return true;
+  // (a position outside its block, which the routing across levels can leave near a corner)
+  if (!(x[0] >= b.x1min && x[0] < b.x1max && x[1] >= b.x2min && x[1] < b.x2max &&
+        x[2] >= b.x3min && x[2] < b.x3max))
+    return true;
+  if (b.xmask == 0) return false;
+  // the offsets the support can reach per axis: bit (o + 1) for o = -1, 0, 1
+  const int a1 = 2 | ((ci == 0) ? 1 : 0) | ((ci == g.nx1 - 1) ? 4 : 0);
+  const int a2 = 2 | ((cj == 0) ? 1 : 0) | ((cj == g.nx2 - 1) ? 4 : 0);
+  const int a3 = 2 | ((ck == 0) ? 1 : 0) | ((ck == g.nx3 - 1) ? 4 : 0);
+  for (int o3 = 0; o3 < 3; ++o3)
+    for (int o2 = 0; o2 < 3; ++o2)
+      for (int o1 = 0; o1 < 3; ++o1)
+        if (((a1 >> o1) & (a2 >> o2) & (a3 >> o3) & 1) &&
+            ((b.xmask >> (o1 + 3*o2 + 9*o3)) & 1))
+          return true;
+  return false;
+}
+
 //----------------------------------------------------------------------------------------
 // key pass, one pass over the store in its present order: the pending consistent-dust
 // correction of the previous stage (dust_pm.cpp:686-696; the actual change of every particle,
@@ -1012,7 +1064,7 @@ __device__ inline Loc LocateInTile(const Geo &g, const Tiles &Tl, const TileInfo
 // cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
 // only the pending correction is applied (written to v and R).  MID: the pc2 stage-2 form, keyed
 // by the half-step position (Pc2Route) instead of pushed.
-template <bool OV, bool MID = false>
+template <bool OV, bool MID = false, bool SMR = false>
 __global__ void __launch_bounds__(kKeyThreadsMax)
 k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
@@ -1114,7 +1166,8 @@ k_key(KArgs A, int do_key, double *pres) {
           if constexpr (MID) {
             if (!Pc2Route(A, b, s, xh)) dk::RecordError(A.err, DK_EROUTE, q);
           } else {
-            if (!PushMigrate(A, b, s, A.stage1 != 0)) dk::RecordError(A.err, DK_EROUTE, q);
+            if (!PushMigrate<true, SMR>(A, b, s, A.stage1 != 0))
+              dk::RecordError(A.err, DK_EROUTE, q);
           }
           for (int d=0; d<3; ++d) {
             A.src.X(d, q) = s.x[d];
@@ -1972,13 +2025,19 @@ k_pp(KArgs A, const float *uv, float *scr) {
         const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
         const float ts = A.src.TS(p), mass = A.src.M(p);
         const int kp = A.key[p];
+        const int orig = A.src.ORIG(p);
         for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d];}
-        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.ORIG(q) = A.src.ORIG(p);
+        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.ORIG(q) = orig;
         const Loc L = Locate<false>(g, b, x, 0.0);
         float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         float val[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        bool xl = false;   // (refined meshes: a cross-level cloud, deposited by k_fxl_predictor)
+        if constexpr (CLOUD)
+          xl = A.xl_list != nullptr && CrossLevel(g, b, A.plev[orig], x, L.ci, L.cj, L.ck);
         if (TileKey(Tl, t.m, L.ci, L.cj, L.ck) != kp) {
           dk::RecordError(A.err, DK_EHALO, p);
+        } else if (xl) {
+          A.xl_list[atomicAdd(A.xl_n, 1)] = q;
         } else {
           ox = static_cast<float>(L.dx);
           oy = static_cast<float>(L.dy);
@@ -2114,13 +2173,19 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
         float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
         const float ts = A.src.TS(p), mass = A.src.M(p);
         const int kp = A.key[p];
+        const int orig = A.src.ORIG(p);
         double xh[3];
         Pc2Mid(A, x, v, xh);
         const Loc L = Locate<false>(g, b, xh, 0.0);
         float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         double dv[3] = {0.0, 0.0, 0.0};
+        bool xl = false;   // (refined meshes: a cross-level cloud, advanced by k_fxl_advance)
+        if constexpr (CLOUD)
+          xl = A.xl_list != nullptr && CrossLevel(g, b, A.plev[orig], xh, L.ci, L.cj, L.ck);
         if (TileKey(Tl, t.m, L.ci, L.cj, L.ck) != kp) {
           dk::RecordError(A.err, DK_EHALO, p);
+        } else if (xl) {
+          A.xl_list[atomicAdd(A.xl_n, 1)] = q;
         } else {
           ox = static_cast<float>(L.dx);
           oy = static_cast<float>(L.dy);
@@ -2166,7 +2231,7 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
           A.dst.V(d, q) = v[d];
           A.dst.R(d, q) = dv[d];
         }
-        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.ORIG(q) = A.src.ORIG(p);
+        A.dst.TS(q) = ts; A.dst.M(q) = mass; A.dst.ORIG(q) = orig;
         const double fac = -static_cast<double>(mass)*ivol;
         for (int d=0; d<3; ++d) {
           const double imp = fac*dv[d];
@@ -2229,6 +2294,111 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
     }
     if constexpr (!OV) break;
     __syncthreads();   // (the next item reuses LDS)
+  }
+}
+
+// Refined meshes: the cross-level clouds the tile kernels listed (CrossLevel), in the store's
+// new order, walked over the actual leaves of every level (LeafTraversal3D::ForEach,
+// src/dust/dust_leaf3d.hpp:383-517, as the FP64 form walks them) with each leaf cell's own
+// volume; the arithmetic of the particle is the tile kernels'.  Their deposits are atomic
+// additions into the folded fields, so the order of these additions (and the last bits of the
+// cells they touch) is not fixed.  A launch has a thread per particle of the store (up to the
+// grid limit; the threads beyond the list's length exit at once): each cloud's walk is a chain
+// of dependent loads, which only many threads in flight hide.  The walk would take 336 registers
+// (one wavefront per SIMD); a bound of 3 wavefronts per SIMD spills some of them and runs
+// C4's cross-level clouds 12 % faster (2 and 4: 11 % and 8 %).
+
+// LeafApi<LeafTraversal3D>::GatherLinear (src/dust/dust_leaf_api.hpp:202-220) from the FP32
+// 12-component pack of leaf b
+__device__ inline void GatherLinearF(const Geo &g, const DevBlock &b, const dk::CloudGeo &cg,
+                                     const float *gp, int m, int k, int j, int i, double w,
+                                     double m1x, double m1y, double m1z, const double x[3],
+                                     double u[3]) {
+  const double xc = b.x1min + (i - g.is + 0.5)*b.dx1;
+  const double yc = b.x2min + (j - g.js + 0.5)*b.dx2;
+  const double zc = b.x3min + (k - g.ks + 0.5)*b.dx3;
+  const double cx = 0.5*(m1x + dk::MinImage(x[0] - xc, cg.len[0])*w)/b.dx1;
+  const double cy = 0.5*(m1y + dk::MinImage(x[1] - yc, cg.len[1])*w)/b.dx2;
+  const double cz = 0.5*(m1z + dk::MinImage(x[2] - zc, cg.len[2])*w)/b.dx3;
+#pragma unroll
+  for (int c=0; c<3; ++c)
+    u[c] += w*gp[Idx(g, m, 12, c, k, j, i)] + cx*gp[Idx(g, m, 12, 3 + c, k, j, i)] +
+            cy*gp[Idx(g, m, 12, 6 + c, k, j, i)] + cz*gp[Idx(g, m, 12, 9 + c, k, j, i)];
+}
+
+// CloudPredictorT (src/dust/dust_cloud3d.cpp:1369-1437): the gather of the pack at x, the rate
+// and the impulse -dt rate (u - v) into qdep 0 and 1-3 (k_pp's values, per leaf cell volume)
+__global__ void __launch_bounds__(kBlock, 3) k_fxl_predictor(KArgs A, dk::CloudGeo cg,
+                                                          const float *gp, float *qdep) {
+  const Geo &g = A.g;
+  const int n = *A.xl_n;
+  for (int t = blockIdx.x*blockDim.x + threadIdx.x; t < n; t += gridDim.x*blockDim.x) {
+    const int q = A.xl_list[t];
+    const double x[3] = {A.dst.X(0, q), A.dst.X(1, q), A.dst.X(2, q)};
+    const float v[3] = {A.dst.V(0, q), A.dst.V(1, q), A.dst.V(2, q)};
+    const float ts = A.dst.TS(q), mass = A.dst.M(q);
+    const int lev = A.plev[A.dst.ORIG(q)];
+    double ug[3] = {0.0, 0.0, 0.0};
+    const bool ok = dk::SmrCloudForEach<true>(g, A.B, A.tree, cg, x, lev,
+        [&](int b, int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+          GatherLinearF(g, A.B[b], cg, gp, b, k, j, i, w, m1x, m1y, m1z, x, ug);
+        });
+    if (!ok) {dk::RecordError(A.err, DK_EHALO, q); continue;}
+    const double mrate = static_cast<double>(mass)/static_cast<double>(ts);
+    double gi[3];
+    for (int d=0; d<3; ++d) gi[d] = -A.dt*mrate*(static_cast<double>(static_cast<float>(ug[d])) -
+                                                 static_cast<double>(v[d]));
+    dk::SmrCloudForEach<false>(g, A.B, A.tree, cg, x, lev,
+        [&](int b, int k, int j, int i, double w) {
+          const double wv = w*A.B[b].ivol;
+          if (A.rate_dep)
+            unsafeAtomicAdd(&qdep[Idx(g, b, 4, 0, k, j, i)], static_cast<float>(wv*mrate));
+          for (int d=0; d<3; ++d)
+            unsafeAtomicAdd(&qdep[Idx(g, b, 4, 1 + d, k, j, i)], static_cast<float>(wv*gi[d]));
+        });
+  }
+}
+
+// CloudAdvanceT (src/dust/dust_cloud3d.cpp:1449-1536) with CloudPC2Push (:118-158): the gather
+// of u* at the half-step position, k_pa's kick and drift, R and the impulse -m (v - v0) into
+// dmom (FP64, per leaf cell volume)
+__global__ void __launch_bounds__(kBlock, 3) k_fxl_advance(KArgs A, dk::CloudGeo cg,
+                                                        const float *gp, double *dmom) {
+  const Geo &g = A.g;
+  const int n = *A.xl_n;
+  const float hf = static_cast<float>(0.5*A.dt);
+  for (int t = blockIdx.x*blockDim.x + threadIdx.x; t < n; t += gridDim.x*blockDim.x) {
+    const int q = A.xl_list[t];
+    double x[3] = {A.dst.X(0, q), A.dst.X(1, q), A.dst.X(2, q)};
+    float v[3] = {A.dst.V(0, q), A.dst.V(1, q), A.dst.V(2, q)};
+    const float ts = A.dst.TS(q), mass = A.dst.M(q);
+    const int lev = A.plev[A.dst.ORIG(q)];
+    double xh[3];
+    Pc2Mid(A, x, v, xh);
+    double ug[3] = {0.0, 0.0, 0.0};
+    const bool ok = dk::SmrCloudForEach<true>(g, A.B, A.tree, cg, xh, lev,
+        [&](int b, int k, int j, int i, double w, double m1x, double m1y, double m1z) {
+          GatherLinearF(g, A.B[b], cg, gp, b, k, j, i, w, m1x, m1y, m1z, xh, ug);
+        });
+    if (!ok) {dk::RecordError(A.err, DK_EHALO, q); continue;}
+    const float lam = hf/ts;
+    const float aa = 1.0f + lam;
+    double dv[3];
+    for (int d=0; d<3; ++d) {
+      const float vo = v[d];
+      const float vh = (vo + lam*static_cast<float>(ug[d]))/aa;
+      const float vn = 2.0f*vh - vo;
+      x[d] = x[d] + A.dt*static_cast<double>(vh);
+      v[d] = vn;
+      dv[d] = static_cast<double>(vn) - static_cast<double>(vo);
+    }
+    for (int d=0; d<3; ++d) {A.dst.X(d, q) = x[d]; A.dst.V(d, q) = v[d]; A.dst.R(d, q) = dv[d];}
+    const double fac = -static_cast<double>(mass);
+    dk::SmrCloudForEach<false>(g, A.B, A.tree, cg, xh, lev,
+        [&](int b, int k, int j, int i, double w) {
+          const double wv = fac*w*A.B[b].ivol;
+          for (int d=0; d<3; ++d) unsafeAtomicAdd(&dmom[Idx(g, b, 3, d, k, j, i)], wv*dv[d]);
+        });
   }
 }
 
@@ -2641,8 +2811,10 @@ __global__ void k_export(KArgs A, Store S, Store O, const int *perm, const int *
       s.vres[d] = 0.0;
     }
     s.parked = 0;
-    if (migrate && !PushMigrate<false>(A, A.B[s.gid], s, false))
-      dk::RecordError(A.err, DK_EROUTE, q);
+    const bool ok = !migrate ||
+        ((A.tree.leaf != nullptr) ? PushMigrate<false, true>(A, A.B[s.gid], s, false)
+                                  : PushMigrate<false, false>(A, A.B[s.gid], s, false));
+    if (!ok) dk::RecordError(A.err, DK_EROUTE, q);
     for (int d=0; d<3; ++d) {
       r[(long long)(DK_IPX + 2*d)*cap + q] = s.x[d];
       r[(long long)(DK_IPVX + 2*d)*cap + q] = static_cast<double>(s.v[d]);
@@ -2752,6 +2924,8 @@ void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres, bool mid = f
     for (int cls = 0; cls < 2; ++cls) {
       const KArgs Ac = ClassArgs(c, A, cls, sy);
       if (mid) k_key<false, true><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
+      else if (c->smr)
+        k_key<false, false, true><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
       else k_key<false><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
     }
     return;
@@ -2943,6 +3117,11 @@ KArgs MakeArgs(dk_ctx *c) {
   A.pending = f->pending; A.a_prev = f->a_prev;
   A.inv_a_prev = f->pending ? 1.0/f->a_prev : 0.0;
   A.regs_live = !f->regs_dead;
+  A.tree = c->tree;
+  if (c->smr) {
+    A.plev = f->itag + (long long)(DK_PLEV - 1)*f->np;
+    A.xl_list = f->xl_list; A.xl_n = f->xl_n;
+  }
   A.res = f->res; A.dc0 = f->dc[0]; A.dc1 = f->dc[1]; A.dc2 = f->dc[2];
   A.item_start = f->item_start; A.item_tile = f->item_tile;
   A.W = PartChunks(f->T.SY)*kPPT*3*f->T.TC;
@@ -3155,6 +3334,7 @@ int FastCreate(dk_ctx *c) {
   e = e ? e : Alloc((void**)&f->scr_f, sizeof(float)*4*(size_t)T.RN*T.nt);
   e = e ? e : Alloc((void**)&f->defect, sizeof(double)*3*(size_t)T.nt);
   if (cl) e = e ? e : Alloc((void**)&f->cpack, sizeof(float)*12*nc);
+  if (c->smr) e = e ? e : Alloc((void**)&f->xl_n, sizeof(int));
   if (e) return e;
   if (cl) (void)hipMemset(f->cpack, 0, sizeof(float)*12*nc);
   (void)hipMemset(f->qt[0], 0, sizeof(float)*nc);
@@ -3190,7 +3370,8 @@ void FastDestroy(dk_ctx *c) {
   for (void *p : {(void*)f->key, (void*)f->rank, (void*)f->count, (void*)f->start,
                   (void*)f->perm, (void*)f->big, (void*)f->stay, f->scan_tmp, (void*)f->itag, (void*)f->qdep,
                   (void*)f->ustar, (void*)f->ap, (void*)f->res, (void*)f->dmom, (void*)f->pres[0], (void*)f->pres[1],
-                  (void*)f->scr_f, (void*)f->defect, (void*)f->cpack})
+                  (void*)f->scr_f, (void*)f->defect, (void*)f->cpack, (void*)f->xl_list,
+                  (void*)f->xl_n})
     if (p) (void)hipFree(p);
   for (int q=0; q<3; ++q) {
     if (f->qt[q]) (void)hipFree(f->qt[q]);
@@ -3222,6 +3403,10 @@ int FastBind(dk_ctx *c) {
     for (void **p : {(void**)&f->key, (void**)&f->rank, (void**)&f->perm}) {
       if (*p) (void)hipFree(*p);
       if (Alloc(p, sizeof(int)*n)) return DK_ENOMEM;
+    }
+    if (c->smr) {
+      if (f->xl_list) (void)hipFree(f->xl_list);
+      if (Alloc((void**)&f->xl_list, sizeof(int)*n)) return DK_ENOMEM;
     }
     if (f->itag) (void)hipFree(f->itag);
     if (Alloc((void**)&f->itag, sizeof(int32_t)*(DK_NIDATA - 1)*n)) return DK_ENOMEM;
@@ -3413,11 +3598,15 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
       A, 1, f->pres[1 - f->pres_rd])));
   if ((e = SortKeys(c))) return e;
   if ((e = BuildItems(c))) return e;
+  if (c->smr && (e = FillV<int>(c, f->xl_n, 1, 0))) return e;
   if (s1) {
     TIMED("pc2_predictor", LaunchPP(c, A, gf, f->scr_f, cl));
     TIMED("combine", (k_combine<float, 4><<<CombineGrid(c), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, f->qdep, nullptr, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<float>(c, f->qdep, 4, st->time))) return e;
+    if (c->smr)
+      TIMED("cloud_xlevel", (k_fxl_predictor<<<Grid(f->np), kBlock, 0, c->stream>>>(
+          A, dk::MakeCloudGeo(c), gf, f->qdep)));
     if (dk::StiffnessActive(c->P) && (e = StiffnessRecord<float>(c, f->qdep, 4, 0, st->dt))) return e;
     TIMED("pc2_apply", (k_pc2_apply<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->u0, f->qdep, c->P.pc2_damping, st->dt)));
@@ -3426,6 +3615,9 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     TIMED("combine", (k_combine<float, 3, double><<<CombineGrid(c), kBlock, 0, c->stream>>>(
         g, T, f->scr_f, f->dmom, f->defect, f->item_start, f->ov_scr, f->ov_def)));
     if ((e = FoldT<double>(c, f->dmom, 3, st->time))) return e;
+    if (c->smr)
+      TIMED("cloud_xlevel", (k_fxl_advance<<<Grid(f->np), kBlock, 0, c->stream>>>(
+          A, dk::MakeCloudGeo(c), gf, f->dmom)));
     TIMED("pc2_commit", (k_pc2_commit<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->u0, f->dmom, f->qdep, st->gam1[st->nexp_stages - 1])));
   }

@@ -1288,12 +1288,12 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
       mesh->nmbx1 < 1 || mesh->nmbx2 < 1 || mesh->nmbx3 < 1)
     return DK_EINVAL;
   if (form != DK_FORM_FP64 && form != DK_FORM_FP32) return DK_EINVAL;
-  // a statically refined mesh: refinable clouds with pc2 in the plain FP64 form (their
-  // traversal of the actual leaves; point-particle deposits would need the cross-level fold)
+  // a statically refined mesh: refinable clouds with pc2 (their traversal of the actual leaves;
+  // point-particle deposits would need the cross-level fold)
   int maxlev = 0;
   for (int m=0; m<mesh->nmb; ++m) maxlev = std::max(maxlev, blocks[m].level);
   const bool smr = maxlev > 0 || mesh->nmb != mesh->nmbx1*mesh->nmbx2*mesh->nmbx3;
-  if (smr && (form != DK_FORM_FP64 || !phys->clouds)) return DK_EUNSUPPORTED;
+  if (smr && !phys->clouds) return DK_EUNSUPPORTED;
   if (maxlev > 20) return DK_EINVAL;
   // the mesh passes launch one grid row per block (at most 65535), and index a block's padded
   // cells in 32 bits
@@ -1342,6 +1342,7 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     d.ivol = 1.0/(d.dx1*d.dx2*d.dx3);
     d.lx1 = b.lx1; d.lx2 = b.lx2; d.lx3 = b.lx3;
     d.level = b.level;
+    d.xmask = 0;
     {
       const double lx = b.x1max - b.x1min, ly = b.x2max - b.x2min, lz = b.x3max - b.x3min;
       d.face = ((b.x1min < mesh->x1min + 0.5*lx) ? 1 : 0) | ((b.x1max > mesh->x1max - 0.5*lx) ? 2 : 0) |
@@ -1428,6 +1429,28 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     }
     for (size_t q=0; q<leaf.size() && !e; ++q)
       if (leaf[q] < 0) e = DK_EINVAL;
+    // per block, the directions whose same-level region is not one leaf of its level (a
+    // region aligned at the block's level holds one leaf of that level exactly when the leaf
+    // at its first finest slot is of that level), and the same-level neighbours from the leaf
+    // map (the FP32 form's fold reads them; the caller's nghbr[] is not used on a refined mesh)
+    for (int m=0; m<mesh->nmb && !e; ++m) {
+      DevBlock &d = hb[m];
+      const int r = 1 << (maxlev - d.level);
+      const int l[3] = {d.lx1, d.lx2, d.lx3};
+      for (int o3=-1; o3<=1; ++o3)
+        for (int o2=-1; o2<=1; ++o2)
+          for (int o1=-1; o1<=1; ++o1) {
+            if (o1 == 0 && o2 == 0 && o3 == 0) continue;
+            const int o[3] = {o1, o2, o3};
+            int s[3];
+            for (int a=0; a<3; ++a) s[a] = (((l[a] + o[a])*r) % F[a] + F[a]) % F[a];
+            const int n = leaf[((size_t)s[2]*F[1] + s[1])*F[0] + s[0]];
+            if (blocks[n].level != d.level) d.xmask |= 1 << Dir(o1, o2, o3);
+            d.nghbr[Dir(o1, o2, o3)] = (blocks[n].level == d.level) ? n : -1;
+          }
+    }
+    if (!e) e = dk::Check(hipMemcpy(c->d_blocks, hb.data(), sizeof(DevBlock)*mesh->nmb,
+                                    hipMemcpyHostToDevice));
     alloc((void**)&c->d_leafmap, sizeof(int)*leaf.size());
     if (!e) e = dk::Check(hipMemcpy(c->d_leafmap, leaf.data(), sizeof(int)*leaf.size(),
                                     hipMemcpyHostToDevice));

@@ -35,6 +35,8 @@ struct DevBlock {
   int level;               // refinement level (0 on a uniform mesh)
   int face;                // bits 1,2 / 4,8 / 16,32: lower/upper face within half a block of
                            // the mesh face in x1 / x2 / x3 (the test of bvals_part.cpp)
+  int xmask;               // refined meshes: bit Dir(o) set when the same-level region beside
+                           // the block in direction o is not one leaf of the block's level
   int nghbr[27];
 };
 

@@ -316,7 +316,9 @@ __global__ void k_fold(Geo g, const DevBlock *B, const int *slot_dir, T *a, int 
       const int o1 = d % 3, o2 = (d/3) % 3, o3 = d/9;
       if (!((a1 >> o1) & 1) || !((a2 >> o2) & 1) || !((a3 >> o3) & 1)) continue;
       const int nb = bm.nghbr[d];
-      if (nb < 0) continue;
+      // (on a refined mesh only same-level neighbours fold: no deposit of the FP32 tile
+      // kernels reaches a region of another level, whose clouds go to the leaf traversal)
+      if (nb < 0 || B[nb].level != bm.level) continue;
       if (shear && o1 != 1) {
         const bool crosses = (o1 == 0) ? (bm.lx1 == 0) : (bm.lx1 == g.nmbx1 - 1);
         if (crosses) continue;

@@ -61,8 +61,9 @@ enum {
                          atomic-free deposits, fused stage passes (dk_stage_run only).
                          imex with local/dc1/dc2 on any mesh of the FP64 form; pc2 with
                          point particles or clouds at the mesh level on periodic meshes
-                         without the shearing box; otherwise dk_create returns
-                         DK_EUNSUPPORTED */
+                         without the shearing box, clouds on statically refined periodic
+                         meshes (atomic deposits for the clouds near a level change);
+                         otherwise dk_create returns DK_EUNSUPPORTED */
 };
 
 /* error codes */
@@ -87,7 +88,9 @@ enum {
 /* A MeshBlock: its box (RegionSize, src/mesh/mesh.hpp:25-29), logical location and level,
  * and the gid of the same-level neighbour in each direction
  * d = (ox1+1) + 3*(ox2+1) + 9*(ox3+1) (d = 13 is the block itself; -1 = none, a physical
- * face).  Replaces MeshBlock::mb_size/nghbr (src/mesh/meshblock.hpp). */
+ * face).  On a statically refined mesh (any level > 0) nghbr is not read: the library finds
+ * neighbours of every level from the levels and logical locations.  Replaces
+ * MeshBlock::mb_size/nghbr (src/mesh/meshblock.hpp). */
 typedef struct dk_block {
   double x1min, x1max, x2min, x2max, x3min, x3max;
   int32_t lx1, lx2, lx3, level;
@@ -125,8 +128,9 @@ typedef struct dk_physics {
                                (StiffnessActive, src/dust/dust.hpp:1150-1152) */
   int32_t clouds;           /* pc2 with 3-D refinable clouds at the mesh level (plain codes:
                                CloudPredictorT / CloudAdvanceT of dust_cloud3d.cpp:1369-1536);
-                               same-level meshes periodic in x1, x2, x3 only (DK_EUNSUPPORTED
-                               otherwise); 0: point particles */
+                               meshes periodic in x1, x2, x3 only, same-level or statically
+                               refined (DK_EUNSUPPORTED otherwise); 0: point particles (refused
+                               on refined meshes) */
 } dk_physics;
 
 /* What a task read from Driver and Mesh for one stage. */

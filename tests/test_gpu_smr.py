@@ -1,11 +1,12 @@
 """The uneven-block case on a statically refined mesh (pc2 with refinable clouds of their block's
-level, lognormal particle counts per block) in the plain FP64 form of the HIP library, against
-the oracle on the same seeded inputs.
+level, lognormal particle counts per block) in both forms of the HIP library, against the oracle
+on the same seeded inputs.
 
-Tolerances as for the other FP64-form parity tests: velocities and gas momenta 1e-11 of their
-maxima, positions 1e-12 of the box; block ids identical (integer work).  Both sides run the
-reference's arithmetic in its order without fused multiply-adds and differ only in the order of
-the atomic deposit additions."""
+Plain FP64 form: the tolerances of the other FP64-form parity tests, velocities and gas momenta
+1e-11 of their maxima, positions 1e-12 of the box; block ids identical (integer work).  Both
+sides run the reference's arithmetic in its order without fused multiply-adds and differ only in
+the order of the atomic deposit additions.  FP32 form: the tolerances of the other FP32 pc2
+tests (1e-5)."""
 
 import sys
 
@@ -28,7 +29,9 @@ def say(msg):
     sys.stdout.flush()
 
 
-def small_case(back_reaction=1, damping=True):
+def small_case(back_reaction=1, damping=True, sigma=1.5):
+    # (sigma 1.5: the densest blocks reach eps ~ 90, chi = dt eps/t_s ~ 18, where the damped
+    # pc2 cycle is violent and amplifies round-off; the FP32 comparisons take sigma 0.5)
     mesh, blocks = abi.smr_mesh((8, 8, 8), (2, 2, 4), BOX, REGIONS)
     phys = workloads._phys(coupling=abi.DK_PC2, solver=abi.DK_LOCAL, clouds=1,
                            back_reaction=back_reaction)
@@ -36,7 +39,7 @@ def small_case(back_reaction=1, damping=True):
         phys.stiffness, phys.pc2_damping = 1, 1
     su = workloads._setup(seed=4, ppc=2, taus=(0.05,), eps=(1.0,), gas_noise=1e-2,
                           dust_noise=1e-2, gas_vz=0.6, gas_vx=0.3, block_lognormal=1,
-                          block_sigma=1.5)
+                          block_sigma=sigma)
     return mesh, blocks, phys, su
 
 
@@ -110,14 +113,63 @@ def test_pc2_clouds_on_the_refined_mesh_match_the_oracle():
 
 
 def test_refined_meshes_are_refused_where_unsupported():
-    """The FP32 form and point particles on a refined mesh are refused at creation."""
+    """Point particles on a refined mesh are refused at creation, in both forms."""
     from athenak_amd.lib import DustContext, DustError
     mesh, blocks, phys, su = small_case()
-    with pytest.raises(DustError, match="option not provided"):
-        DustContext(mesh, blocks, phys, 64, form=abi.DK_FORM_FP32)
     phys.clouds = 0
-    with pytest.raises(DustError, match="option not provided"):
-        DustContext(mesh, blocks, phys, 64, form=abi.DK_FORM_FP64)
+    for form in (abi.DK_FORM_FP64, abi.DK_FORM_FP32):
+        with pytest.raises(DustError, match="option not provided"):
+            DustContext(mesh, blocks, phys, 64, form=form)
+
+
+def test_fp32_pc2_clouds_on_the_refined_mesh_match_the_oracle():
+    """The FP32 form on the refined mesh: same-level clouds through the tile kernels, the
+    clouds of another level than their block's or near a level change through the leaf walk;
+    the FP32 tolerances of the other pc2 tests (1e-5), block ids identical, momentum 1e-12."""
+    from tests.test_gpu_fp32 import pc2_check
+    mesh, blocks, phys, su = small_case(sigma=0.5)
+    n = workloads.Workload("t", mesh, blocks, phys, su, None, 0.01, 1).npart
+    assert n > 1000
+    w = workloads.Workload("fp32-smr", mesh, blocks, phys, su, None, 0.01, 10)
+    say("fp32 on the refined mesh: %d particles" % n)
+    pc2_check(w, 10)
+    pc2_check(w, 3, stage1=True)
+
+
+def test_full_size_c4_fp32_against_the_fp64_form():
+    """The uneven-block case at its size in both forms, 3 steps: velocities 1e-5 of their
+    maximum, positions 1e-5 of the distance travelled, block ids identical, momentum 1e-12."""
+    import torch
+    from athenak_amd.lib import DustContext
+    w = workloads.c4()
+    say("c4 fp32 vs fp64: %d blocks, %d particles" % (w.mesh.nmb, w.npart))
+    out = []
+    for form in (abi.DK_FORM_FP64, abi.DK_FORM_FP32):
+        g = DustContext(w.mesh, w.blocks, w.phys, w.npart + 16, form=form,
+                        stream=torch.cuda.Stream())
+        g.setup(w.setup)
+        d0 = g.diag()
+        for s in range(3):
+            say("c4 form %d step %d" % (form, s))
+            g.step(w.dt)
+            g.check_device_error()
+        d1 = g.diag()
+        drift = np.abs((d1[0:3] + d1[3:6]) - (d0[0:3] + d0[3:6])).max() / d0[8]
+        assert drift < 1e-12, (form, drift)
+        if form == abi.DK_FORM_FP32:
+            g.export()
+        g.stream.synchronize()
+        out.append(by_tag(g.rdata.cpu().numpy(), g.idata.cpu().numpy(), g.npart))
+        g.close()
+    (r64, i64), (r32, i32) = out
+    np.testing.assert_array_equal(i64[abi.PGID], i32[abi.PGID])
+    v = [abi.IPVX, abi.IPVY, abi.IPVZ]
+    vmax = np.abs(r64[v]).max()
+    ev = max(np.abs(r64[c] - r32[c]).max() for c in v) / vmax
+    ex = max(np.abs(r64[c] - r32[c]).max() for c in (abi.IPX, abi.IPY, abi.IPZ)) / (
+        vmax * w.dt * 3)
+    say("c4 fp32 vs fp64: v %.3e, x %.3e" % (ev, ex))
+    assert ev < 1e-5 and ex < 1e-5, (ev, ex)
 
 
 def test_full_size_c4_conserves_and_matches_the_oracle():
@@ -142,3 +194,39 @@ def test_full_size_c4_conserves_and_matches_the_oracle():
     d1 = g.diag()
     assert np.abs((d1[0:3] + d1[3:6]) - (d0[0:3] + d0[3:6])).max() / d0[8] < 1e-12
     g.close()
+
+
+def test_refined_mesh_stages_run_as_graphs():
+    """Both forms capture each stage of the refined-mesh case into one graph and replay it: the
+    replays match the same steps run launch by launch (FP64 form to 1e-11 of the maxima: only
+    the order of the atomic deposits differs; FP32 form to 1e-5: its cross-level deposits are
+    atomic too)."""
+    import torch
+    from athenak_amd.lib import DustContext
+    mesh, blocks, phys, su = small_case(sigma=0.5)
+    cap = workloads.Workload("t", mesh, blocks, phys, su, None, 0.01, 1).npart + 16
+    for form, tol in ((abi.DK_FORM_FP64, 1e-11), (abi.DK_FORM_FP32, 1e-5)):
+        res = []
+        for graphs in (False, True):
+            g = DustContext(mesh, blocks, phys, cap, form=form, stream=torch.cuda.Stream())
+            g.setup(su)
+            g.set_graphs(graphs)
+            for s in range(6):
+                g.step(0.01)
+                g.stream.synchronize()   # (eager host work between the replays)
+            g.check_device_error()
+            if graphs:
+                assert g.graphs_captured() >= 12
+            if form == abi.DK_FORM_FP32:
+                g.export()
+            g.stream.synchronize()
+            r, i = by_tag(g.rdata.cpu().numpy(), g.idata.cpu().numpy(), g.npart)
+            res.append((r, i, g.u0.cpu().numpy()))
+            g.close()
+        (ra, ia, ua), (rb, ib, ub) = res
+        np.testing.assert_array_equal(ia, ib)
+        v = [abi.IPVX, abi.IPVY, abi.IPVZ]
+        ev = np.abs(ra[v] - rb[v]).max() / np.abs(ra[v]).max()
+        eu = np.abs(ua[:, 1:] - ub[:, 1:]).max() / np.abs(ua[:, 1:]).max()
+        say("graphs form %d: v %.3e, gas %.3e" % (form, ev, eu))
+        assert ev < tol and eu < tol, (form, ev, eu)
