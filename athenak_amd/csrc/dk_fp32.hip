@@ -941,10 +941,23 @@ __host__ __device__ inline int RecArea(const Tiles &Tl, int ch, int nrec, int nc
   return (nrec*ch > 3*nc*Tl.RN) ? nrec*ch : 3*nc*Tl.RN;
 }
 
+// The key pass's record of a particle: the correction's m delta/V by component, the push
+// residue m vres/V, the centre row + 2 (0: none); 48 bytes, read as three 16-byte words.
+struct KeyRec {
+  double dv[3];
+  float pr[3];
+  int code, pad;
+};
+static_assert(sizeof(KeyRec) == 48, "KeyRec is three 16-byte words");
+
+// Every cell's record sums are split by the parity of the record's slot within the cell (two
+// half sums, added at the end of the work item): two threads per cell sum a chunk's records,
+// and the order of every sum is fixed whatever the chunk size.
+constexpr int kKeyHalves = 2;
+
 // the key pass's LDS before its per-cell sums: records, cell offsets, stayer counts
 __host__ __device__ inline size_t LdsKeyBase(const Tiles &T, int threads) {
-  return (size_t)threads*(3*sizeof(double) + 3*sizeof(float) + sizeof(int)) +
-         sizeof(int)*(2*T.TC + 1);
+  return (size_t)threads*sizeof(KeyRec) + sizeof(int)*(2*T.TC + 1);
 }
 
 // the overflow slot of part >= 1 of a tile
@@ -1054,6 +1067,56 @@ __device__ inline bool CrossLevel(const Geo &g, const DevBlock &b, int lev, cons
   return false;
 }
 
+// What the key pass reads of one particle (store slot q): position, velocity, rate (when the
+// push or the pending correction uses it), the stage registers x1, v1 (at stage 2 of imex: where
+// the stage-1 key pass wrote them, the other store at the slot of the last sort's permutation),
+// mass and, with a pending correction, stopping time, the slot's tile-local cell and the
+// residual rows of that cell and the cells beside it in x2 (the centre is one of them).
+struct KeyIn {
+  double x[3], x1[3], r[3];
+  float v[3], v1[3];
+  float mass, ts;
+  float rrow[3][3];
+  int cell;
+};
+
+template <bool MID>
+__device__ inline void KeyFetch(const KArgs &A, const TileInfo &t, const int *st, int q,
+                                int do_key, KeyIn &k) {
+  const Geo &g = A.g;
+  const Tiles &Tl = A.T;
+  const bool rates = A.atw_dt != 0.0 || !do_key;
+  k.cell = 0;
+  k.ts = 0.0f;
+#pragma unroll
+  for (int r=0; r<3; ++r)
+#pragma unroll
+    for (int d=0; d<3; ++d) k.rrow[r][d] = 0.0f;
+  if (A.pending) {
+    k.cell = SlotCell(st, Tl.TC, q);
+    int ci, cj, ck;
+    CellIdx(Tl, t, k.cell, ci, cj, ck);
+    const long long i0 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+#pragma unroll
+    for (int r=0; r<3; ++r)
+#pragma unroll
+      for (int d=0; d<3; ++d) k.rrow[r][d] = A.res[i0 + (r - 1)*g.n1 + d*g.ncb];
+    k.ts = A.src.TS(q);
+  }
+  for (int d=0; d<3; ++d) {
+    k.x[d] = A.src.X(d, q);
+    k.v[d] = A.src.V(d, q);
+    k.r[d] = rates ? A.src.R(d, q) : 0.0;
+    k.x1[d] = 0.0;
+    k.v1[d] = 0.0f;
+  }
+  if (!MID && do_key && !A.stage1) {
+    const int pq = A.perm[q];
+    for (int d=0; d<3; ++d) {k.x1[d] = A.dst.X1(d, pq); k.v1[d] = A.dst.V1(d, pq);}
+  }
+  k.mass = A.src.M(q);
+}
+
 //----------------------------------------------------------------------------------------
 // key pass, one pass over the store in its present order: the pending consistent-dust
 // correction of the previous stage (dust_pm.cpp:686-696; the actual change of every particle,
@@ -1081,18 +1144,17 @@ k_key(KArgs A, int do_key, double *pres) {
     const int part = it - A.item_start[tile];
     const TileInfo t = TileOf(Tl, tile);
     const int tid = threadIdx.x, CH = blockDim.x;
-    double *rdc = lds_d;                                   // [3][CH] m delta/V
-    float *rpr = reinterpret_cast<float*>(rdc + 3*CH);     // [3][CH] m vres/V
-    int *rcode = reinterpret_cast<int*>(rpr + 3*CH);       // [CH] centre row + 2 (0: none)
-    int *st = rcode + CH;                                  // [TC + 1]
+    KeyRec *rec = reinterpret_cast<KeyRec*>(lds_d);        // [CH]
+    double *rdc = lds_d;                                   // (after the chunks: wave sums)
+    int *st = reinterpret_cast<int*>(rec + CH);            // [TC + 1]
     int *lcnt = st + Tl.TC + 1;                            // [TC]
-    // every cell's sums (the correction by centre row, the push residue) in LDS slots that only
-    // the cell's thread touches: they stay out of registers during the particle phase
-    double *cacc = reinterpret_cast<double*>(lds_d) + (LdsKeyBase(Tl, CH) + 7)/8;   // [9][TC]
-    float *pacc = reinterpret_cast<float*>(cacc + 9*Tl.TC);                   // [3][TC]
-    if (tid < Tl.TC) {
-      for (int k=0; k<9; ++k) cacc[k*Tl.TC + tid] = 0.0;
-      for (int k=0; k<3; ++k) pacc[k*Tl.TC + tid] = 0.0f;
+    // every cell's half sums (the correction by centre row, the push residue) in LDS slots
+    // that only one thread touches: they stay out of registers during the particle phase
+    double *cacc = reinterpret_cast<double*>(lds_d) + (LdsKeyBase(Tl, CH) + 7)/8;   // [2][9][TC]
+    float *pacc = reinterpret_cast<float*>(cacc + kKeyHalves*9*Tl.TC);        // [2][3][TC]
+    for (int i = tid; i < kKeyHalves*Tl.TC; i += CH) {
+      for (int k=0; k<9; ++k) cacc[k*Tl.TC + (i/Tl.TC)*9*Tl.TC + i % Tl.TC] = 0.0;
+      for (int k=0; k<3; ++k) pacc[k*Tl.TC + (i/Tl.TC)*3*Tl.TC + i % Tl.TC] = 0.0f;
     }
     for (int i = tid; i <= Tl.TC; i += CH) st[i] = A.start[t.key0 + i];
     for (int i = tid; i < Tl.TC; i += CH) lcnt[i] = 0;
@@ -1103,32 +1165,26 @@ k_key(KArgs A, int do_key, double *pres) {
     const double ivol = b.ivol;
     double pmass = 0.0;   // (outflow faces: the mass and count this thread parked, chunk order)
     int pcnt = 0;
+    // every particle's inputs are fetched one chunk ahead: the next chunk's loads are in flight
+    // during this chunk's per-cell sums and barriers
+    KeyIn nx;
+    if (P0 + tid < P1) KeyFetch<MID>(A, t, st, P0 + tid, do_key, nx);
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
       const int q = c0 + tid;
       if (q < c1) {
-        // the correction's residual rows (the slot's cell and the cells beside it in x2),
-        // fetched alongside the particle: the centre is one of them
+        const KeyIn in = nx;
+        // the slot's cell (its residual rows: this cell and the cells beside it in x2)
         int ci = 0, cj = 0, ck = 0;
-        float rrow[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-        if (A.pending) {
-          CellIdx(Tl, t, SlotCell(st, Tl.TC, q), ci, cj, ck);
-          const long long i0 = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
-#pragma unroll
-          for (int r=0; r<3; ++r)
-#pragma unroll
-            for (int d=0; d<3; ++d) rrow[r][d] = A.res[i0 + (r - 1)*g.n1 + d*g.ncb];
-        }
+        if (A.pending) CellIdx(Tl, t, in.cell, ci, cj, ck);
         PState s;
-        LoadP(A.src, q, s, false, A.atw_dt != 0.0 || !do_key);
-        if (!MID && do_key && !A.stage1) {
-          // the stage registers x1, v1 where the stage-1 key pass wrote them: the store before
-          // the last sort (the other store), at the slot the permutation gives
-          const int pq = A.perm[q];
-          for (int d=0; d<3; ++d) {s.x1[d] = A.dst.X1(d, pq); s.v1[d] = A.dst.V1(d, pq);}
+        for (int d=0; d<3; ++d) {
+          s.x[d] = in.x[d]; s.v[d] = in.v[d]; s.x1[d] = in.x1[d]; s.v1[d] = in.v1[d];
+          s.r[d] = in.r[d]; s.vres[d] = 0.0;
         }
+        s.parked = 0;
         s.gid = t.m;   // (the tile's particles are in its block: the store is tile-major)
-        const float mass = A.src.M(q);
+        const float mass = in.mass;
         int code = 0;
         double dv[3] = {0.0, 0.0, 0.0};
         if (A.pending) {
@@ -1139,10 +1195,11 @@ k_key(KArgs A, int do_key, double *pres) {
               dk::RecordError(A.err, DK_EHALO, q);
             } else {
               const float apf = static_cast<float>(A.a_prev);
-              const float cjf = apf/(A.src.TS(q) + apf);
+              const float cjf = apf/(in.ts + apf);
               for (int d=0; d<3; ++d) {
                 const float vo = s.v[d];
-                const float rd = (dj < 0) ? rrow[0][d] : ((dj > 0) ? rrow[2][d] : rrow[1][d]);
+                const float rd = (dj < 0) ? in.rrow[0][d]
+                                          : ((dj > 0) ? in.rrow[2][d] : in.rrow[1][d]);
                 const float vn = vo + cjf*rd;
                 s.v[d] = vn;
                 // the recorded rate takes the actual change, so that the history terms of dust
@@ -1198,35 +1255,49 @@ k_key(KArgs A, int do_key, double *pres) {
           A.rank[q] = (lk >= 0 && lk < Tl.TC) ? part*kPartRank + atomicAdd(&lcnt[lk], 1)
                                               : -1 - atomicAdd(&A.count[key], 1);
         }
-        for (int d=0; d<3; ++d) {rdc[d*CH + tid] = dv[d]; rpr[d*CH + tid] = pr[d];}
-        rcode[tid] = code;
+        KeyRec kr;
+        for (int d=0; d<3; ++d) {kr.dv[d] = dv[d]; kr.pr[d] = pr[d];}
+        kr.code = code;
+        kr.pad = 0;
+        rec[tid] = kr;
       }
+      // (the next chunk's slots: no particle of this chunk is written there)
+      if (q + CH < P1) KeyFetch<MID>(A, t, st, q + CH, do_key, nx);
       __syncthreads();
-      if (A.cell_sums && tid < Tl.TC) {
-        double dcs[3][3];
-        float prs[3];
-        for (int k=0; k<9; ++k) dcs[k/3][k%3] = cacc[k*Tl.TC + tid];
-        for (int k=0; k<3; ++k) prs[k] = pacc[k*Tl.TC + tid];
-        const int s0 = max(st[tid], c0), s1 = min(st[tid + 1], c1);
-        for (int qq = s0; qq < s1; ++qq) {
-          const int r = qq - c0;
-          const int code = rcode[r];
-          const double d0 = rdc[r], d1 = rdc[CH + r], d2 = rdc[2*CH + r];
-          if (code == 1) {dcs[0][0] += d0; dcs[0][1] += d1; dcs[0][2] += d2;}
-          else if (code == 2) {dcs[1][0] += d0; dcs[1][1] += d1; dcs[1][2] += d2;}
-          else if (code == 3) {dcs[2][0] += d0; dcs[2][1] += d1; dcs[2][2] += d2;}
-          prs[0] += rpr[r]; prs[1] += rpr[CH + r]; prs[2] += rpr[2*CH + r];
+      if (A.cell_sums) {
+        // (cell l, half h): the records of l in this chunk whose slot has the parity h within
+        // the cell, in slot order
+        for (int i = tid; i < kKeyHalves*Tl.TC; i += CH) {
+          const int l = i % Tl.TC, h = i/Tl.TC;
+          double *ca = cacc + h*9*Tl.TC + l;
+          float *pa = pacc + h*3*Tl.TC + l;
+          double dcs[3][3];
+          float prs[3];
+          for (int k=0; k<9; ++k) dcs[k/3][k%3] = ca[k*Tl.TC];
+          for (int k=0; k<3; ++k) prs[k] = pa[k*Tl.TC];
+          const int a = st[l];
+          int s0 = max(a, c0);
+          s0 += (s0 - a + h) & 1;
+          const int s1 = min(st[l + 1], c1);
+          for (int qq = s0; qq < s1; qq += 2) {
+            const KeyRec kr = rec[qq - c0];
+            const int code = kr.code;
+            if (code == 1) {dcs[0][0] += kr.dv[0]; dcs[0][1] += kr.dv[1]; dcs[0][2] += kr.dv[2];}
+            else if (code == 2) {dcs[1][0] += kr.dv[0]; dcs[1][1] += kr.dv[1]; dcs[1][2] += kr.dv[2];}
+            else if (code == 3) {dcs[2][0] += kr.dv[0]; dcs[2][1] += kr.dv[1]; dcs[2][2] += kr.dv[2];}
+            prs[0] += kr.pr[0]; prs[1] += kr.pr[1]; prs[2] += kr.pr[2];
+          }
+          for (int k=0; k<9; ++k) ca[k*Tl.TC] = dcs[k/3][k%3];
+          for (int k=0; k<3; ++k) pa[k*Tl.TC] = prs[k];
         }
-        for (int k=0; k<9; ++k) cacc[k*Tl.TC + tid] = dcs[k/3][k%3];
-        for (int k=0; k<3; ++k) pacc[k*Tl.TC + tid] = prs[k];
       }
       __syncthreads();
     }
     double dcs[3][3];
     float prs[3];
-    if (tid < Tl.TC) {
-      for (int k=0; k<9; ++k) dcs[k/3][k%3] = cacc[k*Tl.TC + tid];
-      for (int k=0; k<3; ++k) prs[k] = pacc[k*Tl.TC + tid];
+    if (tid < Tl.TC) {   // (the two half sums in order)
+      for (int k=0; k<9; ++k) dcs[k/3][k%3] = cacc[k*Tl.TC + tid] + cacc[(9 + k)*Tl.TC + tid];
+      for (int k=0; k<3; ++k) prs[k] = pacc[k*Tl.TC + tid] + pacc[(3 + k)*Tl.TC + tid];
     }
     if (do_key && part == 0)
       for (int i = tid; i < Tl.TC; i += CH) A.stay[t.key0 + i] = lcnt[i];
@@ -2886,7 +2957,8 @@ int FillV(dk_ctx *c, T *a, long long n, T v) {
 
 // dynamic LDS of the tile kernels (NT = 3 TC threads, CH = kPPT NT records)
 size_t LdsKey(const Tiles &T, int threads) {
-  return (LdsKeyBase(T, threads) + 7)/8*8 + (size_t)T.TC*(9*sizeof(double) + 3*sizeof(float));
+  return (LdsKeyBase(T, threads) + 7)/8*8 +
+         (size_t)kKeyHalves*T.TC*(9*sizeof(double) + 3*sizeof(float));
 }
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1 + PlanInts(T.TC)); }
 // (sy: the x2-shift code, whose P2/P3 records carry the shift in one more word)
