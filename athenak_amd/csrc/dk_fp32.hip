@@ -633,9 +633,13 @@ __device__ inline void AddSlab(T (&acc)[3][3][NC], T (&spl)[3][NC], int &spdir, 
 // the 9 phases: the phases then need no workgroup barrier, only program order within the
 // wavefront (its LDS operations complete in issue order; the empty asm keeps the compiler from
 // moving one phase's LDS accesses across the next).  Starts and ends with a barrier.
+// dep: the thread holds sums to deposit (in the grouped map of ChunkSums only the first lane of
+// a group does); layers: the thread map is the one where every wavefront holds whole x1-x2
+// layers of one slab (the 9 phases then need no barrier when Tl.wrows).
 template <typename T, int NC, bool SY>
 __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int slab,
-                                    const T (&acc)[3][3][NC], const T (&spl)[3][NC], int spdir) {
+                                    const T (&acc)[3][3][NC], const T (&spl)[3][NC], int spdir,
+                                    bool dep = true, bool layers = true) {
   const int n3 = 3*NC*Tl.RN;
   __syncthreads();
   for (int i = threadIdx.x; i < n3; i += blockDim.x) cp[i] = T(0);
@@ -643,23 +647,25 @@ __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int s
   const int lx = l & (Tl.TX - 1), ly = (l >> Tl.LX) & (Tl.TY - 1), lz = l >> (Tl.LX + Tl.LY);
   const int rbase = ((lz + slab)*Tl.RY + (ly + Tl.SY))*Tl.RX + lx;
   T *mine = cp + slab*NC*Tl.RN;
-  const bool wrows = Tl.wrows != 0;
+  const bool wrows = Tl.wrows != 0 && layers;
 #pragma unroll
   for (int pb=0; pb<3; ++pb)
 #pragma unroll
     for (int pa=0; pa<3; ++pa) {
       const int rc = rbase + pb*Tl.RX + pa;
+      if (dep) {
 #pragma unroll
-      for (int n=0; n<NC; ++n) mine[n*Tl.RN + rc] += acc[pb][pa][n];
+        for (int n=0; n<NC; ++n) mine[n*Tl.RN + rc] += acc[pb][pa][n];
+      }
       if (wrows) asm volatile("" ::: "memory");
       else __syncthreads();
     }
   if (wrows) __syncthreads();
   if constexpr (SY) {
-    if (__syncthreads_or(spdir != 0)) {
+    if (__syncthreads_or(dep && spdir != 0)) {
 #pragma unroll
       for (int pa=0; pa<3; ++pa) {
-        if (spdir != 0) {
+        if (dep && spdir != 0) {
           const int rc = rbase + (1 + 2*spdir)*Tl.RX + pa;
 #pragma unroll
           for (int n=0; n<NC; ++n) mine[n*Tl.RN + rc] += spl[pa][n];
@@ -677,6 +683,86 @@ __device__ inline void DepositSlabs(const Tiles &Tl, T *reg, T *cp, int l, int s
     reg[i] = v;
   }
   __syncthreads();
+}
+
+// The cell phase of one chunk [c0, c1) outside the crowded-cell split: every (cell, slab)'s sums
+// of the chunk's records of that cell.  The plain map gives thread (cell l, slab) all of l's
+// records in store order.  A chunk whose records span at most half of the tile's cells (a dense
+// tile: in a stratified box's midplane a chunk of 768 records covers ~27 cells) would leave most
+// threads idle and a few with long loops; then a group of G = 2, 4 or 8 adjacent lanes takes one
+// (cell, slab), lane k the records whose slot within the cell is k mod G, in store order, and a
+// butterfly over the group's lanes adds their sums (a fixed order) into its first lane, which
+// deposits them.  G depends only on the chunk's cell span.  add(r, slab, acc, spl, spdir, e)
+// adds record r (chunk index); e: three extra sums per thread, added over the group too.
+// Returns the thread's cell and slab and whether it deposits; layers: the plain map.
+struct CellMap {
+  int l, slab;
+  bool dep, layers;
+};
+constexpr int kGroupMax = 8;
+
+// The cells of the chunk's first and last slots into span[0], span[1] (threads of the first TC;
+// the slots' cells are the non-empty cells whose range holds them); read after a barrier.
+constexpr int kSpanSlot = 10;   // (two ints of the plan area that CellSums leaves unused)
+__device__ inline void MarkChunkSpan(const Tiles &Tl, const int *st, int c0, int c1, int *span) {
+  const int l = threadIdx.x;
+  if (l < Tl.TC) {
+    const int a = st[l], b = st[l + 1];
+    if (a <= c0 && c0 < b) span[0] = l;
+    if (a < c1 && c1 <= b) span[1] = l;
+  }
+}
+
+template <int NC, bool SY, class Add>
+__device__ inline CellMap ChunkSums(const Tiles &Tl, const int *st, int c0, int c1,
+                                    const int *ends, const Add &add, float (&acc)[3][3][NC],
+                                    float (&spl)[3][NC], int &spdir, float (&e)[3]) {
+  const int tid = threadIdx.x;
+  const int la = ends[0], lb = ends[1];
+  const int span = lb - la + 1;
+  int G = 1;
+  while (G < kGroupMax && 2*G*span <= Tl.TC) G *= 2;
+  CellMap m;
+  int sub = 0;
+  bool act = true;
+  if (G == 1) {
+    m.slab = tid/Tl.TC;
+    m.l = tid - m.slab*Tl.TC;
+  } else {
+    const int grp = tid/G;
+    sub = tid - grp*G;
+    m.slab = grp/span;
+    m.l = la + (grp - m.slab*span);
+    act = m.slab < 3;
+    if (!act) {m.slab = 0; m.l = la;}
+  }
+  m.dep = act && sub == 0;
+  m.layers = (G == 1);
+  const int a = st[m.l];
+  int s0 = max(a, c0);
+  const int s1 = act ? min(st[m.l + 1], c1) : s0;
+  s0 += (sub - (s0 - a)) & (G - 1);
+  for (int qq = s0; qq < s1; qq += G) add(qq - c0, m.slab, acc, spl, spdir, e);
+  // (G is the same in the whole workgroup: every lane of a wavefront takes part)
+  for (int k = 1; k < G; k <<= 1) {
+#pragma unroll
+    for (int bb=0; bb<3; ++bb)
+#pragma unroll
+      for (int aa=0; aa<3; ++aa)
+#pragma unroll
+        for (int n=0; n<NC; ++n) acc[bb][aa][n] += __shfl_xor(acc[bb][aa][n], k, 64);
+    if constexpr (SY) {
+#pragma unroll
+      for (int aa=0; aa<3; ++aa)
+#pragma unroll
+        for (int n=0; n<NC; ++n) spl[aa][n] += __shfl_xor(spl[aa][n], k, 64);
+      const int o = __shfl_xor(spdir, k, 64);
+      if (spdir == 0) spdir = o;
+    }
+#pragma unroll
+    for (int n=0; n<3; ++n) e[n] += __shfl_xor(e[n], k, 64);
+  }
+  return m;
 }
 
 // Crowded cells.  The cell phase gives each cell's records to one thread per slab, in store
@@ -1614,7 +1700,7 @@ __global__ void __launch_bounds__(1024) k_bigsort(const int *start, int *perm, c
 // the deposit of Q = sum m c W/V and P = sum m c W v/V (dust_pm.cpp:225-274) and of the centre
 // mass m c/V (dust_pm.cpp:660-667)
 template <int S, bool SY, bool OV, bool SPLIT = false>
-__global__ void __launch_bounds__(kCellThreads, 4)
+__global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 6)
 k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1636,8 +1722,10 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
     float *reg = rd + RecArea(Tl, CH, 8, 4);               // [4][RN]
     int *st = reinterpret_cast<int*>(reg + 4*Tl.RN);       // [TC + 1]
     [[maybe_unused]] int *plan = st + Tl.TC + 1;           // [PlanInts(TC)]
+    float *q3s = reinterpret_cast<float*>(plan + PlanInts(Tl.TC));   // [3][TC] centre masses
     for (int i = tid; i <= Tl.TC; i += NT) st[i] = A.start[t.key0 + i];
     for (int i = tid; i < 4*Tl.RN; i += NT) reg[i] = 0.0f;
+    for (int i = tid; i < 3*Tl.TC; i += NT) q3s[i] = 0.0f;
     __syncthreads();
     // this part's particles: at most W from the tile's range
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
@@ -1691,6 +1779,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
         rv[CH + r] = mu*v[0]; rv[2*CH + r] = mu*v[1]; rv[3*CH + r] = mu*v[2];
         rcode[r] = code;
       }
+      MarkChunkSpan(Tl, st, c0, c1, plan + kSpanSlot);
       __syncthreads();
       float acc[3][3][4], spl[3][4];
 #pragma unroll
@@ -1719,26 +1808,37 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
           }
         };
         CellSums<4, 3>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 8, 4), plan, add, acc, q3);
+        DepositSlabs<float, 4, SY>(Tl, reg, rd, l, slab, acc, spl, spdir);
       } else {
-        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-        for (int qq = s0; qq < s1; ++qq) {
-          const int r = qq - c0;
+        // (the centre masses: summed by the slab-1 threads, the chunk's sums of a cell added to
+        // its running sums in LDS)
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        const auto add = [&](int r, int sl, float (&ac)[3][3][4], float (&sp)[3][4], int &sd,
+                             float (&ee)[3]) {
           float wx[3], wy[3];
           W3<S, float>(rd[r], wx);
           W3<S, float>(rd[CH + r], wy);
-          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float wz = W1<S, float>(sl, rd[2*CH + r]);
           const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
           const int code = rcode[r];
-          AddSlab<float, 4, SY>(acc, spl, spdir, ((code >> 2) & 3) - 1, wz, wy, wx, val);
-          if (slab == 1) {
+          AddSlab<float, 4, SY>(ac, sp, sd, ((code >> 2) & 3) - 1, wz, wy, wx, val);
+          if (sl == 1) {
             const int cen = code & 3;
-            q3[0] += (cen == 1) ? val[0] : 0.0f;
-            q3[1] += (cen == 2) ? val[0] : 0.0f;
-            q3[2] += (cen == 3) ? val[0] : 0.0f;
+            ee[0] += (cen == 1) ? val[0] : 0.0f;
+            ee[1] += (cen == 2) ? val[0] : 0.0f;
+            ee[2] += (cen == 3) ? val[0] : 0.0f;
           }
-        }
+        };
+        const CellMap m = ChunkSums<4, SY>(Tl, st, c0, c1, plan + kSpanSlot, add, acc, spl,
+                                                   spdir, e);
+        if (m.dep && m.slab == 1)
+          for (int k=0; k<3; ++k) q3s[k*Tl.TC + m.l] += e[k];
+        DepositSlabs<float, 4, SY>(Tl, reg, rd, m.l, m.slab, acc, spl, spdir, m.dep,
+                                      m.layers);
       }
-      DepositSlabs<float, 4, SY>(Tl, reg, rd, l, slab, acc, spl, spdir);
+    }
+    if constexpr (!SPLIT) {   // (the last deposit ended with a barrier)
+      for (int k=0; k<3; ++k) q3[k] = q3s[k*Tl.TC + l];
     }
     if (slab == 1 && part == 0) {
       int ci, cj, ck;
@@ -1761,7 +1861,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
 
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
 template <int S, bool SY, bool OV, bool SPLIT = false>
-__global__ void __launch_bounds__(kCellThreads, 4)
+__global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 8)
 k_p2(KArgs A, const float *us, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1835,6 +1935,7 @@ k_p2(KArgs A, const float *us, float *scr) {
         rg[r] = fac*gth[0]; rg[CH + r] = fac*gth[1]; rg[2*CH + r] = fac*gth[2];
         if constexpr (SY) rcode[r] = sj + 1;
       }
+      MarkChunkSpan(Tl, st, c0, c1, plan + kSpanSlot);
       __syncthreads();
       float acc[3][3][3], spl[3][3];
 #pragma unroll
@@ -1857,19 +1958,23 @@ k_p2(KArgs A, const float *us, float *scr) {
           AddSlab<float, 3, SY>(ac, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
         };
         CellSums<3, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 6, 3), plan, add, acc, nex);
+        DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
       } else {
-        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-        for (int qq = s0; qq < s1; ++qq) {
-          const int r = qq - c0;
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        const auto add = [&](int r, int sl, float (&ac)[3][3][3], float (&sp)[3][3], int &sd,
+                             float (&)[3]) {
           float wx[3], wy[3];
           W3<S, float>(rd[r], wx);
           W3<S, float>(rd[CH + r], wy);
-          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float wz = W1<S, float>(sl, rd[2*CH + r]);
           const float val[3] = {rg[r], rg[CH + r], rg[2*CH + r]};
-          AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
-        }
+          AddSlab<float, 3, SY>(ac, sp, sd, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+        };
+        const CellMap m = ChunkSums<3, SY>(Tl, st, c0, c1, plan + kSpanSlot, add, acc, spl,
+                                                   spdir, e);
+        DepositSlabs<float, 3, SY>(Tl, dep, rd, m.l, m.slab, acc, spl, spdir, m.dep,
+                                      m.layers);
       }
-      DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
     if (part == 0) FlushRegion<float, 3>(Tl, t, dep, scr);
     else FlushOverflow<3>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN);
@@ -1971,6 +2076,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
         rd[2*CH + r] = static_cast<float>(L.dz);
         if constexpr (SY) rcode[r] = sj + 1;
       }
+      MarkChunkSpan(Tl, st, c0, c1, plan + kSpanSlot);
       __syncthreads();
       float acc[3][3][3], spl[3][3];
 #pragma unroll
@@ -1993,19 +2099,23 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
           AddSlab<float, 3, SY>(ac, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
         };
         CellSums<3, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 6, 3), plan, add, acc, nex);
+        DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
       } else {
-        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-        for (int qq = s0; qq < s1; ++qq) {
-          const int r = qq - c0;
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        const auto add = [&](int r, int sl, float (&ac)[3][3][3], float (&sp)[3][3], int &sd,
+                             float (&)[3]) {
           float wx[3], wy[3];
           W3<S, float>(rd[r], wx);
           W3<S, float>(rd[CH + r], wy);
-          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float wz = W1<S, float>(sl, rd[2*CH + r]);
           const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
-          AddSlab<float, 3, SY>(acc, spl, spdir, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
-        }
+          AddSlab<float, 3, SY>(ac, sp, sd, SY ? rcode[r] - 1 : 0, wz, wy, wx, val);
+        };
+        const CellMap m = ChunkSums<3, SY>(Tl, st, c0, c1, plan + kSpanSlot, add, acc, spl,
+                                                   spdir, e);
+        DepositSlabs<float, 3, SY>(Tl, dep, rd, m.l, m.slab, acc, spl, spdir, m.dep,
+                                      m.layers);
       }
-      DepositSlabs<float, 3, SY>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
     // the flush, every thread summing (FP64) the words it writes; the defect: every thread's
     // impulses minus those words, summed over each wavefront, then over the wavefronts in order
@@ -2050,7 +2160,7 @@ k_p3n(KArgs A, const float *us, float *scr, double *defect) {
 // the source order (the next key pass and an export read them through this sort's permutation).
 // (CLOUD: refinable clouds at the mesh level, the linear gather from the 12-component pack uv)
 template <int S, bool OV, bool SPLIT = false, bool CLOUD = false>
-__global__ void __launch_bounds__(kCellThreads, 4)
+__global__ void __launch_bounds__(kCellThreads, (OV || SPLIT || CLOUD) ? 4 : 6)
 k_pp(KArgs A, const float *uv, float *scr) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -2145,6 +2255,7 @@ k_pp(KArgs A, const float *uv, float *scr) {
         rd[r] = ox; rd[CH + r] = oy; rd[2*CH + r] = oz;
         for (int n=0; n<4; ++n) rv[n*CH + r] = val[n];
       }
+      MarkChunkSpan(Tl, st, c0, c1, plan + kSpanSlot);
       __syncthreads();
       float acc[3][3][4], spl[3][4];
 #pragma unroll
@@ -2167,19 +2278,23 @@ k_pp(KArgs A, const float *uv, float *scr) {
           AddSlab<float, 4, false>(ac, spl, spdir, 0, wz, wy, wx, val);
         };
         CellSums<4, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 7, 4), plan, add, acc, nex);
+        DepositSlabs<float, 4, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
       } else {
-        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-        for (int qq = s0; qq < s1; ++qq) {
-          const int r = qq - c0;
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        const auto add = [&](int r, int sl, float (&ac)[3][3][4], float (&sp)[3][4], int &sd,
+                             float (&)[3]) {
           float wx[3], wy[3];
           W3<S, float>(rd[r], wx);
           W3<S, float>(rd[CH + r], wy);
-          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float wz = W1<S, float>(sl, rd[2*CH + r]);
           const float val[4] = {rv[r], rv[CH + r], rv[2*CH + r], rv[3*CH + r]};
-          AddSlab<float, 4, false>(acc, spl, spdir, 0, wz, wy, wx, val);
-        }
+          AddSlab<float, 4, false>(ac, sp, sd, 0, wz, wy, wx, val);
+        };
+        const CellMap m = ChunkSums<4, false>(Tl, st, c0, c1, plan + kSpanSlot, add, acc, spl,
+                                                   spdir, e);
+        DepositSlabs<float, 4, false>(Tl, dep, rd, m.l, m.slab, acc, spl, spdir, m.dep,
+                                      m.layers);
       }
-      DepositSlabs<float, 4, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
     if (part == 0) FlushRegion<float, 4>(Tl, t, dep, scr);
     else FlushOverflow<4>(Tl, dep, A.ov_scr + OvSlot(A.item_start, tile, part)*4*Tl.RN);
@@ -2311,6 +2426,7 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
         }
         rd[r] = ox; rd[CH + r] = oy; rd[2*CH + r] = oz;
       }
+      MarkChunkSpan(Tl, st, c0, c1, plan + kSpanSlot);
       __syncthreads();
       float acc[3][3][3], spl[3][3];
 #pragma unroll
@@ -2333,19 +2449,23 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
           AddSlab<float, 3, false>(ac, spl, spdir, 0, wz, wy, wx, val);
         };
         CellSums<3, 1>(Tl, st, c0, c1, l, slab, rd, RecArea(Tl, CH, 6, 3), plan, add, acc, nex);
+        DepositSlabs<float, 3, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
       } else {
-        const int s0 = max(st[l], c0), s1 = min(st[l + 1], c1);
-        for (int qq = s0; qq < s1; ++qq) {
-          const int r = qq - c0;
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        const auto add = [&](int r, int sl, float (&ac)[3][3][3], float (&sp)[3][3], int &sd,
+                             float (&)[3]) {
           float wx[3], wy[3];
           W3<S, float>(rd[r], wx);
           W3<S, float>(rd[CH + r], wy);
-          const float wz = W1<S, float>(slab, rd[2*CH + r]);
+          const float wz = W1<S, float>(sl, rd[2*CH + r]);
           const float val[3] = {rv[r], rv[CH + r], rv[2*CH + r]};
-          AddSlab<float, 3, false>(acc, spl, spdir, 0, wz, wy, wx, val);
-        }
+          AddSlab<float, 3, false>(ac, sp, sd, 0, wz, wy, wx, val);
+        };
+        const CellMap m = ChunkSums<3, false>(Tl, st, c0, c1, plan + kSpanSlot, add, acc, spl,
+                                                   spdir, e);
+        DepositSlabs<float, 3, false>(Tl, dep, rd, m.l, m.slab, acc, spl, spdir, m.dep,
+                                      m.layers);
       }
-      DepositSlabs<float, 3, false>(Tl, dep, rd, l, slab, acc, spl, spdir);
     }
     double ws[3];
     if (part == 0) FlushRegion<float, 3>(Tl, t, dep, scr, ws);
@@ -2963,7 +3083,7 @@ size_t LdsKey(const Tiles &T, int threads) {
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1 + PlanInts(T.TC)); }
 // (sy: the x2-shift code, whose P2/P3 records carry the shift in one more word)
 size_t LdsP1(const Tiles &T, bool = true) {
-  return sizeof(float)*(RecArea(T, kPPT*3*T.TC, 8, 4) + 4*T.RN) + LdsTail(T);
+  return sizeof(float)*(RecArea(T, kPPT*3*T.TC, 8, 4) + 4*T.RN + 3*T.TC) + LdsTail(T);
 }
 size_t LdsP2(const Tiles &T, bool sy = true) {
   return sizeof(float)*(6*T.RN + RecArea(T, kPPT*3*T.TC, sy ? 7 : 6, 3)) + LdsTail(T);
