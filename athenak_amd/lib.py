@@ -191,10 +191,16 @@ class DustContext:
         return (abi.imex2p_stages(self.time, dt) if self.phys.coupling == abi.DK_IMEX
                 else abi.rk2_stages(self.time, dt))
 
-    def step(self, dt, src=None):
+    def step(self, dt, src=None, check=True):
+        """One step's stages.  check: read the device error word afterwards (a host
+        synchronisation; a timed loop passes False and checks once after it), so that an error
+        recorded by a kernel (a cell beyond the sort's capacity, a particle outside the deposit
+        halo) stops the run at the step where it happened."""
         for st in self.stages(dt):
             self.stage(st, src)
         self.time += dt
+        if check:
+            self.check_device_error()
 
     def set_graphs(self, on):
         self._check(lib().dk_set_graphs(self.h, 1 if on else 0), "dk_set_graphs")
