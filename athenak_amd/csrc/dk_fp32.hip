@@ -1046,6 +1046,12 @@ __host__ __device__ inline size_t LdsKeyBase(const Tiles &T, int threads) {
   return (size_t)threads*sizeof(KeyRec) + sizeof(int)*(2*T.TC + 1);
 }
 
+// the key pass's per-wavefront stayer counts of a chunk: [threads/64][TC] ints, each the chunk's
+// stamp << 16 | the count (a stale entry has another stamp)
+__host__ __device__ inline size_t LdsKeyWaves(const Tiles &T, int threads) {
+  return sizeof(int)*(size_t)(threads/64)*T.TC;
+}
+
 // the overflow slot of part >= 1 of a tile
 __device__ inline long long OvSlot(const int *item_start, int tile, int part) {
   return (long long)item_start[tile] - tile + part - 1;
@@ -1214,7 +1220,7 @@ __device__ inline void KeyFetch(const KArgs &A, const TileInfo &t, const int *st
 // only the pending correction is applied (written to v and R).  MID: the pc2 stage-2 form, keyed
 // by the half-step position (Pc2Route) instead of pushed.
 template <bool OV, bool MID = false, bool SMR = false>
-__global__ void __launch_bounds__(kKeyThreadsMax)
+__global__ void __launch_bounds__(kKeyThreadsMax, (OV || SMR) ? 2 : 4)
 k_key(KArgs A, int do_key, double *pres) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
@@ -1238,6 +1244,8 @@ k_key(KArgs A, int do_key, double *pres) {
     // that only one thread touches: they stay out of registers during the particle phase
     double *cacc = reinterpret_cast<double*>(lds_d) + (LdsKeyBase(Tl, CH) + 7)/8;   // [2][9][TC]
     float *pacc = reinterpret_cast<float*>(cacc + kKeyHalves*9*Tl.TC);        // [2][3][TC]
+    int *wcnt = reinterpret_cast<int*>(pacc + kKeyHalves*3*Tl.TC);          // [CH/64][TC]
+    for (int i = tid; i < (CH >> 6)*Tl.TC; i += CH) wcnt[i] = 0;
     for (int i = tid; i < kKeyHalves*Tl.TC; i += CH) {
       for (int k=0; k<9; ++k) cacc[k*Tl.TC + (i/Tl.TC)*9*Tl.TC + i % Tl.TC] = 0.0;
       for (int k=0; k<3; ++k) pacc[k*Tl.TC + (i/Tl.TC)*3*Tl.TC + i % Tl.TC] = 0.0f;
@@ -1255,7 +1263,10 @@ k_key(KArgs A, int do_key, double *pres) {
     // during this chunk's per-cell sums and barriers
     KeyIn nx;
     if (P0 + tid < P1) KeyFetch<MID>(A, t, st, P0 + tid, do_key, nx);
+    int stamp = 0;   // (the chunk's number within the item, from 1)
     for (int c0 = P0; c0 < P1; c0 += CH) {
+      ++stamp;
+      int lkey = -1;   // a stayer's cell in the tile
       const int c1 = min(c0 + CH, P1);
       const int q = c0 + tid;
       if (q < c1) {
@@ -1336,10 +1347,9 @@ k_key(KArgs A, int do_key, double *pres) {
           const long long key = TileKey(Tl, s.gid, ni, nj, nk);
           A.key[q] = static_cast<int>(key);
           const long long lk = key - t.key0;
-          // (a stayer's rank is its rank among its part's stayers of the cell, tagged with the
-          // part: the placement adds the stayers of the earlier parts)
-          A.rank[q] = (lk >= 0 && lk < Tl.TC) ? part*kPartRank + atomicAdd(&lcnt[lk], 1)
-                                              : -1 - atomicAdd(&A.count[key], 1);
+          // (a stayer's rank is taken below; an arrival's from the cell's global count)
+          if (lk >= 0 && lk < Tl.TC) lkey = static_cast<int>(lk);
+          else A.rank[q] = -1 - atomicAdd(&A.count[key], 1);
         }
         KeyRec kr;
         for (int d=0; d<3; ++d) {kr.dv[d] = dv[d]; kr.pr[d] = pr[d];}
@@ -1347,9 +1357,42 @@ k_key(KArgs A, int do_key, double *pres) {
         kr.pad = 0;
         rec[tid] = kr;
       }
+      // A stayer's rank is its rank among its part's stayers of its new cell in slot order (so
+      // the stayers of a cell need no sort): the earlier chunks' count (lcnt), the earlier
+      // wavefronts' counts in this chunk, the earlier lanes of its wavefront.  Within the
+      // wavefront: one pass per distinct cell, its lanes found by a ballot.
+      const int lane = tid & 63, wv = tid >> 6;
+      int wrank = 0;
+      if (do_key) {
+        unsigned long long pend = __ballot(lkey >= 0);
+        while (pend) {
+          const int leader = __ffsll(static_cast<long long>(pend)) - 1;
+          const int cell = __shfl(lkey, leader, 64);
+          const unsigned long long peers = __ballot(lkey == cell);
+          if (lkey == cell) wrank = __popcll(peers & ((1ull << lane) - 1ull));
+          if (lane == leader) wcnt[wv*Tl.TC + cell] = (stamp << 16) | __popcll(peers);
+          pend &= ~peers;
+        }
+      }
       // (the next chunk's slots: no particle of this chunk is written there)
       if (q + CH < P1) KeyFetch<MID>(A, t, st, q + CH, do_key, nx);
       __syncthreads();
+      int ctot = 0;   // (thread l < TC: cell l's stayers of this chunk)
+      if (do_key) {
+        if (lkey >= 0) {
+          int r = lcnt[lkey] + wrank;
+          for (int w = 0; w < wv; ++w) {
+            const int x = wcnt[w*Tl.TC + lkey];
+            if ((x >> 16) == stamp) r += x & 0xffff;
+          }
+          A.rank[q] = part*kPartRank + r;
+        }
+        if (tid < Tl.TC)
+          for (int w = 0; w < (CH >> 6); ++w) {
+            const int x = wcnt[w*Tl.TC + tid];
+            if ((x >> 16) == stamp) ctot += x & 0xffff;
+          }
+      }
       if (A.cell_sums) {
         // (cell l, half h): the records of l in this chunk whose slot has the parity h within
         // the cell, in slot order
@@ -1378,6 +1421,7 @@ k_key(KArgs A, int do_key, double *pres) {
         }
       }
       __syncthreads();
+      if (tid < Tl.TC) lcnt[tid] += ctot;   // (read again only after the next chunk's barrier)
     }
     double dcs[3][3];
     float prs[3];
@@ -1543,8 +1587,8 @@ __global__ void k_place(const int *key, const int *rank, const int *start, const
   }
 }
 
-// every cell's list in ascending source slot (the stable order); medium and long lists are
-// queued.  A workgroup takes kBlock consecutive keys at a time, or, when their lists do not fit
+// every cell's arrivals (its list after its stay[] stayers, which the key pass ranked in slot
+// order) in ascending source slot; medium and long lists of arrivals are queued.  A workgroup takes kBlock consecutive keys at a time, or, when their lists do not fit
 // in kSegStage slots, the same keys in runs of kSegKeys.  A run whose lists fit is staged in LDS
 // (one coalesced read, one coalesced write) and every element of a list of up to kSegRank is
 // placed by its rank (the count of smaller source slots in its list: the slots are distinct), a
@@ -1555,9 +1599,9 @@ constexpr int kSegKeys = 64;
 constexpr int kSegStage = 4096;
 constexpr int kSegRank = 128;       // longest list a staged run ranks itself
 
-__device__ inline void SegSortRun(const int *start, long long nkeys, int *perm, int *big,
-                                  long long c0, int nk, int *sh, int *out, int *cs, int *qn,
-                                  int *qbase) {
+__device__ inline void SegSortRun(const int *start, const int *stay, long long nkeys, int *perm,
+                                  int *big, long long c0, int nk, int *sh, int *out, int *cs,
+                                  int *ts, int *qn, int *qbase) {
   const int t = threadIdx.x;
   const long long c = c0 + t;
   const long long ce = (c0 + nk < nkeys) ? c0 + nk : nkeys;
@@ -1565,11 +1609,13 @@ __device__ inline void SegSortRun(const int *start, long long nkeys, int *perm, 
   const int r0 = start[c0], r1 = start[ce];
   const bool staged = (r1 - r0) <= kSegStage;   // (the same for the whole workgroup)
   for (int q = t; q <= nc; q += kBlock) cs[q] = start[c0 + q] - r0;   // (nc + 1 entries)
+  for (int q = t; q < nc; q += kBlock) ts[q] = start[c0 + q] + stay[c0 + q] - r0;
   if (t < 2) qn[t] = 0;
   if (staged)
     for (int i = r0 + t; i < r1; i += kBlock) sh[i - r0] = perm[i];
   __syncthreads();
-  const int n = (t < nc) ? cs[t + 1] - cs[t] : 0;
+  // (a cell's list to order: its arrivals, after its stayers)
+  const int n = (t < nc) ? cs[t + 1] - ts[t] : 0;
   // (a staged run ranks its lists of up to kSegRank elements: longer medium ones, and the
   // medium ones of a run that is not staged, are queued)
   const int cls = (n > kMedMax) ? 1 : ((n > (staged ? kSegRank : kSegSmall)) ? 0 : -1);
@@ -1580,7 +1626,7 @@ __device__ inline void SegSortRun(const int *start, long long nkeys, int *perm, 
   if (cls == 1) big[1 + nkeys - (qbase[1] + qs)] = static_cast<int>(c);
   else if (cls == 0) big[2 + qbase[0] + qs] = static_cast<int>(c);
   if (!staged && n > 1 && n <= kSegSmall) {
-    int *a = perm + r0 + cs[t];
+    int *a = perm + r0 + ts[t];
     for (int i=1; i<n; ++i) {
       const int v = a[i];
       int j = i - 1;
@@ -1597,9 +1643,11 @@ __device__ inline void SegSortRun(const int *start, long long nkeys, int *perm, 
         const int mid = (lo + hi) >> 1;
         if (cs[mid] <= e) lo = mid; else hi = mid;
       }
-      const int s0 = cs[lo], s1 = cs[lo + 1], nl = s1 - s0;
+      const int s0 = ts[lo], s1 = cs[lo + 1], nl = s1 - s0;
       const int v = sh[e];
-      if (nl <= kSegRank) {
+      if (e < s0) {
+        out[e] = v;   // (a stayer: in place)
+      } else if (nl <= kSegRank) {
         int rank = 0;
         for (int q = s0; q < s1; ++q) rank += (sh[q] < v) ? 1 : 0;
         out[s0 + rank] = v;
@@ -1613,33 +1661,34 @@ __device__ inline void SegSortRun(const int *start, long long nkeys, int *perm, 
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kBlock) k_segsort(const int *start, long long nkeys, int *perm,
-                                                    int *big) {
+__global__ void __launch_bounds__(kBlock) k_segsort(const int *start, const int *stay,
+                                                    long long nkeys, int *perm, int *big) {
   __shared__ int sh[kSegStage], out[kSegStage];
-  __shared__ int cs[kBlock + 1];
+  __shared__ int cs[kBlock + 1], ts[kBlock];
   __shared__ int qn[2], qbase[2];   // the run's medium / long lists and their queue base
   for (long long c0 = (long long)blockIdx.x*kBlock; c0 < nkeys;
        c0 += (long long)gridDim.x*kBlock) {
     const long long ce = (c0 + kBlock < nkeys) ? c0 + kBlock : nkeys;
     if (start[ce] - start[c0] <= kSegStage) {
-      SegSortRun(start, nkeys, perm, big, c0, kBlock, sh, out, cs, qn, qbase);
+      SegSortRun(start, stay, nkeys, perm, big, c0, kBlock, sh, out, cs, ts, qn, qbase);
     } else {
       for (long long r = c0; r < ce; r += kSegKeys)
-        SegSortRun(start, nkeys, perm, big, r, kSegKeys, sh, out, cs, qn, qbase);
+        SegSortRun(start, stay, nkeys, perm, big, r, kSegKeys, sh, out, cs, ts, qn, qbase);
     }
   }
 }
 
 // one wavefront per medium list: an element's place is the count of smaller source slots in the
 // list (the slots are distinct), the list staged in LDS
-__global__ void __launch_bounds__(64) k_medsort(const int *start, int *perm, const int *big) {
+__global__ void __launch_bounds__(64) k_medsort(const int *start, const int *stay, int *perm,
+                                                 const int *big) {
   constexpr int kPer = kMedMax/64;
   __shared__ int sh[kMedMax];
   const int nmed = big[0];
   const int lane = threadIdx.x;
   for (int e = blockIdx.x; e < nmed; e += gridDim.x) {
     const int c = big[2 + e];
-    const int s0 = start[c], n = start[c + 1] - s0;
+    const int s0 = start[c] + stay[c], n = start[c + 1] - s0;
     int v[kPer], r[kPer];
 #pragma unroll
     for (int t=0; t<kPer; ++t) {
@@ -1662,13 +1711,14 @@ __global__ void __launch_bounds__(64) k_medsort(const int *start, int *perm, con
 }
 
 // one workgroup per long list: bitonic sort in LDS (padded to a power of two)
-__global__ void __launch_bounds__(1024) k_bigsort(const int *start, int *perm, const int *big,
-                                                  long long nkeys, dk::ErrWord *err) {
+__global__ void __launch_bounds__(1024) k_bigsort(const int *start, const int *stay, int *perm,
+                                                  const int *big, long long nkeys,
+                                                  dk::ErrWord *err) {
   __shared__ int sh[kBigMax];
   const int nbig = big[1];
   for (int e = blockIdx.x; e < nbig; e += gridDim.x) {
     const int c = big[1 + nkeys - e];
-    const int s0 = start[c], n = start[c + 1] - s0;
+    const int s0 = start[c] + stay[c], n = start[c + 1] - s0;
     if (n > kBigMax) {
       if (threadIdx.x == 0) dk::RecordError(err, DK_ECAPACITY, c);
       continue;
@@ -3078,7 +3128,7 @@ int FillV(dk_ctx *c, T *a, long long n, T v) {
 // dynamic LDS of the tile kernels (NT = 3 TC threads, CH = kPPT NT records)
 size_t LdsKey(const Tiles &T, int threads) {
   return (LdsKeyBase(T, threads) + 7)/8*8 +
-         (size_t)kKeyHalves*T.TC*(9*sizeof(double) + 3*sizeof(float));
+         (size_t)kKeyHalves*T.TC*(9*sizeof(double) + 3*sizeof(float)) + LdsKeyWaves(T, threads);
 }
 size_t LdsTail(const Tiles &T) { return sizeof(int)*(T.TC + 1 + PlanInts(T.TC)); }
 // (sy: the x2-shift code, whose P2/P3 records carry the shift in one more word)
@@ -3326,7 +3376,9 @@ KArgs MakeArgs(dk_ctx *c) {
   return A;
 }
 
-// counting sort of the store by the keys of the key pass: scan, placement, stable lists
+// counting sort of the store by the keys of the key pass: scan, placement, and the arrivals of
+// every cell in ascending source slot after its stayers (whose ranks are in slot order): a fixed
+// order
 int SortKeys(dk_ctx *c) {
   Fast *f = c->fast;
   int e;
@@ -3339,10 +3391,10 @@ int SortKeys(dk_ctx *c) {
       f->key, f->rank, f->start, f->stay, f->np, f->T.TC, f->item_start, f->ov_cnt, f->perm)));
   if ((e = FillV<int>(c, f->big, 2, 0))) return e;
   TIMED("segsort", (k_segsort<<<Grid(f->nkeys), kBlock, 0, c->stream>>>(
-                        f->start, f->nkeys, f->perm, f->big)));
-  TIMED("medsort", (k_medsort<<<4096, 64, 0, c->stream>>>(f->start, f->perm, f->big)));
-  TIMED("bigsort", (k_bigsort<<<256, 1024, 0, c->stream>>>(f->start, f->perm, f->big, f->nkeys,
-                                                            c->d_err)));
+                        f->start, f->stay, f->nkeys, f->perm, f->big)));
+  TIMED("medsort", (k_medsort<<<4096, 64, 0, c->stream>>>(f->start, f->stay, f->perm, f->big)));
+  TIMED("bigsort", (k_bigsort<<<256, 1024, 0, c->stream>>>(f->start, f->stay, f->perm, f->big,
+                                                            f->nkeys, c->d_err)));
   return dk::Check(hipGetLastError());
 }
 

@@ -144,20 +144,29 @@ def _tile_keys(x, gid, mesh, blocks, tile=(8, 4, 4)):
 
 
 def test_fp32_sort_output_is_the_stable_counting_sort():
+    """The FP32 sort's output against numpy: every cell's list holds first the particles that
+    stay in their tile, then those that arrive from other tiles, each group in ascending source
+    slot (numpy's lexicographic sort of (key, arrival, source slot))."""
     w = workloads.c1(n=16, nb=2, ppc=3)
     w.setup.dust_noise, w.dt = 3.0, 2e-3      # particles change cells, tiles and blocks
     g = ctx(w, abi.DK_FORM_FP32)
     st = g.stages(w.dt)
     g.step(w.dt)
+    # the tile of every slot of the present order: the keys of the last sort, in its new order
+    key0, perm0, _ = g.sort_state()
+    TC = 8 * 4 * 4
+    tile_of_slot = key0[perm0] // TC
     g.stage(g.stages(w.dt)[0])                # stage 1 of the next step: its sort is the last
     key, perm, start = g.sort_state()
     g.check_device_error()
     n = g.npart
     nk = w.mesh.nmb * w.mesh.nx1 * w.mesh.nx2 * w.mesh.nx3
     assert len(st) == 3 and len(key) == n and len(perm) == n and len(start) == nk + 1
-    # a permutation, and the stable order of the keys
+    # a permutation, and the order of (key, stayer before arrival, source slot)
     assert np.array_equal(np.sort(perm), np.arange(n))
-    np.testing.assert_array_equal(perm, np.argsort(key, kind="stable"))
+    arrival = (key // TC) != tile_of_slot
+    np.testing.assert_array_equal(perm, np.lexsort((np.arange(n), arrival, key)))
+    assert 0 < arrival.sum() < n              # both kinds occur
     # the cell starts are the exclusive scan of the key counts
     counts = np.bincount(key, minlength=nk)
     np.testing.assert_array_equal(start, np.concatenate([[0], np.cumsum(counts)]))
