@@ -3973,9 +3973,9 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
 //----------------------------------------------------------------------------------------
 // The autotuner.  Its choices are launch parameters that leave every result bit-identical: the
 // dealing of tiles to the XCDs (runs of tiles on one XCD, so that neighbouring tiles share its
-// L2, or the plain order) and the key pass's workgroup size (its per-cell sums run over each
-// cell's particles in store order whatever the chunk, and its ranks are re-sorted by source
-// slot).  The tile shape is not among them: it groups the cross-tile sums.  Tuning runs online,
+// L2, or the plain order) and the key pass's workgroup size (its per-cell sums are split by the
+// slot's parity within the cell whatever the chunk, its stayers are ranked in slot order and its
+// arrivals re-sorted by source slot).  The tile shape is not among them: it groups the cross-tile sums.  Tuning runs online,
 // on the stages the caller runs anyway: after kTuneStages warm-up stages each candidate runs
 // kTuneStages active stages, timed with events on the context's stream, in kTuneRounds rounds
 // over the candidates, and the one with the fastest visit is kept (the smallest of its visits:
