@@ -583,9 +583,15 @@ __device__ inline void W3(T d, T w[3]) {
 
 template <int S, typename T>
 __device__ inline T W1(int a, T d) {
-  T w[3];
-  W3<S, T>(d, w);
-  return (a == 0) ? w[0] : ((a == 1) ? w[1] : w[2]);
+  if constexpr (S == 2) {
+    // (only the weight asked for: 1/2 (1/2 - d)^2 as 1/2 (d - 1/2)^2, the same bits)
+    const T h = d + ((a == 0) ? T(-0.5) : T(0.5));
+    return (a == 1) ? T(0.75) - d*d : T(0.5)*h*h;
+  } else {
+    T w[3];
+    W3<S, T>(d, w);
+    return (a == 0) ? w[0] : ((a == 1) ? w[1] : w[2]);
+  }
 }
 
 // One record's 9 terms of one slab, w_z w_y(b) w_x(a) val[n], into acc[b][a][n].  With SY (the
