@@ -19,6 +19,9 @@ DK_X3_PERIODIC, DK_X3_REFLECT, DK_X3_OUTFLOW = 0, 1, 2
 DK_FORM_FP64, DK_FORM_FP32 = 0, 1
 DK_MAXSPEC = 8
 
+(DK_OK, DK_EINVAL, DK_EHIP, DK_ENOMEM, DK_EHALO, DK_EROUTE, DK_ESTOP, DK_EUNSUPPORTED,
+ DK_ECAPACITY, DK_ESTATE, DK_ENOTFINITE, DK_ESOLVER) = range(12)
+
 ERRORS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory",
           4: "particle outside deposit halo", 5: "particle routing", 6: "stopping time <= 0",
           7: "unsupported option", 8: "capacity", 9: "call order", 10: "non-finite value",
@@ -62,7 +65,8 @@ class dk_stage(C.Structure):
 class dk_gas_src(C.Structure):
     _fields_ = [("shearing_box", C.c_int32), ("stratified", C.c_int32),
                 ("omega0", C.c_double), ("qshear", C.c_double),
-                ("const_accel", C.c_double), ("const_accel_dir", C.c_int32)]
+                ("const_accel", C.c_double), ("const_accel_dir", C.c_int32),
+                ("hydro", C.c_int32), ("iso_cs", C.c_double), ("dfloor", C.c_double)]
 
 
 class dk_setup(C.Structure):

@@ -2884,6 +2884,14 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
   }
 }
 
+// the drag history of stage 2 alone (dust_pm.cpp:734-739), the last step of k_cell_start,
+// for a stage whose gas update runs between them (the isothermal fluxes of dk_gas_src.hydro)
+__global__ void k_drag_history(Geo g, double *u0, const double *dmom, double atw_dt) {
+  BLOCK_CELLS(m, k, j, i)
+    for (int c=0; c<3; ++c) u0[Idx(g,m,DK_NGAS,1+c,k,j,i)] += atw_dt*dmom[Idx(g,m,3,c,k,j,i)];
+  }
+}
+
 // PackGatherField3D (src/dust/dust_cloud3d.cpp:1319-1340) in FP32: the velocity and its central
 // differences along x1, x2, x3 of every cell with both neighbours in each direction (the
 // active cells and the first ghost layer, which the tile regions read)
@@ -3826,12 +3834,15 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   A.rate_dep = dk::StiffnessActive(c->P);
   A.cell_sums = 0;
   {
+    // with isothermal fluxes the gas update runs after the cell pass (it reads u0's ghosts)
+    const bool hydro = (src != nullptr && src->hydro);
     dk_gas_src z;
     std::memset(&z, 0, sizeof(z));
     TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
-        f->pres[f->pres_rd], 0, 0.0, A.stage1, 1, st->gam0[s], st->gam1[s],
+        f->pres[f->pres_rd], 0, 0.0, A.stage1, hydro ? 0 : 1, st->gam0[s], st->gam1[s],
         st->beta[s]*st->dt, src ? 1 : 0, src ? *src : z, 0.0, f->ustar, s1 ? 1 : 2)));
+    if (hydro && (e = dk::GasHook(c, st, src))) return e;
   }
   if ((e = FillGhostsT<float>(c, f->ustar, 3, true, st->time))) return e;
   // clouds: the gather pack of the velocity (PackGatherField3D)
@@ -3922,15 +3933,24 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
       A, 1, f->pres[1 - f->pres_rd])));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history
   {
-    const int hook = 1;
+    // with isothermal fluxes the gas update and the drag history run after the cell pass (the
+    // fluxes read u0's ghosts, refilled once the pending share is in)
+    const bool hydro = (src != nullptr && src->hydro);
     dk_gas_src z;
     std::memset(&z, 0, sizeof(z));
     const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
+    const double atw = (c->P.back_reaction && st->stage == 2) ? st->a_twid22*st->dt : 0.0;
     TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
         g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
         f->pres[f->pres_rd], f->pending,
-        f->pending ? 1.0/f->a_prev : 0.0, A.stage1, hook, gam0, gam1, bdt, src ? 1 : 0,
-        src ? *src : z, (c->P.back_reaction && st->stage == 2) ? st->a_twid22*st->dt : 0.0)));
+        f->pending ? 1.0/f->a_prev : 0.0, A.stage1, hydro ? 0 : 1, gam0, gam1, bdt,
+        src ? 1 : 0, src ? *src : z, hydro ? 0.0 : atw)));
+    if (hydro) {
+      if ((e = dk::GasHook(c, st, src))) return e;
+      if (atw != 0.0)
+        TIMED("drag_history", (k_drag_history<<<CellGrid(c), kBlock, 0, c->stream>>>(
+            g, c->u0, f->dmom, atw)));
+    }
   }
   f->pending = 0;
   A.pending = 0;

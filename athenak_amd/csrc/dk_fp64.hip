@@ -728,9 +728,134 @@ __global__ void k_cell_update(Geo g, double *u0, double *dmom, const double *qde
   }
 }
 
-// Hydro::RKUpdate with zero fluxes + HydroSrcTerms (see dustk.h dk_gas_src)
+// ---- the explicit gas update: isothermal fluxes (dk_gas_src.hydro) ----
+//
+// reconstruct/plm.hpp:13-28: the limited slope of q (van Leer's harmonic mean on a uniform
+// mesh); lo = the right state of the cell's lower face, hi = the left state of its upper face
+__device__ inline void PlmFace(double qm, double q0, double qp, double &lo, double &hi) {
+  const double dql = q0 - qm, dqr = qp - q0;
+  const double dq2 = dql*dqr;
+  double dqm = dq2/(dql + dqr);
+  if (dq2 <= 0.0) dqm = 0.0;
+  hi = q0 + dqm;
+  lo = q0 - dqm;
+}
+
+// src/hydro/rsolvers/hlle_hyd.hpp:22-150 with the isothermal EOS; states and flux in the face's
+// frame (d, v_n, v_t1, v_t2)
+__device__ inline void HlleIsoFlux(const double wl[4], const double wr[4], double cs,
+                                   double f[4]) {
+  const double sdl = sqrt(wl[0]), sdr = sqrt(wr[0]);
+  const double isd = 1.0/(sdl + sdr);
+  const double vroe = (sdl*wl[1] + sdr*wr[1])*isd;
+  const double al = fmin(vroe - cs, wl[1] - cs);
+  const double ar = fmax(vroe + cs, wr[1] + cs);
+  const double bp = (ar > 0.0) ? ar : 1.0e-20;
+  const double bm = (al < 0.0) ? al : -1.0e-20;
+  const double qa = wl[1] - bm, qb = wr[1] - bp;
+  double fl[4], fr[4];
+  fl[0] = wl[0]*qa;
+  fr[0] = wr[0]*qb;
+#pragma unroll
+  for (int c=1; c<4; ++c) {
+    fl[c] = wl[0]*wl[c]*qa;
+    fr[c] = wr[0]*wr[c]*qb;
+  }
+  fl[1] += (cs*cs)*wl[0];
+  fr[1] += (cs*cs)*wr[0];
+  double h = 0.0;
+  if (bp != bm) h = 0.5*(bp + bm)/(bp - bm);
+#pragma unroll
+  for (int c=0; c<4; ++c) f[c] = 0.5*(fl[c] + fr[c]) + h*(fl[c] - fr[c]);
+}
+
+// IsothermalHydro::ConsToPrim of one cell (src/eos/isothermal_hyd.cpp:34-48: density floor,
+// then the velocities by the reciprocal density); at = the cell's density index
+__device__ inline void PrimOf(const double *u, long long at, long long ncb, double dfloor,
+                              double w[4]) {
+  double d = u[at];
+  if (d < dfloor) d = dfloor;
+  const double di = 1.0/d;
+  w[0] = d;
+  w[1] = di*u[at + ncb];
+  w[2] = di*u[at + 2*ncb];
+  w[3] = di*u[at + 3*ncb];
+}
+
+// Hydro::Fluxes (src/hydro/hydro_fluxes.cpp:20-140, plm + hlle) and the flux divergence of
+// Hydro::RKUpdate (src/hydro/hydro_update.cpp:55-77): one thread per active cell computes the
+// six face fluxes of its cell from the primitives of the 13 cells on the three lines through
+// it (u0 ghosts filled).  Both cells beside a face compute its flux with the same operations
+// on the same values, so the divergence telescopes as the reference's stored face fluxes do;
+// the axes are added in the reference's order x1, x2, x3.
+__global__ __launch_bounds__(kBlock, 4) void k_hydro_divf(Geo g, const DevBlock *B,
+                                                       const double *u0, double *divf,
+                                                       double cs, double dfloor) {
+  const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
+  const long long ncb = g.ncb;
+  GRID_STRIDE(t, n) {
+    long long rr = t;
+    const int i = g.is + static_cast<int>(rr % g.nx1); rr /= g.nx1;
+    const int j = g.js + static_cast<int>(rr % g.nx2); rr /= g.nx2;
+    const int k = g.ks + static_cast<int>(rr % g.nx3);
+    const int m = static_cast<int>(rr/g.nx3);
+    const long long at = Idx(g,m,DK_NGAS,DK_IDN,k,j,i);
+    const DevBlock &b = B[m];
+    double wc[4];
+    PrimOf(u0, at, ncb, dfloor, wc);
+    double dv[4] = {0.0, 0.0, 0.0, 0.0};
+    // one axis at a time (not unrolled: the live state stays within the register budget)
+#pragma unroll 1
+    for (int d=0; d<3; ++d) {
+      const long long sd = (d == 0) ? 1 : ((d == 1) ? (long long)g.n1 : (long long)g.n1*g.n2);
+      const double dx = (d == 0) ? b.dx1 : ((d == 1) ? b.dx2 : b.dx3);
+      // primitives of the cells -2..2 along the axis in the face frame (d, v_n, v_t1, v_t2)
+      double w[5][4];
+#pragma unroll
+      for (int o=0; o<5; ++o) {
+        double p[4];
+        if (o == 2) {
+#pragma unroll
+          for (int v=0; v<4; ++v) p[v] = wc[v];
+        } else {
+          PrimOf(u0, at + (o - 2)*sd, ncb, dfloor, p);
+        }
+        w[o][0] = p[0];
+        w[o][1] = (d == 0) ? p[1] : ((d == 1) ? p[2] : p[3]);
+        w[o][2] = (d == 0) ? p[2] : ((d == 1) ? p[3] : p[1]);
+        w[o][3] = (d == 0) ? p[3] : ((d == 1) ? p[1] : p[2]);
+      }
+      // face states: the lower face (cells -1 | 0) and the upper face (cells 0 | +1)
+      double wlm[4], wrm[4], wlp[4], wrp[4];
+#pragma unroll
+      for (int v=0; v<4; ++v) {
+        double unused;
+        PlmFace(w[0][v], w[1][v], w[2][v], unused, wlm[v]);
+        PlmFace(w[1][v], w[2][v], w[3][v], wrm[v], wlp[v]);
+        PlmFace(w[2][v], w[3][v], w[4][v], wrp[v], unused);
+      }
+      double fm[4], fp[4];
+      HlleIsoFlux(wlm, wrm, cs, fm);
+      HlleIsoFlux(wlp, wrp, cs, fp);
+      double r[4];
+#pragma unroll
+      for (int v=0; v<4; ++v) r[v] = (fp[v] - fm[v])/dx;
+      // back to the gas components; the axes add in the order x1, x2, x3
+      dv[0] += r[0];
+      dv[1] += (d == 0) ? r[1] : ((d == 1) ? r[3] : r[2]);
+      dv[2] += (d == 0) ? r[2] : ((d == 1) ? r[1] : r[3]);
+      dv[3] += (d == 0) ? r[3] : ((d == 1) ? r[2] : r[1]);
+    }
+#pragma unroll
+    for (int c=0; c<4; ++c) divf[at + c*ncb] = dv[c];
+  }
+}
+
+// Hydro::RKUpdate (zero fluxes, or minus beta dt divf when divf is given) + HydroSrcTerms
+// (see dustk.h dk_gas_src)
 __global__ void k_gas_hook(Geo g, const DevBlock *B, double *u0, const double *u1, double gam0,
-                           double gam1, double bdt, int use_src, dk_gas_src src) {
+                           double gam1, double bdt, int use_src, dk_gas_src src,
+                           const double *divf) {
   const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
   GRID_STRIDE(t, n) {
     long long rr = t;
@@ -739,12 +864,19 @@ __global__ void k_gas_hook(Geo g, const DevBlock *B, double *u0, const double *u
     const int k = g.ks + static_cast<int>(rr % g.nx3);
     const int m = static_cast<int>(rr/g.nx3);
     double w[4];
-    const double den = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+    double den = u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)];
+    if (divf != nullptr && den < src.dfloor) {
+      den = src.dfloor;
+      u0[Idx(g,m,DK_NGAS,DK_IDN,k,j,i)] = den;
+    }
     w[0] = den;
     for (int c=1; c<4; ++c) w[c] = u0[Idx(g,m,DK_NGAS,c,k,j,i)]/den;
     for (int v=0; v<DK_NGAS; ++v) {
       double &u = u0[Idx(g,m,DK_NGAS,v,k,j,i)];
-      u = gam0*u + gam1*u1[Idx(g,m,DK_NGAS,v,k,j,i)];
+      if (divf != nullptr)
+        u = gam0*u + gam1*u1[Idx(g,m,DK_NGAS,v,k,j,i)] - bdt*divf[Idx(g,m,DK_NGAS,v,k,j,i)];
+      else
+        u = gam0*u + gam1*u1[Idx(g,m,DK_NGAS,v,k,j,i)];
     }
     if (!use_src) continue;
     if (src.const_accel != 0.0)
@@ -1226,15 +1358,49 @@ int AdaptiveSolve(dk_ctx *c, double a_dt, double sh, double outer_dt, double t) 
 
 namespace dk {
 
+// The options the explicit gas update supports (dk_gas_src in dustk.h)
+int HydroSupported(const dk_ctx *c, const dk_gas_src *src) {
+  if (!(src->iso_cs > 0.0)) return DK_EINVAL;
+  if (c->smr || c->g.ng < 2 || c->g.bc_x1 != DK_X1_PERIODIC || c->g.bc_x3 == DK_X3_REFLECT)
+    return DK_EUNSUPPORTED;
+  return DK_OK;
+}
+
+// the flux divergence field (allocated once, outside any stream capture)
+int HydroField(dk_ctx *c) {
+  if (c->hdivf != nullptr) return DK_OK;
+  if (hipMalloc(&c->hdivf, sizeof(double)*DK_NGAS*NCells(c)) != hipSuccess) {
+    (void)hipGetLastError();
+    c->hdivf = nullptr;
+    return DK_ENOMEM;
+  }
+  return DK_OK;
+}
+
 int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   const int s = st->stage - 1;
   const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
+  const bool hydro = (src != nullptr && src->hydro);
+  int e;
+  if (hydro && (e = HydroSupported(c, src))) return e;
   if (bdt == 0.0 && gam0 == 1.0 && gam1 == 0.0) return DK_OK;   // identity stage
   dk_gas_src z;
   std::memset(&z, 0, sizeof(z));
+  if (hydro) {
+    // the u0 ghosts of the stage start (the previous stage's Hydro::SendU/RecvU and
+    // ApplyPhysicalBCs; x1 is periodic, so the time is not read), then the flux divergence
+    if ((e = HydroField(c))) return e;
+    if ((e = FillGhosts(c, c->u0, DK_NGAS, false, st->time))) return e;
+    {const int _pt = dk::ProfBegin(c, "hydro_divf");
+    k_hydro_divf<<<CGRID(ActiveCells(c))>>>(c->g, c->d_blocks, c->u0, c->hdivf, src->iso_cs,
+                                            src->dfloor);
+     dk::ProfEnd(c, _pt);}
+    if ((e = Launched(c))) return e;
+  }
   {const int _pt = dk::ProfBegin(c, "gas_hook");
   k_gas_hook<<<CGRID(ActiveCells(c))>>>(c->g, c->d_blocks, c->u0, c->u1, gam0, gam1, bdt,
-                                        src ? 1 : 0, src ? *src : z);
+                                        src ? 1 : 0, src ? *src : z,
+                                        hydro ? c->hdivf : nullptr);
    dk::ProfEnd(c, _pt);}
   return Launched(c);
 }
@@ -1483,7 +1649,8 @@ int dk_destroy(dk_ctx *c) {
                   (void*)c->dmom, (void*)c->ap, (void*)c->resid, c->sort_tmp,
                   (void*)c->sort_keys, (void*)c->sort_idx, (void*)c->sort_rbuf,
                   (void*)c->sort_ibuf, (void*)c->gpack, (void*)c->cvel, (void*)c->solver_r,
-                  (void*)c->solver_p, (void*)c->d_leafmap, (void*)c->d_block_off})
+                  (void*)c->solver_p, (void*)c->d_leafmap, (void*)c->d_block_off,
+                  (void*)c->hdivf})
     if (p) (void)hipFree(p);
   delete c;
   return DK_OK;
@@ -1804,7 +1971,12 @@ int StageGraph(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
 
 int dk_stage_run(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   if (c == nullptr || st == nullptr) return DK_EINVAL;
-  int e = c->fast ? dk::FastTuneBegin(c, st) : DK_OK;
+  // a stage with isothermal fluxes is checked before any of its work and allocates its field
+  // before a graph capture can begin
+  int e = DK_OK;
+  if (src != nullptr && src->hydro && c->u0 != nullptr && !(e = dk::HydroSupported(c, src)))
+    e = dk::HydroField(c);
+  if (!e && c->fast) e = dk::FastTuneBegin(c, st);
   if (!e) e = c->graphs ? StageGraph(c, st, src) : StageBody(c, st, src);
   if (!e && c->fast) e = dk::FastTuneEnd(c, st);
   return e;

@@ -116,6 +116,9 @@ struct dk_ctx {
   double *u0 = nullptr, *u1 = nullptr;
   // library-owned work fields (layout of the gas, nvar components)
   double *qdep = nullptr, *ustar = nullptr, *dmom = nullptr, *ap = nullptr, *resid = nullptr;
+  // the explicit gas update's flux divergence (4 components), allocated by the first stage
+  // with isothermal fluxes (dk_gas_src.hydro)
+  double *hdivf = nullptr;
   // refinable clouds: the packed gather field (m, k, j, i, 12) and a velocity field (3)
   double *gpack = nullptr, *cvel = nullptr;
   // the strict solver (pcg, adaptive): residual and search direction (3 components), the
@@ -197,6 +200,9 @@ template <typename T> int StiffnessRecord(dk_ctx *c, const T *q, int nv, int com
 
 // dk_fp64.hip pieces reused by the FP32 form
 int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);
+// the explicit gas update's options check and its flux divergence field (allocated on first use)
+int HydroSupported(const dk_ctx *c, const dk_gas_src *src);
+int HydroField(dk_ctx *c);
 
 }  // namespace dk
 

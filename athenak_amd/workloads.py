@@ -16,6 +16,7 @@ device memory.
       particles (the uneven-block case in a form both layouts run)
   c5  128^3 periodic in 8 blocks of 64^3, ppc 8, two species (tau_s 0.01 and 1, eps 0.5
       each), half the particles in 64 Gaussian clumps of width 2 cells, imex2+/dc1, 100 steps
+  c1h c1's box at 256^3 in 64 blocks with the isothermal gas update (PLM + HLLE fluxes, cs 1)
 """
 
 import dataclasses
@@ -169,6 +170,27 @@ def c5(n=128, nbx=2, ppc=8, steps=100, seed=1):
     su = _setup(seed=seed, ppc=ppc, taus=(0.01, 1.0), eps=(0.5, 0.5), gas_noise=1e-3,
                 dust_noise=1e-3, nclump=64, clump_frac=0.5, clump_sigma=2.0 / n)
     return Workload("c5", mesh, blocks, _phys(), su, None, 1.0e-3, steps)
+
+
+FLT_MIN = 1.17549435e-38   # the density floor's default (<hydro>/dfloor)
+
+
+def with_hydro(w, cs=1.0, dfloor=FLT_MIN):
+    """w with the explicit gas update's isothermal PLM + HLLE fluxes (dk_gas_src.hydro, sound
+    speed cs) on top of its gas source terms; its name gains an "h"."""
+    src = abi.dk_gas_src()
+    if w.src is not None:
+        for f, _ in abi.dk_gas_src._fields_:
+            setattr(src, f, getattr(w.src, f))
+    src.hydro, src.iso_cs, src.dfloor = 1, cs, dfloor
+    return dataclasses.replace(w, name=w.name + "h", src=src)
+
+
+def c1h(n=256, nb=4, ppc=1, steps=50, coupling=abi.DK_IMEX, solver=abi.DK_DC1, seed=1):
+    """c1's box at n^3 in nb^3 blocks with the isothermal gas update (cs = 1; dt 1e-3 is a
+    Courant number of 0.256 at 256^3): the gas moves by its own fluxes as well as the drag."""
+    return with_hydro(c1(n=n, nb=nb, ppc=ppc, steps=steps, coupling=coupling, solver=solver,
+                         seed=seed))
 
 
 def c4u(n=64, nbx=4, ppc=4, steps=50, seed=1, clump_frac=0.25, clouds=True):

@@ -143,15 +143,25 @@ typedef struct dk_stage {
   double a_impl;            /* imex2+ gamma */
 } dk_stage;
 
-/* Gas source terms of the harness's explicit gas update (a stand-in for Hydro::RKUpdate with
- * zero fluxes plus Hydro::HydroSrcTerms, src/hydro/hydro_update.cpp:23-83,
- * src/hydro/hydro_tasks.cpp:268-297). */
+/* The explicit gas update of a stage: Hydro::RKUpdate (src/hydro/hydro_update.cpp:23-83)
+ * plus Hydro::HydroSrcTerms (src/hydro/hydro_tasks.cpp:268-297).  hydro = 0: zero fluxes (the
+ * gas moves only by the source terms and the drag).  hydro = 1: the flux divergence of an
+ * isothermal gas (Hydro::Fluxes, src/hydro/hydro_fluxes.cpp, with reconstruct = plm,
+ * src/reconstruct/plm.hpp, and rsolver = hlle, src/hydro/rsolvers/hlle_hyd.hpp) from the
+ * primitives of u0 at the stage start (IsothermalHydro::ConsToPrim with the density floor,
+ * src/eos/isothermal_hyd.cpp:34-110); the stage first refills the u0 ghosts.  It needs at least
+ * 2 ghost cells, a uniform mesh, periodic x1 and periodic or outflow x3 (DK_EUNSUPPORTED
+ * otherwise) and iso_cs > 0 (DK_EINVAL); the first stage with hydro allocates the flux
+ * divergence field (4 components, the gas layout). */
 typedef struct dk_gas_src {
   int32_t shearing_box;     /* ShearingBoxCC::SourceTermsCC (3-D) */
   int32_t stratified;
   double omega0, qshear;
   double const_accel;       /* SourceTerms::ConstantAccel value (0 = off) */
   int32_t const_accel_dir;  /* 1..3 */
+  int32_t hydro;            /* 1: PLM + HLLE fluxes of an isothermal gas */
+  double iso_cs;            /* <hydro>/iso_sound_speed */
+  double dfloor;            /* <hydro>/dfloor (the reference's default is FLT_MIN) */
 } dk_gas_src;
 
 typedef struct dk_ctx dk_ctx;
@@ -279,7 +289,8 @@ int dk_set_tune_config(dk_ctx *ctx, int32_t xcd, int32_t key_threads);
 
 /* ---- gas stand-ins of the harness (not part of the reference's dust path) --------------- */
 
-/* Hydro::RKUpdate with zero fluxes + Hydro::HydroSrcTerms (see dk_gas_src). */
+/* Hydro::RKUpdate (zero fluxes, or the isothermal PLM + HLLE fluxes when src->hydro) +
+ * Hydro::HydroSrcTerms (see dk_gas_src). */
 int dk_gas_hook(dk_ctx *ctx, const dk_stage *st, const dk_gas_src *src);
 /* Hydro::SendU/RecvU: periodic (and shear-periodic) copy of the u0 ghost cells. */
 int dk_fill_gas_ghosts(dk_ctx *ctx, const dk_stage *st);

@@ -2295,14 +2295,137 @@ int or_add_drag_history_gas(or_sim *s, const dk_stage *st) {
   return DK_OK;
 }
 
-// Hydro::RKUpdate with zero fluxes (src/hydro/hydro_update.cpp:23-83) and HydroSrcTerms
-// (src/hydro/hydro_tasks.cpp:268-297: ConstantAccel srcterms.cpp:128-145, ShearingBoxCC::
-// SourceTermsCC shearing_box_srcterms.cpp:26-60), primitives from u0 at the stage start
+namespace {
+
+// reconstruct/plm.hpp:13-28: the limited slope of q_i (van Leer's harmonic mean on a uniform
+// mesh) gives the left state of face i+1/2 and the right state of face i-1/2
+void Plm(double qm, double q0, double qp, double &ql_next, double &qr_here) {
+  const double dql = q0 - qm, dqr = qp - q0;
+  const double dq2 = dql*dqr;
+  double dqm = dq2/(dql + dqr);
+  if (dq2 <= 0.0) dqm = 0.0;
+  ql_next = q0 + dqm;
+  qr_here = q0 - dqm;
+}
+
+// src/hydro/rsolvers/hlle_hyd.hpp:22-150 with the isothermal EOS: L/R states (d, v_n, v_t1,
+// v_t2) in the face's frame, the flux in the same order
+void HlleIso(const double wl[4], const double wr[4], double cs, double f[4]) {
+  const double sdl = std::sqrt(wl[0]), sdr = std::sqrt(wr[0]);
+  const double isd = 1.0/(sdl + sdr);
+  const double vroe = (sdl*wl[1] + sdr*wr[1])*isd;
+  const double al = std::fmin(vroe - cs, wl[1] - cs);
+  const double ar = std::fmax(vroe + cs, wr[1] + cs);
+  const double bp = (ar > 0.0) ? ar : 1.0e-20;
+  const double bm = (al < 0.0) ? al : -1.0e-20;
+  const double qa = wl[1] - bm, qb = wr[1] - bp;
+  double fl[4], fr[4];
+  fl[0] = wl[0]*qa;
+  fr[0] = wr[0]*qb;
+  for (int c=1; c<4; ++c) {
+    fl[c] = wl[0]*wl[c]*qa;
+    fr[c] = wr[0]*wr[c]*qb;
+  }
+  fl[1] += (cs*cs)*wl[0];
+  fr[1] += (cs*cs)*wr[0];
+  double h = 0.0;
+  if (bp != bm) h = 0.5*(bp + bm)/(bp - bm);
+  for (int c=0; c<4; ++c) f[c] = 0.5*(fl[c] + fr[c]) + h*(fl[c] - fr[c]);
+}
+
+// The options the explicit gas update supports (dk_gas_src in dustk.h)
+int HydroSupported(const or_sim *s, const dk_gas_src *src) {
+  if (!(src->iso_cs > 0.0)) return DK_EINVAL;
+  if (s->smr || s->ng < 2 || s->M.bc_x1 != DK_X1_PERIODIC || s->M.bc_x3 == DK_X3_REFLECT)
+    return DK_EUNSUPPORTED;
+  return DK_OK;
+}
+
+// Hydro::Fluxes (src/hydro/hydro_fluxes.cpp:20-140, plm + hlle) and the flux divergence of
+// Hydro::RKUpdate (src/hydro/hydro_update.cpp:55-77) from the primitives of u0, ghosts
+// included (IsothermalHydro::ConsToPrim, src/eos/isothermal_hyd.cpp:34-110; the density floor
+// is applied where a density is read): divf (4 components, the gas layout, active cells)
+void HydroDivf(or_sim *s, const dk_gas_src *src, std::vector<double> &divf) {
+  const int nmb = s->M.nmb;
+  const int64_t nc = (int64_t)nmb*NG_VARS*s->ncb;
+  std::vector<double> w(nc);
+  for (int m=0; m<nmb; ++m)
+    for (int k=0; k<s->n3; ++k)
+      for (int j=0; j<s->n2; ++j)
+        for (int i=0; i<s->n1; ++i) {
+          double d = s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+          if (d < src->dfloor) d = src->dfloor;
+          const double di = 1.0/d;
+          w[s->Idx(m,NG_VARS,0,k,j,i)] = d;
+          for (int c=1; c<4; ++c)
+            w[s->Idx(m,NG_VARS,c,k,j,i)] = di*s->u0[s->Idx(m,NG_VARS,c,k,j,i)];
+        }
+  // face fluxes: flx[d] at face index f of axis d lies between cells f-1 and f
+  std::vector<double> flx[3];
+  const int64_t stride[3] = {1, s->n1, (int64_t)s->n1*s->n2};
+  for (int d=0; d<3; ++d) {
+    flx[d].assign(nc, 0.0);
+    const int iu = s->ie + (d == 0), ju = s->je + (d == 1), ku = s->ke + (d == 2);
+    for (int m=0; m<nmb; ++m)
+      for (int k=s->ks; k<=ku; ++k)
+        for (int j=s->js; j<=ju; ++j)
+          for (int i=s->is; i<=iu; ++i) {
+            double wl[4], wr[4], f[4], dummy;
+            for (int v=0; v<4; ++v) {
+              // the face frame's component v: density, then v_d, v_d+1, v_d+2
+              const int c = (v == 0) ? 0 : 1 + (d + v - 1) % 3;
+              const int64_t at = s->Idx(m,NG_VARS,c,k,j,i);
+              const int64_t sd = stride[d];
+              Plm(w[at - 2*sd], w[at - sd], w[at], wl[v], dummy);
+              Plm(w[at - sd], w[at], w[at + sd], dummy, wr[v]);
+            }
+            HlleIso(wl, wr, src->iso_cs, f);
+            for (int v=0; v<4; ++v) {
+              const int c = (v == 0) ? 0 : 1 + (d + v - 1) % 3;
+              flx[d][s->Idx(m,NG_VARS,c,k,j,i)] = f[v];
+            }
+          }
+  }
+  divf.assign(nc, 0.0);
+  for (int m=0; m<nmb; ++m) {
+    const double dx[3] = {s->dx1[m], s->dx2[m], s->dx3[m]};
+    for (int v=0; v<NG_VARS; ++v)
+      for (int k=s->ks; k<=s->ke; ++k)
+        for (int j=s->js; j<=s->je; ++j)
+          for (int i=s->is; i<=s->ie; ++i) {
+            const int64_t at = s->Idx(m,NG_VARS,v,k,j,i);
+            double dv = (flx[0][at + stride[0]] - flx[0][at])/dx[0];
+            dv += (flx[1][at + stride[1]] - flx[1][at])/dx[1];
+            dv += (flx[2][at + stride[2]] - flx[2][at])/dx[2];
+            divf[at] = dv;
+          }
+  }
+}
+
+}  // namespace
+
+// Hydro::RKUpdate (src/hydro/hydro_update.cpp:23-83; zero fluxes unless src->hydro) and
+// HydroSrcTerms (src/hydro/hydro_tasks.cpp:268-297: ConstantAccel srcterms.cpp:128-145,
+// ShearingBoxCC::SourceTermsCC shearing_box_srcterms.cpp:26-60), primitives from u0 at the
+// stage start
 int or_gas_hook(or_sim *s, const dk_stage *st, const dk_gas_src *src) {
   const int sidx = st->stage - 1;
   const double gam0 = st->gam0[sidx], gam1 = st->gam1[sidx];
   const double bdt = st->beta[sidx]*st->dt;
+  const bool hydro = (src != nullptr && src->hydro);
+  if (hydro) {
+    const int e = HydroSupported(s, src);
+    if (e) return e;
+  }
   if (bdt == 0.0 && gam0 == 1.0 && gam1 == 0.0) return DK_OK;   // identity stage
+  std::vector<double> divf;
+  if (hydro) {
+    // the u0 ghosts of the stage start (the previous stage's Hydro::SendU/RecvU and
+    // ApplyPhysicalBCs), then the flux divergence
+    CopyGhosts(s, s->u0, NG_VARS);
+    X3BCs(s, s->u0, NG_VARS, false);
+    HydroDivf(s, src, divf);
+  }
   const int nmb = s->M.nmb;
 #pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
   for (int m=0; m<nmb; ++m)
@@ -2310,12 +2433,20 @@ int or_gas_hook(or_sim *s, const dk_stage *st, const dk_gas_src *src) {
       for (int j=s->js; j<=s->je; ++j)
         for (int i=s->is; i<=s->ie; ++i) {
           double w[4];
-          const double den = s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+          double den = s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)];
+          if (hydro && den < src->dfloor) {
+            den = src->dfloor;
+            s->u0[s->Idx(m,NG_VARS,DK_IDN,k,j,i)] = den;
+          }
           w[0] = den;
           for (int c=1; c<4; ++c) w[c] = s->u0[s->Idx(m,NG_VARS,c,k,j,i)]/den;
           for (int v=0; v<NG_VARS; ++v) {
             double &u = s->u0[s->Idx(m,NG_VARS,v,k,j,i)];
-            u = gam0*u + gam1*s->u1[s->Idx(m,NG_VARS,v,k,j,i)];
+            if (hydro)
+              u = gam0*u + gam1*s->u1[s->Idx(m,NG_VARS,v,k,j,i)] -
+                  bdt*divf[s->Idx(m,NG_VARS,v,k,j,i)];
+            else
+              u = gam0*u + gam1*s->u1[s->Idx(m,NG_VARS,v,k,j,i)];
           }
           if (src == nullptr) continue;
           if (src->const_accel != 0.0) {

@@ -76,7 +76,12 @@ def compare(o, g, w, vtol=1e-11, xtol=1e-12, what=""):
     ng = w.mesh.ng
     uo = o.u0()[:, :, ng:-ng, ng:-ng, ng:-ng]
     ug = u0g[:, :, ng:-ng, ng:-ng, ng:-ng]
-    np.testing.assert_array_equal(uo[:, 0], ug[:, 0])
+    if w.src is not None and w.src.hydro:
+        # with the gas fluxes on, the density follows the momenta, whose deposits sum in
+        # another order: the same relative tolerance
+        assert np.abs(uo[:, 0] - ug[:, 0]).max() <= vtol * np.abs(uo[:, 0]).max(), what
+    else:
+        np.testing.assert_array_equal(uo[:, 0], ug[:, 0])
     mscale = np.abs(uo[:, 1:]).max()
     assert np.abs(uo[:, 1:] - ug[:, 1:]).max() <= vtol * max(mscale, 1e-300), what
 
@@ -97,7 +102,10 @@ def run(w, steps, vtol=1e-11, check_every=None):
     d1 = g.diag()
     drift = np.abs((d1[0:3] + d1[3:6]) - (d0[0:3] + d0[3:6])).max() / d0[8]
     say("%s: %d steps, momentum drift %.3e" % (w.name, steps, drift))
-    if not w.phys.shearing_box:   # (a shearing box has external forces: no invariant)
+    # (no invariant in a shearing box, which has external forces, or when gas flows through
+    # open x3 faces)
+    open_faces = (w.src is not None and w.src.hydro and w.mesh.bc_x3 != abi.DK_X3_PERIODIC)
+    if not w.phys.shearing_box and not open_faces:
         assert drift < 1e-13
     return o, g
 
