@@ -186,10 +186,10 @@ def with_hydro(w, cs=1.0, dfloor=FLT_MIN):
     return dataclasses.replace(w, name=w.name + "h", src=src)
 
 
-def c1h(n=256, nb=4, ppc=1, steps=50, coupling=abi.DK_IMEX, solver=abi.DK_DC1, seed=1):
-    """c1's box at n^3 in nb^3 blocks with the isothermal gas update (cs = 1; dt 1e-3 is a
+def c1h(n=256, nbx=4, ppc=1, steps=50, coupling=abi.DK_IMEX, solver=abi.DK_DC1, seed=1):
+    """c1's box at n^3 in nbx^3 blocks with the isothermal gas update (cs = 1; dt 1e-3 is a
     Courant number of 0.256 at 256^3): the gas moves by its own fluxes as well as the drag."""
-    return with_hydro(c1(n=n, nb=nb, ppc=ppc, steps=steps, coupling=coupling, solver=solver,
+    return with_hydro(c1(n=n, nb=nbx, ppc=ppc, steps=steps, coupling=coupling, solver=solver,
                          seed=seed))
 
 
@@ -259,4 +259,4 @@ def c4b(n=128, nbx=8, ppc=4, steps=50, seed=1, sigma=1.5):
 
 
 WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c4u": c4u, "c4p": c4p, "c4b": c4b,
-             "c5": c5}
+             "c5": c5, "c1h": c1h}

@@ -9,6 +9,8 @@ over [1e-3, 1] with one particle per bin per cell (117,440,512 particles), x3 ou
 imex2+/dc1.  `--workload c2`: the unstratified shearing box at ppc 4 (67,108,864 particles).
 `--workload c4p`: a step is one rk2 cycle of the pc2 coupling (predictor, advance) on 128^3
 cells in 512 blocks of 16^3 with clumps (8,388,608 point particles, stiff cells damped).
+`--workload c1h`: 256^3 periodic in 64 blocks, ppc 1, imex2+/dc1, with the explicit gas update's
+isothermal PLM + HLLE fluxes (the JSON line adds the gas update's own roofline).
 
 Prints one JSON line: value = Mparticle-steps/s of the measured form, the roofline of the
 dominant kernel (algorithmic bytes per launch / measured average launch time, HIP events on the
@@ -86,10 +88,34 @@ KERNEL_BYTES_PC2 = {
 }
 
 
+# The explicit gas update with isothermal fluxes (dk_gas_src.hydro), both forms, per active
+# cell: the divergence kernel reads u0 (4 values; its 13-cell stencil is served from the caches)
+# and writes divf (4); the RK combination reads u0, u1 and divf and writes u0 (4 each)
+GAS_UPDATE_BYTES = {"hydro_divf": 8 * 8, "gas_hook": 16 * 8}
+
+
 def kernel_bytes(label, form, w):
+    if w.src is not None and w.src.hydro and label in GAS_UPDATE_BYTES:
+        return (0, GAS_UPDATE_BYTES[label])
     if w.phys.coupling == 1 and label in KERNEL_BYTES_PC2[form]:
         return KERNEL_BYTES_PC2[form][label]
     return KERNEL_BYTES.get(label, (0, 0))
+
+
+def gas_update_roofline(r, w, form):
+    """The achieved bandwidth of the gas update's kernels (hydro workloads only)."""
+    if w.src is None or not w.src.hydro:
+        return None
+    out = {}
+    for label in GAS_UPDATE_BYTES:
+        if label not in r["prof"]:
+            continue
+        tot, cnt = r["prof"][label]
+        alg = kernel_bytes(label, form, w)[1] * w.ncells
+        gbs = alg / (tot / cnt * 1e-3) / 1e9
+        out[label] = {"avg_launch_ms": round(tot / cnt, 4), "alg_bytes_per_launch": alg,
+                      "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def parse():
@@ -441,6 +467,7 @@ def main():
                           "achieved": round(mps * 1e6 * step_alg / 1e9, 1), "unit": "GB/s",
                           "frac": round(mps * 1e6 * step_alg / 1e9 / HBM_PEAK_GBS, 4)},
         "momentum_drift": drift_or_none(w, r["drift"]),
+        "gas_update_roofline": gas_update_roofline(r, w, main_form),
         "particles_timed": r["n_timed"],
         "graphs": r["graphs"],
         "autotune": r["tuned"],
@@ -469,6 +496,7 @@ def main():
                                 mps64 * 1e6 * balg_step("fp64", w.ppc, pc2, bool(w.phys.clouds)) /
                                 1e9 / HBM_PEAK_GBS, 4),
                             "roofline": kernel_roofline(res["fp64"], w, "fp64"),
+                            "gas_update_roofline": gas_update_roofline(res["fp64"], w, "fp64"),
                             "momentum_drift": drift_or_none(w, res["fp64"]["drift"])}
         out["speedup_vs_fp64"] = round(mps / mps64, 3)
         out["max_rel_velocity_err_vs_fp64"] = verr

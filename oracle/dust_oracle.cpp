@@ -2349,6 +2349,7 @@ void HydroDivf(or_sim *s, const dk_gas_src *src, std::vector<double> &divf) {
   const int nmb = s->M.nmb;
   const int64_t nc = (int64_t)nmb*NG_VARS*s->ncb;
   std::vector<double> w(nc);
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
   for (int m=0; m<nmb; ++m)
     for (int k=0; k<s->n3; ++k)
       for (int j=0; j<s->n2; ++j)
@@ -2366,6 +2367,7 @@ void HydroDivf(or_sim *s, const dk_gas_src *src, std::vector<double> &divf) {
   for (int d=0; d<3; ++d) {
     flx[d].assign(nc, 0.0);
     const int iu = s->ie + (d == 0), ju = s->je + (d == 1), ku = s->ke + (d == 2);
+#pragma omp parallel for collapse(2) num_threads(s->nthreads) schedule(static)
     for (int m=0; m<nmb; ++m)
       for (int k=s->ks; k<=ku; ++k)
         for (int j=s->js; j<=ju; ++j)
@@ -2387,6 +2389,7 @@ void HydroDivf(or_sim *s, const dk_gas_src *src, std::vector<double> &divf) {
           }
   }
   divf.assign(nc, 0.0);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
   for (int m=0; m<nmb; ++m) {
     const double dx[3] = {s->dx1[m], s->dx2[m], s->dx3[m]};
     for (int v=0; v<NG_VARS; ++v)
