@@ -69,6 +69,15 @@ class dk_gas_src(C.Structure):
                 ("hydro", C.c_int32), ("iso_cs", C.c_double), ("dfloor", C.c_double)]
 
 
+class dk_output_info(C.Structure):
+    _fields_ = [("time", C.c_double), ("dt", C.c_double), ("cycle", C.c_int64),
+                ("file_number", C.c_int32), ("single_precision", C.c_int32),
+                ("stopping_time_mode", C.c_int32), ("nspecies", C.c_int32),
+                ("taus", C.c_double * 8),
+                ("basename", C.c_char_p), ("output_block", C.c_char_p),
+                ("source_commit", C.c_char_p), ("input_dump", C.c_char_p)]
+
+
 class dk_setup(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("ppc", C.c_int32), ("nspecies", C.c_int32),
                 ("taus", C.c_double * DK_MAXSPEC), ("eps", C.c_double * DK_MAXSPEC),

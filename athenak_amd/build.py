@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdustk.so")
 SOURCES = ["dk_fp64.hip", "dk_fp32.hip", "dk_setup.hip", "dk_prof.hip", "dk_mesh.hip",
-           "dk_clouds.hip"]
+           "dk_clouds.hip", "dk_output.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: no fused multiply-adds, so that the FP64 form rounds as the oracle does
@@ -16,7 +16,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 PER_FILE = {"dk_fp64.hip": ["-ffp-contract=off"], "dk_setup.hip": ["-ffp-contract=off"],
             "dk_fp32.hip": [], "dk_prof.hip": [], "dk_mesh.hip": ["-ffp-contract=off"],
-            "dk_clouds.hip": ["-ffp-contract=off"]}
+            "dk_clouds.hip": ["-ffp-contract=off"], "dk_output.hip": ["-ffp-contract=off"]}
 
 
 def build(verbose=False, jobs=3):

@@ -545,6 +545,9 @@ __global__ void k_stiff_record(const double *partial, int nblk, double dt, doubl
   rec[0] = chi;
   rec[1] = (rec[1] < chi) ? chi : rec[1];
   rec[2] = r;
+  // the interval records of the particle history (since its last row; -1: none yet)
+  rec[5] = (rec[5] < chi) ? chi : rec[5];
+  if (r > 0.0) rec[6] = (rec[6] < r) ? r : rec[6];
 }
 
 // RefreshStoppingTimeMaximum (dust_newdt.cpp:34-98): per workgroup the largest and smallest
@@ -1502,6 +1505,9 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
     d.dx1 = (b.x1max - b.x1min)/static_cast<double>(g.nx1);   // meshblock.cpp:113-118
     d.dx2 = (b.x2max - b.x2min)/static_cast<double>(g.nx2);
     d.dx3 = (b.x3max - b.x3min)/static_cast<double>(g.nx3);
+    c->dx1.push_back(d.dx1);
+    c->dx2.push_back(d.dx2);
+    c->dx3.push_back(d.dx3);
     d.sx1 = static_cast<double>(g.nx1)/(b.x1max - b.x1min);
     d.sx2 = static_cast<double>(g.nx2)/(b.x2max - b.x2min);
     d.sx3 = static_cast<double>(g.nx3)/(b.x3max - b.x3min);
@@ -1565,6 +1571,10 @@ int dk_create(const dk_mesh *mesh, const dk_block *blocks, const dk_physics *phy
   if (!e) e = dk::Check(hipMemcpy(c->d_slot_dir, sd, sizeof(sd), hipMemcpyHostToDevice));
   if (!e) e = dk::Check(hipMemset(c->d_err, 0, sizeof(dk::ErrWord)));
   if (!e) e = dk::Check(hipMemset(c->d_scalar, 0, sizeof(double)*16));
+  if (!e) {   // d_scalar[14..15]: the particle history's interval records, none yet
+    const double none[2] = {-1.0, -1.0};
+    e = dk::Check(hipMemcpy(c->d_scalar + 14, none, sizeof(none), hipMemcpyHostToDevice));
+  }
   if (!e && form == DK_FORM_FP64) {
     for (double *p : {c->qdep, c->ustar, c->dmom, c->ap, c->resid}) {
       const long long nv = (p == c->qdep) ? 5 : ((p == c->resid) ? 4 : 3);

@@ -102,7 +102,7 @@ struct dk_ctx {
   dk_mesh M;
   dk_physics P;
   std::vector<dk_block> B;
-  std::vector<double> dx1, dx2, dx3;
+  std::vector<double> dx1, dx2, dx3;   // per block (MeshBlock::mb_size dx1..dx3)
   int form = DK_FORM_FP64;
   hipStream_t stream = nullptr;
   int device = 0;

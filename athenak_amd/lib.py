@@ -25,7 +25,8 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_set_solver", "dk_solver_stats",
             "dk_set_tune_config", "dk_graph_launches", "dk_stiffness",
             "dk_stopping_times", "dk_remove_outflow_particles", "dk_x3_removed",
-            "dk_sort_state", "dk_sort_particles_noalloc"] +
+            "dk_sort_state", "dk_sort_particles_noalloc", "dk_write_pbin",
+            "dk_particle_history", "dk_write_phst"] +
            ["dk_" + t for t in TASKS])
 
 
@@ -82,6 +83,10 @@ def lib():
         L.dk_prof_reset.argtypes = [P]
         L.dk_prof_read.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32,
                                    C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        info = C.POINTER(abi.dk_output_info)
+        L.dk_write_pbin.argtypes = [P, C.c_char_p, info]
+        L.dk_particle_history.argtypes = [P, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.dk_write_phst.argtypes = [P, C.c_char_p, info, C.POINTER(C.c_double), C.c_int32]
         _LIB = L
     return _LIB
 
@@ -258,6 +263,39 @@ class DustContext:
         if stiff_thresh > 0.0 and rate > 0.0 and self.phys.coupling == abi.DK_PC2:
             dt = min(dt, stiff_thresh / rate)
         return dt
+
+    # ---- science outputs ----
+    @staticmethod
+    def output_info(time, dt, cycle=0, file_number=0, taus=(0.1,), stopping_time_mode=0,
+                    single=False, basename="dust", output_block="output2",
+                    source_commit=None, input_dump=""):
+        """A dk_output_info (the strings are kept alive on the returned object)."""
+        info = abi.dk_output_info(time=time, dt=dt, cycle=cycle, file_number=file_number,
+                                  single_precision=int(bool(single)),
+                                  stopping_time_mode=stopping_time_mode, nspecies=len(taus))
+        for k, t in enumerate(taus):
+            info.taus[k] = t
+        info._keep = [x.encode() if isinstance(x, str) else x
+                      for x in (basename, output_block, source_commit, input_dump)]
+        info.basename, info.output_block, info.source_commit, info.input_dump = info._keep
+        return info
+
+    def write_pbin(self, path, info):
+        """ParticleBinaryOutput: the pbin file (format version 2) of the current particles."""
+        self._check(lib().dk_write_pbin(self.h, os.fsencode(path), C.byref(info)),
+                    "dk_write_pbin")
+
+    def particle_history(self, time, stiff_thresh=0.0):
+        """DustGasDrag::ParticleHistory: the 14 values of a phst row after time and dt."""
+        row = (C.c_double * 14)()
+        self._check(lib().dk_particle_history(self.h, time, stiff_thresh, row),
+                    "dk_particle_history")
+        return list(row)
+
+    def write_phst(self, path, info, row, header=False):
+        r = (C.c_double * 14)(*row)
+        self._check(lib().dk_write_phst(self.h, os.fsencode(path), C.byref(info), r,
+                                        int(bool(header))), "dk_write_phst")
 
     def graphs_captured(self):
         """Stage graphs launched so far (0 on the null stream, where stages run eagerly)."""

@@ -339,6 +339,42 @@ int dk_error(dk_ctx *ctx, int32_t *code, int64_t *particle);
 void *dk_field(dk_ctx *ctx, int32_t which, int32_t *nvar);
 const char *dk_strerror(int code);
 
+/* ---- science outputs: the particle file (pbin) and the particle history (phst) ---------- */
+
+/* What the outputs record beside the particles: Mesh::time, Mesh::dt, Mesh::ncycle, the output
+ * block's file_number and precision, the species table, and the run's strings (NULL: empty;
+ * source_commit NULL: "unknown"; input_dump: the ParameterInput::ParameterDump text that a
+ * pbin file carries as its [input] section). */
+typedef struct dk_output_info {
+  double time, dt;
+  int64_t cycle;
+  int32_t file_number;
+  int32_t single_precision;   /* pbin <outputN>/precision = single: particle reals as float32 */
+  int32_t stopping_time_mode; /* 0 species_fixed, 1 particle_static, 2 dynamic */
+  int32_t nspecies;           /* 1..8 */
+  double taus[8];             /* the species' stopping times */
+  const char *basename, *output_block, *source_commit, *input_dump;
+} dk_output_info;
+
+/* ParticleBinaryOutput::WriteOutputFile (src/outputs/pbin.cpp:6-60 the format, :450-1026):
+ * writes the active particles (mass != 0) and the blocks of the context to `path`, format
+ * version 2 (raw columns; species, stopping time and mass as [constants] under the reference's
+ * rules), as path.tmp renamed when complete.  One rank.  The FP32 form exports its particles
+ * first.  Waits for the stream. */
+int dk_write_pbin(dk_ctx *ctx, const char *path, const dk_output_info *info);
+/* DustGasDrag::ParticleHistory (src/dust/dust_science.cpp:372-484): row[0..13] = n_particles,
+ * mass, mom1..3 (sum m v), ekin, z_rms (the particle sums per chunk of 4096 rows, the chunks
+ * added in order on the host), rho_max and the center and level of its cell (the mass deposit
+ * of the point particles with the coupling's stencil and fold at `time`; with refinable clouds
+ * rho_max = -1: not built), chi_max and dt_stiff_min (the interval records of UpdateStiffness
+ * since the previous call, -1 when none; dt_stiff_min needs stiff_thresh > 0).  Resets the
+ * interval records.  One rank.  Waits for the stream. */
+int dk_particle_history(dk_ctx *ctx, double time, double stiff_thresh, double row[14]);
+/* ParticleHistoryOutput::WriteOutputFile (src/outputs/phst.cpp:80-163): appends time, dt and
+ * row to `path` (%.16e columns), preceded by the header block when header != 0. */
+int dk_write_phst(dk_ctx *ctx, const char *path, const dk_output_info *info, const double row[14],
+                  int32_t header);
+
 #ifdef __cplusplus
 }
 #endif

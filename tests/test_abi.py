@@ -44,14 +44,14 @@ def test_struct_sizes_match_the_header_layout(tmp_path):
         pytest.skip("no gcc")
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "dustk.h"\n#include "dustk_setup.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(dk_block),'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(dk_block),'
                    ' sizeof(dk_stage), sizeof(dk_mesh), sizeof(dk_physics), sizeof(dk_gas_src),'
-                   ' sizeof(dk_setup)); return 0;}\n')
+                   ' sizeof(dk_setup), sizeof(dk_output_info)); return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
     want = [ctypes.sizeof(t) for t in (abi.dk_block, abi.dk_stage, abi.dk_mesh, abi.dk_physics,
-                                       abi.dk_gas_src, abi.dk_setup)]
+                                       abi.dk_gas_src, abi.dk_setup, abi.dk_output_info)]
     assert got == want
 
 
