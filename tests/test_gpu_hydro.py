@@ -49,6 +49,19 @@ def test_fp64_outflow_x3_with_fluxes_matches_the_oracle():
     fp64_run(w, 10, check_every=5)
 
 
+def test_fp64_fluxes_on_larger_and_uneven_blocks_match_the_oracle():
+    """16^3 blocks, and 24 x 20 x 8 blocks (cells per block differing by axis)."""
+    w = workloads.with_hydro(workloads.c1(n=32, nb=2, ppc=1))
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 0.5, 0.2, 2e-3
+    w.name = "fp64-16-blocks-hydro"
+    fp64_run(w, 6, check_every=3)
+    w = crossing()
+    mesh, blocks = abi.uniform_mesh((24, 20, 8), (2, 2, 2), ((0.0, 1.2), (0.0, 1.0), (0.0, 0.4)),
+                                    ng=2)
+    w.mesh, w.blocks, w.name = mesh, blocks, "fp64-uneven-blocks-hydro"
+    fp64_run(w, 4, check_every=2)
+
+
 def test_fp32_imex_with_fluxes_within_1e5_of_the_oracle():
     check_against_oracle(crossing(), 20)
 
