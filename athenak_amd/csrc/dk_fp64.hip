@@ -2299,7 +2299,17 @@ int dk_new_time_step(dk_ctx *c, double dt_cfl, double *dt_out) {
   int64_t part = -1;
   if ((e = dk_error(c, &code, &part))) return e;
   if (code == DK_ESTOP) return DK_ESTOP;
-  if (dt_out) *dt_out = dt_cfl*m;
+  double dt = dt_cfl*m;
+  if (c->form == DK_FORM_FP32 && c->P.shearing_box) {
+    // the FP32 form's shearing-box limit: a stage's orbital-advection frame shift
+    // q Omega c dt |x1|max, with the stage abscissa c at most 1 + 1/sqrt(2) (imex2+), must stay
+    // below half a cell in x2 (FastStage refuses a larger one), so no proposed dt reaches it
+    const double xmax = std::max(std::fabs(c->M.x1min), std::fabs(c->M.x1max));
+    const double dy = (c->M.x2max - c->M.x2min)/(c->g.nx2*c->g.nmbx2);
+    const double cmax = 1.7071067811865475;
+    if (qo*xmax > 0.0) dt = std::fmin(dt, (1.0 - 1.0e-9)*0.5*dy/(qo*cmax*xmax));
+  }
+  if (dt_out) *dt_out = dt;
   return DK_OK;
 }
 

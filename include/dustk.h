@@ -244,7 +244,10 @@ int dk_remove_outflow_particles(dk_ctx *ctx, double out[4]);
  * Waits for the stream. */
 int dk_x3_removed(dk_ctx *ctx, double out[2]);
 /* DustGasDrag::NewTimeStep, src/dust/dust_newdt.cpp:108-149: dt_out = dt_cfl min dx/|v|
- * (both forms; the FP32 form applies a pending correction first, from its FP32 velocities) */
+ * (both forms; the FP32 form applies a pending correction first, from its FP32 velocities).
+ * The FP32 form in the shearing box also caps dt_out where a stage's orbital-advection frame
+ * shift q Omega c dt |x1|max (c <= 1 + 1/sqrt(2)) would reach half a cell in x2, the largest
+ * shift its stages accept (DK_EUNSUPPORTED beyond); the reference has no such limit. */
 int dk_new_time_step(dk_ctx *ctx, double dt_cfl, double *dt_out);
 
 /* The strict drag solver (dk_physics.solver = DK_PCG or DK_ADAPTIVE, FP64 form): its

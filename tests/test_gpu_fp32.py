@@ -479,6 +479,26 @@ def test_fp32_new_time_step_and_sort():
     g.close()
 
 
+def test_fp32_shearing_box_new_time_step_stays_within_the_frame_limit():
+    # the FP32 form's stages refuse an orbital-advection frame shift beyond half a cell in x2;
+    # its new time step never proposes such a dt: a stage at the proposed cap runs, one 20 %
+    # above it is refused before any work
+    from athenak_amd.lib import DustError
+    w = workloads.c2(n=16, nbx=2, ppc=2)
+    g = fp32_ctx(w)
+    dt = g.new_time_step(1.0e6)   # (no particle CFL limit: the frame limit alone)
+    dy = (w.mesh.x2max - w.mesh.x2min) / (w.mesh.nx2 * w.mesh.nmbx2)
+    xmax = max(abs(w.mesh.x1min), abs(w.mesh.x1max))
+    cap = 0.5 * dy / (w.phys.qshear * w.phys.omega0 * (1.0 + 0.5 ** 0.5) * xmax)
+    say("shearing-box dt cap %.9e (frame limit %.9e)" % (dt, cap))
+    assert cap * (1 - 1e-8) <= dt <= cap
+    g.step(dt, w.src)
+    g.check_device_error()
+    with pytest.raises(DustError, match="option not provided"):
+        g.step(1.2 * dt, w.src)
+    g.close()
+
+
 def test_pc2_stiffness_record_both_forms():
     # UpdateStiffness (dust_pm.cpp:288-342): chi of the last cycle and its running maximum, in
     # the FP64 form to the oracle's round-off and in the FP32 form to FP32 round-off
