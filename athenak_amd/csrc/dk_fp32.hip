@@ -1594,93 +1594,42 @@ __global__ void k_place(const int *key, const int *rank, const int *start, const
 }
 
 // every cell's arrivals (its list after its stay[] stayers, which the key pass ranked in slot
-// order) in ascending source slot; medium and long lists of arrivals are queued.  A workgroup takes kBlock consecutive keys at a time, or, when their lists do not fit
-// in kSegStage slots, the same keys in runs of kSegKeys.  A run whose lists fit is staged in LDS
-// (one coalesced read, one coalesced write) and every element of a list of up to kSegRank is
-// placed by its rank (the count of smaller source slots in its list: the slots are distinct), a
-// thread per element; otherwise each short list is ordered in place in global memory by its
-// key's thread (insertion sort).  Longer lists are queued for the medium and long sorts, with
-// one global atomic per run and class.
-constexpr int kSegKeys = 64;
-constexpr int kSegStage = 4096;
-constexpr int kSegRank = 128;       // longest list a staged run ranks itself
-
-__device__ inline void SegSortRun(const int *start, const int *stay, long long nkeys, int *perm,
-                                  int *big, long long c0, int nk, int *sh, int *out, int *cs,
-                                  int *ts, int *qn, int *qbase) {
-  const int t = threadIdx.x;
-  const long long c = c0 + t;
-  const long long ce = (c0 + nk < nkeys) ? c0 + nk : nkeys;
-  const int nc = static_cast<int>(ce - c0);
-  const int r0 = start[c0], r1 = start[ce];
-  const bool staged = (r1 - r0) <= kSegStage;   // (the same for the whole workgroup)
-  for (int q = t; q <= nc; q += kBlock) cs[q] = start[c0 + q] - r0;   // (nc + 1 entries)
-  for (int q = t; q < nc; q += kBlock) ts[q] = start[c0 + q] + stay[c0 + q] - r0;
-  if (t < 2) qn[t] = 0;
-  if (staged)
-    for (int i = r0 + t; i < r1; i += kBlock) sh[i - r0] = perm[i];
-  __syncthreads();
-  // (a cell's list to order: its arrivals, after its stayers)
-  const int n = (t < nc) ? cs[t + 1] - ts[t] : 0;
-  // (a staged run ranks its lists of up to kSegRank elements: longer medium ones, and the
-  // medium ones of a run that is not staged, are queued)
-  const int cls = (n > kMedMax) ? 1 : ((n > (staged ? kSegRank : kSegSmall)) ? 0 : -1);
-  const int qs = (cls >= 0) ? atomicAdd(&qn[cls], 1) : 0;
-  __syncthreads();
-  if (t < 2) qbase[t] = qn[t] ? atomicAdd(&big[t], qn[t]) : 0;
-  __syncthreads();
-  if (cls == 1) big[1 + nkeys - (qbase[1] + qs)] = static_cast<int>(c);
-  else if (cls == 0) big[2 + qbase[0] + qs] = static_cast<int>(c);
-  if (!staged && n > 1 && n <= kSegSmall) {
-    int *a = perm + r0 + ts[t];
-    for (int i=1; i<n; ++i) {
-      const int v = a[i];
-      int j = i - 1;
-      while (j >= 0 && a[j] > v) {a[j + 1] = a[j]; --j;}
-      a[j + 1] = v;
-    }
-  }
-  if (staged) {
-    const int ne = r1 - r0;
-    for (int e = t; e < ne; e += kBlock) {
-      // the element's key: the last list start at or below it
-      int lo = 0, hi = nc;   // cs[lo] <= e < cs[hi]
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (cs[mid] <= e) lo = mid; else hi = mid;
-      }
-      const int s0 = ts[lo], s1 = cs[lo + 1], nl = s1 - s0;
-      const int v = sh[e];
-      if (e < s0) {
-        out[e] = v;   // (a stayer: in place)
-      } else if (nl <= kSegRank) {
-        int rank = 0;
-        for (int q = s0; q < s1; ++q) rank += (sh[q] < v) ? 1 : 0;
-        out[s0 + rank] = v;
-      } else {
-        out[e] = v;   // (ordered later by the medium or long sort)
-      }
-    }
-    __syncthreads();
-    for (int i = r0 + t; i < r1; i += kBlock) perm[i] = out[i - r0];
-  }
-  __syncthreads();
-}
-
+// order) in ascending source slot, a thread per key: a short list of arrivals (2..kSegSmall) is
+// ordered in place in global memory by insertion sort; longer ones are queued for the medium
+// and long sorts, with one global atomic per workgroup pass and class.  Stayers and lists of
+// one arrival are not touched: arrivals are a small share of a stage's particles, so a pass
+// reads the keys' starts and stayer counts and little else.
 __global__ void __launch_bounds__(kBlock) k_segsort(const int *start, const int *stay,
                                                     long long nkeys, int *perm, int *big) {
-  __shared__ int sh[kSegStage], out[kSegStage];
-  __shared__ int cs[kBlock + 1], ts[kBlock];
-  __shared__ int qn[2], qbase[2];   // the run's medium / long lists and their queue base
+  __shared__ int qn[2], qbase[2];   // the pass's medium / long lists and their queue base
+  const int t = threadIdx.x;
   for (long long c0 = (long long)blockIdx.x*kBlock; c0 < nkeys;
        c0 += (long long)gridDim.x*kBlock) {
-    const long long ce = (c0 + kBlock < nkeys) ? c0 + kBlock : nkeys;
-    if (start[ce] - start[c0] <= kSegStage) {
-      SegSortRun(start, stay, nkeys, perm, big, c0, kBlock, sh, out, cs, ts, qn, qbase);
-    } else {
-      for (long long r = c0; r < ce; r += kSegKeys)
-        SegSortRun(start, stay, nkeys, perm, big, r, kSegKeys, sh, out, cs, ts, qn, qbase);
+    const long long c = c0 + t;
+    int n = 0, s0 = 0;
+    if (c < nkeys) {
+      s0 = start[c] + stay[c];
+      n = start[c + 1] - s0;
     }
+    const int cls = (n > kMedMax) ? 1 : ((n > kSegSmall) ? 0 : -1);
+    if (t < 2) qn[t] = 0;
+    __syncthreads();
+    const int qs = (cls >= 0) ? atomicAdd(&qn[cls], 1) : 0;
+    __syncthreads();
+    if (t < 2) qbase[t] = qn[t] ? atomicAdd(&big[t], qn[t]) : 0;
+    __syncthreads();
+    if (cls == 1) big[1 + nkeys - (qbase[1] + qs)] = static_cast<int>(c);
+    else if (cls == 0) big[2 + qbase[0] + qs] = static_cast<int>(c);
+    if (n > 1 && n <= kSegSmall) {
+      int *a = perm + s0;
+      for (int i=1; i<n; ++i) {
+        const int v = a[i];
+        int j = i - 1;
+        while (j >= 0 && a[j] > v) {a[j + 1] = a[j]; --j;}
+        a[j + 1] = v;
+      }
+    }
+    __syncthreads();   // (qn and qbase are reset by the next pass)
   }
 }
 
