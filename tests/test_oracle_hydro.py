@@ -264,3 +264,14 @@ def test_dusty_wave_follows_the_dispersion_relation(coupling):
         phase.append((abs(np.angle(cg / eg)), abs(np.angle(cd / ed))))
     assert phase[1][0] < 0.005 and phase[1][1] < 0.005
     assert phase[0][0] / phase[1][0] > 3.0 and phase[0][1] / phase[1][1] > 3.0
+
+
+def test_streaming_instability_lina_eigenvalue():
+    """The linear system of tests/si_linear.py gives the published linA growth rate (Youdin &
+    Johansen 2007, Table 1: t_s = 0.1, eps = 3, K = 30, s = 0.4190 Omega) in the incompressible
+    limit, and nearly the same with eta v_K = c_s/2 (the GPU test's case)."""
+    import si_linear
+    lam, _ = si_linear.growing_mode(30.0, 0.05, 0.1, 3.0)
+    assert abs(lam.real - 0.4190) < 5e-4
+    lam2, _ = si_linear.growing_mode(30.0, 0.5, 0.1, 3.0)
+    assert 0.41 < lam2.real < lam.real
