@@ -56,9 +56,12 @@ __global__ void k_phst_chunks(const double *r, long long cap, long long np, int 
 
 // the mass density of the point particles with the coupling's stencil on the owner block
 // (DepositPointScienceFields, src/dust/dust_gravity_points.cpp:273-301, mass only); the ghost
-// parts are folded afterwards.  bad counts rows outside their block's stencil.
+// parts are folded afterwards.  Refinable clouds of level 0 on a uniform mesh deposit the hat's
+// exact cell integrals, which are the TSC weights (the representation's "uniform level-0 mesh:
+// canonical TSC", dust_science.cpp:97-98): scheme = TSC for them.  bad counts rows outside
+// their block's stencil.
 __global__ void k_sci_mass(Geo g, const DevBlock *B, const double *r, const int32_t *iv,
-                           long long cap, long long np, double *f, int *bad) {
+                           long long cap, long long np, int scheme, double *f, int *bad) {
   for (long long p = blockIdx.x*(long long)blockDim.x + threadIdx.x; p < np;
        p += (long long)gridDim.x*blockDim.x) {
     const double mass = PR(DK_IPM,p);
@@ -72,9 +75,9 @@ __global__ void k_sci_mass(Geo g, const DevBlock *B, const double *r, const int3
     const DevBlock &s = B[b];
     int ip, jp, kp;
     double wx[3], wy[3], wz[3];
-    dk::PMWeights(PR(DK_IPX,p), s.x1min, s.x1max, g.nx1, g.is, g.deposit, ip, wx);
-    dk::PMWeights(PR(DK_IPY,p), s.x2min, s.x2max, g.nx2, g.js, g.deposit, jp, wy);
-    dk::PMWeights(PR(DK_IPZ,p), s.x3min, s.x3max, g.nx3, g.ks, g.deposit, kp, wz);
+    dk::PMWeights(PR(DK_IPX,p), s.x1min, s.x1max, g.nx1, g.is, scheme, ip, wx);
+    dk::PMWeights(PR(DK_IPY,p), s.x2min, s.x2max, g.nx2, g.js, scheme, jp, wy);
+    dk::PMWeights(PR(DK_IPZ,p), s.x3min, s.x3max, g.nx3, g.ks, scheme, kp, wz);
     if (ip < 1 || ip >= g.n1 - 1 || jp < 1 || jp >= g.n2 - 1 || kp < 1 || kp >= g.n3 - 1) {
       atomicAdd(bad, 1);
       continue;
@@ -515,10 +518,11 @@ int dk_particle_history(dk_ctx *c, double time, double stiff_thresh, double row[
   }
   for (int q=0; q<6; ++q) row[q] = sums[q];
   row[6] = (sums[1] > 0.0) ? std::sqrt(sums[6]/sums[1]) : 0.0;
-  // ---- the densest cell of the mass deposit (point particles; refinable clouds: -1)
+  // ---- the densest cell of the mass deposit (point particles, and clouds on a uniform
+  // mesh; clouds on a refined mesh: -1, not built)
   double where[4] = {0.0, 0.0, 0.0, -1.0};
   row[7] = -1.0;
-  if (!c->P.clouds) {
+  if (!c->P.clouds || !c->smr) {
     const long long ntot = (long long)g.nmb*g.ncb;
     double *f = nullptr, *part = nullptr;
     int *bad = nullptr;
@@ -530,8 +534,9 @@ int dk_particle_history(dk_ctx *c, double time, double stiff_thresh, double row[
     if (!e) e = Hip(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
     if (!e && np > 0) {
       const int grid = static_cast<int>(std::min<long long>((np + kThreads - 1)/kThreads, 65536));
+      const int scheme = c->P.clouds ? DK_TSC : c->P.deposit;
       k_sci_mass<<<grid, kThreads, 0, c->stream>>>(g, c->d_blocks, c->rdata, c->idata,
-                                                   (long long)c->cap, np, f, bad);
+                                                   (long long)c->cap, np, scheme, f, bad);
       e = Hip(hipGetLastError());
     }
     if (!e) e = dk::FoldT<double>(c, f, 1, time);

@@ -198,3 +198,17 @@ def test_history_stiffness_interval_records():
     row = g.particle_history(4 * w.dt, stiff_thresh=0.0)
     assert row[12] == s[0] and row[13] == -1.0
     g.close()
+
+
+@pytest.mark.parametrize("form", [abi.DK_FORM_FP64, abi.DK_FORM_FP32])
+def test_history_rho_max_of_clouds_on_a_uniform_mesh(form):
+    # level-0 clouds: the hat's exact cell integrals are the TSC weights
+    w = workloads.c4u(n=16, nbx=1, ppc=2)
+    g = ctx(w, form)
+    g.step(w.dt, w.src)
+    row = g.particle_history(w.dt)
+    r, _ = exported(g)
+    rho = tsc_density(r, w.mesh, w.blocks[0])
+    say("clouds form %d: rho_max %.12e (B-spline closed form %.12e)" % (form, row[7], rho.max()))
+    assert abs(row[7] - rho.max()) <= 1e-12 * rho.max()
+    g.close()
