@@ -1364,8 +1364,7 @@ namespace dk {
 // The options the explicit gas update supports (dk_gas_src in dustk.h)
 int HydroSupported(const dk_ctx *c, const dk_gas_src *src) {
   if (!(src->iso_cs > 0.0)) return DK_EINVAL;
-  if (c->smr || c->g.ng < 2 || c->g.bc_x1 != DK_X1_PERIODIC || c->g.bc_x3 == DK_X3_REFLECT)
-    return DK_EUNSUPPORTED;
+  if (c->smr || c->g.ng < 2 || c->g.bc_x1 != DK_X1_PERIODIC) return DK_EUNSUPPORTED;
   return DK_OK;
 }
 
@@ -1393,7 +1392,7 @@ int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     // the u0 ghosts of the stage start (the previous stage's Hydro::SendU/RecvU and
     // ApplyPhysicalBCs; x1 is periodic, so the time is not read), then the flux divergence
     if ((e = HydroField(c))) return e;
-    if ((e = FillGhosts(c, c->u0, DK_NGAS, false, st->time))) return e;
+    if ((e = FillGhosts(c, c->u0, DK_NGAS, true, st->time))) return e;
     {const int _pt = dk::ProfBegin(c, "hydro_divf");
     k_hydro_divf<<<CGRID(ActiveCells(c))>>>(c->g, c->d_blocks, c->u0, c->hdivf, src->iso_cs,
                                             src->dfloor);
@@ -1912,7 +1911,7 @@ int dk_gas_hook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
 
 int dk_fill_gas_ghosts(dk_ctx *c, const dk_stage *st) {
   if (c == nullptr || c->u0 == nullptr) return DK_ESTATE;
-  return FillGhosts(c, c->u0, DK_NGAS, false, st->time + st->dt);
+  return FillGhosts(c, c->u0, DK_NGAS, true, st->time + st->dt);
 }
 
 namespace {

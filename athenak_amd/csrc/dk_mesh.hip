@@ -135,10 +135,13 @@ __global__ void k_smr_ghosts(Geo g, const DevBlock *B, dk::SmrTree tree, T *a, i
   }
 }
 
+// velocity: the field's normal component (the last of 3, or the gas's momentum IM3 of 4: the
+// hydro's reflect, src/bvals/physics/hydro_bcs.cpp:229-290) is negated at a reflecting face
 template <typename T>
 __global__ void k_x3bcs(Geo g, const DevBlock *B, T *a, int nvar, int velocity) {
   const long long n = (long long)g.nmb*g.ng*g.n2*g.n1;
   const bool refl = (g.bc_x3 == DK_X3_REFLECT);
+  const int vn = (nvar == DK_NGAS) ? DK_IM3 : 2;
   GRID_STRIDE(t, n) {
     long long rr = t;
     const int i = static_cast<int>(rr % g.n1); rr /= g.n1;
@@ -149,14 +152,14 @@ __global__ void k_x3bcs(Geo g, const DevBlock *B, T *a, int nvar, int velocity) 
       const int kg = g.ks - 1 - gg, kc = refl ? (g.ks + gg) : g.ks;
       for (int v=0; v<nvar; ++v) {
         const T val = a[Idx(g,m,nvar,v,kc,j,i)];
-        a[Idx(g,m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+        a[Idx(g,m,nvar,v,kg,j,i)] = (refl && velocity && v == vn) ? -val : val;
       }
     }
     if (B[m].nghbr[Dir(0,0,1)] < 0) {
       const int kg = g.ke + 1 + gg, kc = refl ? (g.ke - gg) : g.ke;
       for (int v=0; v<nvar; ++v) {
         const T val = a[Idx(g,m,nvar,v,kc,j,i)];
-        a[Idx(g,m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+        a[Idx(g,m,nvar,v,kg,j,i)] = (refl && velocity && v == vn) ? -val : val;
       }
     }
   }

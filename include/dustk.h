@@ -150,8 +150,8 @@ typedef struct dk_stage {
  * src/reconstruct/plm.hpp, and rsolver = hlle, src/hydro/rsolvers/hlle_hyd.hpp) from the
  * primitives of u0 at the stage start (IsothermalHydro::ConsToPrim with the density floor,
  * src/eos/isothermal_hyd.cpp:34-110); the stage first refills the u0 ghosts.  It needs at least
- * 2 ghost cells, a uniform mesh, periodic x1 and periodic or outflow x3 (DK_EUNSUPPORTED
- * otherwise) and iso_cs > 0 (DK_EINVAL); the first stage with hydro allocates the flux
+ * 2 ghost cells, a uniform mesh and periodic x1 (DK_EUNSUPPORTED otherwise; x3 may be
+ * periodic, outflow or reflecting, the gas mirrored with IM3 negated) and iso_cs > 0 (DK_EINVAL); the first stage with hydro allocates the flux
  * divergence field (4 components, the gas layout). */
 typedef struct dk_gas_src {
   int32_t shearing_box;     /* ShearingBoxCC::SourceTermsCC (3-D) */

@@ -807,8 +807,12 @@ void CopyGhosts(or_sim *s, std::vector<double> &a, int nvar) {
   }
 }
 
-// src/dust/dust_tasks.cpp:445-469 (reflect: mirror with v_z negated; else zero gradient)
+// src/dust/dust_tasks.cpp:445-469 (reflect: mirror with v_z negated; else zero gradient); for
+// the gas (nvar = 4: density, momenta) the hydro's reflect negates IM3 the same way
+// (src/bvals/physics/hydro_bcs.cpp:229-290).  velocity: the field's last of 3 components, or
+// the gas's momentum IM3, is the normal one
 void X3BCs(or_sim *s, std::vector<double> &a, int nvar, bool velocity) {
+  const int vn = (nvar == NG_VARS) ? DK_IM3 : 2;
   if (s->M.bc_x3 == DK_X3_PERIODIC) return;
   const bool refl = (s->M.bc_x3 == DK_X3_REFLECT);
   const int ng = s->ng;
@@ -822,14 +826,14 @@ void X3BCs(or_sim *s, std::vector<double> &a, int nvar, bool velocity) {
             const int kg = s->ks - 1 - g, kc = refl ? (s->ks + g) : s->ks;
             for (int v=0; v<nvar; ++v) {
               const double val = a[s->Idx(m,nvar,v,kc,j,i)];
-              a[s->Idx(m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+              a[s->Idx(m,nvar,v,kg,j,i)] = (refl && velocity && v == vn) ? -val : val;
             }
           }
           if (hi) {
             const int kg = s->ke + 1 + g, kc = refl ? (s->ke - g) : s->ke;
             for (int v=0; v<nvar; ++v) {
               const double val = a[s->Idx(m,nvar,v,kc,j,i)];
-              a[s->Idx(m,nvar,v,kg,j,i)] = (refl && velocity && v == 2) ? -val : val;
+              a[s->Idx(m,nvar,v,kg,j,i)] = (refl && velocity && v == vn) ? -val : val;
             }
           }
         }
@@ -1348,7 +1352,7 @@ int or_setup(or_sim *s, const dk_setup *su) {
   }
   CopyGhosts(s, s->u0, NG_VARS);
   if (s->M.bc_x1 == DK_X1_SHEAR) ShearGhostsX1(s, s->u0, NG_VARS, 0.0);
-  X3BCs(s, s->u0, NG_VARS, false);
+  X3BCs(s, s->u0, NG_VARS, true);
   s->u1 = s->u0;
   return DK_OK;
 }
@@ -2336,8 +2340,7 @@ void HlleIso(const double wl[4], const double wr[4], double cs, double f[4]) {
 // The options the explicit gas update supports (dk_gas_src in dustk.h)
 int HydroSupported(const or_sim *s, const dk_gas_src *src) {
   if (!(src->iso_cs > 0.0)) return DK_EINVAL;
-  if (s->smr || s->ng < 2 || s->M.bc_x1 != DK_X1_PERIODIC || s->M.bc_x3 == DK_X3_REFLECT)
-    return DK_EUNSUPPORTED;
+  if (s->smr || s->ng < 2 || s->M.bc_x1 != DK_X1_PERIODIC) return DK_EUNSUPPORTED;
   return DK_OK;
 }
 
@@ -2426,7 +2429,7 @@ int or_gas_hook(or_sim *s, const dk_stage *st, const dk_gas_src *src) {
     // the u0 ghosts of the stage start (the previous stage's Hydro::SendU/RecvU and
     // ApplyPhysicalBCs), then the flux divergence
     CopyGhosts(s, s->u0, NG_VARS);
-    X3BCs(s, s->u0, NG_VARS, false);
+    X3BCs(s, s->u0, NG_VARS, true);
     HydroDivf(s, src, divf);
   }
   const int nmb = s->M.nmb;
@@ -2478,7 +2481,7 @@ int or_gas_hook(or_sim *s, const dk_stage *st, const dk_gas_src *src) {
 int or_fill_gas_ghosts(or_sim *s, const dk_stage *st) {
   CopyGhosts(s, s->u0, NG_VARS);
   if (s->M.bc_x1 == DK_X1_SHEAR) ShearGhostsX1(s, s->u0, NG_VARS, st->time + st->dt);
-  X3BCs(s, s->u0, NG_VARS, false);
+  X3BCs(s, s->u0, NG_VARS, true);
   return DK_OK;
 }
 

@@ -41,12 +41,20 @@ def test_fp64_pc2_with_fluxes_matches_the_oracle():
     fp64_run(w, 20, check_every=5)
 
 
-def test_fp64_outflow_x3_with_fluxes_matches_the_oracle():
+@pytest.mark.parametrize("bc", [abi.DK_X3_OUTFLOW, abi.DK_X3_REFLECT])
+def test_fp64_physical_x3_faces_with_fluxes_match_the_oracle(bc):
+    w = crossing()
+    mesh, blocks = abi.uniform_mesh((8, 8, 8), (2, 2, 2), ((0.0, 1.0),) * 3, ng=2, bc_x3=bc)
+    w.mesh, w.blocks, w.name = mesh, blocks, "fp64-x3-%d-hydro" % bc
+    fp64_run(w, 10, check_every=5)
+
+
+def test_fp32_reflecting_x3_with_fluxes_within_1e5_of_the_oracle():
     w = crossing()
     mesh, blocks = abi.uniform_mesh((8, 8, 8), (2, 2, 2), ((0.0, 1.0),) * 3, ng=2,
-                                    bc_x3=abi.DK_X3_OUTFLOW)
-    w.mesh, w.blocks, w.name = mesh, blocks, "fp64-outflow-hydro"
-    fp64_run(w, 10, check_every=5)
+                                    bc_x3=abi.DK_X3_REFLECT)
+    w.mesh, w.blocks, w.name = mesh, blocks, "fp32-reflect-hydro"
+    check_against_oracle(w, 10)
 
 
 def test_fp64_fluxes_on_larger_and_uneven_blocks_match_the_oracle():

@@ -165,13 +165,40 @@ def test_unsupported_options_are_refused():
     mesh, blocks = abi.uniform_mesh((8, 8, 8), (1, 1, 1), ((0, 1),) * 3, ng=1)
     o = Oracle(mesh, blocks, workloads._phys(), 16)
     assert gas_hook(o, st, hydro_src()) == abi.DK_EUNSUPPORTED
-    mesh, blocks = abi.uniform_mesh((8, 8, 8), (1, 1, 1), ((0, 1),) * 3, ng=2,
-                                    bc_x3=abi.DK_X3_REFLECT)
-    o = Oracle(mesh, blocks, workloads._phys(), 16)
-    assert gas_hook(o, st, hydro_src()) == abi.DK_EUNSUPPORTED
     mesh, blocks = abi.uniform_mesh((8, 8, 8), (1, 1, 1), ((0, 1),) * 3, ng=2)
     o = Oracle(mesh, blocks, workloads._phys(), 16)
     assert gas_hook(o, st, hydro_src(0.0)) == abi.DK_EINVAL
+
+
+def test_reflecting_x3_walls_hold_mass_and_tangential_momentum():
+    """Reflecting x3 faces mirror the gas with IM3 negated (the hydro's reflect,
+    src/bvals/physics/hydro_bcs.cpp:229-290): the face states at a wall are mirror images, so
+    no mass and no x1/x2 momentum crosses it; the wall pushes only on IM3."""
+    rng = np.random.default_rng(8)
+    mesh, blocks = abi.uniform_mesh((8, 8, 8), (1, 1, 2), ((0, 1), (0, 1), (0, 2)), ng=2,
+                                    bc_x3=abi.DK_X3_REFLECT)
+    o = Oracle(mesh, blocks, workloads._phys(), 16)
+    u0 = o.u0()
+    d = 1.0 + 0.5 * rng.random(u0[:, 0].shape)
+    u0[:, 0] = d
+    for c in range(1, 4):
+        u0[:, c] = d * rng.normal(scale=0.3, size=d.shape)
+    tot0 = active(o.u0(), 2).sum(axis=(0, 2, 3, 4))
+    for s in range(4):
+        o.u1()[:] = o.u0()
+        for st in abi.rk2_stages(0.0, 0.01):
+            assert gas_hook(o, st, hydro_src(1.0)) == abi.DK_OK
+    ua = active(o.u0(), 2)
+    tot1 = ua.sum(axis=(0, 2, 3, 4))
+    scale = np.abs(ua).sum(axis=(0, 2, 3, 4))
+    assert np.all(np.abs(tot1[:3] - tot0[:3]) < 1e-14 * scale[:3])
+    assert abs(tot1[3] - tot0[3]) > 1e-6 * scale[3]     # (the walls push on IM3)
+    # the ghosts (refilled, as a stage's tail does) mirror the first active layers, IM3 negated
+    lib().or_fill_gas_ghosts(o.h, C.byref(euler_stage(0.0)))
+    g = o.u0()
+    np.testing.assert_array_equal(g[0, 0, 1], g[0, 0, 2])
+    np.testing.assert_array_equal(g[0, 3, 1], -g[0, 3, 2])
+    np.testing.assert_array_equal(g[1, 1, -2], g[1, 1, -3])
 
 
 # ---- the damped dusty sound wave -----------------------------------------------------------
