@@ -368,8 +368,10 @@ int dk_write_pbin(dk_ctx *ctx, const char *path, const dk_output_info *info);
 /* DustGasDrag::ParticleHistory (src/dust/dust_science.cpp:372-484): row[0..13] = n_particles,
  * mass, mom1..3 (sum m v), ekin, z_rms (the particle sums per chunk of 4096 rows, the chunks
  * added in order on the host), rho_max and the center and level of its cell (the mass deposit
- * of the point particles with the coupling's stencil and fold at `time`, of clouds on a uniform
- * mesh with TSC, their exact cell integrals; clouds on a refined mesh: rho_max = -1, not built), chi_max and dt_stiff_min (the interval records of UpdateStiffness
+ * of the point particles with the coupling's stencil and fold at `time` (with x3_dep_drop the
+ * parts beyond physical x3 faces are dropped, as the coupling's; the reference's science deposit
+ * folds them back), of clouds on a uniform mesh with TSC, their exact cell integrals; clouds on
+ * a refined mesh: rho_max = -1, not built), chi_max and dt_stiff_min (the interval records of UpdateStiffness
  * since the previous call, -1 when none; dt_stiff_min needs stiff_thresh > 0).  Resets the
  * interval records.  One rank.  Waits for the stream. */
 int dk_particle_history(dk_ctx *ctx, double time, double stiff_thresh, double row[14]);
