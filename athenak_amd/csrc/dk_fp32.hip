@@ -2833,14 +2833,6 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
   }
 }
 
-// the drag history of stage 2 alone (dust_pm.cpp:734-739), the last step of k_cell_start,
-// for a stage whose gas update runs between them (the isothermal fluxes of dk_gas_src.hydro)
-__global__ void k_drag_history(Geo g, double *u0, const double *dmom, double atw_dt) {
-  BLOCK_CELLS(m, k, j, i)
-    for (int c=0; c<3; ++c) u0[Idx(g,m,DK_NGAS,1+c,k,j,i)] += atw_dt*dmom[Idx(g,m,3,c,k,j,i)];
-  }
-}
-
 // PackGatherField3D (src/dust/dust_cloud3d.cpp:1319-1340) in FP32: the velocity and its central
 // differences along x1, x2, x3 of every cell with both neighbours in each direction (the
 // active cells and the first ghost layer, which the tile regions read)
@@ -3882,8 +3874,8 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
       A, 1, f->pres[1 - f->pres_rd])));
   // 2. stage start cell pass: previous split's gas share, u1 <- u0, gas update, drag history
   {
-    // with isothermal fluxes the gas update and the drag history run after the cell pass (the
-    // fluxes read u0's ghosts, refilled once the pending share is in)
+    // with isothermal fluxes the gas update and the drag history run after the cell pass, in
+    // one pass (the fluxes read u0's ghosts, refilled once the pending share is in)
     const bool hydro = (src != nullptr && src->hydro);
     dk_gas_src z;
     std::memset(&z, 0, sizeof(z));
@@ -3894,12 +3886,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
         f->pres[f->pres_rd], f->pending,
         f->pending ? 1.0/f->a_prev : 0.0, A.stage1, hydro ? 0 : 1, gam0, gam1, bdt,
         src ? 1 : 0, src ? *src : z, hydro ? 0.0 : atw)));
-    if (hydro) {
-      if ((e = dk::GasHook(c, st, src))) return e;
-      if (atw != 0.0)
-        TIMED("drag_history", (k_drag_history<<<CellGrid(c), kBlock, 0, c->stream>>>(
-            g, c->u0, f->dmom, atw)));
-    }
+    if (hydro && (e = dk::GasHook(c, st, src, atw != 0.0 ? f->dmom : nullptr, atw))) return e;
   }
   f->pending = 0;
   A.pending = 0;

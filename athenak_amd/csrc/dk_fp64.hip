@@ -855,10 +855,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_hydro_divf(Geo g, const DevBlock 
 }
 
 // Hydro::RKUpdate (zero fluxes, or minus beta dt divf when divf is given) + HydroSrcTerms
-// (see dustk.h dk_gas_src)
+// (see dustk.h dk_gas_src); with dmom, then AddDragHistoryGas's u0.M += atw dmom
+// (dust_pm.cpp:721-742), for a caller whose drag history follows the gas update directly
 __global__ void k_gas_hook(Geo g, const DevBlock *B, double *u0, const double *u1, double gam0,
                            double gam1, double bdt, int use_src, dk_gas_src src,
-                           const double *divf) {
+                           const double *divf, const double *dmom, double atw) {
   const long long n = (long long)g.nmb*g.nx3*g.nx2*g.nx1;
   GRID_STRIDE(t, n) {
     long long rr = t;
@@ -881,10 +882,9 @@ __global__ void k_gas_hook(Geo g, const DevBlock *B, double *u0, const double *u
       else
         u = gam0*u + gam1*u1[Idx(g,m,DK_NGAS,v,k,j,i)];
     }
-    if (!use_src) continue;
-    if (src.const_accel != 0.0)
+    if (use_src && src.const_accel != 0.0)
       u0[Idx(g,m,DK_NGAS,src.const_accel_dir,k,j,i)] += bdt*src.const_accel*w[0];
-    if (src.shearing_box) {
+    if (use_src && src.shearing_box) {
       const double coef1 = 2.0*bdt*src.omega0;
       const double coef2 = (2.0-src.qshear)*bdt*src.omega0;
       const double coef3 = bdt*(src.omega0*src.omega0);
@@ -899,6 +899,8 @@ __global__ void k_gas_hook(Geo g, const DevBlock *B, double *u0, const double *u
         u0[Idx(g,m,DK_NGAS,DK_IM3,k,j,i)] -= coef3*w[0]*x3v;
       }
     }
+    if (dmom != nullptr)
+      for (int c=0; c<3; ++c) u0[Idx(g,m,DK_NGAS,1+c,k,j,i)] += atw*dmom[Idx(g,m,3,c,k,j,i)];
   }
 }
 
@@ -1379,7 +1381,8 @@ int HydroField(dk_ctx *c) {
   return DK_OK;
 }
 
-int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
+int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src, const double *dmom,
+            double atw) {
   const int s = st->stage - 1;
   const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
   const bool hydro = (src != nullptr && src->hydro);
@@ -1402,7 +1405,7 @@ int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   {const int _pt = dk::ProfBegin(c, "gas_hook");
   k_gas_hook<<<CGRID(ActiveCells(c))>>>(c->g, c->d_blocks, c->u0, c->u1, gam0, gam1, bdt,
                                         src ? 1 : 0, src ? *src : z,
-                                        hydro ? c->hdivf : nullptr);
+                                        hydro ? c->hdivf : nullptr, dmom, atw);
    dk::ProfEnd(c, _pt);}
   return Launched(c);
 }

@@ -199,7 +199,10 @@ int FastStoppingTimes(dk_ctx *c, double *partial, int nb);
 template <typename T> int StiffnessRecord(dk_ctx *c, const T *q, int nv, int comp, double dt);
 
 // dk_fp64.hip pieces reused by the FP32 form
-int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src);
+// the gas update of a stage; dmom: the drag history u0.M += atw dmom applied after it in the same
+// pass (the FP32 form's stage 2 with isothermal fluxes)
+int GasHook(dk_ctx *c, const dk_stage *st, const dk_gas_src *src, const double *dmom = nullptr,
+            double atw = 0.0);
 // the explicit gas update's options check and its flux divergence field (allocated on first use)
 int HydroSupported(const dk_ctx *c, const dk_gas_src *src);
 int HydroField(dk_ctx *c);

@@ -90,13 +90,16 @@ KERNEL_BYTES_PC2 = {
 
 # The explicit gas update with isothermal fluxes (dk_gas_src.hydro), both forms, per active
 # cell: the divergence kernel reads u0 (4 values; its 13-cell stencil is served from the caches)
-# and writes divf (4); the RK combination reads u0, u1 and divf and writes u0 (4 each)
+# and writes divf (4); the RK combination reads u0, u1 and divf and writes u0 (4 each); in the
+# FP32 form it also applies stage 2's drag history (dmom, 3 values, read: 24 B averaged over
+# the two stages)
 GAS_UPDATE_BYTES = {"hydro_divf": 8 * 8, "gas_hook": 16 * 8}
 
 
 def kernel_bytes(label, form, w):
     if w.src is not None and w.src.hydro and label in GAS_UPDATE_BYTES:
-        return (0, GAS_UPDATE_BYTES[label])
+        extra = 12 if (label == "gas_hook" and form == "fp32" and w.phys.coupling == 0) else 0
+        return (0, GAS_UPDATE_BYTES[label] + extra)
     if w.phys.coupling == 1 and label in KERNEL_BYTES_PC2[form]:
         return KERNEL_BYTES_PC2[form][label]
     return KERNEL_BYTES.get(label, (0, 0))
