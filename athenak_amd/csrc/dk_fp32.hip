@@ -128,7 +128,9 @@ struct Fast {
   float *qt[3] = {nullptr, nullptr, nullptr};   // centre mass m c/V: own, from below, above
   float *ustar = nullptr, *ap = nullptr, *res = nullptr;
   double *dmom = nullptr;           // D of the kick, then R_g
-  double *dc[3] = {nullptr, nullptr, nullptr};  // actual dust correction: own, below, above
+  // actual dust correction: own (with its tile neighbours' shares), from the tile below, from
+  // the tile above
+  double *dc[3] = {nullptr, nullptr, nullptr};
   double *pres[2] = {nullptr, nullptr};   // push rounding residue per cell (to the gas):
   int pres_rd = 0;                        // the buffer of the last push, read by the next stage
   float *scr_f = nullptr;
@@ -1440,6 +1442,17 @@ k_key(KArgs A, int do_key, double *pres) {
     else if (do_key)
       for (int i = tid; i < Tl.TC; i += CH)
         A.ov_cnt[OvSlot(A.item_start, tile, part)*Tl.TC + i] = lcnt[i];
+    // The correction's shares for the cells beside each cell in x2 (centre one row below: dcs[0],
+    // one row above: dcs[2]) are exchanged in LDS (the records' area, free once the chunks are
+    // done): a cell's dc0 holds its own share plus those its tile neighbours hold for it, and
+    // only a tile's last and first x2 rows pass theirs on, to the neighbouring tiles' rows, in
+    // dc1 (from below) and dc2 (from above).  Every target is written by exactly one thread.
+    double *nb = lds_d;   // [2][3][TC]
+    if (A.pending && part == 0) {
+      if (tid < Tl.TC)
+        for (int d=0; d<3; ++d) {nb[d*Tl.TC + tid] = dcs[0][d]; nb[(3 + d)*Tl.TC + tid] = dcs[2][d];}
+      __syncthreads();
+    }
     if (tid < Tl.TC && part == 0) {
       int ci, cj, ck;
       CellIdx(Tl, t, tid, ci, cj, ck);
@@ -1447,11 +1460,14 @@ k_key(KArgs A, int do_key, double *pres) {
       if (do_key)
         for (int d=0; d<3; ++d) pres[i0 + d*g.ncb] = static_cast<double>(prs[d]);
       if (A.pending) {
-        // own cell, and the cells beside it in x2 (every target written by exactly one thread)
+        const int row = (tid >> Tl.LX) & (Tl.TY - 1);
         for (int d=0; d<3; ++d) {
-          A.dc0[i0 + d*g.ncb] = dcs[1][d];
-          A.dc1[i0 + g.n1 + d*g.ncb] = dcs[2][d];   // to the cell above: "from below" there
-          A.dc2[i0 - g.n1 + d*g.ncb] = dcs[0][d];   // to the cell below: "from above" there
+          double own = dcs[1][d];
+          if (row > 0) own += nb[(3 + d)*Tl.TC + tid - Tl.TX];
+          if (row < Tl.TY - 1) own += nb[d*Tl.TC + tid + Tl.TX];
+          A.dc0[i0 + d*g.ncb] = own;
+          if (row == Tl.TY - 1) A.dc1[i0 + g.n1 + d*g.ncb] = dcs[2][d];
+          if (row == 0) A.dc2[i0 - g.n1 + d*g.ncb] = dcs[0][d];
         }
       }
     } else if (tid < Tl.TC) {
@@ -1573,23 +1589,36 @@ __global__ void k_total(int *count, const int *stay, long long nkeys) {
 
 // placement: stayers first (part*kPartRank + their rank in the part: a later part's stayers after
 // the offset in its overflow slot), then arrivals (rank -1 - r); item_start: the items of the
-// order the ranks were taken in
-__global__ void k_place(const int *key, const int *rank, const int *start, const int *stay,
-                        long long np, int TC, const int *item_start, const int *ov_cnt,
-                        int *perm) {
-  GRID_STRIDE(p, np) {
-    const int k = key[p], r = rank[p];
-    int off;
-    if (r < 0) {
-      off = stay[k] - 1 - r;
-    } else if (r < kPartRank) {
-      off = r;
-    } else {
-      const int part = r/kPartRank, tile = k/TC;
-      off = ov_cnt[((long long)item_start[tile] - tile + part - 1)*TC + (k - tile*TC)] +
-            (r - part*kPartRank);
+// order the ranks were taken in.  Each thread places kPlaceU particles a block apart, their loads
+// issued together: the kernel is bound by the latency of its dependent loads (key, then the
+// cell's start), not by their bytes.
+constexpr int kPlaceU = 4;
+__device__ inline int PlaceOff(int k, int r, const int *stay, int TC, const int *item_start,
+                               const int *ov_cnt) {
+  if (r < 0) return stay[k] - 1 - r;
+  if (r < kPartRank) return r;
+  const int part = r/kPartRank, tile = k/TC;
+  return ov_cnt[((long long)item_start[tile] - tile + part - 1)*TC + (k - tile*TC)] +
+         (r - part*kPartRank);
+}
+__global__ void __launch_bounds__(kBlock)
+k_place(const int *key, const int *rank, const int *start, const int *stay, long long np, int TC,
+        const int *item_start, const int *ov_cnt, int *perm) {
+  const long long span = (long long)gridDim.x*kBlock*kPlaceU;
+  for (long long p0 = (long long)blockIdx.x*kBlock*kPlaceU + threadIdx.x; p0 < np; p0 += span) {
+    int k[kPlaceU], r[kPlaceU], s[kPlaceU];
+#pragma unroll
+    for (int u=0; u<kPlaceU; ++u) {
+      const long long p = p0 + u*kBlock;
+      k[u] = -1; r[u] = 0;
+      if (p < np) {k[u] = key[p]; r[u] = rank[p];}
     }
-    perm[start[k] + off] = static_cast<int>(p);
+#pragma unroll
+    for (int u=0; u<kPlaceU; ++u)
+      if (k[u] >= 0) s[u] = start[k[u]] + PlaceOff(k[u], r[u], stay, TC, item_start, ov_cnt);
+#pragma unroll
+    for (int u=0; u<kPlaceU; ++u)
+      if (k[u] >= 0) perm[s[u]] = static_cast<int>(p0 + u*kBlock);
   }
 }
 
@@ -2635,23 +2664,35 @@ __global__ void k_fix_key(KArgs A, int do_key, double *pres) {
     int ci, cj, ck;
     CellIdx(Tl, t, l, ci, cj, ck);
     const long long c = Idx(g, t.m, 3, 0, g.ks + ck, g.js + cj, g.is + ci);
+    // the later parts' shares in the key pass's form (own cell plus the tile neighbours' shares
+    // for it in dc0, the tile's edge rows passing theirs on in dc1 / dc2), part by part
+    const int row = (l >> Tl.LX) & (Tl.TY - 1);
     for (int d=0; d<3; ++d) {
       double pr = do_key ? pres[c + d*g.ncb] : 0.0;
-      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-      if (A.pending) {a0 = A.dc2[c - g.n1 + d*g.ncb]; a1 = A.dc0[c + d*g.ncb];
-                      a2 = A.dc1[c + g.n1 + d*g.ncb];}
+      double own = 0.0, up = 0.0, dn = 0.0;
+      if (A.pending) {
+        own = A.dc0[c + d*g.ncb];
+        if (row == Tl.TY - 1) up = A.dc1[c + g.n1 + d*g.ncb];
+        if (row == 0) dn = A.dc2[c - g.n1 + d*g.ncb];
+      }
       for (int p = 1; p < np; ++p) {
         const long long o = (long long)i0 - tile + p - 1;
         if (do_key) pr += static_cast<double>(A.ov_pr[(o*3 + d)*Tl.TC + l]);
         if (A.pending) {
-          a0 += A.ov_dc[(o*9 + d)*Tl.TC + l];
-          a1 += A.ov_dc[(o*9 + 3 + d)*Tl.TC + l];
-          a2 += A.ov_dc[(o*9 + 6 + d)*Tl.TC + l];
+          const double *od = A.ov_dc + o*9*Tl.TC;
+          own += od[(3 + d)*Tl.TC + l];
+          if (row > 0) own += od[(6 + d)*Tl.TC + l - Tl.TX];
+          if (row < Tl.TY - 1) own += od[d*Tl.TC + l + Tl.TX];
+          if (row == Tl.TY - 1) up += od[(6 + d)*Tl.TC + l];
+          if (row == 0) dn += od[d*Tl.TC + l];
         }
       }
       if (do_key) pres[c + d*g.ncb] = pr;
-      if (A.pending) {A.dc2[c - g.n1 + d*g.ncb] = a0; A.dc0[c + d*g.ncb] = a1;
-                      A.dc1[c + g.n1 + d*g.ncb] = a2;}
+      if (A.pending) {
+        A.dc0[c + d*g.ncb] = own;
+        if (row == Tl.TY - 1) A.dc1[c + g.n1 + d*g.ncb] = up;
+        if (row == 0) A.dc2[c - g.n1 + d*g.ncb] = dn;
+      }
     }
     if (do_key) {
       // the stayers of each later part follow those of the parts before it: its count becomes
@@ -2774,7 +2815,7 @@ __global__ void k_combine(Geo g, Tiles Tl, const T *scr, TO *out, const double *
 // M/rho of u1 (= u0 here, stage 1) that the predictor gathers (dust_pm.cpp:163-172), fmode 2
 // u* = (M1 + M0)/(rho1 + rho0) that the advance gathers (dust_push.cpp:45-58)
 __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, double *dmom,
-                             const double *dc0, const double *dc1, const double *dc2,
+                             const double *dc0, const double *dc1, const double *dc2, int ty,
                              const double *pres,
                              int pending, double inv_aprev, int stage1, int hook, double gam0,
                              double gam1, double bdt, int use_src, dk_gas_src src,
@@ -2795,7 +2836,12 @@ __global__ void k_cell_start(Geo g, const DevBlock *B, double *u0, double *u1, d
     if (pending) {
       for (int c=0; c<3; ++c) {
         const long long id = Idx(g,m,3,c,k,j,i);
-        const double dgas = dmom[id] - ((dc0[id] + dc1[id]) + dc2[id]);
+        // dc0: the cell's own share and its tile neighbours'; dc1 / dc2: the shares from the
+        // tiles below / above (a tile's first / last x2 row only)
+        double dc = dc0[id];
+        if (((j - g.js) & (ty - 1)) == 0) dc += dc1[id];
+        if (((j - g.js) & (ty - 1)) == ty - 1) dc += dc2[id];
+        const double dgas = dmom[id] - dc;
         u[1+c] += dgas + pres[id];   // pres: what the FP32 rounding of the push withheld
         rg[c] = dgas*inv_aprev;
         dmom[id] = rg[c];
@@ -3342,7 +3388,7 @@ int SortKeys(dk_ctx *c) {
   TIMED("scan", e = dk::Check(hipcub::DeviceScan::ExclusiveSum(
       f->scan_tmp, f->scan_bytes, f->count, f->start, (int)(f->nkeys + 1), c->stream)));
   if (e) return e;
-  TIMED("place", (k_place<<<Grid(f->np), kBlock, 0, c->stream>>>(
+  TIMED("place", (k_place<<<Grid((f->np + kPlaceU - 1)/kPlaceU), kBlock, 0, c->stream>>>(
       f->key, f->rank, f->start, f->stay, f->np, f->T.TC, f->item_start, f->ov_cnt, f->perm)));
   if ((e = FillV<int>(c, f->big, 2, 0))) return e;
   TIMED("segsort", (k_segsort<<<Grid(f->nkeys), kBlock, 0, c->stream>>>(
@@ -3447,7 +3493,7 @@ int Flush(dk_ctx *c) {
   std::memset(&z, 0, sizeof(z));
   const long long nact = (long long)c->g.nmb*c->g.nx1*c->g.nx2*c->g.nx3;
   TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
-      c->g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
+      c->g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->T.TY,
       f->pres[f->pres_rd], 1,
       1.0/f->a_prev, 0, 0, 1.0, 0.0, 0.0, 0, z, 0.0)));
   f->pending = 0;
@@ -3780,7 +3826,7 @@ static int FastStagePc2(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     dk_gas_src z;
     std::memset(&z, 0, sizeof(z));
     TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
-        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
+        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->T.TY,
         f->pres[f->pres_rd], 0, 0.0, A.stage1, hydro ? 0 : 1, st->gam0[s], st->gam1[s],
         st->beta[s]*st->dt, src ? 1 : 0, src ? *src : z, 0.0, f->ustar, s1 ? 1 : 2)));
     if (hydro && (e = dk::GasHook(c, st, src))) return e;
@@ -3882,7 +3928,7 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
     const double gam0 = st->gam0[s], gam1 = st->gam1[s], bdt = st->beta[s]*st->dt;
     const double atw = (c->P.back_reaction && st->stage == 2) ? st->a_twid22*st->dt : 0.0;
     TIMED("cell_start", (k_cell_start<<<CellGrid(c), kBlock, 0, c->stream>>>(
-        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2],
+        g, c->d_blocks, c->u0, c->u1, f->dmom, f->dc[0], f->dc[1], f->dc[2], f->T.TY,
         f->pres[f->pres_rd], f->pending,
         f->pending ? 1.0/f->a_prev : 0.0, A.stage1, hydro ? 0 : 1, gam0, gam1, bdt,
         src ? 1 : 0, src ? *src : z, hydro ? 0.0 : atw)));
