@@ -1180,9 +1180,12 @@ struct KeyIn {
   int cell;
 };
 
+// pq: the slot in the other store where the stage-1 key pass left this particle's registers
+// (the last sort's permutation at q, loaded a chunk earlier by the caller: a load that waits on
+// one issued just before it would wait on every load in flight)
 template <bool MID>
 __device__ inline void KeyFetch(const KArgs &A, const TileInfo &t, const int *st, int q,
-                                int do_key, KeyIn &k) {
+                                int do_key, int pq, KeyIn &k) {
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
   const bool rates = A.atw_dt != 0.0 || !do_key;
@@ -1210,10 +1213,8 @@ __device__ inline void KeyFetch(const KArgs &A, const TileInfo &t, const int *st
     k.x1[d] = 0.0;
     k.v1[d] = 0.0f;
   }
-  if (!MID && do_key && !A.stage1) {
-    const int pq = A.perm[q];
+  if (!MID && do_key && !A.stage1)
     for (int d=0; d<3; ++d) {k.x1[d] = A.dst.X1(d, pq); k.v1[d] = A.dst.V1(d, pq);}
-  }
   k.mass = A.src.M(q);
 }
 
@@ -1227,9 +1228,11 @@ __device__ inline void KeyFetch(const KArgs &A, const TileInfo &t, const int *st
 // cells: stay[]), one that leaves it from a global count of arrivals (count[]).  With do_key = 0
 // only the pending correction is applied (written to v and R).  MID: the pc2 stage-2 form, keyed
 // by the half-step position (Pc2Route) instead of pushed.
+// (Bk: the block table again, as a restrict pointer: the tile's block, read by every particle,
+// is then read with scalar loads once per item instead of vector loads in every chunk)
 template <bool OV, bool MID = false, bool SMR = false>
 __global__ void __launch_bounds__(kKeyThreadsMax, (OV || SMR) ? 2 : 4)
-k_key(KArgs A, int do_key, double *pres) {
+k_key(KArgs A, int do_key, double *pres, const DevBlock *__restrict__ Bk) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -1263,18 +1266,27 @@ k_key(KArgs A, int do_key, double *pres) {
     __syncthreads();
     // this part's particles: at most W from the tile's range
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
-    const DevBlock &b = A.B[t.m];
+    const DevBlock &b = Bk[t.m];
     const double ivol = b.ivol;
     double pmass = 0.0;   // (outflow faces: the mass and count this thread parked, chunk order)
     int pcnt = 0;
     // every particle's inputs are fetched one chunk ahead: the next chunk's loads are in flight
     // during this chunk's per-cell sums and barriers
+    const bool gather_regs = !MID && do_key && !A.stage1;
+    int pq_n = 0;   // (the permutation at the next chunk's slot: stage registers' source)
+    if (gather_regs && P0 + tid < P1) pq_n = A.perm[P0 + tid];
     KeyIn nx;
-    if (P0 + tid < P1) KeyFetch<MID>(A, t, st, P0 + tid, do_key, nx);
+    if (P0 + tid < P1) KeyFetch<MID>(A, t, st, P0 + tid, do_key, pq_n, nx);
+    if (gather_regs && P0 + CH + tid < P1) pq_n = A.perm[P0 + CH + tid];
     int stamp = 0;   // (the chunk's number within the item, from 1)
+    int acount = -1, aq = 0;   // an arrival's place in its cell's count, its slot (stored a chunk
+                               // later)
     for (int c0 = P0; c0 < P1; c0 += CH) {
       ++stamp;
       int lkey = -1;   // a stayer's cell in the tile
+      // the previous chunk's arrival's rank, stored now: its atomic's return is not waited for
+      // before the next chunk's loads are issued, and those loads are waited for here anyway
+      if (acount >= 0) {A.rank[aq] = -1 - acount; acount = -1;}
       const int c1 = min(c0 + CH, P1);
       const int q = c0 + tid;
       if (q < c1) {
@@ -1345,19 +1357,22 @@ k_key(KArgs A, int do_key, double *pres) {
           // a particle that failed to route keeps its block: its key stays valid
           int ni, nj, nk, nju;
           // the tile's block (read once by the workgroup), or the block a particle migrated to
-          double lo[3] = {b.x1min, b.x2min, b.x3min}, sc[3] = {b.sx1, b.sx2, b.sx3};
-          if (s.gid != t.m) {
+          // (its fields read and used in the branch, so that only a wavefront holding a migrated
+          // particle waits for them)
+          if (s.gid == t.m) {
+            const double lo[3] = {b.x1min, b.x2min, b.x3min}, sc[3] = {b.sx1, b.sx2, b.sx3};
+            DepositCellOf(g, lo, sc, MID ? xh : s.x, A.sh, ni, nj, nk, nju);
+          } else {
             const DevBlock &bn = A.B[s.gid];
-            lo[0] = bn.x1min; lo[1] = bn.x2min; lo[2] = bn.x3min;
-            sc[0] = bn.sx1; sc[1] = bn.sx2; sc[2] = bn.sx3;
+            const double lo[3] = {bn.x1min, bn.x2min, bn.x3min}, sc[3] = {bn.sx1, bn.sx2, bn.sx3};
+            DepositCellOf(g, lo, sc, MID ? xh : s.x, A.sh, ni, nj, nk, nju);
           }
-          DepositCellOf(g, lo, sc, MID ? xh : s.x, A.sh, ni, nj, nk, nju);
           const long long key = TileKey(Tl, s.gid, ni, nj, nk);
           A.key[q] = static_cast<int>(key);
           const long long lk = key - t.key0;
           // (a stayer's rank is taken below; an arrival's from the cell's global count)
           if (lk >= 0 && lk < Tl.TC) lkey = static_cast<int>(lk);
-          else A.rank[q] = -1 - atomicAdd(&A.count[key], 1);
+          else {acount = atomicAdd(&A.count[key], 1); aq = q;}
         }
         KeyRec kr;
         for (int d=0; d<3; ++d) {kr.dv[d] = dv[d]; kr.pr[d] = pr[d];}
@@ -1383,7 +1398,8 @@ k_key(KArgs A, int do_key, double *pres) {
         }
       }
       // (the next chunk's slots: no particle of this chunk is written there)
-      if (q + CH < P1) KeyFetch<MID>(A, t, st, q + CH, do_key, nx);
+      if (q + CH < P1) KeyFetch<MID>(A, t, st, q + CH, do_key, pq_n, nx);
+      if (gather_regs && q + 2*CH < P1) pq_n = A.perm[q + 2*CH];
       __syncthreads();
       int ctot = 0;   // (thread l < TC: cell l's stayers of this chunk)
       if (do_key) {
@@ -1431,6 +1447,7 @@ k_key(KArgs A, int do_key, double *pres) {
       __syncthreads();
       if (tid < Tl.TC) lcnt[tid] += ctot;   // (read again only after the next chunk's barrier)
     }
+    if (acount >= 0) A.rank[aq] = -1 - acount;   // (the last chunk's arrival)
     double dcs[3][3];
     float prs[3];
     if (tid < Tl.TC) {   // (the two half sums in order)
@@ -3166,20 +3183,20 @@ void LaunchKey(dk_ctx *c, const KArgs &A, int do_key, double *pres, bool mid = f
     bool sy;
     for (int cls = 0; cls < 2; ++cls) {
       const KArgs Ac = ClassArgs(c, A, cls, sy);
-      if (mid) k_key<false, true><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
+      if (mid) k_key<false, true><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres, c->d_blocks);
       else if (c->smr)
-        k_key<false, false, true><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
-      else k_key<false><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres);
+        k_key<false, false, true><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres, c->d_blocks);
+      else k_key<false><<<Ac.nfirst, f->key_threads, lds, c->stream>>>(Ac, do_key, pres, c->d_blocks);
     }
     return;
   }
   if (mid) {
-    k_key<false, true><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres);
-    k_key<true, true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres);
+    k_key<false, true><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres, c->d_blocks);
+    k_key<true, true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres, c->d_blocks);
     return;
   }
-  k_key<false><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres);
-  k_key<true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres);
+  k_key<false><<<f->T.nt, f->key_threads, lds, c->stream>>>(A, do_key, pres, c->d_blocks);
+  k_key<true><<<OverGrid(c), f->key_threads, lds, c->stream>>>(A, do_key, pres, c->d_blocks);
 }
 
 // a deposit kernel of the scheme of the context with the x2-shift code SYV
