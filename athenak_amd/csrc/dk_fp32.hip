@@ -1752,7 +1752,7 @@ __global__ void __launch_bounds__(1024) k_bigsort(const int *start, const int *s
 // mass m c/V (dust_pm.cpp:660-667)
 template <int S, bool SY, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 6)
-k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
+k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr, const DevBlock *__restrict__ Bk) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -1780,7 +1780,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
     __syncthreads();
     // this part's particles: at most W from the tile's range
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
-    const DevBlock &b = A.B[t.m];
+    const DevBlock &b = Bk[t.m];
     const double ivol = b.ivol;
     const int slab = tid/Tl.TC;
     const int l = tid - slab*Tl.TC;
@@ -1913,7 +1913,7 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr) {
 // P2: the particle part of the dc1 operator, result += G^T (m c/V) G u* (dust_solver.cpp:151-201)
 template <int S, bool SY, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 8)
-k_p2(KArgs A, const float *us, float *scr) {
+k_p2(KArgs A, const float *us, float *scr, const DevBlock *__restrict__ Bk) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -1944,7 +1944,7 @@ k_p2(KArgs A, const float *us, float *scr) {
     __syncthreads();
     // this part's particles: at most W from the tile's range
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
-    const DevBlock &b = A.B[t.m];
+    const DevBlock &b = Bk[t.m];
     const double ivol = b.ivol;
     const int slab = tid/Tl.TC;
     const int l = tid - slab*Tl.TC;
@@ -2041,7 +2041,8 @@ k_p2(KArgs A, const float *us, float *scr) {
 // (REGS: the drag rate R is written, for a later stage; a template argument, as a run-time test
 // around the store costs the kernel registers)
 template <int S, bool SY, bool REGS, bool OV, bool SPLIT = false>
-__device__ inline void P3Body(const KArgs &A, const float *us, float *scr, double *defect) {
+__device__ inline void P3Body(const KArgs &A, const float *us, float *scr, double *defect,
+                              const DevBlock *__restrict__ Bk) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -2072,7 +2073,7 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
     __syncthreads();
     // this part's particles: at most W from the tile's range
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
-    const DevBlock &b = A.B[t.m];
+    const DevBlock &b = Bk[t.m];
     const double ivol = b.ivol;
     const int slab = tid/Tl.TC;
     const int l = tid - slab*Tl.TC;
@@ -2194,14 +2195,14 @@ __device__ inline void P3Body(const KArgs &A, const float *us, float *scr, doubl
 
 template <int S, bool SY, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 8)
-k_p3(KArgs A, const float *us, float *scr, double *defect) {
-  P3Body<S, SY, true, OV, SPLIT>(A, us, scr, defect);
+k_p3(KArgs A, const float *us, float *scr, double *defect, const DevBlock *__restrict__ Bk) {
+  P3Body<S, SY, true, OV, SPLIT>(A, us, scr, defect, Bk);
 }
 
 template <int S, bool SY, bool OV, bool SPLIT = false>
 __global__ void __launch_bounds__(kCellThreads, (SY || OV || SPLIT) ? 4 : 8)
-k_p3n(KArgs A, const float *us, float *scr, double *defect) {
-  P3Body<S, SY, false, OV, SPLIT>(A, us, scr, defect);
+k_p3n(KArgs A, const float *us, float *scr, double *defect, const DevBlock *__restrict__ Bk) {
+  P3Body<S, SY, false, OV, SPLIT>(A, us, scr, defect, Bk);
 }
 
 // pc2 predictor (dust_pm.cpp:141-193): the particles (routed in place by the key pass) moved into
@@ -2212,7 +2213,7 @@ k_p3n(KArgs A, const float *us, float *scr, double *defect) {
 // (CLOUD: refinable clouds at the mesh level, the linear gather from the 12-component pack uv)
 template <int S, bool OV, bool SPLIT = false, bool CLOUD = false>
 __global__ void __launch_bounds__(kCellThreads, (OV || SPLIT || CLOUD) ? 4 : 6)
-k_pp(KArgs A, const float *uv, float *scr) {
+k_pp(KArgs A, const float *uv, float *scr, const DevBlock *__restrict__ Bk) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -2240,7 +2241,7 @@ k_pp(KArgs A, const float *uv, float *scr) {
     if constexpr (!CLOUD) uf.Store(Tl, 3, uv_);
     __syncthreads();
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
-    const DevBlock &b = A.B[t.m];
+    const DevBlock &b = Bk[t.m];
     const float ivolf = static_cast<float>(b.ivol);
     const float dtf = static_cast<float>(A.dt);
     const int slab = tid/Tl.TC;
@@ -2364,7 +2365,7 @@ k_pp(KArgs A, const float *uv, float *scr) {
 // (CLOUD: as k_pp, the linear gather from the 12-component pack us of u*)
 template <int S, bool OV, bool SPLIT = false, bool CLOUD = false>
 __global__ void __launch_bounds__(kCellThreads, 4)
-k_pa(KArgs A, const float *us, float *scr, double *defect) {
+k_pa(KArgs A, const float *us, float *scr, double *defect, const DevBlock *__restrict__ Bk) {
   extern __shared__ double lds_d[];
   const Geo &g = A.g;
   const Tiles &Tl = A.T;
@@ -2392,7 +2393,7 @@ k_pa(KArgs A, const float *us, float *scr, double *defect) {
     if constexpr (!CLOUD) uf.Store(Tl, 3, us_);
     __syncthreads();
     const int P0 = min(st[0] + part*A.W, st[Tl.TC]), P1 = min(P0 + A.W, st[Tl.TC]);
-    const DevBlock &b = A.B[t.m];
+    const DevBlock &b = Bk[t.m];
     const double ivol = b.ivol;
     const float hf = static_cast<float>(0.5*A.dt);
     const int slab = tid/Tl.TC;
@@ -3257,20 +3258,20 @@ KArgs ClassArgs(const dk_ctx *c, const KArgs &A0, int cls, bool &sy) {
 void LaunchP1(dk_ctx *c, const KArgs &A0, float *qt0, float *qt1, float *qt2, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  LAUNCH_CLASSES(k_p1, LdsP1, (A, qt0, qt1, qt2, scr));
+  LAUNCH_CLASSES(k_p1, LdsP1, (A, qt0, qt1, qt2, scr, c->d_blocks));
 }
 
 void LaunchP2(dk_ctx *c, const KArgs &A0, const float *us, float *scr) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  LAUNCH_CLASSES(k_p2, LdsP2, (A, us, scr));
+  LAUNCH_CLASSES(k_p2, LdsP2, (A, us, scr, c->d_blocks));
 }
 
 void LaunchP3(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect) {
   const Tiles &T = c->fast->T;
   const dim3 blk(3*T.TC);
-  if (A0.regs_live) {LAUNCH_CLASSES(k_p3, LdsP3, (A, us, scr, defect));}
-  else {LAUNCH_CLASSES(k_p3n, LdsP3, (A, us, scr, defect));}
+  if (A0.regs_live) {LAUNCH_CLASSES(k_p3, LdsP3, (A, us, scr, defect, c->d_blocks));}
+  else {LAUNCH_CLASSES(k_p3n, LdsP3, (A, us, scr, defect, c->d_blocks));}
 }
 
 // the pc2 tile kernels (one class: no shearing box), first parts then the extra parts
@@ -3289,11 +3290,11 @@ void LaunchPP(dk_ctx *c, const KArgs &A0, const float *uv, float *scr, bool clou
   const size_t lds = LdsPP(T, cloud);
   bool sy;
   KArgs A = ClassArgs(c, A0, 0, sy);
-  if (cloud) k_pp<2, false, false, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, uv, scr);
-  else {SWITCH_S2(k_pp, false, false, dim3(A.nfirst), (A, uv, scr))}
+  if (cloud) k_pp<2, false, false, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, uv, scr, c->d_blocks);
+  else {SWITCH_S2(k_pp, false, false, dim3(A.nfirst), (A, uv, scr, c->d_blocks))}
   A = ClassArgs(c, A0, 1, sy);
-  if (cloud) k_pp<2, false, true, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, uv, scr);
-  else {SWITCH_S2(k_pp, false, true, dim3(A.nfirst), (A, uv, scr))}
+  if (cloud) k_pp<2, false, true, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, uv, scr, c->d_blocks);
+  else {SWITCH_S2(k_pp, false, true, dim3(A.nfirst), (A, uv, scr, c->d_blocks))}
 }
 
 void LaunchPA(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *defect,
@@ -3304,15 +3305,15 @@ void LaunchPA(dk_ctx *c, const KArgs &A0, const float *us, float *scr, double *d
   bool sy;
   KArgs A = ClassArgs(c, A0, 0, sy);
   if (cloud) {
-    k_pa<2, false, false, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, us, scr, defect);
+    k_pa<2, false, false, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, us, scr, defect, c->d_blocks);
   } else {
-    SWITCH_S2(k_pa, false, false, dim3(A.nfirst), (A, us, scr, defect))
+    SWITCH_S2(k_pa, false, false, dim3(A.nfirst), (A, us, scr, defect, c->d_blocks))
   }
   A = ClassArgs(c, A0, 1, sy);
   if (cloud) {
-    k_pa<2, false, true, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, us, scr, defect);
+    k_pa<2, false, true, true><<<dim3(A.nfirst), blk, lds, c->stream>>>(A, us, scr, defect, c->d_blocks);
   } else {
-    SWITCH_S2(k_pa, false, true, dim3(A.nfirst), (A, us, scr, defect))
+    SWITCH_S2(k_pa, false, true, dim3(A.nfirst), (A, us, scr, defect, c->d_blocks))
   }
 }
 
