@@ -1787,12 +1787,15 @@ k_p1(KArgs A, float *qt0, float *qt1, float *qt2, float *scr, const DevBlock *__
     float q3[3] = {0.0f, 0.0f, 0.0f};
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
+      // the permutation of the chunk's slots, one particle ahead
+      int pn = (c0 + tid < c1) ? A.perm[c0 + tid] : 0;
 #pragma unroll 1
       for (int k=0; k<kPPT; ++k) {
         const int r = tid + k*NT;
         const int q = c0 + r;
         if (q >= c1) break;
-        const int p = A.perm[q];
+        const int p = pn;
+        if (k + 1 < kPPT && q + NT < c1) pn = A.perm[q + NT];
         const double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
         const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
         const float ts = A.src.TS(p), mass = A.src.M(p);
@@ -1950,13 +1953,24 @@ k_p2(KArgs A, const float *us, float *scr, const DevBlock *__restrict__ Bk) {
     const int l = tid - slab*Tl.TC;
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
+      // each particle's inputs are loaded one particle ahead (in flight during the gather)
+      double nx[3] = {0.0, 0.0, 0.0};
+      float nts = 0.0f, nm = 0.0f;
+      if (c0 + tid < c1) {
+        for (int d=0; d<3; ++d) nx[d] = A.src.X(d, c0 + tid);
+        nts = A.src.TS(c0 + tid); nm = A.src.M(c0 + tid);
+      }
 #pragma unroll 1
       for (int k=0; k<kPPT; ++k) {
         const int r = tid + k*NT;
         const int q = c0 + r;
         if (q >= c1) break;
-        const double x[3] = {A.src.X(0, q), A.src.X(1, q), A.src.X(2, q)};
-        const float tsq = A.src.TS(q), mq = A.src.M(q);
+        const double x[3] = {nx[0], nx[1], nx[2]};
+        const float tsq = nts, mq = nm;
+        if (k + 1 < kPPT && q + NT < c1) {
+          for (int d=0; d<3; ++d) nx[d] = A.src.X(d, q + NT);
+          nts = A.src.TS(q + NT); nm = A.src.M(q + NT);
+        }
         const Loc L = LocateInTile<SY>(g, Tl, t, b, x, A.sh);
         const int sj = L.sj;
         float wx[3], wy[3], wz[3];
@@ -2248,12 +2262,15 @@ k_pp(KArgs A, const float *uv, float *scr, const DevBlock *__restrict__ Bk) {
     const int l = tid - slab*Tl.TC;
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
+      // the permutation of the chunk's slots, one particle ahead
+      int pn = (c0 + tid < c1) ? A.perm[c0 + tid] : 0;
 #pragma unroll 1
       for (int k=0; k<kPPT; ++k) {
         const int r = tid + k*NT;
         const int q = c0 + r;
         if (q >= c1) break;
-        const int p = A.perm[q];
+        const int p = pn;
+        if (k + 1 < kPPT && q + NT < c1) pn = A.perm[q + NT];
         const double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
         const float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
         const float ts = A.src.TS(p), mass = A.src.M(p);
@@ -2401,12 +2418,15 @@ k_pa(KArgs A, const float *us, float *scr, double *defect, const DevBlock *__res
     double ex[3] = {0.0, 0.0, 0.0};
     for (int c0 = P0; c0 < P1; c0 += CH) {
       const int c1 = min(c0 + CH, P1);
+      // the permutation of the chunk's slots, one particle ahead
+      int pn = (c0 + tid < c1) ? A.perm[c0 + tid] : 0;
 #pragma unroll 1
       for (int k=0; k<kPPT; ++k) {
         const int r = tid + k*NT;
         const int q = c0 + r;
         if (q >= c1) break;
-        const int p = A.perm[q];
+        const int p = pn;
+        if (k + 1 < kPPT && q + NT < c1) pn = A.perm[q + NT];
         double x[3] = {A.src.X(0, p), A.src.X(1, p), A.src.X(2, p)};
         float v[3] = {A.src.V(0, p), A.src.V(1, p), A.src.V(2, p)};
         const float ts = A.src.TS(p), mass = A.src.M(p);
