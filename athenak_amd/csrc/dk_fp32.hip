@@ -142,6 +142,7 @@ struct Fast {
   int pending = 0;
   double a_prev = 0.0;
   int regs_dead = 0;                // the store holds no registers (after a step's last stage)
+  int keep_regs = 0;                // every stage keeps the registers (dk_keep_registers)
   // pc2: R lives in the other store (after a predictor); the positions await the migration
   // that the next key pass applies (after an advance)
   int r_other = 0, unmigrated = 0;
@@ -3935,7 +3936,8 @@ int FastStage(dk_ctx *c, const dk_stage *st, const dk_gas_src *src) {
   KArgs A = MakeArgs(c);
   A.stage1 = (st->stage == 1);
   // the last active stage (stage 3 of a three-stage scheme is the identity)
-  A.regs_live = !(st->stage == st->nexp_stages || (st->nexp_stages == 3 && st->stage == 2));
+  A.regs_live = f->keep_regs ||
+                !(st->stage == st->nexp_stages || (st->nexp_stages == 3 && st->stage == 2));
   A.gam0 = st->gam0[s]; A.gam1 = st->gam1[s]; A.beta_dt = st->beta[s]*st->dt;
   A.atw_dt = (st->stage == 2) ? st->a_twid22*st->dt : 0.0;
   A.a_dt = a_dt; A.sh = sh; A.dt = st->dt;
@@ -4209,6 +4211,12 @@ int dk_sort_state(dk_ctx *c, int32_t which, int32_t *host, int64_t n) {
   if (c == nullptr || host == nullptr) return DK_EINVAL;
   if (c->form != DK_FORM_FP32) return DK_EUNSUPPORTED;
   return dk::FastSortState(c, which, host, n);
+}
+
+int dk_keep_registers(dk_ctx *c, int32_t on) {
+  if (c == nullptr || (on != 0 && on != 1)) return DK_EINVAL;
+  if (c->fast != nullptr) c->fast->keep_regs = on;   // (the plain form always keeps them)
+  return DK_OK;
 }
 
 int dk_set_tune_config(dk_ctx *c, int32_t xcd, int32_t key_threads) {

@@ -23,7 +23,7 @@ EXPORTS = (["dk_create", "dk_destroy", "dk_bind_particles", "dk_bind_gas",
             "dk_error", "dk_field", "dk_strerror", "dk_setup_problem", "dk_prof_enable",
             "dk_prof_reset", "dk_prof_read", "dk_autotune", "dk_tune_config",
             "dk_set_solver", "dk_solver_stats",
-            "dk_set_tune_config", "dk_graph_launches", "dk_stiffness",
+            "dk_set_tune_config", "dk_graph_launches", "dk_stiffness", "dk_keep_registers",
             "dk_stopping_times", "dk_remove_outflow_particles", "dk_x3_removed",
             "dk_sort_state", "dk_sort_particles_noalloc", "dk_write_pbin",
             "dk_particle_history", "dk_write_phst"] +
@@ -69,6 +69,7 @@ def lib():
         L.dk_solver_stats.argtypes = [P, C.POINTER(C.c_double)]
         L.dk_tune_config.argtypes = [P] + [C.POINTER(C.c_int32)] * 3
         L.dk_set_tune_config.argtypes = [P, C.c_int32, C.c_int32]
+        L.dk_keep_registers.argtypes = [P, C.c_int32]
         L.dk_sort_particles.argtypes = [P]
         L.dk_sort_particles_noalloc.argtypes = [P]
         L.dk_new_time_step.argtypes = [P, C.c_double, C.POINTER(C.c_double)]
@@ -311,6 +312,11 @@ class DustContext:
         v = [C.c_int32(0) for _ in range(3)]
         self._check(lib().dk_tune_config(self.h, *[C.byref(x) for x in v]), "dk_tune_config")
         return tuple(x.value for x in v)
+
+    def keep_registers(self, on):
+        """Keep the stage registers x1, v1, R through every stage, so that an export after a step
+        carries the reference's values (the FP32 form otherwise writes x1 = x, v1 = v, R = 0)."""
+        self._check(lib().dk_keep_registers(self.h, 1 if on else 0), "dk_keep_registers")
 
     def set_tune_config(self, xcd, key_threads):
         self._check(lib().dk_set_tune_config(self.h, int(xcd), int(key_threads)),

@@ -184,6 +184,14 @@ int dk_bind_particles(dk_ctx *ctx, double *rdata, int32_t *idata, int64_t capaci
                       int64_t npart);
 int dk_bind_gas(dk_ctx *ctx, double *u0, double *u1);
 int dk_export_particles(dk_ctx *ctx);
+/* The stage registers at step boundaries (no counterpart in the reference, whose particle
+ * arrays always hold them; for callers that write restarts or pbin files from exported
+ * particles).  on = 1: every FP32-form stage keeps x1, v1 and R, so that an export after a step
+ * writes the reference's values (x1, v1: the stage-1 copies as the step's migration left them;
+ * R: the last stage's drag rate with its consistent correction), at the cost of the last
+ * stage's register writes (60 B per particle-step); on = 0 (the default): as above.  The plain
+ * FP64 form always holds them. */
+int dk_keep_registers(dk_ctx *ctx, int32_t on);
 int64_t dk_npart(const dk_ctx *ctx);
 
 /* ---- the task entry points (DK_FORM_FP64) ------------------------------------------ */

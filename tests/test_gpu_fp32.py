@@ -396,6 +396,39 @@ def test_fp32_registers_mid_step_and_at_step_end():
     g.close()
 
 
+def test_fp32_registers_kept_at_step_end():
+    # with dk_keep_registers every stage keeps x1, v1 and R: the export after whole steps carries
+    # the oracle's x1, v1 (the stage-1 copies as the step's migration across blocks and the
+    # periodic faces left them) and R (the last stage's drag rate with its consistent correction)
+    w = workloads.c1(n=16, nb=2, ppc=2)
+    w.setup.dust_noise, w.setup.gas_noise, w.dt = 1.0, 0.3, 2e-3
+    o = Oracle(w.mesh, w.blocks, w.phys, w.npart + 16)
+    o.setup(w.setup)
+    g = fp32_ctx(w)
+    g.keep_registers(True)
+    for _ in range(3):
+        o.step(w.dt, w.src)
+        g.step(w.dt, w.src)
+    g.check_device_error()
+    rg, ig, _ = exported(g)
+    n = o.npart
+    order = np.argsort(o.idata()[abi.PTAG, :n], kind="stable")
+    ro = o.rdata()[:, :n][:, order]
+    np.testing.assert_array_equal(o.idata()[abi.PTAG, :n][order], ig[abi.PTAG])
+    vmax = np.abs(ro[[abi.IPVX, abi.IPVY, abi.IPVZ]]).max()
+    travel = vmax * w.dt
+    err = max(np.abs(ro[c] - rg[c]).max() for c in (abi.IPX1, abi.IPY1, abi.IPZ1)) / travel
+    say("step-end x1: max err %.3e of a step's travel" % err)
+    assert err < 1e-5, err
+    for cols in ((abi.IPVX1, abi.IPVY1, abi.IPVZ1), (abi.IPRX, abi.IPRY, abi.IPRZ)):
+        scale = np.abs(ro[list(cols)]).max()
+        assert scale > 0.0
+        err = np.abs(ro[list(cols)] - rg[list(cols)]).max() / scale
+        say("step-end registers %s: max rel err %.3e" % (cols, err))
+        assert err < 1e-5, (cols, err)
+    g.close()
+
+
 def test_fp32_autotuner_choices_are_bit_identical(tmp_path):
     # every choice the autotuner makes (tile dealing to XCDs, key-pass workgroup size) leaves
     # the results bit-identical; online tuning over the caller's stages and the cache file
